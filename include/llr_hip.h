@@ -1,0 +1,259 @@
+/*
+ * llr_hip.h - C ABI of the MI355X-native LLR diffusion-MRI reconstruction path.
+ *
+ * Drop-in boundary for the CUDA path of wyatt-A/llr-reco. Every entry point
+ * below replaces one Rust function of that crate (cited file:line) and keeps
+ * its argument meaning, order and preconditions. Plain pointers and sizes only;
+ * no exception or panic crosses the boundary: each call returns an llr_status
+ * and llr_last_error() holds the message of the last failure on the calling
+ * thread (the reference panics with the same conditions).
+ *
+ * Data conventions (shared with the reference):
+ *   - complex64 = interleaved {re, im} float pairs (num_complex::Complex32 ==
+ *     cuComplex, bindings.rs:9189-9191);
+ *   - every array is column-major: voxel index x + nx*(y + ny*z), the volume
+ *     index slowest (array_utils.rs:4-17, lib.rs:374-388);
+ *   - a set of volumes is one contiguous buffer of n_volumes * nx*ny*nz
+ *     elements (the slices lib.rs:374-388 builds are views of one such buffer).
+ *
+ * Host-pointer entry points are synchronous and allocate their device buffers
+ * per call, as the reference wrappers do (e.g. dti_subspace.rs:150-165). The
+ * device-resident reconstruction (llr_recon_*) keeps the iterate, the
+ * measurements and every workspace in HBM for the whole run.
+ *
+ * Threading: a context is used by one host thread at a time. All device work of
+ * a context runs on its own stream (default priority).
+ */
+#ifndef LLR_HIP_H
+#define LLR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct llr_ctx llr_ctx;
+typedef struct llr_recon llr_recon;
+
+typedef enum {
+  LLR_OK = 0,
+  LLR_ERR_ARG = 1,         /* precondition violated (the reference asserts/panics) */
+  LLR_ERR_HIP = 2,         /* HIP runtime failure */
+  LLR_ERR_NOMEM = 3,       /* device allocation failed */
+  LLR_ERR_UNSUPPORTED = 4, /* shape outside what the kernels implement */
+  LLR_ERR_NUMERIC = 5      /* non-finite value met where the reference would fail */
+} llr_status;
+
+typedef struct {
+  float re, im;
+} llr_cf32;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* llr_last_error(void);
+/* ABI version of this header (major*100 + minor). */
+int llr_abi_version(void);
+
+/* ---------------------------------------------------------------- context
+ * Replaces the per-call device selection and allocation of cuda_api.rs:8-191:
+ * one context owns a HIP stream and cached twiddle/phase tables. */
+llr_status llr_ctx_create(int device, llr_ctx** out);
+void llr_ctx_destroy(llr_ctx* ctx);
+llr_status llr_ctx_synchronize(llr_ctx* ctx);
+/* Device number of the context (as given to llr_ctx_create). */
+int llr_ctx_device(const llr_ctx* ctx);
+
+/* ---------------------------------------------------------------- block.rs */
+/* grid_dim (block.rs:112-118): grid[a] = vol_size[a] / block_size[a]. */
+void llr_grid_dim(const size_t vol_size[3], const size_t block_size[3], size_t grid[3]);
+
+/* lift_data_set (block.rs:147-164): gather n_matrices shifted blocks
+ * (block indices matrix_idx_start .. +n_matrices, corners from block_coord,
+ * block.rs:122-133, periodic wrap block.rs:8-28) of n_volumes volumes into
+ * column-major (prod(block_size) x n_volumes) matrices, contiguous.
+ * elem_bytes is the size of the element type T (the Rust API is generic).
+ * matrix_len (elements) must be >= prod(block_size)*n_volumes*n_matrices
+ * (block.rs:152); every block dimension must be <= the volume dimension
+ * (block.rs:69-71). Bit-exact data movement. */
+llr_status llr_lift_data_set(llr_ctx* ctx, const void* volumes, size_t n_volumes, size_t elem_bytes,
+                             void* matrix_data, size_t matrix_len, const size_t vol_size[3],
+                             const size_t block_size[3], const size_t shift[3], size_t matrix_idx_start,
+                             size_t n_matrices);
+
+/* un_lift_data_set (block.rs:166-184): the inverse scatter of llr_lift_data_set. */
+llr_status llr_un_lift_data_set(llr_ctx* ctx, void* volumes, size_t n_volumes, size_t elem_bytes,
+                                const void* matrix_data, size_t matrix_len, const size_t vol_size[3],
+                                const size_t block_size[3], const size_t shift[3], size_t matrix_idx_start,
+                                size_t n_matrices);
+
+/* --------------------------------------------------------- dti_subspace.rs */
+/* dti_project (dti_subspace.rs:141-166): compressed (srank x nvox per matrix)
+ * = S^H X^H for each of batch_size image matrices X (nvox x nq); S is nq x srank.
+ * compressed_len must equal nvox*srank*batch_size (dti_subspace.rs:148). */
+llr_status llr_dti_project(llr_ctx* ctx, const llr_cf32* subspace, size_t subspace_len,
+                           const llr_cf32* image_matrices, size_t image_len, llr_cf32* compressed,
+                           size_t compressed_len, size_t batch_size, size_t nq, size_t nvox, size_t srank);
+
+/* dti_back_project (dti_subspace.rs:172-207): image matrices (nvox x nq) =
+ * (S * compressed)^H with compressed given as srank x nvox per matrix.
+ * image_len must equal nq*nvox*batch_size (dti_subspace.rs:175). */
+llr_status llr_dti_back_project(llr_ctx* ctx, const llr_cf32* subspace, size_t subspace_len,
+                                llr_cf32* image_matrices, size_t image_len, const llr_cf32* compressed,
+                                size_t compressed_len, size_t batch_size, size_t nq, size_t nvox,
+                                size_t srank);
+
+/* conj_transpose_matrix (dti_subspace.rs:211-221): (m x n) -> (n x m)^H per matrix. */
+llr_status llr_conj_transpose_matrix(llr_ctx* ctx, llr_cf32* matrix_data, size_t len, size_t m, size_t n,
+                                     size_t batch_size);
+
+/* subspace_constraint (dti_subspace.rs:89-134): X <- transpose(S S^H X^H),
+ * keeping the reference's final plain (non-conjugating) transpose. */
+llr_status llr_subspace_constraint(llr_ctx* ctx, const llr_cf32* subspace, size_t subspace_len,
+                                   llr_cf32* image_matrices, size_t image_len, size_t batch_size, size_t nq,
+                                   size_t nvox, size_t srank);
+
+/* ------------------------------------------------ low_rank.rs / cublas.rs */
+/* cu_low_rank_approx_batch (low_rank.rs:10-69): replace each m x n matrix
+ * (m > n) by its rank-`rank` truncation U_r diag(s_r) V_r^H. */
+llr_status llr_low_rank_approx_batch(llr_ctx* ctx, size_t m, size_t n, size_t rank, size_t batch_size,
+                                     llr_cf32* matrix_data, size_t len);
+
+/* Singular-value soft threshold of each m x n matrix: U diag(max(s-lambda,0)) V^H.
+ * No reference counterpart (the reference only truncates); same layout as
+ * llr_low_rank_approx_batch. sigma_max (may be NULL) receives the largest
+ * singular value met. */
+llr_status llr_svt_batch(llr_ctx* ctx, size_t m, size_t n, float lambda, size_t batch_size,
+                         llr_cf32* matrix_data, size_t len, float* sigma_max);
+
+/* cublas_cgemm_strided_batched_device (cublas.rs:12-105) on host buffers:
+ * C = alpha*op(A)*op(B) + beta*C, op = conjugate transpose when flagged, A
+ * broadcast over the batch when broadcast_a != 0. */
+llr_status llr_cgemm_strided_batched(llr_ctx* ctx, size_t m, size_t k, size_t n, size_t batch_size,
+                                     int broadcast_a, llr_cf32 alpha, llr_cf32 beta, int transpose_a,
+                                     int transpose_b, const llr_cf32* a, size_t a_len, const llr_cf32* b,
+                                     size_t b_len, llr_cf32* c, size_t c_len);
+
+/* rhs_cdgmm_batched_device (cublas.rs:119-127, kernels.cu:11-67): A[:,j,b] *= d[j,b]. */
+llr_status llr_rhs_cdgmm_batched(llr_ctx* ctx, size_t m, size_t n, size_t batch_size, llr_cf32* matrix,
+                                 size_t len, const float* diag);
+
+/* ------------------------------------------------ signal_model.rs / cufft.rs */
+#define LLR_FORWARD 0
+#define LLR_INVERSE 1
+#define LLR_NORM_NONE 0
+#define LLR_NORM_INVERSE 1
+#define LLR_NORM_UNITARY 2
+
+/* signal_model (signal_model.rs:34-79): Forward = ramp(ksp corr, +1) -> unitary
+ * forward DFT -> ramp(img corr, -1); Inverse = ramp(img, +1) -> unitary inverse
+ * DFT -> ramp(ksp, -1); ramp phase (i - n/2) * 2*pi*(c + n/2)/n per axis
+ * (cufft.rs:290-326, signal_model.rs:110-117). lin_*_corr hold batch_size x 3
+ * floats. The ramps are evaluated exactly (turn-reduced), not in f32. */
+llr_status llr_signal_model(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t image_size[3],
+                            size_t batch_size, const float* lin_ksp_corr, const float* lin_img_corr,
+                            int direction);
+
+/* Device-pointer form of llr_signal_model for callers that keep the data in HBM:
+ * x is a device pointer on the context's device; returns when the transform is
+ * complete. Same semantics and preconditions as llr_signal_model. */
+llr_status llr_signal_model_device(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t image_size[3],
+                                   size_t batch_size, const float* lin_ksp_corr, const float* lin_img_corr,
+                                   int direction);
+
+/* estimate_linear_correction (signal_model.rs:100-106 via max_coord,
+ * cufft.rs:329-337): arg-max of |k|^2 (lowest index on ties) minus (n+1)/2. */
+llr_status llr_estimate_linear_correction(llr_ctx* ctx, const llr_cf32* ksp, size_t len,
+                                          const size_t vol_size[3], float out[3]);
+
+/* cu_fftn_batch (cufft.rs:124-180): batched in-place 3-D C2C transform of
+ * col-major volumes, norm = LLR_NORM_* (cufft.rs:19-29, 152-166). */
+llr_status llr_fftn_batch(llr_ctx* ctx, const size_t dims[3], size_t batches, int direction, int norm,
+                          llr_cf32* data, size_t len);
+
+/* ---------------------------------------------------------------- fftshift.rs */
+/* fftshift / ifftshift (fftshift.rs:103-116): circular shift by floor(d/2) /
+ * ceil(d/2) along each of the (up to 3) dims, generic element size. */
+llr_status llr_fftshift(llr_ctx* ctx, void* data, size_t elem_bytes, const size_t dims[3]);
+llr_status llr_ifftshift(llr_ctx* ctx, void* data, size_t elem_bytes, const size_t dims[3]);
+/* phase_shift3 (fftshift.rs:293-318): x *= exp(i*sign*pi*sum_a (i_a - n_a/2)),
+ * sign -1 for LLR_FORWARD, +1 for LLR_INVERSE, f32 phase as the reference. */
+llr_status llr_phase_shift3(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t dims[3], int direction);
+
+/* ------------------------------------------------------------- lib.rs helpers */
+/* wavelet_denoise (lib.rs:430-458) for n_volumes volumes: multi-level 3-D db2
+ * analysis (levels < 0: floor(log2(min dim / 3))), soft threshold
+ * shrink_complex (lib.rs:460-465) of every subband, synthesis. |x| = 0 maps to
+ * 0 where the reference produces NaN. */
+llr_status llr_wavelet_denoise(llr_ctx* ctx, llr_cf32* volumes, size_t n_volumes, const size_t vol_size[3],
+                               float alpha, int levels);
+
+/* hard_project (lib.rs:392-406): where y != 0, err += |x - y|^2 and x = y. */
+llr_status llr_hard_project(llr_ctx* ctx, llr_cf32* x, const llr_cf32* y, size_t len, double* sum_sq_error);
+
+/* calc_undersampling_factor (lib.rs:409-417). */
+llr_status llr_calc_undersampling_factor(llr_ctx* ctx, const llr_cf32* y, size_t len, double* factor,
+                                         double* total_energy);
+
+/* under_sample_kspace (data_import.rs:416-438): zero each readout line whose
+ * phase-encode mask entry (y, z, q) is zero; mask is ny x nz x nq complex. */
+llr_status llr_under_sample_kspace(llr_ctx* ctx, llr_cf32* data, const size_t dims[3], size_t nq,
+                                   const llr_cf32* mask);
+
+/* ---------------------------------------------------- device-resident recon */
+#define LLR_MODE_RANK 0 /* hard rank truncation (low_rank.rs:10-69) */
+#define LLR_MODE_SVT 1  /* singular-value soft threshold */
+
+#define LLR_PREC_FP32 0  /* plain form: FP32 throughout */
+#define LLR_PREC_MIXED 1 /* BF16 MFMA Gram/projection + FP32 refinement + FP64 fallback */
+
+typedef struct {
+  size_t vol[3];         /* nx, ny, nz */
+  size_t nq;             /* volumes */
+  size_t blk[3];         /* llr_block_size (lib.rs:50) */
+  size_t n_iter;         /* n_iter (lib.rs:62): maximum iterations */
+  int llr_mode;          /* LLR_MODE_* */
+  int precision;         /* LLR_PREC_* */
+  size_t rank;           /* llr_block_rank (lib.rs:58) for LLR_MODE_RANK */
+  double svt_lambda;     /* LLR_MODE_SVT: absolute threshold if > 0 ... */
+  double lambda_factor;  /* ... else lambda_factor * sigma_max of the zero-filled blocks at shift 0 */
+  double wavelet_alpha;  /* wavelet_shrink_factor (lib.rs:60); 0 disables the wavelet step */
+  int wavelet_levels;    /* < 0: floor(log2(min dim / 3)) */
+  size_t srank;          /* subspace rank; 0: no subspace (lib.rs:281-292) */
+  const llr_cf32* subspace; /* nq x srank, col-major (lib.rs:210-219); host pointer, copied */
+  const size_t* shifts;  /* n_iter x 3 block shifts (lib.rs:266); host pointer, copied */
+  double residual_stop;  /* > 0: stop once the relative residual falls to this value */
+} llr_recon_params;
+
+/* Create a reconstruction (allocates the iterate, the measurements and all
+ * workspaces on the context's device). */
+llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_recon** out);
+void llr_recon_destroy(llr_recon* r);
+/* Measurements (k-space, nx*ny*nz*nq complex64, zero = unsampled). on_device != 0:
+ * y is a device pointer on the context's device. */
+llr_status llr_recon_set_measurements(llr_recon* r, const llr_cf32* y, size_t len, int on_device);
+/* Optional warm start of the iterate (image space); otherwise the zero-filled adjoint. */
+llr_status llr_recon_set_initial(llr_recon* r, const llr_cf32* x, size_t len, int on_device);
+/* llr_recon_exec loop (lib.rs:227-341): estimate linear corrections, zero-filled
+ * adjoint (unless a warm start was set), then n_iter iterations of
+ * LLR -> wavelet -> forward model -> hard projection -> adjoint.
+ * err_hist (may be NULL) receives err/total_energy per iteration (lib.rs:321);
+ * iters_done (may be NULL) the number of iterations run. */
+llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done);
+/* Copy the current image out (to a device pointer when on_device != 0). */
+llr_status llr_recon_get_image(llr_recon* r, llr_cf32* out, size_t len, int on_device);
+/* Threshold used by the SVT step and the per-volume k-space corrections found. */
+llr_status llr_recon_info(llr_recon* r, double* lambda, float* ksp_corr);
+
+/* Stage timing with HIP events on the context's stream (for roofline reports).
+ * Stages: 0 LLR block kernel, 1 subspace projection, 2 wavelet, 3 FFT x pass,
+ * 4 FFT y pass, 5 FFT z pass + hard projection, 6 reductions. */
+#define LLR_NUM_STAGES 7
+llr_status llr_recon_enable_timing(llr_recon* r, int enable);
+llr_status llr_recon_stage_times(llr_recon* r, double* total_ms, size_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LLR_HIP_H */

@@ -1,0 +1,864 @@
+// C ABI of the LLR HIP library: host-pointer drop-ins for the reference's Rust
+// API and the device-resident reconstruction loop (llr_recon_exec,
+// lib.rs:113-363). Declarations and reference citations are in llr_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+
+namespace llr {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+llr_status fail(llr_status st, const std::string& msg) {
+  g_err = msg;
+  return st;
+}
+
+llr_status hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? LLR_ERR_NOMEM : LLR_ERR_HIP;
+}
+
+llr_status get_twiddles(llr_ctx* ctx, int n, const float2** out) {
+  for (auto& p : ctx->twiddles)
+    if (p.first == n) {
+      *out = p.second;
+      return LLR_OK;
+    }
+  std::vector<float2> h(n);
+  const long double TWO_PI = 6.283185307179586476925286766559005768L;
+  for (int m = 0; m < n; ++m) {
+    long double ang = -TWO_PI * (long double)m / (long double)n;
+    h[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+  float2* d = nullptr;
+  LLR_HIP(hipMalloc(&d, sizeof(float2) * n));
+  LLR_HIP(hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+  ctx->twiddles.push_back({n, d});
+  *out = d;
+  return LLR_OK;
+}
+
+llr_status red_scratch(llr_ctx* ctx, size_t n, double** out) {
+  if (ctx->red_cap < n) {
+    if (ctx->d_red) (void)hipFree(ctx->d_red);
+    ctx->d_red = nullptr;
+    LLR_HIP(hipMalloc(&ctx->d_red, n * sizeof(double)));
+    ctx->red_cap = n;
+  }
+  *out = ctx->d_red;
+  return LLR_OK;
+}
+
+// exp(+i * 2*pi * frac((i - n/2) * s / n)), s = c + n/2 (signal_model.rs:110-117,
+// cufft.rs:320-326), reduced to turns in long double.
+void ramp_table(std::vector<float2>& out, size_t n, const float* corr, size_t stride, size_t nvols) {
+  out.resize(n * nvols);
+  const long double TWO_PI = 6.283185307179586476925286766559005768L;
+  for (size_t v = 0; v < nvols; ++v) {
+    long double s = (long double)(corr ? corr[v * stride] : 0.0f) + 0.5L * (long double)n;
+    for (size_t i = 0; i < n; ++i) {
+      long double tr = ((long double)i - 0.5L * (long double)n) * s / (long double)n;
+      tr -= std::floor(tr);
+      long double ang = TWO_PI * tr;
+      out[v * n + i] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+  }
+}
+
+// ------------------------------------------------------------- helpers
+static llr_status upload(DevBuf& b, const void* h, size_t bytes) {
+  LLR_HIP(b.alloc(bytes));
+  if (h && bytes) LLR_HIP(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+  return LLR_OK;
+}
+
+static llr_status download(void* h, const DevBuf& b, size_t bytes, llr_ctx* ctx) {
+  LLR_HIP(hipStreamSynchronize(ctx->stream));
+  if (bytes) LLR_HIP(hipMemcpy(h, b.p, bytes, hipMemcpyDeviceToHost));
+  return LLR_OK;
+}
+
+struct RampTables {
+  DevBuf pk[3], pi[3];
+};
+
+static llr_status make_tables(RampTables& t, const size_t* dims, size_t nv, const float* ksp, const float* img) {
+  for (int a = 0; a < 3; ++a) {
+    std::vector<float2> h;
+    ramp_table(h, dims[a], ksp ? ksp + a : nullptr, 3, nv);
+    llr_status st = upload(t.pk[a], h.data(), h.size() * sizeof(float2));
+    if (st) return st;
+    ramp_table(h, dims[a], img ? img + a : nullptr, 3, nv);
+    st = upload(t.pi[a], h.data(), h.size() * sizeof(float2));
+    if (st) return st;
+  }
+  return LLR_OK;
+}
+
+static llr_status check_fft_dims(const size_t* dims) {
+  for (int a = 0; a < 3; ++a)
+    if (dims[a] < 2 || !fft_size_supported(dims[a]))
+      return fail(LLR_ERR_UNSUPPORTED, "volume dimension " + std::to_string(dims[a]) +
+                                           " is not a supported FFT length (powers of two 2..512)");
+  return LLR_OK;
+}
+
+// signal model (signal_model.rs:34-79) on device data; src -> dst (may alias)
+static llr_status model_device(llr_ctx* ctx, const float2* src, float2* dst, const size_t* dims, size_t nv,
+                               const RampTables& t, int dir) {
+  const float2* in = src;
+  for (int a = 0; a < 3; ++a) {
+    float s = (float)(1.0 / std::sqrt((double)dims[a]));
+    llr_status st;
+    if (dir == LLR_FORWARD)
+      st = fft_axis_pass(ctx, in, dst, dims, nv, a, false, t.pk[a].as<float2>(), false, t.pi[a].as<float2>(), true, s);
+    else
+      st = fft_axis_pass(ctx, in, dst, dims, nv, a, true, t.pi[a].as<float2>(), false, t.pk[a].as<float2>(), true, s);
+    if (st) return st;
+    in = dst;
+  }
+  return LLR_OK;
+}
+
+static llr_status check_blocks(const size_t* vol, const size_t* blk) {
+  for (int a = 0; a < 3; ++a) {
+    if (blk[a] == 0) return fail(LLR_ERR_ARG, "block dimension must be positive");
+    if (blk[a] > vol[a]) return fail(LLR_ERR_ARG, "block dimension must be less than or equal to volume dimension");
+  }
+  return LLR_OK;
+}
+
+}  // namespace llr
+
+using namespace llr;
+
+// ===================================================================== ABI
+extern "C" {
+
+const char* llr_last_error(void) { return g_err.c_str(); }
+int llr_abi_version(void) { return 100; }
+
+llr_status llr_ctx_create(int device, llr_ctx** out) {
+  if (!out) return fail(LLR_ERR_ARG, "null output pointer");
+  int n = 0;
+  LLR_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(LLR_ERR_ARG, "no such HIP device: " + std::to_string(device));
+  LLR_HIP(hipSetDevice(device));
+  llr_ctx* c = new llr_ctx();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = c;
+  return LLR_OK;
+}
+
+void llr_ctx_destroy(llr_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& p : ctx->twiddles) (void)hipFree(p.second);
+  if (ctx->d_red) (void)hipFree(ctx->d_red);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+llr_status llr_ctx_synchronize(llr_ctx* ctx) {
+  LLR_HIP(hipStreamSynchronize(ctx->stream));
+  return LLR_OK;
+}
+
+int llr_ctx_device(const llr_ctx* ctx) { return ctx ? ctx->device : -1; }
+
+void llr_grid_dim(const size_t vol_size[3], const size_t block_size[3], size_t grid[3]) {
+  for (int a = 0; a < 3; ++a) grid[a] = block_size[a] ? vol_size[a] / block_size[a] : 0;
+}
+
+// ------------------------------------------------------------------ block.rs
+static llr_status lift_common(llr_ctx* ctx, const void* volumes, void* vol_out, size_t nvols, size_t eb,
+                              const void* mats_in, void* mats_out, size_t matrix_len, const size_t* vol,
+                              const size_t* blk, const size_t* shift, size_t start, size_t n, bool unlift) {
+  if (!ctx) return fail(LLR_ERR_ARG, "null context");
+  if (eb == 0) return fail(LLR_ERR_ARG, "element size must be positive");
+  llr_status st = check_blocks(vol, blk);
+  if (st) return st;
+  size_t m = blk[0] * blk[1] * blk[2];
+  size_t need = m * nvols * n;
+  if (matrix_len < need) return fail(LLR_ERR_ARG, "matrix data buffer is too small");
+  size_t N = vol[0] * vol[1] * vol[2];
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf dv, dm;
+  st = upload(dv, volumes, N * nvols * eb);
+  if (st) return st;
+  st = upload(dm, unlift ? mats_in : nullptr, need * eb);
+  if (st) return st;
+  st = lift_device(ctx, dv.p, dm.p, nvols, eb, vol, blk, shift, start, n, unlift);
+  if (st) return st;
+  if (unlift) return download(vol_out, dv, N * nvols * eb, ctx);
+  return download(mats_out, dm, need * eb, ctx);
+}
+
+llr_status llr_lift_data_set(llr_ctx* ctx, const void* volumes, size_t n_volumes, size_t elem_bytes, void* matrix_data,
+                             size_t matrix_len, const size_t vol_size[3], const size_t block_size[3],
+                             const size_t shift[3], size_t matrix_idx_start, size_t n_matrices) {
+  return lift_common(ctx, volumes, nullptr, n_volumes, elem_bytes, nullptr, matrix_data, matrix_len, vol_size,
+                     block_size, shift, matrix_idx_start, n_matrices, false);
+}
+
+llr_status llr_un_lift_data_set(llr_ctx* ctx, void* volumes, size_t n_volumes, size_t elem_bytes,
+                                const void* matrix_data, size_t matrix_len, const size_t vol_size[3],
+                                const size_t block_size[3], const size_t shift[3], size_t matrix_idx_start,
+                                size_t n_matrices) {
+  return lift_common(ctx, volumes, volumes, n_volumes, elem_bytes, matrix_data, nullptr, matrix_len, vol_size,
+                     block_size, shift, matrix_idx_start, n_matrices, true);
+}
+
+// ----------------------------------------------------------- dti_subspace.rs
+llr_status llr_dti_project(llr_ctx* ctx, const llr_cf32* subspace, size_t subspace_len, const llr_cf32* image_matrices,
+                           size_t image_len, llr_cf32* compressed, size_t compressed_len, size_t batch_size, size_t nq,
+                           size_t nvox, size_t srank) {
+  if (image_len < nq * nvox * batch_size) return fail(LLR_ERR_ARG, "image matrices doesn't have enough entries");
+  if (subspace_len != nq * srank) return fail(LLR_ERR_ARG, "unexpected subspace matrix size");
+  if (compressed_len != nvox * srank * batch_size)
+    return fail(LLR_ERR_ARG, "compressed result doesn't have enough entries");
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf dS, dX, dC;
+  llr_status st;
+  if ((st = upload(dS, subspace, subspace_len * 8))) return st;
+  if ((st = upload(dX, image_matrices, nq * nvox * batch_size * 8))) return st;
+  if ((st = upload(dC, nullptr, compressed_len * 8))) return st;
+  st = cgemm_device(ctx, srank, nq, nvox, batch_size, true, make_float2(1, 0), make_float2(0, 0), true, true,
+                    dS.as<float2>(), dX.as<float2>(), dC.as<float2>());
+  if (st) return st;
+  return download(compressed, dC, compressed_len * 8, ctx);
+}
+
+llr_status llr_dti_back_project(llr_ctx* ctx, const llr_cf32* subspace, size_t subspace_len, llr_cf32* image_matrices,
+                                size_t image_len, const llr_cf32* compressed, size_t compressed_len, size_t batch_size,
+                                size_t nq, size_t nvox, size_t srank) {
+  if (image_len != nq * nvox * batch_size) return fail(LLR_ERR_ARG, "image matrices doesn't have enough entries");
+  if (subspace_len != nq * srank) return fail(LLR_ERR_ARG, "unexpected subspace matrix size");
+  if (compressed_len != nvox * srank * batch_size)
+    return fail(LLR_ERR_ARG, "compressed result doesn't have enough entries");
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf dS, dB, dC, dX;
+  llr_status st;
+  if ((st = upload(dS, subspace, subspace_len * 8))) return st;
+  if ((st = upload(dC, compressed, compressed_len * 8))) return st;
+  if ((st = upload(dB, nullptr, image_len * 8))) return st;
+  if ((st = upload(dX, nullptr, image_len * 8))) return st;
+  st = cgemm_device(ctx, nq, srank, nvox, batch_size, true, make_float2(1, 0), make_float2(0, 0), false, false,
+                    dS.as<float2>(), dC.as<float2>(), dB.as<float2>());
+  if (st) return st;
+  st = conj_transpose_device(ctx, dB.as<float2>(), dX.as<float2>(), nq, nvox, batch_size, true);
+  if (st) return st;
+  return download(image_matrices, dX, image_len * 8, ctx);
+}
+
+llr_status llr_conj_transpose_matrix(llr_ctx* ctx, llr_cf32* matrix_data, size_t len, size_t m, size_t n,
+                                     size_t batch_size) {
+  if (len != m * n * batch_size) return fail(LLR_ERR_ARG, "matrix data doesn't have enough entries");
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf a, b;
+  llr_status st;
+  if ((st = upload(a, matrix_data, len * 8))) return st;
+  if ((st = upload(b, nullptr, len * 8))) return st;
+  if ((st = conj_transpose_device(ctx, a.as<float2>(), b.as<float2>(), m, n, batch_size, true))) return st;
+  return download(matrix_data, b, len * 8, ctx);
+}
+
+llr_status llr_subspace_constraint(llr_ctx* ctx, const llr_cf32* subspace, size_t subspace_len, llr_cf32* image_matrices,
+                                   size_t image_len, size_t batch_size, size_t nq, size_t nvox, size_t srank) {
+  if (image_len < nq * nvox * batch_size) return fail(LLR_ERR_ARG, "image matrices doesn't have enough entries");
+  if (subspace_len != nq * srank) return fail(LLR_ERR_ARG, "unexpected subspace matrix size");
+  size_t xl = nq * nvox * batch_size;
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf dS, dX, dC, dB;
+  llr_status st;
+  if ((st = upload(dS, subspace, subspace_len * 8))) return st;
+  if ((st = upload(dX, image_matrices, xl * 8))) return st;
+  if ((st = upload(dC, nullptr, nvox * srank * batch_size * 8))) return st;
+  if ((st = upload(dB, nullptr, xl * 8))) return st;
+  if ((st = cgemm_device(ctx, srank, nq, nvox, batch_size, true, make_float2(1, 0), make_float2(0, 0), true, true,
+                         dS.as<float2>(), dX.as<float2>(), dC.as<float2>())))
+    return st;
+  if ((st = cgemm_device(ctx, nq, srank, nvox, batch_size, true, make_float2(1, 0), make_float2(0, 0), false, false,
+                         dS.as<float2>(), dC.as<float2>(), dB.as<float2>())))
+    return st;
+  // dti_subspace.rs:127-133 writes the plain transpose of (nq x nvox) into (nvox x nq)
+  if ((st = conj_transpose_device(ctx, dB.as<float2>(), dX.as<float2>(), nq, nvox, batch_size, false))) return st;
+  return download(image_matrices, dX, xl * 8, ctx);
+}
+
+// ------------------------------------------------- low_rank.rs / cublas.rs
+static llr_status lowrank_common(llr_ctx* ctx, size_t m, size_t n, int mode, size_t rank, float lambda, size_t batch,
+                                 llr_cf32* data, size_t len, float* smax_out) {
+  if (len < m * n * batch) return fail(LLR_ERR_ARG, "matrix data buffer is too small");
+  if (n == 0 || m < n) return fail(LLR_ERR_ARG, "only tall matrices (m >= n > 0) are supported");
+  if (mode == 0 && (rank == 0 || rank > n)) return fail(LLR_ERR_ARG, "rank must be in 1..=n");
+  if (batch == 0) return LLR_OK;
+  LLR_HIP(hipSetDevice(ctx->device));
+  size_t el = m * n * batch;
+  DevBuf d, smax;
+  llr_status st;
+  if ((st = upload(d, data, el * 8))) return st;
+  if ((st = upload(smax, nullptr, sizeof(unsigned)))) return st;
+  LLR_HIP(hipMemsetAsync(smax.p, 0, sizeof(unsigned), ctx->stream));
+  if (n <= 8) {
+    size_t vol[3] = {m, 1, 1}, zero[3] = {0, 0, 0};
+    st = llr_block_run(ctx, d.as<float2>(), d.as<float2>(), nullptr, 0, vol, vol, zero, (int)n, 1, 0, (long long)batch,
+                       mode, (int)rank, lambda, smax.as<unsigned>(), nullptr);
+  } else {
+    st = lowrank_generic_device(ctx, d.as<float2>(), (long long)m, (int)n, (long long)batch, mode, (int)rank, lambda,
+                                smax.as<unsigned>());
+  }
+  if (st) return st;
+  if (smax_out) {
+    unsigned bits = 0;
+    LLR_HIP(hipStreamSynchronize(ctx->stream));
+    LLR_HIP(hipMemcpy(&bits, smax.p, sizeof(unsigned), hipMemcpyDeviceToHost));
+    std::memcpy(smax_out, &bits, sizeof(float));
+  }
+  return download(data, d, el * 8, ctx);
+}
+
+llr_status llr_low_rank_approx_batch(llr_ctx* ctx, size_t m, size_t n, size_t rank, size_t batch_size,
+                                     llr_cf32* matrix_data, size_t len) {
+  return lowrank_common(ctx, m, n, 0, rank, 0.f, batch_size, matrix_data, len, nullptr);
+}
+
+llr_status llr_svt_batch(llr_ctx* ctx, size_t m, size_t n, float lambda, size_t batch_size, llr_cf32* matrix_data,
+                         size_t len, float* sigma_max) {
+  if (!(lambda >= 0.f)) return fail(LLR_ERR_ARG, "lambda must be non-negative");
+  return lowrank_common(ctx, m, n, 1, 0, lambda, batch_size, matrix_data, len, sigma_max);
+}
+
+llr_status llr_cgemm_strided_batched(llr_ctx* ctx, size_t m, size_t k, size_t n, size_t batch_size, int broadcast_a,
+                                     llr_cf32 alpha, llr_cf32 beta, int transpose_a, int transpose_b, const llr_cf32* a,
+                                     size_t a_len, const llr_cf32* b, size_t b_len, llr_cf32* c, size_t c_len) {
+  size_t na = broadcast_a ? m * k : m * k * batch_size;
+  if (a_len < na || b_len < k * n * batch_size || c_len < m * n * batch_size)
+    return fail(LLR_ERR_ARG, "matrix buffers are too small");
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf dA, dB, dC;
+  llr_status st;
+  if ((st = upload(dA, a, na * 8))) return st;
+  if ((st = upload(dB, b, k * n * batch_size * 8))) return st;
+  if ((st = upload(dC, c, m * n * batch_size * 8))) return st;
+  if ((st = cgemm_device(ctx, m, k, n, batch_size, broadcast_a != 0, make_float2(alpha.re, alpha.im),
+                         make_float2(beta.re, beta.im), transpose_a != 0, transpose_b != 0, dA.as<float2>(),
+                         dB.as<float2>(), dC.as<float2>())))
+    return st;
+  return download(c, dC, m * n * batch_size * 8, ctx);
+}
+
+llr_status llr_rhs_cdgmm_batched(llr_ctx* ctx, size_t m, size_t n, size_t batch_size, llr_cf32* matrix, size_t len,
+                                 const float* diag) {
+  if (len < m * n * batch_size) return fail(LLR_ERR_ARG, "matrix buffer is too small");
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf dA, dD;
+  llr_status st;
+  if ((st = upload(dA, matrix, m * n * batch_size * 8))) return st;
+  if ((st = upload(dD, diag, n * batch_size * 4))) return st;
+  if ((st = cdgmm_device(ctx, dA.as<float2>(), dD.as<float>(), m, n, batch_size))) return st;
+  return download(matrix, dA, m * n * batch_size * 8, ctx);
+}
+
+// ----------------------------------------------- signal_model.rs / cufft.rs
+llr_status llr_signal_model(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t image_size[3], size_t batch_size,
+                            const float* lin_ksp_corr, const float* lin_img_corr, int direction) {
+  size_t N = image_size[0] * image_size[1] * image_size[2];
+  if (len != N * batch_size) return fail(LLR_ERR_ARG, "data layout and data length inconsistency");
+  if (direction != LLR_FORWARD && direction != LLR_INVERSE) return fail(LLR_ERR_ARG, "bad direction");
+  llr_status st = check_fft_dims(image_size);
+  if (st) return st;
+  LLR_HIP(hipSetDevice(ctx->device));
+  RampTables t;
+  if ((st = make_tables(t, image_size, batch_size, lin_ksp_corr, lin_img_corr))) return st;
+  DevBuf d;
+  if ((st = upload(d, x, len * 8))) return st;
+  if ((st = model_device(ctx, d.as<float2>(), d.as<float2>(), image_size, batch_size, t, direction))) return st;
+  return download(x, d, len * 8, ctx);
+}
+
+llr_status llr_signal_model_device(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t image_size[3],
+                                   size_t batch_size, const float* lin_ksp_corr, const float* lin_img_corr,
+                                   int direction) {
+  size_t N = image_size[0] * image_size[1] * image_size[2];
+  if (len != N * batch_size) return fail(LLR_ERR_ARG, "data layout and data length inconsistency");
+  if (direction != LLR_FORWARD && direction != LLR_INVERSE) return fail(LLR_ERR_ARG, "bad direction");
+  llr_status st = check_fft_dims(image_size);
+  if (st) return st;
+  LLR_HIP(hipSetDevice(ctx->device));
+  RampTables t;
+  if ((st = make_tables(t, image_size, batch_size, lin_ksp_corr, lin_img_corr))) return st;
+  if ((st = model_device(ctx, (float2*)x, (float2*)x, image_size, batch_size, t, direction))) return st;
+  LLR_HIP(hipStreamSynchronize(ctx->stream));
+  return LLR_OK;
+}
+
+llr_status llr_estimate_linear_correction(llr_ctx* ctx, const llr_cf32* ksp, size_t len, const size_t vol_size[3],
+                                          float out[3]) {
+  size_t N = vol_size[0] * vol_size[1] * vol_size[2];
+  if (len != N) return fail(LLR_ERR_ARG, "mismatch between vol size and number of samples");
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf d, best;
+  llr_status st;
+  if ((st = upload(d, ksp, len * 8))) return st;
+  if ((st = upload(best, nullptr, 8))) return st;
+  LLR_HIP(hipMemsetAsync(best.p, 0, 8, ctx->stream));
+  if ((st = argmax_volumes(ctx, d.as<float2>(), N, 1, best.as<unsigned long long>()))) return st;
+  unsigned long long key = 0;
+  if ((st = download(&key, best, 8, ctx))) return st;
+  size_t idx = (size_t)(0xFFFFFFFFu - (unsigned)(key & 0xFFFFFFFFull));
+  size_t s[3] = {idx % vol_size[0], (idx / vol_size[0]) % vol_size[1], idx / (vol_size[0] * vol_size[1])};
+  for (int a = 0; a < 3; ++a) out[a] = (float)((long long)s[a] - ((long long)vol_size[a] + 1) / 2);
+  return LLR_OK;
+}
+
+llr_status llr_fftn_batch(llr_ctx* ctx, const size_t dims[3], size_t batches, int direction, int norm, llr_cf32* data,
+                          size_t len) {
+  size_t N = dims[0] * dims[1] * dims[2];
+  if (len != N * batches) return fail(LLR_ERR_ARG, "data layout and data length inconsistency");
+  llr_status st = check_fft_dims(dims);
+  if (st) return st;
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf d;
+  if ((st = upload(d, data, len * 8))) return st;
+  for (int a = 0; a < 3; ++a) {
+    double s = 1.0;
+    if (norm == LLR_NORM_UNITARY) s = 1.0 / std::sqrt((double)dims[a]);
+    if (norm == LLR_NORM_INVERSE && direction == LLR_INVERSE) s = 1.0 / (double)dims[a];
+    if ((st = fft_axis_pass(ctx, d.as<float2>(), d.as<float2>(), dims, batches, a, direction == LLR_INVERSE, nullptr,
+                            false, nullptr, false, (float)s)))
+      return st;
+  }
+  return download(data, d, len * 8, ctx);
+}
+
+// ---------------------------------------------------------------- fftshift.rs
+static llr_status shift_common(llr_ctx* ctx, void* data, size_t eb, const size_t* dims, bool inverse) {
+  size_t N = dims[0] * dims[1] * dims[2];
+  if (eb == 0 || N == 0) return fail(LLR_ERR_ARG, "empty data");
+  long long sh[3];
+  for (int a = 0; a < 3; ++a) sh[a] = inverse ? (long long)((dims[a] + 1) / 2) : (long long)(dims[a] / 2);
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf a, b;
+  llr_status st;
+  if ((st = upload(a, data, N * eb))) return st;
+  if ((st = upload(b, nullptr, N * eb))) return st;
+  if ((st = circshift_device(ctx, a.p, b.p, eb, dims, sh))) return st;
+  return download(data, b, N * eb, ctx);
+}
+
+llr_status llr_fftshift(llr_ctx* ctx, void* data, size_t elem_bytes, const size_t dims[3]) {
+  return shift_common(ctx, data, elem_bytes, dims, false);
+}
+llr_status llr_ifftshift(llr_ctx* ctx, void* data, size_t elem_bytes, const size_t dims[3]) {
+  return shift_common(ctx, data, elem_bytes, dims, true);
+}
+
+llr_status llr_phase_shift3(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t dims[3], int direction) {
+  if (len != dims[0] * dims[1] * dims[2]) return fail(LLR_ERR_ARG, "dims must agree with the size of x");
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf d;
+  llr_status st;
+  if ((st = upload(d, x, len * 8))) return st;
+  if ((st = phase_shift3_device(ctx, d.as<float2>(), dims, direction == LLR_FORWARD ? -1 : 1))) return st;
+  return download(x, d, len * 8, ctx);
+}
+
+// --------------------------------------------------------------- lib.rs helpers
+llr_status llr_wavelet_denoise(llr_ctx* ctx, llr_cf32* volumes, size_t n_volumes, const size_t vol_size[3], float alpha,
+                               int levels) {
+  size_t N = vol_size[0] * vol_size[1] * vol_size[2];
+  if (N == 0 || n_volumes == 0) return LLR_OK;
+  int L = levels < 0 ? wavelet_max_level(vol_size) : levels;
+  size_t ce = 0, te = 0;
+  wavelet_workspace_elems(vol_size, L, &ce, &te);
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf d, coef, t1, t2;
+  llr_status st;
+  if ((st = upload(d, volumes, N * n_volumes * 8))) return st;
+  if ((st = upload(coef, nullptr, ce * n_volumes * 8))) return st;
+  if ((st = upload(t1, nullptr, te * n_volumes * 8))) return st;
+  if ((st = upload(t2, nullptr, te * n_volumes * 8))) return st;
+  if ((st = wavelet_denoise_device(ctx, d.as<float2>(), n_volumes, vol_size, alpha, L, coef.as<float2>(),
+                                   t1.as<float2>(), t2.as<float2>())))
+    return st;
+  return download(volumes, d, N * n_volumes * 8, ctx);
+}
+
+llr_status llr_hard_project(llr_ctx* ctx, llr_cf32* x, const llr_cf32* y, size_t len, double* sum_sq_error) {
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf dx, dy, part, out;
+  const size_t np = 1024;
+  llr_status st;
+  if ((st = upload(dx, x, len * 8))) return st;
+  if ((st = upload(dy, y, len * 8))) return st;
+  if ((st = upload(part, nullptr, np * 8))) return st;
+  if ((st = upload(out, nullptr, 8))) return st;
+  if ((st = hard_project_partials(ctx, dx.as<float2>(), dy.as<float2>(), len, part.as<double>(), np))) return st;
+  if ((st = reduce_strided(ctx, part.as<double>(), np, 1, out.as<double>()))) return st;
+  double e = 0;
+  if ((st = download(&e, out, 8, ctx))) return st;
+  if (sum_sq_error) *sum_sq_error = e;
+  return download(x, dx, len * 8, ctx);
+}
+
+llr_status llr_calc_undersampling_factor(llr_ctx* ctx, const llr_cf32* y, size_t len, double* factor,
+                                         double* total_energy) {
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf dy, part, out;
+  const size_t np = 1024;
+  llr_status st;
+  if ((st = upload(dy, y, len * 8))) return st;
+  if ((st = upload(part, nullptr, 2 * np * 8))) return st;
+  if ((st = upload(out, nullptr, 16))) return st;
+  if ((st = energy_partials(ctx, dy.as<float2>(), len, part.as<double>(), np))) return st;
+  if ((st = reduce_strided(ctx, part.as<double>(), np, 2, out.as<double>()))) return st;
+  double r[2];
+  if ((st = download(r, out, 16, ctx))) return st;
+  if (factor) *factor = (double)len / r[1];
+  if (total_energy) *total_energy = r[0];
+  return LLR_OK;
+}
+
+llr_status llr_under_sample_kspace(llr_ctx* ctx, llr_cf32* data, const size_t dims[3], size_t nq, const llr_cf32* mask) {
+  size_t n = dims[0] * dims[1] * dims[2] * nq;
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf d, m;
+  llr_status st;
+  if ((st = upload(d, data, n * 8))) return st;
+  if ((st = upload(m, mask, dims[1] * dims[2] * nq * 8))) return st;
+  if ((st = under_sample_device(ctx, d.as<float2>(), dims, nq, m.as<float2>()))) return st;
+  return download(data, d, n * 8, ctx);
+}
+
+}  // extern "C"
+
+// ======================================================= device-resident recon
+struct llr_recon {
+  llr_ctx* ctx = nullptr;
+  llr_recon_params p{};
+  std::vector<size_t> shifts;
+  size_t N = 0, nq = 0, D = 0;
+  int nc = 0;       // columns of the LLR matrices (srank or nq)
+  int wL = 0;       // wavelet levels
+  DevBuf X, Y, Cb, S, coef, t1, t2, partials, stats, hist, smax, best;
+  RampTables tabs;
+  size_t nparts = 0;
+  bool have_meas = false, have_init = false;
+  double lambda = 0.0, energy = 0.0;
+  std::vector<float> ksp_corr;
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double stage_ms[LLR_NUM_STAGES] = {0};
+  size_t stage_n[LLR_NUM_STAGES] = {0};
+};
+
+namespace {
+
+struct StageTimer {
+  llr_recon* r;
+  int stage;
+  StageTimer(llr_recon* r_, int s) : r(r_), stage(s) {
+    if (r->timing) (void)hipEventRecord(r->ev0, r->ctx->stream);
+  }
+  ~StageTimer() {
+    if (!r->timing) return;
+    (void)hipEventRecord(r->ev1, r->ctx->stream);
+    (void)hipEventSynchronize(r->ev1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, r->ev0, r->ev1);
+    r->stage_ms[stage] += ms;
+    r->stage_n[stage] += 1;
+  }
+};
+
+llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, unsigned* smax) {
+  llr_ctx* ctx = r->ctx;
+  const llr_recon_params& p = r->p;
+  size_t grid[3];
+  llr_grid_dim(p.vol, p.blk, grid);
+  long long nb = (long long)(grid[0] * grid[1] * grid[2]);
+  llr_status st;
+  if (p.srank) {
+    {
+      StageTimer t(r, 1);
+      st = project_device(ctx, r->X.as<float2>(), r->Cb.as<float2>(), r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank);
+    }
+    if (st) return st;
+    StageTimer t(r, 0);
+    return llr_block_run(ctx, r->Cb.as<float2>(), r->X.as<float2>(), r->S.as<float2>(), (int)r->nq, p.vol, p.blk, shift,
+                         r->nc, 0, 0, nb, mode, (int)p.rank, lambda, smax, nullptr);
+  }
+  StageTimer t(r, 0);
+  return llr_block_run(ctx, r->X.as<float2>(), r->X.as<float2>(), nullptr, 0, p.vol, p.blk, shift, r->nc, 0, 0, nb, mode,
+                       (int)p.rank, lambda, smax, nullptr);
+}
+
+llr_status recon_iteration(llr_recon* r, size_t it) {
+  llr_ctx* ctx = r->ctx;
+  const llr_recon_params& p = r->p;
+  llr_status st = recon_llr(r, &r->shifts[3 * it], p.llr_mode, (float)r->lambda, nullptr);
+  if (st) return st;
+  if (p.wavelet_alpha > 0) {
+    StageTimer t(r, 2);
+    st = wavelet_denoise_device(ctx, r->X.as<float2>(), r->nq, p.vol, (float)p.wavelet_alpha, r->wL, r->coef.as<float2>(),
+                                r->t1.as<float2>(), r->t2.as<float2>());
+    if (st) return st;
+  }
+  float2* X = r->X.as<float2>();
+  const RampTables& T = r->tabs;
+  float sx = (float)(1.0 / std::sqrt((double)p.vol[0])), sy = (float)(1.0 / std::sqrt((double)p.vol[1]));
+  {
+    StageTimer t(r, 3);
+    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, false, T.pk[0].as<float2>(), false, T.pi[0].as<float2>(), true, sx);
+  }
+  if (st) return st;
+  {
+    StageTimer t(r, 4);
+    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, false, T.pk[1].as<float2>(), false, T.pi[1].as<float2>(), true, sy);
+  }
+  if (st) return st;
+  {
+    StageTimer t(r, 5);
+    size_t np = 0;
+    st = fft_zproj(ctx, X, r->Y.as<float2>(), p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(),
+                   r->partials.as<double>(), &np);
+  }
+  if (st) return st;
+  {
+    StageTimer t(r, 4);
+    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, true, T.pi[1].as<float2>(), false, T.pk[1].as<float2>(), true, sy);
+  }
+  if (st) return st;
+  {
+    StageTimer t(r, 3);
+    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false, T.pk[0].as<float2>(), true, sx);
+  }
+  if (st) return st;
+  StageTimer t(r, 6);
+  return reduce_strided(ctx, r->partials.as<double>(), r->nparts, 1, r->hist.as<double>() + it);
+}
+
+}  // namespace
+
+extern "C" {
+
+llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_recon** out) {
+  if (!ctx || !params || !out) return fail(LLR_ERR_ARG, "null argument");
+  const llr_recon_params& p = *params;
+  llr_status st = check_fft_dims(p.vol);
+  if (st) return st;
+  if ((st = check_blocks(p.vol, p.blk))) return st;
+  if (p.nq == 0) return fail(LLR_ERR_ARG, "no volumes");
+  size_t m = p.blk[0] * p.blk[1] * p.blk[2];
+  if (m <= p.nq) return fail(LLR_ERR_ARG, "only tall matrices are supported");  // lib.rs:203
+  int nc = (int)(p.srank ? p.srank : p.nq);
+  if (p.llr_mode == LLR_MODE_RANK && (p.rank == 0 || p.rank > (size_t)nc))
+    return fail(LLR_ERR_ARG, "rank must be in 1..=columns");  // lib.rs:205-207
+  if (p.srank && p.rank > p.srank && p.llr_mode == LLR_MODE_RANK)
+    return fail(LLR_ERR_ARG, "llr rank must be less than or equal to subspace rank");  // lib.rs:217
+  if (nc > 8) return fail(LLR_ERR_UNSUPPORTED, "the fused LLR kernel handles up to 8 matrix columns");
+  if (p.srank && !p.subspace) return fail(LLR_ERR_ARG, "subspace rank given without a subspace matrix");
+  if (p.n_iter && !p.shifts) return fail(LLR_ERR_ARG, "shift sequence required");
+  if (p.precision != LLR_PREC_FP32)
+    return fail(LLR_ERR_UNSUPPORTED, "only the FP32 precision form is available in this build");
+  LLR_HIP(hipSetDevice(ctx->device));
+  llr_recon* r = new llr_recon();
+  r->ctx = ctx;
+  r->p = p;
+  r->nq = p.nq;
+  r->N = p.vol[0] * p.vol[1] * p.vol[2];
+  r->D = r->N * p.nq;
+  r->nc = nc;
+  r->shifts.assign(p.shifts, p.shifts + 3 * p.n_iter);
+  r->p.shifts = nullptr;
+  r->p.subspace = nullptr;
+  auto bail = [&](llr_status s) {
+    delete r;
+    return s;
+  };
+  if ((st = upload(r->X, nullptr, r->D * 8))) return bail(st);
+  if ((st = upload(r->Y, nullptr, r->D * 8))) return bail(st);
+  if (p.srank) {
+    if ((st = upload(r->Cb, nullptr, r->N * p.srank * 8))) return bail(st);
+    if ((st = upload(r->S, p.subspace, p.nq * p.srank * 8))) return bail(st);
+  }
+  if (p.wavelet_alpha > 0) {
+    r->wL = p.wavelet_levels < 0 ? wavelet_max_level(p.vol) : p.wavelet_levels;
+    size_t ce = 0, te = 0;
+    wavelet_workspace_elems(p.vol, r->wL, &ce, &te);
+    if ((st = upload(r->coef, nullptr, ce * p.nq * 8))) return bail(st);
+    if ((st = upload(r->t1, nullptr, te * p.nq * 8))) return bail(st);
+    if ((st = upload(r->t2, nullptr, te * p.nq * 8))) return bail(st);
+  }
+  size_t np = 0;
+  fft_zproj(ctx, nullptr, nullptr, p.vol, p.nq, nullptr, nullptr, nullptr, &np);
+  r->nparts = np;
+  if ((st = upload(r->partials, nullptr, std::max<size_t>(np, 2048) * 8))) return bail(st);
+  if ((st = upload(r->stats, nullptr, 16))) return bail(st);
+  if ((st = upload(r->hist, nullptr, std::max<size_t>(p.n_iter, 1) * 8))) return bail(st);
+  if ((st = upload(r->smax, nullptr, 8))) return bail(st);
+  if ((st = upload(r->best, nullptr, p.nq * 8))) return bail(st);
+  if (hipEventCreate(&r->ev0) != hipSuccess || hipEventCreate(&r->ev1) != hipSuccess)
+    return bail(fail(LLR_ERR_HIP, "hipEventCreate failed"));
+  *out = r;
+  return LLR_OK;
+}
+
+void llr_recon_destroy(llr_recon* r) {
+  if (!r) return;
+  (void)hipSetDevice(r->ctx->device);
+  (void)hipStreamSynchronize(r->ctx->stream);
+  if (r->ev0) (void)hipEventDestroy(r->ev0);
+  if (r->ev1) (void)hipEventDestroy(r->ev1);
+  delete r;
+}
+
+llr_status llr_recon_set_measurements(llr_recon* r, const llr_cf32* y, size_t len, int on_device) {
+  if (!r || !y) return fail(LLR_ERR_ARG, "null argument");
+  if (len != r->D) return fail(LLR_ERR_ARG, "measurements must hold nx*ny*nz*nq samples");
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  LLR_HIP(hipMemcpyAsync(r->Y.p, y, len * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                         r->ctx->stream));
+  LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  r->have_meas = true;
+  return LLR_OK;
+}
+
+llr_status llr_recon_set_initial(llr_recon* r, const llr_cf32* x, size_t len, int on_device) {
+  if (!r || !x) return fail(LLR_ERR_ARG, "null argument");
+  if (len != r->D) return fail(LLR_ERR_ARG, "initial image must hold nx*ny*nz*nq samples");
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  LLR_HIP(hipMemcpyAsync(r->X.p, x, len * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                         r->ctx->stream));
+  LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  r->have_init = true;
+  return LLR_OK;
+}
+
+llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done) {
+  if (!r) return fail(LLR_ERR_ARG, "null recon");
+  if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set");
+  llr_ctx* ctx = r->ctx;
+  const llr_recon_params& p = r->p;
+  LLR_HIP(hipSetDevice(ctx->device));
+  llr_status st;
+  // total energy (lib.rs:224, 409-417)
+  {
+    StageTimer t(r, 6);
+    if ((st = energy_partials(ctx, r->Y.as<float2>(), r->D, r->partials.as<double>(), 1024))) return st;
+    if ((st = reduce_strided(ctx, r->partials.as<double>(), 1024, 2, r->stats.as<double>()))) return st;
+  }
+  // linear k-space corrections (lib.rs:229-231)
+  LLR_HIP(hipMemsetAsync(r->best.p, 0, r->nq * 8, ctx->stream));
+  if ((st = argmax_volumes(ctx, r->Y.as<float2>(), r->N, r->nq, r->best.as<unsigned long long>()))) return st;
+  std::vector<unsigned long long> best(r->nq);
+  double stats[2];
+  LLR_HIP(hipStreamSynchronize(ctx->stream));
+  LLR_HIP(hipMemcpy(best.data(), r->best.p, r->nq * 8, hipMemcpyDeviceToHost));
+  LLR_HIP(hipMemcpy(stats, r->stats.p, 16, hipMemcpyDeviceToHost));
+  r->energy = stats[0];
+  r->ksp_corr.assign(3 * r->nq, 0.f);
+  for (size_t q = 0; q < r->nq; ++q) {
+    size_t idx = (size_t)(0xFFFFFFFFu - (unsigned)(best[q] & 0xFFFFFFFFull));
+    size_t s[3] = {idx % p.vol[0], (idx / p.vol[0]) % p.vol[1], idx / (p.vol[0] * p.vol[1])};
+    for (int a = 0; a < 3; ++a) r->ksp_corr[3 * q + a] = (float)((long long)s[a] - ((long long)p.vol[a] + 1) / 2);
+  }
+  if ((st = make_tables(r->tabs, p.vol, r->nq, r->ksp_corr.data(), nullptr))) return st;
+  // zero-filled adjoint (lib.rs:233-240) unless warm-started
+  if (!r->have_init) {
+    StageTimer t(r, 3);
+    if ((st = model_device(ctx, r->Y.as<float2>(), r->X.as<float2>(), p.vol, r->nq, r->tabs, LLR_INVERSE))) return st;
+  }
+  // threshold: lambda_factor * sigma_max of the starting blocks at shift 0
+  if (p.llr_mode == LLR_MODE_SVT) {
+    if (p.svt_lambda > 0) {
+      r->lambda = p.svt_lambda;
+    } else {
+      LLR_HIP(hipMemsetAsync(r->smax.p, 0, 8, ctx->stream));
+      size_t zero[3] = {0, 0, 0};
+      if ((st = recon_llr(r, zero, 2, 0.f, r->smax.as<unsigned>()))) return st;
+      unsigned bits = 0;
+      LLR_HIP(hipStreamSynchronize(ctx->stream));
+      LLR_HIP(hipMemcpy(&bits, r->smax.p, 4, hipMemcpyDeviceToHost));
+      float smax;
+      std::memcpy(&smax, &bits, 4);
+      r->lambda = p.lambda_factor * (double)smax;
+    }
+  }
+  size_t it = 0;
+  for (; it < p.n_iter; ++it) {
+    if ((st = recon_iteration(r, it))) return st;
+    if (p.residual_stop > 0) {
+      double e = 0;
+      LLR_HIP(hipStreamSynchronize(ctx->stream));
+      LLR_HIP(hipMemcpy(&e, r->hist.as<double>() + it, 8, hipMemcpyDeviceToHost));
+      if (e / r->energy <= p.residual_stop) {
+        ++it;
+        break;
+      }
+    }
+  }
+  LLR_HIP(hipStreamSynchronize(ctx->stream));
+  if (err_hist && it) {
+    std::vector<double> h(it);
+    LLR_HIP(hipMemcpy(h.data(), r->hist.p, it * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < it; ++i) err_hist[i] = h[i] / r->energy;
+  }
+  if (iters_done) *iters_done = it;
+  return LLR_OK;
+}
+
+llr_status llr_recon_get_image(llr_recon* r, llr_cf32* out, size_t len, int on_device) {
+  if (!r || !out) return fail(LLR_ERR_ARG, "null argument");
+  if (len != r->D) return fail(LLR_ERR_ARG, "output must hold nx*ny*nz*nq samples");
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  LLR_HIP(hipMemcpyAsync(out, r->X.p, len * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                         r->ctx->stream));
+  LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  return LLR_OK;
+}
+
+llr_status llr_recon_info(llr_recon* r, double* lambda, float* ksp_corr) {
+  if (!r) return fail(LLR_ERR_ARG, "null recon");
+  if (lambda) *lambda = r->lambda;
+  if (ksp_corr && !r->ksp_corr.empty()) std::memcpy(ksp_corr, r->ksp_corr.data(), r->ksp_corr.size() * sizeof(float));
+  return LLR_OK;
+}
+
+llr_status llr_recon_enable_timing(llr_recon* r, int enable) {
+  if (!r) return fail(LLR_ERR_ARG, "null recon");
+  r->timing = enable != 0;
+  for (int i = 0; i < LLR_NUM_STAGES; ++i) {
+    r->stage_ms[i] = 0;
+    r->stage_n[i] = 0;
+  }
+  return LLR_OK;
+}
+
+llr_status llr_recon_stage_times(llr_recon* r, double* total_ms, size_t* launches) {
+  if (!r) return fail(LLR_ERR_ARG, "null recon");
+  for (int i = 0; i < LLR_NUM_STAGES; ++i) {
+    if (total_ms) total_ms[i] = r->stage_ms[i];
+    if (launches) launches[i] = r->stage_n[i];
+  }
+  return LLR_OK;
+}
+
+}  // extern "C"

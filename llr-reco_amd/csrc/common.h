@@ -1,0 +1,95 @@
+// Shared device helpers and host-side context for the LLR HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/llr_hip.h"
+
+// ------------------------------------------------------------ complex math
+// Interleaved complex64 as float2; textbook formulas (no inf/nan recovery).
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__host__ __device__ inline float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// conj(a) * b
+__host__ __device__ inline float2 cconjmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__host__ __device__ inline float cnorm(float2 a) { return a.x * a.x + a.y * a.y; }
+
+// ------------------------------------------------------------- host context
+struct llr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // cached FFT twiddles exp(-2*pi*i*m/n), m in [0, n), keyed by n
+  std::vector<std::pair<int, float2*>> twiddles;
+  // per-call scratch for reductions
+  double* d_red = nullptr;
+  size_t red_cap = 0;
+};
+
+namespace llr {
+
+void set_error(const std::string& msg);
+llr_status fail(llr_status st, const std::string& msg);
+llr_status hip_fail(hipError_t e, const char* what);
+
+#define LLR_HIP(call)                                         \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return ::llr::hip_fail(e_, #call);  \
+  } while (0)
+
+#define LLR_CHECK(cond, msg)                                       \
+  do {                                                             \
+    if (!(cond)) return ::llr::fail(LLR_ERR_ARG, (msg));           \
+  } while (0)
+
+// RAII device buffer used by the host-pointer entry points.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t b) {
+    bytes = b;
+    return hipMalloc(&p, b ? b : 1);
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+// Twiddle table of length n (exp(-2*pi*i*m/n)) computed in long double on the
+// host; cached per context.
+llr_status get_twiddles(llr_ctx* ctx, int n, const float2** out);
+
+// Launch-shape helper: blocks for `total` work items, capped for grid-stride loops.
+inline unsigned grid_for(size_t total, unsigned block, unsigned cap = 65535u * 8u) {
+  size_t g = (total + block - 1) / block;
+  if (g == 0) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// Reduction scratch of `n` doubles on the context's device.
+llr_status red_scratch(llr_ctx* ctx, size_t n, double** out);
+
+}  // namespace llr

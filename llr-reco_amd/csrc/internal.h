@@ -1,0 +1,58 @@
+// Internal interfaces between the kernel files and the C ABI layer.
+#pragma once
+
+#include <array>
+#include <vector>
+
+#include "common.h"
+
+namespace llr {
+
+// fft.hip
+bool fft_size_supported(size_t n);
+llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const size_t* dims, size_t nvols, int axis,
+                         bool inverse, const float2* pre, bool pre_conj, const float2* post, bool post_conj,
+                         float scale);
+llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_t* dims, size_t nvols,
+                     const float2* pk, const float2* pi, double* partial, size_t* nparts);
+
+// llr.hip
+llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols, size_t eb, const size_t* vol,
+                       const size_t* blk, const size_t* shift, size_t start, size_t n, bool unlift);
+llr_status project_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K);
+llr_status llr_block_run(llr_ctx* ctx, const float2* F, float2* out, const float2* S, int nq, const size_t* vol,
+                         const size_t* blk, const size_t* shift, int nc, int layout, long long block0,
+                         long long nblocks, int mode, int rank, float lambda, unsigned* smax_bits, float* svals);
+llr_status lowrank_generic_device(llr_ctx* ctx, float2* mats, long long m, int n, long long batch, int mode, int rank,
+                                  float lambda, unsigned* smax_bits);
+
+// wavelet.hip
+int wavelet_max_level(const size_t* dims);
+void wavelet_geometry(const size_t* dims, int L, std::vector<std::array<size_t, 3>>& in_dims,
+                      std::vector<std::array<size_t, 3>>& cdims);
+size_t wavelet_workspace_elems(const size_t* dims, int L, size_t* coef_elems, size_t* tmp_elems);
+llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, const size_t* dims, float alpha, int L,
+                                  float2* coef, float2* t1, float2* t2);
+
+// misc.hip
+llr_status shrink_all_device(llr_ctx* ctx, float2* x, size_t n, float alpha);
+llr_status energy_partials(llr_ctx* ctx, const float2* y, size_t n, double* partials, size_t nparts);
+llr_status reduce_strided(llr_ctx* ctx, const double* partials, size_t n, size_t stride, double* out);
+llr_status reduce_sum(llr_ctx* ctx, const double* partials, size_t nparts, double* out, double* tmp, size_t tmp_n);
+llr_status hard_project_partials(llr_ctx* ctx, float2* x, const float2* y, size_t n, double* partials, size_t nparts);
+llr_status argmax_volumes(llr_ctx* ctx, const float2* y, size_t N, size_t nvols, unsigned long long* best);
+llr_status under_sample_device(llr_ctx* ctx, float2* data, const size_t* dims, size_t nq, const float2* mask);
+llr_status circshift_device(llr_ctx* ctx, const void* src, void* dst, size_t eb, const size_t* dims, const long long* sh);
+llr_status phase_shift3_device(llr_ctx* ctx, float2* x, const size_t* dims, int sign);
+llr_status scale_device(llr_ctx* ctx, float2* x, size_t n, float s);
+llr_status cdgmm_device(llr_ctx* ctx, float2* a, const float* d, size_t m, size_t n, size_t batch);
+llr_status cgemm_device(llr_ctx* ctx, size_t m, size_t k, size_t n, size_t batch, bool bcast_a, float2 alpha,
+                        float2 beta, bool ta, bool tb, const float2* A, const float2* B, float2* C);
+llr_status conj_transpose_device(llr_ctx* ctx, const float2* src, float2* dst, size_t m, size_t n, size_t batch,
+                                 bool conj);
+
+// phase tables: exp(+i * 2*pi * frac((i - n/2) * (c + n/2) / n)) for each volume
+// (rows) and index i; host-side construction in long double.
+void ramp_table(std::vector<float2>& out, size_t n, const float* corr, size_t stride, size_t nvols);
+
+}  // namespace llr

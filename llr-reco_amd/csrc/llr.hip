@@ -1,0 +1,608 @@
+// LLR proximal step on gfx950.
+//
+//  - k_lift / k_unlift: the shifted-block gather/scatter of block.rs:8-184 as
+//    pure data movement (bit-exact), any element size.
+//  - k_project: C[k][v] = sum_q X[q][v] S[q][k], the subspace projection X*S of
+//    dti_project + conj_transpose (lib.rs:285-286) done per voxel over the
+//    volume-major iterate, fully coalesced.
+//  - k_llr_block: one workgroup per shifted block: gather the m x n Casorati
+//    matrix F (n <= 8) into LDS, Gram G = F^H F, Hermitian Jacobi eigen-solve of
+//    G by one wavefront (one matrix element per lane, the 7-round round-robin
+//    ordering of 8 indices), W = V diag(g(s)) V^H, then F*W (or F*W*S^H with a
+//    subspace, lib.rs:287-289) scattered back. g(s) = 1 for the top `rank`
+//    singular values (low_rank.rs:10-69) or max(s - lambda, 0)/s (SVT).
+//  - generic kernels for matrices with n > 8 columns (host drop-in API only):
+//    Gram, serial per-matrix Jacobi, apply.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "common.h"
+#include "internal.h"
+
+namespace {
+
+// ------------------------------------------------------------ lift / unlift
+struct GeomArgs {
+  long long vol[3], blk[3], grid[3], shift[3];
+  long long N;  // voxels per volume
+  long long m;  // voxels per block
+};
+
+__device__ __forceinline__ long long block_voxel_addr(const GeomArgs& g, long long b, long long i) {
+  long long bx = b % g.grid[0], by = (b / g.grid[0]) % g.grid[1], bz = b / (g.grid[0] * g.grid[1]);
+  long long ix = i % g.blk[0], iy = (i / g.blk[0]) % g.blk[1], iz = i / (g.blk[0] * g.blk[1]);
+  long long x = (ix + bx * g.blk[0] + g.shift[0]) % g.vol[0];
+  long long y = (iy + by * g.blk[1] + g.shift[1]) % g.vol[1];
+  long long z = (iz + bz * g.blk[2] + g.shift[2]) % g.vol[2];
+  return x + g.vol[0] * (y + g.vol[1] * z);
+}
+
+template <class E>
+__global__ void k_lift(const E* __restrict__ vols, E* __restrict__ mats, GeomArgs g, long long nvols,
+                       long long start, long long total) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    long long i = e % g.m, rest = e / g.m;
+    long long q = rest % nvols, c = rest / nvols;
+    mats[e] = vols[q * g.N + block_voxel_addr(g, start + c, i)];
+  }
+}
+
+template <class E>
+__global__ void k_unlift(E* __restrict__ vols, const E* __restrict__ mats, GeomArgs g, long long nvols,
+                         long long start, long long total) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    long long i = e % g.m, rest = e / g.m;
+    long long q = rest % nvols, c = rest / nvols;
+    vols[q * g.N + block_voxel_addr(g, start + c, i)] = mats[e];
+  }
+}
+
+// byte-granular fallback for element sizes other than 1/2/4/8/16
+__global__ void k_lift_bytes(const unsigned char* __restrict__ vols, unsigned char* __restrict__ mats, GeomArgs g,
+                             long long nvols, long long start, long long total, long long eb, int dir) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    long long byte = e % eb, el = e / eb;
+    long long i = el % g.m, rest = el / g.m;
+    long long q = rest % nvols, c = rest / nvols;
+    long long va = (q * g.N + block_voxel_addr(g, start + c, i)) * eb + byte;
+    if (dir == 0)
+      mats[e] = vols[va];
+    else
+      ((unsigned char*)vols)[va] = mats[e];
+  }
+}
+
+// ------------------------------------------------------ subspace projection
+template <int K>
+__global__ void __launch_bounds__(256) k_project(const float2* __restrict__ X, float2* __restrict__ Cp,
+                                                 const float2* __restrict__ S, long long N, int nq) {
+  for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < N; v += (long long)gridDim.x * blockDim.x) {
+    float2 acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = make_float2(0.f, 0.f);
+    for (int q = 0; q < nq; ++q) {
+      float2 x = X[(long long)q * N + v];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        float2 s = S[q + (long long)k * nq];
+        acc[k].x += x.x * s.x - x.y * s.y;
+        acc[k].y += x.x * s.y + x.y * s.x;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) Cp[(long long)k * N + v] = acc[k];
+  }
+}
+
+// ----------------------------------------------------- wavefront Jacobi (8x8)
+__device__ __forceinline__ float2 shfl2(float2 v, int src) {
+  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+// partner of index i in round k of the circle-method round robin on 8 indices
+__device__ __forceinline__ int rr_partner(int i, int k) {
+  if (i == 7) return k;
+  if (i == k) return 7;
+  int p = (2 * k - i) % 7;
+  return p < 0 ? p + 7 : p;
+}
+
+struct Rot {
+  float cs, sn;
+  float2 u;  // unit phase of A[p][q]
+};
+
+__device__ __forceinline__ Rot rot_params(float app, float aqq, float2 apq) {
+  Rot r;
+  float a = sqrtf(apq.x * apq.x + apq.y * apq.y);
+  if (a == 0.0f) {
+    r.cs = 1.f;
+    r.sn = 0.f;
+    r.u = make_float2(1.f, 0.f);
+    return r;
+  }
+  r.u = make_float2(apq.x / a, apq.y / a);
+  float theta = (aqq - app) / (2.0f * a);
+  float at = fabsf(theta);
+  float t = (at > 1e18f) ? 0.5f / theta : ((theta >= 0.f ? 1.f : -1.f) / (at + sqrtf(theta * theta + 1.f)));
+  r.cs = 1.0f / sqrtf(t * t + 1.f);
+  r.sn = t * r.cs;
+  return r;
+}
+
+// Element (row `which`, col `which`-partner) of the 2x2 rotation J for the
+// pair containing index i: returns J[i][i] and J[partner][i].
+__device__ __forceinline__ void rot_entries(const Rot& R, bool i_is_p, float2* Jii, float2* Jpi) {
+  float2 cu = make_float2(R.u.x, -R.u.y);
+  if (i_is_p) {
+    *Jii = make_float2(R.cs, 0.f);
+    *Jpi = make_float2(-R.sn * cu.x, -R.sn * cu.y);
+  } else {
+    *Jii = make_float2(R.cs * cu.x, R.cs * cu.y);
+    *Jpi = make_float2(R.sn, 0.f);
+  }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Jacobi eigen-solve of the Hermitian 8x8 matrix held one element per lane
+// (lane = r + 8c holds A[r][c]). On return a holds the (diagonal) eigenvalue
+// matrix and vv the eigenvectors (V[r][c], column c = eigenvector c).
+__device__ void wave_jacobi8(float2& a, float2& vv, int lane, int max_sweeps, float tol) {
+  const int r = lane & 7, c = lane >> 3;
+  vv = make_float2(r == c ? 1.f : 0.f, 0.f);
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    float off = wave_sum(r != c ? cnorm(a) : 0.f);
+    float tot = wave_sum(cnorm(a));
+    if (off <= tol * tol * tot || tot == 0.f) break;
+    for (int k = 0; k < 7; ++k) {
+      int pr = rr_partner(r, k), pc = rr_partner(c, k);
+      int p1 = min(r, pr), q1 = max(r, pr);
+      int p2 = min(c, pc), q2 = max(c, pc);
+      float app1 = __shfl(a.x, p1 * 9, 64), aqq1 = __shfl(a.x, q1 * 9, 64);
+      float2 apq1 = shfl2(a, p1 + 8 * q1);
+      float app2 = __shfl(a.x, p2 * 9, 64), aqq2 = __shfl(a.x, q2 * 9, 64);
+      float2 apq2 = shfl2(a, p2 + 8 * q2);
+      float2 a_r_pc = shfl2(a, r + 8 * pc);
+      float2 a_pr_c = shfl2(a, pr + 8 * c);
+      float2 a_pr_pc = shfl2(a, pr + 8 * pc);
+      float2 v_r_pc = shfl2(vv, r + 8 * pc);
+      Rot Rr = rot_params(app1, aqq1, apq1);
+      Rot Rc = rot_params(app2, aqq2, apq2);
+      float2 Jcc, Jpcc, Jrr, Jprr;
+      rot_entries(Rc, c == p2, &Jcc, &Jpcc);
+      rot_entries(Rr, r == p1, &Jrr, &Jprr);
+      float2 aj_r = cadd(cmul(a, Jcc), cmul(a_r_pc, Jpcc));
+      float2 aj_pr = cadd(cmul(a_pr_c, Jcc), cmul(a_pr_pc, Jpcc));
+      float2 an = cadd(cconjmul(Jrr, aj_r), cconjmul(Jprr, aj_pr));
+      vv = cadd(cmul(vv, Jcc), cmul(v_r_pc, Jpcc));
+      if (r != c && c == pr) an = make_float2(0.f, 0.f);
+      if (r == c) an.y = 0.f;
+      a = an;
+    }
+  }
+}
+
+// g(sigma_l) for eigenvalue index l, given all eigenvalues lam[0..7]
+__device__ __forceinline__ float shrink_weight(const float* lam, int l, int nc, int mode, int rank, float lambda) {
+  if (l >= nc) return 0.f;
+  if (mode == 0) {
+    int pos = 0;
+#pragma unroll
+    for (int o = 0; o < 8; ++o)
+      if (o < nc) pos += (lam[o] > lam[l] || (lam[o] == lam[l] && o < l)) ? 1 : 0;
+    return pos < rank ? 1.f : 0.f;
+  }
+  float s = sqrtf(fmaxf(lam[l], 0.f));
+  return (s > lambda && s > 0.f) ? (s - lambda) / s : 0.f;
+}
+
+// ---------------------------------------------------------- fused block kernel
+struct BlockArgs {
+  const float2* F;    // planes (layout 0) or matrices (layout 1)
+  float2* out;        // planes/matrices; with a subspace: nq image planes
+  const float2* S;    // nq x nc col-major, or null
+  GeomArgs g;
+  int nc;             // columns (<= 8)
+  int nq;             // output planes with a subspace
+  int layout;         // 0: shifted blocks of planes (plane stride g.N); 1: contiguous m x nc matrices
+  long long nblocks;  // blocks/matrices handled by this launch
+  long long block0;   // first block index
+  int per_xcd;        // blocks per XCD slot (XCD-aware remap)
+  int mode;           // 0 rank, 1 SVT, 2 singular values only
+  int rank;
+  float lambda;
+  unsigned* smax_bits;  // atomicMax of the largest singular value (float bits)
+  float* svals;         // optional [nblocks][8] singular values (unsorted)
+  int stage_lds;        // 1: F block staged in LDS
+};
+
+__device__ __forceinline__ long long elem_addr(const BlockArgs& A, long long b, long long i, int col) {
+  if (A.layout == 1) return b * A.g.m * A.nc + (long long)col * A.g.m + i;
+  return (long long)col * A.g.N + block_voxel_addr(A.g, b, i);
+}
+
+__global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
+  extern __shared__ float2 smem[];
+  const int tid = threadIdx.x;
+  const int nc = A.nc;
+  const long long m = A.g.m;
+  // XCD-aware remap: workgroups are dealt round-robin to the 8 XCDs, so give
+  // each XCD a contiguous run of blocks (neighbouring blocks share L2 lines).
+  long long slot = (long long)(blockIdx.x % 8) * A.per_xcd + blockIdx.x / 8;
+  if (slot >= A.nblocks) return;
+  const long long b = A.block0 + slot;
+  const long long mp = m + 1;  // padded column stride in LDS
+  float2* Fs = smem;                                      // nc * mp (when staged)
+  float2* Gp = smem + (A.stage_lds ? nc * mp : 0);        // 256 partial Gram entries
+  float2* Ws = Gp + 256;                                  // 64: W
+  float2* Ps = Ws + 64;                                   // nc * nq: W S^H
+
+  if (A.stage_lds) {
+    for (long long e = tid; e < m * nc; e += 256) {
+      long long i = e % m;
+      int col = (int)(e / m);
+      Fs[col * mp + i] = A.F[elem_addr(A, b, i, col)];
+    }
+    __syncthreads();
+  }
+  // Gram: entry e = (i <= j) for thread t % NE, rows interleaved by t / NE
+  const int NE = nc * (nc + 1) / 2;
+  const int NH = 256 / NE;
+  {
+    int e = tid % NE, h = tid / NE;
+    int i = 0, j = 0;
+    {
+      int rem = e;
+      for (int jj = 0; jj < nc; ++jj) {
+        if (rem <= jj) {
+          i = rem;
+          j = jj;
+          break;
+        }
+        rem -= jj + 1;
+      }
+    }
+    float2 acc = make_float2(0.f, 0.f);
+    if (h < NH) {
+      for (long long r = h; r < m; r += NH) {
+        float2 fi, fj;
+        if (A.stage_lds) {
+          fi = Fs[i * mp + r];
+          fj = Fs[j * mp + r];
+        } else {
+          fi = A.F[elem_addr(A, b, r, i)];
+          fj = A.F[elem_addr(A, b, r, j)];
+        }
+        acc.x += fi.x * fj.x + fi.y * fj.y;
+        acc.y += fi.x * fj.y - fi.y * fj.x;
+      }
+    }
+    Gp[tid] = acc;
+  }
+  __syncthreads();
+  // wave 0: reduce partials, Jacobi, weights, W
+  if (tid < 64) {
+    const int r = tid & 7, c = tid >> 3;
+    float2 a = make_float2(0.f, 0.f);
+    if (r < nc && c < nc) {
+      int i = min(r, c), j = max(r, c);
+      int e = j * (j + 1) / 2 + i;
+      float2 s = make_float2(0.f, 0.f);
+      for (int h = 0; h < NH; ++h) s = cadd(s, Gp[h * NE + e]);
+      a = (r <= c) ? s : cconj(s);
+      if (r == c) a.y = 0.f;
+    }
+    float2 vv;
+    wave_jacobi8(a, vv, tid, 16, 1.0e-7f);
+    float lam[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) lam[l] = __shfl(a.x, 9 * l, 64);
+    float lmax = 0.f;
+#pragma unroll
+    for (int l = 0; l < 8; ++l)
+      if (l < nc) lmax = fmaxf(lmax, lam[l]);
+    if (tid == 0) {
+      float smax = sqrtf(fmaxf(lmax, 0.f));
+      if (A.smax_bits) atomicMax(A.smax_bits, __float_as_uint(smax));
+    }
+    if (A.svals && tid < 8) A.svals[slot * 8 + tid] = sqrtf(fmaxf(__shfl(a.x, 9 * tid, 64), 0.f));
+    if (A.mode != 2) {
+      float gl[8];
+#pragma unroll
+      for (int l = 0; l < 8; ++l) gl[l] = shrink_weight(lam, l, nc, A.mode, A.rank, A.lambda);
+      float2 w = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int l = 0; l < 8; ++l) {
+        float2 vrl = shfl2(vv, r + 8 * l);
+        float2 vcl = shfl2(vv, c + 8 * l);
+        float2 t = cmulc(vrl, vcl);
+        w.x += gl[l] * t.x;
+        w.y += gl[l] * t.y;
+      }
+      Ws[tid] = w;  // W[r][c]
+    }
+  }
+  if (A.mode == 2) return;
+  __syncthreads();
+  if (A.S) {
+    // P[k][q] = sum_c W[k][c] conj(S[q][c])
+    for (int e = tid; e < nc * A.nq; e += 256) {
+      int k = e % nc, q = e / nc;
+      float2 acc = make_float2(0.f, 0.f);
+      for (int cc = 0; cc < nc; ++cc) acc = cadd(acc, cmulc(Ws[k + 8 * cc], A.S[q + (long long)cc * A.nq]));
+      Ps[k + nc * q] = acc;
+    }
+    __syncthreads();
+  }
+  // apply: each thread handles rows r = tid, tid+256, ...
+  for (long long rr = tid; rr < m; rr += 256) {
+    float2 f[8];
+    for (int k = 0; k < 8; ++k) {
+      if (k < nc)
+        f[k] = A.stage_lds ? Fs[k * mp + rr] : A.F[elem_addr(A, b, rr, k)];
+      else
+        f[k] = make_float2(0.f, 0.f);
+    }
+    if (A.S) {
+      long long addr = block_voxel_addr(A.g, b, rr);
+      for (int q = 0; q < A.nq; ++q) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (k < nc) acc = cadd(acc, cmul(f[k], Ps[k + nc * q]));
+        A.out[(long long)q * A.g.N + addr] = acc;
+      }
+    } else {
+      for (int cc = 0; cc < nc; ++cc) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (k < nc) acc = cadd(acc, cmul(f[k], Ws[k + 8 * cc]));
+        A.out[elem_addr(A, b, rr, cc)] = acc;
+      }
+    }
+  }
+}
+
+// ----------------------------------------------- generic path (n > 8 columns)
+// Gram of contiguous m x n matrices: G[b][i + n*j] for all i, j.
+__global__ void __launch_bounds__(256) k_gram_generic(const float2* __restrict__ Am, float2* __restrict__ G, long long m,
+                                                      int n) {
+  long long b = blockIdx.x;
+  const float2* a = Am + b * m * n;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    int i = e % n, j = e / n;
+    if (i > j) continue;
+    float2 acc = make_float2(0.f, 0.f);
+    for (long long r = 0; r < m; ++r) acc = cadd(acc, cconjmul(a[(long long)i * m + r], a[(long long)j * m + r]));
+    if (i == j) acc.y = 0.f;
+    G[b * n * n + i + (long long)n * j] = acc;
+    G[b * n * n + j + (long long)n * i] = cconj(acc);
+  }
+}
+
+// Serial cyclic Jacobi per matrix (one thread per matrix); G is overwritten
+// with the eigenvalue matrix, V receives the eigenvectors, W the weights matrix.
+__global__ void k_eig_serial(float2* __restrict__ G, float2* __restrict__ V, float2* __restrict__ W, long long batch,
+                             int n, int mode, int rank, float lambda, unsigned* smax_bits) {
+  long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  float2* A = G + b * n * n;
+  float2* Vb = V + b * n * n;
+  float2* Wb = W + b * n * n;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) Vb[i + n * j] = make_float2(i == j ? 1.f : 0.f, 0.f);
+  const float eps = FLT_EPSILON;
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    float off = 0.f, nrm = 0.f;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        float v = cnorm(A[i + n * j]);
+        nrm += v;
+        if (i != j) off += v;
+      }
+    if (off <= eps * eps * nrm || nrm == 0.f) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        float2 apq = A[p + n * q];
+        Rot R = rot_params(A[p + n * p].x, A[q + n * q].x, apq);
+        if (R.sn == 0.f) continue;
+        float2 cu = cconj(R.u);
+        for (int i = 0; i < n; ++i) {
+          float2 xp = A[i + n * p], xq = A[i + n * q];
+          A[i + n * p] = csub(cscale(xp, R.cs), cscale(cmul(cu, xq), R.sn));
+          A[i + n * q] = cadd(cscale(xp, R.sn), cscale(cmul(cu, xq), R.cs));
+        }
+        for (int j = 0; j < n; ++j) {
+          float2 xp = A[p + n * j], xq = A[q + n * j];
+          A[p + n * j] = csub(cscale(xp, R.cs), cscale(cmul(R.u, xq), R.sn));
+          A[q + n * j] = cadd(cscale(xp, R.sn), cscale(cmul(R.u, xq), R.cs));
+        }
+        A[p + n * q] = make_float2(0.f, 0.f);
+        A[q + n * p] = make_float2(0.f, 0.f);
+        A[p + n * p].y = 0.f;
+        A[q + n * q].y = 0.f;
+        for (int i = 0; i < n; ++i) {
+          float2 xp = Vb[i + n * p], xq = Vb[i + n * q];
+          Vb[i + n * p] = csub(cscale(xp, R.cs), cscale(cmul(cu, xq), R.sn));
+          Vb[i + n * q] = cadd(cscale(xp, R.sn), cscale(cmul(cu, xq), R.cs));
+        }
+      }
+  }
+  float lmax = 0.f;
+  for (int l = 0; l < n; ++l) lmax = fmaxf(lmax, A[l + n * l].x);
+  if (smax_bits) atomicMax(smax_bits, __float_as_uint(sqrtf(fmaxf(lmax, 0.f))));
+  // weights into the diagonal of A's imaginary slot is not needed; compute W
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      float2 acc = make_float2(0.f, 0.f);
+      for (int l = 0; l < n; ++l) {
+        float g;
+        if (mode == 0) {
+          int pos = 0;
+          for (int o = 0; o < n; ++o)
+            pos += (A[o + n * o].x > A[l + n * l].x || (A[o + n * o].x == A[l + n * l].x && o < l)) ? 1 : 0;
+          g = pos < rank ? 1.f : 0.f;
+        } else {
+          float s = sqrtf(fmaxf(A[l + n * l].x, 0.f));
+          g = (s > lambda && s > 0.f) ? (s - lambda) / s : 0.f;
+        }
+        acc = cadd(acc, cscale(cmulc(Vb[i + n * l], Vb[j + n * l]), g));
+      }
+      Wb[i + n * j] = acc;
+    }
+}
+
+// out = A W for contiguous m x n matrices (in place allowed: rows are read
+// into registers before being written, one row per thread).
+__global__ void __launch_bounds__(256) k_apply_generic(float2* __restrict__ Am, const float2* __restrict__ W, long long m,
+                                                       int n) {
+  long long b = blockIdx.y;
+  float2* a = Am + b * m * n;
+  const float2* w = W + b * n * n;
+  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < m; r += (long long)gridDim.x * blockDim.x) {
+    float2 row[128];
+    for (int k = 0; k < n; ++k) row[k] = a[(long long)k * m + r];
+    for (int j = 0; j < n; ++j) {
+      float2 acc = make_float2(0.f, 0.f);
+      for (int k = 0; k < n; ++k) acc = cadd(acc, cmul(row[k], w[k + n * j]));
+      a[(long long)j * m + r] = acc;
+    }
+  }
+}
+
+}  // namespace
+
+namespace llr {
+
+GeomArgs make_geom(const size_t* vol, const size_t* blk, const size_t* shift) {
+  GeomArgs g;
+  for (int a = 0; a < 3; ++a) {
+    g.vol[a] = (long long)vol[a];
+    g.blk[a] = (long long)blk[a];
+    g.grid[a] = (long long)(vol[a] / blk[a]);
+    g.shift[a] = (long long)shift[a];
+  }
+  g.N = g.vol[0] * g.vol[1] * g.vol[2];
+  g.m = g.blk[0] * g.blk[1] * g.blk[2];
+  return g;
+}
+
+llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols, size_t eb, const size_t* vol,
+                       const size_t* blk, const size_t* shift, size_t start, size_t n, bool unlift) {
+  GeomArgs g = make_geom(vol, blk, shift);
+  long long total = (long long)n * (long long)nvols * g.m;
+  if (total == 0) return LLR_OK;
+  unsigned grid = grid_for((size_t)total, 256);
+  hipStream_t s = ctx->stream;
+#define LIFT_CASE(E)                                                                                        \
+  if (unlift)                                                                                               \
+    hipLaunchKernelGGL(k_unlift<E>, dim3(grid), dim3(256), 0, s, (E*)vols, (const E*)mats, g, (long long)nvols, \
+                       (long long)start, total);                                                            \
+  else                                                                                                      \
+    hipLaunchKernelGGL(k_lift<E>, dim3(grid), dim3(256), 0, s, (const E*)vols, (E*)mats, g, (long long)nvols, \
+                       (long long)start, total);
+  switch (eb) {
+    case 1: LIFT_CASE(uint8_t); break;
+    case 2: LIFT_CASE(uint16_t); break;
+    case 4: LIFT_CASE(uint32_t); break;
+    case 8: LIFT_CASE(uint2); break;
+    case 16: LIFT_CASE(uint4); break;
+    default: {
+      long long tb = total * (long long)eb;
+      hipLaunchKernelGGL(k_lift_bytes, dim3(grid_for((size_t)tb, 256)), dim3(256), 0, s, (const unsigned char*)vols,
+                         (unsigned char*)mats, g, (long long)nvols, (long long)start, tb, (long long)eb, unlift ? 1 : 0);
+    }
+  }
+#undef LIFT_CASE
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+llr_status project_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K) {
+  unsigned grid = grid_for(N, 256, 65536u);
+#define PROJ_CASE(KK) \
+  case KK: hipLaunchKernelGGL(k_project<KK>, dim3(grid), dim3(256), 0, ctx->stream, X, Cp, S, (long long)N, nq); break;
+  switch (K) {
+    PROJ_CASE(1) PROJ_CASE(2) PROJ_CASE(3) PROJ_CASE(4) PROJ_CASE(5) PROJ_CASE(6) PROJ_CASE(7) PROJ_CASE(8)
+    PROJ_CASE(9) PROJ_CASE(10) PROJ_CASE(11) PROJ_CASE(12) PROJ_CASE(13) PROJ_CASE(14) PROJ_CASE(15) PROJ_CASE(16)
+    default: return fail(LLR_ERR_UNSUPPORTED, "subspace rank above 16");
+  }
+#undef PROJ_CASE
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+// Shared-memory bytes of k_llr_block for a block of m rows and nc columns.
+size_t llr_block_smem(long long m, int nc, int nq, bool stage) {
+  size_t f = stage ? (size_t)nc * (size_t)(m + 1) : 0;
+  return (f + 256 + 64 + (size_t)8 * (size_t)nq) * sizeof(float2);
+}
+
+// Fused LLR prox on blocks [block0, block0 + nblocks).
+//   layout 0: F = nc planes of N voxels, blocks are shifted blocks (geom g)
+//   layout 1: F = contiguous m x nc matrices
+llr_status llr_block_run(llr_ctx* ctx, const float2* F, float2* out, const float2* S, int nq, const size_t* vol,
+                         const size_t* blk, const size_t* shift, int nc, int layout, long long block0,
+                         long long nblocks, int mode, int rank, float lambda, unsigned* smax_bits, float* svals) {
+  GeomArgs g = make_geom(vol, blk, shift);
+  if (nc < 1 || nc > 8) return fail(LLR_ERR_UNSUPPORTED, "fused LLR kernel handles 1..8 columns");
+  if (nblocks <= 0) return LLR_OK;
+  BlockArgs A{};
+  A.F = F;
+  A.out = out;
+  A.S = S;
+  A.g = g;
+  A.nc = nc;
+  A.nq = S ? nq : 0;
+  A.layout = layout;
+  A.nblocks = nblocks;
+  A.block0 = block0;
+  A.per_xcd = (int)((nblocks + 7) / 8);
+  A.mode = mode;
+  A.rank = rank;
+  A.lambda = lambda;
+  A.smax_bits = smax_bits;
+  A.svals = svals;
+  size_t smem = llr_block_smem(g.m, nc, A.nq, true);
+  A.stage_lds = smem <= 64 * 1024 ? 1 : 0;
+  if (!A.stage_lds) smem = llr_block_smem(g.m, nc, A.nq, false);
+  if (smem > 64 * 1024) return fail(LLR_ERR_UNSUPPORTED, "subspace with too many volumes for the LLR kernel");
+  dim3 grid((unsigned)(8 * A.per_xcd));
+  hipLaunchKernelGGL(k_llr_block, grid, dim3(256), smem, ctx->stream, A);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+// Generic low-rank/SVT of contiguous m x n matrices (any n <= 128).
+llr_status lowrank_generic_device(llr_ctx* ctx, float2* mats, long long m, int n, long long batch, int mode, int rank,
+                                  float lambda, unsigned* smax_bits) {
+  if (n > 128) return fail(LLR_ERR_UNSUPPORTED, "more than 128 columns");
+  DevBuf G, V, W;
+  size_t nn = (size_t)n * n * (size_t)batch * sizeof(float2);
+  LLR_HIP(G.alloc(nn));
+  LLR_HIP(V.alloc(nn));
+  LLR_HIP(W.alloc(nn));
+  hipLaunchKernelGGL(k_gram_generic, dim3((unsigned)batch), dim3(256), 0, ctx->stream, mats, G.as<float2>(), m, n);
+  LLR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_eig_serial, dim3(grid_for((size_t)batch, 64)), dim3(64), 0, ctx->stream, G.as<float2>(),
+                     V.as<float2>(), W.as<float2>(), batch, n, mode, rank, lambda, smax_bits);
+  LLR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_apply_generic, dim3(grid_for((size_t)m, 256, 1024), (unsigned)batch), dim3(256), 0, ctx->stream,
+                     mats, W.as<float2>(), m, n);
+  LLR_HIP(hipGetLastError());
+  LLR_HIP(hipStreamSynchronize(ctx->stream));
+  return LLR_OK;
+}
+
+}  // namespace llr

@@ -1,0 +1,174 @@
+"""ctypes loader for libllr_hip.so (the C ABI declared in include/llr_hip.h).
+
+The library is required: importing any compute entry point without it raises,
+there is no CPU fallback.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libllr_hip.so")
+
+LLR_FORWARD = 0
+LLR_INVERSE = 1
+LLR_NORM_NONE = 0
+LLR_NORM_INVERSE = 1
+LLR_NORM_UNITARY = 2
+LLR_MODE_RANK = 0
+LLR_MODE_SVT = 1
+LLR_PREC_FP32 = 0
+LLR_PREC_MIXED = 1
+LLR_NUM_STAGES = 7
+STAGE_NAMES = ["llr_block", "subspace_project", "wavelet", "fft_x", "fft_y", "fft_z_project", "reductions"]
+
+# every symbol include/llr_hip.h declares
+EXPORTS = [
+    "llr_last_error", "llr_abi_version", "llr_ctx_create", "llr_ctx_destroy", "llr_ctx_synchronize",
+    "llr_ctx_device", "llr_grid_dim", "llr_lift_data_set", "llr_un_lift_data_set", "llr_dti_project",
+    "llr_dti_back_project", "llr_conj_transpose_matrix", "llr_subspace_constraint", "llr_low_rank_approx_batch",
+    "llr_svt_batch", "llr_cgemm_strided_batched", "llr_rhs_cdgmm_batched", "llr_signal_model",
+    "llr_signal_model_device", "llr_estimate_linear_correction", "llr_fftn_batch", "llr_fftshift", "llr_ifftshift",
+    "llr_phase_shift3", "llr_wavelet_denoise", "llr_hard_project", "llr_calc_undersampling_factor",
+    "llr_under_sample_kspace", "llr_recon_create", "llr_recon_destroy", "llr_recon_set_measurements",
+    "llr_recon_set_initial", "llr_recon_run", "llr_recon_get_image", "llr_recon_info", "llr_recon_enable_timing",
+    "llr_recon_stage_times",
+]
+
+
+class LlrError(RuntimeError):
+    """A nonzero llr_status from the library (the reference panics here)."""
+
+    def __init__(self, status, msg):
+        super().__init__(f"llr status {status}: {msg}")
+        self.status = status
+
+
+class CF32(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_float), ("im", ctypes.c_float)]
+
+
+class ReconParams(ctypes.Structure):
+    _fields_ = [
+        ("vol", ctypes.c_size_t * 3),
+        ("nq", ctypes.c_size_t),
+        ("blk", ctypes.c_size_t * 3),
+        ("n_iter", ctypes.c_size_t),
+        ("llr_mode", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("rank", ctypes.c_size_t),
+        ("svt_lambda", ctypes.c_double),
+        ("lambda_factor", ctypes.c_double),
+        ("wavelet_alpha", ctypes.c_double),
+        ("wavelet_levels", ctypes.c_int),
+        ("srank", ctypes.c_size_t),
+        ("subspace", ctypes.c_void_p),
+        ("shifts", ctypes.c_void_p),
+        ("residual_stop", ctypes.c_double),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                                  "(make -C llr-reco_amd/csrc); there is no CPU fallback")
+            lib = ctypes.CDLL(LIB_PATH)
+            lib.llr_last_error.restype = ctypes.c_char_p
+            lib.llr_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+            lib.llr_ctx_destroy.argtypes = [ctypes.c_void_p]
+            lib.llr_ctx_destroy.restype = None
+            lib.llr_recon_destroy.argtypes = [ctypes.c_void_p]
+            lib.llr_recon_destroy.restype = None
+            lib.llr_grid_dim.restype = None
+            lib.llr_svt_batch.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_float,
+                                          ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+            lib.llr_wavelet_denoise.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                                ctypes.c_float, ctypes.c_int]
+            lib.llr_cgemm_strided_batched.argtypes = [
+                ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int,
+                CF32, CF32, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t]
+            _lib = lib
+    return _lib
+
+
+def last_error():
+    return load().llr_last_error().decode(errors="replace")
+
+
+def check(status):
+    if status != 0:
+        raise LlrError(status, last_error())
+
+
+def sz3(v):
+    v = [int(x) for x in v]
+    if len(v) != 3:
+        raise ValueError("expected 3 dimensions")
+    return (ctypes.c_size_t * 3)(*v)
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def as_cf32(a, name="array"):
+    a = np.asarray(a)
+    if a.dtype != np.complex64:
+        raise TypeError(f"{name} must be complex64 (Complex32), got {a.dtype}")
+    if not (a.flags["C_CONTIGUOUS"] or a.flags["F_CONTIGUOUS"]):
+        raise ValueError(f"{name} must be contiguous")
+    return a
+
+
+class Context:
+    """One HIP device + stream (llr_ctx)."""
+
+    def __init__(self, device=0):
+        lib = load()
+        h = ctypes.c_void_p()
+        check(lib.llr_ctx_create(int(device), ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def synchronize(self):
+        check(load().llr_ctx_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            load().llr_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = {}
+
+
+def default_context(device=0):
+    """Process-wide context per device, created on first use."""
+    with _lock:
+        c = _default_ctx.get(device)
+    if c is None:
+        c = Context(device)
+        with _lock:
+            _default_ctx[device] = c
+    return c
+
+
+def ctx_handle(ctx):
+    return (ctx or default_context()).handle

@@ -1,0 +1,158 @@
+"""The reconstruction loop of llr_recon_exec (lib.rs:113-363) on one GPU, plus
+the lib.rs / data_import.rs helpers it calls.
+
+`Recon` keeps the iterate, the measurements and every workspace in HBM for
+the whole run (llr_recon_* in include/llr_hip.h); only the per-iteration
+residuals come back to the host.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+_sz = ctypes.c_size_t
+
+
+def hard_project(x, y, ctx=None):
+    """lib.rs:392-406: where y != 0, x = y; returns sum |x - y|^2 (f64). In place on x."""
+    X = _lib.as_cf32(x, "x")
+    Y = _lib.as_cf32(y, "y")
+    if X.size != Y.size:
+        raise ValueError("dataset and measurements must have the same dimensions")
+    err = ctypes.c_double()
+    _lib.check(_lib.load().llr_hard_project(_lib.ctx_handle(ctx), _lib.ptr(X), _lib.ptr(Y), _sz(X.size),
+                                            ctypes.byref(err)))
+    return err.value
+
+
+def calc_undersampling_factor(y, ctx=None):
+    """lib.rs:409-417: (len / nonzero, total energy)."""
+    Y = _lib.as_cf32(y, "y")
+    f = ctypes.c_double()
+    e = ctypes.c_double()
+    _lib.check(_lib.load().llr_calc_undersampling_factor(_lib.ctx_handle(ctx), _lib.ptr(Y), _sz(Y.size),
+                                                         ctypes.byref(f), ctypes.byref(e)))
+    return f.value, e.value
+
+
+def under_sample_kspace(data, dims, nq, mask, ctx=None):
+    """data_import.rs:416-438: zero readout lines whose (y, z, q) mask entry is 0. In place."""
+    D = _lib.as_cf32(data, "data")
+    M = _lib.as_cf32(mask, "mask")
+    if M.size != dims[1] * dims[2] * nq:
+        raise ValueError("phase encoding dimension mismatch")
+    _lib.check(_lib.load().llr_under_sample_kspace(_lib.ctx_handle(ctx), _lib.ptr(D), _lib.sz3(dims), _sz(nq),
+                                                   _lib.ptr(M)))
+
+
+def wavelet_denoise(volumes, n_volumes, vol_size, alpha, levels=-1, ctx=None):
+    """lib.rs:430-458 for n_volumes volumes, in place."""
+    V = _lib.as_cf32(volumes, "volumes")
+    _lib.check(_lib.load().llr_wavelet_denoise(_lib.ctx_handle(ctx), _lib.ptr(V), n_volumes, _lib.sz3(vol_size),
+                                               float(alpha), int(levels)))
+
+
+class Recon:
+    """Device-resident LLR reconstruction (llr_recon_exec's loop, lib.rs:227-341).
+
+    llr_mode: "svt" (singular-value soft threshold, lambda = lambda_factor *
+    sigma_max of the starting blocks unless svt_lambda > 0) or "rank" (hard
+    truncation to `rank`, low_rank.rs:10-69).
+    """
+
+    def __init__(self, vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0,
+                 lambda_factor=0.02, wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0,
+                 precision="fp32", ctx=None):
+        self.ctx = ctx or _lib.default_context()
+        self.vol = [int(v) for v in vol]
+        self.nq = int(nq)
+        self.D = int(np.prod(self.vol)) * self.nq
+        sh = np.ascontiguousarray(np.asarray(shifts, np.uint64).reshape(-1, 3))
+        self.n_iter = sh.shape[0]
+        p = _lib.ReconParams()
+        p.vol[:] = self.vol
+        p.nq = self.nq
+        p.blk[:] = [int(b) for b in blk]
+        p.n_iter = self.n_iter
+        p.llr_mode = {"rank": _lib.LLR_MODE_RANK, "svt": _lib.LLR_MODE_SVT}[llr_mode]
+        p.precision = {"fp32": _lib.LLR_PREC_FP32, "mixed": _lib.LLR_PREC_MIXED}[precision]
+        p.rank = int(rank)
+        p.svt_lambda = float(svt_lambda)
+        p.lambda_factor = float(lambda_factor)
+        p.wavelet_alpha = float(wavelet_alpha)
+        p.wavelet_levels = int(wavelet_levels)
+        sub = None
+        if subspace is not None:
+            sub = np.asfortranarray(np.asarray(subspace, np.complex64))
+            if sub.shape[0] != self.nq:
+                raise ValueError("unexpected subspace dimension")
+            p.srank = sub.shape[1]
+            sub = sub.ravel(order="F").copy()
+            p.subspace = ctypes.c_void_p(sub.ctypes.data)
+        else:
+            p.srank = 0
+            p.subspace = None
+        p.shifts = ctypes.c_void_p(sh.ctypes.data) if self.n_iter else None
+        p.residual_stop = float(residual_stop)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().llr_recon_create(self.ctx.handle, ctypes.byref(p), ctypes.byref(h)))
+        self.handle = h
+
+    def set_measurements(self, y=None, device_ptr=None):
+        if device_ptr is not None:
+            _lib.check(_lib.load().llr_recon_set_measurements(self.handle, ctypes.c_void_p(int(device_ptr)),
+                                                              _sz(self.D), 1))
+        else:
+            Y = _lib.as_cf32(y, "measurements")
+            if Y.size != self.D:
+                raise ValueError("measurements must hold nx*ny*nz*nq samples")
+            _lib.check(_lib.load().llr_recon_set_measurements(self.handle, _lib.ptr(Y), _sz(self.D), 0))
+
+    def set_initial(self, x=None, device_ptr=None):
+        if device_ptr is not None:
+            _lib.check(_lib.load().llr_recon_set_initial(self.handle, ctypes.c_void_p(int(device_ptr)), _sz(self.D), 1))
+        else:
+            X = _lib.as_cf32(x, "initial image")
+            _lib.check(_lib.load().llr_recon_set_initial(self.handle, _lib.ptr(X), _sz(self.D), 0))
+
+    def run(self):
+        """Run the iterations; returns the relative residual history (lib.rs:321)."""
+        hist = np.zeros(max(1, self.n_iter), np.float64)
+        done = ctypes.c_size_t()
+        _lib.check(_lib.load().llr_recon_run(self.handle, _lib.ptr(hist), ctypes.byref(done)))
+        return hist[:done.value]
+
+    def image(self):
+        out = np.empty(self.D, np.complex64)
+        _lib.check(_lib.load().llr_recon_get_image(self.handle, _lib.ptr(out), _sz(self.D), 0))
+        return out
+
+    def image_to_device(self, device_ptr):
+        _lib.check(_lib.load().llr_recon_get_image(self.handle, ctypes.c_void_p(int(device_ptr)), _sz(self.D), 1))
+
+    def info(self):
+        lam = ctypes.c_double()
+        kc = np.zeros(3 * self.nq, np.float32)
+        _lib.check(_lib.load().llr_recon_info(self.handle, ctypes.byref(lam), _lib.ptr(kc)))
+        return lam.value, kc.reshape(self.nq, 3)
+
+    def enable_timing(self, on=True):
+        _lib.check(_lib.load().llr_recon_enable_timing(self.handle, int(on)))
+
+    def stage_times(self):
+        ms = np.zeros(_lib.LLR_NUM_STAGES, np.float64)
+        n = np.zeros(_lib.LLR_NUM_STAGES, np.uint64)
+        _lib.check(_lib.load().llr_recon_stage_times(self.handle, _lib.ptr(ms), _lib.ptr(n)))
+        return {name: (float(ms[i]), int(n[i])) for i, name in enumerate(_lib.STAGE_NAMES)}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().llr_recon_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

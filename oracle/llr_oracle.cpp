@@ -1,0 +1,1024 @@
+// CPU oracle for the locally-low-rank (LLR) diffusion-MRI reconstruction path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker: tests/, the smoke check
+// and the CPU-baseline leg of bench.py load it to produce expected results and
+// CPU timings. The product library (libllr_hip.so) never links or calls it.
+//
+// Every routine is templated on the real type T:
+//   T = double : the FP64 oracle (exact phase ramps, FP64 Gram/Jacobi).
+//   T = float  : the reference CPU path restated in FP32. Phase ramps follow
+//                the reference's f32 arithmetic (cufft.rs:320-326) unless the
+//                caller asks for exact ramps; Gram + Jacobi eigen-solve in FP32,
+//                which is how cuSOLVER gesvda forms its approximate SVD.
+//
+// Pinning (see tests/test_oracle_*.py): brute-force index formulas, the
+// reference's KATs (low_rank.rs:387-461 cgemm, cublas.rs:139-183 diag,
+// dti_subspace.rs:239-261 identity subspace, block.rs:285-343 round trip,
+// fftshift.rs:370-378 wrap, cufft.rs:415-422 FFT round trip), numpy FP64
+// FFT/SVD, closed-form db2 filters and perfect reconstruction.
+//
+// Parity status: every function is pinned except the multi-level wavelet's
+// level-count/boundary bookkeeping, which lives in the un-vendored `dwt` crate
+// (lib.rs:439-454). That part follows pywt's conventions (max level, symmetric
+// extension, trim on reconstruction) and is "parity unpinned" beyond the
+// perfect-reconstruction invariant and the single-level formulas of the
+// reference's own CUDA kernels (kernels.cu:155-268).
+
+#include <algorithm>
+#include <array>
+#include <limits>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+template <class T>
+using C = std::complex<T>;
+
+// ---------------------------------------------------------------- indexing
+// array_utils.rs:4-10
+static inline void ind2sub3(size_t idx, const size_t* n, size_t* s) {
+  size_t nxy = n[0] * n[1];
+  s[2] = idx / nxy;
+  size_t r = idx % nxy;
+  s[1] = r / n[0];
+  s[0] = r % n[0];
+}
+
+// block.rs:112-118
+static inline void grid_dim(const size_t* vol, const size_t* blk, size_t* grid) {
+  for (int a = 0; a < 3; ++a) grid[a] = vol[a] / blk[a];
+}
+
+// block.rs:122-133
+static inline void block_coord(size_t b, const size_t* vol, const size_t* blk, const size_t* shift,
+                               size_t* corner) {
+  size_t grid[3], s[3];
+  grid_dim(vol, blk, grid);
+  ind2sub3(b, grid, s);
+  for (int a = 0; a < 3; ++a) corner[a] = s[a] * blk[a] + shift[a];
+}
+
+// _read_block_unchecked / _write_block_unchecked address (block.rs:14-25, 44-57)
+static inline size_t block_voxel_address(size_t idx, const size_t* blk, const size_t* corner,
+                                         const size_t* vol) {
+  size_t s[3];
+  ind2sub3(idx, blk, s);
+  size_t z = (s[2] + corner[2]) % vol[2];
+  size_t y = (s[1] + corner[1]) % vol[1];
+  size_t x = (s[0] + corner[0]) % vol[0];
+  return vol[0] * vol[1] * z + vol[0] * y + x;
+}
+
+// lift_data_set (block.rs:147-164) -> read_matrix (block.rs:64-84)
+static int lift(const uint8_t* vols, size_t nvols, size_t eb, uint8_t* mats, const size_t* vol,
+                const size_t* blk, const size_t* shift, size_t start, size_t n) {
+  for (int a = 0; a < 3; ++a)
+    if (blk[a] > vol[a] || blk[a] == 0) return -1;  // block.rs:69-71
+  size_t m = blk[0] * blk[1] * blk[2];
+  size_t N = vol[0] * vol[1] * vol[2];
+#pragma omp parallel for schedule(static)
+  for (long long c = 0; c < (long long)n; ++c) {
+    size_t corner[3];
+    block_coord(start + (size_t)c, vol, blk, shift, corner);
+    for (size_t q = 0; q < nvols; ++q)
+      for (size_t i = 0; i < m; ++i) {
+        size_t addr = block_voxel_address(i, blk, corner, vol);
+        std::memcpy(mats + (((size_t)c * nvols + q) * m + i) * eb, vols + (q * N + addr) * eb, eb);
+      }
+  }
+  return 0;
+}
+
+// un_lift_data_set (block.rs:166-184) -> write_matrix (block.rs:86-109)
+static int unlift(uint8_t* vols, size_t nvols, size_t eb, const uint8_t* mats, const size_t* vol,
+                  const size_t* blk, const size_t* shift, size_t start, size_t n) {
+  for (int a = 0; a < 3; ++a)
+    if (blk[a] > vol[a] || blk[a] == 0) return -1;
+  size_t m = blk[0] * blk[1] * blk[2];
+  size_t N = vol[0] * vol[1] * vol[2];
+  // blocks of one call never overlap (grid = floor(vol/blk)), so the order of
+  // writes is irrelevant; the reference serialises them with a Mutex.
+#pragma omp parallel for schedule(static)
+  for (long long c = 0; c < (long long)n; ++c) {
+    size_t corner[3];
+    block_coord(start + (size_t)c, vol, blk, shift, corner);
+    for (size_t q = 0; q < nvols; ++q)
+      for (size_t i = 0; i < m; ++i) {
+        size_t addr = block_voxel_address(i, blk, corner, vol);
+        std::memcpy(vols + (q * N + addr) * eb, mats + (((size_t)c * nvols + q) * m + i) * eb, eb);
+      }
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------ small BLAS
+// cublasCgemmStridedBatched semantics as wrapped by cublas.rs:12-105:
+// C = alpha*op(A)*op(B) + beta*C, op = conjugate transpose when flagged,
+// lda/ldb are the physical leading dimensions (cublas.rs:39-61), A broadcast
+// gives stride 0 (cublas.rs:63-67).
+template <class T>
+static void cgemm(size_t m, size_t k, size_t n, size_t batch, bool bcast_a, C<T> alpha, C<T> beta,
+                  bool ta, bool tb, const C<T>* A, const C<T>* B, C<T>* Cm) {
+  size_t lda = ta ? k : m, ldb = tb ? n : k, ldc = m;
+  size_t sa = bcast_a ? 0 : m * k, sb = k * n, sc = m * n;
+#pragma omp parallel for schedule(static)
+  for (long long b = 0; b < (long long)batch; ++b) {
+    const C<T>* a = A + (size_t)b * sa;
+    const C<T>* bb = B + (size_t)b * sb;
+    C<T>* c = Cm + (size_t)b * sc;
+    for (size_t j = 0; j < n; ++j)
+      for (size_t i = 0; i < m; ++i) {
+        C<T> acc(0, 0);
+        for (size_t l = 0; l < k; ++l) {
+          C<T> av = ta ? std::conj(a[l + i * lda]) : a[i + l * lda];
+          C<T> bv = tb ? std::conj(bb[j + l * ldb]) : bb[l + j * ldb];
+          acc += av * bv;
+        }
+        c[i + j * ldc] = alpha * acc + beta * c[i + j * ldc];
+      }
+  }
+}
+
+// conj_transpose_matrix (dti_subspace.rs:211-221): (m x n) -> (n x m)^H
+template <class T>
+static void conj_transpose(C<T>* data, size_t m, size_t n, size_t batch) {
+  std::vector<C<T>> tmp(data, data + m * n * batch);
+#pragma omp parallel for schedule(static)
+  for (long long b = 0; b < (long long)batch; ++b) {
+    C<T>* a = data + (size_t)b * m * n;
+    const C<T>* s = tmp.data() + (size_t)b * m * n;
+    for (size_t j = 0; j < n; ++j)
+      for (size_t i = 0; i < m; ++i) a[i * n + j] = std::conj(s[j * m + i]);
+  }
+}
+
+// ------------------------------------------------------ Hermitian Jacobi
+// Cyclic two-sided Jacobi for a Hermitian n x n matrix (Golub & Van Loan,
+// "Matrix Computations", Alg. 8.5.2/8.5.3, with the complex phase removed by a
+// diagonal unitary before each real rotation). A (column-major) is destroyed.
+// Eigenvalues are returned in descending order with matching columns of V.
+template <class T>
+static void herm_eig(int n, C<T>* A, T* w, C<T>* V) {
+  const T eps = std::numeric_limits<T>::epsilon();
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) V[i + j * n] = C<T>(i == j ? 1 : 0, 0);
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    T off = 0, nrm = 0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        T v = std::norm(A[i + j * n]);
+        nrm += v;
+        if (i != j) off += v;
+      }
+    if (off <= eps * eps * nrm || nrm == 0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        C<T> apq = A[p + q * n];
+        T a = std::abs(apq);
+        if (a == 0) continue;
+        T app = A[p + p * n].real(), aqq = A[q + q * n].real();
+        C<T> u = apq / a;
+        T theta = (aqq - app) / (2 * a);
+        T t = (theta >= 0 ? T(1) : T(-1)) / (std::abs(theta) + std::sqrt(theta * theta + 1));
+        T c = 1 / std::sqrt(t * t + 1);
+        T s = t * c;
+        C<T> cu = std::conj(u);
+        // A <- A J : columns p, q
+        for (int i = 0; i < n; ++i) {
+          C<T> xp = A[i + p * n], xq = A[i + q * n];
+          A[i + p * n] = c * xp - s * cu * xq;
+          A[i + q * n] = s * xp + c * cu * xq;
+        }
+        // A <- J^H A : rows p, q
+        for (int j = 0; j < n; ++j) {
+          C<T> xp = A[p + j * n], xq = A[q + j * n];
+          A[p + j * n] = c * xp - s * u * xq;
+          A[q + j * n] = s * xp + c * u * xq;
+        }
+        A[p + q * n] = 0;
+        A[q + p * n] = 0;
+        A[p + p * n] = C<T>(A[p + p * n].real(), 0);
+        A[q + q * n] = C<T>(A[q + q * n].real(), 0);
+        // V <- V J
+        for (int i = 0; i < n; ++i) {
+          C<T> xp = V[i + p * n], xq = V[i + q * n];
+          V[i + p * n] = c * xp - s * cu * xq;
+          V[i + q * n] = s * xp + c * cu * xq;
+        }
+      }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::vector<T> d(n);
+  for (int i = 0; i < n; ++i) d[i] = A[i + i * n].real();
+  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] > d[y]; });
+  std::vector<C<T>> Vs(V, V + n * n);
+  for (int j = 0; j < n; ++j) {
+    w[j] = d[idx[j]];
+    for (int i = 0; i < n; ++i) V[i + j * n] = Vs[i + idx[j] * n];
+  }
+}
+
+// Low-rank operator for one m x n (col-major) matrix, via its Gram matrix
+// G = A^H A = V diag(s^2) V^H (the route cuSOLVER gesvda takes, cusolver.rs:68-126):
+//   mode 0 (rank, low_rank.rs:10-69): A <- A V_r V_r^H = U_r diag(s_r) V_r^H
+//   mode 1 (SVT):                     A <- A V diag(max(s-lam,0)/s) V^H
+// Returns the largest singular value.
+template <class T>
+static T lowrank_one(size_t m, int n, C<T>* A, int mode, int rank, T lam, std::vector<C<T>>& ws) {
+  ws.resize(3 * n * n + m * n);
+  C<T>* G = ws.data();
+  C<T>* V = G + n * n;
+  C<T>* W = V + n * n;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i <= j; ++i) {
+      C<T> acc(0, 0);
+      const C<T>* ai = A + i * m;
+      const C<T>* aj = A + j * m;
+      for (size_t r = 0; r < m; ++r) acc += std::conj(ai[r]) * aj[r];
+      G[i + j * n] = acc;
+      G[j + i * n] = std::conj(acc);
+    }
+  std::vector<T> w(n);
+  herm_eig<T>(n, G, w.data(), V);
+  std::vector<T> g(n);
+  for (int i = 0; i < n; ++i) {
+    T s = std::sqrt(std::max(w[i], T(0)));
+    if (mode == 0)
+      g[i] = (i < rank) ? T(1) : T(0);
+    else
+      g[i] = (s > lam && s > 0) ? (s - lam) / s : T(0);
+  }
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      C<T> acc(0, 0);
+      for (int l = 0; l < n; ++l) acc += V[i + l * n] * g[l] * std::conj(V[j + l * n]);
+      W[i + j * n] = acc;
+    }
+  C<T>* out = W + n * n;
+  for (int j = 0; j < n; ++j)
+    for (size_t r = 0; r < m; ++r) {
+      C<T> acc(0, 0);
+      for (int l = 0; l < n; ++l) acc += A[r + l * m] * W[l + j * n];
+      out[r + j * m] = acc;
+    }
+  std::memcpy(A, out, sizeof(C<T>) * m * n);
+  return std::sqrt(std::max(w[0], T(0)));
+}
+
+template <class T>
+static T lowrank_batch(size_t m, size_t n, size_t batch, C<T>* data, int mode, int rank, T lam) {
+  T smax = 0;
+#pragma omp parallel
+  {
+    std::vector<C<T>> ws;
+    T local = 0;
+#pragma omp for schedule(static)
+    for (long long b = 0; b < (long long)batch; ++b)
+      local = std::max(local, lowrank_one<T>(m, (int)n, data + (size_t)b * m * n, mode, rank, lam, ws));
+#pragma omp critical
+    smax = std::max(smax, local);
+  }
+  return smax;
+}
+
+// largest singular value per matrix (no modification)
+template <class T>
+static void block_svals(size_t m, size_t n, size_t batch, const C<T>* data, T* s_out) {
+#pragma omp parallel
+  {
+    std::vector<C<T>> G(n * n), V(n * n);
+    std::vector<T> w(n);
+#pragma omp for schedule(static)
+    for (long long b = 0; b < (long long)batch; ++b) {
+      const C<T>* A = data + (size_t)b * m * n;
+      for (size_t j = 0; j < n; ++j)
+        for (size_t i = 0; i <= j; ++i) {
+          C<T> acc(0, 0);
+          for (size_t r = 0; r < m; ++r) acc += std::conj(A[r + i * m]) * A[r + j * m];
+          G[i + j * n] = acc;
+          G[j + i * n] = std::conj(acc);
+        }
+      herm_eig<T>((int)n, G.data(), w.data(), V.data());
+      for (size_t i = 0; i < n; ++i) s_out[(size_t)b * n + i] = std::sqrt(std::max(w[i], T(0)));
+    }
+  }
+}
+
+// ------------------------------------------------------------------ FFT
+// One-dimensional complex DFT of length n (any n with prime factors handled by
+// the generic radix loop), X[k] = sum_j x[j] exp(sign*2*pi*i*j*k/n), no scaling.
+// Twiddles are computed in double (std::cos/std::sin) then rounded to T.
+template <class T>
+struct Fft1 {
+  size_t n;
+  int sign;
+  std::vector<int> radix;  // factorisation
+  std::vector<C<T>> tw;    // exp(sign*2*pi*i*m/n), m in [0,n)
+  explicit Fft1(size_t n_, int sign_) : n(n_), sign(sign_) {
+    size_t r = n;
+    for (int f : {4, 2, 3, 5})
+      while (r % f == 0) {
+        radix.push_back(f);
+        r /= f;
+      }
+    for (size_t f = 7; r > 1; f += 2)
+      while (r % f == 0) {
+        radix.push_back((int)f);
+        r /= f;
+      }
+    tw.resize(n);
+    for (size_t m = 0; m < n; ++m) {
+      long double ang = (long double)sign * 2.0L * 3.14159265358979323846264338327950288L *
+                        (long double)m / (long double)n;
+      tw[m] = C<T>((T)std::cos(ang), (T)std::sin(ang));
+    }
+  }
+  // recursive decimation in time: x strided by `stride`, output contiguous in y
+  void rec(const C<T>* x, size_t stride, C<T>* y, size_t len, size_t level, C<T>* scratch) const {
+    if (len == 1) {
+      y[0] = x[0];
+      return;
+    }
+    int r = radix[level];
+    size_t m = len / r;
+    for (int q = 0; q < r; ++q) rec(x + q * stride, stride * r, y + q * m, m, level + 1, scratch);
+    size_t tstep = n / len;  // twiddle index step for exp(sign*2pi i /len)
+    C<T> tmp[64];
+    for (size_t k = 0; k < m; ++k) {
+      for (int q = 0; q < r; ++q) tmp[q] = y[q * m + k] * tw[(q * k * tstep) % n];
+      for (int s = 0; s < r; ++s) {
+        C<T> acc(0, 0);
+        for (int q = 0; q < r; ++q) acc += tmp[q] * tw[((size_t)q * s * m * tstep) % n];
+        scratch[k + s * m] = acc;
+      }
+    }
+    for (size_t i = 0; i < len; ++i) y[i] = scratch[i];
+  }
+  void run(const C<T>* x, size_t stride, C<T>* y, C<T>* scratch) const { rec(x, stride, y, n, 0, scratch); }
+};
+
+// batched in-place 3D C2C transform on col-major (x fastest) volumes,
+// the semantics of cu_fftn_batch (cufft.rs:124-180) without scaling.
+template <class T>
+static void fft3_batch(C<T>* data, const size_t* dims, size_t nvols, int sign) {
+  size_t nx = dims[0], ny = dims[1], nz = dims[2];
+  size_t N = nx * ny * nz;
+  for (int axis = 0; axis < 3; ++axis) {
+    size_t n = dims[axis];
+    if (n == 1) continue;
+    size_t stride = axis == 0 ? 1 : (axis == 1 ? nx : nx * ny);
+    size_t nlines = N / n;
+    Fft1<T> plan(n, sign);
+#pragma omp parallel
+    {
+      std::vector<C<T>> in(n), out(n), scratch(n);
+#pragma omp for schedule(static) collapse(2)
+      for (long long v = 0; v < (long long)nvols; ++v)
+        for (long long l = 0; l < (long long)nlines; ++l) {
+          size_t base;
+          if (axis == 0)
+            base = (size_t)l * nx;
+          else if (axis == 1)
+            base = ((size_t)l % nx) + ((size_t)l / nx) * nx * ny;
+          else
+            base = (size_t)l;
+          C<T>* p = data + (size_t)v * N + base;
+          for (size_t j = 0; j < n; ++j) in[j] = p[j * stride];
+          plan.run(in.data(), 1, out.data(), scratch.data());
+          for (size_t j = 0; j < n; ++j) p[j * stride] = out[j];
+        }
+    }
+  }
+}
+
+// ---------------------------------------------------------- phase ramps
+// phase_shift_sub_vox (cufft.rs:320-326) evaluated in f32 exactly as the reference.
+static inline float ref_phase_f32(size_t index, size_t n, float shift) {
+  const float PI_F = 3.14159265358979323846f;
+  float nf = (float)n;
+  float coord = (float)index - (nf / 2.f);
+  float mult = 2.f * PI_F * shift / nf;
+  return coord * mult;
+}
+
+// calc_shift (signal_model.rs:110-117)
+static inline void calc_shift_f32(const float* corr, const size_t* dims, float* s) {
+  for (int a = 0; a < 3; ++a) s[a] = corr[a] + 0.5f * (float)dims[a];
+}
+
+// phase_shift3_sub_vox (cufft.rs:290-304) on one volume.
+//   phase_mode 0: reference f32 arithmetic (T = float path)
+//   phase_mode 1: exact (turn-reduced in long double)
+template <class T>
+static void phase_ramp(C<T>* x, const size_t* dims, const float* corr, int sign, int phase_mode) {
+  size_t nx = dims[0], ny = dims[1], nz = dims[2];
+  float s[3];
+  calc_shift_f32(corr, dims, s);
+  if (phase_mode == 0) {
+#pragma omp parallel for schedule(static)
+    for (long long iz = 0; iz < (long long)nz; ++iz) {
+      float zsh = ref_phase_f32((size_t)iz, nz, s[2]);
+      for (size_t iy = 0; iy < ny; ++iy) {
+        float ysh = ref_phase_f32(iy, ny, s[1]);
+        for (size_t ix = 0; ix < nx; ++ix) {
+          float xsh = ref_phase_f32(ix, nx, s[0]);
+          float tot = (float)sign * (xsh + ysh + zsh);
+          C<T> ph((T)std::cos(tot), (T)std::sin(tot));
+          C<T>& v = x[ix + nx * (iy + ny * (size_t)iz)];
+          v = v * ph;
+        }
+      }
+    }
+  } else {
+    // exact: per-axis turns (i - n/2) * s / n reduced modulo 1
+    std::vector<long double> t[3];
+    for (int a = 0; a < 3; ++a) {
+      t[a].resize(dims[a]);
+      long double sa = (long double)corr[a] + 0.5L * (long double)dims[a];
+      for (size_t i = 0; i < dims[a]; ++i) {
+        long double tr = ((long double)i - 0.5L * (long double)dims[a]) * sa / (long double)dims[a];
+        t[a][i] = tr - std::floor(tr);
+      }
+    }
+    const long double TWO_PI = 6.283185307179586476925286766559005768L;
+#pragma omp parallel for schedule(static)
+    for (long long iz = 0; iz < (long long)nz; ++iz)
+      for (size_t iy = 0; iy < ny; ++iy)
+        for (size_t ix = 0; ix < nx; ++ix) {
+          long double tr = t[0][ix] + t[1][iy] + t[2][(size_t)iz];
+          tr -= std::floor(tr);
+          long double ang = (long double)sign * TWO_PI * tr;
+          C<T> ph((T)std::cos(ang), (T)std::sin(ang));
+          C<T>& v = x[ix + nx * (iy + ny * (size_t)iz)];
+          v = v * ph;
+        }
+  }
+}
+
+// signal_model (signal_model.rs:34-79). dir 0 = Forward, 1 = Inverse.
+// Unitary scaling 1/sqrt(N) in both directions (cufft.rs:161-163).
+template <class T>
+static void signal_model(C<T>* x, const size_t* dims, size_t nvols, const float* ksp_corr,
+                         const float* img_corr, int dir, int phase_mode) {
+  size_t N = dims[0] * dims[1] * dims[2];
+  for (size_t v = 0; v < nvols; ++v) {
+    if (dir == 0)
+      phase_ramp<T>(x + v * N, dims, ksp_corr + 3 * v, +1, phase_mode);
+    else
+      phase_ramp<T>(x + v * N, dims, img_corr + 3 * v, +1, phase_mode);
+  }
+  fft3_batch<T>(x, dims, nvols, dir == 0 ? -1 : +1);
+  T scale = (T)(1.0 / std::sqrt((double)N));
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)(N * nvols); ++i) x[i] *= scale;
+  for (size_t v = 0; v < nvols; ++v) {
+    if (dir == 0)
+      phase_ramp<T>(x + v * N, dims, img_corr + 3 * v, -1, phase_mode);
+    else
+      phase_ramp<T>(x + v * N, dims, ksp_corr + 3 * v, -1, phase_mode);
+  }
+}
+
+// estimate_linear_correction (signal_model.rs:100-106) via max_coord (cufft.rs:329-337).
+// Ties resolve to the lowest linear index.
+template <class T>
+static void estimate_linear_correction(const C<T>* ksp, const size_t* dims, float* out) {
+  size_t N = dims[0] * dims[1] * dims[2];
+  size_t best = 0;
+  T bv = -1;
+  for (size_t i = 0; i < N; ++i) {
+    T v = std::norm(ksp[i]);
+    if (v > bv) {
+      bv = v;
+      best = i;
+    }
+  }
+  size_t s[3];
+  ind2sub3(best, dims, s);
+  for (int a = 0; a < 3; ++a) out[a] = (float)((long long)s[a] - ((long long)dims[a] + 1) / 2);
+}
+
+// -------------------------------------------------------------- wavelet
+// Daubechies-2 (4 taps). Closed form h = [1+r3, 3+r3, 3-r3, 1-r3]/(4 sqrt 2)
+// (Daubechies 1988); pywt ordering: rec_lo = h, dec_lo = reverse(h),
+// rec_hi[k] = (-1)^k dec_lo[k], dec_hi = reverse(rec_hi).
+template <class T>
+struct Db2 {
+  T dlo[4], dhi[4], rlo[4], rhi[4];
+  Db2() {
+    long double r3 = std::sqrt(3.0L), d = 4.0L * std::sqrt(2.0L);
+    long double h[4] = {(1 + r3) / d, (3 + r3) / d, (3 - r3) / d, (1 - r3) / d};
+    for (int k = 0; k < 4; ++k) {
+      rlo[k] = (T)h[k];
+      dlo[k] = (T)h[3 - k];
+    }
+    for (int k = 0; k < 4; ++k) rhi[k] = (k % 2 == 0) ? dlo[k] : -dlo[k];
+    for (int k = 0; k < 4; ++k) dhi[k] = rhi[3 - k];
+  }
+};
+
+// kernels.cu:97-105 symmetric (half-sample) extension
+static inline long long sym_idx(long long i, long long n) {
+  while (i < 0 || i >= n) {
+    if (i < 0) i = -i - 1;
+    if (i >= n) i = 2 * n - i - 1;
+  }
+  return i;
+}
+
+static inline size_t sub_band_size(size_t n, size_t f) { return (n + f - 1) / 2; }  // kernels.cu:150-152
+
+// pywt.dwt_max_level
+static inline int dwt_max_level(size_t len, size_t flen) {
+  if (len < flen - 1) return 0;
+  return (int)std::floor(std::log2((double)len / (double)(flen - 1)));
+}
+
+// one-level analysis along `axis` of box `in` (dims ind, pitch ip) into `out`
+// (dims outd = in with axis -> 2*nc, pitch op). kernels.cu:155-212 formula:
+// a[c] = sum_k lo_d[k] x[sym(2c+1-k)], d likewise; approx -> [0,nc), detail -> [nc,2nc).
+template <class T>
+static void dwt_axis(const C<T>* in, const size_t* ind, const size_t* ip, C<T>* out, const size_t* op,
+                     int axis, const Db2<T>& f) {
+  size_t n = ind[axis], nc = sub_band_size(n, 4);
+  int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+#pragma omp parallel for schedule(static) collapse(2)
+  for (long long u2 = 0; u2 < (long long)ind[a2]; ++u2)
+    for (long long u1 = 0; u1 < (long long)ind[a1]; ++u1) {
+      size_t ib = (size_t)u1 * ip[a1] + (size_t)u2 * ip[a2];
+      size_t ob = (size_t)u1 * op[a1] + (size_t)u2 * op[a2];
+      for (size_t c = 0; c < nc; ++c) {
+        C<T> a(0, 0), d(0, 0);
+        for (int k = 0; k < 4; ++k) {
+          long long j = sym_idx((long long)(2 * c + 1) - k, (long long)n);
+          C<T> x = in[ib + (size_t)j * ip[axis]];
+          a += f.dlo[k] * x;
+          d += f.dhi[k] * x;
+        }
+        out[ob + c * op[axis]] = a;
+        out[ob + (c + nc) * op[axis]] = d;
+      }
+    }
+}
+
+// one-level synthesis along axis (kernels.cu:215-268 formula): from 2*nc coefficients
+// produce `nout` samples (nout <= 2*nc-2; pywt keeps 2*nc-F+2 and trims to the
+// next finer shape). y[s] = sum_j rlo[j]*up_a[s+2-j] + rhi[j]*up_d[s+2-j].
+template <class T>
+static void idwt_axis(const C<T>* in, const size_t* ind, const size_t* ip, C<T>* out, const size_t* op,
+                      size_t nout, int axis, const Db2<T>& f) {
+  size_t nc = ind[axis] / 2;
+  int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+#pragma omp parallel for schedule(static) collapse(2)
+  for (long long u2 = 0; u2 < (long long)ind[a2]; ++u2)
+    for (long long u1 = 0; u1 < (long long)ind[a1]; ++u1) {
+      size_t ib = (size_t)u1 * ip[a1] + (size_t)u2 * ip[a2];
+      size_t ob = (size_t)u1 * op[a1] + (size_t)u2 * op[a2];
+      for (size_t s = 0; s < nout; ++s) {
+        C<T> r(0, 0);
+        long long c = (long long)s + 2;  // start = F-2
+        for (int j = 0; j < 4; ++j) {
+          long long idx = c - j;
+          if (idx >= 0 && idx < (long long)(2 * nc) && (idx % 2) == 0) {
+            size_t t = (size_t)idx / 2;
+            r += f.rlo[j] * in[ib + t * ip[axis]] + f.rhi[j] * in[ib + (t + nc) * ip[axis]];
+          }
+        }
+        out[ob + s * op[axis]] = r;
+      }
+    }
+}
+
+// shrink_complex (lib.rs:460-465); |x| = 0 returns 0 (the reference yields NaN).
+template <class T>
+static inline C<T> shrink(C<T> x, T alpha) {
+  T m = std::abs(x);
+  if (m == 0) return C<T>(0, 0);
+  T d = std::max(m - alpha, T(0));
+  return x * (d / m);
+}
+
+// wavelet_denoise (lib.rs:430-458): multi-level 3D db2 decomposition, shrink
+// every subband (all details + deepest approximation), reconstruct.
+template <class T>
+static void wavelet_denoise(C<T>* vol, const size_t* dims, T alpha, int levels_override) {
+  Db2<T> f;
+  size_t smin = std::min(dims[0], std::min(dims[1], dims[2]));
+  int L = levels_override >= 0 ? levels_override : dwt_max_level(smin, 4);
+  if (L <= 0) {
+    size_t N = dims[0] * dims[1] * dims[2];
+    for (size_t i = 0; i < N; ++i) vol[i] = shrink(vol[i], alpha);
+    return;
+  }
+  std::vector<std::vector<C<T>>> coef(L);
+  std::vector<std::array<size_t, 3>> in_dims(L), cdims(L);
+  size_t cur[3] = {dims[0], dims[1], dims[2]};
+  const C<T>* src = vol;
+  size_t sp[3] = {1, dims[0], dims[0] * dims[1]};
+  for (int l = 0; l < L; ++l) {
+    size_t cd[3];
+    for (int a = 0; a < 3; ++a) cd[a] = 2 * sub_band_size(cur[a], 4);
+    in_dims[l] = {cur[0], cur[1], cur[2]};
+    cdims[l] = {cd[0], cd[1], cd[2]};
+    size_t d1[3] = {cd[0], cur[1], cur[2]}, d2[3] = {cd[0], cd[1], cur[2]};
+    std::vector<C<T>> t1(d1[0] * d1[1] * d1[2]), t2(d2[0] * d2[1] * d2[2]);
+    coef[l].assign(cd[0] * cd[1] * cd[2], C<T>(0, 0));
+    size_t p1[3] = {1, d1[0], d1[0] * d1[1]}, p2[3] = {1, d2[0], d2[0] * d2[1]},
+           pc[3] = {1, cd[0], cd[0] * cd[1]};
+    dwt_axis<T>(src, cur, sp, t1.data(), p1, 0, f);
+    dwt_axis<T>(t1.data(), d1, p1, t2.data(), p2, 1, f);
+    dwt_axis<T>(t2.data(), d2, p2, coef[l].data(), pc, 2, f);
+    // next level input = approx sub-box of coef[l]
+    for (int a = 0; a < 3; ++a) cur[a] = cd[a] / 2;
+    src = coef[l].data();
+    for (int a = 0; a < 3; ++a) sp[a] = pc[a];
+  }
+  // shrink: every coefficient of each level except the approx sub-box of
+  // non-deepest levels (those are replaced on reconstruction)
+  for (int l = 0; l < L; ++l) {
+    size_t* cd = cdims[l].data();
+    size_t h[3] = {cd[0] / 2, cd[1] / 2, cd[2] / 2};
+    C<T>* c = coef[l].data();
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < (long long)cd[2]; ++k)
+      for (size_t j = 0; j < cd[1]; ++j)
+        for (size_t i = 0; i < cd[0]; ++i) {
+          bool approx = i < h[0] && j < h[1] && (size_t)k < h[2];
+          if (approx && l != L - 1) continue;
+          C<T>& v = c[i + cd[0] * (j + cd[1] * (size_t)k)];
+          v = shrink(v, alpha);
+        }
+  }
+  // reconstruct from the deepest level; the output of level l (trimmed to
+  // in_dims[l]) overwrites the approx sub-box of level l-1 (or the volume).
+  for (int l = L - 1; l >= 0; --l) {
+    size_t* cd = cdims[l].data();
+    size_t* od = in_dims[l].data();
+    size_t pc[3] = {1, cd[0], cd[0] * cd[1]};
+    size_t d2[3] = {cd[0], cd[1], od[2]}, d1[3] = {cd[0], od[1], od[2]};
+    std::vector<C<T>> t2(d2[0] * d2[1] * d2[2]), t1(d1[0] * d1[1] * d1[2]);
+    size_t p2[3] = {1, d2[0], d2[0] * d2[1]}, p1[3] = {1, d1[0], d1[0] * d1[1]};
+    idwt_axis<T>(coef[l].data(), cd, pc, t2.data(), p2, od[2], 2, f);
+    idwt_axis<T>(t2.data(), d2, p2, t1.data(), p1, od[1], 1, f);
+    if (l == 0) {
+      size_t pv[3] = {1, dims[0], dims[0] * dims[1]};
+      idwt_axis<T>(t1.data(), d1, p1, vol, pv, od[0], 0, f);
+    } else {
+      size_t* pd = cdims[l - 1].data();
+      size_t pp[3] = {1, pd[0], pd[0] * pd[1]};
+      idwt_axis<T>(t1.data(), d1, p1, coef[l - 1].data(), pp, od[0], 0, f);
+    }
+  }
+}
+
+// -------------------------------------------------- data consistency etc.
+// hard_project (lib.rs:392-406)
+template <class T>
+static double hard_project(C<T>* x, const C<T>* y, size_t n) {
+  double err = 0;
+#pragma omp parallel for reduction(+ : err) schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) {
+    if (y[i] != C<T>(0, 0)) {
+      err += (double)std::norm(x[i] - y[i]);
+      x[i] = y[i];
+    }
+  }
+  return err;
+}
+
+// calc_undersampling_factor (lib.rs:409-417)
+template <class T>
+static void calc_undersampling(const C<T>* y, size_t n, double* us, double* energy) {
+  size_t nz = 0;
+  double e = 0;
+#pragma omp parallel for reduction(+ : nz, e) schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) {
+    nz += (y[i] != C<T>(0, 0)) ? 1 : 0;
+    e += (double)std::norm(y[i]);
+  }
+  *us = (double)n / (double)nz;
+  *energy = e;
+}
+
+// under_sample_kspace (data_import.rs:416-438): zero readout lines whose
+// phase-encode mask entry (y, z, q) is 0.
+template <class T>
+static void under_sample(C<T>* data, const size_t* dims, size_t nq, const C<T>* mask) {
+  size_t nx = dims[0], pe = dims[1] * dims[2];
+#pragma omp parallel for schedule(static)
+  for (long long q = 0; q < (long long)nq; ++q)
+    for (size_t l = 0; l < pe; ++l)
+      if (mask[(size_t)q * pe + l] == C<T>(0, 0))
+        for (size_t i = 0; i < nx; ++i) data[((size_t)q * pe + l) * nx + i] = C<T>(0, 0);
+}
+
+// fftshift.rs:76-116 circshift (out[i] = in[i - s] mod d, Euclidean)
+template <class E>
+static void circshift3(E* data, const size_t* dims, const long long* sh) {
+  size_t N = dims[0] * dims[1] * dims[2];
+  std::vector<E> src(data, data + N);
+  for (size_t z = 0; z < dims[2]; ++z)
+    for (size_t y = 0; y < dims[1]; ++y)
+      for (size_t x = 0; x < dims[0]; ++x) {
+        long long c[3] = {(long long)x - sh[0], (long long)y - sh[1], (long long)z - sh[2]};
+        size_t s[3];
+        for (int a = 0; a < 3; ++a) {
+          long long d = (long long)dims[a];
+          s[a] = (size_t)(((c[a] % d) + d) % d);
+        }
+        data[x + dims[0] * (y + dims[1] * z)] = src[s[0] + dims[0] * (s[1] + dims[1] * s[2])];
+      }
+}
+
+// fftshift.rs:285-318 phase_shift3: phase pi*(i - n/2) per axis, f32 as the reference
+static void phase_shift3_f32(C<float>* x, const size_t* dims, int sign) {
+  const float PI_F = 3.14159265358979323846f;
+  for (size_t iz = 0; iz < dims[2]; ++iz)
+    for (size_t iy = 0; iy < dims[1]; ++iy)
+      for (size_t ix = 0; ix < dims[0]; ++ix) {
+        float zs = PI_F * ((float)iz - ((float)dims[2] / 2.f));
+        float ys = PI_F * ((float)iy - ((float)dims[1] / 2.f));
+        float xs = PI_F * ((float)ix - ((float)dims[0] / 2.f));
+        float tot = (float)sign * (xs + ys + zs);
+        C<float>& v = x[ix + dims[0] * (iy + dims[1] * iz)];
+        v = v * C<float>(std::cos(tot), std::sin(tot));
+      }
+}
+
+}  // namespace orc
+
+// ======================================================= recon (lib.rs loop)
+struct orc_recon_params {
+  size_t vol[3];
+  size_t nq;
+  size_t blk[3];
+  size_t n_iter;
+  int llr_mode;          // 0 rank truncation (reference), 1 singular-value soft threshold
+  int phase_mode;        // 0 reference f32 ramps, 1 exact ramps
+  size_t rank;           // mode 0
+  double svt_lambda;     // mode 1: absolute threshold if > 0
+  double lambda_factor;  // mode 1: threshold = factor * sigma_max(zero-filled, shift 0) if svt_lambda <= 0
+  double wavelet_alpha;  // 0: no wavelet step
+  int wavelet_levels;    // -1: pywt max level
+  size_t srank;          // 0: no subspace
+  const void* subspace;  // nq x srank complex, col-major (T precision)
+  const size_t* shifts;  // n_iter x 3
+};
+
+namespace orc {
+
+// The iteration loop of llr_recon_exec (lib.rs:227-341) from the measurements
+// on: linear-correction estimate, zero-filled adjoint, then per iteration
+// LLR (lift, [project], low-rank, [back-project], unlift) -> wavelet ->
+// forward model -> hard projection -> adjoint. Blocks of one iteration never
+// overlap, so the reference's matrix batching does not change the result.
+template <class T>
+static int recon(const orc_recon_params* p, const C<T>* meas, C<T>* x, double* err_hist,
+                 double* out_lambda, float* out_ksp_corr) {
+  const size_t* vol = p->vol;
+  size_t nq = p->nq;
+  size_t N = vol[0] * vol[1] * vol[2];
+  size_t m = p->blk[0] * p->blk[1] * p->blk[2];
+  size_t grid[3];
+  grid_dim(vol, p->blk, grid);
+  size_t nblocks = grid[0] * grid[1] * grid[2];
+  size_t ncols = p->srank ? p->srank : nq;
+  if (m <= nq) return -2;  // lib.rs:203
+  if (p->llr_mode == 0 && (p->rank == 0 || p->rank > ncols)) return -3;
+  const C<T>* S = (const C<T>*)p->subspace;
+
+  double us, energy;
+  calc_undersampling<T>(meas, N * nq, &us, &energy);
+  std::vector<float> kc(3 * nq), ic(3 * nq, 0.f);
+  for (size_t q = 0; q < nq; ++q) estimate_linear_correction<T>(meas + q * N, vol, &kc[3 * q]);
+  if (out_ksp_corr) std::memcpy(out_ksp_corr, kc.data(), sizeof(float) * 3 * nq);
+  std::memcpy(x, meas, sizeof(C<T>) * N * nq);
+  signal_model<T>(x, vol, nq, kc.data(), ic.data(), 1, p->phase_mode);
+
+  std::vector<C<T>> mats(nblocks * m * nq), comp;
+  if (p->srank) comp.resize(nblocks * m * p->srank);
+
+  auto llr = [&](const size_t* shift, int mode, T lam, bool modify) -> T {
+    lift((const uint8_t*)x, nq, sizeof(C<T>), (uint8_t*)mats.data(), vol, p->blk, shift, 0, nblocks);
+    T smax;
+    C<T>* target;
+    if (p->srank) {
+      // dti_project -> conj_transpose (lib.rs:285-286): comp = X S, (m x srank) per block
+      cgemm<T>(p->srank, nq, m, nblocks, true, C<T>(1, 0), C<T>(0, 0), true, true, S, mats.data(),
+               comp.data());
+      conj_transpose<T>(comp.data(), p->srank, m, nblocks);
+      target = comp.data();
+    } else {
+      target = mats.data();
+    }
+    if (!modify) {
+      std::vector<T> s(nblocks * ncols);
+      block_svals<T>(m, ncols, nblocks, target, s.data());
+      smax = 0;
+      for (size_t b = 0; b < nblocks; ++b) smax = std::max(smax, s[b * ncols]);
+      return smax;
+    }
+    smax = lowrank_batch<T>(m, ncols, nblocks, target, mode, (int)p->rank, lam);
+    if (p->srank) {
+      // conj_transpose -> dti_back_project (lib.rs:288-289): X = L S^H
+      conj_transpose<T>(comp.data(), m, p->srank, nblocks);
+      cgemm<T>(nq, p->srank, m, nblocks, true, C<T>(1, 0), C<T>(0, 0), false, false, S, comp.data(),
+               mats.data());
+      conj_transpose<T>(mats.data(), nq, m, nblocks);
+    }
+    unlift((uint8_t*)x, nq, sizeof(C<T>), (const uint8_t*)mats.data(), vol, p->blk, shift, 0, nblocks);
+    return smax;
+  };
+
+  T lam = 0;
+  if (p->llr_mode == 1) {
+    if (p->svt_lambda > 0)
+      lam = (T)p->svt_lambda;
+    else {
+      size_t zero[3] = {0, 0, 0};
+      lam = (T)(p->lambda_factor * (double)llr(zero, 1, 0, false));
+    }
+  }
+  if (out_lambda) *out_lambda = (double)lam;
+
+  for (size_t it = 0; it < p->n_iter; ++it) {
+    const size_t* shift = p->shifts + 3 * it;
+    llr(shift, p->llr_mode, lam, true);
+    if (p->wavelet_alpha > 0)
+      for (size_t q = 0; q < nq; ++q) wavelet_denoise<T>(x + q * N, vol, (T)p->wavelet_alpha, p->wavelet_levels);
+    signal_model<T>(x, vol, nq, kc.data(), ic.data(), 0, p->phase_mode);
+    double e = hard_project<T>(x, meas, N * nq);
+    if (err_hist) err_hist[it] = e / energy;
+    signal_model<T>(x, vol, nq, kc.data(), ic.data(), 1, p->phase_mode);
+  }
+  return 0;
+}
+
+}  // namespace orc
+
+// ================================================================ C ABI
+#define CF(T) std::complex<T>
+extern "C" {
+
+int orc_num_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+void orc_grid_dim(const size_t* vol, const size_t* blk, size_t* grid) { orc::grid_dim(vol, blk, grid); }
+
+int orc_lift_data_set(const void* vols, size_t nvols, size_t eb, void* mats, const size_t* vol,
+                      const size_t* blk, const size_t* shift, size_t start, size_t n) {
+  return orc::lift((const uint8_t*)vols, nvols, eb, (uint8_t*)mats, vol, blk, shift, start, n);
+}
+int orc_un_lift_data_set(void* vols, size_t nvols, size_t eb, const void* mats, const size_t* vol,
+                         const size_t* blk, const size_t* shift, size_t start, size_t n) {
+  return orc::unlift((uint8_t*)vols, nvols, eb, (const uint8_t*)mats, vol, blk, shift, start, n);
+}
+
+#define ORC_TYPED(SUF, T)                                                                              \
+  void orc_cgemm_##SUF(size_t m, size_t k, size_t n, size_t batch, int bcast, const T* alpha,          \
+                       const T* beta, int ta, int tb, const void* A, const void* B, void* Cm) {        \
+    orc::cgemm<T>(m, k, n, batch, bcast != 0, CF(T)(alpha[0], alpha[1]), CF(T)(beta[0], beta[1]),     \
+                  ta != 0, tb != 0, (const CF(T)*)A, (const CF(T)*)B, (CF(T)*)Cm);                     \
+  }                                                                                                    \
+  void orc_conj_transpose_##SUF(void* d, size_t m, size_t n, size_t batch) {                           \
+    orc::conj_transpose<T>((CF(T)*)d, m, n, batch);                                                    \
+  }                                                                                                    \
+  void orc_dti_project_##SUF(const void* S, const void* X, void* Cm, size_t batch, size_t nq,          \
+                             size_t nvox, size_t srank) {                                              \
+    orc::cgemm<T>(srank, nq, nvox, batch, true, CF(T)(1, 0), CF(T)(0, 0), true, true, (const CF(T)*)S, \
+                  (const CF(T)*)X, (CF(T)*)Cm);                                                        \
+  }                                                                                                    \
+  void orc_dti_back_project_##SUF(const void* S, void* X, const void* Cm, size_t batch, size_t nq,     \
+                                  size_t nvox, size_t srank) {                                         \
+    orc::cgemm<T>(nq, srank, nvox, batch, true, CF(T)(1, 0), CF(T)(0, 0), false, false,                \
+                  (const CF(T)*)S, (const CF(T)*)Cm, (CF(T)*)X);                                       \
+    orc::conj_transpose<T>((CF(T)*)X, nq, nvox, batch);                                                \
+  }                                                                                                    \
+  void orc_subspace_constraint_##SUF(const void* S, void* X, size_t batch, size_t nq, size_t nvox,     \
+                                     size_t srank) {                                                   \
+    std::vector<CF(T)> c(nvox * srank * batch), b(nq * nvox * batch);                                  \
+    orc::cgemm<T>(srank, nq, nvox, batch, true, CF(T)(1, 0), CF(T)(0, 0), true, true, (const CF(T)*)S, \
+                  (const CF(T)*)X, c.data());                                                          \
+    orc::cgemm<T>(nq, srank, nvox, batch, true, CF(T)(1, 0), CF(T)(0, 0), false, false,                \
+                  (const CF(T)*)S, c.data(), b.data());                                                \
+    CF(T)* x = (CF(T)*)X;                                                                              \
+    for (size_t bb = 0; bb < batch; ++bb)                                                              \
+      for (size_t j = 0; j < nvox; ++j)                                                                \
+        for (size_t i = 0; i < nq; ++i)                                                                \
+          x[bb * nq * nvox + i * nvox + j] = b[bb * nq * nvox + j * nq + i];                           \
+  }                                                                                                    \
+  double orc_low_rank_approx_batch_##SUF(size_t m, size_t n, size_t rank, size_t batch, void* d) {     \
+    return (double)orc::lowrank_batch<T>(m, n, batch, (CF(T)*)d, 0, (int)rank, (T)0);                  \
+  }                                                                                                    \
+  double orc_svt_batch_##SUF(size_t m, size_t n, double lam, size_t batch, void* d) {                  \
+    return (double)orc::lowrank_batch<T>(m, n, batch, (CF(T)*)d, 1, 0, (T)lam);                        \
+  }                                                                                                    \
+  void orc_block_svals_##SUF(size_t m, size_t n, size_t batch, const void* d, T* s) {                  \
+    orc::block_svals<T>(m, n, batch, (const CF(T)*)d, s);                                              \
+  }                                                                                                    \
+  void orc_herm_eig_##SUF(int n, void* A, T* w, void* V) {                                             \
+    orc::herm_eig<T>(n, (CF(T)*)A, w, (CF(T)*)V);                                                      \
+  }                                                                                                    \
+  void orc_fft3_batch_##SUF(void* d, const size_t* dims, size_t nvols, int sign) {                     \
+    orc::fft3_batch<T>((CF(T)*)d, dims, nvols, sign);                                                  \
+  }                                                                                                    \
+  void orc_signal_model_##SUF(void* x, const size_t* dims, size_t nvols, const float* kc,              \
+                              const float* ic, int dir, int phase_mode) {                              \
+    orc::signal_model<T>((CF(T)*)x, dims, nvols, kc, ic, dir, phase_mode);                             \
+  }                                                                                                    \
+  void orc_estimate_linear_correction_##SUF(const void* k, const size_t* dims, float* out) {           \
+    orc::estimate_linear_correction<T>((const CF(T)*)k, dims, out);                                    \
+  }                                                                                                    \
+  void orc_wavelet_denoise_##SUF(void* v, const size_t* dims, double alpha, int levels) {              \
+    orc::wavelet_denoise<T>((CF(T)*)v, dims, (T)alpha, levels);                                        \
+  }                                                                                                    \
+  void orc_dwt3_##SUF(const void* v, const size_t* dims, void* out) {                                  \
+    orc::Db2<T> f;                                                                                     \
+    size_t nc[3], cd[3];                                                                               \
+    for (int a = 0; a < 3; ++a) {                                                                      \
+      nc[a] = orc::sub_band_size(dims[a], 4);                                                          \
+      cd[a] = 2 * nc[a];                                                                               \
+    }                                                                                                  \
+    size_t d1[3] = {cd[0], dims[1], dims[2]}, d2[3] = {cd[0], cd[1], dims[2]};                         \
+    std::vector<CF(T)> t1(d1[0] * d1[1] * d1[2]), t2(d2[0] * d2[1] * d2[2]);                           \
+    size_t pv[3] = {1, dims[0], dims[0] * dims[1]}, p1[3] = {1, d1[0], d1[0] * d1[1]},                 \
+           p2[3] = {1, d2[0], d2[0] * d2[1]}, pc[3] = {1, cd[0], cd[0] * cd[1]};                       \
+    orc::dwt_axis<T>((const CF(T)*)v, dims, pv, t1.data(), p1, 0, f);                                  \
+    orc::dwt_axis<T>(t1.data(), d1, p1, t2.data(), p2, 1, f);                                          \
+    orc::dwt_axis<T>(t2.data(), d2, p2, (CF(T)*)out, pc, 2, f);                                        \
+  }                                                                                                    \
+  void orc_idwt3_##SUF(const void* coef, const size_t* cd, void* v, const size_t* dims) {              \
+    orc::Db2<T> f;                                                                                     \
+    size_t d2[3] = {cd[0], cd[1], dims[2]}, d1[3] = {cd[0], dims[1], dims[2]};                         \
+    std::vector<CF(T)> t2(d2[0] * d2[1] * d2[2]), t1(d1[0] * d1[1] * d1[2]);                           \
+    size_t pc[3] = {1, cd[0], cd[0] * cd[1]}, p2[3] = {1, d2[0], d2[0] * d2[1]},                       \
+           p1[3] = {1, d1[0], d1[0] * d1[1]}, pv[3] = {1, dims[0], dims[0] * dims[1]};                 \
+    orc::idwt_axis<T>((const CF(T)*)coef, cd, pc, t2.data(), p2, dims[2], 2, f);                       \
+    orc::idwt_axis<T>(t2.data(), d2, p2, t1.data(), p1, dims[1], 1, f);                                \
+    orc::idwt_axis<T>(t1.data(), d1, p1, (CF(T)*)v, pv, dims[0], 0, f);                                \
+  }                                                                                                    \
+  double orc_hard_project_##SUF(void* x, const void* y, size_t n) {                                    \
+    return orc::hard_project<T>((CF(T)*)x, (const CF(T)*)y, n);                                        \
+  }                                                                                                    \
+  void orc_calc_undersampling_##SUF(const void* y, size_t n, double* us, double* e) {                  \
+    orc::calc_undersampling<T>((const CF(T)*)y, n, us, e);                                             \
+  }                                                                                                    \
+  void orc_under_sample_kspace_##SUF(void* d, const size_t* dims, size_t nq, const void* mask) {       \
+    orc::under_sample<T>((CF(T)*)d, dims, nq, (const CF(T)*)mask);                                     \
+  }                                                                                                    \
+  int orc_recon_##SUF(const orc_recon_params* p, const void* meas, void* x, double* hist, double* lam, \
+                      float* kc) {                                                                     \
+    return orc::recon<T>(p, (const CF(T)*)meas, (CF(T)*)x, hist, lam, kc);                             \
+  }
+
+ORC_TYPED(f32, float)
+ORC_TYPED(f64, double)
+
+void orc_fftshift(void* data, size_t eb, const size_t* dims, int inverse) {
+  long long sh[3];
+  for (int a = 0; a < 3; ++a)
+    sh[a] = inverse ? (long long)((dims[a] + 1) / 2) : (long long)(dims[a] / 2);  // fftshift.rs:103-116
+  if (eb == 8)
+    orc::circshift3<uint64_t>((uint64_t*)data, dims, sh);
+  else if (eb == 4)
+    orc::circshift3<uint32_t>((uint32_t*)data, dims, sh);
+  else if (eb == 16) {
+    struct E16 {
+      uint64_t a, b;
+    };
+    orc::circshift3<E16>((E16*)data, dims, sh);
+  }
+}
+
+void orc_phase_shift3_f32(void* x, const size_t* dims, int sign) {
+  orc::phase_shift3_f32((std::complex<float>*)x, dims, sign);
+}
+
+// fftshift.rs:13-25 coord_to_col_maj_index with Euclidean wrap
+size_t orc_coord_to_col_maj_index(const long long* coords, const size_t* dims, size_t nd) {
+  size_t idx = 0, stride = 1;
+  for (size_t a = 0; a < nd; ++a) {
+    long long d = (long long)dims[a];
+    long long c = ((coords[a] % d) + d) % d;
+    idx += (size_t)c * stride;
+    stride *= dims[a];
+  }
+  return idx;
+}
+
+}  // extern "C"
