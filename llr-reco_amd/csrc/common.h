@@ -68,6 +68,10 @@ struct DevBuf {
     if (p) (void)hipFree(p);
   }
   hipError_t alloc(size_t b) {
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
     bytes = b;
     return hipMalloc(&p, b ? b : 1);
   }

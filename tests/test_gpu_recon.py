@@ -1,0 +1,82 @@
+"""Parity of the device-resident reconstruction (llr_recon_*) with the CPU
+reference path (the oracle's FP32 restatement of lib.rs:227-341) and the FP64
+oracle, on identical measurements and shift sequences.
+
+Bar: relative NRMSE of the plain FP32 GPU image vs the FP32 CPU reference path
+<= 1e-4.
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def run_gpu(ds, **kw):
+    from llr_hip import recon
+
+    cfg = ds["cfg"]
+    r = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode=kw.get("llr_mode", "svt"),
+                    rank=kw.get("rank", 0), lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
+                    subspace=ds["subspace"])
+    r.set_measurements(ds["meas"])
+    hist = r.run()
+    img = r.image()
+    lam, kc = r.info()
+    r.close()
+    return img, hist, lam, kc
+
+
+def run_cpu(ds, dtype, **kw):
+    cfg = ds["cfg"]
+    sub = ds["subspace"]
+    return O.recon(ds["meas"].astype(dtype), ds["vol"], ds["nq"], cfg["blk"], ds["shifts"],
+                   llr_mode={"svt": 1, "rank": 0}[kw.get("llr_mode", "svt")], rank=kw.get("rank", 0),
+                   lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
+                   subspace=None if sub is None else sub.astype(dtype))
+
+
+def check(ds, label, **kw):
+    img, hist, lam, kc = run_gpu(ds, **kw)
+    x32, h32, lam32, kc32 = run_cpu(ds, np.complex64, **kw)
+    x64, h64, lam64, kc64 = run_cpu(ds, np.complex128, **kw)
+    e_ref = H.nrmse(img, x32)
+    e_gpu64 = H.nrmse(img, x64)
+    e_ref64 = H.nrmse(x32, x64)
+    print(f"[{label}] NRMSE gpu-vs-cpu32 {e_ref:.3e}  gpu-vs-fp64 {e_gpu64:.3e}  cpu32-vs-fp64 {e_ref64:.3e}  "
+          f"lambda gpu {lam:.6g} cpu {lam32:.6g}  final residual gpu {hist[-1]:.4e} cpu {h32[-1]:.4e}", flush=True)
+    assert np.array_equal(kc, kc32) and np.array_equal(kc, kc64)
+    assert np.all(np.isfinite(img))
+    assert abs(lam - lam64) <= 1e-5 * lam64 + 1e-30
+    assert np.allclose(hist, h64, rtol=1e-3)
+    assert e_ref <= TOL, e_ref
+    assert e_gpu64 <= TOL
+    return e_ref
+
+
+def test_recon_small_plain_svt(llr):
+    check(H.make(H.SMALL_PLAIN), "32^3 x7 svt")
+
+
+def test_recon_small_subspace_wavelet(llr):
+    check(H.make(H.SMALL_SUBSPACE), "32x32x16 x13 subspace7 wavelet")
+
+
+def test_recon_small_rank_mode(llr):
+    ds = H.make(dict(H.SMALL_PLAIN, n_iter=4))
+    check(ds, "32^3 x7 rank5", llr_mode="rank", rank=5)
+
+
+def test_recon_config1_parity(llr):
+    """Config 1: 64^3, 1 b0 + 6 directions, R=4, 8^3 blocks, SVT 0.02 sigma_max, 20 iterations."""
+    ds = H.make("c1")
+    check(ds, "c1 64^3 x7 20 it")
+
+
+def test_recon_residual_decreases(llr):
+    img, hist, lam, kc = run_gpu(H.make(H.SMALL_SUBSPACE))
+    assert hist[-1] < hist[0]
