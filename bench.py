@@ -1,0 +1,217 @@
+"""Benchmark: seconds per LLR reconstruction at 256x256x128 with 67 volumes.
+
+A "step" is one whole reconstruction (linear-correction estimate, zero-filled
+adjoint, SVT threshold, 30 iterations of LLR -> wavelet -> forward model ->
+hard projection -> adjoint) of one synthetic data set whose measurements are
+already resident in HBM. Prints one JSON line on stdout; logs go to stderr.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1]
+                  [--no-cpu-baseline] [--no-stage-timing]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "llr-reco_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_measurements(preset, torch, llr):
+    """Synthetic data set of the preset with its measurements on the GPU."""
+    import numpy as np
+
+    from llr_hip import phantom, signal_model
+
+    cfg = dict(phantom.PRESETS[preset])
+    s_ph, s_noise, s_mask, s_shift = cfg["seeds"]
+    dirs = phantom.six_directions() if cfg["dirs"] == "six" else phantom.repulsion_directions(cfg["n_dirs"], s_ph + 1000)
+    bvals, bvecs = phantom.b_table(cfg["n_b0"], dirs, cfg["bval"])
+    nq = len(bvals)
+    vol = cfg["vol"]
+    N = int(np.prod(vol))
+    t = time.time()
+    img, truth = phantom.phantom_image(vol, bvals, bvecs, s_ph, cfg["n_ell"], return_truth=True)
+    s0_obj = float(np.mean(np.abs(img[:N][truth["labels"].ravel(order="F") > 0])))
+    log(f"phantom {vol} x{nq}: {time.time() - t:.1f} s")
+    dev = torch.device("cuda", 0)
+    x = torch.from_numpy(img).to(dev)
+    del img
+    torch.cuda.synchronize()
+    zero = np.zeros((nq, 3), np.float32)
+    signal_model.signal_model_device(x.data_ptr(), x.numel(), vol, nq, zero, zero, signal_model.ModelDirection.Forward)
+    g = torch.Generator(device=dev)
+    g.manual_seed(s_noise)
+    sigma = s0_obj / 30.0
+    noise = torch.randn(2 * x.numel(), device=dev, dtype=torch.float32, generator=g) * (sigma / np.sqrt(2.0))
+    x += torch.view_as_complex(noise.view(-1, 2))
+    del noise
+    mask = phantom.pe_mask(vol[1], vol[2], nq, cfg["R"], s_mask)  # (ny, nz, nq)
+    m = torch.from_numpy(np.ascontiguousarray(np.real(mask).transpose(2, 1, 0))).to(dev)  # (nq, nz, ny)
+    xv = x.view(nq, vol[2], vol[1], vol[0])
+    xv *= (m != 0).to(torch.complex64)[..., None]
+    torch.cuda.synchronize()
+    sub = phantom.loglinear_subspace(bvals, bvecs, cfg["srank"]) if cfg["srank"] else None
+    shifts = phantom.shift_sequence(vol, cfg["n_iter"], s_shift)
+    log(f"measurements on device: {time.time() - t:.1f} s, undersampling R={cfg['R']}")
+    return dict(y=x, vol=vol, nq=nq, cfg=cfg, subspace=sub, shifts=shifts)
+
+
+def stage_bytes(ds):
+    """Algorithmic HBM bytes per launch of each timed kernel."""
+    N = int(ds["vol"][0] * ds["vol"][1] * ds["vol"][2])
+    Q = ds["nq"]
+    K = ds["cfg"]["srank"] or Q
+    D = N * Q * 8
+    return {
+        # gather K coefficient planes, write Q image planes (with a subspace) or K planes in place
+        "llr_block": N * K * 8 + N * (Q if ds["cfg"]["srank"] else K) * 8,
+        "subspace_project": D + N * K * 8,  # read X, write C
+        "wavelet": 2 * D,                    # one call: read and write the iterate once
+        "fft_x": 2 * D,
+        "fft_y": 2 * D,
+        "fft_z_project": 3 * D,              # read X, read Y, write X
+    }
+
+
+def cpu_baseline(ds, torch):
+    """The oracle's FP32 reference path on the same measurements: setup alone
+    and setup + 1 iteration, extrapolated to the preset's iteration count."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    cores = O.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    y = ds["y"].cpu().numpy()
+    cfg = ds["cfg"]
+    sub = ds["subspace"]
+    times = []
+    for n in (0, 1):
+        t = time.time()
+        O.recon(y, ds["vol"], ds["nq"], cfg["blk"], ds["shifts"][:n], llr_mode=1,
+                lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"], subspace=sub)
+        times.append(time.time() - t)
+        log(f"cpu reference path, {n} iteration(s): {times[-1]:.1f} s on {cores} threads")
+    per_it = max(times[1] - times[0], 0.0)
+    value = times[0] + cfg["n_iter"] * per_it
+    del y
+    return {"value": round(value, 2), "unit": "s/recon", "cores": cores, "kind": "port",
+            "sample": (f"FP32 C++ restatement of lib.rs:227-341 on the full {ds['vol']}x{ds['nq']} measurements: "
+                       f"setup ({times[0]:.1f} s) and setup + 1 iteration ({times[1]:.1f} s) timed, "
+                       f"extrapolated to {cfg['n_iter']} iterations")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stage-timing", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy kernel (counter calibration) and exit")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        n = (1 << 30) // 4
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 copies of 1 GiB (read 1 GiB + write 1 GiB each)")
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is design-only.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import llr_hip
+    from llr_hip import recon
+
+    llr_hip.load()
+    ds = build_measurements(args.workload, torch, llr_hip)
+    cfg = ds["cfg"]
+    r = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode="svt",
+                    lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"], subspace=ds["subspace"])
+    r.set_measurements(device_ptr=ds["y"].data_ptr())
+
+    for _ in range(args.warmup):
+        r.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = None
+    for _ in range(args.steps):
+        hist = r.run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s_per = elapsed / max(args.steps, 1)
+    lam, _ = r.info()
+    log(f"timed {args.steps} recon(s): {s_per:.4f} s/recon; lambda {lam:.5g}; residual {hist[0]:.4e} -> {hist[-1]:.4e}")
+
+    roof = None
+    stages = {}
+    if not args.no_stage_timing:
+        r.enable_timing(True)
+        r.run()
+        st = r.stage_times()
+        r.enable_timing(False)
+        nbytes = stage_bytes(ds)
+        for name, (ms, n) in st.items():
+            if n:
+                stages[name] = {"total_ms": round(ms, 3), "launches": n, "avg_ms": round(ms / n, 4)}
+                if name in nbytes:
+                    gbs = nbytes[name] / (ms / n * 1e-3) / 1e9
+                    stages[name]["alg_GBps"] = round(gbs, 1)
+        log("stage timing (HIP events on the library stream): " + json.dumps(stages))
+        kernels = [k for k in ("fft_z_project", "fft_y", "fft_x", "llr_block", "subspace_project") if k in stages]
+        dom = max(kernels, key=lambda k: stages[k]["total_ms"])
+        ach = stages[dom]["alg_GBps"]
+        roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "alg_bytes_per_launch": nbytes[dom], "avg_launch_ms": stages[dom]["avg_ms"]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        r.close()
+        cpu = cpu_baseline(ds, torch)
+
+    out = {
+        "metric": "s/recon (256²×128, 67 vols): learned+BF16 vs plain FP32 at equal NRMSE",
+        "value": round(s_per, 5),
+        "unit": "s/recon",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s_per * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded DTI ellipsoid phantom, variable-density PE mask, complex Gaussian noise)",
+        "config": {"workload": (f"plain FP32 LLR recon {ds['vol'][0]}x{ds['vol'][1]}x{ds['vol'][2]}x{ds['nq']}, "
+                                f"{cfg['blk'][0]}^3 blocks, SVT 0.02*sigma_max, rank-{cfg['srank']} subspace, "
+                                f"db2 wavelet {cfg['wavelet_alpha']}, {cfg['n_iter']} iterations"),
+                   "form": "plain_fp32", "iterations": cfg["n_iter"], "stages": stages},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

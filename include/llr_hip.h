@@ -247,8 +247,9 @@ llr_status llr_recon_info(llr_recon* r, double* lambda, float* ksp_corr);
 
 /* Stage timing with HIP events on the context's stream (for roofline reports).
  * Stages: 0 LLR block kernel, 1 subspace projection, 2 wavelet, 3 FFT x pass,
- * 4 FFT y pass, 5 FFT z pass + hard projection, 6 reductions. */
-#define LLR_NUM_STAGES 7
+ * 4 FFT y pass, 5 FFT z pass + hard projection, 6 residual reduction, 7 run set-up
+ * (statistics, corrections, zero-filled adjoint, threshold). */
+#define LLR_NUM_STAGES 8
 llr_status llr_recon_enable_timing(llr_recon* r, int enable);
 llr_status llr_recon_stage_times(llr_recon* r, double* total_ms, size_t* launches);
 

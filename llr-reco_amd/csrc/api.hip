@@ -761,7 +761,7 @@ llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done) {
   llr_status st;
   // total energy (lib.rs:224, 409-417)
   {
-    StageTimer t(r, 6);
+    StageTimer t(r, 7);
     if ((st = energy_partials(ctx, r->Y.as<float2>(), r->D, r->partials.as<double>(), 1024))) return st;
     if ((st = reduce_strided(ctx, r->partials.as<double>(), 1024, 2, r->stats.as<double>()))) return st;
   }
@@ -783,7 +783,7 @@ llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done) {
   if ((st = make_tables(r->tabs, p.vol, r->nq, r->ksp_corr.data(), nullptr))) return st;
   // zero-filled adjoint (lib.rs:233-240) unless warm-started
   if (!r->have_init) {
-    StageTimer t(r, 3);
+    StageTimer t(r, 7);
     if ((st = model_device(ctx, r->Y.as<float2>(), r->X.as<float2>(), p.vol, r->nq, r->tabs, LLR_INVERSE))) return st;
   }
   // threshold: lambda_factor * sigma_max of the starting blocks at shift 0

@@ -117,16 +117,23 @@ struct Rot {
   float2 u;  // unit phase of A[p][q]
 };
 
+// The modulus of A[p][q] is formed scaled (hypot-style): for off-diagonal
+// entries near the FP32 underflow range sqrt(x^2 + y^2) loses its digits and
+// the phase u = a_pq/|a_pq| would not be unit-modulus, making the rotation
+// non-unitary. Entries below 1e-9 of the diagonal pair are left alone.
 __device__ __forceinline__ Rot rot_params(float app, float aqq, float2 apq) {
   Rot r;
-  float a = sqrtf(apq.x * apq.x + apq.y * apq.y);
-  if (a == 0.0f) {
+  float s = fmaxf(fabsf(apq.x), fabsf(apq.y));
+  if (s == 0.0f || s <= 1e-9f * (fabsf(app) + fabsf(aqq))) {
     r.cs = 1.f;
     r.sn = 0.f;
     r.u = make_float2(1.f, 0.f);
     return r;
   }
-  r.u = make_float2(apq.x / a, apq.y / a);
+  float xs = apq.x / s, ys = apq.y / s;
+  float rn = 1.0f / sqrtf(xs * xs + ys * ys);
+  float a = s / rn;
+  r.u = make_float2(xs * rn, ys * rn);
   float theta = (aqq - app) / (2.0f * a);
   float at = fabsf(theta);
   float t = (at > 1e18f) ? 0.5f / theta : ((theta >= 0.f ? 1.f : -1.f) / (at + sqrtf(theta * theta + 1.f)));
@@ -303,7 +310,7 @@ __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
       if (r == c) a.y = 0.f;
     }
     float2 vv;
-    wave_jacobi8(a, vv, tid, 16, 1.0e-7f);
+    wave_jacobi8(a, vv, tid, 16, 2.5e-7f);
     float lam[8];
 #pragma unroll
     for (int l = 0; l < 8; ++l) lam[l] = __shfl(a.x, 9 * l, 64);

@@ -60,6 +60,17 @@ __global__ void __launch_bounds__(RB) k_reduce(const double* __restrict__ in, lo
   if (threadIdx.x == 0) out[j] = s;
 }
 
+// out[b] = sum of in[b*chunk .. (b+1)*chunk) in a fixed order
+__global__ void __launch_bounds__(RB) k_reduce_chunks(const double* __restrict__ in, long long n, long long chunk,
+                                                      double* out) {
+  __shared__ double sh[RB / 64];
+  long long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
+  double s = 0.0;
+  for (long long i = lo + threadIdx.x; i < hi; i += RB) s += in[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
 __global__ void __launch_bounds__(RB) k_hard_project(float2* __restrict__ x, const float2* __restrict__ y, long long n,
                                                      double* partials, long long chunk) {
   __shared__ double sh[RB / 64];
@@ -183,8 +194,10 @@ __global__ void k_cgemm(long long m, long long k, long long n, long long batch, 
       float2 bv = tb ? cconj(bb[j + l * ldb]) : bb[l + j * ldb];
       acc = cadd(acc, cmul(av, bv));
     }
-    float2 prev = C[b * sc + i + j * m];
-    C[b * sc + i + j * m] = cadd(cmul(alpha, acc), cmul(beta, prev));
+    float2 r = cmul(alpha, acc);
+    // like BLAS, C is not read when beta == 0 (it may hold uninitialised memory)
+    if (beta.x != 0.f || beta.y != 0.f) r = cadd(r, cmul(beta, C[b * sc + i + j * m]));
+    C[b * sc + i + j * m] = r;
   }
 }
 
@@ -228,6 +241,19 @@ llr_status reduce_sum(llr_ctx* ctx, const double* partials, size_t nparts, doubl
 }
 
 llr_status reduce_strided(llr_ctx* ctx, const double* partials, size_t n, size_t stride, double* out) {
+  if (stride == 1 && n > 8192) {
+    // two fixed-order levels: 256 chunk sums, then their sum
+    const size_t nb = 256;
+    double* tmp = nullptr;
+    llr_status st = red_scratch(ctx, nb, &tmp);
+    if (st) return st;
+    long long chunk = (long long)((n + nb - 1) / nb);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)nb), dim3(RB), 0, ctx->stream, partials, (long long)n, chunk,
+                       tmp);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(RB), 0, ctx->stream, tmp, (long long)nb, 1LL, out);
+    LLR_HIP(hipGetLastError());
+    return LLR_OK;
+  }
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)stride), dim3(RB), 0, ctx->stream, partials, (long long)n,
                      (long long)stride, out);
   LLR_HIP(hipGetLastError());

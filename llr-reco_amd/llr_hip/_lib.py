@@ -21,8 +21,8 @@ LLR_MODE_RANK = 0
 LLR_MODE_SVT = 1
 LLR_PREC_FP32 = 0
 LLR_PREC_MIXED = 1
-LLR_NUM_STAGES = 7
-STAGE_NAMES = ["llr_block", "subspace_project", "wavelet", "fft_x", "fft_y", "fft_z_project", "reductions"]
+LLR_NUM_STAGES = 8
+STAGE_NAMES = ["llr_block", "subspace_project", "wavelet", "fft_x", "fft_y", "fft_z_project", "reductions", "setup"]
 
 # every symbol include/llr_hip.h declares
 EXPORTS = [

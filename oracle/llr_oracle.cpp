@@ -997,11 +997,22 @@ void orc_fftshift(void* data, size_t eb, const size_t* dims, int inverse) {
     orc::circshift3<uint64_t>((uint64_t*)data, dims, sh);
   else if (eb == 4)
     orc::circshift3<uint32_t>((uint32_t*)data, dims, sh);
-  else if (eb == 16) {
-    struct E16 {
-      uint64_t a, b;
-    };
-    orc::circshift3<E16>((E16*)data, dims, sh);
+  else {
+    // any element size: move eb-byte records
+    size_t N = dims[0] * dims[1] * dims[2];
+    std::vector<uint8_t> src((uint8_t*)data, (uint8_t*)data + N * eb);
+    for (size_t z = 0; z < dims[2]; ++z)
+      for (size_t y = 0; y < dims[1]; ++y)
+        for (size_t x = 0; x < dims[0]; ++x) {
+          long long c[3] = {(long long)x - sh[0], (long long)y - sh[1], (long long)z - sh[2]};
+          size_t s[3];
+          for (int a = 0; a < 3; ++a) {
+            long long d = (long long)dims[a];
+            s[a] = (size_t)(((c[a] % d) + d) % d);
+          }
+          std::memcpy((uint8_t*)data + (x + dims[0] * (y + dims[1] * z)) * eb,
+                      src.data() + (s[0] + dims[0] * (s[1] + dims[1] * s[2])) * eb, eb);
+        }
   }
 }
 

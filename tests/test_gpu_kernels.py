@@ -271,7 +271,8 @@ def test_phase_shift3(llr):
         fftshift.phase_shift3(dims, g, d)
         r = x.copy()
         O.phase_shift3(r, dims, sign)
-        assert rel(g, r) < 1e-6
+        # f32 phases up to ~88 rad: device sincosf vs host cos/sin differ by a few ulp of the angle
+        assert rel(g, r) < 5e-6
 
 
 # ------------------------------------------------------------------ wavelet
