@@ -1,15 +1,29 @@
 // Multi-level 3-D Daubechies-2 wavelet soft threshold on gfx950 (the
 // wavelet_denoise step of lib.rs:430-465).
 //
-// Analysis along one axis (kernels.cu:155-212 formula): a[c] = sum_k lo_d[k]
-// x[sym(2c+1-k)], d[c] likewise with hi_d, symmetric (half-sample) extension
-// (kernels.cu:97-105), nc = (n+3)/2 coefficients per band (kernels.cu:150-152),
-// approximation in [0, nc) and detail in [nc, 2nc). Synthesis (kernels.cu:
-// 215-268 formula) keeps the central samples and trims to the finer level's
-// length, as pywt's waverecn does. Every volume of the batch is handled by one
-// launch per axis; the soft threshold (shrink_complex, lib.rs:460-465) is fused
-// into the last analysis axis: it applies to every detail coefficient and, at
-// the deepest level only, to the approximation.
+// Filters and boundary rules follow the reference's CUDA kernels: analysis
+// a[c] = sum_k lo_d[k] x[sym(2c+1-k)] with symmetric (half-sample) extension
+// (kernels.cu:97-105, 185-203), nc = (n+3)/2 coefficients per band
+// (kernels.cu:150-152), approximation in [0,nc) and detail in [nc,2nc);
+// synthesis keeps the samples from F-2 on (kernels.cu:230-261) and is trimmed
+// to the finer level's length (pywt waverecn). Every detail subband and the
+// deepest approximation are soft-thresholded (shrink_complex, lib.rs:460-465).
+//
+// One level takes three passes over its data instead of one pass per axis and
+// direction:
+//   1. k_wav_xy_an: an LDS tile (with its 2-sample halo) of one z-slice is
+//      filtered along x then y, giving the four xy-bands of the level, stored
+//      with z untouched (band buffer B, 2ncx x 2ncy x nz).
+//   2. k_wav_z: a tile of z-lines of B is analysed along z, shrunk and
+//      synthesised back in place while it sits in LDS. For the lines of the LL
+//      band the z-approximation (LLL) is the next level's input; it is written
+//      out and the line keeps only the synthesis of its shrunk detail, which
+//      is valid because synthesis is linear: syn(a', d') = syn(a', 0) + syn(0, d').
+//   3. k_wav_xy_syn: the xy-bands are synthesised along y then x from an LDS
+//      tile; the LL band picks up syn_z(a', 0) of the reconstructed deeper level
+//      while it is loaded.
+// Levels run coarse-to-fine on the way back; the deepest level handles its
+// LLL band completely in pass 2.
 #include <hip/hip_runtime.h>
 
 #include <array>
@@ -24,114 +38,6 @@ struct Db2F {
   float dlo[4], dhi[4], rlo[4], rhi[4];
 };
 
-struct DwtArgs {
-  const float2* in;
-  float2* out;
-  long long d[3];   // iteration box: analysis -> input dims with axis replaced by nc; synthesis -> output dims
-  long long ip[3];  // input pitches
-  long long ivp;    // input volume pitch
-  long long op[3];
-  long long ovp;
-  long long n;      // analysis: input length along axis; synthesis: 2*nc coefficients
-  long long nc;
-  int axis;
-  int nvols;
-  int shrink;       // analysis: apply soft threshold
-  int keep_approx;  // do not shrink the approximation sub-box (non-deepest level)
-  long long h[3];   // approximation sub-box dims (for keep_approx)
-  float alpha;
-  Db2F f;
-};
-
-__device__ __forceinline__ long long sym(long long i, long long n) {
-  // symmetric half-sample extension; |i| never exceeds n for 4-tap filters on n >= 2
-  if (i < 0) i = -i - 1;
-  if (i >= n) i = 2 * n - i - 1;
-  if (i < 0) i = 0;  // n == 1 edge
-  return i;
-}
-
-__device__ __forceinline__ float2 shrink(float2 x, float alpha) {
-  float m = sqrtf(x.x * x.x + x.y * x.y);
-  if (m == 0.f) return make_float2(0.f, 0.f);
-  float s = fmaxf(m - alpha, 0.f) / m;
-  return make_float2(x.x * s, x.y * s);
-}
-
-__global__ void __launch_bounds__(256) k_dwt_axis(DwtArgs a) {
-  long long total = a.d[0] * a.d[1] * a.d[2] * a.nvols;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
-    long long c0 = e % a.d[0], r = e / a.d[0];
-    long long c1 = r % a.d[1];
-    r /= a.d[1];
-    long long c2 = r % a.d[2], v = r / a.d[2];
-    long long co[3] = {c0, c1, c2};
-    long long c = co[a.axis];
-    long long ib = v * a.ivp;
-    long long ob = v * a.ovp;
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax)
-      if (ax != a.axis) {
-        ib += co[ax] * a.ip[ax];
-        ob += co[ax] * a.op[ax];
-      }
-    float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      long long j = sym(2 * c + 1 - k, a.n);
-      float2 x = a.in[ib + j * a.ip[a.axis]];
-      sa.x += a.f.dlo[k] * x.x;
-      sa.y += a.f.dlo[k] * x.y;
-      sd.x += a.f.dhi[k] * x.x;
-      sd.y += a.f.dhi[k] * x.y;
-    }
-    if (a.shrink) {
-      bool approx_box = true;
-#pragma unroll
-      for (int ax = 0; ax < 3; ++ax)
-        if (ax != a.axis && co[ax] >= a.h[ax]) approx_box = false;
-      if (!(a.keep_approx && approx_box)) sa = shrink(sa, a.alpha);
-      sd = shrink(sd, a.alpha);
-    }
-    a.out[ob + c * a.op[a.axis]] = sa;
-    a.out[ob + (c + a.nc) * a.op[a.axis]] = sd;
-  }
-}
-
-__global__ void __launch_bounds__(256) k_idwt_axis(DwtArgs a) {
-  long long total = a.d[0] * a.d[1] * a.d[2] * a.nvols;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
-    long long c0 = e % a.d[0], r = e / a.d[0];
-    long long c1 = r % a.d[1];
-    r /= a.d[1];
-    long long c2 = r % a.d[2], v = r / a.d[2];
-    long long co[3] = {c0, c1, c2};
-    long long s = co[a.axis];
-    long long ib = v * a.ivp;
-    long long ob = v * a.ovp;
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax)
-      if (ax != a.axis) {
-        ib += co[ax] * a.ip[ax];
-        ob += co[ax] * a.op[ax];
-      }
-    float2 acc = make_float2(0.f, 0.f);
-    long long cc = s + 2;  // keep-window start F-2
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      long long idx = cc - j;
-      if (idx >= 0 && idx < 2 * a.nc && (idx & 1) == 0) {
-        long long t = idx >> 1;
-        float2 av = a.in[ib + t * a.ip[a.axis]];
-        float2 dv = a.in[ib + (t + a.nc) * a.ip[a.axis]];
-        acc.x += a.f.rlo[j] * av.x + a.f.rhi[j] * dv.x;
-        acc.y += a.f.rlo[j] * av.y + a.f.rhi[j] * dv.y;
-      }
-    }
-    a.out[ob + s * a.op[a.axis]] = acc;
-  }
-}
-
 Db2F db2_filters() {
   long double r3 = std::sqrt(3.0L), dd = 4.0L * std::sqrt(2.0L);
   long double h[4] = {(1 + r3) / dd, (3 + r3) / dd, (3 - r3) / dd, (1 - r3) / dd};
@@ -143,6 +49,229 @@ Db2F db2_filters() {
   for (int k = 0; k < 4; ++k) f.rhi[k] = (k % 2 == 0) ? f.dlo[k] : -f.dlo[k];
   for (int k = 0; k < 4; ++k) f.dhi[k] = f.rhi[3 - k];
   return f;
+}
+
+// symmetric half-sample extension; always returns an index in [0, n)
+__device__ __forceinline__ int sym(int i, int n) {
+  if (i < 0) i = -i - 1;
+  if (i >= n) i = 2 * n - i - 1;
+  if (i < 0) i = 0;
+  return i;
+}
+
+__device__ __forceinline__ float2 shrink(float2 x, float alpha) {
+  float m = sqrtf(x.x * x.x + x.y * x.y);
+  if (m == 0.f) return make_float2(0.f, 0.f);
+  float s = fmaxf(m - alpha, 0.f) / m;
+  return make_float2(x.x * s, x.y * s);
+}
+
+__device__ __forceinline__ void fma2(float2& acc, float w, float2 v) {
+  acc.x += w * v.x;
+  acc.y += w * v.y;
+}
+
+constexpr int AN_TCX = 32, AN_TCY = 16;              // coefficient tile of pass 1
+constexpr int AN_W = 2 * AN_TCX + 2, AN_H = 2 * AN_TCY + 2;
+constexpr int SY_TX = 64, SY_TY = 16;                // sample tile of pass 3
+constexpr int SY_NX = SY_TX / 2 + 1, SY_NY = SY_TY / 2 + 1;
+constexpr int ZT = 16;                               // z-lines per workgroup in pass 2
+
+struct XYAnArgs {
+  const float2* in;
+  long long ip1, ip2, ivp;  // input pitches (y, z) and volume pitch; x pitch 1
+  float2* B;
+  long long bx, by, bvp;    // band buffer dims (2ncx, 2ncy) and volume pitch
+  int ax, ay, az, ncx, ncy, tiles_x;
+  Db2F f;
+};
+
+__global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
+  __shared__ float2 s_in[AN_H * AN_W];
+  __shared__ float2 s_x[AN_H * 2 * AN_TCX];
+  const int tid = threadIdx.x;
+  const int cx0 = (blockIdx.x % a.tiles_x) * AN_TCX, cy0 = (blockIdx.x / a.tiles_x) * AN_TCY;
+  const int z = blockIdx.y % a.az;
+  const long long v = blockIdx.y / a.az;
+  const float2* src = a.in + v * a.ivp + (long long)z * a.ip2;
+  for (int e = tid; e < AN_W * AN_H; e += 256) {
+    int i = e % AN_W, j = e / AN_W;
+    int xs = sym(2 * cx0 - 2 + i, a.ax), ys = sym(2 * cy0 - 2 + j, a.ay);
+    s_in[e] = src[(long long)ys * a.ip1 + xs];
+  }
+  __syncthreads();
+  // along x: local sample 2c+3-k <-> global 2(cx0+c)+1-k
+  for (int e = tid; e < AN_H * AN_TCX; e += 256) {
+    int c = e % AN_TCX, j = e / AN_TCX;
+    float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float2 x = s_in[j * AN_W + 2 * c + 3 - k];
+      fma2(sa, a.f.dlo[k], x);
+      fma2(sd, a.f.dhi[k], x);
+    }
+    s_x[j * 2 * AN_TCX + c] = sa;
+    s_x[j * 2 * AN_TCX + AN_TCX + c] = sd;
+  }
+  __syncthreads();
+  float2* dst = a.B + v * a.bvp + (long long)z * a.bx * a.by;
+  for (int e = tid; e < AN_TCY * 2 * AN_TCX; e += 256) {
+    int c2 = e % (2 * AN_TCX), cy = e / (2 * AN_TCX);
+    float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float2 x = s_x[(2 * cy + 3 - k) * 2 * AN_TCX + c2];
+      fma2(sa, a.f.dlo[k], x);
+      fma2(sd, a.f.dhi[k], x);
+    }
+    int cxl = c2 < AN_TCX ? c2 : c2 - AN_TCX;
+    if (cx0 + cxl < a.ncx && cy0 + cy < a.ncy) {
+      long long gx = (c2 < AN_TCX) ? cx0 + cxl : a.ncx + cx0 + cxl;
+      dst[(long long)(cy0 + cy) * a.bx + gx] = sa;
+      dst[(long long)(a.ncy + cy0 + cy) * a.bx + gx] = sd;
+    }
+  }
+}
+
+struct ZArgs {
+  float2* B;
+  long long bx, by, bvp;
+  int az, ncz, ncx, ncy, tiles_x;
+  float2* A;  // next-level input (ncx x ncy x ncz per volume) or null at the deepest level
+  long long avp;
+  int deepest;
+  float alpha;
+  Db2F f;
+};
+
+__global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
+  extern __shared__ float2 zs[];
+  float2* L = zs;                    // az x ZT
+  float2* Cf = zs + a.az * ZT;       // 2ncz x ZT
+  const int tid = threadIdx.x;
+  const int t = tid % ZT, r = tid / ZT;  // 16 lanes per line index group
+  const int X0 = (blockIdx.x % a.tiles_x) * ZT;
+  const int Y = blockIdx.x / a.tiles_x;
+  const long long v = blockIdx.y;
+  const int X = X0 + t;
+  const bool act = X < a.bx;
+  float2* base = a.B + v * a.bvp + (long long)Y * a.bx + X;
+  const long long zp = a.bx * a.by;
+  for (int z = r; z < a.az; z += 256 / ZT) L[z * ZT + t] = act ? base[z * zp] : make_float2(0.f, 0.f);
+  __syncthreads();
+  const bool ll = X < a.ncx && Y < a.ncy;
+  for (int c = r; c < a.ncz; c += 256 / ZT) {
+    float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float2 x = L[sym(2 * c + 1 - k, a.az) * ZT + t];
+      fma2(sa, a.f.dlo[k], x);
+      fma2(sd, a.f.dhi[k], x);
+    }
+    sd = shrink(sd, a.alpha);
+    if (!ll || a.deepest) {
+      sa = shrink(sa, a.alpha);
+    } else {
+      a.A[v * a.avp + (long long)c * a.ncx * a.ncy + (long long)Y * a.ncx + X] = sa;
+      sa = make_float2(0.f, 0.f);
+    }
+    Cf[c * ZT + t] = sa;
+    Cf[(a.ncz + c) * ZT + t] = sd;
+  }
+  __syncthreads();
+  for (int s = r; s < a.az; s += 256 / ZT) {
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int idx = s + 2 - j;
+      if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0) {
+        int tt = idx >> 1;
+        fma2(acc, a.f.rlo[j], Cf[tt * ZT + t]);
+        fma2(acc, a.f.rhi[j], Cf[(a.ncz + tt) * ZT + t]);
+      }
+    }
+    if (act) base[s * zp] = acc;
+  }
+}
+
+struct XYSynArgs {
+  const float2* B;
+  long long bx, by, bvp;
+  int ncx, ncy;
+  const float2* Ap;  // reconstructed next level (ncx x ncy x ncz) or null
+  long long apvp;
+  int ncz;           // this level's z coefficients per band (Ap's z length)
+  float2* out;
+  long long op1, op2, ovp;
+  int ax, ay, az, tiles_x;
+  Db2F f;
+};
+
+__global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
+  __shared__ float2 band[4 * SY_NY * SY_NX];   // [xb + 2*yb][iy][ix]
+  __shared__ float2 yrow[2 * SY_TY * SY_NX];   // [xb][sy][ix]
+  const int tid = threadIdx.x;
+  const int x0 = (blockIdx.x % a.tiles_x) * SY_TX, y0 = (blockIdx.x / a.tiles_x) * SY_TY;
+  const int z = blockIdx.y % a.az;
+  const long long v = blockIdx.y / a.az;
+  const int tx0 = x0 / 2, ty0 = y0 / 2;
+  const float2* src = a.B + v * a.bvp + (long long)z * a.bx * a.by;
+  for (int e = tid; e < 4 * SY_NY * SY_NX; e += 256) {
+    int ix = e % SY_NX, rem = e / SY_NX;
+    int iy = rem % SY_NY, b = rem / SY_NY;
+    int cx = tx0 + ix, cy = ty0 + iy;
+    float2 val = make_float2(0.f, 0.f);
+    if (cx < a.ncx && cy < a.ncy) {
+      long long gx = (b & 1) ? a.ncx + cx : cx;
+      long long gy = (b >> 1) ? a.ncy + cy : cy;
+      val = src[gy * a.bx + gx];
+      if (b == 0 && a.Ap) {
+        const float2* ap = a.Ap + v * a.apvp + (long long)cy * a.ncx + cx;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          int idx = z + 2 - j;
+          if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0)
+            fma2(val, a.f.rlo[j], ap[(long long)(idx >> 1) * a.ncx * a.ncy]);
+        }
+      }
+    }
+    band[e] = val;
+  }
+  __syncthreads();
+  // along y: yrow[xb][sy][ix] from the (xb, yb = 0 / 1) bands
+  for (int e = tid; e < 2 * SY_TY * SY_NX; e += 256) {
+    int ix = e % SY_NX, rem = e / SY_NX;
+    int sy = rem % SY_TY, xb = rem / SY_TY;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int idx = y0 + sy + 2 - j;
+      if (idx >= 0 && idx < 2 * a.ncy && (idx & 1) == 0) {
+        int t = (idx >> 1) - ty0;
+        fma2(acc, a.f.rlo[j], band[(xb * SY_NY + t) * SY_NX + ix]);
+        fma2(acc, a.f.rhi[j], band[((xb + 2) * SY_NY + t) * SY_NX + ix]);
+      }
+    }
+    yrow[e] = acc;
+  }
+  __syncthreads();
+  float2* dst = a.out + v * a.ovp + (long long)z * a.op2;
+  for (int e = tid; e < SY_TY * SY_TX; e += 256) {
+    int sx = e % SY_TX, sy = e / SY_TX;
+    int gx = x0 + sx, gy = y0 + sy;
+    if (gx >= a.ax || gy >= a.ay) continue;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int idx = gx + 2 - j;
+      if (idx >= 0 && idx < 2 * a.ncx && (idx & 1) == 0) {
+        int t = (idx >> 1) - tx0;
+        fma2(acc, a.f.rlo[j], yrow[sy * SY_NX + t]);
+        fma2(acc, a.f.rhi[j], yrow[(SY_TY + sy) * SY_NX + t]);
+      }
+    }
+    dst[(long long)gy * a.op1 + gx] = acc;
+  }
 }
 
 }  // namespace
@@ -172,176 +301,123 @@ void wavelet_geometry(const size_t* dims, int L, std::vector<std::array<size_t, 
   }
 }
 
-// Workspace elements needed per volume: coefficient boxes of every level plus
-// two axis temporaries sized for level 0.
+// Per-volume workspace: the band buffer of every level (2ncx x 2ncy x nz) and
+// the next-level inputs of levels 1..L-1 (ncx x ncy x ncz of the level above).
 size_t wavelet_workspace_elems(const size_t* dims, int L, size_t* coef_elems, size_t* tmp_elems) {
   std::vector<std::array<size_t, 3>> in_dims, cdims;
   wavelet_geometry(dims, L, in_dims, cdims);
   size_t c = 0;
-  for (int l = 0; l < L; ++l) c += cdims[l][0] * cdims[l][1] * cdims[l][2];
-  size_t t = L > 0 ? cdims[0][0] * cdims[0][1] * cdims[0][2] : 0;
+  for (int l = 0; l < L; ++l) {
+    c += cdims[l][0] * cdims[l][1] * in_dims[l][2];
+    if (l > 0) c += in_dims[l][0] * in_dims[l][1] * in_dims[l][2];
+  }
   *coef_elems = c;
-  *tmp_elems = t;
-  return c + 2 * t;
+  *tmp_elems = 0;
+  return c;
 }
 
-static hipError_t launch_dwt(DwtArgs& a, bool synth, hipStream_t s) {
-  long long total = a.d[0] * a.d[1] * a.d[2] * a.nvols;
-  unsigned grid = grid_for((size_t)total, 256, 1u << 20);
-  if (synth)
-    hipLaunchKernelGGL(k_idwt_axis, dim3(grid), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_dwt_axis, dim3(grid), dim3(256), 0, s, a);
-  return hipGetLastError();
-}
-
-// In-place wavelet soft threshold of nvols volumes. coef: workspace of
-// nvols * coef_elems, t1/t2: nvols * tmp_elems each.
+// In-place wavelet soft threshold of nvols volumes; coef holds nvols *
+// coef_elems elements (t1, t2 unused).
 llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, const size_t* dims, float alpha, int L,
                                   float2* coef, float2* t1, float2* t2) {
+  (void)t1;
+  (void)t2;
   Db2F f = db2_filters();
   size_t N = dims[0] * dims[1] * dims[2];
   if (L <= 0) return shrink_all_device(ctx, vols, N * nvols, alpha);  // the volume is its own approximation
   std::vector<std::array<size_t, 3>> in_dims, cdims;
   wavelet_geometry(dims, L, in_dims, cdims);
-  std::vector<float2*> cb(L);
-  std::vector<size_t> cvol(L);
+  std::vector<float2*> Bp(L), Ap(L, nullptr);
+  std::vector<size_t> bvol(L), avol(L, 0);
   {
     float2* p = coef;
     for (int l = 0; l < L; ++l) {
-      cvol[l] = cdims[l][0] * cdims[l][1] * cdims[l][2];
-      cb[l] = p;
-      p += cvol[l] * nvols;
+      bvol[l] = cdims[l][0] * cdims[l][1] * in_dims[l][2];
+      Bp[l] = p;
+      p += bvol[l] * nvols;
+    }
+    for (int l = 1; l < L; ++l) {
+      avol[l] = in_dims[l][0] * in_dims[l][1] * in_dims[l][2];
+      Ap[l] = p;
+      p += avol[l] * nvols;
     }
   }
-  size_t tvol = cvol[0];
   hipStream_t s = ctx->stream;
-  // ---------------- analysis
   for (int l = 0; l < L; ++l) {
-    const float2* src;
-    long long sp[3], svp;
-    if (l == 0) {
-      src = vols;
-      sp[0] = 1;
-      sp[1] = (long long)dims[0];
-      sp[2] = (long long)(dims[0] * dims[1]);
-      svp = (long long)N;
-    } else {
-      src = cb[l - 1];
-      sp[0] = 1;
-      sp[1] = (long long)cdims[l - 1][0];
-      sp[2] = (long long)(cdims[l - 1][0] * cdims[l - 1][1]);
-      svp = (long long)cvol[l - 1];
-    }
     const size_t* in = in_dims[l].data();
     const size_t* cd = cdims[l].data();
-    long long d1[3] = {(long long)cd[0], (long long)in[1], (long long)in[2]};
-    long long d2[3] = {(long long)cd[0], (long long)cd[1], (long long)in[2]};
-    long long p1[3] = {1, d1[0], d1[0] * d1[1]}, p2[3] = {1, d2[0], d2[0] * d2[1]};
-    long long pc[3] = {1, (long long)cd[0], (long long)(cd[0] * cd[1])};
-    for (int ax = 0; ax < 3; ++ax) {
-      DwtArgs a{};
-      a.f = f;
-      a.nvols = (int)nvols;
-      a.axis = ax;
-      a.n = (long long)in[ax];
-      a.nc = (long long)(cd[ax] / 2);
-      if (ax == 0) {
-        a.in = src;
-        for (int k = 0; k < 3; ++k) a.ip[k] = sp[k];
-        a.ivp = svp;
-        a.out = t1;
-        for (int k = 0; k < 3; ++k) a.op[k] = p1[k];
-        a.ovp = (long long)tvol;
-        a.d[0] = a.nc;
-        a.d[1] = (long long)in[1];
-        a.d[2] = (long long)in[2];
-      } else if (ax == 1) {
-        a.in = t1;
-        for (int k = 0; k < 3; ++k) a.ip[k] = p1[k];
-        a.ivp = (long long)tvol;
-        a.out = t2;
-        for (int k = 0; k < 3; ++k) a.op[k] = p2[k];
-        a.ovp = (long long)tvol;
-        a.d[0] = d1[0];
-        a.d[1] = a.nc;
-        a.d[2] = (long long)in[2];
-      } else {
-        a.in = t2;
-        for (int k = 0; k < 3; ++k) a.ip[k] = p2[k];
-        a.ivp = (long long)tvol;
-        a.out = cb[l];
-        for (int k = 0; k < 3; ++k) a.op[k] = pc[k];
-        a.ovp = (long long)cvol[l];
-        a.d[0] = d2[0];
-        a.d[1] = d2[1];
-        a.d[2] = a.nc;
-        a.shrink = 1;
-        a.keep_approx = (l != L - 1);
-        a.h[0] = (long long)cd[0] / 2;
-        a.h[1] = (long long)cd[1] / 2;
-        a.h[2] = (long long)cd[2] / 2;
-        a.alpha = alpha;
-      }
-      hipError_t e = launch_dwt(a, false, s);
-      if (e != hipSuccess) return hip_fail(e, "dwt analysis launch");
-    }
+    int ncx = (int)cd[0] / 2, ncy = (int)cd[1] / 2, ncz = (int)cd[2] / 2;
+    // pass 1
+    XYAnArgs a1{};
+    a1.in = l == 0 ? vols : Ap[l];
+    a1.ip1 = (long long)in[0];
+    a1.ip2 = (long long)(in[0] * in[1]);
+    a1.ivp = (long long)(in[0] * in[1] * in[2]);
+    a1.B = Bp[l];
+    a1.bx = cd[0];
+    a1.by = cd[1];
+    a1.bvp = (long long)bvol[l];
+    a1.ax = (int)in[0];
+    a1.ay = (int)in[1];
+    a1.az = (int)in[2];
+    a1.ncx = ncx;
+    a1.ncy = ncy;
+    a1.tiles_x = (ncx + AN_TCX - 1) / AN_TCX;
+    a1.f = f;
+    int tiles_y = (ncy + AN_TCY - 1) / AN_TCY;
+    dim3 g1((unsigned)(a1.tiles_x * tiles_y), (unsigned)(in[2] * nvols));
+    hipLaunchKernelGGL(k_wav_xy_an, g1, dim3(256), 0, s, a1);
+    LLR_HIP(hipGetLastError());
+    // pass 2
+    ZArgs a2{};
+    a2.B = Bp[l];
+    a2.bx = cd[0];
+    a2.by = cd[1];
+    a2.bvp = (long long)bvol[l];
+    a2.az = (int)in[2];
+    a2.ncz = ncz;
+    a2.ncx = ncx;
+    a2.ncy = ncy;
+    a2.tiles_x = (int)((cd[0] + ZT - 1) / ZT);
+    a2.deepest = l == L - 1;
+    a2.A = a2.deepest ? nullptr : Ap[l + 1];
+    a2.avp = a2.deepest ? 0 : (long long)avol[l + 1];
+    a2.alpha = alpha;
+    a2.f = f;
+    size_t smem = (size_t)(in[2] + 2 * ncz) * ZT * sizeof(float2);
+    if (smem > 160 * 1024) return fail(LLR_ERR_UNSUPPORTED, "z length too large for the wavelet z pass");
+    if (smem > 64 * 1024)
+      LLR_HIP(hipFuncSetAttribute((const void*)k_wav_z, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    dim3 g2((unsigned)(a2.tiles_x * cd[1]), (unsigned)nvols);
+    hipLaunchKernelGGL(k_wav_z, g2, dim3(256), smem, s, a2);
+    LLR_HIP(hipGetLastError());
   }
-  // ---------------- synthesis
   for (int l = L - 1; l >= 0; --l) {
-    const size_t* cd = cdims[l].data();
     const size_t* od = in_dims[l].data();
-    long long pc[3] = {1, (long long)cd[0], (long long)(cd[0] * cd[1])};
-    long long d2[3] = {(long long)cd[0], (long long)cd[1], (long long)od[2]};
-    long long d1[3] = {(long long)cd[0], (long long)od[1], (long long)od[2]};
-    long long p2[3] = {1, d2[0], d2[0] * d2[1]}, p1[3] = {1, d1[0], d1[0] * d1[1]};
-    for (int ax = 2; ax >= 0; --ax) {
-      DwtArgs a{};
-      a.f = f;
-      a.nvols = (int)nvols;
-      a.axis = ax;
-      a.nc = (long long)(cd[ax] / 2);
-      a.n = 2 * a.nc;
-      if (ax == 2) {
-        a.in = cb[l];
-        for (int k = 0; k < 3; ++k) a.ip[k] = pc[k];
-        a.ivp = (long long)cvol[l];
-        a.out = t2;
-        for (int k = 0; k < 3; ++k) a.op[k] = p2[k];
-        a.ovp = (long long)tvol;
-        for (int k = 0; k < 3; ++k) a.d[k] = d2[k];
-      } else if (ax == 1) {
-        a.in = t2;
-        for (int k = 0; k < 3; ++k) a.ip[k] = p2[k];
-        a.ivp = (long long)tvol;
-        a.out = t1;
-        for (int k = 0; k < 3; ++k) a.op[k] = p1[k];
-        a.ovp = (long long)tvol;
-        for (int k = 0; k < 3; ++k) a.d[k] = d1[k];
-      } else {
-        a.in = t1;
-        for (int k = 0; k < 3; ++k) a.ip[k] = p1[k];
-        a.ivp = (long long)tvol;
-        if (l == 0) {
-          a.out = vols;
-          a.op[0] = 1;
-          a.op[1] = (long long)dims[0];
-          a.op[2] = (long long)(dims[0] * dims[1]);
-          a.ovp = (long long)N;
-        } else {
-          a.out = cb[l - 1];
-          a.op[0] = 1;
-          a.op[1] = (long long)cdims[l - 1][0];
-          a.op[2] = (long long)(cdims[l - 1][0] * cdims[l - 1][1]);
-          a.ovp = (long long)cvol[l - 1];
-        }
-        a.d[0] = (long long)od[0];
-        a.d[1] = (long long)od[1];
-        a.d[2] = (long long)od[2];
-      }
-      hipError_t e = launch_dwt(a, true, s);
-      if (e != hipSuccess) return hip_fail(e, "dwt synthesis launch");
-    }
+    const size_t* cd = cdims[l].data();
+    XYSynArgs a3{};
+    a3.B = Bp[l];
+    a3.bx = cd[0];
+    a3.by = cd[1];
+    a3.bvp = (long long)bvol[l];
+    a3.ncx = (int)cd[0] / 2;
+    a3.ncy = (int)cd[1] / 2;
+    a3.ncz = (int)cd[2] / 2;
+    a3.Ap = (l < L - 1) ? Ap[l + 1] : nullptr;
+    a3.apvp = (l < L - 1) ? (long long)avol[l + 1] : 0;
+    a3.out = l == 0 ? vols : Ap[l];
+    a3.op1 = (long long)od[0];
+    a3.op2 = (long long)(od[0] * od[1]);
+    a3.ovp = (long long)(od[0] * od[1] * od[2]);
+    a3.ax = (int)od[0];
+    a3.ay = (int)od[1];
+    a3.az = (int)od[2];
+    a3.tiles_x = (int)((od[0] + SY_TX - 1) / SY_TX);
+    a3.f = f;
+    int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
+    dim3 g3((unsigned)(a3.tiles_x * tiles_y), (unsigned)(od[2] * nvols));
+    hipLaunchKernelGGL(k_wav_xy_syn, g3, dim3(256), 0, s, a3);
+    LLR_HIP(hipGetLastError());
   }
   return LLR_OK;
 }
