@@ -134,9 +134,9 @@ __device__ __forceinline__ void dft(float2* v, bool inv) {
 struct PassArgs {
   const float2* src;
   float2* dst;
-  long long stride;   // element stride along the transformed axis
+  int stride;         // element stride along the transformed axis
   int U;              // number of lines along the tiled (neighbour) dimension
-  long long ustride;  // element stride between neighbouring lines
+  int ustride;        // element stride between neighbouring lines
   long long wstride;  // element stride of the grid's y dimension
   long long vstride;  // element stride between volumes
   const float2* pre;  // [v][N] input ramp table or null
@@ -144,7 +144,8 @@ struct PassArgs {
   int pre_conj, post_conj;
   float scale;
   const float2* tw;   // exp(-2*pi*i*m/N)
-  int inverse;
+  int tiles;          // tiles of T lines along U
+  int tiles_per_wg;
 };
 
 template <int R1, int R2>
@@ -157,67 +158,99 @@ struct Layout {
   static constexpr int THREADS = T * P;
 };
 
-__device__ __forceinline__ float2 ld_tab(const float2* tab, long long v, int n, int i, int conj) {
-  float2 p = tab[v * n + i];
-  if (conj) p.y = -p.y;
-  return p;
-}
-
-// One axis pass over all lines of all volumes.
-template <int R1, int R2, bool TFAST>
+// One axis pass. Each workgroup owns `tiles_per_wg` consecutive tiles of T
+// lines of one (w, volume) row: the twiddles and ramp tables (with the
+// direction, conjugation and 1/sqrt(n) folded in) are staged in LDS once, and
+// the next tile's loads are issued before the current tile's second stage so
+// they overlap its compute and stores. Offsets inside a volume are 32-bit.
+template <int R1, int R2, bool TFAST, bool INV, bool RAMPS>
 __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_pass(PassArgs a) {
   using L = Layout<R1, R2>;
-  constexpr int N = L::N, P = L::P, T = L::T;
+  constexpr int N = L::N, P = L::P, T = L::T, TH = L::THREADS;
   __shared__ float2 lds[T * L::LS];
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_pre[RAMPS ? N : 1];
+  __shared__ float2 s_post[RAMPS ? N : 1];
   const int tid = threadIdx.x;
   const int t = TFAST ? (tid % T) : (tid / P);
-  const int j = TFAST ? (tid / T) : (tid % P);  // lane within the line
-  const int u = blockIdx.x * T + t;
+  const int j = TFAST ? (tid / T) : (tid % P);
   const long long v = blockIdx.z;
-  const bool active = u < a.U;
-  const long long base = v * a.vstride + (long long)blockIdx.y * a.wstride + (long long)u * a.ustride;
-  const bool inv = a.inverse != 0;
-
-  // stage 1: lane j2 = j loads x[j1*R2 + j2], R1-point DFT over j1
-  if (active && j < R2) {
-    float2 x[R1];
-#pragma unroll
-    for (int j1 = 0; j1 < R1; ++j1) {
-      int idx = j1 * R2 + j;
-      float2 val = a.src[base + (long long)idx * a.stride];
-      if (a.pre) val = cmul(val, ld_tab(a.pre, v, N, idx, a.pre_conj));
-      x[j1] = val;
-    }
-    dft<R1>(x, inv);
-#pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) {
-      float2 w = a.tw[j * k1];
-      if (inv) w.y = -w.y;
-      lds[t * L::LS + k1 * L::R2P + j] = cmul(x[k1], w);
+  for (int i = tid; i < N; i += TH) {
+    float2 w = a.tw[i];
+    s_tw[i] = INV ? cconj(w) : w;
+    if constexpr (RAMPS) {
+      float2 p = a.pre[v * N + i];
+      s_pre[i] = a.pre_conj ? cconj(p) : p;
+      float2 q = a.post[v * N + i];
+      s_post[i] = cscale(a.post_conj ? cconj(q) : q, a.scale);
     }
   }
-  __syncthreads();
-  // stage 2: lane k1 = j reads the R2 values, R2-point DFT, stores X[k1 + R1*k2]
-  if (active && j < R1) {
-    float2 y[R2];
+  const long long row = v * a.vstride + (long long)blockIdx.y * a.wstride;
+  const float2* __restrict__ src = a.src + row;
+  float2* __restrict__ dst = a.dst + row;
+  const int tile0 = blockIdx.x * a.tiles_per_wg;
+  const int ntiles = min(a.tiles_per_wg, a.tiles - tile0);
+  float2 xn[R1];
+  auto load = [&](int tile, float2* x) {
+    int u = tile * T + t;
+    if (u < a.U && j < R2) {
+      const float2* p = src + (unsigned)(u * a.ustride + j * a.stride);
+      unsigned step = (unsigned)(R2 * a.stride);
 #pragma unroll
-    for (int j2 = 0; j2 < R2; ++j2) y[j2] = lds[t * L::LS + j * L::R2P + j2];
-    dft<R2>(y, inv);
-#pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) {
-      int k = j + R1 * k2;
-      float2 val = cscale(y[k2], a.scale);
-      if (a.post) val = cmul(val, ld_tab(a.post, v, N, k, a.post_conj));
-      a.dst[base + (long long)k * a.stride] = val;
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = p[j1 * step];
     }
+  };
+  load(tile0, xn);
+  __syncthreads();  // tables ready
+  for (int it = 0; it < ntiles; ++it) {
+    const int u = (tile0 + it) * T + t;
+    const bool active = u < a.U;
+    if (active && j < R2) {
+      float2 x[R1];
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = RAMPS ? cmul(xn[j1], s_pre[j1 * R2 + j]) : xn[j1];
+      if (INV) {
+#pragma unroll
+        for (int q = 0; q < R1; ++q) x[q].y = -x[q].y;
+      }
+      SmallDFT<R1>::run(x);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        float2 y = x[k1];
+        if (INV) y.y = -y.y;
+        lds[t * L::LS + k1 * L::R2P + j] = cmul(y, s_tw[j * k1]);
+      }
+    }
+    __syncthreads();
+    if (it + 1 < ntiles) load(tile0 + it + 1, xn);
+    if (active && j < R1) {
+      float2 y[R2];
+#pragma unroll
+      for (int j2 = 0; j2 < R2; ++j2) {
+        y[j2] = lds[t * L::LS + j * L::R2P + j2];
+        if (INV) y[j2].y = -y[j2].y;
+      }
+      SmallDFT<R2>::run(y);
+      float2* p = dst + (unsigned)(u * a.ustride + j * a.stride);
+      unsigned step = (unsigned)(R1 * a.stride);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        float2 val = y[k2];
+        if (INV) val.y = -val.y;
+        val = RAMPS ? cmul(val, s_post[j + R1 * k2]) : cscale(val, a.scale);
+        p[k2 * step] = val;
+      }
+    }
+    __syncthreads();
   }
 }
 
 struct ZProjArgs {
   float2* data;        // image-space iterate after the forward x and y passes
   const float2* meas;  // measurements (k-space), same layout
-  long long stride, ustride, wstride, vstride;
-  int U;
+  int stride, ustride;
+  long long wstride, vstride;
+  int U, tiles, tiles_per_wg;
   const float2* pk;    // [v][nz] exp(+i phi_ksp,z)
   const float2* pi;    // [v][nz] exp(+i phi_img,z)
   float scale;         // 1/sqrt(nz)
@@ -225,80 +258,105 @@ struct ZProjArgs {
   double* partial;     // one partial sum of |x - y|^2 per workgroup
 };
 
-// Forward z pass -> hard projection -> adjoint z pass on one tile of z lines.
+// Forward z pass -> hard projection -> adjoint z pass on tiles of z lines.
 // The forward output ramp conj(pi) and the adjoint input ramp pi cancel on
 // unsampled entries, so the projection compares x*scale with y*pi and, where
 // sampled, substitutes y*pi (|pi| = 1, so |x*conj(pi) - y| = |x - y*pi|).
 template <int R1, int R2>
 __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_z_project(ZProjArgs a) {
   using L = Layout<R1, R2>;
-  constexpr int N = L::N, T = L::T;
+  constexpr int N = L::N, T = L::T, TH = L::THREADS;
   __shared__ float2 lds[T * L::LS];
-  __shared__ double red[L::THREADS / 64];
+  __shared__ float2 s_tw[N], s_pk[N], s_pi[N];
+  __shared__ double red[TH / 64];
   const int tid = threadIdx.x;
   const int t = tid % T;
   const int j = tid / T;
-  const int u = blockIdx.x * T + t;
   const long long v = blockIdx.z;
-  const bool active = u < a.U;
-  const long long base = v * a.vstride + (long long)blockIdx.y * a.wstride + (long long)u * a.ustride;
+  for (int i = tid; i < N; i += TH) {
+    s_tw[i] = a.tw[i];
+    s_pk[i] = a.pk[v * N + i];
+    s_pi[i] = a.pi[v * N + i];
+  }
+  const long long row = v * a.vstride + (long long)blockIdx.y * a.wstride;
+  float2* __restrict__ data = a.data + row;
+  const float2* __restrict__ meas = a.meas + row;
+  const int tile0 = blockIdx.x * a.tiles_per_wg;
+  const int ntiles = min(a.tiles_per_wg, a.tiles - tile0);
   double err = 0.0;
-
-  if (active && j < R2) {
-    float2 x[R1];
+  float2 xn[R1];
+  auto load = [&](int tile, float2* x) {
+    int u = tile * T + t;
+    if (u < a.U && j < R2) {
+      const float2* p = data + (unsigned)(u * a.ustride + j * a.stride);
+      unsigned step = (unsigned)(R2 * a.stride);
 #pragma unroll
-    for (int j1 = 0; j1 < R1; ++j1) {
-      int idx = j1 * R2 + j;
-      x[j1] = cmul(a.data[base + (long long)idx * a.stride], a.pk[v * N + idx]);
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = p[j1 * step];
     }
-    dft<R1>(x, false);
-#pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) lds[t * L::LS + k1 * L::R2P + j] = cmul(x[k1], a.tw[j * k1]);
-  }
+  };
+  load(tile0, xn);
   __syncthreads();
-  float2 y[R2];
-  if (active && j < R1) {
+  for (int it = 0; it < ntiles; ++it) {
+    const int u = (tile0 + it) * T + t;
+    const bool active = u < a.U;
+    if (active && j < R2) {
+      float2 x[R1];
 #pragma unroll
-    for (int j2 = 0; j2 < R2; ++j2) y[j2] = lds[t * L::LS + j * L::R2P + j2];
-    dft<R2>(y, false);
-    // k-space sample k = j + R1*k2: project, then start the adjoint (R2-point
-    // inverse DFT over k2 for the fixed k1 = j)
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = cmul(xn[j1], s_pk[j1 * R2 + j]);
+      SmallDFT<R1>::run(x);
 #pragma unroll
-    for (int k2 = 0; k2 < R2; ++k2) {
-      int k = j + R1 * k2;
-      float2 xv = cscale(y[k2], a.scale);
-      float2 m = a.meas[base + (long long)k * a.stride];
-      if (m.x != 0.0f || m.y != 0.0f) {
-        float2 mp = cmul(m, a.pi[v * N + k]);
-        float2 d = csub(xv, mp);
-        err += (double)cnorm(d);
-        xv = mp;
+      for (int k1 = 0; k1 < R1; ++k1) lds[t * L::LS + k1 * L::R2P + j] = cmul(x[k1], s_tw[j * k1]);
+    }
+    __syncthreads();
+    if (it + 1 < ntiles) load(tile0 + it + 1, xn);
+    float2 y[R2];
+    if (active && j < R1) {
+#pragma unroll
+      for (int j2 = 0; j2 < R2; ++j2) y[j2] = lds[t * L::LS + j * L::R2P + j2];
+      SmallDFT<R2>::run(y);
+      const float2* mp = meas + (unsigned)(u * a.ustride + j * a.stride);
+      unsigned step = (unsigned)(R1 * a.stride);
+      // k-space sample k = j + R1*k2: project, then start the adjoint
+      // (R2-point inverse DFT over k2 for the fixed k1 = j)
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        float2 xv = cscale(y[k2], a.scale);
+        float2 m = mp[k2 * step];
+        if (m.x != 0.0f || m.y != 0.0f) {
+          float2 mpv = cmul(m, s_pi[j + R1 * k2]);
+          float2 d = csub(xv, mpv);
+          err += (double)cnorm(d);
+          xv = mpv;
+        }
+        y[k2] = make_float2(xv.x, -xv.y);  // conjugated: inverse DFT via the forward kernel
       }
-      y[k2] = xv;
+      SmallDFT<R2>::run(y);
     }
-    dft<R2>(y, true);
-  }
-  __syncthreads();  // every stage-2 read of lds is done
-  if (active && j < R1) {
+    __syncthreads();  // every stage-2 read of lds is done
+    if (active && j < R1) {
 #pragma unroll
-    for (int j2 = 0; j2 < R2; ++j2) {
-      float2 w = a.tw[j * j2];
-      w.y = -w.y;
-      lds[t * L::LS + j * L::R2P + j2] = cmul(y[j2], w);
+      for (int j2 = 0; j2 < R2; ++j2)
+        lds[t * L::LS + j * L::R2P + j2] = cmul(make_float2(y[j2].x, -y[j2].y), cconj(s_tw[j * j2]));
     }
-  }
-  __syncthreads();
-  if (active && j < R2) {
-    float2 x[R1];
+    __syncthreads();
+    if (active && j < R2) {
+      float2 x[R1];
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) x[k1] = lds[t * L::LS + k1 * L::R2P + j];
-    dft<R1>(x, true);
+      for (int k1 = 0; k1 < R1; ++k1) {
+        float2 q = lds[t * L::LS + k1 * L::R2P + j];
+        x[k1] = make_float2(q.x, -q.y);
+      }
+      SmallDFT<R1>::run(x);
+      float2* p = data + (unsigned)(u * a.ustride + j * a.stride);
+      unsigned step = (unsigned)(R2 * a.stride);
 #pragma unroll
-    for (int j1 = 0; j1 < R1; ++j1) {
-      int idx = j + R2 * j1;
-      float2 pk = a.pk[v * N + idx];
-      a.data[base + (long long)idx * a.stride] = cscale(cmulc(x[j1], pk), a.scale);
+      for (int j1 = 0; j1 < R1; ++j1) {
+        int idx = j + R2 * j1;
+        float2 r = make_float2(x[j1].x, -x[j1].y);
+        p[j1 * step] = cscale(cmulc(r, s_pk[idx]), a.scale);
+      }
     }
+    __syncthreads();
   }
   // workgroup reduction of err (fixed order: deterministic)
 #pragma unroll
@@ -307,27 +365,38 @@ __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_z_project(ZPr
   __syncthreads();
   if (tid == 0) {
     double s = 0.0;
-    for (int w = 0; w < L::THREADS / 64; ++w) s += red[w];
+    for (int w = 0; w < TH / 64; ++w) s += red[w];
     size_t bid = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     a.partial[bid] = s;
   }
 }
 
 template <int R1, int R2>
-hipError_t launch_pass(const PassArgs& a, long long W, long long nvols, bool tfast, hipStream_t s) {
+hipError_t launch_pass(PassArgs a, long long W, long long nvols, bool tfast, bool inv, hipStream_t s) {
   using L = Layout<R1, R2>;
-  dim3 grid((a.U + L::T - 1) / L::T, (unsigned)W, (unsigned)nvols);
-  if (tfast)
-    hipLaunchKernelGGL((k_fft_pass<R1, R2, true>), grid, dim3(L::THREADS), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_fft_pass<R1, R2, false>), grid, dim3(L::THREADS), 0, s, a);
+  a.tiles = (a.U + L::T - 1) / L::T;
+  a.tiles_per_wg = a.tiles >= 4 ? 4 : a.tiles;
+  dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
+  const bool ramps = a.pre != nullptr;
+#define FFT_LAUNCH(TF, IV, RP) \
+  hipLaunchKernelGGL((k_fft_pass<R1, R2, TF, IV, RP>), grid, dim3(L::THREADS), 0, s, a)
+  if (tfast) {
+    if (inv) { if (ramps) FFT_LAUNCH(true, true, true); else FFT_LAUNCH(true, true, false); }
+    else { if (ramps) FFT_LAUNCH(true, false, true); else FFT_LAUNCH(true, false, false); }
+  } else {
+    if (inv) { if (ramps) FFT_LAUNCH(false, true, true); else FFT_LAUNCH(false, true, false); }
+    else { if (ramps) FFT_LAUNCH(false, false, true); else FFT_LAUNCH(false, false, false); }
+  }
+#undef FFT_LAUNCH
   return hipGetLastError();
 }
 
 template <int R1, int R2>
-hipError_t launch_zproj(const ZProjArgs& a, long long W, long long nvols, hipStream_t s, size_t* nparts) {
+hipError_t launch_zproj(ZProjArgs a, long long W, long long nvols, hipStream_t s, size_t* nparts) {
   using L = Layout<R1, R2>;
-  dim3 grid((a.U + L::T - 1) / L::T, (unsigned)W, (unsigned)nvols);
+  a.tiles = (a.U + L::T - 1) / L::T;
+  a.tiles_per_wg = a.tiles >= 4 ? 4 : a.tiles;
+  dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
   *nparts = (size_t)grid.x * grid.y * grid.z;
   if (a.partial == nullptr) return hipSuccess;  // size query
   hipLaunchKernelGGL((k_fft_z_project<R1, R2>), grid, dim3(L::THREADS), 0, s, a);
@@ -339,39 +408,40 @@ hipError_t launch_zproj(const ZProjArgs& a, long long W, long long nvols, hipStr
 namespace llr {
 
 bool fft_size_supported(size_t n) {
-  return n == 1 || n == 2 || n == 4 || n == 8 || n == 16 || n == 32 || n == 64 || n == 128 || n == 256 ||
-         n == 512;
+  return n == 2 || n == 4 || n == 8 || n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
 }
 
 // Dispatch an axis pass for line length n.
-static hipError_t dispatch_pass(int n, const PassArgs& a, long long W, long long nv, bool tfast, hipStream_t s) {
+static hipError_t dispatch_pass(int n, const PassArgs& a, long long W, long long nv, bool tfast, bool inv,
+                                hipStream_t s) {
   switch (n) {
-    case 2: return launch_pass<2, 1>(a, W, nv, tfast, s);
-    case 4: return launch_pass<2, 2>(a, W, nv, tfast, s);
-    case 8: return launch_pass<4, 2>(a, W, nv, tfast, s);
-    case 16: return launch_pass<4, 4>(a, W, nv, tfast, s);
-    case 32: return launch_pass<8, 4>(a, W, nv, tfast, s);
-    case 64: return launch_pass<8, 8>(a, W, nv, tfast, s);
-    case 128: return launch_pass<16, 8>(a, W, nv, tfast, s);
-    case 256: return launch_pass<16, 16>(a, W, nv, tfast, s);
-    case 512: return launch_pass<32, 16>(a, W, nv, tfast, s);
+    case 2: return launch_pass<2, 1>(a, W, nv, tfast, inv, s);
+    case 4: return launch_pass<2, 2>(a, W, nv, tfast, inv, s);
+    case 8: return launch_pass<4, 2>(a, W, nv, tfast, inv, s);
+    case 16: return launch_pass<4, 4>(a, W, nv, tfast, inv, s);
+    case 32: return launch_pass<8, 4>(a, W, nv, tfast, inv, s);
+    case 64: return launch_pass<8, 8>(a, W, nv, tfast, inv, s);
+    case 128: return launch_pass<16, 8>(a, W, nv, tfast, inv, s);
+    case 256: return launch_pass<16, 16>(a, W, nv, tfast, inv, s);
+    case 512: return launch_pass<32, 16>(a, W, nv, tfast, inv, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 // One pass along `axis` of nvols volumes of size dims; src may equal dst.
-llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const size_t* dims, size_t nvols,
-                         int axis, bool inverse, const float2* pre, bool pre_conj, const float2* post,
-                         bool post_conj, float scale) {
+// pre and post must both be given (ramped signal-model pass) or both be null.
+llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const size_t* dims, size_t nvols, int axis,
+                         bool inverse, const float2* pre, bool pre_conj, const float2* post, bool post_conj,
+                         float scale) {
   int n = (int)dims[axis];
-  if (n == 1 && !pre && !post && scale == 1.0f && src == dst) return LLR_OK;
-  if (!fft_size_supported((size_t)n) && n != 1)
+  if (!fft_size_supported((size_t)n))
     return fail(LLR_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " is not a supported power of two (2..512)");
+  if ((pre == nullptr) != (post == nullptr)) return fail(LLR_ERR_ARG, "ramp tables must be given together");
+  if ((double)dims[0] * dims[1] * dims[2] >= 4294967296.0)
+    return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
   const float2* tw = nullptr;
-  if (n > 1) {
-    llr_status st = get_twiddles(ctx, n, &tw);
-    if (st) return st;
-  }
+  llr_status st = get_twiddles(ctx, n, &tw);
+  if (st) return st;
   long long nx = dims[0], ny = dims[1], nz = dims[2];
   PassArgs a{};
   a.src = src;
@@ -383,44 +453,38 @@ llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const siz
   a.post_conj = post_conj;
   a.scale = scale;
   a.tw = tw;
-  a.inverse = inverse;
   long long W;
   bool tfast;
   if (axis == 0) {
     a.stride = 1;
     a.U = (int)ny;
-    a.ustride = nx;
+    a.ustride = (int)nx;
     W = nz;
     a.wstride = nx * ny;
     tfast = false;
   } else if (axis == 1) {
-    a.stride = nx;
+    a.stride = (int)nx;
     a.U = (int)nx;
     a.ustride = 1;
     W = nz;
     a.wstride = nx * ny;
     tfast = true;
   } else {
-    a.stride = nx * ny;
+    a.stride = (int)(nx * ny);
     a.U = (int)nx;
     a.ustride = 1;
     W = ny;
     a.wstride = nx;
     tfast = true;
   }
-  if (n == 1) {
-    // a length-1 "transform" is the ramp/scale only: run it as a 2-point-free
-    // elementwise pass through the x-pass kernel of length 1 is not provided;
-    // handled by the caller (dims of 1 only occur in tests with pre/post null).
-    return fail(LLR_ERR_UNSUPPORTED, "length-1 axis with phase ramp");
-  }
-  hipError_t e = dispatch_pass(n, a, W, (long long)nvols, tfast, ctx->stream);
+  hipError_t e = dispatch_pass(n, a, W, (long long)nvols, tfast, inverse, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "fft axis pass launch");
   return LLR_OK;
 }
 
 // Fused forward-z / hard projection / adjoint-z. Writes one partial error sum
-// per workgroup into `partial` (size from fft_zproj_parts).
+// per workgroup into `partial` (size returned in nparts; partial == null only
+// queries the size).
 llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_t* dims, size_t nvols,
                      const float2* pk, const float2* pi, double* partial, size_t* nparts) {
   int n = (int)dims[2];
@@ -433,7 +497,7 @@ llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_
   ZProjArgs a{};
   a.data = data;
   a.meas = meas;
-  a.stride = nx * ny;
+  a.stride = (int)(nx * ny);
   a.U = (int)nx;
   a.ustride = 1;
   a.wstride = nx;
