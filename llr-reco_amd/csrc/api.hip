@@ -77,7 +77,7 @@ void ramp_table(std::vector<float2>& out, size_t n, const float* corr, size_t st
 
 // ------------------------------------------------------------- helpers
 static llr_status upload(DevBuf& b, const void* h, size_t bytes) {
-  LLR_HIP(b.alloc(bytes));
+  if (!(b.p && b.bytes == bytes)) LLR_HIP(b.alloc(bytes));  // reuse a buffer of the same size
   if (h && bytes) LLR_HIP(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
   return LLR_OK;
 }
@@ -601,9 +601,17 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
       st = project_device(ctx, r->X.as<float2>(), r->Cb.as<float2>(), r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank);
     }
     if (st) return st;
-    StageTimer t(r, 0);
-    return llr_block_run(ctx, r->Cb.as<float2>(), r->X.as<float2>(), r->S.as<float2>(), (int)r->nq, p.vol, p.blk, shift,
-                         r->nc, 0, 0, nb, mode, (int)p.rank, lambda, smax, nullptr);
+    {
+      // low-rank coefficients replace C in place; X = L S^H is formed by a
+      // streaming pass in voxel order (blocks' voxels only)
+      StageTimer t(r, 0);
+      st = llr_block_run(ctx, r->Cb.as<float2>(), r->Cb.as<float2>(), nullptr, 0, p.vol, p.blk, shift, r->nc, 0, 0, nb,
+                         mode, (int)p.rank, lambda, smax, nullptr);
+    }
+    if (st || mode == 2) return st;
+    StageTimer t(r, 1);
+    return backproject_device(ctx, r->Cb.as<float2>(), r->X.as<float2>(), r->S.as<float2>(), p.vol, p.blk, shift,
+                              (int)r->nq, (int)p.srank);
   }
   StageTimer t(r, 0);
   return llr_block_run(ctx, r->X.as<float2>(), r->X.as<float2>(), nullptr, 0, p.vol, p.blk, shift, r->nc, 0, 0, nb, mode,
