@@ -77,58 +77,83 @@ constexpr int SY_TX = 64, SY_TY = 16;                // sample tile of pass 3
 constexpr int SY_NX = SY_TX / 2 + 1, SY_NY = SY_TY / 2 + 1;
 constexpr int ZT = 16;                               // z-lines per workgroup in pass 2
 
+constexpr int TPW = 4;  // tiles per workgroup (the next tile is prefetched into registers)
+
 struct XYAnArgs {
   const float2* in;
   long long ip1, ip2, ivp;  // input pitches (y, z) and volume pitch; x pitch 1
   float2* B;
   long long bx, by, bvp;    // band buffer dims (2ncx, 2ncy) and volume pitch
-  int ax, ay, az, ncx, ncy, tiles_x;
+  int ax, ay, az, ncx, ncy, tiles_x, groups_x;
   Db2F f;
 };
 
+constexpr int AN_NLD = (AN_W * AN_H + 255) / 256;
+
+// grid.x = groups_x * tiles_y (a group is TPW consecutive x tiles of one tile row),
+// grid.y = z * volumes
 __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
   __shared__ float2 s_in[AN_H * AN_W];
   __shared__ float2 s_x[AN_H * 2 * AN_TCX];
   const int tid = threadIdx.x;
-  const int cx0 = (blockIdx.x % a.tiles_x) * AN_TCX, cy0 = (blockIdx.x / a.tiles_x) * AN_TCY;
+  const int gx = blockIdx.x % a.groups_x;
+  const int cy0 = (blockIdx.x / a.groups_x) * AN_TCY;
   const int z = blockIdx.y % a.az;
   const long long v = blockIdx.y / a.az;
   const float2* src = a.in + v * a.ivp + (long long)z * a.ip2;
-  for (int e = tid; e < AN_W * AN_H; e += 256) {
-    int i = e % AN_W, j = e / AN_W;
-    int xs = sym(2 * cx0 - 2 + i, a.ax), ys = sym(2 * cy0 - 2 + j, a.ay);
-    s_in[e] = src[(long long)ys * a.ip1 + xs];
-  }
-  __syncthreads();
-  // along x: local sample 2c+3-k <-> global 2(cx0+c)+1-k
-  for (int e = tid; e < AN_H * AN_TCX; e += 256) {
-    int c = e % AN_TCX, j = e / AN_TCX;
-    float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      float2 x = s_in[j * AN_W + 2 * c + 3 - k];
-      fma2(sa, a.f.dlo[k], x);
-      fma2(sd, a.f.dhi[k], x);
-    }
-    s_x[j * 2 * AN_TCX + c] = sa;
-    s_x[j * 2 * AN_TCX + AN_TCX + c] = sd;
-  }
-  __syncthreads();
   float2* dst = a.B + v * a.bvp + (long long)z * a.bx * a.by;
-  for (int e = tid; e < AN_TCY * 2 * AN_TCX; e += 256) {
-    int c2 = e % (2 * AN_TCX), cy = e / (2 * AN_TCX);
-    float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+  const int t0 = gx * TPW;
+  const int nt = min(TPW, a.tiles_x - t0);
+  float2 xn[AN_NLD];
+  auto load = [&](int cx0) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      float2 x = s_x[(2 * cy + 3 - k) * 2 * AN_TCX + c2];
-      fma2(sa, a.f.dlo[k], x);
-      fma2(sd, a.f.dhi[k], x);
+    for (int q = 0; q < AN_NLD; ++q) {
+      int e = tid + 256 * q;
+      if (e < AN_W * AN_H) {
+        int i = e % AN_W, j = e / AN_W;
+        xn[q] = src[(long long)sym(2 * cy0 - 2 + j, a.ay) * a.ip1 + sym(2 * cx0 - 2 + i, a.ax)];
+      }
     }
-    int cxl = c2 < AN_TCX ? c2 : c2 - AN_TCX;
-    if (cx0 + cxl < a.ncx && cy0 + cy < a.ncy) {
-      long long gx = (c2 < AN_TCX) ? cx0 + cxl : a.ncx + cx0 + cxl;
-      dst[(long long)(cy0 + cy) * a.bx + gx] = sa;
-      dst[(long long)(a.ncy + cy0 + cy) * a.bx + gx] = sd;
+  };
+  load(t0 * AN_TCX);
+  for (int it = 0; it < nt; ++it) {
+    const int cx0 = (t0 + it) * AN_TCX;
+#pragma unroll
+    for (int q = 0; q < AN_NLD; ++q) {
+      int e = tid + 256 * q;
+      if (e < AN_W * AN_H) s_in[e] = xn[q];
+    }
+    __syncthreads();
+    // along x: local sample 2c+3-k <-> global 2(cx0+c)+1-k
+    for (int e = tid; e < AN_H * AN_TCX; e += 256) {
+      int c = e % AN_TCX, j = e / AN_TCX;
+      float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float2 x = s_in[j * AN_W + 2 * c + 3 - k];
+        fma2(sa, a.f.dlo[k], x);
+        fma2(sd, a.f.dhi[k], x);
+      }
+      s_x[j * 2 * AN_TCX + c] = sa;
+      s_x[j * 2 * AN_TCX + AN_TCX + c] = sd;
+    }
+    __syncthreads();
+    if (it + 1 < nt) load(cx0 + AN_TCX);
+    for (int e = tid; e < AN_TCY * 2 * AN_TCX; e += 256) {
+      int c2 = e % (2 * AN_TCX), cy = e / (2 * AN_TCX);
+      float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float2 x = s_x[(2 * cy + 3 - k) * 2 * AN_TCX + c2];
+        fma2(sa, a.f.dlo[k], x);
+        fma2(sd, a.f.dhi[k], x);
+      }
+      int cxl = c2 < AN_TCX ? c2 : c2 - AN_TCX;
+      if (cx0 + cxl < a.ncx && cy0 + cy < a.ncy) {
+        long long gxo = (c2 < AN_TCX) ? cx0 + cxl : a.ncx + cx0 + cxl;
+        dst[(long long)(cy0 + cy) * a.bx + gxo] = sa;
+        dst[(long long)(a.ncy + cy0 + cy) * a.bx + gxo] = sd;
+      }
     }
   }
 }
@@ -136,7 +161,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
 struct ZArgs {
   float2* B;
   long long bx, by, bvp;
-  int az, ncz, ncx, ncy, tiles_x;
+  int az, ncz, ncx, ncy, tiles_x, groups_x;
   float2* A;  // next-level input (ncx x ncy x ncz per volume) or null at the deepest level
   long long avp;
   int deepest;
@@ -144,53 +169,76 @@ struct ZArgs {
   Db2F f;
 };
 
+// grid.x = groups_x * by (TPW consecutive tiles of ZT x-neighbour z-lines at one Y), grid.y = volumes.
+// NZR >= ceil(az / 16) values of a line are prefetched per thread.
+template <int NZR>
 __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
   extern __shared__ float2 zs[];
   float2* L = zs;                    // az x ZT
   float2* Cf = zs + a.az * ZT;       // 2ncz x ZT
   const int tid = threadIdx.x;
-  const int t = tid % ZT, r = tid / ZT;  // 16 lanes per line index group
-  const int X0 = (blockIdx.x % a.tiles_x) * ZT;
-  const int Y = blockIdx.x / a.tiles_x;
+  const int t = tid % ZT, r = tid / ZT;
+  const int gx = blockIdx.x % a.groups_x;
+  const int Y = blockIdx.x / a.groups_x;
   const long long v = blockIdx.y;
-  const int X = X0 + t;
-  const bool act = X < a.bx;
-  float2* base = a.B + v * a.bvp + (long long)Y * a.bx + X;
   const long long zp = a.bx * a.by;
-  for (int z = r; z < a.az; z += 256 / ZT) L[z * ZT + t] = act ? base[z * zp] : make_float2(0.f, 0.f);
-  __syncthreads();
-  const bool ll = X < a.ncx && Y < a.ncy;
-  for (int c = r; c < a.ncz; c += 256 / ZT) {
-    float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+  float2* row = a.B + v * a.bvp + (long long)Y * a.bx;
+  const int t0 = gx * TPW;
+  const int nt = min(TPW, a.tiles_x - t0);
+  float2 xn[NZR];
+  auto load = [&](int X) {
+    const bool act = X < a.bx;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      float2 x = L[sym(2 * c + 1 - k, a.az) * ZT + t];
-      fma2(sa, a.f.dlo[k], x);
-      fma2(sd, a.f.dhi[k], x);
+    for (int q = 0; q < NZR; ++q) {
+      int z = r + (256 / ZT) * q;
+      if (z < a.az) xn[q] = act ? row[X + z * zp] : make_float2(0.f, 0.f);
     }
-    sd = shrink(sd, a.alpha);
-    if (!ll || a.deepest) {
-      sa = shrink(sa, a.alpha);
-    } else {
-      a.A[v * a.avp + (long long)c * a.ncx * a.ncy + (long long)Y * a.ncx + X] = sa;
-      sa = make_float2(0.f, 0.f);
-    }
-    Cf[c * ZT + t] = sa;
-    Cf[(a.ncz + c) * ZT + t] = sd;
-  }
-  __syncthreads();
-  for (int s = r; s < a.az; s += 256 / ZT) {
-    float2 acc = make_float2(0.f, 0.f);
+  };
+  load(t0 * ZT + t);
+  for (int it = 0; it < nt; ++it) {
+    const int X = (t0 + it) * ZT + t;
+    const bool act = X < a.bx;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      int idx = s + 2 - j;
-      if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0) {
-        int tt = idx >> 1;
-        fma2(acc, a.f.rlo[j], Cf[tt * ZT + t]);
-        fma2(acc, a.f.rhi[j], Cf[(a.ncz + tt) * ZT + t]);
+    for (int q = 0; q < NZR; ++q) {
+      int z = r + (256 / ZT) * q;
+      if (z < a.az) L[z * ZT + t] = xn[q];
+    }
+    __syncthreads();
+    const bool ll = X < a.ncx && Y < a.ncy;
+    for (int c = r; c < a.ncz; c += 256 / ZT) {
+      float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float2 x = L[sym(2 * c + 1 - k, a.az) * ZT + t];
+        fma2(sa, a.f.dlo[k], x);
+        fma2(sd, a.f.dhi[k], x);
       }
+      sd = shrink(sd, a.alpha);
+      if (!ll || a.deepest) {
+        sa = shrink(sa, a.alpha);
+      } else {
+        a.A[v * a.avp + (long long)c * a.ncx * a.ncy + (long long)Y * a.ncx + X] = sa;
+        sa = make_float2(0.f, 0.f);
+      }
+      Cf[c * ZT + t] = sa;
+      Cf[(a.ncz + c) * ZT + t] = sd;
     }
-    if (act) base[s * zp] = acc;
+    __syncthreads();
+    if (it + 1 < nt) load(X + ZT);
+    for (int s = r; s < a.az; s += 256 / ZT) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int idx = s + 2 - j;
+        if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0) {
+          int tt = idx >> 1;
+          fma2(acc, a.f.rlo[j], Cf[tt * ZT + t]);
+          fma2(acc, a.f.rhi[j], Cf[(a.ncz + tt) * ZT + t]);
+        }
+      }
+      if (act) row[X + s * zp] = acc;
+    }
+    __syncthreads();  // L and Cf are rewritten by the next tile
   }
 }
 
@@ -203,74 +251,119 @@ struct XYSynArgs {
   int ncz;           // this level's z coefficients per band (Ap's z length)
   float2* out;
   long long op1, op2, ovp;
-  int ax, ay, az, tiles_x;
+  int ax, ay, az, tiles_x, groups_x;
   Db2F f;
 };
+
+constexpr int SY_NLD = (4 * SY_NY * SY_NX + 255) / 256;
 
 __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   __shared__ float2 band[4 * SY_NY * SY_NX];   // [xb + 2*yb][iy][ix]
   __shared__ float2 yrow[2 * SY_TY * SY_NX];   // [xb][sy][ix]
   const int tid = threadIdx.x;
-  const int x0 = (blockIdx.x % a.tiles_x) * SY_TX, y0 = (blockIdx.x / a.tiles_x) * SY_TY;
+  const int gx = blockIdx.x % a.groups_x;
+  const int y0 = (blockIdx.x / a.groups_x) * SY_TY;
   const int z = blockIdx.y % a.az;
   const long long v = blockIdx.y / a.az;
-  const int tx0 = x0 / 2, ty0 = y0 / 2;
+  const int ty0 = y0 / 2;
   const float2* src = a.B + v * a.bvp + (long long)z * a.bx * a.by;
-  for (int e = tid; e < 4 * SY_NY * SY_NX; e += 256) {
-    int ix = e % SY_NX, rem = e / SY_NX;
-    int iy = rem % SY_NY, b = rem / SY_NY;
-    int cx = tx0 + ix, cy = ty0 + iy;
-    float2 val = make_float2(0.f, 0.f);
-    if (cx < a.ncx && cy < a.ncy) {
-      long long gx = (b & 1) ? a.ncx + cx : cx;
-      long long gy = (b >> 1) ? a.ncy + cy : cy;
-      val = src[gy * a.bx + gx];
-      if (b == 0 && a.Ap) {
-        const float2* ap = a.Ap + v * a.apvp + (long long)cy * a.ncx + cx;
+  const float2* ap = a.Ap ? a.Ap + v * a.apvp : nullptr;
+  // the LL band of this z slice picks up syn_z(Ap, 0): taps j with z+2-j even and in range
+  int za = -1, zb = -1;
+  float wa = 0.f, wb = 0.f;
+  if (ap) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          int idx = z + 2 - j;
-          if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0)
-            fma2(val, a.f.rlo[j], ap[(long long)(idx >> 1) * a.ncx * a.ncy]);
+    for (int j = 0; j < 4; ++j) {
+      int idx = z + 2 - j;
+      if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0) {
+        if (za < 0) {
+          za = idx >> 1;
+          wa = a.f.rlo[j];
+        } else {
+          zb = idx >> 1;
+          wb = a.f.rlo[j];
         }
       }
     }
-    band[e] = val;
   }
-  __syncthreads();
-  // along y: yrow[xb][sy][ix] from the (xb, yb = 0 / 1) bands
-  for (int e = tid; e < 2 * SY_TY * SY_NX; e += 256) {
-    int ix = e % SY_NX, rem = e / SY_NX;
-    int sy = rem % SY_TY, xb = rem / SY_TY;
-    float2 acc = make_float2(0.f, 0.f);
+  const int t0 = gx * TPW;
+  const int nt = min(TPW, a.tiles_x - t0);
+  float2 bn[SY_NLD], an[SY_NLD], cn[SY_NLD];
+  auto load = [&](int x0) {
+    const int tx0 = x0 / 2;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      int idx = y0 + sy + 2 - j;
-      if (idx >= 0 && idx < 2 * a.ncy && (idx & 1) == 0) {
-        int t = (idx >> 1) - ty0;
-        fma2(acc, a.f.rlo[j], band[(xb * SY_NY + t) * SY_NX + ix]);
-        fma2(acc, a.f.rhi[j], band[((xb + 2) * SY_NY + t) * SY_NX + ix]);
+    for (int q = 0; q < SY_NLD; ++q) {
+      int e = tid + 256 * q;
+      bn[q] = make_float2(0.f, 0.f);
+      an[q] = make_float2(0.f, 0.f);
+      cn[q] = make_float2(0.f, 0.f);
+      if (e < 4 * SY_NY * SY_NX) {
+        int ix = e % SY_NX, rem = e / SY_NX;
+        int iy = rem % SY_NY, b = rem / SY_NY;
+        int cx = tx0 + ix, cy = ty0 + iy;
+        if (cx < a.ncx && cy < a.ncy) {
+          long long gxo = (b & 1) ? a.ncx + cx : cx;
+          long long gyo = (b >> 1) ? a.ncy + cy : cy;
+          bn[q] = src[gyo * a.bx + gxo];
+          if (b == 0 && ap) {
+            long long o = (long long)cy * a.ncx + cx;
+            if (za >= 0) an[q] = ap[(long long)za * a.ncx * a.ncy + o];
+            if (zb >= 0) cn[q] = ap[(long long)zb * a.ncx * a.ncy + o];
+          }
+        }
       }
     }
-    yrow[e] = acc;
-  }
-  __syncthreads();
+  };
+  load(t0 * SY_TX);
   float2* dst = a.out + v * a.ovp + (long long)z * a.op2;
-  for (int e = tid; e < SY_TY * SY_TX; e += 256) {
-    int sx = e % SY_TX, sy = e / SY_TX;
-    int gx = x0 + sx, gy = y0 + sy;
-    if (gx >= a.ax || gy >= a.ay) continue;
-    float2 acc = make_float2(0.f, 0.f);
+  for (int it = 0; it < nt; ++it) {
+    const int x0 = (t0 + it) * SY_TX, tx0 = x0 / 2;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      int idx = gx + 2 - j;
-      if (idx >= 0 && idx < 2 * a.ncx && (idx & 1) == 0) {
-        int t = (idx >> 1) - tx0;
-        fma2(acc, a.f.rlo[j], yrow[sy * SY_NX + t]);
-        fma2(acc, a.f.rhi[j], yrow[(SY_TY + sy) * SY_NX + t]);
+    for (int q = 0; q < SY_NLD; ++q) {
+      int e = tid + 256 * q;
+      if (e < 4 * SY_NY * SY_NX) {
+        float2 val = bn[q];
+        fma2(val, wa, an[q]);
+        fma2(val, wb, cn[q]);
+        band[e] = val;
       }
     }
-    dst[(long long)gy * a.op1 + gx] = acc;
+    __syncthreads();
+    if (it + 1 < nt) load(x0 + SY_TX);
+    // along y: yrow[xb][sy][ix] from the (xb, yb = 0 / 1) bands
+    for (int e = tid; e < 2 * SY_TY * SY_NX; e += 256) {
+      int ix = e % SY_NX, rem = e / SY_NX;
+      int sy = rem % SY_TY, xb = rem / SY_TY;
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int idx = y0 + sy + 2 - j;
+        if (idx >= 0 && idx < 2 * a.ncy && (idx & 1) == 0) {
+          int tt = (idx >> 1) - ty0;
+          fma2(acc, a.f.rlo[j], band[(xb * SY_NY + tt) * SY_NX + ix]);
+          fma2(acc, a.f.rhi[j], band[((xb + 2) * SY_NY + tt) * SY_NX + ix]);
+        }
+      }
+      yrow[e] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < SY_TY * SY_TX; e += 256) {
+      int sx = e % SY_TX, sy = e / SY_TX;
+      int gxo = x0 + sx, gyo = y0 + sy;
+      if (gxo >= a.ax || gyo >= a.ay) continue;
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int idx = gxo + 2 - j;
+        if (idx >= 0 && idx < 2 * a.ncx && (idx & 1) == 0) {
+          int tt = (idx >> 1) - tx0;
+          fma2(acc, a.f.rlo[j], yrow[sy * SY_NX + tt]);
+          fma2(acc, a.f.rhi[j], yrow[(SY_TY + sy) * SY_NX + tt]);
+        }
+      }
+      dst[(long long)gyo * a.op1 + gxo] = acc;
+    }
+    __syncthreads();  // band and yrow are rewritten by the next tile
   }
 }
 
@@ -363,9 +456,10 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     a1.ncx = ncx;
     a1.ncy = ncy;
     a1.tiles_x = (ncx + AN_TCX - 1) / AN_TCX;
+    a1.groups_x = (a1.tiles_x + TPW - 1) / TPW;
     a1.f = f;
     int tiles_y = (ncy + AN_TCY - 1) / AN_TCY;
-    dim3 g1((unsigned)(a1.tiles_x * tiles_y), (unsigned)(in[2] * nvols));
+    dim3 g1((unsigned)(a1.groups_x * tiles_y), (unsigned)(in[2] * nvols));
     hipLaunchKernelGGL(k_wav_xy_an, g1, dim3(256), 0, s, a1);
     LLR_HIP(hipGetLastError());
     // pass 2
@@ -379,17 +473,25 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     a2.ncx = ncx;
     a2.ncy = ncy;
     a2.tiles_x = (int)((cd[0] + ZT - 1) / ZT);
+    a2.groups_x = (a2.tiles_x + TPW - 1) / TPW;
     a2.deepest = l == L - 1;
     a2.A = a2.deepest ? nullptr : Ap[l + 1];
     a2.avp = a2.deepest ? 0 : (long long)avol[l + 1];
     a2.alpha = alpha;
     a2.f = f;
     size_t smem = (size_t)(in[2] + 2 * ncz) * ZT * sizeof(float2);
-    if (smem > 160 * 1024) return fail(LLR_ERR_UNSUPPORTED, "z length too large for the wavelet z pass");
-    if (smem > 64 * 1024)
-      LLR_HIP(hipFuncSetAttribute((const void*)k_wav_z, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    dim3 g2((unsigned)(a2.tiles_x * cd[1]), (unsigned)nvols);
-    hipLaunchKernelGGL(k_wav_z, g2, dim3(256), smem, s, a2);
+    if (smem > 160 * 1024 || in[2] > 512) return fail(LLR_ERR_UNSUPPORTED, "z length too large for the wavelet z pass");
+    dim3 g2((unsigned)(a2.groups_x * cd[1]), (unsigned)nvols);
+    int nzr = (int)((in[2] + (256 / ZT) - 1) / (256 / ZT));
+#define WZ_CASE(NN)                                                                                       \
+  if (nzr <= NN) {                                                                                        \
+    if (smem > 64 * 1024)                                                                                 \
+      LLR_HIP(hipFuncSetAttribute((const void*)k_wav_z<NN>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                  (int)smem));                                                            \
+    hipLaunchKernelGGL(k_wav_z<NN>, g2, dim3(256), smem, s, a2);                                          \
+  } else
+    WZ_CASE(1) WZ_CASE(2) WZ_CASE(4) WZ_CASE(8) WZ_CASE(16) WZ_CASE(32) {}
+#undef WZ_CASE
     LLR_HIP(hipGetLastError());
   }
   for (int l = L - 1; l >= 0; --l) {
@@ -413,9 +515,10 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     a3.ay = (int)od[1];
     a3.az = (int)od[2];
     a3.tiles_x = (int)((od[0] + SY_TX - 1) / SY_TX);
+    a3.groups_x = (a3.tiles_x + TPW - 1) / TPW;
     a3.f = f;
     int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
-    dim3 g3((unsigned)(a3.tiles_x * tiles_y), (unsigned)(od[2] * nvols));
+    dim3 g3((unsigned)(a3.groups_x * tiles_y), (unsigned)(od[2] * nvols));
     hipLaunchKernelGGL(k_wav_xy_syn, g3, dim3(256), 0, s, a3);
     LLR_HIP(hipGetLastError());
   }
