@@ -83,7 +83,8 @@ struct XYAnArgs {
   const float2* in;
   long long ip1, ip2, ivp;  // input pitches (y, z) and volume pitch; x pitch 1
   float2* B;
-  long long bx, by, bvp;    // band buffer dims (2ncx, 2ncy) and volume pitch
+  long long bx, by, bvp;    // band buffer row pitch (2 px), rows (2ncy) and volume pitch
+  int px;                   // padded band width (ncx rounded up to 16): the detail band starts at x = px
   int ax, ay, az, ncx, ncy, tiles_x, groups_x;
   Db2F f;
 };
@@ -150,7 +151,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
       }
       int cxl = c2 < AN_TCX ? c2 : c2 - AN_TCX;
       if (cx0 + cxl < a.ncx && cy0 + cy < a.ncy) {
-        long long gxo = (c2 < AN_TCX) ? cx0 + cxl : a.ncx + cx0 + cxl;
+        long long gxo = (c2 < AN_TCX) ? cx0 + cxl : a.px + cx0 + cxl;
         dst[(long long)(cy0 + cy) * a.bx + gxo] = sa;
         dst[(long long)(a.ncy + cy0 + cy) * a.bx + gxo] = sd;
       }
@@ -161,9 +162,10 @@ __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
 struct ZArgs {
   float2* B;
   long long bx, by, bvp;
-  int az, ncz, ncx, ncy, tiles_x, groups_x;
-  float2* A;  // next-level input (ncx x ncy x ncz per volume) or null at the deepest level
+  int az, ncz, ncx, ncy, px, tiles_x, groups_x;
+  float2* A;  // next-level input (ncx x ncy x ncz, row pitch pa) or null at the deepest level
   long long avp;
+  int pa;
   int deepest;
   float alpha;
   Db2F f;
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
   const int nt = min(TPW, a.tiles_x - t0);
   float2 xn[NZR];
   auto load = [&](int X) {
-    const bool act = X < a.bx;
+    const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx;
 #pragma unroll
     for (int q = 0; q < NZR; ++q) {
       int z = r + (256 / ZT) * q;
@@ -197,7 +199,7 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
   load(t0 * ZT + t);
   for (int it = 0; it < nt; ++it) {
     const int X = (t0 + it) * ZT + t;
-    const bool act = X < a.bx;
+    const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx;
 #pragma unroll
     for (int q = 0; q < NZR; ++q) {
       int z = r + (256 / ZT) * q;
@@ -217,7 +219,7 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
       if (!ll || a.deepest) {
         sa = shrink(sa, a.alpha);
       } else {
-        a.A[v * a.avp + (long long)c * a.ncx * a.ncy + (long long)Y * a.ncx + X] = sa;
+        a.A[v * a.avp + (long long)c * a.pa * a.ncy + (long long)Y * a.pa + X] = sa;
         sa = make_float2(0.f, 0.f);
       }
       Cf[c * ZT + t] = sa;
@@ -245,9 +247,10 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
 struct XYSynArgs {
   const float2* B;
   long long bx, by, bvp;
-  int ncx, ncy;
-  const float2* Ap;  // reconstructed next level (ncx x ncy x ncz) or null
+  int ncx, ncy, px;
+  const float2* Ap;  // reconstructed next level (ncx x ncy x ncz, row pitch pa) or null
   long long apvp;
+  int pa;
   int ncz;           // this level's z coefficients per band (Ap's z length)
   float2* out;
   long long op1, op2, ovp;
@@ -302,13 +305,13 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
         int iy = rem % SY_NY, b = rem / SY_NY;
         int cx = tx0 + ix, cy = ty0 + iy;
         if (cx < a.ncx && cy < a.ncy) {
-          long long gxo = (b & 1) ? a.ncx + cx : cx;
+          long long gxo = (b & 1) ? a.px + cx : cx;
           long long gyo = (b >> 1) ? a.ncy + cy : cy;
           bn[q] = src[gyo * a.bx + gxo];
           if (b == 0 && ap) {
-            long long o = (long long)cy * a.ncx + cx;
-            if (za >= 0) an[q] = ap[(long long)za * a.ncx * a.ncy + o];
-            if (zb >= 0) cn[q] = ap[(long long)zb * a.ncx * a.ncy + o];
+            long long o = (long long)cy * a.pa + cx;
+            if (za >= 0) an[q] = ap[(long long)za * a.pa * a.ncy + o];
+            if (zb >= 0) cn[q] = ap[(long long)zb * a.pa * a.ncy + o];
           }
         }
       }
@@ -394,15 +397,20 @@ void wavelet_geometry(const size_t* dims, int L, std::vector<std::array<size_t, 
   }
 }
 
-// Per-volume workspace: the band buffer of every level (2ncx x 2ncy x nz) and
-// the next-level inputs of levels 1..L-1 (ncx x ncy x ncz of the level above).
+
+static inline size_t pad16(size_t n) { return (n + 15) / 16 * 16; }
+
+// Per-volume workspace, with rows padded to 16 elements (128 B) so every band
+// row segment is cache-line aligned: the band buffer of every level
+// (2*pad16(ncx) x 2ncy x nz) and the next-level inputs of levels 1..L-1
+// (pad16(ncx) x ncy x ncz of the level above).
 size_t wavelet_workspace_elems(const size_t* dims, int L, size_t* coef_elems, size_t* tmp_elems) {
   std::vector<std::array<size_t, 3>> in_dims, cdims;
   wavelet_geometry(dims, L, in_dims, cdims);
   size_t c = 0;
   for (int l = 0; l < L; ++l) {
-    c += cdims[l][0] * cdims[l][1] * in_dims[l][2];
-    if (l > 0) c += in_dims[l][0] * in_dims[l][1] * in_dims[l][2];
+    c += 2 * pad16(cdims[l][0] / 2) * cdims[l][1] * in_dims[l][2];
+    if (l > 0) c += pad16(in_dims[l][0]) * in_dims[l][1] * in_dims[l][2];
   }
   *coef_elems = c;
   *tmp_elems = 0;
@@ -421,16 +429,18 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
   std::vector<std::array<size_t, 3>> in_dims, cdims;
   wavelet_geometry(dims, L, in_dims, cdims);
   std::vector<float2*> Bp(L), Ap(L, nullptr);
-  std::vector<size_t> bvol(L), avol(L, 0);
+  std::vector<size_t> bvol(L), avol(L, 0), px(L), pa(L, 0);
   {
     float2* p = coef;
     for (int l = 0; l < L; ++l) {
-      bvol[l] = cdims[l][0] * cdims[l][1] * in_dims[l][2];
+      px[l] = pad16(cdims[l][0] / 2);
+      bvol[l] = 2 * px[l] * cdims[l][1] * in_dims[l][2];
       Bp[l] = p;
       p += bvol[l] * nvols;
     }
     for (int l = 1; l < L; ++l) {
-      avol[l] = in_dims[l][0] * in_dims[l][1] * in_dims[l][2];
+      pa[l] = pad16(in_dims[l][0]);
+      avol[l] = pa[l] * in_dims[l][1] * in_dims[l][2];
       Ap[l] = p;
       p += avol[l] * nvols;
     }
@@ -440,16 +450,18 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     const size_t* in = in_dims[l].data();
     const size_t* cd = cdims[l].data();
     int ncx = (int)cd[0] / 2, ncy = (int)cd[1] / 2, ncz = (int)cd[2] / 2;
+    const long long bx = 2 * (long long)px[l], by = (long long)cd[1];
     // pass 1
     XYAnArgs a1{};
     a1.in = l == 0 ? vols : Ap[l];
-    a1.ip1 = (long long)in[0];
-    a1.ip2 = (long long)(in[0] * in[1]);
-    a1.ivp = (long long)(in[0] * in[1] * in[2]);
+    a1.ip1 = l == 0 ? (long long)in[0] : (long long)pa[l];
+    a1.ip2 = a1.ip1 * (long long)in[1];
+    a1.ivp = a1.ip2 * (long long)in[2];
     a1.B = Bp[l];
-    a1.bx = cd[0];
-    a1.by = cd[1];
+    a1.bx = bx;
+    a1.by = by;
     a1.bvp = (long long)bvol[l];
+    a1.px = (int)px[l];
     a1.ax = (int)in[0];
     a1.ay = (int)in[1];
     a1.az = (int)in[2];
@@ -465,23 +477,25 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     // pass 2
     ZArgs a2{};
     a2.B = Bp[l];
-    a2.bx = cd[0];
-    a2.by = cd[1];
+    a2.bx = bx;
+    a2.by = by;
     a2.bvp = (long long)bvol[l];
     a2.az = (int)in[2];
     a2.ncz = ncz;
     a2.ncx = ncx;
     a2.ncy = ncy;
-    a2.tiles_x = (int)((cd[0] + ZT - 1) / ZT);
+    a2.px = (int)px[l];
+    a2.tiles_x = (int)((bx + ZT - 1) / ZT);
     a2.groups_x = (a2.tiles_x + TPW - 1) / TPW;
     a2.deepest = l == L - 1;
     a2.A = a2.deepest ? nullptr : Ap[l + 1];
     a2.avp = a2.deepest ? 0 : (long long)avol[l + 1];
+    a2.pa = a2.deepest ? 0 : (int)pa[l + 1];
     a2.alpha = alpha;
     a2.f = f;
     size_t smem = (size_t)(in[2] + 2 * ncz) * ZT * sizeof(float2);
     if (smem > 160 * 1024 || in[2] > 512) return fail(LLR_ERR_UNSUPPORTED, "z length too large for the wavelet z pass");
-    dim3 g2((unsigned)(a2.groups_x * cd[1]), (unsigned)nvols);
+    dim3 g2((unsigned)(a2.groups_x * by), (unsigned)nvols);
     int nzr = (int)((in[2] + (256 / ZT) - 1) / (256 / ZT));
 #define WZ_CASE(NN)                                                                                       \
   if (nzr <= NN) {                                                                                        \
@@ -499,18 +513,20 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     const size_t* cd = cdims[l].data();
     XYSynArgs a3{};
     a3.B = Bp[l];
-    a3.bx = cd[0];
-    a3.by = cd[1];
+    a3.bx = 2 * (long long)px[l];
+    a3.by = (long long)cd[1];
     a3.bvp = (long long)bvol[l];
     a3.ncx = (int)cd[0] / 2;
     a3.ncy = (int)cd[1] / 2;
+    a3.px = (int)px[l];
     a3.ncz = (int)cd[2] / 2;
     a3.Ap = (l < L - 1) ? Ap[l + 1] : nullptr;
     a3.apvp = (l < L - 1) ? (long long)avol[l + 1] : 0;
+    a3.pa = (l < L - 1) ? (int)pa[l + 1] : 0;
     a3.out = l == 0 ? vols : Ap[l];
-    a3.op1 = (long long)od[0];
-    a3.op2 = (long long)(od[0] * od[1]);
-    a3.ovp = (long long)(od[0] * od[1] * od[2]);
+    a3.op1 = l == 0 ? (long long)od[0] : (long long)pa[l];
+    a3.op2 = a3.op1 * (long long)od[1];
+    a3.ovp = a3.op2 * (long long)od[2];
     a3.ax = (int)od[0];
     a3.ay = (int)od[1];
     a3.az = (int)od[2];
