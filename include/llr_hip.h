@@ -223,6 +223,9 @@ typedef struct {
   const llr_cf32* subspace; /* nq x srank, col-major (lib.rs:210-219); host pointer, copied */
   const size_t* shifts;  /* n_iter x 3 block shifts (lib.rs:266); host pointer, copied */
   double residual_stop;  /* > 0: stop once the relative residual falls to this value */
+  int partial_blocks;    /* 1: ceil(vol/blk) block grid whose edge blocks are truncated at the
+                            volume end (extension; the reference grid is floor(vol/blk),
+                            block.rs:112-118, leaving the remainder voxels out of the LLR step) */
 } llr_recon_params;
 
 /* Create a reconstruction (allocates the iterate, the measurements and all

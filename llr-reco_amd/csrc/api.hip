@@ -109,7 +109,7 @@ static llr_status check_fft_dims(const size_t* dims) {
   for (int a = 0; a < 3; ++a)
     if (dims[a] < 2 || !fft_size_supported(dims[a]))
       return fail(LLR_ERR_UNSUPPORTED, "volume dimension " + std::to_string(dims[a]) +
-                                           " is not a supported FFT length (powers of two 2..512)");
+                                           " is not a supported FFT length (2..512, radix 2/3)");
   return LLR_OK;
 }
 
@@ -146,7 +146,7 @@ using namespace llr;
 extern "C" {
 
 const char* llr_last_error(void) { return g_err.c_str(); }
-int llr_abi_version(void) { return 100; }
+int llr_abi_version(void) { return 101; }
 
 llr_status llr_ctx_create(int device, llr_ctx** out) {
   if (!out) return fail(LLR_ERR_ARG, "null output pointer");
@@ -591,9 +591,9 @@ struct StageTimer {
 llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, unsigned* smax) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
-  size_t grid[3];
-  llr_grid_dim(p.vol, p.blk, grid);
-  long long nb = (long long)(grid[0] * grid[1] * grid[2]);
+  const bool partial = p.partial_blocks != 0;
+  long long nb = 1;
+  for (int a = 0; a < 3; ++a) nb *= (long long)(partial ? (p.vol[a] + p.blk[a] - 1) / p.blk[a] : p.vol[a] / p.blk[a]);
   llr_status st;
   if (p.srank) {
     {
@@ -606,16 +606,16 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
       // streaming pass in voxel order (blocks' voxels only)
       StageTimer t(r, 0);
       st = llr_block_run(ctx, r->Cb.as<float2>(), r->Cb.as<float2>(), nullptr, 0, p.vol, p.blk, shift, r->nc, 0, 0, nb,
-                         mode, (int)p.rank, lambda, smax, nullptr);
+                         mode, (int)p.rank, lambda, smax, nullptr, partial);
     }
     if (st || mode == 2) return st;
     StageTimer t(r, 1);
     return backproject_device(ctx, r->Cb.as<float2>(), r->X.as<float2>(), r->S.as<float2>(), p.vol, p.blk, shift,
-                              (int)r->nq, (int)p.srank);
+                              (int)r->nq, (int)p.srank, partial);
   }
   StageTimer t(r, 0);
   return llr_block_run(ctx, r->X.as<float2>(), r->X.as<float2>(), nullptr, 0, p.vol, p.blk, shift, r->nc, 0, 0, nb, mode,
-                       (int)p.rank, lambda, smax, nullptr);
+                       (int)p.rank, lambda, smax, nullptr, partial);
 }
 
 llr_status recon_iteration(llr_recon* r, size_t it) {

@@ -47,10 +47,32 @@ __device__ __forceinline__ constexpr float cos16(int k) {
 }
 __device__ __forceinline__ constexpr float sin16(int k) { return cos16(k - 4); }
 
-// forward twiddle exp(-2*pi*i*k/R) for R | 32
+// exp(-2*pi*i*k/12) components, k in [0,12)
+__device__ __forceinline__ constexpr float cos12(int k) {
+  switch (((k % 12) + 12) % 12) {
+    case 0: return 1.0f;
+    case 1: return 0.866025403784438647f;
+    case 2: return 0.5f;
+    case 3: return 0.0f;
+    case 4: return -0.5f;
+    case 5: return -0.866025403784438647f;
+    case 6: return -1.0f;
+    case 7: return -0.866025403784438647f;
+    case 8: return -0.5f;
+    case 9: return 0.0f;
+    case 10: return 0.5f;
+    default: return 0.866025403784438647f;
+  }
+}
+__device__ __forceinline__ constexpr float sin12(int k) { return cos12(k - 3); }
+
+// forward twiddle exp(-2*pi*i*k/R) for R | 32 or R | 12
 template <int R>
 __device__ __forceinline__ float2 tw_const(int k) {
-  if constexpr (R <= 16) {
+  if constexpr (12 % R == 0 && 16 % R != 0) {
+    int k12 = k * (12 / R);
+    return make_float2(cos12(k12), -sin12(k12));
+  } else if constexpr (R <= 16) {
     int k16 = k * (16 / R);
     return make_float2(cos16(k16), -sin16(k16));
   } else {
@@ -104,6 +126,18 @@ struct SmallDFT<2> {
   }
 };
 template <>
+struct SmallDFT<3> {
+  __device__ __forceinline__ static void run(float2* v) {
+    const float h = 0.866025403784438647f;  // sqrt(3)/2
+    float2 a = v[0], t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    float2 m = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
+    float2 r = make_float2(h * t2.y, -h * t2.x);  // -i * (sqrt(3)/2) * t2
+    v[0] = cadd(a, t1);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+};
+template <>
 struct SmallDFT<4> {
   __device__ __forceinline__ static void run(float2* v) {
     float2 a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
@@ -148,11 +182,13 @@ struct PassArgs {
   int tiles_per_wg;
 };
 
+constexpr int pow2_floor(int x) { return x >= 256 ? 256 : x >= 128 ? 128 : x >= 64 ? 64 : x >= 32 ? 32 : x >= 16 ? 16 : x >= 8 ? 8 : x >= 4 ? 4 : x >= 2 ? 2 : 1; }
+
 template <int R1, int R2>
 struct Layout {
   static constexpr int N = R1 * R2;
   static constexpr int P = R1 > R2 ? R1 : R2;
-  static constexpr int T = (256 / P) > 0 ? (256 / P) : 1;
+  static constexpr int T = pow2_floor(256 / P);  // lines per workgroup (power of two: aligned x runs)
   static constexpr int R2P = R2 + 1;
   static constexpr int LS = R1 * R2P + 1;
   static constexpr int THREADS = T * P;
@@ -408,7 +444,13 @@ hipError_t launch_zproj(ZProjArgs a, long long W, long long nvols, hipStream_t s
 namespace llr {
 
 bool fft_size_supported(size_t n) {
-  return n == 2 || n == 4 || n == 8 || n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
+  switch (n) {
+    case 2: case 3: case 4: case 6: case 8: case 12: case 16: case 24: case 32: case 48: case 64: case 96:
+    case 128: case 192: case 256: case 384: case 512:
+      return true;
+    default:
+      return false;
+  }
 }
 
 // Dispatch an axis pass for line length n.
@@ -424,6 +466,14 @@ static hipError_t dispatch_pass(int n, const PassArgs& a, long long W, long long
     case 128: return launch_pass<16, 8>(a, W, nv, tfast, inv, s);
     case 256: return launch_pass<16, 16>(a, W, nv, tfast, inv, s);
     case 512: return launch_pass<32, 16>(a, W, nv, tfast, inv, s);
+    case 3: return launch_pass<3, 1>(a, W, nv, tfast, inv, s);
+    case 6: return launch_pass<3, 2>(a, W, nv, tfast, inv, s);
+    case 12: return launch_pass<6, 2>(a, W, nv, tfast, inv, s);
+    case 24: return launch_pass<6, 4>(a, W, nv, tfast, inv, s);
+    case 48: return launch_pass<12, 4>(a, W, nv, tfast, inv, s);
+    case 96: return launch_pass<12, 8>(a, W, nv, tfast, inv, s);
+    case 192: return launch_pass<12, 16>(a, W, nv, tfast, inv, s);
+    case 384: return launch_pass<12, 32>(a, W, nv, tfast, inv, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -435,7 +485,7 @@ llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const siz
                          float scale) {
   int n = (int)dims[axis];
   if (!fft_size_supported((size_t)n))
-    return fail(LLR_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " is not a supported power of two (2..512)");
+    return fail(LLR_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " is not supported (2^a * 3^b with a radix of at most 32, up to 512)");
   if ((pre == nullptr) != (post == nullptr)) return fail(LLR_ERR_ARG, "ramp tables must be given together");
   if ((double)dims[0] * dims[1] * dims[2] >= 4294967296.0)
     return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
@@ -518,6 +568,14 @@ llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_
     case 128: e = launch_zproj<16, 8>(a, ny, nvols, ctx->stream, nparts); break;
     case 256: e = launch_zproj<16, 16>(a, ny, nvols, ctx->stream, nparts); break;
     case 512: e = launch_zproj<32, 16>(a, ny, nvols, ctx->stream, nparts); break;
+    case 3: e = launch_zproj<3, 1>(a, ny, nvols, ctx->stream, nparts); break;
+    case 6: e = launch_zproj<3, 2>(a, ny, nvols, ctx->stream, nparts); break;
+    case 12: e = launch_zproj<6, 2>(a, ny, nvols, ctx->stream, nparts); break;
+    case 24: e = launch_zproj<6, 4>(a, ny, nvols, ctx->stream, nparts); break;
+    case 48: e = launch_zproj<12, 4>(a, ny, nvols, ctx->stream, nparts); break;
+    case 96: e = launch_zproj<12, 8>(a, ny, nvols, ctx->stream, nparts); break;
+    case 192: e = launch_zproj<12, 16>(a, ny, nvols, ctx->stream, nparts); break;
+    case 384: e = launch_zproj<12, 32>(a, ny, nvols, ctx->stream, nparts); break;
     default: return fail(LLR_ERR_UNSUPPORTED, "z length " + std::to_string(n) + " is not a supported power of two");
   }
   if (e != hipSuccess) return hip_fail(e, "fft z projection launch");

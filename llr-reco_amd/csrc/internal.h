@@ -22,10 +22,11 @@ llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols,
 llr_status project_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K);
 // X[q][v] = sum_k C[k][v] conj(S[q][k]) for voxels inside the block grid of `shift`
 llr_status backproject_device(llr_ctx* ctx, const float2* C, float2* X, const float2* S, const size_t* vol,
-                              const size_t* blk, const size_t* shift, int nq, int K);
+                              const size_t* blk, const size_t* shift, int nq, int K, bool partial = false);
 llr_status llr_block_run(llr_ctx* ctx, const float2* F, float2* out, const float2* S, int nq, const size_t* vol,
                          const size_t* blk, const size_t* shift, int nc, int layout, long long block0,
-                         long long nblocks, int mode, int rank, float lambda, unsigned* smax_bits, float* svals);
+                         long long nblocks, int mode, int rank, float lambda, unsigned* smax_bits, float* svals,
+                         bool partial = false);
 llr_status lowrank_generic_device(llr_ctx* ctx, float2* mats, long long m, int n, long long batch, int mode, int rank,
                                   float lambda, unsigned* smax_bits);
 

@@ -30,12 +30,16 @@ struct GeomArgs {
   long long m;  // voxels per block
 };
 
+// Voxel address of row i of block b, or -1 for a row past the end of the
+// volume (possible only on the ceil(vol/blk) grid of partial edge blocks).
 __device__ __forceinline__ long long block_voxel_addr(const GeomArgs& g, long long b, long long i) {
   long long bx = b % g.grid[0], by = (b / g.grid[0]) % g.grid[1], bz = b / (g.grid[0] * g.grid[1]);
   long long ix = i % g.blk[0], iy = (i / g.blk[0]) % g.blk[1], iz = i / (g.blk[0] * g.blk[1]);
-  long long x = (ix + bx * g.blk[0] + g.shift[0]) % g.vol[0];
-  long long y = (iy + by * g.blk[1] + g.shift[1]) % g.vol[1];
-  long long z = (iz + bz * g.blk[2] + g.shift[2]) % g.vol[2];
+  long long sx = ix + bx * g.blk[0], sy = iy + by * g.blk[1], sz = iz + bz * g.blk[2];
+  if (sx >= g.vol[0] || sy >= g.vol[1] || sz >= g.vol[2]) return -1;
+  long long x = (sx + g.shift[0]) % g.vol[0];
+  long long y = (sy + g.shift[1]) % g.vol[1];
+  long long z = (sz + g.shift[2]) % g.vol[2];
   return x + g.vol[0] * (y + g.vol[1] * z);
 }
 
@@ -265,9 +269,15 @@ struct BlockArgs {
   int stage_lds;        // 1: F block staged in LDS
 };
 
+// element address of row i, column col of block b; -1 for a row outside the volume
 __device__ __forceinline__ long long elem_addr(const BlockArgs& A, long long b, long long i, int col) {
   if (A.layout == 1) return b * A.g.m * A.nc + (long long)col * A.g.m + i;
-  return (long long)col * A.g.N + block_voxel_addr(A.g, b, i);
+  long long v = block_voxel_addr(A.g, b, i);
+  return v < 0 ? -1 : (long long)col * A.g.N + v;
+}
+__device__ __forceinline__ float2 load_elem(const BlockArgs& A, long long b, long long i, int col) {
+  long long e = elem_addr(A, b, i, col);
+  return e < 0 ? make_float2(0.f, 0.f) : A.F[e];
 }
 
 __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
@@ -290,7 +300,7 @@ __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
     for (long long e = tid; e < m * nc; e += 256) {
       long long i = e % m;
       int col = (int)(e / m);
-      Fs[col * mp + i] = A.F[elem_addr(A, b, i, col)];
+      Fs[col * mp + i] = load_elem(A, b, i, col);
     }
     __syncthreads();
   }
@@ -319,8 +329,8 @@ __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
           fi = Fs[i * mp + r];
           fj = Fs[j * mp + r];
         } else {
-          fi = A.F[elem_addr(A, b, r, i)];
-          fj = A.F[elem_addr(A, b, r, j)];
+          fi = load_elem(A, b, r, i);
+          fj = load_elem(A, b, r, j);
         }
         acc.x += fi.x * fj.x + fi.y * fj.y;
         acc.y += fi.x * fj.y - fi.y * fj.x;
@@ -385,6 +395,8 @@ __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
   }
   // apply: each thread handles rows r = tid, tid+256, ...
   for (long long rr = tid; rr < m; rr += 256) {
+    const long long vaddr = A.layout == 0 ? block_voxel_addr(A.g, b, rr) : 0;
+    if (vaddr < 0) continue;  // outside a partial edge block
     float2 f[8];
     for (int k = 0; k < 8; ++k) {
       if (k < nc)
@@ -393,7 +405,7 @@ __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
         f[k] = make_float2(0.f, 0.f);
     }
     if (A.S) {
-      long long addr = block_voxel_addr(A.g, b, rr);
+      const long long addr = vaddr;
       for (int q = 0; q < A.nq; ++q) {
         float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
@@ -407,7 +419,7 @@ __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
 #pragma unroll
         for (int k = 0; k < 8; ++k)
           if (k < nc) acc = cadd(acc, cmul(f[k], Ws[k + 8 * cc]));
-        A.out[elem_addr(A, b, rr, cc)] = acc;
+        A.out[A.layout == 0 ? (long long)cc * A.g.N + vaddr : elem_addr(A, b, rr, cc)] = acc;
       }
     }
   }
@@ -524,12 +536,14 @@ __global__ void __launch_bounds__(256) k_apply_generic(float2* __restrict__ Am, 
 
 namespace llr {
 
-GeomArgs make_geom(const size_t* vol, const size_t* blk, const size_t* shift) {
+// Block grid floor(vol/blk) (block.rs:112-118), or ceil(vol/blk) with
+// truncated edge blocks when `partial`.
+GeomArgs make_geom(const size_t* vol, const size_t* blk, const size_t* shift, bool partial = false) {
   GeomArgs g;
   for (int a = 0; a < 3; ++a) {
     g.vol[a] = (long long)vol[a];
     g.blk[a] = (long long)blk[a];
-    g.grid[a] = (long long)(vol[a] / blk[a]);
+    g.grid[a] = (long long)(partial ? (vol[a] + blk[a] - 1) / blk[a] : vol[a] / blk[a]);
     g.shift[a] = (long long)shift[a];
   }
   g.N = g.vol[0] * g.vol[1] * g.vol[2];
@@ -569,12 +583,13 @@ llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols,
 }
 
 llr_status backproject_device(llr_ctx* ctx, const float2* C, float2* X, const float2* S, const size_t* vol,
-                              const size_t* blk, const size_t* shift, int nq, int K) {
+                              const size_t* blk, const size_t* shift, int nq, int K, bool partial) {
   size_t N = vol[0] * vol[1] * vol[2];
   if (N >= 0xFFFFFFFFull) return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
   int cov[3], sh[3];
   for (int a = 0; a < 3; ++a) {
-    cov[a] = (int)((vol[a] / blk[a]) * blk[a]);  // extent covered by whole blocks
+    // extent covered by the blocks: whole blocks only, or everything with partial edge blocks
+    cov[a] = (int)(partial ? vol[a] : (vol[a] / blk[a]) * blk[a]);
     sh[a] = (int)(shift[a] % vol[a]);
   }
   unsigned grid = grid_for(N, 256, 65536u);
@@ -617,8 +632,9 @@ size_t llr_block_smem(long long m, int nc, int nq, bool stage) {
 //   layout 1: F = contiguous m x nc matrices
 llr_status llr_block_run(llr_ctx* ctx, const float2* F, float2* out, const float2* S, int nq, const size_t* vol,
                          const size_t* blk, const size_t* shift, int nc, int layout, long long block0,
-                         long long nblocks, int mode, int rank, float lambda, unsigned* smax_bits, float* svals) {
-  GeomArgs g = make_geom(vol, blk, shift);
+                         long long nblocks, int mode, int rank, float lambda, unsigned* smax_bits, float* svals,
+                         bool partial) {
+  GeomArgs g = make_geom(vol, blk, shift, partial);
   if (nc < 1 || nc > 8) return fail(LLR_ERR_UNSUPPORTED, "fused LLR kernel handles 1..8 columns");
   if (nblocks <= 0) return LLR_OK;
   BlockArgs A{};

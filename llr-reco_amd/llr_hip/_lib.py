@@ -67,6 +67,7 @@ class ReconParams(ctypes.Structure):
         ("subspace", ctypes.c_void_p),
         ("shifts", ctypes.c_void_p),
         ("residual_stop", ctypes.c_double),
+        ("partial_blocks", ctypes.c_int),
     ]
 
 

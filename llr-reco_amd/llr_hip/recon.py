@@ -59,11 +59,13 @@ class Recon:
     llr_mode: "svt" (singular-value soft threshold, lambda = lambda_factor *
     sigma_max of the starting blocks unless svt_lambda > 0) or "rank" (hard
     truncation to `rank`, low_rank.rs:10-69).
+    partial_blocks: block grid ceil(vol/blk) with edge blocks truncated at the
+    volume end, instead of floor(vol/blk) (block.rs:112-118).
     """
 
     def __init__(self, vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0,
                  lambda_factor=0.02, wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0,
-                 precision="fp32", ctx=None):
+                 precision="fp32", partial_blocks=False, ctx=None):
         self.ctx = ctx or _lib.default_context()
         self.vol = [int(v) for v in vol]
         self.nq = int(nq)
@@ -95,6 +97,7 @@ class Recon:
             p.subspace = None
         p.shifts = ctypes.c_void_p(sh.ctypes.data) if self.n_iter else None
         p.residual_stop = float(residual_stop)
+        p.partial_blocks = 1 if partial_blocks else 0
         h = ctypes.c_void_p()
         _lib.check(_lib.load().llr_recon_create(self.ctx.handle, ctypes.byref(p), ctypes.byref(h)))
         self.handle = h

@@ -771,9 +771,71 @@ struct orc_recon_params {
   size_t srank;          // 0: no subspace
   const void* subspace;  // nq x srank complex, col-major (T precision)
   const size_t* shifts;  // n_iter x 3
+  int partial_blocks;    // 1: ceil(vol/blk) grid with truncated edge blocks (extension)
 };
 
 namespace orc {
+
+// Partial edge blocks, an extension of the block grid of block.rs:112-133
+// (floor(vol/blk) blocks per axis, so the last vol % blk shifted positions of
+// each axis are left out of the LLR step). Here the grid is ceil(vol/blk) and
+// a block that runs past the end of an axis keeps only its first
+// vol - s*blk voxels along it: an (m_b x n) Casorati matrix of its own, with
+// the same projection, low-rank step and back-projection as a whole block.
+// Blocks never overlap, so they are processed in any order.
+template <class T>
+static T llr_partial(C<T>* x, size_t nq, const size_t* vol, const size_t* blk, const size_t* shift,
+                     size_t srank, const C<T>* S, int mode, int rank, T lam, bool modify) {
+  size_t g[3];
+  for (int a = 0; a < 3; ++a) g[a] = (vol[a] + blk[a] - 1) / blk[a];
+  size_t nb = g[0] * g[1] * g[2], N = vol[0] * vol[1] * vol[2], ncols = srank ? srank : nq;
+  T smax = 0;
+#pragma omp parallel
+  {
+    std::vector<C<T>> mat, comp, ws;
+    std::vector<T> sv(ncols);
+    T local = 0;
+#pragma omp for schedule(dynamic, 8)
+    for (long long b = 0; b < (long long)nb; ++b) {
+      size_t s[3], eb[3], corner[3];
+      ind2sub3((size_t)b, g, s);
+      for (int a = 0; a < 3; ++a) {
+        eb[a] = std::min(blk[a], vol[a] - s[a] * blk[a]);
+        corner[a] = s[a] * blk[a] + shift[a];
+      }
+      size_t mb = eb[0] * eb[1] * eb[2];
+      mat.resize(mb * nq);
+      for (size_t q = 0; q < nq; ++q)
+        for (size_t i = 0; i < mb; ++i) mat[q * mb + i] = x[q * N + block_voxel_address(i, eb, corner, vol)];
+      C<T>* target = mat.data();
+      if (srank) {  // comp = X S (m_b x srank)
+        comp.assign(mb * srank, C<T>(0, 0));
+        for (size_t k = 0; k < srank; ++k)
+          for (size_t q = 0; q < nq; ++q)
+            for (size_t i = 0; i < mb; ++i) comp[k * mb + i] += mat[q * mb + i] * S[q + k * nq];
+        target = comp.data();
+      }
+      if (!modify) {
+        block_svals<T>(mb, ncols, 1, target, sv.data());
+        local = std::max(local, sv[0]);
+        continue;
+      }
+      local = std::max(local, lowrank_one<T>(mb, (int)ncols, target, mode, rank, lam, ws));
+      if (srank)  // X = L S^H
+        for (size_t q = 0; q < nq; ++q)
+          for (size_t i = 0; i < mb; ++i) {
+            C<T> acc(0, 0);
+            for (size_t k = 0; k < srank; ++k) acc += comp[k * mb + i] * std::conj(S[q + k * nq]);
+            mat[q * mb + i] = acc;
+          }
+      for (size_t q = 0; q < nq; ++q)
+        for (size_t i = 0; i < mb; ++i) x[q * N + block_voxel_address(i, eb, corner, vol)] = mat[q * mb + i];
+    }
+#pragma omp critical
+    smax = std::max(smax, local);
+  }
+  return smax;
+}
 
 // The iteration loop of llr_recon_exec (lib.rs:227-341) from the measurements
 // on: linear-correction estimate, zero-filled adjoint, then per iteration
@@ -807,6 +869,8 @@ static int recon(const orc_recon_params* p, const C<T>* meas, C<T>* x, double* e
   if (p->srank) comp.resize(nblocks * m * p->srank);
 
   auto llr = [&](const size_t* shift, int mode, T lam, bool modify) -> T {
+    if (p->partial_blocks)
+      return llr_partial<T>(x, nq, vol, p->blk, shift, p->srank, S, mode, (int)p->rank, lam, modify);
     lift((const uint8_t*)x, nq, sizeof(C<T>), (uint8_t*)mats.data(), vol, p->blk, shift, 0, nblocks);
     T smax;
     C<T>* target;
@@ -927,6 +991,12 @@ int orc_un_lift_data_set(void* vols, size_t nvols, size_t eb, const void* mats, 
   }                                                                                                    \
   double orc_svt_batch_##SUF(size_t m, size_t n, double lam, size_t batch, void* d) {                  \
     return (double)orc::lowrank_batch<T>(m, n, batch, (CF(T)*)d, 1, 0, (T)lam);                        \
+  }                                                                                                    \
+  double orc_llr_partial_##SUF(void* x, size_t nq, const size_t* vol, const size_t* blk,               \
+                               const size_t* shift, size_t srank, const void* S, int mode, int rank,   \
+                               double lam) {                                                           \
+    return (double)orc::llr_partial<T>((CF(T)*)x, nq, vol, blk, shift, srank, (const CF(T)*)S, mode,    \
+                                       rank, (T)lam, true);                                            \
   }                                                                                                    \
   void orc_block_svals_##SUF(size_t m, size_t n, size_t batch, const void* d, T* s) {                  \
     orc::block_svals<T>(m, n, batch, (const CF(T)*)d, s);                                              \

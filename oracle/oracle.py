@@ -137,6 +137,23 @@ def svt_batch(m, n, lam, batch, data):
     return getattr(lib(), "orc_svt_batch_" + _suf(data))(m, n, float(lam), batch, _p(data))
 
 
+def llr_partial(x, vol, nq, blk, shift, mode=1, rank=0, lam=0.0, subspace=None):
+    """One LLR step over the ceil(vol/blk) grid with truncated edge blocks, in place on x
+    (nq volumes, column-major). Returns the largest singular value seen."""
+    s = _suf(x)
+    v = (ctypes.c_size_t * 3)(*vol)
+    b = (ctypes.c_size_t * 3)(*blk)
+    sh = (ctypes.c_size_t * 3)(*shift)
+    sub, srank = None, 0
+    if subspace is not None:
+        sub = np.asfortranarray(subspace).astype(x.dtype).ravel(order="F").copy()
+        srank = subspace.shape[1]
+    f = getattr(lib(), "orc_llr_partial_" + s)
+    f.restype = ctypes.c_double
+    return f(_p(x), ctypes.c_size_t(nq), v, b, sh, ctypes.c_size_t(srank),
+             None if sub is None else _p(sub), ctypes.c_int(mode), ctypes.c_int(rank), ctypes.c_double(lam))
+
+
 def block_svals(m, n, batch, data):
     s = _suf(data)
     out = np.zeros(n * batch, np.float32 if s == "f32" else np.float64)
@@ -244,12 +261,16 @@ class ReconParams(ctypes.Structure):
         ("srank", ctypes.c_size_t),
         ("subspace", ctypes.c_void_p),
         ("shifts", ctypes.POINTER(ctypes.c_size_t)),
+        ("partial_blocks", ctypes.c_int),
     ]
 
 
 def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda_factor=0.02,
-          wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, phase_mode=None):
+          wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, phase_mode=None, partial_blocks=False):
     """Run the reconstruction loop of llr_recon_exec (lib.rs:227-341) on CPU.
+
+    partial_blocks: ceil(vol/blk) block grid with truncated edge blocks (an
+    extension; the reference grid is floor(vol/blk), block.rs:112-118).
 
     meas: flat k-space measurements (complex64 -> FP32 path, complex128 -> FP64 oracle).
     shifts: (n_iter, 3) integer block shifts. Returns (image, err_hist, lambda, ksp_corr).
@@ -279,6 +300,7 @@ def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda
         p.srank = 0
         p.subspace = None
     p.shifts = shifts.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t))
+    p.partial_blocks = 1 if partial_blocks else 0
     x = np.zeros_like(meas)
     hist = np.zeros(max(1, shifts.shape[0]), np.float64)
     lam = ctypes.c_double()

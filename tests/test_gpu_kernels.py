@@ -188,7 +188,8 @@ def test_svt_vs_oracle(llr, m, n):
 
 
 # ------------------------------------------------------------- signal model
-@pytest.mark.parametrize("dims", [(64, 64, 64), (256, 256, 128), (8, 16, 4), (2, 512, 32), (128, 4, 256)])
+@pytest.mark.parametrize("dims", [(64, 64, 64), (256, 256, 128), (8, 16, 4), (2, 512, 32), (128, 4, 256),
+                                  (96, 48, 24), (192, 12, 6), (384, 3, 96), (6, 192, 12)])
 def test_fftn_vs_numpy(llr, dims):
     from llr_hip import signal_model as sm
 
@@ -212,7 +213,7 @@ def test_fft_round_trip_ones_kat(llr):
     assert np.abs(x - 1).max() < 1e-6
 
 
-@pytest.mark.parametrize("dims", [(16, 8, 32), (64, 64, 64), (128, 32, 16)])
+@pytest.mark.parametrize("dims", [(16, 8, 32), (64, 64, 64), (128, 32, 16), (96, 96, 48), (24, 192, 12)])
 @pytest.mark.parametrize("direction", [0, 1])
 def test_signal_model_vs_oracle(llr, dims, direction):
     from llr_hip import signal_model as sm

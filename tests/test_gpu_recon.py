@@ -22,7 +22,7 @@ def run_gpu(ds, **kw):
     cfg = ds["cfg"]
     r = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode=kw.get("llr_mode", "svt"),
                     rank=kw.get("rank", 0), lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
-                    subspace=ds["subspace"])
+                    subspace=ds["subspace"], partial_blocks=kw.get("partial_blocks", False))
     r.set_measurements(ds["meas"])
     hist = r.run()
     img = r.image()
@@ -37,7 +37,7 @@ def run_cpu(ds, dtype, **kw):
     return O.recon(ds["meas"].astype(dtype), ds["vol"], ds["nq"], cfg["blk"], ds["shifts"],
                    llr_mode={"svt": 1, "rank": 0}[kw.get("llr_mode", "svt")], rank=kw.get("rank", 0),
                    lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
-                   subspace=None if sub is None else sub.astype(dtype))
+                   subspace=None if sub is None else sub.astype(dtype), partial_blocks=kw.get("partial_blocks", False))
 
 
 def check(ds, label, **kw):
@@ -80,3 +80,17 @@ def test_recon_config1_parity(llr):
 def test_recon_residual_decreases(llr):
     img, hist, lam, kc = run_gpu(H.make(H.SMALL_SUBSPACE))
     assert hist[-1] < hist[0]
+
+
+@pytest.mark.parametrize("vol,blk", [((96, 48, 24), (10, 10, 10)), ((48, 24, 32), (6, 6, 6))])
+def test_recon_partial_edge_blocks_radix3(llr, vol, blk):
+    """Partial edge blocks on volumes with radix-3 FFT lengths (the uneven batch sizes)."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, vol=vol, blk=blk, n_iter=4))
+    check(ds, f"{vol} blk {blk} partial", partial_blocks=True)
+
+
+def test_recon_partial_equals_whole_when_divisible(llr):
+    ds = H.make(dict(H.SMALL_SUBSPACE, n_iter=3))
+    a = run_gpu(ds)[0]
+    b = run_gpu(ds, partial_blocks=True)[0]
+    assert np.array_equal(a, b)

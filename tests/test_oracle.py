@@ -425,3 +425,55 @@ def test_undersampling_and_mask():
     assert e == 48 - 8 and abs(us - 48 / 40) < 1e-12
     Y = y.reshape(nq, 2, 3, 4)
     assert np.all(Y[0, 0, 1] == 0) and np.all(Y[1, 1, 2] == 0)
+
+
+def _numpy_partial_llr(x, vol, nq, blk, shift, op):
+    """Truncated edge blocks by slicing: roll so block corners sit at multiples
+    of blk, cut each (possibly short) block, apply op to its Casorati matrix."""
+    nx, ny, nz = vol
+    v = x.reshape(nq, nz, ny, nx)
+    r = np.roll(v, (-shift[2], -shift[1], -shift[0]), axis=(1, 2, 3)).copy()
+    for z0 in range(0, nz, blk[2]):
+        for y0 in range(0, ny, blk[1]):
+            for x0 in range(0, nx, blk[0]):
+                sl = (slice(None), slice(z0, z0 + blk[2]), slice(y0, y0 + blk[1]), slice(x0, x0 + blk[0]))
+                blkv = r[sl]
+                M = blkv.reshape(nq, -1).T  # rows: voxels, x fastest
+                r[sl] = op(M).T.reshape(blkv.shape)
+    return np.roll(r, (shift[2], shift[1], shift[0]), axis=(1, 2, 3)).ravel()
+
+
+@pytest.mark.parametrize("vol,blk,shift", [((10, 9, 7), (4, 4, 4), (1, 3, 2)), ((6, 5, 9), (3, 2, 4), (0, 1, 3)),
+                                           ((8, 8, 8), (4, 4, 4), (2, 0, 1))])
+@pytest.mark.parametrize("mode", ["svt", "rank", "svt_subspace"])
+def test_partial_edge_blocks_match_sliced_svd(vol, blk, shift, mode):
+    nq = 6
+    N = int(np.prod(vol))
+    x = crandn(nq * N)
+    S = None
+    if mode == "svt_subspace":
+        S, _ = np.linalg.qr(crandn(nq, 3))
+    lam, rank = 1.5, 2
+
+    def op(M):
+        A = M @ S if S is not None else M
+        U, s, Vh = np.linalg.svd(A, full_matrices=False)
+        g = np.maximum(s - lam, 0) if mode != "rank" else np.where(np.arange(s.size) < rank, s, 0)
+        L = (U * g) @ Vh
+        return L @ S.conj().T if S is not None else L
+
+    ref = _numpy_partial_llr(x, vol, nq, blk, shift, op)
+    got = x.copy()
+    O.llr_partial(got, vol, nq, blk, shift, mode=0 if mode == "rank" else 1, rank=rank, lam=lam, subspace=S)
+    assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+def test_partial_blocks_equal_whole_blocks_when_divisible():
+    vol, nq, blk = (8, 8, 4), 5, (4, 4, 2)
+    N = int(np.prod(vol))
+    rng = np.random.default_rng(3)
+    meas = crandn(nq * N)
+    shifts = rng.integers(0, 4, size=(2, 3)) % np.array(blk)
+    a = O.recon(meas, vol, nq, blk, shifts, llr_mode=1, lambda_factor=0.05)[0]
+    b = O.recon(meas, vol, nq, blk, shifts, llr_mode=1, lambda_factor=0.05, partial_blocks=True)[0]
+    assert np.abs(a - b).max() < 1e-12 * np.abs(a).max()
