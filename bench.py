@@ -24,6 +24,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(stage, ds):
+    """HBM bytes per launch of a stage from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of the same workload (copy-calibrated), or None if there is none."""
+    v = ds["vol"]
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{v[0]}x{v[1]}x{v[2]}x{ds['nq']}_plain_fp32.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["stages"][stage]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def build_measurements(preset, torch, llr):
     """Synthetic data set of the preset with its measurements on the GPU."""
     import numpy as np
@@ -182,7 +194,7 @@ def main():
         dom = max(kernels, key=lambda k: stages[k]["total_ms"])
         ach = stages[dom]["alg_GBps"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, ds),
                 "alg_bytes_per_launch": nbytes[dom], "avg_launch_ms": stages[dom]["avg_ms"]}
 
     cpu = None
