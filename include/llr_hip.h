@@ -200,12 +200,29 @@ llr_status llr_calc_undersampling_factor(llr_ctx* ctx, const llr_cf32* y, size_t
 llr_status llr_under_sample_kspace(llr_ctx* ctx, llr_cf32* data, const size_t dims[3], size_t nq,
                                    const llr_cf32* mask);
 
+/* ------------------------------------------------------- diffusion scalars */
+/* The per-voxel part of calc_scalars (diffusion_models.rs:21-129): s0 = mean |x|
+ * over the b0 volumes (|g| <= b0_tol, data_import.rs:322-328), log-linear
+ * least-squares tensor over the other volumes (build_rhs :205-216, solve_tensor
+ * :191-202, build_design_matrix :221-253), its eigenvalues (solve_eigenvalues
+ * :160-186) and FA, MD, RD (:111-116). volumes: n_volumes x n_voxels complex;
+ * b_vecs: n_volumes x 3. scalars: 6 planes of n_voxels floats, in the order
+ * e1 >= e2 >= e3, FA, MD, RD (FA is NaN where all eigenvalues are 0, as in the
+ * reference). The fit and eigen-solve run in FP64. */
+llr_status llr_dti_scalars(llr_ctx* ctx, const llr_cf32* volumes, size_t n_voxels, size_t n_volumes,
+                           const float* b_vals, const float* b_vecs, float b0_tol, float* scalars);
+/* Same on device pointers (volumes, scalars); b_vals/b_vecs are host arrays.
+ * Synchronous with the context's stream. */
+llr_status llr_dti_scalars_device(llr_ctx* ctx, const llr_cf32* d_volumes, size_t n_voxels, size_t n_volumes,
+                                  const float* b_vals, const float* b_vecs, float b0_tol, float* d_scalars);
+
 /* ---------------------------------------------------- device-resident recon */
 #define LLR_MODE_RANK 0 /* hard rank truncation (low_rank.rs:10-69) */
 #define LLR_MODE_SVT 1  /* singular-value soft threshold */
 
 #define LLR_PREC_FP32 0  /* plain form: FP32 throughout */
-#define LLR_PREC_MIXED 1 /* BF16 MFMA Gram/projection + FP32 refinement + FP64 fallback */
+#define LLR_PREC_MIXED 1 /* BF16 MFMA Gram and projection (FP32 accumulation), one FP32 Jacobi
+                          refinement sweep, FP64 eigen-solve for blocks with Gram condition > 1e6 */
 
 typedef struct {
   size_t vol[3];         /* nx, ny, nz */
@@ -247,6 +264,11 @@ llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done);
 llr_status llr_recon_get_image(llr_recon* r, llr_cf32* out, size_t len, int on_device);
 /* Threshold used by the SVT step and the per-volume k-space corrections found. */
 llr_status llr_recon_info(llr_recon* r, double* lambda, float* ksp_corr);
+
+/* Mixed-precision statistics: LLR blocks whose K x K eigen-solve fell back to
+ * FP64 (Gram condition number above 1e6) and blocks processed, since creation.
+ * No counterpart in the reference (its SVD is FP32 only). */
+llr_status llr_recon_mixed_stats(llr_recon* r, unsigned long long* fp64_blocks, unsigned long long* blocks);
 
 /* Stage timing with HIP events on the context's stream (for roofline reports).
  * Stages: 0 LLR block kernel, 1 subspace projection, 2 wavelet, 3 FFT x pass,

@@ -518,6 +518,26 @@ llr_status llr_hard_project(llr_ctx* ctx, llr_cf32* x, const llr_cf32* y, size_t
   return download(x, dx, len * 8, ctx);
 }
 
+llr_status llr_dti_scalars(llr_ctx* ctx, const llr_cf32* volumes, size_t n_voxels, size_t n_volumes, const float* b_vals,
+                           const float* b_vecs, float b0_tol, float* scalars) {
+  LLR_CHECK(ctx && volumes && b_vals && b_vecs && scalars, "null argument");
+  LLR_HIP(hipSetDevice(ctx->device));
+  DevBuf dv, dout;
+  llr_status st;
+  if ((st = upload(dv, volumes, n_voxels * n_volumes * 8))) return st;
+  if ((st = upload(dout, nullptr, n_voxels * 6 * sizeof(float)))) return st;
+  if ((st = dti_scalars_device(ctx, dv.as<float2>(), n_voxels, n_volumes, b_vals, b_vecs, b0_tol, dout.as<float>())))
+    return st;
+  return download(scalars, dout, n_voxels * 6 * sizeof(float), ctx);
+}
+
+llr_status llr_dti_scalars_device(llr_ctx* ctx, const llr_cf32* d_volumes, size_t n_voxels, size_t n_volumes,
+                                  const float* b_vals, const float* b_vecs, float b0_tol, float* d_scalars) {
+  LLR_CHECK(ctx && d_volumes && b_vals && b_vecs && d_scalars, "null argument");
+  LLR_HIP(hipSetDevice(ctx->device));
+  return dti_scalars_device(ctx, (const float2*)d_volumes, n_voxels, n_volumes, b_vals, b_vecs, b0_tol, d_scalars);
+}
+
 llr_status llr_calc_undersampling_factor(llr_ctx* ctx, const llr_cf32* y, size_t len, double* factor,
                                          double* total_energy) {
   LLR_HIP(hipSetDevice(ctx->device));
@@ -557,8 +577,9 @@ struct llr_recon {
   size_t N = 0, nq = 0, D = 0;
   int nc = 0;       // columns of the LLR matrices (srank or nq)
   int wL = 0;       // wavelet levels
-  DevBuf X, Y, Cb, S, coef, t1, t2, partials, stats, hist, smax, best;
+  DevBuf X, Y, Cb, S, coef, t1, t2, partials, stats, hist, smax, best, counters;
   RampTables tabs;
+  unsigned long long llr_blocks = 0;  // blocks processed by the LLR step (mixed-form statistics)
   size_t nparts = 0;
   bool have_meas = false, have_init = false;
   double lambda = 0.0, energy = 0.0;
@@ -595,6 +616,26 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
   long long nb = 1;
   for (int a = 0; a < 3; ++a) nb *= (long long)(partial ? (p.vol[a] + p.blk[a] - 1) / p.blk[a] : p.vol[a] / p.blk[a]);
   llr_status st;
+  if (p.precision == LLR_PREC_MIXED) {
+    // BF16 MFMA projection / Gram, FP32 refinement sweep, FP64 eigen-solve for ill-conditioned blocks
+    unsigned* fp64n = r->counters.as<unsigned>();
+    if (mode != 2) r->llr_blocks += (unsigned long long)nb;
+    float2* F = p.srank ? r->Cb.as<float2>() : r->X.as<float2>();
+    if (p.srank) {
+      StageTimer t(r, 1);
+      st = project_bf16_device(ctx, r->X.as<float2>(), F, r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank);
+      if (st) return st;
+    }
+    {
+      StageTimer t(r, 0);
+      st = llr_block_run_mixed(ctx, F, p.vol, p.blk, shift, r->nc, 0, 0, nb, mode, (int)p.rank, lambda, smax, nullptr,
+                               mode != 2 ? fp64n : nullptr, partial);
+    }
+    if (st || mode == 2 || !p.srank) return st;
+    StageTimer t(r, 1);
+    return backproject_bf16_device(ctx, F, r->X.as<float2>(), r->S.as<float2>(), p.vol, p.blk, shift, (int)r->nq,
+                                   (int)p.srank, partial);
+  }
   if (p.srank) {
     {
       StageTimer t(r, 1);
@@ -684,8 +725,12 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
   if (nc > 8) return fail(LLR_ERR_UNSUPPORTED, "the fused LLR kernel handles up to 8 matrix columns");
   if (p.srank && !p.subspace) return fail(LLR_ERR_ARG, "subspace rank given without a subspace matrix");
   if (p.n_iter && !p.shifts) return fail(LLR_ERR_ARG, "shift sequence required");
-  if (p.precision != LLR_PREC_FP32)
-    return fail(LLR_ERR_UNSUPPORTED, "only the FP32 precision form is available in this build");
+  if (p.precision != LLR_PREC_FP32 && p.precision != LLR_PREC_MIXED) return fail(LLR_ERR_ARG, "unknown precision form");
+  if (p.precision == LLR_PREC_MIXED) {
+    if (llr_block_mixed_smem((long long)(p.blk[0] * p.blk[1] * p.blk[2]), nc) > 64 * 1024)
+      return fail(LLR_ERR_UNSUPPORTED, "block too large for the mixed-precision LLR kernel");
+    if (p.srank && p.nq > 128) return fail(LLR_ERR_UNSUPPORTED, "mixed form handles at most 128 volumes");
+  }
   LLR_HIP(hipSetDevice(ctx->device));
   llr_recon* r = new llr_recon();
   r->ctx = ctx;
@@ -723,6 +768,8 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
   if ((st = upload(r->hist, nullptr, std::max<size_t>(p.n_iter, 1) * 8))) return bail(st);
   if ((st = upload(r->smax, nullptr, 8))) return bail(st);
   if ((st = upload(r->best, nullptr, p.nq * 8))) return bail(st);
+  if ((st = upload(r->counters, nullptr, 8))) return bail(st);
+  LLR_HIP(hipMemset(r->counters.p, 0, 8));
   if (hipEventCreate(&r->ev0) != hipSuccess || hipEventCreate(&r->ev1) != hipSuccess)
     return bail(fail(LLR_ERR_HIP, "hipEventCreate failed"));
   *out = r;
@@ -847,6 +894,17 @@ llr_status llr_recon_info(llr_recon* r, double* lambda, float* ksp_corr) {
   if (!r) return fail(LLR_ERR_ARG, "null recon");
   if (lambda) *lambda = r->lambda;
   if (ksp_corr && !r->ksp_corr.empty()) std::memcpy(ksp_corr, r->ksp_corr.data(), r->ksp_corr.size() * sizeof(float));
+  return LLR_OK;
+}
+
+llr_status llr_recon_mixed_stats(llr_recon* r, unsigned long long* fp64_blocks, unsigned long long* blocks) {
+  if (!r) return fail(LLR_ERR_ARG, "null recon");
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  unsigned n = 0;
+  LLR_HIP(hipMemcpy(&n, r->counters.p, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (fp64_blocks) *fp64_blocks = n;
+  if (blocks) *blocks = r->llr_blocks;
   return LLR_OK;
 }
 

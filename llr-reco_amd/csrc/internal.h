@@ -35,6 +35,15 @@ int wavelet_max_level(const size_t* dims);
 void wavelet_geometry(const size_t* dims, int L, std::vector<std::array<size_t, 3>>& in_dims,
                       std::vector<std::array<size_t, 3>>& cdims);
 size_t wavelet_workspace_elems(const size_t* dims, int L, size_t* coef_elems, size_t* tmp_elems);
+llr_status project_bf16_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K);
+llr_status backproject_bf16_device(llr_ctx* ctx, const float2* C, float2* X, const float2* S, const size_t* vol,
+                                   const size_t* blk, const size_t* shift, int nq, int K, bool partial);
+llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const size_t* blk, const size_t* shift,
+                               int nc, int layout, long long block0, long long nblocks, int mode, int rank,
+                               float lambda, unsigned* smax_bits, float* svals, unsigned* fp64_count, bool partial);
+size_t llr_block_mixed_smem(long long m, int nc);
+llr_status dti_scalars_device(llr_ctx* ctx, const float2* vols, size_t n_vox, size_t nq, const float* bvals,
+                              const float* bvecs, float b0_tol, float* out);
 llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, const size_t* dims, float alpha, int L,
                                   float2* coef, float2* t1, float2* t2);
 

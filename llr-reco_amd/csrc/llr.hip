@@ -18,30 +18,12 @@
 #include <cfloat>
 #include <cmath>
 
+#include "block_common.h"
 #include "common.h"
 #include "internal.h"
 
 namespace {
 
-// ------------------------------------------------------------ lift / unlift
-struct GeomArgs {
-  long long vol[3], blk[3], grid[3], shift[3];
-  long long N;  // voxels per volume
-  long long m;  // voxels per block
-};
-
-// Voxel address of row i of block b, or -1 for a row past the end of the
-// volume (possible only on the ceil(vol/blk) grid of partial edge blocks).
-__device__ __forceinline__ long long block_voxel_addr(const GeomArgs& g, long long b, long long i) {
-  long long bx = b % g.grid[0], by = (b / g.grid[0]) % g.grid[1], bz = b / (g.grid[0] * g.grid[1]);
-  long long ix = i % g.blk[0], iy = (i / g.blk[0]) % g.blk[1], iz = i / (g.blk[0] * g.blk[1]);
-  long long sx = ix + bx * g.blk[0], sy = iy + by * g.blk[1], sz = iz + bz * g.blk[2];
-  if (sx >= g.vol[0] || sy >= g.vol[1] || sz >= g.vol[2]) return -1;
-  long long x = (sx + g.shift[0]) % g.vol[0];
-  long long y = (sy + g.shift[1]) % g.vol[1];
-  long long z = (sz + g.shift[2]) % g.vol[2];
-  return x + g.vol[0] * (y + g.vol[1] * z);
-}
 
 template <class E>
 __global__ void k_lift(const E* __restrict__ vols, E* __restrict__ mats, GeomArgs g, long long nvols,
@@ -135,151 +117,6 @@ __global__ void __launch_bounds__(256) k_backproject(const float2* __restrict__ 
   }
 }
 
-// ----------------------------------------------------- wavefront Jacobi (8x8)
-__device__ __forceinline__ float2 shfl2(float2 v, int src) {
-  return make_float2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
-}
-
-// partner of index i in round k of the circle-method round robin on 8 indices
-__device__ __forceinline__ int rr_partner(int i, int k) {
-  if (i == 7) return k;
-  if (i == k) return 7;
-  int p = (2 * k - i) % 7;
-  return p < 0 ? p + 7 : p;
-}
-
-struct Rot {
-  float cs, sn;
-  float2 u;  // unit phase of A[p][q]
-};
-
-// The modulus of A[p][q] is formed scaled (hypot-style): for off-diagonal
-// entries near the FP32 underflow range sqrt(x^2 + y^2) loses its digits and
-// the phase u = a_pq/|a_pq| would not be unit-modulus, making the rotation
-// non-unitary. Entries below 1e-9 of the diagonal pair are left alone.
-__device__ __forceinline__ Rot rot_params(float app, float aqq, float2 apq) {
-  Rot r;
-  float s = fmaxf(fabsf(apq.x), fabsf(apq.y));
-  if (s == 0.0f || s <= 1e-9f * (fabsf(app) + fabsf(aqq))) {
-    r.cs = 1.f;
-    r.sn = 0.f;
-    r.u = make_float2(1.f, 0.f);
-    return r;
-  }
-  float xs = apq.x / s, ys = apq.y / s;
-  float rn = 1.0f / sqrtf(xs * xs + ys * ys);
-  float a = s / rn;
-  r.u = make_float2(xs * rn, ys * rn);
-  float theta = (aqq - app) / (2.0f * a);
-  float at = fabsf(theta);
-  float t = (at > 1e18f) ? 0.5f / theta : ((theta >= 0.f ? 1.f : -1.f) / (at + sqrtf(theta * theta + 1.f)));
-  r.cs = 1.0f / sqrtf(t * t + 1.f);
-  r.sn = t * r.cs;
-  return r;
-}
-
-// Element (row `which`, col `which`-partner) of the 2x2 rotation J for the
-// pair containing index i: returns J[i][i] and J[partner][i].
-__device__ __forceinline__ void rot_entries(const Rot& R, bool i_is_p, float2* Jii, float2* Jpi) {
-  float2 cu = make_float2(R.u.x, -R.u.y);
-  if (i_is_p) {
-    *Jii = make_float2(R.cs, 0.f);
-    *Jpi = make_float2(-R.sn * cu.x, -R.sn * cu.y);
-  } else {
-    *Jii = make_float2(R.cs * cu.x, R.cs * cu.y);
-    *Jpi = make_float2(R.sn, 0.f);
-  }
-}
-
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// Jacobi eigen-solve of the Hermitian 8x8 matrix held one element per lane
-// (lane = r + 8c holds A[r][c]). On return a holds the (diagonal) eigenvalue
-// matrix and vv the eigenvectors (V[r][c], column c = eigenvector c).
-__device__ void wave_jacobi8(float2& a, float2& vv, int lane, int max_sweeps, float tol) {
-  const int r = lane & 7, c = lane >> 3;
-  vv = make_float2(r == c ? 1.f : 0.f, 0.f);
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    float off = wave_sum(r != c ? cnorm(a) : 0.f);
-    float tot = wave_sum(cnorm(a));
-    if (off <= tol * tol * tot || tot == 0.f) break;
-    for (int k = 0; k < 7; ++k) {
-      int pr = rr_partner(r, k), pc = rr_partner(c, k);
-      int p1 = min(r, pr), q1 = max(r, pr);
-      int p2 = min(c, pc), q2 = max(c, pc);
-      float app1 = __shfl(a.x, p1 * 9, 64), aqq1 = __shfl(a.x, q1 * 9, 64);
-      float2 apq1 = shfl2(a, p1 + 8 * q1);
-      float app2 = __shfl(a.x, p2 * 9, 64), aqq2 = __shfl(a.x, q2 * 9, 64);
-      float2 apq2 = shfl2(a, p2 + 8 * q2);
-      float2 a_r_pc = shfl2(a, r + 8 * pc);
-      float2 a_pr_c = shfl2(a, pr + 8 * c);
-      float2 a_pr_pc = shfl2(a, pr + 8 * pc);
-      float2 v_r_pc = shfl2(vv, r + 8 * pc);
-      Rot Rr = rot_params(app1, aqq1, apq1);
-      Rot Rc = rot_params(app2, aqq2, apq2);
-      float2 Jcc, Jpcc, Jrr, Jprr;
-      rot_entries(Rc, c == p2, &Jcc, &Jpcc);
-      rot_entries(Rr, r == p1, &Jrr, &Jprr);
-      float2 aj_r = cadd(cmul(a, Jcc), cmul(a_r_pc, Jpcc));
-      float2 aj_pr = cadd(cmul(a_pr_c, Jcc), cmul(a_pr_pc, Jpcc));
-      float2 an = cadd(cconjmul(Jrr, aj_r), cconjmul(Jprr, aj_pr));
-      vv = cadd(cmul(vv, Jcc), cmul(v_r_pc, Jpcc));
-      if (r != c && c == pr) an = make_float2(0.f, 0.f);
-      if (r == c) an.y = 0.f;
-      a = an;
-    }
-  }
-}
-
-// g(sigma_l) for eigenvalue index l, given all eigenvalues lam[0..7]
-__device__ __forceinline__ float shrink_weight(const float* lam, int l, int nc, int mode, int rank, float lambda) {
-  if (l >= nc) return 0.f;
-  if (mode == 0) {
-    int pos = 0;
-#pragma unroll
-    for (int o = 0; o < 8; ++o)
-      if (o < nc) pos += (lam[o] > lam[l] || (lam[o] == lam[l] && o < l)) ? 1 : 0;
-    return pos < rank ? 1.f : 0.f;
-  }
-  float s = sqrtf(fmaxf(lam[l], 0.f));
-  return (s > lambda && s > 0.f) ? (s - lambda) / s : 0.f;
-}
-
-// ---------------------------------------------------------- fused block kernel
-struct BlockArgs {
-  const float2* F;    // planes (layout 0) or matrices (layout 1)
-  float2* out;        // planes/matrices; with a subspace: nq image planes
-  const float2* S;    // nq x nc col-major, or null
-  GeomArgs g;
-  int nc;             // columns (<= 8)
-  int nq;             // output planes with a subspace
-  int layout;         // 0: shifted blocks of planes (plane stride g.N); 1: contiguous m x nc matrices
-  long long nblocks;  // blocks/matrices handled by this launch
-  long long block0;   // first block index
-  int per_xcd;        // blocks per XCD slot (XCD-aware remap)
-  int mode;           // 0 rank, 1 SVT, 2 singular values only
-  int rank;
-  float lambda;
-  unsigned* smax_bits;  // atomicMax of the largest singular value (float bits)
-  float* svals;         // optional [nblocks][8] singular values (unsorted)
-  int stage_lds;        // 1: F block staged in LDS
-};
-
-// element address of row i, column col of block b; -1 for a row outside the volume
-__device__ __forceinline__ long long elem_addr(const BlockArgs& A, long long b, long long i, int col) {
-  if (A.layout == 1) return b * A.g.m * A.nc + (long long)col * A.g.m + i;
-  long long v = block_voxel_addr(A.g, b, i);
-  return v < 0 ? -1 : (long long)col * A.g.N + v;
-}
-__device__ __forceinline__ float2 load_elem(const BlockArgs& A, long long b, long long i, int col) {
-  long long e = elem_addr(A, b, i, col);
-  return e < 0 ? make_float2(0.f, 0.f) : A.F[e];
-}
-
 __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
   extern __shared__ float2 smem[];
   const int tid = threadIdx.x;
@@ -352,7 +189,7 @@ __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
       if (r == c) a.y = 0.f;
     }
     float2 vv;
-    wave_jacobi8(a, vv, tid, 16, 2.5e-7f);
+    wave_jacobi8<float>(a, vv, tid, 16, 2.5e-7f);
     float lam[8];
 #pragma unroll
     for (int l = 0; l < 8; ++l) lam[l] = __shfl(a.x, 9 * l, 64);
@@ -535,21 +372,6 @@ __global__ void __launch_bounds__(256) k_apply_generic(float2* __restrict__ Am, 
 }  // namespace
 
 namespace llr {
-
-// Block grid floor(vol/blk) (block.rs:112-118), or ceil(vol/blk) with
-// truncated edge blocks when `partial`.
-GeomArgs make_geom(const size_t* vol, const size_t* blk, const size_t* shift, bool partial = false) {
-  GeomArgs g;
-  for (int a = 0; a < 3; ++a) {
-    g.vol[a] = (long long)vol[a];
-    g.blk[a] = (long long)blk[a];
-    g.grid[a] = (long long)(partial ? (vol[a] + blk[a] - 1) / blk[a] : vol[a] / blk[a]);
-    g.shift[a] = (long long)shift[a];
-  }
-  g.N = g.vol[0] * g.vol[1] * g.vol[2];
-  g.m = g.blk[0] * g.blk[1] * g.blk[2];
-  return g;
-}
 
 llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols, size_t eb, const size_t* vol,
                        const size_t* blk, const size_t* shift, size_t start, size_t n, bool unlift) {

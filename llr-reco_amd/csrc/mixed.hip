@@ -1,0 +1,481 @@
+// Mixed-precision form of the LLR step on gfx950 (LLR_PREC_MIXED).
+//
+//  - k_project_bf16 / k_backproject_bf16: the subspace projection C = X*S and
+//    back-projection X = L*S^H (lib.rs:285-289) as real GEMMs on BF16 MFMA
+//    (v_mfma_f32_16x16x16_bf16) with FP32 accumulation. A 16-voxel tile is
+//    the M dimension; the complex contraction is written as a real one over
+//    interleaved (re, im) pairs, so K = 2*nq (projection) or 2*srank
+//    (back-projection). Output columns are interleaved (re, im) too, so a
+//    lane and its xor-1 neighbour hold the two parts of the same complex
+//    value and each writes two voxels with one 16-byte store.
+//  - k_llr_block_mixed: per shifted block, the Gram matrix of the real
+//    m x 2n matrix [Re F | Im F] on BF16 MFMA (both MFMA operands are the
+//    same registers), an FP32 Jacobi eigen-solve of it giving V0, then the
+//    FP32 refinement: B = F*V0 (FP32), H = B^H B (FP32) and one Jacobi sweep
+//    on H. If H's eigenvalues have a condition number above 1e6 the n x n
+//    eigen-solve is redone in FP64 (Gram of B in FP64 and a full FP64
+//    Jacobi). The block is replaced by B * J diag(g) J^H * V0^H = F W.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "block_common.h"
+#include "common.h"
+#include "internal.h"
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16 bit pattern, round to nearest even (finite inputs)
+__device__ __forceinline__ short bf16_bits(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// D = A*B + C for 16x16 tiles, K = 16. Lane l supplies A[l%16][4(l/16)+j] and
+// B[4(l/16)+j][l%16] (j = 0..3) and receives D[4(l/16)+j][l%16].
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+constexpr int MAX_STEPS = 16;  // 2*nq <= 256
+
+// C[k][v] = sum_q X[q][v] S[q][k]. Row kk = 2q + c of the real operand is
+// (Re, Im)[c] of X[q]; output column n = 2k + o is (Re, Im)[o] of C[k].
+template <int K>
+__global__ void __launch_bounds__(256) k_project_bf16(const float2* __restrict__ X, float2* __restrict__ Cp,
+                                                      const float2* __restrict__ S, long long N, int nq) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = lane & 15, kq = lane >> 4;
+  const int NS = (2 * nq + 15) / 16;
+  bf16x4 bop[MAX_STEPS];
+#pragma unroll
+  for (int s = 0; s < MAX_STEPS; ++s) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kk = 16 * s + 4 * kq + j, q = kk >> 1, ic = kk & 1;
+      const int k = n >> 1, oc = n & 1;
+      float val = 0.f;
+      if (s < NS && q < nq && k < K) {
+        float2 sv = S[q + (long long)k * nq];
+        // Re C += Re X Re S - Im X Im S ; Im C += Re X Im S + Im X Re S
+        val = oc == 0 ? (ic == 0 ? sv.x : -sv.y) : (ic == 0 ? sv.y : sv.x);
+      }
+      bop[s][j] = bf16_bits(val);
+    }
+  }
+  const long long tiles = (N + 15) / 16;
+  for (long long t = (long long)blockIdx.x * 4 + wave; t < tiles; t += (long long)gridDim.x * 4) {
+    const long long v = t * 16 + n;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < MAX_STEPS; ++s) {
+      if (s < NS) {
+        const int q = 8 * s + 2 * kq;
+        float2 x0 = make_float2(0.f, 0.f), x1 = x0;
+        if (v < N) {
+          if (q < nq) x0 = X[(long long)q * N + v];
+          if (q + 1 < nq) x1 = X[(long long)(q + 1) * N + v];
+        }
+        bf16x4 a = {bf16_bits(x0.x), bf16_bits(x0.y), bf16_bits(x1.x), bf16_bits(x1.y)};
+        acc = mfma16(a, bop[s], acc);
+      }
+    }
+    // lane holds column n of voxels 4kq..4kq+3; its xor-1 partner holds the other part
+    float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
+    float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
+    const int k = n >> 1;
+    if (k < K) {
+      const long long v0 = t * 16 + 4 * kq + ((n & 1) ? 2 : 0);
+      float2 c0, c1;
+      if ((n & 1) == 0) {
+        c0 = make_float2(acc[0], p0);
+        c1 = make_float2(acc[1], p1);
+      } else {
+        c0 = make_float2(p2, acc[2]);
+        c1 = make_float2(p3, acc[3]);
+      }
+      float2* dst = Cp + (long long)k * N + v0;
+      if (v0 + 1 < N) {
+        dst[0] = c0;
+        dst[1] = c1;
+      } else if (v0 < N) {
+        dst[0] = c0;
+      }
+    }
+  }
+}
+
+// X[q][v] = sum_k L[k][v] conj(S[q][k]) for voxels inside the block grid's
+// coverage. Row kk = 2k + c is (Re, Im)[c] of L[k]; column n = 2(q - 8t) + o of
+// output tile t is (Re, Im)[o] of X[q].
+template <int K>
+__global__ void __launch_bounds__(256) k_backproject_bf16(const float2* __restrict__ L, float2* __restrict__ X,
+                                                          const float2* __restrict__ S, int nx, int ny, int nz,
+                                                          int cx, int cy, int cz, int sx, int sy, int sz, int nq) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = lane & 15, kq = lane >> 4;
+  const int NT = (2 * nq + 15) / 16;
+  bf16x4 bop[MAX_STEPS];
+#pragma unroll
+  for (int t = 0; t < MAX_STEPS; ++t) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kk = 4 * kq + j, k = kk >> 1, ic = kk & 1;
+      const int nn = 16 * t + n, q = nn >> 1, oc = nn & 1;
+      float val = 0.f;
+      if (t < NT && k < K && q < nq) {
+        float2 sv = S[q + (long long)k * nq];
+        // Re X += Re L Re S + Im L Im S ; Im X += Im L Re S - Re L Im S
+        val = oc == 0 ? (ic == 0 ? sv.x : sv.y) : (ic == 0 ? -sv.y : sv.x);
+      }
+      bop[t][j] = bf16_bits(val);
+    }
+  }
+  const long long N = (long long)nx * ny * nz;
+  const long long tiles = (N + 15) / 16;
+  for (long long t = (long long)blockIdx.x * 4 + wave; t < tiles; t += (long long)gridDim.x * 4) {
+    const long long v = t * 16 + n;
+    const int k0 = 2 * kq;
+    float2 l0 = make_float2(0.f, 0.f), l1 = l0;
+    if (v < N) {
+      if (k0 < K) l0 = L[(long long)k0 * N + v];
+      if (k0 + 1 < K) l1 = L[(long long)(k0 + 1) * N + v];
+    }
+    bf16x4 a = {bf16_bits(l0.x), bf16_bits(l0.y), bf16_bits(l1.x), bf16_bits(l1.y)};
+    // the two voxels this lane writes: 4kq + {0,1} (even n) or 4kq + {2,3} (odd n)
+    const long long w0 = t * 16 + 4 * kq + ((n & 1) ? 2 : 0);
+    bool cov[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      long long w = w0 + e;
+      cov[e] = false;
+      if (w < N) {
+        int x = (int)(w % nx), y = (int)((w / nx) % ny), z = (int)(w / ((long long)nx * ny));
+        int rx = x - sx, ry = y - sy, rz = z - sz;
+        if (rx < 0) rx += nx;
+        if (ry < 0) ry += ny;
+        if (rz < 0) rz += nz;
+        cov[e] = rx < cx && ry < cy && rz < cz;
+      }
+    }
+#pragma unroll
+    for (int tt = 0; tt < MAX_STEPS; ++tt) {
+      if (tt < NT) {
+        f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+        f32x4 acc = mfma16(a, bop[tt], z4);
+        float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
+        float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
+        const int q = 8 * tt + (n >> 1);
+        if (q < nq) {
+          float2 c0, c1;
+          if ((n & 1) == 0) {
+            c0 = make_float2(acc[0], p0);
+            c1 = make_float2(acc[1], p1);
+          } else {
+            c0 = make_float2(p2, acc[2]);
+            c1 = make_float2(p3, acc[3]);
+          }
+          float2* dst = X + (long long)q * N + w0;
+          if (cov[0]) dst[0] = c0;
+          if (cov[1]) dst[1] = c1;
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
+  extern __shared__ double2 smem_d[];  // 16-byte aligned base
+  __shared__ int s_fp64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nc = A.nc;
+  const long long m = A.g.m, mp = m + 1;
+  const long long fsz = (nc * mp + 1) & ~1LL;      // keeps what follows 16-byte aligned
+  const long long slot = xcd_slot(A.per_xcd);
+  if (slot >= A.nblocks) return;
+  const long long b = A.block0 + slot;
+  float2* Fs = (float2*)smem_d;                    // nc * mp: F, then B = F V0
+  float* Rp = (float*)(Fs + fsz);                  // 4 x 256 floats: per-wave BF16 Gram partials
+  double2* Gp64 = (double2*)Rp;                    // 256 double2 (FP64 fallback, aliases Rp)
+  float2* Gp = Fs + fsz + 512;                     // 256 FP32 Gram partials
+  float2* Vs = Gp + 256;                           // 64: V0
+  float2* Ms = Vs + 64;                            // 64: J g J^H V0^H
+
+  for (long long e = tid; e < m * nc; e += 256) {
+    long long i = e % m;
+    int col = (int)(e / m);
+    Fs[col * mp + i] = load_elem(A, b, i, col);
+  }
+  __syncthreads();
+
+  // 1. BF16 MFMA Gram of the real m x 2nc matrix [Re F | Im F] (16 rows per MFMA)
+  {
+    const int cidx = lane & 15, kq = lane >> 4;
+    const bool valid = cidx < 2 * nc, re = cidx < nc;
+    const float2* colp = Fs + (long long)(valid ? (re ? cidx : cidx - nc) : 0) * mp;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const long long tiles = (m + 15) / 16;
+    for (long long t = wave; t < tiles; t += 4) {
+      bf16x4 v;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        long long row = t * 16 + 4 * kq + j;
+        float x = 0.f;
+        if (valid && row < m) {
+          float2 f = colp[row];
+          x = re ? f.x : f.y;
+        }
+        v[j] = bf16_bits(x);
+      }
+      acc = mfma16(v, v, acc);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) Rp[wave * 256 + (4 * kq + j) * 16 + cidx] = acc[j];
+  }
+  __syncthreads();
+
+  // 2. wave 0: complex Gram G = F^H F from the real one, FP32 Jacobi -> V0
+  const int r = lane & 7, c = lane >> 3;
+  if (wave == 0) {
+    float2 a = make_float2(0.f, 0.f);
+    if (r < nc && c < nc) {
+      float rr = 0.f, ii = 0.f, ri = 0.f, ir = 0.f;
+      for (int w = 0; w < 4; ++w) {
+        const float* R = Rp + w * 256;
+        rr += R[r * 16 + c];
+        ii += R[(nc + r) * 16 + nc + c];
+        ri += R[r * 16 + nc + c];
+        ir += R[(nc + r) * 16 + c];
+      }
+      a = make_float2(rr + ii, r == c ? 0.f : ri - ir);
+    }
+    float2 v0;
+    wave_jacobi8<float>(a, v0, lane, 16, 2.5e-7f);
+    Vs[lane] = v0;  // V0[r][c]
+  }
+  __syncthreads();
+
+  // 3. B = F V0 in place (each thread owns whole rows)
+  for (long long rr = tid; rr < m; rr += 256) {
+    float2 f[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] = k < nc ? Fs[k * mp + rr] : make_float2(0.f, 0.f);
+    for (int k = 0; k < nc; ++k) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int cc = 0; cc < 8; ++cc)
+        if (cc < nc) acc = cadd(acc, cmul(f[cc], Vs[cc + 8 * k]));
+      Fs[k * mp + rr] = acc;
+    }
+  }
+  __syncthreads();
+
+  // 4. FP32 Gram of B: entry e = (i <= j) for thread t % NE, rows interleaved by t / NE
+  const int NE = nc * (nc + 1) / 2;
+  const int NH = 256 / NE;
+  int gi = 0, gj = 0;
+  {
+    int rem = tid % NE;
+    for (int jj = 0; jj < nc; ++jj) {
+      if (rem <= jj) {
+        gi = rem;
+        gj = jj;
+        break;
+      }
+      rem -= jj + 1;
+    }
+    const int h = tid / NE;
+    float2 acc = make_float2(0.f, 0.f);
+    if (h < NH)
+      for (long long row = h; row < m; row += NH) acc = cadd(acc, cconjmul(Fs[gi * mp + row], Fs[gj * mp + row]));
+    Gp[tid] = acc;
+  }
+  __syncthreads();
+
+  // 5. wave 0: one FP32 Jacobi sweep on H = B^H B; condition test
+  float2 hv = make_float2(0.f, 0.f), jv = make_float2(0.f, 0.f);
+  if (wave == 0) {
+    if (r < nc && c < nc) {
+      int i = min(r, c), j = max(r, c);
+      int e = j * (j + 1) / 2 + i;
+      float2 s = make_float2(0.f, 0.f);
+      for (int h = 0; h < NH; ++h) s = cadd(s, Gp[h * NE + e]);
+      hv = (r <= c) ? s : cconj(s);
+      if (r == c) hv.y = 0.f;
+    }
+    wave_jacobi8<float>(hv, jv, lane, 1, 0.f);
+    float lmax = 0.f, lmin = FLT_MAX;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      float x = __shfl(hv.x, 9 * l, 64);
+      if (l < nc) {
+        lmax = fmaxf(lmax, x);
+        lmin = fminf(lmin, x);
+      }
+    }
+    if (lane == 0) s_fp64 = (lmax > 0.f && !(lmin > 1e-6f * lmax)) ? 1 : 0;
+  }
+  __syncthreads();
+
+  // 6. FP64 fallback for ill-conditioned blocks: Gram of B and Jacobi in FP64
+  if (s_fp64) {
+    {
+      const int h = tid / NE;
+      double2 acc = make_double2(0.0, 0.0);
+      if (h < NH)
+        for (long long row = h; row < m; row += NH) {
+          float2 x = Fs[gi * mp + row], y = Fs[gj * mp + row];
+          double2 xd = make_double2(x.x, x.y), yd = make_double2(y.x, y.y);
+          acc = cadd(acc, cconjmul(xd, yd));
+        }
+      Gp64[tid] = acc;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      double2 a64 = make_double2(0.0, 0.0), v64;
+      if (r < nc && c < nc) {
+        int i = min(r, c), j = max(r, c);
+        int e = j * (j + 1) / 2 + i;
+        double2 s = make_double2(0.0, 0.0);
+        for (int h = 0; h < NH; ++h) s = cadd(s, Gp64[h * NE + e]);
+        a64 = (r <= c) ? s : make_double2(s.x, -s.y);
+        if (r == c) a64.y = 0.0;
+      }
+      wave_jacobi8<double>(a64, v64, lane, 30, 1e-15);
+      hv = make_float2((float)a64.x, 0.f);
+      jv = make_float2((float)v64.x, (float)v64.y);
+      if (lane == 0 && A.fp64_count) atomicAdd(A.fp64_count, 1u);
+    }
+  }
+
+  // 7. wave 0: weights, M = J diag(g) J^H V0^H
+  if (wave == 0) {
+    float lam[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) lam[l] = __shfl(hv.x, 9 * l, 64);
+    float lmax = 0.f;
+#pragma unroll
+    for (int l = 0; l < 8; ++l)
+      if (l < nc) lmax = fmaxf(lmax, lam[l]);
+    if (lane == 0 && A.smax_bits) atomicMax(A.smax_bits, __float_as_uint(sqrtf(fmaxf(lmax, 0.f))));
+    if (A.svals && lane < 8) A.svals[slot * 8 + lane] = sqrtf(fmaxf(lam[lane], 0.f));
+    if (A.mode != 2) {
+      float2 w = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int l = 0; l < 8; ++l) {
+        float g = shrink_weight(lam, l, nc, A.mode, A.rank, A.lambda);
+        float2 t = cmulc(shfl2(jv, r + 8 * l), shfl2(jv, c + 8 * l));
+        w.x += g * t.x;
+        w.y += g * t.y;
+      }
+      float2 mm = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int l = 0; l < 8; ++l) mm = cadd(mm, cmulc(shfl2(w, r + 8 * l), Vs[c + 8 * l]));
+      Ms[lane] = mm;  // M[r][c]
+    }
+  }
+  if (A.mode == 2) return;
+  __syncthreads();
+
+  // 8. out = B M (rows outside a partial edge block are not written)
+  for (long long rr = tid; rr < m; rr += 256) {
+    const long long vaddr = A.layout == 0 ? block_voxel_addr(A.g, b, rr) : 0;
+    if (vaddr < 0) continue;
+    float2 f[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] = k < nc ? Fs[k * mp + rr] : make_float2(0.f, 0.f);
+    for (int cc = 0; cc < nc; ++cc) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (k < nc) acc = cadd(acc, cmul(f[k], Ms[k + 8 * cc]));
+      A.out[A.layout == 0 ? (long long)cc * A.g.N + vaddr : elem_addr(A, b, rr, cc)] = acc;
+    }
+  }
+}
+
+}  // namespace
+
+namespace llr {
+
+llr_status project_bf16_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K) {
+  if (nq > 8 * MAX_STEPS) return fail(LLR_ERR_UNSUPPORTED, "BF16 projection handles at most 128 volumes");
+  unsigned grid = grid_for((N + 63) / 64, 1, 16384u);
+#define PB_CASE(KK) \
+  case KK: hipLaunchKernelGGL(k_project_bf16<KK>, dim3(grid), dim3(256), 0, ctx->stream, X, Cp, S, (long long)N, nq); break;
+  switch (K) {
+    PB_CASE(1) PB_CASE(2) PB_CASE(3) PB_CASE(4) PB_CASE(5) PB_CASE(6) PB_CASE(7) PB_CASE(8)
+    default: return fail(LLR_ERR_UNSUPPORTED, "BF16 projection handles subspace ranks up to 8");
+  }
+#undef PB_CASE
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+llr_status backproject_bf16_device(llr_ctx* ctx, const float2* C, float2* X, const float2* S, const size_t* vol,
+                                   const size_t* blk, const size_t* shift, int nq, int K, bool partial) {
+  size_t N = vol[0] * vol[1] * vol[2];
+  if (N >= 0xFFFFFFFFull) return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
+  if (nq > 8 * MAX_STEPS) return fail(LLR_ERR_UNSUPPORTED, "BF16 back-projection handles at most 128 volumes");
+  int cov[3], sh[3];
+  for (int a = 0; a < 3; ++a) {
+    cov[a] = (int)(partial ? vol[a] : (vol[a] / blk[a]) * blk[a]);
+    sh[a] = (int)(shift[a] % vol[a]);
+  }
+  unsigned grid = grid_for((N + 63) / 64, 1, 16384u);
+#define BB_CASE(KK)                                                                                                  \
+  case KK:                                                                                                           \
+    hipLaunchKernelGGL(k_backproject_bf16<KK>, dim3(grid), dim3(256), 0, ctx->stream, C, X, S, (int)vol[0],         \
+                       (int)vol[1], (int)vol[2], cov[0], cov[1], cov[2], sh[0], sh[1], sh[2], nq);                   \
+    break;
+  switch (K) {
+    BB_CASE(1) BB_CASE(2) BB_CASE(3) BB_CASE(4) BB_CASE(5) BB_CASE(6) BB_CASE(7) BB_CASE(8)
+    default: return fail(LLR_ERR_UNSUPPORTED, "BF16 back-projection handles subspace ranks up to 8");
+  }
+#undef BB_CASE
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+size_t llr_block_mixed_smem(long long m, int nc) {
+  size_t fsz = ((size_t)nc * (size_t)(m + 1) + 1) & ~(size_t)1;
+  return (fsz + 512 + 256 + 64 + 64) * sizeof(float2);
+}
+
+llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const size_t* blk, const size_t* shift,
+                               int nc, int layout, long long block0, long long nblocks, int mode, int rank,
+                               float lambda, unsigned* smax_bits, float* svals, unsigned* fp64_count, bool partial) {
+  GeomArgs g = make_geom(vol, blk, shift, partial);
+  if (nc < 1 || nc > 8) return fail(LLR_ERR_UNSUPPORTED, "mixed LLR kernel handles 1..8 columns");
+  if (nblocks <= 0) return LLR_OK;
+  size_t smem = llr_block_mixed_smem(g.m, nc);
+  if (smem > 64 * 1024) return fail(LLR_ERR_UNSUPPORTED, "block too large for the mixed LLR kernel");
+  BlockArgs A{};
+  A.F = F;
+  A.out = F;
+  A.S = nullptr;
+  A.g = g;
+  A.nc = nc;
+  A.nq = 0;
+  A.layout = layout;
+  A.nblocks = nblocks;
+  A.block0 = block0;
+  A.per_xcd = (int)((nblocks + 7) / 8);
+  A.mode = mode;
+  A.rank = rank;
+  A.lambda = lambda;
+  A.smax_bits = smax_bits;
+  A.svals = svals;
+  A.stage_lds = 1;
+  A.fp64_count = fp64_count;
+  hipLaunchKernelGGL(k_llr_block_mixed, dim3((unsigned)(8 * A.per_xcd)), dim3(256), smem, ctx->stream, A);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+}  // namespace llr

@@ -32,8 +32,10 @@ EXPORTS = [
     "llr_svt_batch", "llr_cgemm_strided_batched", "llr_rhs_cdgmm_batched", "llr_signal_model",
     "llr_signal_model_device", "llr_estimate_linear_correction", "llr_fftn_batch", "llr_fftshift", "llr_ifftshift",
     "llr_phase_shift3", "llr_wavelet_denoise", "llr_hard_project", "llr_calc_undersampling_factor",
-    "llr_under_sample_kspace", "llr_recon_create", "llr_recon_destroy", "llr_recon_set_measurements",
-    "llr_recon_set_initial", "llr_recon_run", "llr_recon_get_image", "llr_recon_info", "llr_recon_enable_timing",
+    "llr_under_sample_kspace", "llr_dti_scalars", "llr_dti_scalars_device", "llr_recon_create",
+    "llr_recon_destroy", "llr_recon_set_measurements",
+    "llr_recon_set_initial", "llr_recon_run", "llr_recon_get_image", "llr_recon_info", "llr_recon_mixed_stats",
+    "llr_recon_enable_timing",
     "llr_recon_stage_times",
 ]
 

@@ -140,6 +140,13 @@ class Recon:
         _lib.check(_lib.load().llr_recon_info(self.handle, ctypes.byref(lam), _lib.ptr(kc)))
         return lam.value, kc.reshape(self.nq, 3)
 
+    def mixed_stats(self):
+        """(blocks whose eigen-solve fell back to FP64, blocks processed) of the mixed form."""
+        f = ctypes.c_ulonglong()
+        n = ctypes.c_ulonglong()
+        _lib.check(_lib.load().llr_recon_mixed_stats(self.handle, ctypes.byref(f), ctypes.byref(n)))
+        return f.value, n.value
+
     def enable_timing(self, on=True):
         _lib.check(_lib.load().llr_recon_enable_timing(self.handle, int(on)))
 

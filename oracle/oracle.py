@@ -309,3 +309,49 @@ def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda
     if r:
         raise ValueError(f"oracle recon rejected the parameters (code {r})")
     return x, hist[:shifts.shape[0]], lam.value, kc.reshape(nq, 3)
+
+
+# --------------------------------------------------- diffusion_models.rs (FP64)
+def b0_mask(bvecs, tol=1e-6):
+    """data_import.rs:322-328: a volume is a b0 when |g| <= tol."""
+    g = np.asarray(bvecs, np.float64)
+    return np.sqrt(np.sum(g * g, axis=1)) <= tol
+
+
+def dti_design_matrix(bvals, bvecs):
+    """build_design_matrix (diffusion_models.rs:221-253): rows = diffusion-weighted
+    volumes, columns {Dxx, Dyy, Dzz, Dxy, Dxz, Dyz}: -b gx^2, -b gy^2, -b gz^2,
+    -2b gx gy, -2b gx gz, -2b gy gz."""
+    b = np.asarray(bvals, np.float64)
+    g = np.asarray(bvecs, np.float64)
+    return np.stack([-b * g[:, 0] ** 2, -b * g[:, 1] ** 2, -b * g[:, 2] ** 2, -2 * b * g[:, 0] * g[:, 1],
+                     -2 * b * g[:, 0] * g[:, 2], -2 * b * g[:, 1] * g[:, 2]], axis=1)
+
+
+def dti_scalars(vols, n_vox, bvals, bvecs, b0_tol=1e-6):
+    """calc_scalars (diffusion_models.rs:21-129) in FP64 on a flat (nq x n_vox)
+    complex data set: b0 average of |x| (:42-50), rhs = ln(clamp(|x|/s0, FLT_MIN, 1))
+    with s0 = max(b0av, FLT_MIN) (:205-216), least-squares tensor (:191-202),
+    eigh of the lower triangle (:160-186), e1 >= e2 >= e3, MD, RD and
+    FA = sqrt(3/2) |l - MD| / |l| (:111-116; 0/0 gives NaN as there).
+    Returns a (6, n_vox) float64 array: e1, e2, e3, FA, MD, RD."""
+    x = np.abs(np.asarray(vols).reshape(-1, n_vox)).astype(np.float64)
+    m = b0_mask(bvecs, b0_tol)
+    tiny = float(np.finfo(np.float32).tiny)
+    s0 = np.maximum(x[m].mean(axis=0), tiny)
+    dw = np.flatnonzero(~m)
+    A = dti_design_matrix(np.asarray(bvals)[dw], np.asarray(bvecs)[dw])
+    rhs = np.log(np.clip(x[dw] / s0, tiny, 1.0))
+    D, *_ = np.linalg.lstsq(A, rhs, rcond=None)  # (6, n_vox)
+    T = np.zeros((n_vox, 3, 3))
+    T[:, 0, 0], T[:, 1, 1], T[:, 2, 2] = D[0], D[1], D[2]
+    T[:, 1, 0], T[:, 2, 0], T[:, 2, 1] = D[3], D[4], D[5]
+    w = np.linalg.eigvalsh(T, UPLO="L")  # ascending
+    e1, e2, e3 = w[:, 2], w[:, 1], w[:, 0]
+    md = (e1 + e2 + e3) / 3
+    rd = (e2 + e3) / 2
+    with np.errstate(invalid="ignore", divide="ignore"):
+        num = np.sqrt((e1 - md) ** 2 + (e2 - md) ** 2 + (e3 - md) ** 2)
+        den = np.sqrt(e1 ** 2 + e2 ** 2 + e3 ** 2)
+        fa = np.sqrt(1.5) * num / den
+    return np.stack([e1, e2, e3, fa, md, rd])

@@ -327,3 +327,25 @@ def test_under_sample_kspace(llr):
     r = d.copy()
     O.under_sample_kspace(r, dims, nq, mask)
     assert np.array_equal(g, r)
+
+
+# --------------------------------------------------------------- DTI scalars
+@pytest.mark.parametrize("noise", [0.0, 0.05])
+def test_dti_scalars_vs_oracle(llr, noise):
+    from llr_hip import diffusion_models as dm
+    from llr_hip import phantom
+
+    vol = (24, 20, 16)
+    N = int(np.prod(vol))
+    bvals, bvecs = phantom.b_table(2, phantom.repulsion_directions(15, 3), 1000.0)
+    img = phantom.phantom_image(vol, bvals, bvecs, 5, 8)
+    if noise:
+        img = (img + noise * crandn(img.size)).astype(np.complex64)
+    got = dm.calc_scalars(img, N, bvals, bvecs)
+    ref = O.dti_scalars(img, N, bvals, bvecs)
+    for i, k in enumerate(dm.SCALAR_NAMES):
+        g, r = got[k].astype(np.float64), ref[i]
+        assert np.array_equal(np.isnan(g), np.isnan(r)), k
+        ok = ~np.isnan(r)
+        scale = np.abs(ref[:3]).max() if k != "fa" else 1.0
+        assert np.abs(g[ok] - r[ok]).max() <= 2e-6 * scale, k

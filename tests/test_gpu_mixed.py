@@ -1,0 +1,123 @@
+"""Mixed-precision form (BF16 MFMA Gram and projection with FP32 accumulation,
+one FP32 Jacobi refinement sweep, FP64 eigen-solve for blocks whose Gram
+condition number exceeds 1e6, FP64 residual) against the FP64 oracle.
+
+Bars (relative NRMSE vs the FP64 oracle on the same measurements and shifts):
+without a subspace the BF16 products only seed the eigenvectors, which the
+FP32 sweep refines, so the image stays at FP32 accuracy (<= 1e-4); with a
+subspace the BF16 projection and back-projection round every coefficient to
+8 bits of mantissa, bounded here by 5e-3. Config 3 also reports the per-voxel
+FA / MD error of the tensor fit (diffusion_models.rs:21-129) inside the object.
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def run(ds, precision, **kw):
+    from llr_hip import recon
+
+    cfg = ds["cfg"]
+    r = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode=kw.get("llr_mode", "svt"),
+                    rank=kw.get("rank", 0), lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
+                    subspace=ds["subspace"], precision=precision)
+    r.set_measurements(ds["meas"])
+    hist = r.run()
+    img = r.image()
+    lam, _ = r.info()
+    stats = r.mixed_stats()
+    r.close()
+    return img, hist, lam, stats
+
+
+def fp64(ds, **kw):
+    cfg = ds["cfg"]
+    sub = ds["subspace"]
+    return O.recon(ds["meas"].astype(np.complex128), ds["vol"], ds["nq"], cfg["blk"], ds["shifts"],
+                   llr_mode={"svt": 1, "rank": 0}[kw.get("llr_mode", "svt")], rank=kw.get("rank", 0),
+                   lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
+                   subspace=None if sub is None else sub.astype(np.complex128))
+
+
+def test_mixed_plain_small_matches_fp64(llr):
+    ds = H.make(H.SMALL_PLAIN)
+    img, hist, lam, (nf, nb) = run(ds, "mixed")
+    x64, h64, lam64, _ = fp64(ds)
+    e = H.nrmse(img, x64)
+    print(f"[mixed 32^3 x7] NRMSE vs fp64 {e:.3e}, fp64 eigen-solves {nf}/{nb}", flush=True)
+    assert nb == len(ds["shifts"]) * 64
+    assert abs(lam - lam64) <= 1e-4 * lam64
+    assert np.allclose(hist, h64, rtol=1e-3)
+    assert e <= 1e-4
+
+
+def test_mixed_subspace_wavelet_matches_fp64(llr):
+    ds = H.make(H.SMALL_SUBSPACE)
+    img, hist, lam, (nf, nb) = run(ds, "mixed")
+    x64, h64, lam64, _ = fp64(ds)
+    plain = run(ds, "fp32")[0]
+    e, ep = H.nrmse(img, x64), H.nrmse(plain, x64)
+    print(f"[mixed 32x32x16 x13 subspace] NRMSE vs fp64 {e:.3e} (plain fp32 {ep:.3e}), fp64 eigen-solves {nf}/{nb}",
+          flush=True)
+    assert np.all(np.isfinite(img))
+    assert abs(lam - lam64) <= 1e-2 * lam64
+    assert np.allclose(hist, h64, rtol=2e-2)
+    assert e <= 5e-3
+
+
+def test_mixed_rank_mode(llr):
+    ds = H.make(dict(H.SMALL_PLAIN, n_iter=4))
+    img = run(ds, "mixed", llr_mode="rank", rank=5)[0]
+    x64 = fp64(ds, llr_mode="rank", rank=5)[0]
+    assert H.nrmse(img, x64) <= 1e-4
+
+
+def test_config3_mixed_fa_md_error(llr):
+    """Config 3: config-1 inputs through the mixed form; image NRMSE and per-voxel FA/MD
+    error inside the object against the FP64 oracle reconstruction."""
+    from llr_hip import diffusion_models as dm
+    from llr_hip import phantom
+
+    ds = H.make("c1")
+    img, hist, lam, (nf, nb) = run(ds, "mixed")
+    x64 = fp64(ds)[0]
+    e = H.nrmse(img, x64)
+    N = int(np.prod(ds["vol"]))
+    got = dm.calc_scalars(img, N, ds["bvals"], ds["bvecs"])
+    ref = O.dti_scalars(x64, N, ds["bvals"], ds["bvecs"])
+    params = phantom.phantom_params(ds["cfg"]["seeds"][0], ds["cfg"]["n_ell"])
+    obj = phantom.phantom_labels(ds["vol"], params).ravel(order="F") > 0
+    ok = obj & np.isfinite(ref[3]) & np.isfinite(got["fa"])
+    dfa = np.abs(got["fa"][ok] - ref[3][ok])
+    dmd = np.abs(got["md"][ok] - ref[4][ok]) / np.maximum(np.abs(ref[4][ok]), 1e-12)
+    print(f"[config 3] image NRMSE vs fp64 {e:.3e}; FA |err| mean {dfa.mean():.2e} max {dfa.max():.2e}; "
+          f"MD rel err mean {dmd.mean():.2e} p99 {np.percentile(dmd, 99):.2e}; fp64 eigen-solves {nf}/{nb}", flush=True)
+    assert e <= 1e-4
+    assert dfa.mean() <= 1e-4 and np.percentile(dmd, 99) <= 1e-3
+
+
+def test_mixed_fp64_fallback_on_rank_deficient_blocks(llr):
+    """Volumes that are scaled copies of one image make every block's Gram matrix
+    rank one (condition number ~1e14): the FP64 eigen-solve must take over."""
+    from llr_hip import phantom
+
+    vol, nq = (32, 32, 16), 7
+    N = int(np.prod(vol))
+    base = phantom.phantom_image(vol, np.zeros(1), np.zeros((1, 3)), 41, 6)
+    scale = np.linspace(1.0, 0.4, nq) * np.exp(1j * np.linspace(0, 2, nq))
+    img = (scale[:, None] * base[None, :]).astype(np.complex64).ravel()
+    meas = img.copy()
+    H.oracle_forward(meas, vol, nq)
+    shifts = phantom.shift_sequence(vol, 3, 43)
+    ds = dict(meas=meas, vol=vol, nq=nq, shifts=shifts, subspace=None,
+              cfg=dict(blk=(8, 8, 8), lambda_factor=0.02, wavelet_alpha=0.0))
+    out, hist, lam, (nf, nb) = run(ds, "mixed")
+    x64 = fp64(ds)[0]
+    e = H.nrmse(out, x64)
+    print(f"[mixed rank-one blocks] NRMSE vs fp64 {e:.3e}, fp64 eigen-solves {nf}/{nb}", flush=True)
+    assert nf > nb // 4
+    assert e <= 1e-5

@@ -477,3 +477,75 @@ def test_partial_blocks_equal_whole_blocks_when_divisible():
     a = O.recon(meas, vol, nq, blk, shifts, llr_mode=1, lambda_factor=0.05)[0]
     b = O.recon(meas, vol, nq, blk, shifts, llr_mode=1, lambda_factor=0.05, partial_blocks=True)[0]
     assert np.abs(a - b).max() < 1e-12 * np.abs(a).max()
+
+
+# ---------------------------------------------------------------- DTI scalars
+def _dti_signals(D, bvals, bvecs, s0=1.0):
+    """Noise-free S = s0 exp(-b g^T D g) for each volume; one voxel per tensor."""
+    g = np.asarray(bvecs, np.float64)
+    b = np.asarray(bvals, np.float64)
+    return np.stack([s0 * np.exp(-b[q] * np.einsum("i,nij,j->n", g[q], D, g[q])) for q in range(len(b))])
+
+
+def _fa_alt(l1, l2, l3):
+    # the textbook form sqrt(1/2) sqrt((l1-l2)^2 + (l2-l3)^2 + (l3-l1)^2) / |l|
+    return np.sqrt(0.5) * np.sqrt((l1 - l2) ** 2 + (l2 - l3) ** 2 + (l3 - l1) ** 2) / np.sqrt(l1 ** 2 + l2 ** 2 + l3 ** 2)
+
+
+def _btable():
+    from llr_hip import phantom
+    dirs = phantom.repulsion_directions(20, 5)
+    return phantom.b_table(2, dirs, 1000.0)
+
+
+def test_dti_eig_kat():
+    # diffusion_models.rs:148-158: tensor entries [2, 1.5, 1, 0, 0, 0] -> eigenvalues {1, 1.5, 2} (scaled to mm^2/s)
+    bvals, bvecs = _btable()
+    D = np.diag([2.0, 1.5, 1.0])[None] * 1e-3
+    sig = _dti_signals(D, bvals, bvecs).astype(np.complex128)
+    out = O.dti_scalars(sig, 1, bvals, bvecs)
+    np.testing.assert_allclose(out[:3, 0], [2e-3, 1.5e-3, 1e-3], rtol=1e-10)
+    np.testing.assert_allclose(out[4, 0], 1.5e-3, rtol=1e-10)
+    np.testing.assert_allclose(out[5, 0], 1.25e-3, rtol=1e-10)
+    np.testing.assert_allclose(out[3, 0], _fa_alt(2.0, 1.5, 1.0), rtol=1e-9)
+
+
+def test_dti_recovers_random_full_tensors():
+    # a wrong column order, sign or off-diagonal factor of 2 in the design matrix breaks this
+    bvals, bvecs = _btable()
+    rng = np.random.default_rng(7)
+    Q, _ = np.linalg.qr(rng.standard_normal((16, 3, 3)))
+    lam = rng.uniform(0.2e-3, 2.0e-3, (16, 3))
+    D = np.einsum("nij,nj,nkj->nik", Q, lam, Q)
+    sig = _dti_signals(D, bvals, bvecs, s0=0.8).astype(np.complex128) * np.exp(1j * rng.uniform(0, 6, 16))
+    out = O.dti_scalars(sig, 16, bvals, bvecs)
+    ls = np.sort(lam, axis=1)[:, ::-1]
+    np.testing.assert_allclose(out[:3].T, ls, rtol=1e-9)
+    np.testing.assert_allclose(out[3], _fa_alt(*ls.T), rtol=1e-8)
+    np.testing.assert_allclose(out[4], ls.mean(axis=1), rtol=1e-10)
+
+
+def test_dti_phantom_fa_md_closed_form():
+    # the phantom's prolate tensors are built from (FA, MD): the fit must return them
+    from llr_hip import phantom
+    bvals, bvecs = _btable()
+    fa, md = np.array([0.2, 0.5, 0.8]), np.array([0.6e-3, 0.75e-3, 0.9e-3])
+    D = []
+    for f, m, u in zip(fa, md, [(0.1, 0.2), (0.5, 0.9), (0.7, 0.3)]):
+        E = phantom.eigenvector_matrix(phantom.sample_unit_sphere(*u))
+        D.append(E @ np.diag(phantom.fa_md_eigenvalues(f, m)) @ E.T)
+    out = O.dti_scalars(_dti_signals(np.array(D), bvals, bvecs).astype(np.complex128), 3, bvals, bvecs)
+    np.testing.assert_allclose(out[3], fa, rtol=1e-9)
+    np.testing.assert_allclose(out[4], md, rtol=1e-9)
+
+
+def test_dti_clamp_and_nan_rules():
+    # build_rhs clamps |x|/s0 to [FLT_MIN, 1]: signal above s0 -> ln 1 = 0 -> zero tensor -> FA = 0/0 = NaN
+    bvals, bvecs = _btable()
+    nq = len(bvals)
+    sig = np.ones((nq, 2), np.complex128)
+    sig[:2, 1] = 0.0  # b0 = 0 -> s0 = FLT_MIN, every ratio clamps to 1
+    out = O.dti_scalars(sig, 2, bvals, bvecs)
+    assert np.all(out[[0, 1, 2, 4, 5]] == 0.0)
+    assert np.all(np.isnan(out[3]))
+    assert np.array_equal(O.b0_mask([[0, 0, 0], [1e-7, 0, 0], [0, 1, 0]]), [True, True, False])
