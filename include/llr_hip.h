@@ -278,6 +278,31 @@ llr_status llr_recon_mixed_stats(llr_recon* r, unsigned long long* fp64_blocks, 
 llr_status llr_recon_enable_timing(llr_recon* r, int enable);
 llr_status llr_recon_stage_times(llr_recon* r, double* total_ms, size_t* launches);
 
+/* ---------------------------------------------------- batched reconstructions */
+/* Many reconstructions of uneven size (no counterpart in the reference, which
+ * runs one llr_recon_exec at a time). Each member has its own parameters
+ * (sizes, volumes, subspace, mode, threshold, shifts); members share n_iter,
+ * run the FP32 form without residual_stop, and use partial edge blocks. One
+ * iteration of every member is captured as a HIP graph: a projection per
+ * member, ONE persistent LLR launch over the blocks of all members, then each
+ * member's back-projection, wavelet, forward model, hard projection, adjoint
+ * and residual on its own stream; the graph is replayed n_iter times.
+ * autotune != 0: at the first run each member picks its block size from
+ * {6,8,10}^3, the LLR kernel's LDS staging and the FFT tiles per workgroup by
+ * timing them on its own data. */
+typedef struct llr_recon_batch llr_recon_batch;
+llr_status llr_recon_batch_create(llr_ctx* ctx, const llr_recon_params* params, size_t n_recons, int autotune,
+                                  llr_recon_batch** out);
+void llr_recon_batch_destroy(llr_recon_batch* batch);
+llr_status llr_recon_batch_set_measurements(llr_recon_batch* batch, size_t index, const llr_cf32* y, size_t len,
+                                            int on_device);
+/* err_hist: n_recons x n_iter relative residuals (or null). */
+llr_status llr_recon_batch_run(llr_recon_batch* batch, double* err_hist);
+llr_status llr_recon_batch_get_image(llr_recon_batch* batch, size_t index, llr_cf32* out, size_t len, int on_device);
+/* What the autotuner chose for a member (and its SVT threshold). */
+llr_status llr_recon_batch_tuning(llr_recon_batch* batch, size_t index, size_t blk[3], int fft_tpw[3],
+                                  int* llr_stage, double* lambda);
+
 #ifdef __cplusplus
 }
 #endif

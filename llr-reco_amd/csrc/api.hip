@@ -807,12 +807,15 @@ llr_status llr_recon_set_initial(llr_recon* r, const llr_cf32* x, size_t len, in
   return LLR_OK;
 }
 
-llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done) {
-  if (!r) return fail(LLR_ERR_ARG, "null recon");
-  if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set");
+}  // extern "C"
+
+namespace llr {
+
+// Set-up of a run (lib.rs:224-258): total energy, linear k-space corrections
+// and ramp tables, zero-filled adjoint (unless warm-started), SVT threshold.
+static llr_status recon_setup(llr_recon* r) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
-  LLR_HIP(hipSetDevice(ctx->device));
   llr_status st;
   // total energy (lib.rs:224, 409-417)
   {
@@ -857,6 +860,21 @@ llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done) {
       r->lambda = p.lambda_factor * (double)smax;
     }
   }
+  return LLR_OK;
+}
+
+}  // namespace llr
+
+extern "C" {
+
+llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done) {
+  if (!r) return fail(LLR_ERR_ARG, "null recon");
+  if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set");
+  llr_ctx* ctx = r->ctx;
+  const llr_recon_params& p = r->p;
+  LLR_HIP(hipSetDevice(ctx->device));
+  llr_status st = recon_setup(r);
+  if (st) return st;
   size_t it = 0;
   for (; it < p.n_iter; ++it) {
     if ((st = recon_iteration(r, it))) return st;
@@ -924,6 +942,393 @@ llr_status llr_recon_stage_times(llr_recon* r, double* total_ms, size_t* launche
     if (total_ms) total_ms[i] = r->stage_ms[i];
     if (launches) launches[i] = r->stage_n[i];
   }
+  return LLR_OK;
+}
+
+}  // extern "C"
+
+// ======================================================= batched reconstructions
+// Many reconstructions of uneven size run together: each member keeps its own
+// buffers and stream; one iteration of all members is captured once as a HIP
+// graph (fork per member: projection -> join: one persistent LLR launch over
+// the blocks of every member -> fork: back-projection, wavelet, forward model,
+// hard projection, adjoint, residual -> join: iteration counter += 1) and the
+// graph is replayed n_iter times. Every per-iteration quantity (the block
+// shift, the residual slot) is read on the device from the iteration counter.
+struct llr_recon_batch {
+  llr_ctx* ctx = nullptr;               // caller's context (device)
+  hipStream_t s0 = nullptr;             // capture / replay stream
+  std::vector<llr_ctx*> mctx;           // one private context (stream, tables) per member
+  std::vector<llr_recon*> rec;
+  std::vector<DevBuf*> shifts, tmp;     // device shift tables (int n_iter x 3), reduction scratch
+  DevBuf iter, tune_buf;
+  MultiPlan plan;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_mid = nullptr;
+  std::vector<hipEvent_t> ev_a, ev_b;
+  int autotune = 0;
+  bool tuned = false;
+  size_t n_iter = 0;
+  std::vector<int> stage;               // LLR LDS staging per member (-1 auto)
+};
+
+namespace llr {
+
+static const size_t kBlockCandidates[3] = {6, 8, 10};
+
+// Time `fn` (launches on stream s) as the minimum of `reps` runs, in ms.
+template <class F>
+static llr_status time_min(hipStream_t s, int reps, F fn, float* ms_out) {
+  hipEvent_t a, b;
+  LLR_HIP(hipEventCreate(&a));
+  LLR_HIP(hipEventCreate(&b));
+  float best = 1e30f;
+  llr_status st = LLR_OK;
+  for (int i = 0; i < reps && !st; ++i) {
+    (void)hipEventRecord(a, s);
+    st = fn();
+    (void)hipEventRecord(b, s);
+    if (!st && hipEventSynchronize(b) == hipSuccess) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, a, b);
+      best = std::min(best, ms);
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms_out = best;
+  return st;
+}
+
+// Autotuner, per member, after its set-up: block size from {6,8,10}^3 (those
+// that fit the volume with more rows than columns), by the time of one LLR
+// step per voxel; LDS staging of the block kernel; FFT tiles per workgroup per
+// axis from {1, 2, 4, 8}. The LLR step is timed on the coefficient planes
+// (subspace) or in singular-value-only mode (no subspace), so the iterate is
+// not modified; FFT passes are timed out of place into a scratch buffer.
+static llr_status batch_autotune(llr_recon_batch* B) {
+  for (size_t j = 0; j < B->rec.size(); ++j) {
+    llr_recon* r = B->rec[j];
+    llr_ctx* ctx = r->ctx;
+    llr_recon_params& p = r->p;
+    const size_t zero[3] = {0, 0, 0};
+    float best_t = 1e30f;
+    size_t best_b = p.blk[0];
+    double best_lam = r->lambda;
+    for (size_t bc : kBlockCandidates) {
+      if (bc > p.vol[0] || bc > p.vol[1] || bc > p.vol[2] || bc * bc * bc <= (size_t)r->nc) continue;
+      size_t blk[3] = {bc, bc, bc};
+      for (int a = 0; a < 3; ++a) p.blk[a] = bc;
+      double lam = p.svt_lambda;
+      if (p.llr_mode == LLR_MODE_SVT && p.svt_lambda <= 0) {
+        LLR_HIP(hipMemsetAsync(r->smax.p, 0, 8, ctx->stream));
+        llr_status st = recon_llr(r, zero, 2, 0.f, r->smax.as<unsigned>());
+        if (st) return st;
+        unsigned bits = 0;
+        LLR_HIP(hipStreamSynchronize(ctx->stream));
+        LLR_HIP(hipMemcpy(&bits, r->smax.p, 4, hipMemcpyDeviceToHost));
+        float smax;
+        std::memcpy(&smax, &bits, 4);
+        lam = p.lambda_factor * (double)smax;
+      }
+      long long nb = 1;
+      for (int a = 0; a < 3; ++a) nb *= (long long)((p.vol[a] + bc - 1) / bc);
+      float ms = 0.f;
+      llr_status st = time_min(ctx->stream, 2, [&]() -> llr_status {
+        if (p.srank) {
+          llr_status s2 = project_device(ctx, r->X.as<float2>(), r->Cb.as<float2>(), r->S.as<float2>(), r->N,
+                                         (int)r->nq, (int)p.srank);
+          if (s2) return s2;
+          return llr_block_run(ctx, r->Cb.as<float2>(), r->Cb.as<float2>(), nullptr, 0, p.vol, blk, zero, r->nc, 0,
+                               0, nb, p.llr_mode, (int)p.rank, (float)lam, nullptr, nullptr, true);
+        }
+        return llr_block_run(ctx, r->X.as<float2>(), r->X.as<float2>(), nullptr, 0, p.vol, blk, zero, r->nc, 0, 0,
+                             nb, 2, 0, 0.f, nullptr, nullptr, true);
+      }, &ms);
+      if (st) return st;
+      if (ms < best_t * 0.97f) {  // prefer the earlier (smaller) block unless clearly slower
+        best_t = ms;
+        best_b = bc;
+        best_lam = lam;
+      }
+    }
+    for (int a = 0; a < 3; ++a) p.blk[a] = best_b;
+    r->lambda = best_lam;
+    // LDS staging of the block kernel for the chosen size
+    {
+      size_t blk[3] = {best_b, best_b, best_b};
+      long long nb = 1;
+      for (int a = 0; a < 3; ++a) nb *= (long long)((p.vol[a] + best_b - 1) / best_b);
+      float t_on = 0.f, t_off = 0.f;
+      for (int stg = 1; stg >= 0; --stg) {
+        ctx->llr_stage = stg;
+        llr_status st = time_min(ctx->stream, 2, [&]() -> llr_status {
+          return llr_block_run(ctx, p.srank ? r->Cb.as<float2>() : r->X.as<float2>(),
+                               p.srank ? r->Cb.as<float2>() : r->X.as<float2>(), nullptr, 0, p.vol, blk, zero, r->nc,
+                               0, 0, nb, 2, 0, 0.f, nullptr, nullptr, true);
+        }, stg ? &t_on : &t_off);
+        if (st) return st;
+      }
+      ctx->llr_stage = (t_off < 0.9f * t_on) ? 0 : 1;
+      B->stage[j] = ctx->llr_stage;
+    }
+    // FFT tiles per workgroup, per axis (out of place into the scratch buffer)
+    const RampTables& T = r->tabs;
+    for (int a = 0; a < 3; ++a) {
+      float bt = 1e30f;
+      int bw = 4;
+      for (int w : {1, 2, 4, 8}) {
+        ctx->fft_tpw[a] = w;
+        float ms = 0.f;
+        float sc = (float)(1.0 / std::sqrt((double)p.vol[a]));
+        llr_status st = time_min(ctx->stream, 2, [&]() -> llr_status {
+          return fft_axis_pass(ctx, r->X.as<float2>(), B->tune_buf.as<float2>(), p.vol, r->nq, a, false,
+                               T.pk[a].as<float2>(), false, T.pi[a].as<float2>(), true, sc);
+        }, &ms);
+        if (st) return st;
+        if (ms < bt * 0.97f) {
+          bt = ms;
+          bw = w;
+        }
+      }
+      ctx->fft_tpw[a] = bw;
+    }
+  }
+  B->tuned = true;
+  return LLR_OK;
+}
+
+// Work of member j before the LLR launch (projection onto the subspace).
+static llr_status batch_pre(llr_recon* r) {
+  if (!r->p.srank) return LLR_OK;
+  return project_device(r->ctx, r->X.as<float2>(), r->Cb.as<float2>(), r->S.as<float2>(), r->N, (int)r->nq,
+                        (int)r->p.srank);
+}
+
+// Work of member j after the LLR launch: back-projection, wavelet, forward
+// model, hard projection + residual, adjoint, residual into hist[*iter].
+static llr_status batch_post(llr_recon* r, const unsigned* iter, double* tmp) {
+  llr_ctx* ctx = r->ctx;
+  const llr_recon_params& p = r->p;
+  const size_t zero[3] = {0, 0, 0};
+  llr_status st;
+  if (p.srank &&
+      (st = backproject_device(ctx, r->Cb.as<float2>(), r->X.as<float2>(), r->S.as<float2>(), p.vol, p.blk, zero,
+                               (int)r->nq, (int)p.srank, true)))
+    return st;
+  if (p.wavelet_alpha > 0 &&
+      (st = wavelet_denoise_device(ctx, r->X.as<float2>(), r->nq, p.vol, (float)p.wavelet_alpha, r->wL,
+                                   r->coef.as<float2>(), r->t1.as<float2>(), r->t2.as<float2>())))
+    return st;
+  float2* X = r->X.as<float2>();
+  const RampTables& T = r->tabs;
+  float sx = (float)(1.0 / std::sqrt((double)p.vol[0])), sy = (float)(1.0 / std::sqrt((double)p.vol[1]));
+  if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, false, T.pk[0].as<float2>(), false, T.pi[0].as<float2>(), true,
+                          sx)))
+    return st;
+  if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, false, T.pk[1].as<float2>(), false, T.pi[1].as<float2>(), true,
+                          sy)))
+    return st;
+  size_t np = 0;
+  if ((st = fft_zproj(ctx, X, r->Y.as<float2>(), p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(),
+                      r->partials.as<double>(), &np)))
+    return st;
+  if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, true, T.pi[1].as<float2>(), false, T.pk[1].as<float2>(), true,
+                          sy)))
+    return st;
+  if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false, T.pk[0].as<float2>(), true,
+                          sx)))
+    return st;
+  return reduce_at(ctx, r->partials.as<double>(), np, r->hist.as<double>(), iter, tmp);
+}
+
+static llr_status batch_capture(llr_recon_batch* B) {
+  if (B->exec) {
+    (void)hipGraphExecDestroy(B->exec);
+    B->exec = nullptr;
+  }
+  if (B->graph) {
+    (void)hipGraphDestroy(B->graph);
+    B->graph = nullptr;
+  }
+  LLR_HIP(hipStreamBeginCapture(B->s0, hipStreamCaptureModeRelaxed));
+  llr_status st = LLR_OK;
+  const size_t n = B->rec.size();
+  unsigned* iter = B->iter.as<unsigned>();
+  if (hipEventRecord(B->ev_fork, B->s0) != hipSuccess) st = fail(LLR_ERR_HIP, "event record in capture");
+  for (size_t j = 0; j < n && !st; ++j) {
+    hipStream_t sj = B->mctx[j]->stream;
+    if (hipStreamWaitEvent(sj, B->ev_fork, 0) != hipSuccess) st = fail(LLR_ERR_HIP, "stream wait in capture");
+    if (!st) st = batch_pre(B->rec[j]);
+    if (!st && (hipEventRecord(B->ev_a[j], sj) != hipSuccess || hipStreamWaitEvent(B->s0, B->ev_a[j], 0) != hipSuccess))
+      st = fail(LLR_ERR_HIP, "join in capture");
+  }
+  if (!st) st = llr_multi_launch(B->s0, B->plan, iter);
+  if (!st && hipEventRecord(B->ev_mid, B->s0) != hipSuccess) st = fail(LLR_ERR_HIP, "event record in capture");
+  for (size_t j = 0; j < n && !st; ++j) {
+    hipStream_t sj = B->mctx[j]->stream;
+    if (hipStreamWaitEvent(sj, B->ev_mid, 0) != hipSuccess) st = fail(LLR_ERR_HIP, "stream wait in capture");
+    if (!st) st = batch_post(B->rec[j], iter, B->tmp[j]->as<double>());
+    if (!st && (hipEventRecord(B->ev_b[j], sj) != hipSuccess || hipStreamWaitEvent(B->s0, B->ev_b[j], 0) != hipSuccess))
+      st = fail(LLR_ERR_HIP, "join in capture");
+  }
+  if (!st) st = advance_counter(B->s0, iter);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(B->s0, &g);
+  if (st) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+  B->graph = g;
+  LLR_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+  return LLR_OK;
+}
+
+}  // namespace llr
+
+extern "C" {
+
+void llr_recon_batch_destroy(llr_recon_batch* B) {
+  if (!B) return;
+  (void)hipSetDevice(B->ctx->device);
+  if (B->s0) (void)hipStreamSynchronize(B->s0);
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  for (llr_recon* r : B->rec) llr_recon_destroy(r);
+  for (llr_ctx* c : B->mctx) llr_ctx_destroy(c);
+  for (DevBuf* d : B->shifts) delete d;
+  for (DevBuf* d : B->tmp) delete d;
+  for (hipEvent_t e : B->ev_a) (void)hipEventDestroy(e);
+  for (hipEvent_t e : B->ev_b) (void)hipEventDestroy(e);
+  if (B->ev_fork) (void)hipEventDestroy(B->ev_fork);
+  if (B->ev_mid) (void)hipEventDestroy(B->ev_mid);
+  if (B->s0) (void)hipStreamDestroy(B->s0);
+  delete B;
+}
+
+llr_status llr_recon_batch_create(llr_ctx* ctx, const llr_recon_params* params, size_t n, int autotune,
+                                  llr_recon_batch** out) {
+  if (!ctx || !params || !out || n == 0) return fail(LLR_ERR_ARG, "null argument or empty batch");
+  LLR_HIP(hipSetDevice(ctx->device));
+  const size_t n_iter = params[0].n_iter;
+  for (size_t j = 0; j < n; ++j) {
+    if (params[j].n_iter != n_iter) return fail(LLR_ERR_ARG, "batch members must share n_iter");
+    if (params[j].residual_stop > 0) return fail(LLR_ERR_ARG, "residual_stop is not available in a batch");
+    if (params[j].precision != LLR_PREC_FP32) return fail(LLR_ERR_UNSUPPORTED, "batches run the FP32 form");
+  }
+  llr_recon_batch* B = new llr_recon_batch();
+  B->ctx = ctx;
+  B->autotune = autotune;
+  B->n_iter = n_iter;
+  auto bail = [&](llr_status s) {
+    llr_recon_batch_destroy(B);
+    return s;
+  };
+  if (hipStreamCreateWithFlags(&B->s0, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&B->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&B->ev_mid, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(LLR_ERR_HIP, "stream/event creation failed"));
+  llr_status st;
+  size_t maxD = 0;
+  for (size_t j = 0; j < n; ++j) {
+    llr_ctx* c = nullptr;
+    if ((st = llr_ctx_create(ctx->device, &c))) return bail(st);
+    B->mctx.push_back(c);
+    llr_recon_params p = params[j];
+    p.partial_blocks = 1;  // uneven sizes: edge blocks are partial
+    llr_recon* r = nullptr;
+    if ((st = llr_recon_create(c, &p, &r))) return bail(st);
+    B->rec.push_back(r);
+    std::vector<int> sh(3 * n_iter);
+    for (size_t i = 0; i < 3 * n_iter; ++i) sh[i] = (int)(p.shifts[i] % p.vol[i % 3]);
+    DevBuf* d = new DevBuf();
+    B->shifts.push_back(d);
+    if ((st = upload(*d, sh.data(), std::max<size_t>(sh.size(), 3) * sizeof(int)))) return bail(st);
+    DevBuf* t = new DevBuf();
+    B->tmp.push_back(t);
+    if ((st = upload(*t, nullptr, 256 * sizeof(double)))) return bail(st);
+    hipEvent_t a, b;
+    if (hipEventCreateWithFlags(&a, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(LLR_ERR_HIP, "event creation failed"));
+    B->ev_a.push_back(a);
+    B->ev_b.push_back(b);
+    maxD = std::max(maxD, r->D);
+  }
+  B->stage.assign(n, -1);
+  if ((st = upload(B->iter, nullptr, 16))) return bail(st);
+  if (autotune && (st = upload(B->tune_buf, nullptr, maxD * 8))) return bail(st);
+  *out = B;
+  return LLR_OK;
+}
+
+llr_status llr_recon_batch_set_measurements(llr_recon_batch* B, size_t i, const llr_cf32* y, size_t len,
+                                            int on_device) {
+  if (!B || i >= B->rec.size()) return fail(LLR_ERR_ARG, "no such batch member");
+  return llr_recon_set_measurements(B->rec[i], y, len, on_device);
+}
+
+llr_status llr_recon_batch_run(llr_recon_batch* B, double* err_hist) {
+  if (!B) return fail(LLR_ERR_ARG, "null batch");
+  LLR_HIP(hipSetDevice(B->ctx->device));
+  llr_status st;
+  for (llr_recon* r : B->rec) {
+    if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set for every member");
+    if ((st = recon_setup(r))) return st;
+  }
+  if (B->autotune && !B->tuned && (st = batch_autotune(B))) return st;
+  std::vector<MemberSpec> ms(B->rec.size());
+  for (size_t j = 0; j < B->rec.size(); ++j) {
+    llr_recon* r = B->rec[j];
+    const llr_recon_params& p = r->p;
+    MemberSpec& m = ms[j];
+    m.F = p.srank ? r->Cb.as<float2>() : r->X.as<float2>();
+    for (int a = 0; a < 3; ++a) {
+      m.vol[a] = p.vol[a];
+      m.blk[a] = p.blk[a];
+    }
+    m.nc = r->nc;
+    m.mode = p.llr_mode;
+    m.rank = (int)p.rank;
+    m.lambda = (float)r->lambda;
+    m.d_shifts = B->shifts[j]->as<int>();
+    m.partial = true;
+    m.stage = B->stage[j];
+  }
+  if ((st = llr_multi_plan(B->plan, ms))) return st;
+  for (llr_recon* r : B->rec) LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  if ((st = batch_capture(B))) return st;
+  LLR_HIP(hipMemsetAsync(B->iter.p, 0, 16, B->s0));  // outside the graph
+  for (size_t it = 0; it < B->n_iter; ++it) LLR_HIP(hipGraphLaunch(B->exec, B->s0));
+  LLR_HIP(hipStreamSynchronize(B->s0));
+  if (err_hist) {
+    for (size_t j = 0; j < B->rec.size(); ++j) {
+      llr_recon* r = B->rec[j];
+      std::vector<double> h(B->n_iter);
+      if (B->n_iter) LLR_HIP(hipMemcpy(h.data(), r->hist.p, B->n_iter * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < B->n_iter; ++i) err_hist[j * B->n_iter + i] = h[i] / r->energy;
+    }
+  }
+  return LLR_OK;
+}
+
+llr_status llr_recon_batch_get_image(llr_recon_batch* B, size_t i, llr_cf32* out, size_t len, int on_device) {
+  if (!B || i >= B->rec.size()) return fail(LLR_ERR_ARG, "no such batch member");
+  return llr_recon_get_image(B->rec[i], out, len, on_device);
+}
+
+llr_status llr_recon_batch_tuning(llr_recon_batch* B, size_t i, size_t blk[3], int fft_tpw[3], int* llr_stage,
+                                  double* lambda) {
+  if (!B || i >= B->rec.size()) return fail(LLR_ERR_ARG, "no such batch member");
+  llr_recon* r = B->rec[i];
+  for (int a = 0; a < 3; ++a) {
+    if (blk) blk[a] = r->p.blk[a];
+    if (fft_tpw) fft_tpw[a] = r->ctx->fft_tpw[a];
+  }
+  if (llr_stage) *llr_stage = r->ctx->llr_stage;
+  if (lambda) *lambda = r->lambda;
   return LLR_OK;
 }
 

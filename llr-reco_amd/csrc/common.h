@@ -38,6 +38,9 @@ struct llr_ctx {
   // per-call scratch for reductions
   double* d_red = nullptr;
   size_t red_cap = 0;
+  // launch-shape overrides chosen by the autotuner (0 / -1: built-in default)
+  int fft_tpw[3] = {0, 0, 0};  // FFT tiles per workgroup, per axis (z: the fused projection pass)
+  int llr_stage = -1;          // LLR block kernel: 1 stage the block in LDS, 0 read it from HBM twice
 };
 
 namespace llr {

@@ -411,7 +411,8 @@ template <int R1, int R2>
 hipError_t launch_pass(PassArgs a, long long W, long long nvols, bool tfast, bool inv, hipStream_t s) {
   using L = Layout<R1, R2>;
   a.tiles = (a.U + L::T - 1) / L::T;
-  a.tiles_per_wg = a.tiles >= 4 ? 4 : a.tiles;
+  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;  // tiles per workgroup (tunable)
+  a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
   dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
   const bool ramps = a.pre != nullptr;
 #define FFT_LAUNCH(TF, IV, RP) \
@@ -431,7 +432,8 @@ template <int R1, int R2>
 hipError_t launch_zproj(ZProjArgs a, long long W, long long nvols, hipStream_t s, size_t* nparts) {
   using L = Layout<R1, R2>;
   a.tiles = (a.U + L::T - 1) / L::T;
-  a.tiles_per_wg = a.tiles >= 4 ? 4 : a.tiles;
+  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;
+  a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
   dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
   *nparts = (size_t)grid.x * grid.y * grid.z;
   if (a.partial == nullptr) return hipSuccess;  // size query
@@ -527,6 +529,7 @@ llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const siz
     a.wstride = nx;
     tfast = true;
   }
+  a.tiles_per_wg = ctx->fft_tpw[axis];
   hipError_t e = dispatch_pass(n, a, W, (long long)nvols, tfast, inverse, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "fft axis pass launch");
   return LLR_OK;
@@ -557,6 +560,7 @@ llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_
   a.scale = (float)(1.0 / std::sqrt((double)nz));
   a.tw = tw;
   a.partial = partial;
+  a.tiles_per_wg = ctx->fft_tpw[2];
   hipError_t e;
   switch (n) {
     case 2: e = launch_zproj<2, 1>(a, ny, nvols, ctx->stream, nparts); break;
@@ -576,7 +580,7 @@ llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_
     case 96: e = launch_zproj<12, 8>(a, ny, nvols, ctx->stream, nparts); break;
     case 192: e = launch_zproj<12, 16>(a, ny, nvols, ctx->stream, nparts); break;
     case 384: e = launch_zproj<12, 32>(a, ny, nvols, ctx->stream, nparts); break;
-    default: return fail(LLR_ERR_UNSUPPORTED, "z length " + std::to_string(n) + " is not a supported power of two");
+    default: return fail(LLR_ERR_UNSUPPORTED, "z length " + std::to_string(n) + " is not a supported FFT length");
   }
   if (e != hipSuccess) return hip_fail(e, "fft z projection launch");
   return LLR_OK;

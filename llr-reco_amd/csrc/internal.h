@@ -17,6 +17,28 @@ llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_
                      const float2* pk, const float2* pi, double* partial, size_t* nparts);
 
 // llr.hip
+// One reconstruction's share of the persistent multi-reconstruction LLR launch.
+struct MemberSpec {
+  float2* F;            // nc planes, processed in place
+  size_t vol[3], blk[3];
+  int nc, mode, rank;
+  float lambda;
+  const int* d_shifts;  // device n_iter x 3 shift table
+  bool partial;
+  int stage;            // LDS staging override (-1 auto)
+};
+struct MultiPlan {
+  DevBuf desc;
+  int nm = 0;
+  long long total = 0;
+  size_t smem = 0;
+  long long per_wg = 0;
+  int per_xcd = 0;
+  unsigned grid = 0;
+};
+llr_status llr_multi_plan(MultiPlan& plan, const std::vector<MemberSpec>& members);
+// `iter` points at the device iteration counter that selects each member's shift
+llr_status llr_multi_launch(hipStream_t s, const MultiPlan& plan, const unsigned* iter);
 llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols, size_t eb, const size_t* vol,
                        const size_t* blk, const size_t* shift, size_t start, size_t n, bool unlift);
 llr_status project_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K);
@@ -51,6 +73,8 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
 llr_status shrink_all_device(llr_ctx* ctx, float2* x, size_t n, float alpha);
 llr_status energy_partials(llr_ctx* ctx, const float2* y, size_t n, double* partials, size_t nparts);
 llr_status reduce_strided(llr_ctx* ctx, const double* partials, size_t n, size_t stride, double* out);
+llr_status reduce_at(llr_ctx* ctx, const double* partials, size_t n, double* out, const unsigned* at, double* tmp);
+llr_status advance_counter(hipStream_t s, unsigned* c);
 llr_status reduce_sum(llr_ctx* ctx, const double* partials, size_t nparts, double* out, double* tmp, size_t tmp_n);
 llr_status hard_project_partials(llr_ctx* ctx, float2* x, const float2* y, size_t n, double* partials, size_t nparts);
 llr_status argmax_volumes(llr_ctx* ctx, const float2* y, size_t N, size_t nvols, unsigned long long* best);

@@ -15,6 +15,7 @@
 //    Gram, serial per-matrix Jacobi, apply.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -117,15 +118,12 @@ __global__ void __launch_bounds__(256) k_backproject(const float2* __restrict__ 
   }
 }
 
-__global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
-  extern __shared__ float2 smem[];
+// The fused LLR prox of block slot `slot` of A (one workgroup of 256 threads;
+// `smem` is the kernel's dynamic LDS).
+__device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem) {
   const int tid = threadIdx.x;
   const int nc = A.nc;
   const long long m = A.g.m;
-  // XCD-aware remap: workgroups are dealt round-robin to the 8 XCDs, so give
-  // each XCD a contiguous run of blocks (neighbouring blocks share L2 lines).
-  long long slot = (long long)(blockIdx.x % 8) * A.per_xcd + blockIdx.x / 8;
-  if (slot >= A.nblocks) return;
   const long long b = A.block0 + slot;
   const long long mp = m + 1;  // padded column stride in LDS
   float2* Fs = smem;                                      // nc * mp (when staged)
@@ -259,6 +257,42 @@ __global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
         A.out[A.layout == 0 ? (long long)cc * A.g.N + vaddr : elem_addr(A, b, rr, cc)] = acc;
       }
     }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_llr_block(BlockArgs A) {
+  extern __shared__ float2 smem[];
+  const long long slot = xcd_slot(A.per_xcd);
+  if (slot >= A.nblocks) return;
+  llr_block_body(A, slot, smem);
+}
+
+// One member of a batch of reconstructions for the persistent LLR launch.
+// The shift of the current iteration is read from the member's device table at
+// the index held by the device iteration counter, so a captured graph replays
+// every iteration unchanged.
+struct MemberBlocks {
+  BlockArgs A;          // g.shift is filled in per iteration
+  long long first;      // first batch-wide item of this member
+  const int* shifts;    // n_iter x 3
+};
+
+// Persistent LLR launch over the blocks of every member: workgroup g owns the
+// contiguous item range of chunk xcd_slot(per_xcd), so each XCD sweeps
+// neighbouring blocks of one member.
+__global__ void __launch_bounds__(256) k_llr_block_multi(const MemberBlocks* __restrict__ M, int nm, long long total,
+                                                         long long per_wg, int per_xcd, const unsigned* iter) {
+  extern __shared__ float2 smem[];
+  const long long chunk = xcd_slot(per_xcd);
+  const long long lo = chunk * per_wg, hi = min(total, lo + per_wg);
+  const unsigned it = *iter;
+  for (long long item = lo; item < hi; ++item) {
+    int j = 0;
+    while (j + 1 < nm && M[j + 1].first <= item) ++j;
+    BlockArgs A = M[j].A;
+    for (int a = 0; a < 3; ++a) A.g.shift[a] = M[j].shifts[3 * (size_t)it + a];
+    llr_block_body(A, item - M[j].first, smem);
+    __syncthreads();  // LDS is reused by the next item
   }
 }
 
@@ -476,11 +510,63 @@ llr_status llr_block_run(llr_ctx* ctx, const float2* F, float2* out, const float
   A.smax_bits = smax_bits;
   A.svals = svals;
   size_t smem = llr_block_smem(g.m, nc, A.nq, true);
-  A.stage_lds = smem <= 64 * 1024 ? 1 : 0;
+  A.stage_lds = (smem <= 64 * 1024 && ctx->llr_stage != 0) ? 1 : 0;
   if (!A.stage_lds) smem = llr_block_smem(g.m, nc, A.nq, false);
   if (smem > 64 * 1024) return fail(LLR_ERR_UNSUPPORTED, "subspace with too many volumes for the LLR kernel");
   dim3 grid((unsigned)(8 * A.per_xcd));
   hipLaunchKernelGGL(k_llr_block, grid, dim3(256), smem, ctx->stream, A);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+// Persistent LLR launch over the blocks of several reconstructions.
+llr_status llr_multi_plan(MultiPlan& plan, const std::vector<MemberSpec>& members) {
+  std::vector<MemberBlocks> h(members.size());
+  long long total = 0;
+  size_t smem = 0;
+  for (size_t j = 0; j < members.size(); ++j) {
+    const MemberSpec& ms = members[j];
+    if (ms.nc < 1 || ms.nc > 8) return fail(LLR_ERR_UNSUPPORTED, "fused LLR kernel handles 1..8 columns");
+    size_t zero[3] = {0, 0, 0};
+    GeomArgs g = make_geom(ms.vol, ms.blk, zero, ms.partial);
+    BlockArgs A{};
+    A.F = ms.F;
+    A.out = ms.F;
+    A.g = g;
+    A.nc = ms.nc;
+    A.layout = 0;
+    A.nblocks = g.grid[0] * g.grid[1] * g.grid[2];
+    A.mode = ms.mode;
+    A.rank = ms.rank;
+    A.lambda = ms.lambda;
+    size_t sm = llr_block_smem(g.m, ms.nc, 0, true);
+    A.stage_lds = (sm <= 64 * 1024 && ms.stage != 0) ? 1 : 0;
+    if (!A.stage_lds) sm = llr_block_smem(g.m, ms.nc, 0, false);
+    smem = std::max(smem, sm);
+    h[j].A = A;
+    h[j].first = total;
+    h[j].shifts = ms.d_shifts;
+    total += A.nblocks;
+  }
+  if (smem > 64 * 1024) return fail(LLR_ERR_UNSUPPORTED, "block too large for the LLR kernel");
+  LLR_HIP(plan.desc.alloc(h.size() * sizeof(MemberBlocks)));
+  LLR_HIP(hipMemcpy(plan.desc.p, h.data(), h.size() * sizeof(MemberBlocks), hipMemcpyHostToDevice));
+  plan.nm = (int)h.size();
+  plan.total = total;
+  plan.smem = smem;
+  // 256 CUs x 4 workgroups, dealt to the 8 XCDs in contiguous chunks
+  long long G = std::min<long long>(1024, (total + 7) / 8 * 8);
+  G = std::max<long long>(G, 8);
+  plan.grid = (unsigned)G;
+  plan.per_xcd = (int)(G / 8);
+  plan.per_wg = (total + G - 1) / G;
+  return LLR_OK;
+}
+
+llr_status llr_multi_launch(hipStream_t s, const MultiPlan& plan, const unsigned* iter) {
+  if (plan.total == 0) return LLR_OK;
+  hipLaunchKernelGGL(k_llr_block_multi, dim3(plan.grid), dim3(256), plan.smem, s,
+                     (const MemberBlocks*)plan.desc.p, plan.nm, plan.total, plan.per_wg, plan.per_xcd, iter);
   LLR_HIP(hipGetLastError());
   return LLR_OK;
 }

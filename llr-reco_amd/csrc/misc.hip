@@ -60,6 +60,22 @@ __global__ void __launch_bounds__(RB) k_reduce(const double* __restrict__ in, lo
   if (threadIdx.x == 0) out[j] = s;
 }
 
+// out[*at] = sum of in[0..n) in the fixed order of k_reduce (the index is read
+// on the device, so the launch can sit in a replayed graph)
+__global__ void __launch_bounds__(RB) k_reduce_at(const double* __restrict__ in, long long n, double* out,
+                                                  const unsigned* at) {
+  __shared__ double sh[RB / 64];
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += RB) s += in[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[*at] = s;
+}
+
+// *c += 1 (the device iteration counter of a replayed graph)
+__global__ void k_advance(unsigned* c) {
+  if (threadIdx.x == 0) *c += 1u;
+}
+
 // out[b] = sum of in[b*chunk .. (b+1)*chunk) in a fixed order
 __global__ void __launch_bounds__(RB) k_reduce_chunks(const double* __restrict__ in, long long n, long long chunk,
                                                       double* out) {
@@ -256,6 +272,28 @@ llr_status reduce_strided(llr_ctx* ctx, const double* partials, size_t n, size_t
   }
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)stride), dim3(RB), 0, ctx->stream, partials, (long long)n,
                      (long long)stride, out);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+// Sum of n partials into out[*at]; tmp holds 256 doubles for the two-level
+// order used by reduce_strided when n > 8192 (same summation order).
+llr_status reduce_at(llr_ctx* ctx, const double* partials, size_t n, double* out, const unsigned* at, double* tmp) {
+  if (n > 8192) {
+    const size_t nb = 256;
+    long long chunk = (long long)((n + nb - 1) / nb);
+    hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)nb), dim3(RB), 0, ctx->stream, partials, (long long)n, chunk,
+                       tmp);
+    hipLaunchKernelGGL(k_reduce_at, dim3(1), dim3(RB), 0, ctx->stream, tmp, (long long)nb, out, at);
+  } else {
+    hipLaunchKernelGGL(k_reduce_at, dim3(1), dim3(RB), 0, ctx->stream, partials, (long long)n, out, at);
+  }
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+llr_status advance_counter(hipStream_t s, unsigned* c) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, c);
   LLR_HIP(hipGetLastError());
   return LLR_OK;
 }

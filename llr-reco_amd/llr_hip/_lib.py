@@ -36,7 +36,8 @@ EXPORTS = [
     "llr_recon_destroy", "llr_recon_set_measurements",
     "llr_recon_set_initial", "llr_recon_run", "llr_recon_get_image", "llr_recon_info", "llr_recon_mixed_stats",
     "llr_recon_enable_timing",
-    "llr_recon_stage_times",
+    "llr_recon_stage_times", "llr_recon_batch_create", "llr_recon_batch_destroy",
+    "llr_recon_batch_set_measurements", "llr_recon_batch_run", "llr_recon_batch_get_image", "llr_recon_batch_tuning",
 ]
 
 

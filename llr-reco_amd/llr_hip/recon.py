@@ -53,6 +53,42 @@ def wavelet_denoise(volumes, n_volumes, vol_size, alpha, levels=-1, ctx=None):
                                                float(alpha), int(levels)))
 
 
+def make_params(vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0, lambda_factor=0.02,
+                wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0, precision="fp32",
+                partial_blocks=False):
+    """llr_recon_params for the C ABI, plus the arrays it points at (keep them alive)."""
+    vol = [int(v) for v in vol]
+    nq = int(nq)
+    sh = np.ascontiguousarray(np.asarray(shifts, np.uint64).reshape(-1, 3))
+    p = _lib.ReconParams()
+    p.vol[:] = vol
+    p.nq = nq
+    p.blk[:] = [int(b) for b in blk]
+    p.n_iter = sh.shape[0]
+    p.llr_mode = {"rank": _lib.LLR_MODE_RANK, "svt": _lib.LLR_MODE_SVT}[llr_mode]
+    p.precision = {"fp32": _lib.LLR_PREC_FP32, "mixed": _lib.LLR_PREC_MIXED}[precision]
+    p.rank = int(rank)
+    p.svt_lambda = float(svt_lambda)
+    p.lambda_factor = float(lambda_factor)
+    p.wavelet_alpha = float(wavelet_alpha)
+    p.wavelet_levels = int(wavelet_levels)
+    sub = None
+    if subspace is not None:
+        sub = np.asfortranarray(np.asarray(subspace, np.complex64))
+        if sub.shape[0] != nq:
+            raise ValueError("unexpected subspace dimension")
+        p.srank = sub.shape[1]
+        sub = sub.ravel(order="F").copy()
+        p.subspace = ctypes.c_void_p(sub.ctypes.data)
+    else:
+        p.srank = 0
+        p.subspace = None
+    p.shifts = ctypes.c_void_p(sh.ctypes.data) if sh.shape[0] else None
+    p.residual_stop = float(residual_stop)
+    p.partial_blocks = 1 if partial_blocks else 0
+    return p, (sh, sub)
+
+
 class Recon:
     """Device-resident LLR reconstruction (llr_recon_exec's loop, lib.rs:227-341).
 
@@ -67,37 +103,12 @@ class Recon:
                  lambda_factor=0.02, wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0,
                  precision="fp32", partial_blocks=False, ctx=None):
         self.ctx = ctx or _lib.default_context()
+        p, self._keep = make_params(vol, nq, shifts, blk, llr_mode, rank, svt_lambda, lambda_factor, wavelet_alpha,
+                                    wavelet_levels, subspace, residual_stop, precision, partial_blocks)
         self.vol = [int(v) for v in vol]
         self.nq = int(nq)
         self.D = int(np.prod(self.vol)) * self.nq
-        sh = np.ascontiguousarray(np.asarray(shifts, np.uint64).reshape(-1, 3))
-        self.n_iter = sh.shape[0]
-        p = _lib.ReconParams()
-        p.vol[:] = self.vol
-        p.nq = self.nq
-        p.blk[:] = [int(b) for b in blk]
-        p.n_iter = self.n_iter
-        p.llr_mode = {"rank": _lib.LLR_MODE_RANK, "svt": _lib.LLR_MODE_SVT}[llr_mode]
-        p.precision = {"fp32": _lib.LLR_PREC_FP32, "mixed": _lib.LLR_PREC_MIXED}[precision]
-        p.rank = int(rank)
-        p.svt_lambda = float(svt_lambda)
-        p.lambda_factor = float(lambda_factor)
-        p.wavelet_alpha = float(wavelet_alpha)
-        p.wavelet_levels = int(wavelet_levels)
-        sub = None
-        if subspace is not None:
-            sub = np.asfortranarray(np.asarray(subspace, np.complex64))
-            if sub.shape[0] != self.nq:
-                raise ValueError("unexpected subspace dimension")
-            p.srank = sub.shape[1]
-            sub = sub.ravel(order="F").copy()
-            p.subspace = ctypes.c_void_p(sub.ctypes.data)
-        else:
-            p.srank = 0
-            p.subspace = None
-        p.shifts = ctypes.c_void_p(sh.ctypes.data) if self.n_iter else None
-        p.residual_stop = float(residual_stop)
-        p.partial_blocks = 1 if partial_blocks else 0
+        self.n_iter = int(p.n_iter)
         h = ctypes.c_void_p()
         _lib.check(_lib.load().llr_recon_create(self.ctx.handle, ctypes.byref(p), ctypes.byref(h)))
         self.handle = h
@@ -159,6 +170,73 @@ class Recon:
     def close(self):
         if getattr(self, "handle", None):
             _lib.load().llr_recon_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ReconBatch:
+    """Many reconstructions of uneven size in one graph-replayed loop
+    (llr_recon_batch_*): `members` is a list of dicts of Recon keyword
+    arguments (vol, nq, shifts, blk, subspace, ...); all share n_iter.
+    autotune picks each member's block size from {6,8,10}^3 and launch shapes."""
+
+    def __init__(self, members, autotune=False, ctx=None):
+        self.ctx = ctx or _lib.default_context()
+        ps, self._keep = [], []
+        for m in members:
+            p, keep = make_params(**m)
+            ps.append(p)
+            self._keep.append(keep)
+        arr = (_lib.ReconParams * len(ps))(*ps)
+        self.members = [dict(m) for m in members]
+        self.D = [int(np.prod(m["vol"])) * int(m["nq"]) for m in members]
+        self.n_iter = int(ps[0].n_iter)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().llr_recon_batch_create(self.ctx.handle, arr, _sz(len(ps)), int(bool(autotune)),
+                                                      ctypes.byref(h)))
+        self.handle = h
+
+    def __len__(self):
+        return len(self.D)
+
+    def set_measurements(self, i, y=None, device_ptr=None):
+        if device_ptr is not None:
+            _lib.check(_lib.load().llr_recon_batch_set_measurements(self.handle, _sz(i), ctypes.c_void_p(int(device_ptr)),
+                                                                    _sz(self.D[i]), 1))
+        else:
+            Y = _lib.as_cf32(y, "measurements")
+            if Y.size != self.D[i]:
+                raise ValueError("measurements must hold nx*ny*nz*nq samples")
+            _lib.check(_lib.load().llr_recon_batch_set_measurements(self.handle, _sz(i), _lib.ptr(Y), _sz(Y.size), 0))
+
+    def run(self):
+        """Runs every member; returns the (n_members, n_iter) relative residual history."""
+        hist = np.zeros((len(self), max(self.n_iter, 1)), np.float64)
+        _lib.check(_lib.load().llr_recon_batch_run(self.handle, _lib.ptr(hist)))
+        return hist[:, :self.n_iter]
+
+    def image(self, i):
+        out = np.empty(self.D[i], np.complex64)
+        _lib.check(_lib.load().llr_recon_batch_get_image(self.handle, _sz(i), _lib.ptr(out), _sz(out.size), 0))
+        return out
+
+    def tuning(self, i):
+        blk = (ctypes.c_size_t * 3)()
+        tpw = (ctypes.c_int * 3)()
+        stage = ctypes.c_int()
+        lam = ctypes.c_double()
+        _lib.check(_lib.load().llr_recon_batch_tuning(self.handle, _sz(i), blk, tpw, ctypes.byref(stage),
+                                                      ctypes.byref(lam)))
+        return dict(blk=tuple(blk), fft_tpw=tuple(tpw), llr_stage=stage.value, lam=lam.value)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib.load().llr_recon_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):
