@@ -42,7 +42,7 @@ def build_measurements(preset, torch, llr):
 
     from llr_hip import phantom, signal_model
 
-    cfg = dict(phantom.PRESETS[preset])
+    cfg = dict(phantom.PRESETS[preset]) if isinstance(preset, str) else dict(preset)
     s_ph, s_noise, s_mask, s_shift = cfg["seeds"]
     dirs = phantom.six_directions() if cfg["dirs"] == "six" else phantom.repulsion_directions(cfg["n_dirs"], s_ph + 1000)
     bvals, bvecs = phantom.b_table(cfg["n_b0"], dirs, cfg["bval"])
@@ -121,12 +121,67 @@ def cpu_baseline(ds, torch):
                        f"extrapolated to {cfg['n_iter']} iterations")}
 
 
+def bench_batch(args, torch, llr_hip):
+    """16 reconstructions of uneven size (seeds 100-115: sizes from 96x96x64 to
+    256x256x128, 13/31/67 volumes, R 4/6/8) in one batch with graph-replayed
+    iterations and the autotuner. A step is the whole batch."""
+    from llr_hip import phantom, recon
+
+    members = [phantom.batch_member(s) for s in range(100, 116)]
+    dss = [build_measurements(m, torch, llr_hip) for m in members]
+    kw = [dict(vol=d["vol"], nq=d["nq"], shifts=d["shifts"], blk=d["cfg"]["blk"],
+               lambda_factor=d["cfg"]["lambda_factor"], wavelet_alpha=d["cfg"]["wavelet_alpha"],
+               subspace=d["subspace"]) for d in dss]
+    b = recon.ReconBatch(kw, autotune=True)
+    for i, d in enumerate(dss):
+        b.set_measurements(i, device_ptr=d["y"].data_ptr())
+    t = time.perf_counter()
+    b.run()  # first run includes the autotuner
+    torch.cuda.synchronize()
+    log(f"first batch run (with autotuning): {time.perf_counter() - t:.2f} s")
+    for _ in range(args.warmup):
+        b.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        hist = b.run()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / max(args.steps, 1)
+    tun = [b.tuning(i) for i in range(len(dss))]
+    vox = sum(int(d["vol"][0] * d["vol"][1] * d["vol"][2]) * d["nq"] for d in dss)
+    # the same members one at a time (no batch, no tuning) for comparison
+    t1 = time.perf_counter()
+    for i, d in enumerate(dss):
+        r = recon.Recon(**dict(kw[i], blk=tun[i]["blk"], partial_blocks=True))
+        r.set_measurements(device_ptr=d["y"].data_ptr())
+        r.run()
+        r.close()
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t1
+    out = {
+        "metric": "s/batch (16 uneven LLR recons, graph-replayed persistent batch)",
+        "value": round(el, 4), "unit": "s/batch", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(el * 1e3, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded DTI phantoms, seeds 100-115)",
+        "config": {"workload": "config 4: 16 recons, sizes {96x96x64,128x128x96,192x192x96,256x256x128}, "
+                               "volumes {13,31,67}, R {4,6,8}, 30 iterations, rank-7 subspace, db2 0.005, "
+                               "partial edge blocks, autotuned block size",
+                   "members": [dict(vol=list(d["vol"]), nq=d["nq"], R=d["cfg"]["R"], **tun[i])
+                               for i, d in enumerate(dss)],
+                   "recons_per_s": round(len(dss) / el, 3), "voxel_volumes": vox,
+                   "sequential_s": round(seq, 4)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4"],
+                    help="c2: 256x256x128x67 (the metric's workload); c1: 64^3x7; c4: batch of 16 uneven recons")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stage-timing", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -156,6 +211,8 @@ def main():
     from llr_hip import recon
 
     llr_hip.load()
+    if args.workload == "c4":
+        return bench_batch(args, torch, llr_hip)
     ds = build_measurements(args.workload, torch, llr_hip)
     cfg = ds["cfg"]
     r = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode="svt",

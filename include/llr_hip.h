@@ -303,6 +303,44 @@ llr_status llr_recon_batch_get_image(llr_recon_batch* batch, size_t index, llr_c
 llr_status llr_recon_batch_tuning(llr_recon_batch* batch, size_t index, size_t blk[3], int fft_tpw[3],
                                   int* llr_stage, double* lambda);
 
+/* --------------------------------------------------------- learned warm start */
+/* A 3D U-Net in subspace-coefficient space (no counterpart in the reference).
+ * levels L, widths base*2^l; per level two 3^3 conv + ReLU, 2x2x2 average
+ * pooling, nearest upsampling with [upsampled, skip] concatenation, a 1^3
+ * output convolution added to the input. BF16 weights and activations, MFMA
+ * with FP32 accumulation, hand-written forward and backward kernels, Adam
+ * (beta1 0.9, beta2 0.999, eps 1e-8) on FP32 master weights.
+ * Tensors are (nz, ny, nx, channels) FP32 with channels fastest; sizes must be
+ * multiples of 2^(L-1). */
+typedef struct llr_unet llr_unet;
+llr_status llr_unet_create(llr_ctx* ctx, int in_channels, int base_channels, int levels, unsigned long long seed,
+                           llr_unet** out);
+void llr_unet_destroy(llr_unet* net);
+llr_status llr_unet_num_params(llr_unet* net, size_t* n);
+/* Parameters in layer order (enc0a, enc0b, ..., dec{L-2}a, dec{L-2}b, ..., dec0b, out), each
+ * w[o][co][ci] (o = (dz*k + dy)*k + dx) then b[co]. Setting them resets Adam. */
+llr_status llr_unet_get_params(llr_unet* net, float* params, size_t n);
+llr_status llr_unet_set_params(llr_unet* net, const float* params, size_t n);
+/* dims = {nx, ny, nz}. x, y: nx*ny*nz*in_channels floats (host or device). */
+llr_status llr_unet_forward(llr_unet* net, const float* x, float* y, const size_t dims[3], int on_device);
+/* Gradients of sum(gy * forward(x)) w.r.t. the parameters (host arrays). */
+llr_status llr_unet_gradients(llr_unet* net, const float* x, const float* gy, const size_t dims[3], float* grads);
+/* One Adam step on the mean squared error between forward(x) and target. */
+llr_status llr_unet_train_step(llr_unet* net, const float* x, const float* target, const size_t dims[3], float lr,
+                               int on_device, double* loss);
+/* Copy of a stored BF16 activation of the last forward pass (as floats), for
+ * inspection and tests: kind 0/1 encoder conv a/b outputs, 2 pooled input of
+ * the level, 3 decoder concatenation, 4/5 decoder conv a/b outputs; n must be
+ * the tensor's element count. */
+llr_status llr_unet_get_activation(llr_unet* net, int kind, int level, float* out, size_t n);
+/* Device planes of K complex coefficients (x fastest) -> predicted planes; the
+ * network sees the coefficients divided by scale. */
+llr_status llr_unet_predict_coefficients(llr_unet* net, const llr_cf32* d_coef, size_t K, const size_t vol[3],
+                                         float scale, llr_cf32* d_out);
+/* K complex planes -> (voxels, channels) FP32 network tensor divided by scale (device). */
+llr_status llr_unet_pack_coefficients(llr_ctx* ctx, const llr_cf32* d_coef, size_t K, const size_t vol[3],
+                                      int channels, float scale, float* d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,827 @@
+// Warm-start 3D U-Net on gfx950: BF16 MFMA convolutions (FP32 accumulation)
+// with hand-written forward and backward passes, Adam, and the packing of
+// subspace coefficients into network channels.
+//
+// Tensors are (D, H, W, C) with channels fastest, C a multiple of 16.
+// Activations are stored in BF16, gradients and the prediction in FP32.
+// A k x k x k convolution (k = 3, zero padding 1; or k = 1) has FP32 master
+// weights w[o][co][ci] (o = (dz*k + dy)*k + dx) and bias b[co]; two BF16
+// copies are packed for the kernels: w[o][co][ci] (forward) and
+// w[o][ci][co] (backward data, read at the mirrored offset).
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x16_bf16: forward and
+// backward-data take M = 16 voxels per wavefront, N = output channels
+// (16 per MFMA tile), K = offsets x input channels (16 per step);
+// backward-weight takes M = 16 output channels, N = 16 input channels and
+// K = voxels, with per-wavefront partial sums added atomically.
+//
+// Network (levels L, widths c_l = base * 2^l): per level two 3^3 conv+ReLU,
+// 2x2x2 average pooling between encoder levels; the decoder upsamples
+// (nearest), concatenates [upsampled, skip] and applies two 3^3 conv+ReLU;
+// a 1^3 output convolution is added to the input (residual prediction).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short bf16_t;
+
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// four consecutive channels as BF16 MFMA operand
+__device__ __forceinline__ bf16x4 load4(const bf16_t* p) {
+  uint2 u = *(const uint2*)p;
+  bf16x4 r;
+  r[0] = (short)(u.x & 0xFFFF);
+  r[1] = (short)(u.x >> 16);
+  r[2] = (short)(u.y & 0xFFFF);
+  r[3] = (short)(u.y >> 16);
+  return r;
+}
+__device__ __forceinline__ bf16x4 load4(const float* p) {
+  float4 f = *(const float4*)p;
+  bf16x4 r;
+  r[0] = (short)f2bf(f.x);
+  r[1] = (short)f2bf(f.y);
+  r[2] = (short)f2bf(f.z);
+  r[3] = (short)f2bf(f.w);
+  return r;
+}
+
+constexpr int MAX_CO_TILES = 16;  // up to 256 output channels
+
+struct ConvArgs {
+  const void* in;       // D*H*W x cin (BF16 activations or FP32 gradients)
+  const bf16_t* w;      // [kk][cout][cin]
+  const float* bias;    // cout or null
+  const bf16_t* mask;   // D*H*W x cout: output *= (mask > 0), or null
+  void* out;            // D*H*W x cout (BF16 or FP32)
+  int D, H, W, cin, cout, k;
+  int mirror;           // read offset kk-1-o (backward data)
+  int relu, accumulate;
+};
+
+// out[v][co] = sum_{o, ci} in[v + off(o)][ci] w[o][co][ci] (+ bias, ReLU, mask, +=)
+template <typename TIn, typename TOut>
+__global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = lane & 15, kq = lane >> 4;
+  const long long V = (long long)a.D * a.H * a.W;
+  const long long v0 = ((long long)blockIdx.x * 4 + wave) * 16;
+  if (v0 >= V) return;
+  const int nt = a.cout / 16;
+  const int kk = a.k * a.k * a.k, p = a.k / 2;
+  // the A-operand voxel of this lane
+  const long long va = v0 + n;
+  int x = 0, y = 0, z = 0;
+  if (va < V) {
+    x = (int)(va % a.W);
+    y = (int)((va / a.W) % a.H);
+    z = (int)(va / ((long long)a.W * a.H));
+  }
+  f32x4 acc[MAX_CO_TILES];
+#pragma unroll
+  for (int t = 0; t < MAX_CO_TILES; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const TIn* in = (const TIn*)a.in;
+  for (int o = 0; o < kk; ++o) {
+    const int dz = o / (a.k * a.k) - p, dy = (o / a.k) % a.k - p, dx = o % a.k - p;
+    const int zz = z + dz, yy = y + dy, xx = x + dx;
+    const bool valid = va < V && zz >= 0 && zz < a.D && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+    const long long src = (((long long)zz * a.H + yy) * a.W + xx) * a.cin;
+    const int ow = a.mirror ? kk - 1 - o : o;
+    const bf16_t* wo = a.w + (size_t)ow * a.cout * a.cin;
+    for (int c0 = 0; c0 < a.cin; c0 += 16) {
+      bf16x4 av = {0, 0, 0, 0};
+      if (valid) av = load4(in + src + c0 + 4 * kq);
+#pragma unroll
+      for (int t = 0; t < MAX_CO_TILES; ++t) {
+        if (t < nt) {
+          bf16x4 bv = load4(wo + (size_t)(16 * t + n) * a.cin + c0 + 4 * kq);
+          acc[t] = mfma16(av, bv, acc[t]);
+        }
+      }
+    }
+  }
+  TOut* out = (TOut*)a.out;
+#pragma unroll
+  for (int t = 0; t < MAX_CO_TILES; ++t) {
+    if (t < nt) {
+      const int co = 16 * t + n;
+      const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const long long v = v0 + 4 * kq + j;
+        if (v < V) {
+          float r = acc[t][j] + b;
+          if (a.relu) r = fmaxf(r, 0.f);
+          const size_t idx = (size_t)v * a.cout + co;
+          if (a.mask && !(bf2f(a.mask[idx]) > 0.f)) r = 0.f;
+          if constexpr (sizeof(TOut) == 4) {
+            if (a.accumulate) r += ((float*)out)[idx];
+            ((float*)out)[idx] = r;
+          } else {
+            ((bf16_t*)out)[idx] = f2bf(r);
+          }
+        }
+      }
+    }
+  }
+}
+
+// dW[o][co][ci] += sum_v g[v][co] x[v + off(o)][ci] over one chunk of voxels
+// per wavefront (M = 16 co, N = 16 ci, K = voxels).
+__global__ void __launch_bounds__(64) k_conv_wgrad(const float* __restrict__ g, const bf16_t* __restrict__ x,
+                                                   float* __restrict__ dw, int D, int H, int W, int cin, int cout,
+                                                   int k, long long chunk) {
+  const int lane = threadIdx.x;
+  const int n = lane & 15, kq = lane >> 4;
+  const int p = k / 2;
+  const int tci = cin / 16, tco = cout / 16;
+  int tile = blockIdx.y;
+  const int ti = tile % tci;
+  tile /= tci;
+  const int tc = tile % tco;
+  const int o = tile / tco;
+  const int dz = o / (k * k) - p, dy = (o / k) % k - p, dx = o % k - p;
+  const long long V = (long long)D * H * W;
+  const long long lo = (long long)blockIdx.x * chunk, hi = min(V, lo + chunk);
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (long long v0 = lo; v0 < hi; v0 += 16) {
+    bf16x4 av, bv;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const long long v = v0 + 4 * kq + j;
+      float gv = 0.f, xv = 0.f;
+      if (v < hi) {
+        gv = g[(size_t)v * cout + 16 * tc + n];
+        int xx = (int)(v % W) + dx, yy = (int)((v / W) % H) + dy, zz = (int)(v / ((long long)W * H)) + dz;
+        if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
+          xv = bf2f(x[(((size_t)zz * H + yy) * W + xx) * cin + 16 * ti + n]);
+      }
+      av[j] = (short)f2bf(gv);
+      bv[j] = (short)f2bf(xv);
+    }
+    acc = mfma16(av, bv, acc);
+  }
+  float* dst = dw + (size_t)o * cout * cin;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) atomicAdd(dst + (size_t)(16 * tc + 4 * kq + j) * cin + 16 * ti + n, acc[j]);
+}
+
+// db[c] += sum_v g[v][c]
+__global__ void __launch_bounds__(256) k_bias_grad(const float* __restrict__ g, float* __restrict__ db, long long V,
+                                                   int c) {
+  const int ch = threadIdx.x % c;  // c divides 256 for c <= 256 multiples of 16
+  const int rows = 256 / c;
+  float s = 0.f;
+  for (long long v = (long long)blockIdx.x * rows + threadIdx.x / c; v < V; v += (long long)gridDim.x * rows)
+    s += g[(size_t)v * c + ch];
+  if (threadIdx.x / c < rows) atomicAdd(db + ch, s);
+}
+
+// 2x2x2 average pooling (BF16 -> BF16)
+__global__ void k_pool_fwd(const bf16_t* __restrict__ in, bf16_t* __restrict__ out, int D, int H, int W, int C) {
+  const int d = D / 2, h = H / 2, w = W / 2;
+  const long long total = (long long)d * h * w * C;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+    int c = (int)(e % C);
+    long long v = e / C;
+    int x = (int)(v % w), y = (int)((v / w) % h), z = (int)(v / ((long long)w * h));
+    float s = 0.f;
+    for (int q = 0; q < 8; ++q)
+      s += bf2f(in[((((size_t)(2 * z + (q >> 2)) * H + 2 * y + ((q >> 1) & 1)) * W) + 2 * x + (q & 1)) * C + c]);
+    out[e] = f2bf(s * 0.125f);
+  }
+}
+
+// dst_fine = (dst_fine + g_coarse[parent] / 8) * (mask > 0)
+__global__ void k_pool_bwd_add(const float* __restrict__ g, float* __restrict__ dst, const bf16_t* __restrict__ mask,
+                               int D, int H, int W, int C) {
+  const long long total = (long long)D * H * W * C;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+    int c = (int)(e % C);
+    long long v = e / C;
+    int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long long)W * H));
+    float r = dst[e] + 0.125f * g[((((size_t)(z / 2) * (H / 2)) + y / 2) * (W / 2) + x / 2) * C + c];
+    if (mask && !(bf2f(mask[e]) > 0.f)) r = 0.f;
+    dst[e] = r;
+  }
+}
+
+// concat buffer (Ct channels): [0, Cu) = nearest upsampling of the coarse
+// tensor, [Cu, Cu + Cs) = the skip tensor
+__global__ void k_concat_fwd(const bf16_t* __restrict__ coarse, const bf16_t* __restrict__ skip,
+                             bf16_t* __restrict__ out, int D, int H, int W, int Cu, int Cs) {
+  const int Ct = Cu + Cs;
+  const long long total = (long long)D * H * W * Ct;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+    int c = (int)(e % Ct);
+    long long v = e / Ct;
+    if (c < Cu) {
+      int x = (int)(v % W), y = (int)((v / W) % H), z = (int)(v / ((long long)W * H));
+      out[e] = coarse[((((size_t)(z / 2) * (H / 2)) + y / 2) * (W / 2) + x / 2) * Cu + c];
+    } else {
+      out[e] = skip[(size_t)v * Cs + (c - Cu)];
+    }
+  }
+}
+
+// backward of the concat: coarse gradient = sum over the 8 children of the
+// upsampled channels (times the coarse activation's ReLU mask); skip gradient
+// = the remaining channels
+__global__ void k_concat_bwd(const float* __restrict__ g, float* __restrict__ g_coarse,
+                             const bf16_t* __restrict__ coarse_mask, float* __restrict__ g_skip, int D, int H, int W,
+                             int Cu, int Cs) {
+  const int Ct = Cu + Cs;
+  const int d = D / 2, h = H / 2, w = W / 2;
+  const long long nc = (long long)d * h * w * Cu;
+  const long long ns = (long long)D * H * W * Cs;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < nc + ns; e += (long long)gridDim.x * blockDim.x) {
+    if (e < nc) {
+      int c = (int)(e % Cu);
+      long long v = e / Cu;
+      int x = (int)(v % w), y = (int)((v / w) % h), z = (int)(v / ((long long)w * h));
+      float s = 0.f;
+      for (int q = 0; q < 8; ++q)
+        s += g[((((size_t)(2 * z + (q >> 2)) * H + 2 * y + ((q >> 1) & 1)) * W) + 2 * x + (q & 1)) * Ct + c];
+      if (coarse_mask && !(bf2f(coarse_mask[e]) > 0.f)) s = 0.f;
+      g_coarse[e] = s;
+    } else {
+      long long f = e - nc;
+      int c = (int)(f % Cs);
+      long long v = f / Cs;
+      g_skip[f] = g[(size_t)v * Ct + Cu + c];
+    }
+  }
+}
+
+// grad = 2 (pred - target) / n; partial sums of (pred - target)^2 per block
+__global__ void __launch_bounds__(256) k_mse(const float* __restrict__ pred, const float* __restrict__ tgt,
+                                             float* __restrict__ grad, long long n, double* partial) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  const float sc = 2.0f / (float)n;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    float d = pred[e] - tgt[e];
+    grad[e] = sc * d;
+    s += (double)d * d;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Adam (Kingma & Ba, Alg. 1) with bias correction folded into the step size
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       long long n, float lr_t, float b1, float b2, float eps_t) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+    float gi = g[e];
+    float mi = b1 * m[e] + (1.f - b1) * gi;
+    float vi = b2 * v[e] + (1.f - b2) * gi * gi;
+    m[e] = mi;
+    v[e] = vi;
+    p[e] -= lr_t * mi / (sqrtf(vi) + eps_t);
+  }
+}
+
+// BF16 forward [o][co][ci] and transposed [o][ci][co] copies of one layer
+__global__ void k_pack_weights(const float* __restrict__ w, bf16_t* __restrict__ wf, bf16_t* __restrict__ wt, int kk,
+                               int cout, int cin) {
+  const long long total = (long long)kk * cout * cin;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+    int ci = (int)(e % cin);
+    long long r = e / cin;
+    int co = (int)(r % cout);
+    int o = (int)(r / cout);
+    bf16_t h = f2bf(w[e]);
+    wf[e] = h;
+    wt[((size_t)o * cin + ci) * cout + co] = h;
+  }
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ a, bf16_t* __restrict__ b, long long n) {
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x)
+    b[e] = f2bf(a[e]);
+}
+
+// K complex planes of N voxels (x fastest) -> (N, C) FP32 channels
+// [Re c0, Im c0, Re c1, ...] / scale, remaining channels zero
+__global__ void k_pack_coef(const float2* __restrict__ c, float* __restrict__ out, long long N, int K, int C,
+                            float inv_scale) {
+  const long long total = N * C;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+    int ch = (int)(e % C);
+    long long v = e / C;
+    float r = 0.f;
+    if (ch < 2 * K) {
+      float2 z = c[(size_t)(ch >> 1) * N + v];
+      r = ((ch & 1) ? z.y : z.x) * inv_scale;
+    }
+    out[e] = r;
+  }
+}
+
+__global__ void k_unpack_coef(const float* __restrict__ in, float2* __restrict__ c, long long N, int K, int C,
+                              float scale) {
+  const long long total = N * K;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
+    long long v = e % N;
+    int k = (int)(e / N);
+    c[e] = make_float2(in[(size_t)v * C + 2 * k] * scale, in[(size_t)v * C + 2 * k + 1] * scale);
+  }
+}
+
+}  // namespace
+
+// =================================================================== network
+struct UnetLayer {
+  std::string name;
+  int k, cin, cout;
+  bool relu;
+  size_t w_off, b_off;  // into the flat parameter vector (floats)
+  size_t wb_off;        // into the BF16 packed weights (elements)
+};
+
+struct llr_unet {
+  llr_ctx* ctx = nullptr;
+  int cin = 16, base = 16, levels = 4;
+  std::vector<UnetLayer> layers;
+  size_t nparam = 0, nwb = 0;
+  llr::DevBuf P, G, M, Vv, Wf, Wt, part, loss_d;
+  long long step = 0;
+  // activation / gradient workspace for one input size
+  size_t dims[3] = {0, 0, 0};
+  llr::DevBuf xb, xf, y, gy;
+  std::vector<llr::DevBuf*> a, b, pool, cat, da, db;      // per level
+  std::vector<llr::DevBuf*> ga, gb, gin, gcat, gskip, gda, gdb;
+  ~llr_unet() {
+    for (auto* v : {&a, &b, &pool, &cat, &da, &db, &ga, &gb, &gin, &gcat, &gskip, &gda, &gdb})
+      for (llr::DevBuf* d : *v) delete d;
+  }
+};
+
+namespace llr {
+
+static std::vector<UnetLayer> unet_layers(int cin, int base, int levels) {
+  std::vector<UnetLayer> L;
+  std::vector<int> ch(levels);
+  for (int l = 0; l < levels; ++l) ch[l] = base << l;
+  int c = cin;
+  for (int l = 0; l < levels; ++l) {
+    L.push_back({"enc" + std::to_string(l) + "a", 3, c, ch[l], true, 0, 0, 0});
+    L.push_back({"enc" + std::to_string(l) + "b", 3, ch[l], ch[l], true, 0, 0, 0});
+    c = ch[l];
+  }
+  for (int l = levels - 2; l >= 0; --l) {
+    L.push_back({"dec" + std::to_string(l) + "a", 3, c + ch[l], ch[l], true, 0, 0, 0});
+    L.push_back({"dec" + std::to_string(l) + "b", 3, ch[l], ch[l], true, 0, 0, 0});
+    c = ch[l];
+  }
+  L.push_back({"out", 1, c, cin, false, 0, 0, 0});
+  size_t off = 0, wb = 0;
+  for (UnetLayer& u : L) {
+    size_t nw = (size_t)u.k * u.k * u.k * u.cout * u.cin;
+    u.w_off = off;
+    u.b_off = off + nw;
+    u.wb_off = wb;
+    off += nw + u.cout;
+    wb += nw;
+  }
+  return L;
+}
+
+static const UnetLayer& layer(const llr_unet* U, const std::string& name) {
+  for (const UnetLayer& u : U->layers)
+    if (u.name == name) return u;
+  return U->layers.back();
+}
+
+static llr_status pack_all(llr_unet* U) {
+  for (const UnetLayer& u : U->layers) {
+    int kk = u.k * u.k * u.k;
+    size_t nw = (size_t)kk * u.cout * u.cin;
+    hipLaunchKernelGGL(k_pack_weights, dim3(grid_for(nw, 256, 4096)), dim3(256), 0, U->ctx->stream,
+                       U->P.as<float>() + u.w_off, U->Wf.as<bf16_t>() + u.wb_off, U->Wt.as<bf16_t>() + u.wb_off, kk,
+                       u.cout, u.cin);
+  }
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+static llr_status ensure_workspace(llr_unet* U, const size_t* dims) {
+  if (U->dims[0] == dims[0] && U->dims[1] == dims[1] && U->dims[2] == dims[2]) return LLR_OK;
+  const size_t f = (size_t)1 << (U->levels - 1);
+  for (int d = 0; d < 3; ++d)
+    if (dims[d] == 0 || dims[d] % f) return fail(LLR_ERR_ARG, "U-Net input sizes must be multiples of 2^(levels-1)");
+  for (auto* v : {&U->a, &U->b, &U->pool, &U->cat, &U->da, &U->db, &U->ga, &U->gb, &U->gin, &U->gcat, &U->gskip,
+                  &U->gda, &U->gdb}) {
+    for (DevBuf* d : *v) delete d;
+    v->assign(U->levels, nullptr);
+    for (int l = 0; l < U->levels; ++l) (*v)[l] = new DevBuf();
+  }
+  const size_t V = dims[0] * dims[1] * dims[2];
+  LLR_HIP(U->xb.alloc(V * U->cin * 2));
+  LLR_HIP(U->xf.alloc(V * U->cin * 4));
+  LLR_HIP(U->y.alloc(V * U->cin * 4));
+  LLR_HIP(U->gy.alloc(V * U->cin * 4));
+  for (int l = 0; l < U->levels; ++l) {
+    const size_t Vl = V >> (3 * l);
+    const size_t c = (size_t)U->base << l;
+    const size_t cu = (size_t)U->base << (l + 1);
+    LLR_HIP(U->a[l]->alloc(Vl * c * 2));
+    LLR_HIP(U->b[l]->alloc(Vl * c * 2));
+    LLR_HIP(U->ga[l]->alloc(Vl * c * 4));
+    LLR_HIP(U->gb[l]->alloc(Vl * c * 4));
+    if (l > 0) {
+      const size_t cp = (size_t)U->base << (l - 1);
+      LLR_HIP(U->pool[l]->alloc(Vl * cp * 2));
+      LLR_HIP(U->gin[l]->alloc(Vl * cp * 4));
+    }
+    if (l < U->levels - 1) {
+      LLR_HIP(U->cat[l]->alloc(Vl * (c + cu) * 2));
+      LLR_HIP(U->gcat[l]->alloc(Vl * (c + cu) * 4));
+      LLR_HIP(U->gskip[l]->alloc(Vl * c * 4));
+      LLR_HIP(U->da[l]->alloc(Vl * c * 2));
+      LLR_HIP(U->db[l]->alloc(Vl * c * 2));
+      LLR_HIP(U->gda[l]->alloc(Vl * c * 4));
+      LLR_HIP(U->gdb[l]->alloc(Vl * c * 4));
+    }
+  }
+  for (int d = 0; d < 3; ++d) U->dims[d] = dims[d];
+  return LLR_OK;
+}
+
+// dims are (nx, ny, nz) = (W, H, D)
+template <typename TIn, typename TOut>
+static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int lvl, const TIn* in, TOut* out,
+                       bool backward_data, const bf16_t* mask, bool accumulate) {
+  ConvArgs a{};
+  a.in = in;
+  a.w = (backward_data ? U->Wt.as<bf16_t>() : U->Wf.as<bf16_t>()) + u.wb_off;
+  a.bias = backward_data ? nullptr : U->P.as<float>() + u.b_off;
+  a.mask = mask;
+  a.out = out;
+  a.W = (int)(dims[0] >> lvl);
+  a.H = (int)(dims[1] >> lvl);
+  a.D = (int)(dims[2] >> lvl);
+  a.cin = backward_data ? u.cout : u.cin;
+  a.cout = backward_data ? u.cin : u.cout;
+  a.k = u.k;
+  a.mirror = backward_data ? 1 : 0;
+  a.relu = (!backward_data && u.relu) ? 1 : 0;
+  a.accumulate = accumulate ? 1 : 0;
+  if (a.cout > 16 * MAX_CO_TILES || a.cin % 16 || a.cout % 16) return fail(LLR_ERR_UNSUPPORTED, "U-Net channel count");
+  const long long V = (long long)a.D * a.H * a.W;
+  unsigned grid = (unsigned)((V + 63) / 64);
+  hipLaunchKernelGGL((k_conv<TIn, TOut>), dim3(grid), dim3(256), 0, U->ctx->stream, a);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+static llr_status wgrad(llr_unet* U, const UnetLayer& u, const size_t* dims, int lvl, const float* g,
+                        const bf16_t* x) {
+  const int W = (int)(dims[0] >> lvl), H = (int)(dims[1] >> lvl), D = (int)(dims[2] >> lvl);
+  const long long V = (long long)D * H * W;
+  const long long chunk = 1024;
+  const int kk = u.k * u.k * u.k;
+  dim3 grid((unsigned)((V + chunk - 1) / chunk), (unsigned)(kk * (u.cout / 16) * (u.cin / 16)));
+  hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(64), 0, U->ctx->stream, g, x, U->G.as<float>() + u.w_off, D, H, W,
+                     u.cin, u.cout, u.k, chunk);
+  hipLaunchKernelGGL(k_bias_grad, dim3(grid_for((size_t)V, 256 / u.cout, 2048)), dim3(256), 0, U->ctx->stream, g,
+                     U->G.as<float>() + u.b_off, V, u.cout);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+static unsigned ew_grid(size_t n) { return grid_for(n, 256, 16384); }
+
+// forward of the packed input U->xb / U->xf; prediction (input + net) in U->y
+static llr_status unet_forward(llr_unet* U, const size_t* dims) {
+  llr_status st;
+  const int L = U->levels;
+  const size_t W = dims[0], H = dims[1], D = dims[2];
+  hipStream_t s = U->ctx->stream;
+  for (int l = 0; l < L; ++l) {
+    const bf16_t* in = l == 0 ? U->xb.as<bf16_t>() : U->pool[l]->as<bf16_t>();
+    if ((st = conv(U, layer(U, "enc" + std::to_string(l) + "a"), dims, l, in, U->a[l]->as<bf16_t>(), false, nullptr,
+                   false)))
+      return st;
+    if ((st = conv(U, layer(U, "enc" + std::to_string(l) + "b"), dims, l, U->a[l]->as<bf16_t>(),
+                   U->b[l]->as<bf16_t>(), false, nullptr, false)))
+      return st;
+    if (l < L - 1) {
+      const int c = U->base << l;
+      size_t n = (W >> (l + 1)) * (H >> (l + 1)) * (D >> (l + 1)) * c;
+      hipLaunchKernelGGL(k_pool_fwd, dim3(ew_grid(n)), dim3(256), 0, s, U->b[l]->as<bf16_t>(),
+                         U->pool[l + 1]->as<bf16_t>(), (int)(D >> l), (int)(H >> l), (int)(W >> l), c);
+    }
+  }
+  for (int l = L - 2; l >= 0; --l) {
+    const bf16_t* coarse = (l == L - 2) ? U->b[L - 1]->as<bf16_t>() : U->db[l + 1]->as<bf16_t>();
+    const int c = U->base << l, cu = U->base << (l + 1);
+    size_t n = (W >> l) * (H >> l) * (D >> l) * (size_t)(c + cu);
+    hipLaunchKernelGGL(k_concat_fwd, dim3(ew_grid(n)), dim3(256), 0, s, coarse, U->b[l]->as<bf16_t>(),
+                       U->cat[l]->as<bf16_t>(), (int)(D >> l), (int)(H >> l), (int)(W >> l), cu, c);
+    if ((st = conv(U, layer(U, "dec" + std::to_string(l) + "a"), dims, l, U->cat[l]->as<bf16_t>(),
+                   U->da[l]->as<bf16_t>(), false, nullptr, false)))
+      return st;
+    if ((st = conv(U, layer(U, "dec" + std::to_string(l) + "b"), dims, l, U->da[l]->as<bf16_t>(),
+                   U->db[l]->as<bf16_t>(), false, nullptr, false)))
+      return st;
+  }
+  const size_t V = W * H * D;
+  LLR_HIP(hipMemcpyAsync(U->y.p, U->xf.p, V * U->cin * 4, hipMemcpyDeviceToDevice, s));  // residual
+  return conv(U, layer(U, "out"), dims, 0, U->db[0]->as<bf16_t>(), U->y.as<float>(), false, nullptr, true);
+}
+
+// gradients of sum(gy * prediction) into U->G (zeroed first)
+static llr_status unet_backward(llr_unet* U, const size_t* dims) {
+  llr_status st;
+  const int L = U->levels;
+  const size_t W = dims[0], H = dims[1], D = dims[2];
+  hipStream_t s = U->ctx->stream;
+  LLR_HIP(hipMemsetAsync(U->G.p, 0, U->nparam * 4, s));
+  auto name = [](const char* p, int l, const char* q) { return std::string(p) + std::to_string(l) + q; };
+  // output layer (input db[0], a ReLU output with one consumer)
+  const UnetLayer& uo = layer(U, "out");
+  if ((st = wgrad(U, uo, dims, 0, U->gy.as<float>(), U->db[0]->as<bf16_t>()))) return st;
+  if ((st = conv(U, uo, dims, 0, U->gy.as<float>(), U->gdb[0]->as<float>(), true, U->db[0]->as<bf16_t>(), false)))
+    return st;
+  // decoder, finest level first; gdb[l] holds the pre-activation gradient of dec{l}b
+  for (int l = 0; l < L - 1; ++l) {
+    const UnetLayer& ub = layer(U, name("dec", l, "b"));
+    const UnetLayer& ua = layer(U, name("dec", l, "a"));
+    if ((st = wgrad(U, ub, dims, l, U->gdb[l]->as<float>(), U->da[l]->as<bf16_t>()))) return st;
+    if ((st = conv(U, ub, dims, l, U->gdb[l]->as<float>(), U->gda[l]->as<float>(), true, U->da[l]->as<bf16_t>(),
+                   false)))
+      return st;
+    if ((st = wgrad(U, ua, dims, l, U->gda[l]->as<float>(), U->cat[l]->as<bf16_t>()))) return st;
+    if ((st = conv(U, ua, dims, l, U->gda[l]->as<float>(), U->gcat[l]->as<float>(), true, nullptr, false))) return st;
+    const int c = U->base << l, cu = U->base << (l + 1);
+    const bf16_t* coarse_act = (l == L - 2) ? U->b[L - 1]->as<bf16_t>() : U->db[l + 1]->as<bf16_t>();
+    float* g_coarse = (l == L - 2) ? U->gb[L - 1]->as<float>() : U->gdb[l + 1]->as<float>();
+    size_t n = ((W >> (l + 1)) * (H >> (l + 1)) * (D >> (l + 1))) * (size_t)cu + (W >> l) * (H >> l) * (D >> l) * c;
+    hipLaunchKernelGGL(k_concat_bwd, dim3(ew_grid(n)), dim3(256), 0, s, U->gcat[l]->as<float>(), g_coarse, coarse_act,
+                       U->gskip[l]->as<float>(), (int)(D >> l), (int)(H >> l), (int)(W >> l), cu, c);
+  }
+  // encoder, deepest level first; gb[l] holds the pre-activation gradient of enc{l}b
+  for (int l = L - 1; l >= 0; --l) {
+    if (l < L - 1) {
+      // gb[l] = (gskip[l] + pool_bwd(gin[l+1])) * (b[l] > 0)
+      const int c = U->base << l;
+      size_t n = (W >> l) * (H >> l) * (D >> l) * (size_t)c;
+      LLR_HIP(hipMemcpyAsync(U->gb[l]->p, U->gskip[l]->p, n * 4, hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(k_pool_bwd_add, dim3(ew_grid(n)), dim3(256), 0, s, U->gin[l + 1]->as<float>(),
+                         U->gb[l]->as<float>(), U->b[l]->as<bf16_t>(), (int)(D >> l), (int)(H >> l), (int)(W >> l), c);
+    }
+    const UnetLayer& ub = layer(U, name("enc", l, "b"));
+    const UnetLayer& ua = layer(U, name("enc", l, "a"));
+    if ((st = wgrad(U, ub, dims, l, U->gb[l]->as<float>(), U->a[l]->as<bf16_t>()))) return st;
+    if ((st = conv(U, ub, dims, l, U->gb[l]->as<float>(), U->ga[l]->as<float>(), true, U->a[l]->as<bf16_t>(), false)))
+      return st;
+    const bf16_t* in = l == 0 ? U->xb.as<bf16_t>() : U->pool[l]->as<bf16_t>();
+    if ((st = wgrad(U, ua, dims, l, U->ga[l]->as<float>(), in))) return st;
+    if (l > 0 &&
+        (st = conv(U, ua, dims, l, U->ga[l]->as<float>(), U->gin[l]->as<float>(), true, nullptr, false)))
+      return st;
+  }
+  return LLR_OK;
+}
+
+// pack the FP32 (V x cin) input held in U->xf into U->xb
+static llr_status unet_set_input(llr_unet* U, const size_t* dims) {
+  const size_t n = dims[0] * dims[1] * dims[2] * U->cin;
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3(ew_grid(n)), dim3(256), 0, U->ctx->stream, U->xf.as<float>(),
+                     U->xb.as<bf16_t>(), (long long)n);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+// U-Net warm start in subspace-coefficient space: K complex planes in `coef`
+// (device) -> predicted planes in `out` (device), both scaled by `scale`.
+llr_status unet_predict_device(llr_unet* U, const float2* coef, int K, const size_t* vol, float scale, float2* out) {
+  if (2 * K > U->cin) return fail(LLR_ERR_ARG, "more subspace coefficients than network channels");
+  llr_status st;
+  if ((st = ensure_workspace(U, vol))) return st;
+  const long long N = (long long)(vol[0] * vol[1] * vol[2]);
+  hipLaunchKernelGGL(k_pack_coef, dim3(ew_grid((size_t)N * U->cin)), dim3(256), 0, U->ctx->stream, coef,
+                     U->xf.as<float>(), N, K, U->cin, 1.0f / scale);
+  if ((st = unet_set_input(U, vol))) return st;
+  if ((st = unet_forward(U, vol))) return st;
+  hipLaunchKernelGGL(k_unpack_coef, dim3(ew_grid((size_t)N * K)), dim3(256), 0, U->ctx->stream, U->y.as<float>(), out,
+                     N, K, U->cin, scale);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+}  // namespace llr
+
+using namespace llr;
+
+extern "C" {
+
+llr_status llr_unet_create(llr_ctx* ctx, int in_channels, int base_channels, int levels, unsigned long long seed,
+                           llr_unet** out) {
+  if (!ctx || !out) return fail(LLR_ERR_ARG, "null argument");
+  if (in_channels <= 0 || in_channels % 16 || base_channels <= 0 || base_channels % 16 || levels < 1 || levels > 6)
+    return fail(LLR_ERR_ARG, "channel counts must be multiples of 16 and levels in 1..6");
+  {
+    int widest = std::max(in_channels, base_channels << (levels - 1));
+    if (levels > 1) widest = std::max(widest, (base_channels << (levels - 1)) + (base_channels << (levels - 2)));
+    if (widest > 16 * MAX_CO_TILES) return fail(LLR_ERR_UNSUPPORTED, "network too wide for the convolution kernel");
+  }
+  LLR_HIP(hipSetDevice(ctx->device));
+  llr_unet* U = new llr_unet();
+  U->ctx = ctx;
+  U->cin = in_channels;
+  U->base = base_channels;
+  U->levels = levels;
+  U->layers = unet_layers(in_channels, base_channels, levels);
+  const UnetLayer& last = U->layers.back();
+  U->nparam = last.b_off + last.cout;
+  U->nwb = last.wb_off + (size_t)last.k * last.k * last.k * last.cout * last.cin;
+  // He-normal weights, zero output layer (the network starts as the identity map), zero biases
+  std::vector<float> h(U->nparam, 0.f);
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  for (const UnetLayer& u : U->layers) {
+    if (u.name == "out") continue;
+    size_t nw = (size_t)u.k * u.k * u.k * u.cout * u.cin;
+    double sd = std::sqrt(2.0 / ((double)u.k * u.k * u.k * u.cin));
+    for (size_t i = 0; i < nw; ++i) h[u.w_off + i] = (float)(sd * nd(rng));
+  }
+  auto bail = [&](llr_status s) {
+    delete U;
+    return s;
+  };
+  if (U->P.alloc(U->nparam * 4) || U->G.alloc(U->nparam * 4) || U->M.alloc(U->nparam * 4) ||
+      U->Vv.alloc(U->nparam * 4) || U->Wf.alloc(U->nwb * 2) || U->Wt.alloc(U->nwb * 2) || U->part.alloc(1024 * 8))
+    return bail(fail(LLR_ERR_NOMEM, "U-Net parameter allocation failed"));
+  if (hipMemcpy(U->P.p, h.data(), U->nparam * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(U->M.p, 0, U->nparam * 4) != hipSuccess || hipMemset(U->Vv.p, 0, U->nparam * 4) != hipSuccess)
+    return bail(fail(LLR_ERR_HIP, "U-Net parameter upload failed"));
+  llr_status st = pack_all(U);
+  if (st) return bail(st);
+  *out = U;
+  return LLR_OK;
+}
+
+void llr_unet_destroy(llr_unet* U) {
+  if (!U) return;
+  (void)hipSetDevice(U->ctx->device);
+  (void)hipStreamSynchronize(U->ctx->stream);
+  delete U;
+}
+
+llr_status llr_unet_num_params(llr_unet* U, size_t* n) {
+  if (!U || !n) return fail(LLR_ERR_ARG, "null argument");
+  *n = U->nparam;
+  return LLR_OK;
+}
+
+llr_status llr_unet_get_params(llr_unet* U, float* host, size_t n) {
+  if (!U || !host || n != U->nparam) return fail(LLR_ERR_ARG, "parameter buffer must hold num_params floats");
+  LLR_HIP(hipStreamSynchronize(U->ctx->stream));
+  LLR_HIP(hipMemcpy(host, U->P.p, n * 4, hipMemcpyDeviceToHost));
+  return LLR_OK;
+}
+
+llr_status llr_unet_set_params(llr_unet* U, const float* host, size_t n) {
+  if (!U || !host || n != U->nparam) return fail(LLR_ERR_ARG, "parameter buffer must hold num_params floats");
+  LLR_HIP(hipSetDevice(U->ctx->device));
+  LLR_HIP(hipStreamSynchronize(U->ctx->stream));
+  LLR_HIP(hipMemcpy(U->P.p, host, n * 4, hipMemcpyHostToDevice));
+  LLR_HIP(hipMemset(U->M.p, 0, n * 4));
+  LLR_HIP(hipMemset(U->Vv.p, 0, n * 4));
+  U->step = 0;
+  return pack_all(U);
+}
+
+llr_status llr_unet_forward(llr_unet* U, const float* x, float* y, const size_t dims[3], int on_device) {
+  if (!U || !x || !y || !dims) return fail(LLR_ERR_ARG, "null argument");
+  LLR_HIP(hipSetDevice(U->ctx->device));
+  llr_status st;
+  if ((st = ensure_workspace(U, dims))) return st;
+  const size_t n = dims[0] * dims[1] * dims[2] * U->cin;
+  hipStream_t s = U->ctx->stream;
+  LLR_HIP(hipMemcpyAsync(U->xf.p, x, n * 4, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  if ((st = unet_set_input(U, dims)) || (st = unet_forward(U, dims))) return st;
+  LLR_HIP(hipMemcpyAsync(y, U->y.p, n * 4, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  LLR_HIP(hipStreamSynchronize(s));
+  return LLR_OK;
+}
+
+llr_status llr_unet_gradients(llr_unet* U, const float* x, const float* gy, const size_t dims[3], float* grads) {
+  if (!U || !x || !gy || !dims || !grads) return fail(LLR_ERR_ARG, "null argument");
+  LLR_HIP(hipSetDevice(U->ctx->device));
+  llr_status st;
+  if ((st = ensure_workspace(U, dims))) return st;
+  const size_t n = dims[0] * dims[1] * dims[2] * U->cin;
+  hipStream_t s = U->ctx->stream;
+  LLR_HIP(hipMemcpyAsync(U->xf.p, x, n * 4, hipMemcpyHostToDevice, s));
+  LLR_HIP(hipMemcpyAsync(U->gy.p, gy, n * 4, hipMemcpyHostToDevice, s));
+  if ((st = unet_set_input(U, dims)) || (st = unet_forward(U, dims)) || (st = unet_backward(U, dims))) return st;
+  LLR_HIP(hipStreamSynchronize(s));
+  LLR_HIP(hipMemcpy(grads, U->G.p, U->nparam * 4, hipMemcpyDeviceToHost));
+  return LLR_OK;
+}
+
+llr_status llr_unet_train_step(llr_unet* U, const float* x, const float* target, const size_t dims[3], float lr,
+                               int on_device, double* loss) {
+  if (!U || !x || !target || !dims) return fail(LLR_ERR_ARG, "null argument");
+  LLR_HIP(hipSetDevice(U->ctx->device));
+  llr_status st;
+  if ((st = ensure_workspace(U, dims))) return st;
+  const size_t n = dims[0] * dims[1] * dims[2] * U->cin;
+  hipStream_t s = U->ctx->stream;
+  const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  LLR_HIP(hipMemcpyAsync(U->xf.p, x, n * 4, kind, s));
+  if ((st = unet_set_input(U, dims)) || (st = unet_forward(U, dims))) return st;
+  // target staged in gy, replaced by the loss gradient in place
+  LLR_HIP(hipMemcpyAsync(U->gy.p, target, n * 4, kind, s));
+  const unsigned nb = 1024;
+  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, U->y.as<float>(), U->gy.as<float>(), U->gy.as<float>(),
+                     (long long)n, U->part.as<double>());
+  LLR_HIP(hipGetLastError());
+  if ((st = unet_backward(U, dims))) return st;
+  U->step += 1;
+  const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+  const double c1 = 1.0 - std::pow(b1, (double)U->step), c2 = 1.0 - std::pow(b2, (double)U->step);
+  // p -= lr * (m/c1) / (sqrt(v/c2) + eps) = lr_t * m / (sqrt(v) + eps_t)
+  const float lr_t = (float)(lr * std::sqrt(c2) / c1), eps_t = (float)(eps * std::sqrt(c2));
+  hipLaunchKernelGGL(k_adam, dim3(ew_grid(U->nparam)), dim3(256), 0, s, U->P.as<float>(), U->G.as<float>(),
+                     U->M.as<float>(), U->Vv.as<float>(), (long long)U->nparam, lr_t, (float)b1, (float)b2, eps_t);
+  LLR_HIP(hipGetLastError());
+  if ((st = pack_all(U))) return st;
+  if (loss) {
+    std::vector<double> p(nb);
+    LLR_HIP(hipStreamSynchronize(s));
+    LLR_HIP(hipMemcpy(p.data(), U->part.p, nb * 8, hipMemcpyDeviceToHost));
+    double t = 0;
+    for (double v : p) t += v;
+    *loss = t / (double)n;
+  }
+  return LLR_OK;
+}
+
+llr_status llr_unet_get_activation(llr_unet* U, int kind, int level, float* host, size_t n) {
+  if (!U || !host || level < 0 || level >= U->levels) return fail(LLR_ERR_ARG, "bad activation request");
+  std::vector<DevBuf*>* t[6] = {&U->a, &U->b, &U->pool, &U->cat, &U->da, &U->db};
+  if (kind < 0 || kind > 5 || U->dims[0] == 0) return fail(LLR_ERR_ARG, "no such activation");
+  DevBuf* d = (*t[kind])[level];
+  if (!d || !d->p || d->bytes != n * 2) return fail(LLR_ERR_ARG, "activation size mismatch");
+  std::vector<unsigned short> h(n);
+  LLR_HIP(hipStreamSynchronize(U->ctx->stream));
+  LLR_HIP(hipMemcpy(h.data(), d->p, n * 2, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) {
+    unsigned u = (unsigned)h[i] << 16;
+    std::memcpy(host + i, &u, 4);
+  }
+  return LLR_OK;
+}
+
+llr_status llr_unet_predict_coefficients(llr_unet* U, const llr_cf32* d_coef, size_t K, const size_t vol[3],
+                                         float scale, llr_cf32* d_out) {
+  if (!U || !d_coef || !vol || !d_out) return fail(LLR_ERR_ARG, "null argument");
+  if (!(scale > 0.f)) return fail(LLR_ERR_ARG, "scale must be positive");
+  LLR_HIP(hipSetDevice(U->ctx->device));
+  llr_status st = unet_predict_device(U, (const float2*)d_coef, (int)K, vol, scale, (float2*)d_out);
+  if (st) return st;
+  LLR_HIP(hipStreamSynchronize(U->ctx->stream));
+  return LLR_OK;
+}
+
+llr_status llr_unet_pack_coefficients(llr_ctx* ctx, const llr_cf32* d_coef, size_t K, const size_t vol[3],
+                                      int channels, float scale, float* d_out) {
+  if (!ctx || !d_coef || !vol || !d_out) return fail(LLR_ERR_ARG, "null argument");
+  if ((int)(2 * K) > channels || !(scale > 0.f)) return fail(LLR_ERR_ARG, "bad channel count or scale");
+  LLR_HIP(hipSetDevice(ctx->device));
+  const long long N = (long long)(vol[0] * vol[1] * vol[2]);
+  hipLaunchKernelGGL(k_pack_coef, dim3(ew_grid((size_t)N * channels)), dim3(256), 0, ctx->stream,
+                     (const float2*)d_coef, d_out, N, (int)K, channels, 1.0f / scale);
+  LLR_HIP(hipGetLastError());
+  LLR_HIP(hipStreamSynchronize(ctx->stream));
+  return LLR_OK;
+}
+
+}  // extern "C"

@@ -38,6 +38,9 @@ EXPORTS = [
     "llr_recon_enable_timing",
     "llr_recon_stage_times", "llr_recon_batch_create", "llr_recon_batch_destroy",
     "llr_recon_batch_set_measurements", "llr_recon_batch_run", "llr_recon_batch_get_image", "llr_recon_batch_tuning",
+    "llr_unet_create", "llr_unet_destroy", "llr_unet_num_params", "llr_unet_get_params", "llr_unet_set_params",
+    "llr_unet_forward", "llr_unet_gradients", "llr_unet_train_step", "llr_unet_predict_coefficients",
+    "llr_unet_pack_coefficients", "llr_unet_get_activation",
 ]
 
 

@@ -26,6 +26,23 @@ PRESETS = {
                blk=(8, 8, 8), lambda_factor=0.02, wavelet_alpha=0.005, srank=7, n_ell=20, seeds=(11, 12, 13, 14)),
 }
 
+BATCH_SIZES = [(96, 96, 64), (128, 128, 96), (192, 192, 96), (256, 256, 128)]
+BATCH_VOLUMES = [13, 31, 67]
+BATCH_R = [4.0, 6.0, 8.0]
+
+
+def batch_member(seed):
+    """One member of the uneven batch workload: matrix size, volume count and
+    undersampling drawn from the seed; rank-7 subspace, db2 0.005, 30 iterations."""
+    rng = np.random.default_rng(seed)
+    vol = BATCH_SIZES[rng.integers(len(BATCH_SIZES))]
+    nq = BATCH_VOLUMES[rng.integers(len(BATCH_VOLUMES))]
+    R = BATCH_R[rng.integers(len(BATCH_R))]
+    n_b0 = 7 if nq == 67 else 1
+    return dict(vol=vol, n_b0=n_b0, dirs="repulsion", n_dirs=nq - n_b0, bval=3000.0, R=R, n_iter=30, blk=(8, 8, 8),
+                lambda_factor=0.02, wavelet_alpha=0.005, srank=7, n_ell=20,
+                seeds=(seed, seed + 1000, seed + 2000, seed + 3000))
+
 
 def sample_unit_sphere(u1, u2):
     """dti_subspace.rs:65-76."""
