@@ -252,7 +252,8 @@ void llr_recon_destroy(llr_recon* r);
 /* Measurements (k-space, nx*ny*nz*nq complex64, zero = unsampled). on_device != 0:
  * y is a device pointer on the context's device. */
 llr_status llr_recon_set_measurements(llr_recon* r, const llr_cf32* y, size_t len, int on_device);
-/* Optional warm start of the iterate (image space); otherwise the zero-filled adjoint. */
+/* Optional start of the iterate (image space) for the next run, otherwise the
+ * zero-filled adjoint; the SVT threshold is then taken from this start. */
 llr_status llr_recon_set_initial(llr_recon* r, const llr_cf32* x, size_t len, int on_device);
 /* llr_recon_exec loop (lib.rs:227-341): estimate linear corrections, zero-filled
  * adjoint (unless a warm start was set), then n_iter iterations of
@@ -260,6 +261,19 @@ llr_status llr_recon_set_initial(llr_recon* r, const llr_cf32* x, size_t len, in
  * err_hist (may be NULL) receives err/total_energy per iteration (lib.rs:321);
  * iters_done (may be NULL) the number of iterations run. */
 llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done);
+/* Set-up of a run without iterating (lib.rs:224-258): total energy, k-space
+ * corrections and ramp tables, the zero-filled adjoint (or the start set with
+ * llr_recon_set_initial) and the SVT threshold. The next llr_recon_run starts
+ * from this state instead of repeating it. */
+llr_status llr_recon_prepare(llr_recon* r);
+/* Subspace coefficients of the current iterate, C = X S (srank planes of
+ * nx*ny*nz complex values, x fastest; lib.rs:285-286 over every voxel). Needs
+ * a subspace. With llr_recon_prepare these are the zero-filled adjoint's. */
+llr_status llr_recon_get_coefficients(llr_recon* r, llr_cf32* out, size_t len, int on_device);
+/* Replace the iterate with C S^H for every voxel (srank planes). After
+ * llr_recon_prepare the threshold of the zero-filled adjoint is kept. */
+llr_status llr_recon_set_coefficients(llr_recon* r, const llr_cf32* c, size_t len, int on_device);
+
 /* Copy the current image out (to a device pointer when on_device != 0). */
 llr_status llr_recon_get_image(llr_recon* r, llr_cf32* out, size_t len, int on_device);
 /* Threshold used by the SVT step and the per-volume k-space corrections found. */
@@ -333,13 +347,44 @@ llr_status llr_unet_train_step(llr_unet* net, const float* x, const float* targe
  * the level, 3 decoder concatenation, 4/5 decoder conv a/b outputs; n must be
  * the tensor's element count. */
 llr_status llr_unet_get_activation(llr_unet* net, int kind, int level, float* out, size_t n);
-/* Device planes of K complex coefficients (x fastest) -> predicted planes; the
- * network sees the coefficients divided by scale. */
+/* Coefficient <-> channel mapping (K <= 8): per voxel the network sees
+ * b = to_net * c / scale as channels [Re b0, Im b0, Re b1, ...] (remaining
+ * channels zero), and a prediction p maps back to from_net * p * scale.
+ * to_net / from_net are K x K real, row-major host arrays; NULL = identity. */
+/* Device planes of K complex coefficients (x fastest) -> predicted planes (may alias). */
 llr_status llr_unet_predict_coefficients(llr_unet* net, const llr_cf32* d_coef, size_t K, const size_t vol[3],
-                                         float scale, llr_cf32* d_out);
-/* K complex planes -> (voxels, channels) FP32 network tensor divided by scale (device). */
+                                         float scale, const float* to_net, const float* from_net, llr_cf32* d_out);
+/* K complex planes -> (voxels, channels) network tensor on the device, FP32
+ * (bf16_out = 0) or BF16 (bf16_out = 1); channels a multiple of 16. */
 llr_status llr_unet_pack_coefficients(llr_ctx* ctx, const llr_cf32* d_coef, size_t K, const size_t vol[3],
-                                      int channels, float scale, float* d_out);
+                                      int channels, float scale, const float* to_net, int bf16_out, void* d_out);
+/* The warm start's input scale: sqrt(mean over voxels of |(to_net * c)_0|^2). */
+llr_status llr_unet_coefficient_scale(llr_ctx* ctx, const llr_cf32* d_coef, size_t K, const size_t vol[3],
+                                      const float* to_net, double* scale);
+/* One Adam step on a crop of stored BF16 (input, target) tensors (device,
+ * (nz, ny, nx, in_channels) each): crop_dims voxels at origin ({x, y, z}). */
+llr_status llr_unet_train_step_crop(llr_unet* net, const void* x_full, const void* t_full, const size_t full_dims[3],
+                                    const size_t origin[3], const size_t crop_dims[3], float lr, double* loss);
+
+/* Learned warm start of a reconstruction with a subspace. The zero-filled
+ * adjoint is computed as for a cold start (its SVT threshold is kept), its
+ * subspace coefficients go through the network (mapping above, scale from
+ * llr_unet_coefficient_scale) and the iterate becomes the prediction's
+ * back-projection. llr_recon_run then continues from it. */
+llr_status llr_recon_warm_start(llr_recon* r, llr_unet* net, const float* to_net, const float* from_net,
+                                double* scale);
+typedef struct {
+  const float* to_net;   /* K x K, see above (NULL = identity) */
+  const float* from_net; /* K x K (NULL = identity) */
+  double residual_stop;  /* the iterations stop once the relative residual falls to this value */
+  size_t max_iter;       /* iteration budget of the warm-started run (0 or > n_iter: n_iter) */
+} llr_warm_params;
+/* Warm start, then iterations until residual_stop. If the warm-started run
+ * does not reach residual_stop within max_iter iterations the cold start is
+ * run instead (zero-filled adjoint, n_iter iterations) and kept: fell_back = 1.
+ * err_hist (n_iter entries) receives the kept run's relative residuals. */
+llr_status llr_recon_run_learned(llr_recon* r, llr_unet* net, const llr_warm_params* wp, double* err_hist,
+                                 size_t* iters_done, int* fell_back);
 
 #ifdef __cplusplus
 }

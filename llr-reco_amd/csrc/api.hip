@@ -146,7 +146,7 @@ using namespace llr;
 extern "C" {
 
 const char* llr_last_error(void) { return g_err.c_str(); }
-int llr_abi_version(void) { return 101; }
+int llr_abi_version(void) { return 102; }
 
 llr_status llr_ctx_create(int device, llr_ctx** out) {
   if (!out) return fail(LLR_ERR_ARG, "null output pointer");
@@ -582,6 +582,7 @@ struct llr_recon {
   unsigned long long llr_blocks = 0;  // blocks processed by the LLR step (mixed-form statistics)
   size_t nparts = 0;
   bool have_meas = false, have_init = false;
+  bool prepared = false;  // set-up done (llr_recon_prepare / warm start) and not yet consumed by a run
   double lambda = 0.0, energy = 0.0;
   std::vector<float> ksp_corr;
   bool timing = false;
@@ -863,26 +864,26 @@ static llr_status recon_setup(llr_recon* r) {
   return LLR_OK;
 }
 
-}  // namespace llr
+// X = C S^H for every voxel (one block covering the whole volume)
+static llr_status backproject_all(llr_recon* r, const float2* C) {
+  const size_t zero[3] = {0, 0, 0};
+  return backproject_device(r->ctx, C, r->X.as<float2>(), r->S.as<float2>(), r->p.vol, r->p.vol, zero, (int)r->nq,
+                            (int)r->p.srank, false);
+}
 
-extern "C" {
-
-llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done) {
-  if (!r) return fail(LLR_ERR_ARG, "null recon");
-  if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set");
+// Iterations it = 0 .. n_max-1 of the prepared run; stop (if > 0) ends the run
+// once the relative residual falls to that value. Relative residuals out.
+static llr_status run_iterations(llr_recon* r, size_t n_max, double stop, double* err_hist, size_t* iters_done) {
   llr_ctx* ctx = r->ctx;
-  const llr_recon_params& p = r->p;
-  LLR_HIP(hipSetDevice(ctx->device));
-  llr_status st = recon_setup(r);
-  if (st) return st;
+  llr_status st;
   size_t it = 0;
-  for (; it < p.n_iter; ++it) {
+  for (; it < n_max; ++it) {
     if ((st = recon_iteration(r, it))) return st;
-    if (p.residual_stop > 0) {
+    if (stop > 0) {
       double e = 0;
       LLR_HIP(hipStreamSynchronize(ctx->stream));
       LLR_HIP(hipMemcpy(&e, r->hist.as<double>() + it, 8, hipMemcpyDeviceToHost));
-      if (e / r->energy <= p.residual_stop) {
+      if (e / r->energy <= stop) {
         ++it;
         break;
       }
@@ -895,6 +896,127 @@ llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done) {
     for (size_t i = 0; i < it; ++i) err_hist[i] = h[i] / r->energy;
   }
   if (iters_done) *iters_done = it;
+  return LLR_OK;
+}
+
+// Set-up from the zero-filled adjoint (threshold included), then the iterate
+// becomes the U-Net's prediction from the adjoint's subspace coefficients.
+static llr_status warm_start(llr_recon* r, llr_unet* net, const float* to_net, const float* from_net,
+                             double* scale_out) {
+  llr_status st;
+  r->have_init = false;
+  if ((st = recon_setup(r))) return st;
+  const int K = (int)r->p.srank;
+  float2* C = r->Cb.as<float2>();
+  if ((st = project_device(r->ctx, r->X.as<float2>(), C, r->S.as<float2>(), r->N, (int)r->nq, K))) return st;
+  double scale = 0;
+  if ((st = unet_coef_scale(r->ctx, C, K, r->N, to_net, &scale))) return st;
+  if (!(scale > 0) || !std::isfinite(scale)) return fail(LLR_ERR_ARG, "zero-filled adjoint has no signal");
+  llr_ctx* nctx = unet_context(net);
+  if (nctx != r->ctx) LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  if ((st = unet_predict_device(net, C, K, r->p.vol, (float)scale, to_net, from_net, C))) return st;
+  if (nctx != r->ctx) LLR_HIP(hipStreamSynchronize(nctx->stream));
+  if ((st = backproject_all(r, C))) return st;
+  r->have_init = true;
+  r->prepared = true;
+  if (scale_out) *scale_out = scale;
+  return LLR_OK;
+}
+
+}  // namespace llr
+
+extern "C" {
+
+llr_status llr_recon_run(llr_recon* r, double* err_hist, size_t* iters_done) {
+  if (!r) return fail(LLR_ERR_ARG, "null recon");
+  if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set");
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  llr_status st;
+  if (!r->prepared && (st = recon_setup(r))) return st;
+  r->prepared = false;
+  r->have_init = false;
+  return run_iterations(r, r->p.n_iter, r->p.residual_stop, err_hist, iters_done);
+}
+
+llr_status llr_recon_prepare(llr_recon* r) {
+  if (!r) return fail(LLR_ERR_ARG, "null recon");
+  if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set");
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  llr_status st = recon_setup(r);
+  if (st) return st;
+  LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  r->prepared = true;
+  return LLR_OK;
+}
+
+llr_status llr_recon_get_coefficients(llr_recon* r, llr_cf32* out, size_t len, int on_device) {
+  if (!r || !out) return fail(LLR_ERR_ARG, "null argument");
+  if (!r->p.srank) return fail(LLR_ERR_ARG, "the reconstruction has no subspace");
+  if (len != r->N * r->p.srank) return fail(LLR_ERR_ARG, "output must hold nx*ny*nz*srank samples");
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  float2* dst = on_device ? (float2*)out : r->Cb.as<float2>();
+  llr_status st = project_device(r->ctx, r->X.as<float2>(), dst, r->S.as<float2>(), r->N, (int)r->nq, (int)r->p.srank);
+  if (st) return st;
+  if (!on_device) return download(out, r->Cb, len * 8, r->ctx);
+  LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  return LLR_OK;
+}
+
+llr_status llr_recon_set_coefficients(llr_recon* r, const llr_cf32* c, size_t len, int on_device) {
+  if (!r || !c) return fail(LLR_ERR_ARG, "null argument");
+  if (!r->p.srank) return fail(LLR_ERR_ARG, "the reconstruction has no subspace");
+  if (len != r->N * r->p.srank) return fail(LLR_ERR_ARG, "coefficients must hold nx*ny*nz*srank samples");
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  const float2* src = (const float2*)c;
+  if (!on_device) {
+    LLR_HIP(hipMemcpyAsync(r->Cb.p, c, len * 8, hipMemcpyHostToDevice, r->ctx->stream));
+    src = r->Cb.as<float2>();
+  }
+  llr_status st = backproject_all(r, src);
+  if (st) return st;
+  LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  r->have_init = true;
+  return LLR_OK;
+}
+
+llr_status llr_recon_warm_start(llr_recon* r, llr_unet* net, const float* to_net, const float* from_net,
+                                double* scale) {
+  if (!r || !net) return fail(LLR_ERR_ARG, "null argument");
+  if (!r->p.srank) return fail(LLR_ERR_ARG, "the warm start works in subspace-coefficient space: no subspace");
+  if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set");
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  llr_status st = warm_start(r, net, to_net, from_net, scale);
+  if (st) return st;
+  LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+  return LLR_OK;
+}
+
+llr_status llr_recon_run_learned(llr_recon* r, llr_unet* net, const llr_warm_params* wp, double* err_hist,
+                                 size_t* iters_done, int* fell_back) {
+  if (!r || !net || !wp) return fail(LLR_ERR_ARG, "null argument");
+  if (!r->p.srank) return fail(LLR_ERR_ARG, "the warm start works in subspace-coefficient space: no subspace");
+  if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set");
+  if (!(wp->residual_stop > 0)) return fail(LLR_ERR_ARG, "residual_stop must be positive");
+  const size_t budget = wp->max_iter ? std::min(wp->max_iter, r->p.n_iter) : r->p.n_iter;
+  LLR_HIP(hipSetDevice(r->ctx->device));
+  llr_status st = warm_start(r, net, wp->to_net, wp->from_net, nullptr);
+  if (st) return st;
+  size_t done = 0;
+  std::vector<double> h(std::max<size_t>(r->p.n_iter, 1));
+  if ((st = run_iterations(r, budget, wp->residual_stop, h.data(), &done))) return st;
+  const bool reached = done > 0 && h[done - 1] <= wp->residual_stop;
+  if (fell_back) *fell_back = reached ? 0 : 1;
+  if (!reached) {
+    // cold start: the plain run from the zero-filled adjoint, n_iter iterations
+    r->have_init = false;
+    if ((st = recon_setup(r))) return st;
+    if ((st = run_iterations(r, r->p.n_iter, 0.0, h.data(), &done))) return st;
+  }
+  r->prepared = false;
+  r->have_init = false;
+  if (err_hist)
+    for (size_t i = 0; i < done; ++i) err_hist[i] = h[i];
+  if (iters_done) *iters_done = done;
   return LLR_OK;
 }
 

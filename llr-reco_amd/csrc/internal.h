@@ -69,6 +69,12 @@ llr_status dti_scalars_device(llr_ctx* ctx, const float2* vols, size_t n_vox, si
 llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, const size_t* dims, float alpha, int L,
                                   float2* coef, float2* t1, float2* t2);
 
+// unet.hip
+llr_status unet_predict_device(llr_unet* U, const float2* coef, int K, const size_t* vol, float scale,
+                               const float* to_net, const float* from_net, float2* out);
+llr_status unet_coef_scale(llr_ctx* ctx, const float2* coef, int K, size_t N, const float* to_net, double* scale);
+llr_ctx* unet_context(llr_unet* U);
+
 // misc.hip
 llr_status shrink_all_device(llr_ctx* ctx, float2* x, size_t n, float alpha);
 llr_status energy_partials(llr_ctx* ctx, const float2* y, size_t n, double* partials, size_t nparts);

@@ -2,7 +2,9 @@
 //
 //  - k_project_bf16 / k_backproject_bf16: the subspace projection C = X*S and
 //    back-projection X = L*S^H (lib.rs:285-289) as real GEMMs on BF16 MFMA
-//    (v_mfma_f32_16x16x16_bf16) with FP32 accumulation. A 16-voxel tile is
+//    (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; S enters as two BF16
+//    operands (hi + lo) so only the streamed X or L carries BF16 rounding
+//    and the subspace itself is kept to ~2^-17. A 16-voxel tile is
 //    the M dimension; the complex contraction is written as a real one over
 //    interleaved (re, im) pairs, so K = 2*nq (projection) or 2*srank
 //    (back-projection). Output columns are interleaved (re, im) too, so a
@@ -44,6 +46,9 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
 
 constexpr int MAX_STEPS = 16;  // 2*nq <= 256
 
+// BF16 value nearest to f, as float
+__device__ __forceinline__ float bf16_round(float f) { return __uint_as_float(((unsigned)(unsigned short)bf16_bits(f)) << 16); }
+
 // C[k][v] = sum_q X[q][v] S[q][k]. Row kk = 2q + c of the real operand is
 // (Re, Im)[c] of X[q]; output column n = 2k + o is (Re, Im)[o] of C[k].
 template <int K>
@@ -52,7 +57,9 @@ __global__ void __launch_bounds__(256) k_project_bf16(const float2* __restrict__
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = lane & 15, kq = lane >> 4;
   const int NS = (2 * nq + 15) / 16;
-  bf16x4 bop[MAX_STEPS];
+  // S as a sum of two BF16 operands (hi + lo), so the subspace the projection
+  // uses is S to ~2^-17 and only the streamed data carries BF16 rounding
+  bf16x4 bop[MAX_STEPS], bol[MAX_STEPS];
 #pragma unroll
   for (int s = 0; s < MAX_STEPS; ++s) {
 #pragma unroll
@@ -66,6 +73,7 @@ __global__ void __launch_bounds__(256) k_project_bf16(const float2* __restrict__
         val = oc == 0 ? (ic == 0 ? sv.x : -sv.y) : (ic == 0 ? sv.y : sv.x);
       }
       bop[s][j] = bf16_bits(val);
+      bol[s][j] = bf16_bits(val - bf16_round(val));
     }
   }
   const long long tiles = (N + 15) / 16;
@@ -83,6 +91,7 @@ __global__ void __launch_bounds__(256) k_project_bf16(const float2* __restrict__
         }
         bf16x4 a = {bf16_bits(x0.x), bf16_bits(x0.y), bf16_bits(x1.x), bf16_bits(x1.y)};
         acc = mfma16(a, bop[s], acc);
+        acc = mfma16(a, bol[s], acc);
       }
     }
     // lane holds column n of voxels 4kq..4kq+3; its xor-1 partner holds the other part
@@ -120,7 +129,7 @@ __global__ void __launch_bounds__(256) k_backproject_bf16(const float2* __restri
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = lane & 15, kq = lane >> 4;
   const int NT = (2 * nq + 15) / 16;
-  bf16x4 bop[MAX_STEPS];
+  bf16x4 bop[MAX_STEPS], bol[MAX_STEPS];  // S^H as hi + lo BF16 operands
 #pragma unroll
   for (int t = 0; t < MAX_STEPS; ++t) {
 #pragma unroll
@@ -134,6 +143,7 @@ __global__ void __launch_bounds__(256) k_backproject_bf16(const float2* __restri
         val = oc == 0 ? (ic == 0 ? sv.x : sv.y) : (ic == 0 ? -sv.y : sv.x);
       }
       bop[t][j] = bf16_bits(val);
+      bol[t][j] = bf16_bits(val - bf16_round(val));
     }
   }
   const long long N = (long long)nx * ny * nz;
@@ -167,7 +177,7 @@ __global__ void __launch_bounds__(256) k_backproject_bf16(const float2* __restri
     for (int tt = 0; tt < MAX_STEPS; ++tt) {
       if (tt < NT) {
         f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
-        f32x4 acc = mfma16(a, bop[tt], z4);
+        f32x4 acc = mfma16(a, bol[tt], mfma16(a, bop[tt], z4));
         float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
         float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
         const int q = 8 * tt + (n >> 1);

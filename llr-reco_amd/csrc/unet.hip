@@ -68,6 +68,7 @@ __device__ __forceinline__ bf16x4 load4(const float* p) {
 }
 
 constexpr int MAX_CO_TILES = 16;  // up to 256 output channels
+constexpr int MAX_MIX = 8;        // subspace coefficients mapped to network channels
 
 struct ConvArgs {
   const void* in;       // D*H*W x cin (BF16 activations or FP32 gradients)
@@ -324,30 +325,109 @@ __global__ void k_f32_to_bf16(const float* __restrict__ a, bf16_t* __restrict__ 
     b[e] = f2bf(a[e]);
 }
 
-// K complex planes of N voxels (x fastest) -> (N, C) FP32 channels
-// [Re c0, Im c0, Re c1, ...] / scale, remaining channels zero
-__global__ void k_pack_coef(const float2* __restrict__ c, float* __restrict__ out, long long N, int K, int C,
-                            float inv_scale) {
-  const long long total = N * C;
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
-    int ch = (int)(e % C);
-    long long v = e / C;
-    float r = 0.f;
-    if (ch < 2 * K) {
-      float2 z = c[(size_t)(ch >> 1) * N + v];
-      r = ((ch & 1) ? z.y : z.x) * inv_scale;
+// Real K x K mixing of complex coefficients (row-major), passed by value.
+struct Mix {
+  float m[MAX_MIX * MAX_MIX];
+};
+
+// K complex planes of N voxels (x fastest) -> (N, C) channels
+// [Re b0, Im b0, Re b1, ...] / scale with b = M c per voxel, remaining
+// channels zero. One thread per voxel: K coalesced plane reads, C/4 16-byte
+// stores (FP32) or C/8 (BF16).
+template <typename TOut>
+__global__ void __launch_bounds__(256) k_pack_coef(const float2* __restrict__ c, TOut* __restrict__ out, long long N,
+                                                   int K, int C, Mix M, float inv_scale) {
+  for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < N; v += (long long)gridDim.x * blockDim.x) {
+    float2 z[MAX_MIX];
+#pragma unroll
+    for (int k = 0; k < MAX_MIX; ++k) z[k] = k < K ? c[(size_t)k * N + v] : make_float2(0.f, 0.f);
+    float ch[2 * MAX_MIX];
+#pragma unroll
+    for (int j = 0; j < MAX_MIX; ++j) {
+      float2 b = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < MAX_MIX; ++k) {
+        b.x += M.m[j * MAX_MIX + k] * z[k].x;
+        b.y += M.m[j * MAX_MIX + k] * z[k].y;
+      }
+      ch[2 * j] = j < K ? b.x * inv_scale : 0.f;
+      ch[2 * j + 1] = j < K ? b.y * inv_scale : 0.f;
     }
-    out[e] = r;
+    TOut* o = out + (size_t)v * C;  // C >= 2 * MAX_MIX, a multiple of 16
+    if constexpr (sizeof(TOut) == 4) {
+#pragma unroll
+      for (int q = 0; q < 2 * MAX_MIX; q += 4) *(float4*)(o + q) = make_float4(ch[q], ch[q + 1], ch[q + 2], ch[q + 3]);
+      for (int q = 2 * MAX_MIX; q < C; q += 4) *(float4*)(o + q) = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 2 * MAX_MIX; q += 8) {
+        uint4 u;
+        u.x = (unsigned)f2bf(ch[q]) | ((unsigned)f2bf(ch[q + 1]) << 16);
+        u.y = (unsigned)f2bf(ch[q + 2]) | ((unsigned)f2bf(ch[q + 3]) << 16);
+        u.z = (unsigned)f2bf(ch[q + 4]) | ((unsigned)f2bf(ch[q + 5]) << 16);
+        u.w = (unsigned)f2bf(ch[q + 6]) | ((unsigned)f2bf(ch[q + 7]) << 16);
+        *(uint4*)(o + q) = u;
+      }
+      for (int q = 2 * MAX_MIX; q < C; q += 8) *(uint4*)(o + q) = make_uint4(0u, 0u, 0u, 0u);
+    }
   }
 }
 
-__global__ void k_unpack_coef(const float* __restrict__ in, float2* __restrict__ c, long long N, int K, int C,
-                              float scale) {
-  const long long total = N * K;
+// (N, C) FP32 channels -> K complex planes: c = M b * scale per voxel
+__global__ void __launch_bounds__(256) k_unpack_coef(const float* __restrict__ in, float2* __restrict__ c, long long N,
+                                                     int K, int C, Mix M, float scale) {
+  for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < N; v += (long long)gridDim.x * blockDim.x) {
+    const float* p = in + (size_t)v * C;
+    float2 b[MAX_MIX];
+#pragma unroll
+    for (int j = 0; j < MAX_MIX; ++j) b[j] = j < K ? make_float2(p[2 * j], p[2 * j + 1]) : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < MAX_MIX; ++k) {
+      if (k < K) {
+        float2 z = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < MAX_MIX; ++j) {
+          z.x += M.m[k * MAX_MIX + j] * b[j].x;
+          z.y += M.m[k * MAX_MIX + j] * b[j].y;
+        }
+        c[(size_t)k * N + v] = make_float2(z.x * scale, z.y * scale);
+      }
+    }
+  }
+}
+
+// partial sums of |sum_k m0[k] c_k|^2 over voxels (the network's first channel pair)
+__global__ void __launch_bounds__(256) k_coef_energy(const float2* __restrict__ c, long long N, int K, Mix M,
+                                                     double* __restrict__ partial) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < N; v += (long long)gridDim.x * blockDim.x) {
+    float2 b = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < MAX_MIX; ++k) {
+      if (k < K) {
+        float2 z = c[(size_t)k * N + v];
+        b.x += M.m[k] * z.x;
+        b.y += M.m[k] * z.y;
+      }
+    }
+    s += (double)b.x * b.x + (double)b.y * b.y;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// crop (cz, cy, cx) at origin (oz, oy, ox) of a (D, H, W, C) BF16 tensor -> FP32
+__global__ void k_crop_bf16(const bf16_t* __restrict__ src, float* __restrict__ dst, int H, int W, int C, int oz,
+                            int oy, int ox, int cz, int cy, int cx) {
+  const long long total = (long long)cz * cy * cx * C;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total; e += (long long)gridDim.x * blockDim.x) {
-    long long v = e % N;
-    int k = (int)(e / N);
-    c[e] = make_float2(in[(size_t)v * C + 2 * k] * scale, in[(size_t)v * C + 2 * k + 1] * scale);
+    const int ch = (int)(e % C);
+    long long v = e / C;
+    const int x = (int)(v % cx), y = (int)((v / cx) % cy), z = (int)(v / ((long long)cx * cy));
+    dst[e] = bf2f(src[((((size_t)(z + oz) * H) + y + oy) * W + x + ox) * C + ch]);
   }
 }
 
@@ -617,20 +697,86 @@ static llr_status unet_set_input(llr_unet* U, const size_t* dims) {
   return LLR_OK;
 }
 
+static Mix make_mix(const float* m, int K) {
+  Mix M{};
+  for (int j = 0; j < MAX_MIX; ++j)
+    for (int k = 0; k < MAX_MIX; ++k)
+      M.m[j * MAX_MIX + k] = (j < K && k < K) ? (m ? m[j * K + k] : (j == k ? 1.f : 0.f)) : 0.f;
+  return M;
+}
+
+static unsigned vox_grid(size_t n) { return grid_for(n, 256, 16384); }
+
 // U-Net warm start in subspace-coefficient space: K complex planes in `coef`
-// (device) -> predicted planes in `out` (device), both scaled by `scale`.
-llr_status unet_predict_device(llr_unet* U, const float2* coef, int K, const size_t* vol, float scale, float2* out) {
-  if (2 * K > U->cin) return fail(LLR_ERR_ARG, "more subspace coefficients than network channels");
+// (device) -> predicted planes in `out` (device; may alias coef). The network
+// sees b = to_net * c / scale and its prediction p maps back to
+// from_net * p * scale (NULL matrices are the identity).
+llr_status unet_predict_device(llr_unet* U, const float2* coef, int K, const size_t* vol, float scale,
+                               const float* to_net, const float* from_net, float2* out) {
+  if (K < 1 || K > MAX_MIX || 2 * K > U->cin) return fail(LLR_ERR_ARG, "more subspace coefficients than network channels");
   llr_status st;
   if ((st = ensure_workspace(U, vol))) return st;
   const long long N = (long long)(vol[0] * vol[1] * vol[2]);
-  hipLaunchKernelGGL(k_pack_coef, dim3(ew_grid((size_t)N * U->cin)), dim3(256), 0, U->ctx->stream, coef,
-                     U->xf.as<float>(), N, K, U->cin, 1.0f / scale);
+  hipLaunchKernelGGL(k_pack_coef<float>, dim3(vox_grid((size_t)N)), dim3(256), 0, U->ctx->stream, coef,
+                     U->xf.as<float>(), N, K, U->cin, make_mix(to_net, K), 1.0f / scale);
   if ((st = unet_set_input(U, vol))) return st;
   if ((st = unet_forward(U, vol))) return st;
-  hipLaunchKernelGGL(k_unpack_coef, dim3(ew_grid((size_t)N * K)), dim3(256), 0, U->ctx->stream, U->y.as<float>(), out,
-                     N, K, U->cin, scale);
+  hipLaunchKernelGGL(k_unpack_coef, dim3(vox_grid((size_t)N)), dim3(256), 0, U->ctx->stream, U->y.as<float>(), out,
+                     N, K, U->cin, make_mix(from_net, K), scale);
   LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+llr_ctx* unet_context(llr_unet* U) { return U->ctx; }
+
+// sqrt(mean over voxels of |sum_k to_net[0][k] c_k|^2): the warm start's input scale
+llr_status unet_coef_scale(llr_ctx* ctx, const float2* coef, int K, size_t N, const float* to_net, double* scale) {
+  if (K < 1 || K > MAX_MIX) return fail(LLR_ERR_ARG, "subspace rank out of range");
+  double* part = nullptr;
+  const unsigned nb = 1024;
+  llr_status st = red_scratch(ctx, nb, &part);
+  if (st) return st;
+  Mix M = make_mix(to_net, K);  // row 0 holds the first channel's weights
+  hipLaunchKernelGGL(k_coef_energy, dim3(nb), dim3(256), 0, ctx->stream, coef, (long long)N, K, M, part);
+  LLR_HIP(hipGetLastError());
+  std::vector<double> h(nb);
+  LLR_HIP(hipStreamSynchronize(ctx->stream));
+  LLR_HIP(hipMemcpy(h.data(), part, nb * 8, hipMemcpyDeviceToHost));
+  double t = 0;
+  for (double v : h) t += v;
+  *scale = std::sqrt(t / (double)N);
+  return LLR_OK;
+}
+
+// One Adam step on the mean squared error: input staged in U->xf (FP32),
+// target staged in U->gy (replaced by the loss gradient in place).
+llr_status unet_adam_on_staged(llr_unet* U, const size_t* dims, float lr, double* loss) {
+  llr_status st;
+  const size_t n = dims[0] * dims[1] * dims[2] * U->cin;
+  hipStream_t s = U->ctx->stream;
+  if ((st = unet_set_input(U, dims)) || (st = unet_forward(U, dims))) return st;
+  const unsigned nb = 1024;
+  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, U->y.as<float>(), U->gy.as<float>(), U->gy.as<float>(),
+                     (long long)n, U->part.as<double>());
+  LLR_HIP(hipGetLastError());
+  if ((st = unet_backward(U, dims))) return st;
+  U->step += 1;
+  const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+  const double c1 = 1.0 - std::pow(b1, (double)U->step), c2 = 1.0 - std::pow(b2, (double)U->step);
+  // p -= lr * (m/c1) / (sqrt(v/c2) + eps) = lr_t * m / (sqrt(v) + eps_t)
+  const float lr_t = (float)(lr * std::sqrt(c2) / c1), eps_t = (float)(eps * std::sqrt(c2));
+  hipLaunchKernelGGL(k_adam, dim3(ew_grid(U->nparam)), dim3(256), 0, s, U->P.as<float>(), U->G.as<float>(),
+                     U->M.as<float>(), U->Vv.as<float>(), (long long)U->nparam, lr_t, (float)b1, (float)b2, eps_t);
+  LLR_HIP(hipGetLastError());
+  if ((st = pack_all(U))) return st;
+  if (loss) {
+    std::vector<double> p(nb);
+    LLR_HIP(hipStreamSynchronize(s));
+    LLR_HIP(hipMemcpy(p.data(), U->part.p, nb * 8, hipMemcpyDeviceToHost));
+    double t = 0;
+    for (double v : p) t += v;
+    *loss = t / (double)n;
+  }
   return LLR_OK;
 }
 
@@ -756,32 +902,8 @@ llr_status llr_unet_train_step(llr_unet* U, const float* x, const float* target,
   hipStream_t s = U->ctx->stream;
   const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   LLR_HIP(hipMemcpyAsync(U->xf.p, x, n * 4, kind, s));
-  if ((st = unet_set_input(U, dims)) || (st = unet_forward(U, dims))) return st;
-  // target staged in gy, replaced by the loss gradient in place
   LLR_HIP(hipMemcpyAsync(U->gy.p, target, n * 4, kind, s));
-  const unsigned nb = 1024;
-  hipLaunchKernelGGL(k_mse, dim3(nb), dim3(256), 0, s, U->y.as<float>(), U->gy.as<float>(), U->gy.as<float>(),
-                     (long long)n, U->part.as<double>());
-  LLR_HIP(hipGetLastError());
-  if ((st = unet_backward(U, dims))) return st;
-  U->step += 1;
-  const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
-  const double c1 = 1.0 - std::pow(b1, (double)U->step), c2 = 1.0 - std::pow(b2, (double)U->step);
-  // p -= lr * (m/c1) / (sqrt(v/c2) + eps) = lr_t * m / (sqrt(v) + eps_t)
-  const float lr_t = (float)(lr * std::sqrt(c2) / c1), eps_t = (float)(eps * std::sqrt(c2));
-  hipLaunchKernelGGL(k_adam, dim3(ew_grid(U->nparam)), dim3(256), 0, s, U->P.as<float>(), U->G.as<float>(),
-                     U->M.as<float>(), U->Vv.as<float>(), (long long)U->nparam, lr_t, (float)b1, (float)b2, eps_t);
-  LLR_HIP(hipGetLastError());
-  if ((st = pack_all(U))) return st;
-  if (loss) {
-    std::vector<double> p(nb);
-    LLR_HIP(hipStreamSynchronize(s));
-    LLR_HIP(hipMemcpy(p.data(), U->part.p, nb * 8, hipMemcpyDeviceToHost));
-    double t = 0;
-    for (double v : p) t += v;
-    *loss = t / (double)n;
-  }
-  return LLR_OK;
+  return unet_adam_on_staged(U, dims, lr, loss);
 }
 
 llr_status llr_unet_get_activation(llr_unet* U, int kind, int level, float* host, size_t n) {
@@ -801,27 +923,62 @@ llr_status llr_unet_get_activation(llr_unet* U, int kind, int level, float* host
 }
 
 llr_status llr_unet_predict_coefficients(llr_unet* U, const llr_cf32* d_coef, size_t K, const size_t vol[3],
-                                         float scale, llr_cf32* d_out) {
+                                         float scale, const float* to_net, const float* from_net, llr_cf32* d_out) {
   if (!U || !d_coef || !vol || !d_out) return fail(LLR_ERR_ARG, "null argument");
   if (!(scale > 0.f)) return fail(LLR_ERR_ARG, "scale must be positive");
   LLR_HIP(hipSetDevice(U->ctx->device));
-  llr_status st = unet_predict_device(U, (const float2*)d_coef, (int)K, vol, scale, (float2*)d_out);
+  llr_status st = unet_predict_device(U, (const float2*)d_coef, (int)K, vol, scale, to_net, from_net, (float2*)d_out);
   if (st) return st;
   LLR_HIP(hipStreamSynchronize(U->ctx->stream));
   return LLR_OK;
 }
 
 llr_status llr_unet_pack_coefficients(llr_ctx* ctx, const llr_cf32* d_coef, size_t K, const size_t vol[3],
-                                      int channels, float scale, float* d_out) {
+                                      int channels, float scale, const float* to_net, int bf16_out, void* d_out) {
   if (!ctx || !d_coef || !vol || !d_out) return fail(LLR_ERR_ARG, "null argument");
-  if ((int)(2 * K) > channels || !(scale > 0.f)) return fail(LLR_ERR_ARG, "bad channel count or scale");
+  if (K < 1 || K > (size_t)MAX_MIX || channels < 2 * MAX_MIX || channels % 16 || !(scale > 0.f))
+    return fail(LLR_ERR_ARG, "bad subspace rank, channel count or scale");
   LLR_HIP(hipSetDevice(ctx->device));
   const long long N = (long long)(vol[0] * vol[1] * vol[2]);
-  hipLaunchKernelGGL(k_pack_coef, dim3(ew_grid((size_t)N * channels)), dim3(256), 0, ctx->stream,
-                     (const float2*)d_coef, d_out, N, (int)K, channels, 1.0f / scale);
+  Mix M = make_mix(to_net, (int)K);
+  if (bf16_out)
+    hipLaunchKernelGGL(k_pack_coef<bf16_t>, dim3(vox_grid((size_t)N)), dim3(256), 0, ctx->stream,
+                       (const float2*)d_coef, (bf16_t*)d_out, N, (int)K, channels, M, 1.0f / scale);
+  else
+    hipLaunchKernelGGL(k_pack_coef<float>, dim3(vox_grid((size_t)N)), dim3(256), 0, ctx->stream,
+                       (const float2*)d_coef, (float*)d_out, N, (int)K, channels, M, 1.0f / scale);
   LLR_HIP(hipGetLastError());
   LLR_HIP(hipStreamSynchronize(ctx->stream));
   return LLR_OK;
+}
+
+llr_status llr_unet_coefficient_scale(llr_ctx* ctx, const llr_cf32* d_coef, size_t K, const size_t vol[3],
+                                      const float* to_net, double* scale) {
+  if (!ctx || !d_coef || !vol || !scale) return fail(LLR_ERR_ARG, "null argument");
+  LLR_HIP(hipSetDevice(ctx->device));
+  return unet_coef_scale(ctx, (const float2*)d_coef, (int)K, vol[0] * vol[1] * vol[2], to_net, scale);
+}
+
+llr_status llr_unet_train_step_crop(llr_unet* U, const void* x_full, const void* t_full, const size_t full_dims[3],
+                                    const size_t origin[3], const size_t crop_dims[3], float lr, double* loss) {
+  if (!U || !x_full || !t_full || !full_dims || !origin || !crop_dims) return fail(LLR_ERR_ARG, "null argument");
+  for (int d = 0; d < 3; ++d)
+    if (crop_dims[d] == 0 || origin[d] + crop_dims[d] > full_dims[d])
+      return fail(LLR_ERR_ARG, "crop outside the stored tensor");
+  LLR_HIP(hipSetDevice(U->ctx->device));
+  llr_status st;
+  if ((st = ensure_workspace(U, crop_dims))) return st;
+  const size_t n = crop_dims[0] * crop_dims[1] * crop_dims[2] * U->cin;
+  hipStream_t s = U->ctx->stream;
+  const int W = (int)full_dims[0], H = (int)full_dims[1];
+  const int ox = (int)origin[0], oy = (int)origin[1], oz = (int)origin[2];
+  const int cx = (int)crop_dims[0], cy = (int)crop_dims[1], cz = (int)crop_dims[2];
+  hipLaunchKernelGGL(k_crop_bf16, dim3(ew_grid(n)), dim3(256), 0, s, (const bf16_t*)x_full, U->xf.as<float>(), H, W,
+                     U->cin, oz, oy, ox, cz, cy, cx);
+  hipLaunchKernelGGL(k_crop_bf16, dim3(ew_grid(n)), dim3(256), 0, s, (const bf16_t*)t_full, U->gy.as<float>(), H, W,
+                     U->cin, oz, oy, ox, cz, cy, cx);
+  LLR_HIP(hipGetLastError());
+  return unet_adam_on_staged(U, crop_dims, lr, loss);
 }
 
 }  // extern "C"

@@ -40,7 +40,9 @@ EXPORTS = [
     "llr_recon_batch_set_measurements", "llr_recon_batch_run", "llr_recon_batch_get_image", "llr_recon_batch_tuning",
     "llr_unet_create", "llr_unet_destroy", "llr_unet_num_params", "llr_unet_get_params", "llr_unet_set_params",
     "llr_unet_forward", "llr_unet_gradients", "llr_unet_train_step", "llr_unet_predict_coefficients",
-    "llr_unet_pack_coefficients", "llr_unet_get_activation",
+    "llr_unet_pack_coefficients", "llr_unet_get_activation", "llr_unet_coefficient_scale", "llr_unet_train_step_crop",
+    "llr_recon_prepare", "llr_recon_get_coefficients", "llr_recon_set_coefficients", "llr_recon_warm_start",
+    "llr_recon_run_learned",
 ]
 
 
@@ -54,6 +56,15 @@ class LlrError(RuntimeError):
 
 class CF32(ctypes.Structure):
     _fields_ = [("re", ctypes.c_float), ("im", ctypes.c_float)]
+
+
+class WarmParams(ctypes.Structure):
+    _fields_ = [
+        ("to_net", ctypes.c_void_p),
+        ("from_net", ctypes.c_void_p),
+        ("residual_stop", ctypes.c_double),
+        ("max_iter", ctypes.c_size_t),
+    ]
 
 
 class ReconParams(ctypes.Structure):
@@ -89,6 +100,16 @@ def load():
             if not os.path.exists(LIB_PATH):
                 raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                                   "(make -C llr-reco_amd/csrc); there is no CPU fallback")
+            # torch ships its own copy of the HIP runtime under the same soname
+            # (libamdhip64.so.7). Loaded first, it is the one this library binds
+            # to, so a process that also uses torch for device memory has one
+            # HIP runtime; loaded after, torch brings a second one, and this
+            # library's runtime then failed to find the device once torch had
+            # initialised it.
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             lib = ctypes.CDLL(LIB_PATH)
             lib.llr_last_error.restype = ctypes.c_char_p
             lib.llr_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]

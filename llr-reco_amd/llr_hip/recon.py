@@ -109,6 +109,7 @@ class Recon:
         self.nq = int(nq)
         self.D = int(np.prod(self.vol)) * self.nq
         self.n_iter = int(p.n_iter)
+        self.srank = int(p.srank)
         h = ctypes.c_void_p()
         _lib.check(_lib.load().llr_recon_create(self.ctx.handle, ctypes.byref(p), ctypes.byref(h)))
         self.handle = h
@@ -129,6 +130,54 @@ class Recon:
         else:
             X = _lib.as_cf32(x, "initial image")
             _lib.check(_lib.load().llr_recon_set_initial(self.handle, _lib.ptr(X), _sz(self.D), 0))
+
+    def prepare(self):
+        """Set-up without iterating: statistics, corrections, zero-filled adjoint
+        (or the set initial image) and the SVT threshold; the next run() starts here."""
+        _lib.check(_lib.load().llr_recon_prepare(self.handle))
+
+    def coefficients(self):
+        """Subspace coefficients C = X S of the current iterate, (srank, nz, ny, nx) complex64 flat planes."""
+        out = np.empty(self.srank * (self.D // self.nq), np.complex64)
+        _lib.check(_lib.load().llr_recon_get_coefficients(self.handle, _lib.ptr(out), _sz(out.size), 0))
+        return out
+
+    def coefficients_to_device(self, device_ptr, n):
+        _lib.check(_lib.load().llr_recon_get_coefficients(self.handle, ctypes.c_void_p(int(device_ptr)), _sz(n), 1))
+
+    def set_coefficients(self, c=None, device_ptr=None, n=None):
+        if device_ptr is not None:
+            _lib.check(_lib.load().llr_recon_set_coefficients(self.handle, ctypes.c_void_p(int(device_ptr)), _sz(n), 1))
+        else:
+            C = _lib.as_cf32(c, "coefficients")
+            _lib.check(_lib.load().llr_recon_set_coefficients(self.handle, _lib.ptr(C), _sz(C.size), 0))
+
+    def warm_start(self, net, to_net=None, from_net=None):
+        """Iterate <- U-Net prediction from the zero-filled adjoint's coefficients; returns the input scale."""
+        from .unet import _mix, _mptr
+        K = self.srank
+        tn, fn = _mix(to_net, K), _mix(from_net, K)
+        sc = ctypes.c_double()
+        _lib.check(_lib.load().llr_recon_warm_start(self.handle, net.handle, _mptr(tn), _mptr(fn), ctypes.byref(sc)))
+        return sc.value
+
+    def run_learned(self, net, residual_stop, to_net=None, from_net=None, max_iter=0):
+        """Warm start + iterations to residual_stop, cold-start fallback (llr_recon_run_learned).
+        Returns (relative residual history, fell_back)."""
+        from .unet import _mix, _mptr
+        K = self.srank
+        tn, fn = _mix(to_net, K), _mix(from_net, K)
+        wp = _lib.WarmParams()
+        wp.to_net = ctypes.c_void_p(tn.ctypes.data) if tn is not None else None
+        wp.from_net = ctypes.c_void_p(fn.ctypes.data) if fn is not None else None
+        wp.residual_stop = float(residual_stop)
+        wp.max_iter = int(max_iter)
+        hist = np.zeros(max(1, self.n_iter), np.float64)
+        done = ctypes.c_size_t()
+        fb = ctypes.c_int()
+        _lib.check(_lib.load().llr_recon_run_learned(self.handle, net.handle, ctypes.byref(wp), _lib.ptr(hist),
+                                                     ctypes.byref(done), ctypes.byref(fb)))
+        return hist[:done.value], bool(fb.value)
 
     def run(self):
         """Run the iterations; returns the relative residual history (lib.rs:321)."""

@@ -76,10 +76,36 @@ class UNet:
                                                        _sz(out.size)))
         return out.reshape(shape)
 
-    def predict_coefficients_device(self, coef_ptr, K, vol, scale, out_ptr):
+    def predict_coefficients_device(self, coef_ptr, K, vol, scale, out_ptr, to_net=None, from_net=None):
+        """K complex planes (device) -> predicted planes (device; may alias)."""
+        tn, fn = _mix(to_net, K), _mix(from_net, K)
         _lib.check(_lib.load().llr_unet_predict_coefficients(
             self.handle, ctypes.c_void_p(int(coef_ptr)), _sz(K), (ctypes.c_size_t * 3)(*[int(v) for v in vol]),
-            ctypes.c_float(scale), ctypes.c_void_p(int(out_ptr))))
+            ctypes.c_float(scale), _mptr(tn), _mptr(fn), ctypes.c_void_p(int(out_ptr))))
+
+    def train_step_crop(self, x_ptr, t_ptr, full_shape, origin, crop_shape, lr=1e-3):
+        """One Adam step on a crop of stored BF16 tensors (device, (nz, ny, nx, channels)).
+        origin and crop_shape are (z, y, x)."""
+        loss = ctypes.c_double()
+        _lib.check(_lib.load().llr_unet_train_step_crop(
+            self.handle, ctypes.c_void_p(int(x_ptr)), ctypes.c_void_p(int(t_ptr)), _dims(full_shape),
+            (ctypes.c_size_t * 3)(int(origin[2]), int(origin[1]), int(origin[0])), _dims(crop_shape),
+            ctypes.c_float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def save(self, path, **meta):
+        """Parameters and the architecture (plus any metadata) as an .npz file."""
+        np.savez_compressed(path, params=self.get_params(), cin=self.cin, base=self.base, levels=self.levels,
+                            **{k: np.asarray(v) for k, v in meta.items()})
+
+    @classmethod
+    def load(cls, path, ctx=None):
+        with np.load(path, allow_pickle=False) as f:
+            net = cls(int(f["cin"]), int(f["base"]), int(f["levels"]), seed=0, ctx=ctx)
+            net.set_params(f["params"])
+            meta = {k: f[k] for k in f.files if k not in ("params", "cin", "base", "levels")}
+        net.meta = meta
+        return net
 
     def close(self):
         if getattr(self, "handle", None):
@@ -93,10 +119,35 @@ class UNet:
             pass
 
 
-def pack_coefficients_device(coef_ptr, K, vol, channels, scale, out_ptr, ctx=None):
+def _mix(m, K):
+    if m is None:
+        return None
+    m = np.ascontiguousarray(np.asarray(m, np.float32))
+    if m.shape != (K, K):
+        raise ValueError(f"mixing matrix must be {K}x{K}")
+    return m
+
+
+def _mptr(m):
+    return None if m is None else _lib.ptr(m)
+
+
+def pack_coefficients_device(coef_ptr, K, vol, channels, scale, out_ptr, to_net=None, bf16=False, ctx=None):
+    """K complex planes (device) -> (voxels, channels) network tensor (device), FP32 or BF16."""
+    tn = _mix(to_net, K)
     _lib.check(_lib.load().llr_unet_pack_coefficients(
         _lib.ctx_handle(ctx), ctypes.c_void_p(int(coef_ptr)), _sz(K), (ctypes.c_size_t * 3)(*[int(v) for v in vol]),
-        int(channels), ctypes.c_float(scale), ctypes.c_void_p(int(out_ptr))))
+        int(channels), ctypes.c_float(scale), _mptr(tn), int(bool(bf16)), ctypes.c_void_p(int(out_ptr))))
+
+
+def coefficient_scale_device(coef_ptr, K, vol, to_net=None, ctx=None):
+    """sqrt(mean over voxels of |(to_net c)_0|^2) of K complex planes (device)."""
+    tn = _mix(to_net, K)
+    out = ctypes.c_double()
+    _lib.check(_lib.load().llr_unet_coefficient_scale(
+        _lib.ctx_handle(ctx), ctypes.c_void_p(int(coef_ptr)), _sz(K), (ctypes.c_size_t * 3)(*[int(v) for v in vol]),
+        _mptr(tn), ctypes.byref(out)))
+    return out.value
 
 
 def params_to_dict(p, cin=16, base=16, levels=4):

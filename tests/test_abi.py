@@ -40,7 +40,7 @@ def test_library_carries_gfx950_code_object():
 
 def test_abi_version_and_error_string_without_gpu():
     lib = ctypes.CDLL(LIB)
-    assert lib.llr_abi_version() == 101
+    assert lib.llr_abi_version() == 102
     lib.llr_last_error.restype = ctypes.c_char_p
     assert lib.llr_last_error() == b""
     g = (ctypes.c_size_t * 3)()

@@ -1,0 +1,176 @@
+"""Learned warm start: coefficient packing, the warm-start entry points of the
+reconstruction, the cold-start fallback, and what the trained network gains
+over the cold start (llr_recon_warm_start / llr_recon_run_learned).
+
+Bars: packing within one BF16 rounding of the numpy restatement; with an
+untrained network (its output layer is zero, so it is the identity map) the
+warm start is the projection of the zero-filled adjoint onto the subspace,
+which the LLR step applies anyway, so the warm-started run reproduces the cold
+run's residuals within FP32 rounding (1e-4 relative); a fallback run is the
+cold run itself (bit-identical image).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(vol=(32, 32, 16), n_b0=1, dirs="repulsion", n_dirs=12, bval=3000.0, R=4.0, n_iter=8, blk=(8, 8, 8),
+             lambda_factor=0.02, wavelet_alpha=0.005, srank=7, n_ell=8)
+
+
+def nrmse_t(a, b):
+    import torch
+
+    return float(torch.linalg.vector_norm((a - b).to(torch.complex128)) /
+                 torch.linalg.vector_norm(b.to(torch.complex128)))
+
+
+def small_ds(seed=41):
+    from llr_hip import warm_start as W
+
+    return W.dataset_device(W.member_config(seed, SMALL))
+
+
+def test_pack_and_identity_prediction(llr):
+    import torch
+
+    from llr_hip import unet
+
+    ds = small_ds()
+    vol, K = ds["vol"], 7
+    N = int(np.prod(vol))
+    rng = np.random.default_rng(3)
+    c = (rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))).astype(np.complex64)
+    cd = torch.from_numpy(c.ravel()).cuda()
+    scale = unet.coefficient_scale_device(cd.data_ptr(), K, vol, ds["to_net"])
+    beta = ds["to_net"].astype(np.float64) @ c.astype(np.complex128)  # (K, N)
+    assert abs(scale - np.sqrt(np.mean(np.abs(beta[0]) ** 2))) < 1e-5 * scale
+    out = torch.empty((vol[2], vol[1], vol[0], 16), dtype=torch.float32, device="cuda")
+    unet.pack_coefficients_device(cd.data_ptr(), K, vol, 16, scale, out.data_ptr(), ds["to_net"], False)
+    ref = np.zeros((N, 16))
+    ref[:, 0:2 * K:2] = (beta.real / scale).T
+    ref[:, 1:2 * K:2] = (beta.imag / scale).T
+    got = out.cpu().numpy().reshape(N, 16)
+    assert np.max(np.abs(got - ref)) < 1e-5 * np.max(np.abs(ref))
+    outb = torch.empty((vol[2], vol[1], vol[0], 16), dtype=torch.int16, device="cuda")
+    unet.pack_coefficients_device(cd.data_ptr(), K, vol, 16, scale, outb.data_ptr(), ds["to_net"], True)
+    gb = (outb.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(np.float32).reshape(N, 16)
+    assert np.all(np.abs(gb - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-30)
+    # a fresh network is the identity map: prediction == input (FP32 rounding of the two mixings)
+    net = unet.UNet(16, 16, 4, seed=5)
+    pd = torch.empty_like(cd)
+    net.predict_coefficients_device(cd.data_ptr(), K, vol, scale, pd.data_ptr(), ds["to_net"], ds["from_net"])
+    e = nrmse_t(pd, cd)
+    print(f"[identity prediction] rel err {e:.2e}", flush=True)
+    assert e < 1e-5
+    net.close()
+
+
+def test_tensor_mixing_is_an_inverse_pair(llr):
+    from llr_hip import warm_start as W
+
+    ds = small_ds(43)
+    assert np.allclose(ds["to_net"].astype(np.float64) @ ds["from_net"], np.eye(7), atol=1e-5)
+
+
+def test_coefficients_round_trip(llr):
+    import torch
+
+    ds = small_ds(44)
+    from llr_hip import warm_start as W
+
+    r = W.recon_for(ds)
+    r.prepare()
+    K, N = 7, int(np.prod(ds["vol"]))
+    c0 = torch.empty(K * N, dtype=torch.complex64, device="cuda")
+    r.coefficients_to_device(c0.data_ptr(), c0.numel())
+    r.set_coefficients(device_ptr=c0.data_ptr(), n=c0.numel())
+    c1 = torch.empty_like(c0)
+    r.coefficients_to_device(c1.data_ptr(), c1.numel())
+    assert nrmse_t(c1, c0) < 1e-6
+    r.close()
+
+
+def test_identity_warm_start_reproduces_cold_run(llr):
+    import torch
+
+    from llr_hip import unet
+    from llr_hip import warm_start as W
+
+    ds = small_ds(45)
+    cold = W.recon_for(ds)
+    h_cold = cold.run()
+    img_cold = torch.empty_like(ds["y"])
+    cold.image_to_device(img_cold.data_ptr())
+    lam_cold, _ = cold.info()
+    cold.close()
+    net = unet.UNet(16, 16, 4, seed=1)
+    warm = W.recon_for(ds)
+    scale = warm.warm_start(net, ds["to_net"], ds["from_net"])
+    assert scale > 0
+    h_warm = warm.run()
+    lam_warm, _ = warm.info()
+    assert lam_warm == lam_cold  # the threshold comes from the zero-filled adjoint in both
+    rel = np.abs(h_warm - h_cold) / h_cold
+    print(f"[identity warm start] residual rel diff max {rel.max():.2e}", flush=True)
+    assert rel.max() < 1e-4
+    img = torch.empty_like(ds["y"])
+    warm.image_to_device(img.data_ptr())
+    assert nrmse_t(img, img_cold) < 1e-4
+    # learned run with an unreachable stop: falls back to the cold run itself
+    hist, fell_back = warm.run_learned(net, 1e-12, ds["to_net"], ds["from_net"])
+    assert fell_back and len(hist) == ds["cfg"]["n_iter"]
+    warm.image_to_device(img.data_ptr())
+    assert torch.equal(img, img_cold)
+    assert np.array_equal(hist, h_cold)
+    # reachable stop: the warm-started run ends as soon as it reaches it
+    thr = float(h_cold[3])
+    hist, fell_back = warm.run_learned(net, thr, ds["to_net"], ds["from_net"])
+    assert not fell_back and hist[-1] <= thr and np.all(hist[:-1] > thr)
+    assert len(hist) <= ds["cfg"]["n_iter"]
+    warm.close()
+    net.close()
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "llr-reco_amd", "llr_hip",
+                                                    "weights", "warm_start_unet.npz")),
+                    reason="no trained warm-start weights in the tree")
+def test_trained_warm_start_gains_over_cold_start(llr):
+    """Held-out seed of the training workload (128x128x96, 31 volumes): the
+    trained network's start reaches the plain run's final residual in fewer
+    iterations than the plain run, at an NRMSE vs the true image no worse than
+    1% above the plain run's, and is better than the cold start after the same
+    number of iterations."""
+    import torch
+
+    from llr_hip import warm_start as W
+
+    net = W.load_net()
+    ds = W.dataset_device(W.member_config(9001))
+    truth = ds["truth"]
+    img = torch.empty_like(ds["y"])
+    cold = W.recon_for(ds)
+    h_cold = cold.run()
+    cold.image_to_device(img.data_ptr())
+    e_cold = nrmse_t(img, truth)
+    cold.close()
+    learned = W.recon_for(ds, precision="mixed")
+    hist, fell_back = learned.run_learned(net, float(h_cold[-1]), ds["to_net"], ds["from_net"])
+    learned.image_to_device(img.data_ptr())
+    e_learned = nrmse_t(img, truth)
+    learned.close()
+    k = len(hist)
+    same = W.recon_for(ds, shifts=ds["shifts"][:k])
+    same.run()
+    same.image_to_device(img.data_ptr())
+    e_same = nrmse_t(img, truth)
+    same.close()
+    print(f"[trained warm start] iterations {k} vs {len(h_cold)}; NRMSE vs truth learned {e_learned:.5f}, "
+          f"cold@{len(h_cold)} {e_cold:.5f}, cold@{k} {e_same:.5f}; fallback {fell_back}", flush=True)
+    assert not fell_back
+    assert k < len(h_cold)
+    assert e_learned <= 1.01 * e_cold
+    assert e_learned < e_same
+    net.close()
