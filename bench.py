@@ -1,12 +1,21 @@
-"""Benchmark: seconds per LLR reconstruction at 256x256x128 with 67 volumes.
+"""Benchmark: seconds per LLR reconstruction at 256x256x128 with 67 volumes,
+learned warm start + BF16 form against the plain FP32 form at equal NRMSE.
 
-A "step" is one whole reconstruction (linear-correction estimate, zero-filled
-adjoint, SVT threshold, 30 iterations of LLR -> wavelet -> forward model ->
-hard projection -> adjoint) of one synthetic data set whose measurements are
-already resident in HBM. Prints one JSON line on stdout; logs go to stderr.
+A "step" is one whole reconstruction of one synthetic data set whose
+measurements are already resident in HBM:
+  plain FP32: linear-correction estimate, zero-filled adjoint, SVT threshold,
+    30 iterations of LLR -> wavelet -> forward model -> hard projection -> adjoint;
+  learned BF16: the same set-up, the U-Net's prediction from the adjoint's
+    subspace coefficients, then iterations of the mixed-precision form until
+    the residual falls to the plain run's converged level (the mean of its last
+    five residuals), with the plain cold start as the fallback.
+Both results are compared with the true phantom image (NRMSE in FP64). The
+headline value is the learned form's s/recon when the trained weights are in
+the tree, else the plain form's. Prints one JSON line on stdout; logs go to
+stderr.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1]
-                  [--no-cpu-baseline] [--no-stage-timing]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c4]
+                  [--no-cpu-baseline] [--no-stage-timing] [--plain-only]
 """
 import argparse
 import json
@@ -36,7 +45,7 @@ def measured_traffic(stage, ds):
         return None
 
 
-def build_measurements(preset, torch, llr):
+def build_measurements(preset, torch, llr, keep_truth=False):
     """Synthetic data set of the preset with its measurements on the GPU."""
     import numpy as np
 
@@ -50,11 +59,12 @@ def build_measurements(preset, torch, llr):
     vol = cfg["vol"]
     N = int(np.prod(vol))
     t = time.time()
-    img, truth = phantom.phantom_image(vol, bvals, bvecs, s_ph, cfg["n_ell"], return_truth=True)
-    s0_obj = float(np.mean(np.abs(img[:N][truth["labels"].ravel(order="F") > 0])))
+    img, geo = phantom.phantom_image(vol, bvals, bvecs, s_ph, cfg["n_ell"], return_truth=True)
+    s0_obj = float(np.mean(np.abs(img[:N][geo["labels"].ravel(order="F") > 0])))
     log(f"phantom {vol} x{nq}: {time.time() - t:.1f} s")
     dev = torch.device("cuda", 0)
     x = torch.from_numpy(img).to(dev)
+    truth = x.clone() if keep_truth else None
     del img
     torch.cuda.synchronize()
     zero = np.zeros((nq, 3), np.float32)
@@ -73,7 +83,7 @@ def build_measurements(preset, torch, llr):
     sub = phantom.loglinear_subspace(bvals, bvecs, cfg["srank"]) if cfg["srank"] else None
     shifts = phantom.shift_sequence(vol, cfg["n_iter"], s_shift)
     log(f"measurements on device: {time.time() - t:.1f} s, undersampling R={cfg['R']}")
-    return dict(y=x, vol=vol, nq=nq, cfg=cfg, subspace=sub, shifts=shifts)
+    return dict(y=x, vol=vol, nq=nq, cfg=cfg, subspace=sub, shifts=shifts, truth=truth, bvals=bvals, bvecs=bvecs)
 
 
 def stage_bytes(ds):
@@ -184,6 +194,7 @@ def main():
                     help="c2: 256x256x128x67 (the metric's workload); c1: 64^3x7; c4: batch of 16 uneven recons")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stage-timing", action="store_true")
+    ap.add_argument("--plain-only", action="store_true", help="time the plain FP32 form only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (counter calibration) and exit")
     args = ap.parse_args()
@@ -213,73 +224,136 @@ def main():
     llr_hip.load()
     if args.workload == "c4":
         return bench_batch(args, torch, llr_hip)
-    ds = build_measurements(args.workload, torch, llr_hip)
+    ds = build_measurements(args.workload, torch, llr_hip, keep_truth=True)
     cfg = ds["cfg"]
+
+    def nrmse(img):
+        return float(torch.linalg.vector_norm((img - ds["truth"]).to(torch.complex128)) /
+                     torch.linalg.vector_norm(ds["truth"].to(torch.complex128)))
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = None
+        for _ in range(args.steps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / max(args.steps, 1), out
+
+    img = torch.empty_like(ds["y"])
+    # ---- plain FP32 form: n_iter iterations from the zero-filled adjoint
     r = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode="svt",
                     lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"], subspace=ds["subspace"])
     r.set_measurements(device_ptr=ds["y"].data_ptr())
-
-    for _ in range(args.warmup):
-        r.run()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    hist = None
-    for _ in range(args.steps):
-        hist = r.run()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    s_per = elapsed / max(args.steps, 1)
+    s_plain, hist = timed(r.run)
+    r.image_to_device(img.data_ptr())
+    e_plain = nrmse(img)
     lam, _ = r.info()
-    log(f"timed {args.steps} recon(s): {s_per:.4f} s/recon; lambda {lam:.5g}; residual {hist[0]:.4e} -> {hist[-1]:.4e}")
-
-    roof = None
+    r_stop = float(np.mean(hist[-5:]))
+    log(f"plain FP32: {s_plain:.4f} s/recon; lambda {lam:.5g}; residual {hist[0]:.4e} -> {hist[-1]:.4e} "
+        f"(stop value {r_stop:.4e}); NRMSE vs truth {e_plain:.5f}")
     stages = {}
     if not args.no_stage_timing:
-        r.enable_timing(True)
-        r.run()
-        st = r.stage_times()
-        r.enable_timing(False)
-        nbytes = stage_bytes(ds)
-        for name, (ms, n) in st.items():
-            if n:
-                stages[name] = {"total_ms": round(ms, 3), "launches": n, "avg_ms": round(ms / n, 4)}
-                if name in nbytes:
-                    gbs = nbytes[name] / (ms / n * 1e-3) / 1e9
-                    stages[name]["alg_GBps"] = round(gbs, 1)
-        log("stage timing (HIP events on the library stream): " + json.dumps(stages))
-        kernels = [k for k in ("fft_z_project", "fft_y", "fft_x", "llr_block", "subspace_project") if k in stages]
-        dom = max(kernels, key=lambda k: stages[k]["total_ms"])
-        ach = stages[dom]["alg_GBps"]
+        stages["plain_fp32"] = stage_record(r, ds)
+    r.close()
+
+    # ---- learned warm start + BF16 form, to the plain run's residual level
+    learned = None
+    weights = os.path.join(ROOT, "llr-reco_amd", "llr_hip", "weights", "warm_start_unet.npz")
+    have_net = os.path.exists(weights) and not args.plain_only and cfg["srank"]
+    if have_net:
+        from llr_hip import warm_start as W
+
+        net = W.load_net(weights)
+        to_net, from_net = W.tensor_mixing(ds["bvals"], ds["bvecs"], ds["subspace"])
+        rl = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode="svt",
+                         lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
+                         subspace=ds["subspace"], precision="mixed")
+        rl.set_measurements(device_ptr=ds["y"].data_ptr())
+        s_learned, (hl, fell_back) = timed(lambda: rl.run_learned(net, r_stop, to_net, from_net))
+        rl.image_to_device(img.data_ptr())
+        e_learned = nrmse(img)
+        fb64, nb = rl.mixed_stats()
+        log(f"learned BF16: {s_learned:.4f} s/recon; {len(hl)} iterations (fallback {fell_back}); "
+            f"residual {hl[0]:.4e} -> {hl[-1]:.4e}; NRMSE vs truth {e_learned:.5f} "
+            f"({e_learned / e_plain:.4f} x plain); FP64 eigen-solves {fb64}/{nb} blocks")
+        if not args.no_stage_timing:
+            stages["learned_bf16"] = stage_record(rl, ds, lambda: rl.run_learned(net, r_stop, to_net, from_net))
+        learned = dict(s_per_recon=round(s_learned, 5), iterations=len(hl), fell_back=bool(fell_back),
+                       nrmse_vs_truth=round(e_learned, 6), residual_final=float(hl[-1]),
+                       fp64_eigen_blocks=int(fb64), weights=os.path.relpath(weights, ROOT),
+                       unet_training=dict(pairs=int(net.meta.get("pairs", 0)), epochs=int(net.meta.get("epochs", 0)),
+                                          seeds=str(net.meta.get("seeds", ""))))
+        rl.close()
+        net.close()
+    del img
+
+    roof = None
+    run_key = "learned_bf16" if learned else "plain_fp32"
+    if stages.get(run_key):
+        st = stages[run_key]
+        kernels = [k for k in ("fft_z_project", "fft_y", "fft_x", "llr_block", "subspace_project") if k in st]
+        dom = max(kernels, key=lambda k: st[k]["total_ms"])
+        ach = st[dom]["alg_GBps"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, ds),
-                "alg_bytes_per_launch": nbytes[dom], "avg_launch_ms": stages[dom]["avg_ms"]}
+                "alg_bytes_per_launch": stage_bytes(ds)[dom], "avg_launch_ms": st[dom]["avg_ms"]}
 
     cpu = None
     if not args.no_cpu_baseline:
-        r.close()
         cpu = cpu_baseline(ds, torch)
 
+    value = learned["s_per_recon"] if learned else s_plain
+    shape = f"{ds['vol'][0]}x{ds['vol'][1]}x{ds['vol'][2]}x{ds['nq']}"
     out = {
         "metric": "s/recon (256²×128, 67 vols): learned+BF16 vs plain FP32 at equal NRMSE",
-        "value": round(s_per, 5),
+        "value": round(value, 5),
         "unit": "s/recon",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(s_per * 1e3, 3),
+        "ms_per_step": round(value * 1e3, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "bf16+fp32" if learned else "fp32",
         "data": "synthetic (seeded DTI ellipsoid phantom, variable-density PE mask, complex Gaussian noise)",
-        "config": {"workload": (f"plain FP32 LLR recon {ds['vol'][0]}x{ds['vol'][1]}x{ds['vol'][2]}x{ds['nq']}, "
-                                f"{cfg['blk'][0]}^3 blocks, SVT 0.02*sigma_max, rank-{cfg['srank']} subspace, "
-                                f"db2 wavelet {cfg['wavelet_alpha']}, {cfg['n_iter']} iterations"),
-                   "form": "plain_fp32", "iterations": cfg["n_iter"], "stages": stages},
+        "config": {"workload": (f"LLR recon {shape}, {cfg['blk'][0]}^3 blocks, SVT 0.02*sigma_max, "
+                                f"rank-{cfg['srank']} subspace, db2 wavelet {cfg['wavelet_alpha']}; plain FP32 "
+                                f"{cfg['n_iter']} iterations vs learned warm start + BF16 form to the plain "
+                                f"residual level"),
+                   "form": "learned_bf16" if learned else "plain_fp32",
+                   "plain_fp32": dict(s_per_recon=round(s_plain, 5), iterations=cfg["n_iter"],
+                                      nrmse_vs_truth=round(e_plain, 6), residual_stop_value=r_stop,
+                                      svt_lambda=lam),
+                   "learned_bf16": learned,
+                   "speedup_vs_plain": round(s_plain / learned["s_per_recon"], 3) if learned else None,
+                   "nrmse_ratio": round(learned["nrmse_vs_truth"] / e_plain, 5) if learned else None,
+                   "equal_nrmse_within_1pct": (learned["nrmse_vs_truth"] <= 1.01 * e_plain) if learned else None,
+                   "stages": stages},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def stage_record(r, ds, run=None):
+    """Per-stage HIP-event timing of one more run of `r` (library stream)."""
+    r.enable_timing(True)
+    (run or r.run)()
+    st = r.stage_times()
+    r.enable_timing(False)
+    nbytes = stage_bytes(ds)
+    rec = {}
+    for name, (ms, n) in st.items():
+        if n:
+            rec[name] = {"total_ms": round(ms, 3), "launches": n, "avg_ms": round(ms / n, 4)}
+            if name in nbytes:
+                rec[name]["alg_GBps"] = round(nbytes[name] / (ms / n * 1e-3) / 1e9, 1)
+    log("stage timing (HIP events on the library stream): " + json.dumps(rec))
+    return rec
 
 
 if __name__ == "__main__":

@@ -369,10 +369,13 @@ llr_status llr_unet_train_step_crop(llr_unet* net, const void* x_full, const voi
 /* Learned warm start of a reconstruction with a subspace. The zero-filled
  * adjoint is computed as for a cold start (its SVT threshold is kept), its
  * subspace coefficients go through the network (mapping above, scale from
- * llr_unet_coefficient_scale) and the iterate becomes the prediction's
- * back-projection. llr_recon_run then continues from it. */
+ * llr_unet_coefficient_scale), and the iterate becomes the prediction's
+ * back-projection made data-consistent (forward model, hard projection,
+ * adjoint), as every iterate that enters the LLR step is. llr_recon_run then
+ * continues from it. scale and residual (may be NULL) receive the input scale
+ * and the prediction's relative data-consistency error. */
 llr_status llr_recon_warm_start(llr_recon* r, llr_unet* net, const float* to_net, const float* from_net,
-                                double* scale);
+                                double* scale, double* residual);
 typedef struct {
   const float* to_net;   /* K x K, see above (NULL = identity) */
   const float* from_net; /* K x K (NULL = identity) */

@@ -577,7 +577,7 @@ struct llr_recon {
   size_t N = 0, nq = 0, D = 0;
   int nc = 0;       // columns of the LLR matrices (srank or nq)
   int wL = 0;       // wavelet levels
-  DevBuf X, Y, Cb, S, coef, t1, t2, partials, stats, hist, smax, best, counters;
+  DevBuf X, Y, Cb, S, coef, t1, t2, partials, stats, hist, smax, best, counters, werr;
   RampTables tabs;
   unsigned long long llr_blocks = 0;  // blocks processed by the LLR step (mixed-form statistics)
   size_t nparts = 0;
@@ -660,17 +660,12 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
                        (int)p.rank, lambda, smax, nullptr, partial);
 }
 
-llr_status recon_iteration(llr_recon* r, size_t it) {
+// Forward model -> hard projection with its FP64 residual into *err -> adjoint
+// (lib.rs:311-333), in place on the iterate.
+llr_status data_consistency(llr_recon* r, double* err) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
-  llr_status st = recon_llr(r, &r->shifts[3 * it], p.llr_mode, (float)r->lambda, nullptr);
-  if (st) return st;
-  if (p.wavelet_alpha > 0) {
-    StageTimer t(r, 2);
-    st = wavelet_denoise_device(ctx, r->X.as<float2>(), r->nq, p.vol, (float)p.wavelet_alpha, r->wL, r->coef.as<float2>(),
-                                r->t1.as<float2>(), r->t2.as<float2>());
-    if (st) return st;
-  }
+  llr_status st;
   float2* X = r->X.as<float2>();
   const RampTables& T = r->tabs;
   float sx = (float)(1.0 / std::sqrt((double)p.vol[0])), sy = (float)(1.0 / std::sqrt((double)p.vol[1]));
@@ -702,7 +697,22 @@ llr_status recon_iteration(llr_recon* r, size_t it) {
   }
   if (st) return st;
   StageTimer t(r, 6);
-  return reduce_strided(ctx, r->partials.as<double>(), r->nparts, 1, r->hist.as<double>() + it);
+  return reduce_strided(ctx, r->partials.as<double>(), r->nparts, 1, err);
+}
+
+
+llr_status recon_iteration(llr_recon* r, size_t it) {
+  llr_ctx* ctx = r->ctx;
+  const llr_recon_params& p = r->p;
+  llr_status st = recon_llr(r, &r->shifts[3 * it], p.llr_mode, (float)r->lambda, nullptr);
+  if (st) return st;
+  if (p.wavelet_alpha > 0) {
+    StageTimer t(r, 2);
+    st = wavelet_denoise_device(ctx, r->X.as<float2>(), r->nq, p.vol, (float)p.wavelet_alpha, r->wL, r->coef.as<float2>(),
+                                r->t1.as<float2>(), r->t2.as<float2>());
+    if (st) return st;
+  }
+  return data_consistency(r, r->hist.as<double>() + it);
 }
 
 }  // namespace
@@ -770,6 +780,7 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
   if ((st = upload(r->smax, nullptr, 8))) return bail(st);
   if ((st = upload(r->best, nullptr, p.nq * 8))) return bail(st);
   if ((st = upload(r->counters, nullptr, 8))) return bail(st);
+  if ((st = upload(r->werr, nullptr, 8))) return bail(st);
   LLR_HIP(hipMemset(r->counters.p, 0, 8));
   if (hipEventCreate(&r->ev0) != hipSuccess || hipEventCreate(&r->ev1) != hipSuccess)
     return bail(fail(LLR_ERR_HIP, "hipEventCreate failed"));
@@ -900,9 +911,12 @@ static llr_status run_iterations(llr_recon* r, size_t n_max, double stop, double
 }
 
 // Set-up from the zero-filled adjoint (threshold included), then the iterate
-// becomes the U-Net's prediction from the adjoint's subspace coefficients.
+// becomes the U-Net's prediction from the adjoint's subspace coefficients,
+// made data-consistent (forward model, hard projection, adjoint) like every
+// iterate the loop hands to the LLR step. residual_out: the prediction's
+// relative data-consistency error before the projection.
 static llr_status warm_start(llr_recon* r, llr_unet* net, const float* to_net, const float* from_net,
-                             double* scale_out) {
+                             double* scale_out, double* residual_out) {
   llr_status st;
   r->have_init = false;
   if ((st = recon_setup(r))) return st;
@@ -917,9 +931,16 @@ static llr_status warm_start(llr_recon* r, llr_unet* net, const float* to_net, c
   if ((st = unet_predict_device(net, C, K, r->p.vol, (float)scale, to_net, from_net, C))) return st;
   if (nctx != r->ctx) LLR_HIP(hipStreamSynchronize(nctx->stream));
   if ((st = backproject_all(r, C))) return st;
+  if ((st = data_consistency(r, r->werr.as<double>()))) return st;
   r->have_init = true;
   r->prepared = true;
   if (scale_out) *scale_out = scale;
+  if (residual_out) {
+    double e = 0;
+    LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+    LLR_HIP(hipMemcpy(&e, r->werr.p, 8, hipMemcpyDeviceToHost));
+    *residual_out = e / r->energy;
+  }
   return LLR_OK;
 }
 
@@ -980,12 +1001,12 @@ llr_status llr_recon_set_coefficients(llr_recon* r, const llr_cf32* c, size_t le
 }
 
 llr_status llr_recon_warm_start(llr_recon* r, llr_unet* net, const float* to_net, const float* from_net,
-                                double* scale) {
+                                double* scale, double* residual) {
   if (!r || !net) return fail(LLR_ERR_ARG, "null argument");
   if (!r->p.srank) return fail(LLR_ERR_ARG, "the warm start works in subspace-coefficient space: no subspace");
   if (!r->have_meas) return fail(LLR_ERR_ARG, "measurements not set");
   LLR_HIP(hipSetDevice(r->ctx->device));
-  llr_status st = warm_start(r, net, to_net, from_net, scale);
+  llr_status st = warm_start(r, net, to_net, from_net, scale, residual);
   if (st) return st;
   LLR_HIP(hipStreamSynchronize(r->ctx->stream));
   return LLR_OK;
@@ -999,7 +1020,7 @@ llr_status llr_recon_run_learned(llr_recon* r, llr_unet* net, const llr_warm_par
   if (!(wp->residual_stop > 0)) return fail(LLR_ERR_ARG, "residual_stop must be positive");
   const size_t budget = wp->max_iter ? std::min(wp->max_iter, r->p.n_iter) : r->p.n_iter;
   LLR_HIP(hipSetDevice(r->ctx->device));
-  llr_status st = warm_start(r, net, wp->to_net, wp->from_net, nullptr);
+  llr_status st = warm_start(r, net, wp->to_net, wp->from_net, nullptr, nullptr);
   if (st) return st;
   size_t done = 0;
   std::vector<double> h(std::max<size_t>(r->p.n_iter, 1));

@@ -148,55 +148,190 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
   }
 }
 
+// 3^3 convolution with the input staged in LDS. A workgroup owns CV_M
+// consecutive voxels (each wavefront CV_MT m-tiles of 16); for every chunk of
+// 16 input channels it stages the nine (dz, dy) runs of CV_M + 2 voxels its
+// 27 offsets read, so each input value is fetched from memory about 9 / 3
+// times per workgroup instead of once per offset and lane. FP32 inputs (the
+// backward-data gradients) are rounded to BF16 while staged. Neighbours
+// outside the volume are masked per lane, as in k_conv.
+constexpr int CV_MT = 2;
+constexpr int CV_M = 64 * CV_MT;
+constexpr int CV_RUN = CV_M + 2;
+
+template <typename TIn, typename TOut, int NTM>
+__global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
+  __shared__ __align__(16) bf16_t s_run[9 * CV_RUN * 16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = lane & 15, kq = lane >> 4;
+  const long long HW = (long long)a.H * a.W, V = HW * a.D;
+  const long long vb = (long long)blockIdx.x * CV_M;
+  const int nt = a.cout / 16;
+  int vx[CV_MT], vy[CV_MT], vz[CV_MT];
+  bool vin[CV_MT];
+#pragma unroll
+  for (int mt = 0; mt < CV_MT; ++mt) {
+    const long long v = vb + (wave * CV_MT + mt) * 16 + n;
+    vin[mt] = v < V;
+    vx[mt] = (int)(v % a.W);
+    vy[mt] = (int)((v / a.W) % a.H);
+    vz[mt] = (int)(v / HW);
+  }
+  f32x4 acc[CV_MT][NTM];
+#pragma unroll
+  for (int mt = 0; mt < CV_MT; ++mt)
+#pragma unroll
+    for (int t = 0; t < NTM; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < a.cin; c0 += 16) {
+    for (int e = tid; e < 9 * CV_RUN * 2; e += 256) {
+      const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
+      const long long idx = vb - 1 + i + (long long)(r / 3 - 1) * HW + (long long)(r % 3 - 1) * a.W;
+      uint4 u = make_uint4(0u, 0u, 0u, 0u);
+      if (idx >= 0 && idx < V) {
+        if constexpr (sizeof(TIn) == 2) {
+          u = *(const uint4*)((const bf16_t*)a.in + idx * a.cin + c0 + 8 * half);
+        } else {
+          const float* q = (const float*)a.in + idx * a.cin + c0 + 8 * half;
+          const float4 f0 = *(const float4*)q, f1 = *(const float4*)(q + 4);
+          u.x = (unsigned)f2bf(f0.x) | ((unsigned)f2bf(f0.y) << 16);
+          u.y = (unsigned)f2bf(f0.z) | ((unsigned)f2bf(f0.w) << 16);
+          u.z = (unsigned)f2bf(f1.x) | ((unsigned)f2bf(f1.y) << 16);
+          u.w = (unsigned)f2bf(f1.z) | ((unsigned)f2bf(f1.w) << 16);
+        }
+      }
+      *(uint4*)(s_run + (size_t)(r * CV_RUN + i) * 16 + 8 * half) = u;
+    }
+    __syncthreads();
+    for (int o = 0; o < 27; ++o) {
+      const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1, r = o / 3;
+      bf16x4 av[CV_MT];
+#pragma unroll
+      for (int mt = 0; mt < CV_MT; ++mt) {
+        const int xx = vx[mt] + dx, yy = vy[mt] + dy, zz = vz[mt] + dz;
+        const bool valid = vin[mt] && xx >= 0 && xx < a.W && yy >= 0 && yy < a.H && zz >= 0 && zz < a.D;
+        const int pos = (wave * CV_MT + mt) * 16 + n + dx + 1;
+        av[mt] = bf16x4{0, 0, 0, 0};
+        if (valid) av[mt] = load4(s_run + (size_t)(r * CV_RUN + pos) * 16 + 4 * kq);
+      }
+      const int ow = a.mirror ? 26 - o : o;
+      const bf16_t* wo = a.w + (size_t)ow * a.cout * a.cin;
+#pragma unroll
+      for (int t = 0; t < NTM; ++t) {
+        if (t < nt) {
+          const bf16x4 bv = load4(wo + (size_t)(16 * t + n) * a.cin + c0 + 4 * kq);
+#pragma unroll
+          for (int mt = 0; mt < CV_MT; ++mt) acc[mt][t] = mfma16(av[mt], bv, acc[mt][t]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  TOut* out = (TOut*)a.out;
+#pragma unroll
+  for (int mt = 0; mt < CV_MT; ++mt) {
+#pragma unroll
+    for (int t = 0; t < NTM; ++t) {
+      if (t < nt) {
+        const int co = 16 * t + n;
+        const float b = a.bias ? a.bias[co] : 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const long long v = vb + (wave * CV_MT + mt) * 16 + 4 * kq + j;
+          if (v < V) {
+            float rr = acc[mt][t][j] + b;
+            if (a.relu) rr = fmaxf(rr, 0.f);
+            const size_t idx = (size_t)v * a.cout + co;
+            if (a.mask && !(bf2f(a.mask[idx]) > 0.f)) rr = 0.f;
+            if constexpr (sizeof(TOut) == 4) {
+              if (a.accumulate) rr += ((float*)out)[idx];
+              ((float*)out)[idx] = rr;
+            } else {
+              ((bf16_t*)out)[idx] = f2bf(rr);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
 // dW[o][co][ci] += sum_v g[v][co] x[v + off(o)][ci] over one chunk of voxels
-// per wavefront (M = 16 co, N = 16 ci, K = voxels).
+// per wavefront (M = 16 co, N = 16 ci, K = voxels), for all KK offsets at
+// once: the g operand of 16 voxels is loaded once and feeds KK MFMAs.
+template <int KK>
 __global__ void __launch_bounds__(64) k_conv_wgrad(const float* __restrict__ g, const bf16_t* __restrict__ x,
                                                    float* __restrict__ dw, int D, int H, int W, int cin, int cout,
-                                                   int k, long long chunk) {
+                                                   long long chunk) {
+  constexpr int k = KK == 27 ? 3 : 1, p = k / 2;
   const int lane = threadIdx.x;
   const int n = lane & 15, kq = lane >> 4;
-  const int p = k / 2;
-  const int tci = cin / 16, tco = cout / 16;
-  int tile = blockIdx.y;
-  const int ti = tile % tci;
-  tile /= tci;
-  const int tc = tile % tco;
-  const int o = tile / tco;
-  const int dz = o / (k * k) - p, dy = (o / k) % k - p, dx = o % k - p;
+  const int tci = cin / 16;
+  const int ti = blockIdx.y % tci, tc = blockIdx.y / tci;
   const long long V = (long long)D * H * W;
   const long long lo = (long long)blockIdx.x * chunk, hi = min(V, lo + chunk);
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  f32x4 acc[KK];
+#pragma unroll
+  for (int o = 0; o < KK; ++o) acc[o] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (long long v0 = lo; v0 < hi; v0 += 16) {
-    bf16x4 av, bv;
+    bf16x4 av;
+    int vx[4], vy[4], vz[4];
+    bool in[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const long long v = v0 + 4 * kq + j;
-      float gv = 0.f, xv = 0.f;
-      if (v < hi) {
+      in[j] = v < hi;
+      float gv = 0.f;
+      vx[j] = vy[j] = vz[j] = 0;
+      if (in[j]) {
         gv = g[(size_t)v * cout + 16 * tc + n];
-        int xx = (int)(v % W) + dx, yy = (int)((v / W) % H) + dy, zz = (int)(v / ((long long)W * H)) + dz;
-        if (xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
-          xv = bf2f(x[(((size_t)zz * H + yy) * W + xx) * cin + 16 * ti + n]);
+        vx[j] = (int)(v % W);
+        vy[j] = (int)((v / W) % H);
+        vz[j] = (int)(v / ((long long)W * H));
       }
       av[j] = (short)f2bf(gv);
-      bv[j] = (short)f2bf(xv);
     }
-    acc = mfma16(av, bv, acc);
-  }
-  float* dst = dw + (size_t)o * cout * cin;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) atomicAdd(dst + (size_t)(16 * tc + 4 * kq + j) * cin + 16 * ti + n, acc[j]);
+    for (int o = 0; o < KK; ++o) {
+      const int dz = o / (k * k) - p, dy = (o / k) % k - p, dx = o % k - p;
+      bf16x4 bv;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int xx = vx[j] + dx, yy = vy[j] + dy, zz = vz[j] + dz;
+        bf16_t xv = 0;
+        if (in[j] && xx >= 0 && xx < W && yy >= 0 && yy < H && zz >= 0 && zz < D)
+          xv = x[(((size_t)zz * H + yy) * W + xx) * cin + 16 * ti + n];
+        bv[j] = (short)xv;
+      }
+      acc[o] = mfma16(av, bv, acc[o]);
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < KK; ++o) {
+    float* dst = dw + (size_t)o * cout * cin;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) atomicAdd(dst + (size_t)(16 * tc + 4 * kq + j) * cin + 16 * ti + n, acc[o][j]);
+  }
 }
 
-// db[c] += sum_v g[v][c]
+// db[c] += sum_v g[v][c]: per-thread sums over rows of the block, an LDS
+// reduction over the block's rows, then one atomic per channel per block
 __global__ void __launch_bounds__(256) k_bias_grad(const float* __restrict__ g, float* __restrict__ db, long long V,
                                                    int c) {
-  const int ch = threadIdx.x % c;  // c divides 256 for c <= 256 multiples of 16
+  __shared__ float sh[256];
+  const int ch = threadIdx.x % c;  // c <= 256, a multiple of 16
   const int rows = 256 / c;
+  const int row = threadIdx.x / c;
   float s = 0.f;
-  for (long long v = (long long)blockIdx.x * rows + threadIdx.x / c; v < V; v += (long long)gridDim.x * rows)
-    s += g[(size_t)v * c + ch];
-  if (threadIdx.x / c < rows) atomicAdd(db + ch, s);
+  if (row < rows)
+    for (long long v = (long long)blockIdx.x * rows + row; v < V; v += (long long)gridDim.x * rows)
+      s += g[(size_t)v * c + ch];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < c) {
+    float t = 0.f;
+    for (int r = 0; r < rows; ++r) t += sh[r * c + threadIdx.x];
+    atomicAdd(db + threadIdx.x, t);
+  }
 }
 
 // 2x2x2 average pooling (BF16 -> BF16)
@@ -572,8 +707,24 @@ static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int 
   a.accumulate = accumulate ? 1 : 0;
   if (a.cout > 16 * MAX_CO_TILES || a.cin % 16 || a.cout % 16) return fail(LLR_ERR_UNSUPPORTED, "U-Net channel count");
   const long long V = (long long)a.D * a.H * a.W;
-  unsigned grid = (unsigned)((V + 63) / 64);
-  hipLaunchKernelGGL((k_conv<TIn, TOut>), dim3(grid), dim3(256), 0, U->ctx->stream, a);
+  if (a.k == 3) {
+    const int nt = a.cout / 16;
+    dim3 g((unsigned)((V + CV_M - 1) / CV_M));
+    hipStream_t s = U->ctx->stream;
+    if (nt <= 1)
+      hipLaunchKernelGGL((k_conv3<TIn, TOut, 1>), g, dim3(256), 0, s, a);
+    else if (nt <= 2)
+      hipLaunchKernelGGL((k_conv3<TIn, TOut, 2>), g, dim3(256), 0, s, a);
+    else if (nt <= 4)
+      hipLaunchKernelGGL((k_conv3<TIn, TOut, 4>), g, dim3(256), 0, s, a);
+    else if (nt <= 8)
+      hipLaunchKernelGGL((k_conv3<TIn, TOut, 8>), g, dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((k_conv3<TIn, TOut, MAX_CO_TILES>), g, dim3(256), 0, s, a);
+  } else {
+    unsigned grid = (unsigned)((V + 63) / 64);
+    hipLaunchKernelGGL((k_conv<TIn, TOut>), dim3(grid), dim3(256), 0, U->ctx->stream, a);
+  }
   LLR_HIP(hipGetLastError());
   return LLR_OK;
 }
@@ -582,12 +733,20 @@ static llr_status wgrad(llr_unet* U, const UnetLayer& u, const size_t* dims, int
                         const bf16_t* x) {
   const int W = (int)(dims[0] >> lvl), H = (int)(dims[1] >> lvl), D = (int)(dims[2] >> lvl);
   const long long V = (long long)D * H * W;
-  const long long chunk = 1024;
-  const int kk = u.k * u.k * u.k;
-  dim3 grid((unsigned)((V + chunk - 1) / chunk), (unsigned)(kk * (u.cout / 16) * (u.cin / 16)));
-  hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(64), 0, U->ctx->stream, g, x, U->G.as<float>() + u.w_off, D, H, W,
-                     u.cin, u.cout, u.k, chunk);
-  hipLaunchKernelGGL(k_bias_grad, dim3(grid_for((size_t)V, 256 / u.cout, 2048)), dim3(256), 0, U->ctx->stream, g,
+  const int tiles = (u.cout / 16) * (u.cin / 16);
+  // about 2048 wavefronts per layer, at least 256 voxels each (fewer atomics)
+  long long chunk = (V * tiles + 2047) / 2048;
+  chunk = std::max<long long>(256, (chunk + 15) / 16 * 16);
+  dim3 grid((unsigned)((V + chunk - 1) / chunk), (unsigned)tiles);
+  if (u.k == 3)
+    hipLaunchKernelGGL(k_conv_wgrad<27>, grid, dim3(64), 0, U->ctx->stream, g, x, U->G.as<float>() + u.w_off, D, H, W,
+                       u.cin, u.cout, chunk);
+  else if (u.k == 1)
+    hipLaunchKernelGGL(k_conv_wgrad<1>, grid, dim3(64), 0, U->ctx->stream, g, x, U->G.as<float>() + u.w_off, D, H, W,
+                       u.cin, u.cout, chunk);
+  else
+    return fail(LLR_ERR_UNSUPPORTED, "U-Net kernel size");
+  hipLaunchKernelGGL(k_bias_grad, dim3(grid_for((size_t)V, 256 / u.cout, 1024)), dim3(256), 0, U->ctx->stream, g,
                      U->G.as<float>() + u.b_off, V, u.cout);
   LLR_HIP(hipGetLastError());
   return LLR_OK;

@@ -4,6 +4,7 @@ The library is required: importing any compute entry point without it raises,
 there is no CPU fallback.
 """
 import ctypes
+import sys
 import os
 import threading
 
@@ -177,7 +178,11 @@ class Context:
             load().llr_ctx_destroy(self.handle)
             self.handle = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the HIP runtime may already be torn down: leave
+        # the handle to the process exit instead of calling into it
+        if _finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -6,6 +6,7 @@ the whole run (llr_recon_* in include/llr_hip.h); only the per-iteration
 residuals come back to the host.
 """
 import ctypes
+import sys
 
 import numpy as np
 
@@ -153,13 +154,17 @@ class Recon:
             _lib.check(_lib.load().llr_recon_set_coefficients(self.handle, _lib.ptr(C), _sz(C.size), 0))
 
     def warm_start(self, net, to_net=None, from_net=None):
-        """Iterate <- U-Net prediction from the zero-filled adjoint's coefficients; returns the input scale."""
+        """Iterate <- the U-Net's prediction from the zero-filled adjoint's
+        coefficients, made data-consistent. Returns (input scale, relative
+        data-consistency error of the prediction)."""
         from .unet import _mix, _mptr
         K = self.srank
         tn, fn = _mix(to_net, K), _mix(from_net, K)
         sc = ctypes.c_double()
-        _lib.check(_lib.load().llr_recon_warm_start(self.handle, net.handle, _mptr(tn), _mptr(fn), ctypes.byref(sc)))
-        return sc.value
+        res = ctypes.c_double()
+        _lib.check(_lib.load().llr_recon_warm_start(self.handle, net.handle, _mptr(tn), _mptr(fn), ctypes.byref(sc),
+                                                    ctypes.byref(res)))
+        return sc.value, res.value
 
     def run_learned(self, net, residual_stop, to_net=None, from_net=None, max_iter=0):
         """Warm start + iterations to residual_stop, cold-start fallback (llr_recon_run_learned).
@@ -221,7 +226,11 @@ class Recon:
             _lib.load().llr_recon_destroy(self.handle)
             self.handle = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the HIP runtime may already be torn down: leave
+        # the handle to the process exit instead of calling into it
+        if _finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -288,7 +297,11 @@ class ReconBatch:
             _lib.load().llr_recon_batch_destroy(self.handle)
             self.handle = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the HIP runtime may already be torn down: leave
+        # the handle to the process exit instead of calling into it
+        if _finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -2,6 +2,7 @@
 Adam on the GPU. Tensors are numpy (host) or device pointers, shaped
 (nz, ny, nx, channels) with channels fastest."""
 import ctypes
+import sys
 
 import numpy as np
 
@@ -112,7 +113,11 @@ class UNet:
             _lib.load().llr_unet_destroy(self.handle)
             self.handle = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the HIP runtime may already be torn down: leave
+        # the handle to the process exit instead of calling into it
+        if _finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -201,26 +201,63 @@ def crop_origins(rng, full, crop):
     return tuple(int(rng.integers(0, f - c + 1)) for f, c in zip(full, crop))
 
 
-def train(net, xs, ts, epochs=20, lr=1e-3, crop=(48, 64, 64), seed=0, progress=True):
+def bf16_to_f32(a):
+    """BF16 bit patterns (int16 tensor) -> float32 tensor."""
+    import torch
+
+    return (a.to(torch.int32).bitwise_and(0xFFFF) << 16).view(torch.float32)
+
+
+def validation_loss(net, vxs, vts):
+    """(network, identity map) mean squared error on whole held-out pairs."""
+    import torch
+
+    tot = ident = 0.0
+    for x, t in zip(vxs, vts):
+        xf, tf = bf16_to_f32(x).contiguous(), bf16_to_f32(t)
+        y = torch.empty_like(xf)
+        torch.cuda.synchronize()
+        net.forward_device(xf.data_ptr(), y.data_ptr(), xf.shape)
+        torch.cuda.synchronize()
+        tot += float(torch.mean((y - tf) ** 2))
+        ident += float(torch.mean((xf - tf) ** 2))
+    return tot / max(len(vxs), 1), ident / max(len(vxs), 1)
+
+
+def train(net, xs, ts, epochs=20, lr=1e-3, crop=(48, 64, 64), seed=0, progress=True, val=None, time_budget=None,
+          on_epoch=None, warmup=0):
     """Adam on random (z, y, x) crops of the stored pairs, one crop per pair per
-    epoch in a shuffled order. Returns the mean loss of every epoch."""
+    epoch in a shuffled order; the step size ramps linearly to `lr` over the
+    first `warmup` steps. Returns per-epoch records (mean training loss,
+    validation and identity-map losses when `val` = (xs, ts) is given). Stops
+    early, after a whole epoch, once `time_budget` seconds have passed."""
     import torch
 
     rng = np.random.default_rng(seed)
     full = tuple(xs[0].shape[:3])
     crop = tuple(min(c, f) for c, f in zip(crop, full))
     torch.cuda.synchronize()
-    losses = []
+    records = []
     t0 = time.time()
+    step = 0
     for ep in range(epochs):
         tot = 0.0
         for i in rng.permutation(len(xs)):
             o = crop_origins(rng, full, crop)
-            tot += net.train_step_crop(xs[i].data_ptr(), ts[i].data_ptr(), full, o, crop, lr)
-        losses.append(tot / len(xs))
+            step += 1
+            lr_s = lr * min(1.0, step / warmup) if warmup else lr
+            tot += net.train_step_crop(xs[i].data_ptr(), ts[i].data_ptr(), full, o, crop, lr_s)
+        rec = dict(epoch=ep + 1, train=tot / len(xs), t=round(time.time() - t0, 1))
+        if val is not None:
+            rec["val"], rec["identity"] = validation_loss(net, *val)
+        records.append(rec)
         if progress:
-            log(f"epoch {ep + 1}/{epochs}: loss {losses[-1]:.4e} ({time.time() - t0:.1f} s)")
-    return losses
+            log("epoch " + json.dumps(rec))
+        if on_epoch is not None:
+            on_epoch(records)
+        if time_budget is not None and time.time() - t0 > time_budget:
+            break
+    return records
 
 
 def load_net(path=None, ctx=None):
@@ -232,9 +269,12 @@ def main(argv=None):
     ap.add_argument("cmd", choices=["train"])
     ap.add_argument("--pairs", type=int, default=2000)
     ap.add_argument("--first-seed", type=int, default=1000)
+    ap.add_argument("--val", type=int, default=8, help="held-out pairs (seeds 5000+) for the validation loss")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--warmup", type=int, default=0, help="steps of linear learning-rate warm-up")
     ap.add_argument("--crop", type=int, nargs=3, default=[48, 64, 64], help="crop (z y x)")
+    ap.add_argument("--time-budget", type=float, default=None, help="stop after the epoch that passes this (s)")
     ap.add_argument("--out", default=WEIGHTS)
     args = ap.parse_args(argv)
     import torch
@@ -245,19 +285,23 @@ def main(argv=None):
     t0 = time.time()
     seeds = list(range(args.first_seed, args.first_seed + args.pairs))
     xs, ts, infos = generate_pairs(seeds)
+    vx, vt, _ = generate_pairs(list(range(5000, 5000 + args.val)), progress=0)
     t_gen = time.time() - t0
-    log(f"generated {len(xs)} pairs in {t_gen:.1f} s")
+    log(f"generated {len(xs)} + {len(vx)} pairs in {t_gen:.1f} s")
     net = unet.UNet(CHANNELS, 16, 4, seed=args.first_seed)
-    t1 = time.time()
-    losses = train(net, xs, ts, args.epochs, args.lr, tuple(args.crop), seed=args.first_seed)
-    t_train = time.time() - t1
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
-    meta = dict(pairs=len(xs), seeds=f"{seeds[0]}-{seeds[-1]}", epochs=args.epochs, lr=args.lr, crop=list(args.crop),
-                member=json.dumps({k: v for k, v in TRAIN_MEMBER.items()}), losses=np.asarray(losses, np.float64),
-                generation_s=t_gen, training_s=t_train)
-    net.save(args.out, **meta)
-    print(json.dumps(dict(out=args.out, pairs=len(xs), epochs=args.epochs, first_loss=losses[0], last_loss=losses[-1],
-                          generation_s=round(t_gen, 1), training_s=round(t_train, 1))), flush=True)
+    meta = dict(pairs=len(xs), seeds=f"{seeds[0]}-{seeds[-1]}", val_seeds=f"5000-{5000 + args.val - 1}",
+                lr=args.lr, warmup=args.warmup, crop=list(args.crop), member=json.dumps(TRAIN_MEMBER),
+                generation_s=t_gen)
+
+    def save(records):
+        net.save(args.out, epochs=len(records), records=json.dumps(records), **meta)
+
+    t1 = time.time()
+    records = train(net, xs, ts, args.epochs, args.lr, tuple(args.crop), seed=args.first_seed, val=(vx, vt),
+                    time_budget=args.time_budget, on_epoch=save, warmup=args.warmup)
+    print(json.dumps(dict(out=args.out, pairs=len(xs), epochs=len(records), first=records[0], last=records[-1],
+                          generation_s=round(t_gen, 1), training_s=round(time.time() - t1, 1))), flush=True)
     del xs, ts
     torch.cuda.empty_cache()
 

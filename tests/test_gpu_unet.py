@@ -77,15 +77,16 @@ def gpu_cache(net, x, levels=LEVELS):
     return cache
 
 
-def test_unet_gradients_vs_oracle(llr):
+@pytest.mark.parametrize("dims", [(16, 16, 16), (24, 32, 48)])
+def test_unet_gradients_vs_oracle(llr, dims):
     """Backward kernels against the oracle's backward on the GPU's own forward
     activations (with independent forwards, BF16 rounding flips change ReLU
     masks and the gradients by several percent: the comparison would test the
-    chaos, not the kernels). The forward itself is checked above."""
+    chaos, not the kernels). The forward itself is checked above. The larger
+    shape splits every weight-gradient reduction over many workgroups."""
     from llr_hip import unet
 
     net, P, rng = setup_net(9)
-    dims = (16, 16, 16)
     x = rng.standard_normal(dims + (16,)).astype(np.float32)
     gy = rng.standard_normal(dims + (16,)).astype(np.float32)
     g = unet.params_to_dict(net.gradients(x, gy))
