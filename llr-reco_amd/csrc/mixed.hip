@@ -49,19 +49,30 @@ constexpr int MAX_STEPS = 16;  // 2*nq <= 256
 // BF16 value nearest to f, as float
 __device__ __forceinline__ float bf16_round(float f) { return __uint_as_float(((unsigned)(unsigned short)bf16_bits(f)) << 16); }
 
+// Both projections work on tiles of PB_TV = 64 consecutive voxels (one
+// 16-voxel MFMA row block per wavefront) and move the nq-plane side of the
+// product through LDS, so every global access of a plane is a 512-byte run
+// (64 voxels x 8 bytes) instead of a 128-byte one per wavefront.
+constexpr int PB_TV = 64;
+constexpr int PB_ROW = 2 * 8 * MAX_STEPS + 8;  // BF16 values per staged voxel row (padded)
+
 // C[k][v] = sum_q X[q][v] S[q][k]. Row kk = 2q + c of the real operand is
-// (Re, Im)[c] of X[q]; output column n = 2k + o is (Re, Im)[o] of C[k].
-template <int K>
+// (Re, Im)[c] of X[q]; output column n = 2k + o is (Re, Im)[o] of C[k]. The
+// X tile is staged in LDS as BF16 rows [voxel][Re X0, Im X0, Re X1, ...] so a
+// lane's four K values are one 8-byte LDS read. MS bounds the K steps
+// (2 nq <= 16 MS).
+template <int K, int MS>
 __global__ void __launch_bounds__(256) k_project_bf16(const float2* __restrict__ X, float2* __restrict__ Cp,
                                                       const float2* __restrict__ S, long long N, int nq) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ __align__(16) short s_x[PB_TV * PB_ROW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = lane & 15, kq = lane >> 4;
   const int NS = (2 * nq + 15) / 16;
   // S as a sum of two BF16 operands (hi + lo), so the subspace the projection
   // uses is S to ~2^-17 and only the streamed data carries BF16 rounding
-  bf16x4 bop[MAX_STEPS], bol[MAX_STEPS];
+  bf16x4 bop[MS], bol[MS];
 #pragma unroll
-  for (int s = 0; s < MAX_STEPS; ++s) {
+  for (int s = 0; s < MS; ++s) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int kk = 16 * s + 4 * kq + j, q = kk >> 1, ic = kk & 1;
@@ -76,62 +87,77 @@ __global__ void __launch_bounds__(256) k_project_bf16(const float2* __restrict__
       bol[s][j] = bf16_bits(val - bf16_round(val));
     }
   }
-  const long long tiles = (N + 15) / 16;
-  for (long long t = (long long)blockIdx.x * 4 + wave; t < tiles; t += (long long)gridDim.x * 4) {
-    const long long v = t * 16 + n;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  const long long v0 = (long long)blockIdx.x * PB_TV;
+  // stage: plane q, voxel vi -> row vi, columns 2q, 2q+1 (zeros past nq, up to
+  // 16 NS); every load of the tile is issued before the first LDS store
+  constexpr int NLD = 8 * MS * PB_TV / 256;
+  float2 xl[NLD];
 #pragma unroll
-    for (int s = 0; s < MAX_STEPS; ++s) {
-      if (s < NS) {
-        const int q = 8 * s + 2 * kq;
-        float2 x0 = make_float2(0.f, 0.f), x1 = x0;
-        if (v < N) {
-          if (q < nq) x0 = X[(long long)q * N + v];
-          if (q + 1 < nq) x1 = X[(long long)(q + 1) * N + v];
-        }
-        bf16x4 a = {bf16_bits(x0.x), bf16_bits(x0.y), bf16_bits(x1.x), bf16_bits(x1.y)};
-        acc = mfma16(a, bop[s], acc);
-        acc = mfma16(a, bol[s], acc);
-      }
+  for (int it = 0; it < NLD; ++it) {
+    const int e = tid + 256 * it, vi = e % PB_TV, q = e / PB_TV;
+    xl[it] = make_float2(0.f, 0.f);
+    if (q < nq && v0 + vi < N) xl[it] = X[(long long)q * N + v0 + vi];
+  }
+#pragma unroll
+  for (int it = 0; it < NLD; ++it) {
+    const int e = tid + 256 * it, vi = e % PB_TV, q = e / PB_TV;
+    if (q < 8 * NS) {
+      const unsigned u =
+          (unsigned)(unsigned short)bf16_bits(xl[it].x) | ((unsigned)(unsigned short)bf16_bits(xl[it].y) << 16);
+      *(unsigned*)&s_x[vi * PB_ROW + 2 * q] = u;
     }
-    // lane holds column n of voxels 4kq..4kq+3; its xor-1 partner holds the other part
-    float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
-    float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
-    const int k = n >> 1;
-    if (k < K) {
-      const long long v0 = t * 16 + 4 * kq + ((n & 1) ? 2 : 0);
-      float2 c0, c1;
-      if ((n & 1) == 0) {
-        c0 = make_float2(acc[0], p0);
-        c1 = make_float2(acc[1], p1);
-      } else {
-        c0 = make_float2(p2, acc[2]);
-        c1 = make_float2(p3, acc[3]);
-      }
-      float2* dst = Cp + (long long)k * N + v0;
-      if (v0 + 1 < N) {
-        dst[0] = c0;
-        dst[1] = c1;
-      } else if (v0 < N) {
-        dst[0] = c0;
-      }
+  }
+  __syncthreads();
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  const short* row = s_x + (16 * wave + n) * PB_ROW + 4 * kq;
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    if (s < NS) {
+      const uint2 u = *(const uint2*)(row + 16 * s);
+      bf16x4 a = {(short)(u.x & 0xFFFF), (short)(u.x >> 16), (short)(u.y & 0xFFFF), (short)(u.y >> 16)};
+      acc = mfma16(a, bop[s], acc);
+      acc = mfma16(a, bol[s], acc);
+    }
+  }
+  // lane holds column n of voxels 4kq..4kq+3; its xor-1 partner holds the other part
+  const float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
+  const float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
+  const int k = n >> 1;
+  if (k < K) {
+    const long long w0 = v0 + 16 * wave + 4 * kq + ((n & 1) ? 2 : 0);
+    float2 c0, c1;
+    if ((n & 1) == 0) {
+      c0 = make_float2(acc[0], p0);
+      c1 = make_float2(acc[1], p1);
+    } else {
+      c0 = make_float2(p2, acc[2]);
+      c1 = make_float2(p3, acc[3]);
+    }
+    float2* dst = Cp + (long long)k * N + w0;
+    if (w0 + 1 < N) {
+      dst[0] = c0;
+      dst[1] = c1;
+    } else if (w0 < N) {
+      dst[0] = c0;
     }
   }
 }
 
 // X[q][v] = sum_k L[k][v] conj(S[q][k]) for voxels inside the block grid's
 // coverage. Row kk = 2k + c is (Re, Im)[c] of L[k]; column n = 2(q - 8t) + o of
-// output tile t is (Re, Im)[o] of X[q].
-template <int K>
+// output tile t is (Re, Im)[o] of X[q]. The product tile is gathered in LDS
+// as [q][voxel] and written out plane by plane in 512-byte runs.
+template <int K, int MS>
 __global__ void __launch_bounds__(256) k_backproject_bf16(const float2* __restrict__ L, float2* __restrict__ X,
                                                           const float2* __restrict__ S, int nx, int ny, int nz,
                                                           int cx, int cy, int cz, int sx, int sy, int sz, int nq) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ __align__(16) float2 s_o[8 * MS * PB_TV];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = lane & 15, kq = lane >> 4;
   const int NT = (2 * nq + 15) / 16;
-  bf16x4 bop[MAX_STEPS], bol[MAX_STEPS];  // S^H as hi + lo BF16 operands
+  bf16x4 bop[MS], bol[MS];  // S^H as hi + lo BF16 operands
 #pragma unroll
-  for (int t = 0; t < MAX_STEPS; ++t) {
+  for (int t = 0; t < MS; ++t) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int kk = 4 * kq + j, k = kk >> 1, ic = kk & 1;
@@ -147,55 +173,39 @@ __global__ void __launch_bounds__(256) k_backproject_bf16(const float2* __restri
     }
   }
   const long long N = (long long)nx * ny * nz;
-  const long long tiles = (N + 15) / 16;
-  for (long long t = (long long)blockIdx.x * 4 + wave; t < tiles; t += (long long)gridDim.x * 4) {
-    const long long v = t * 16 + n;
-    const int k0 = 2 * kq;
-    float2 l0 = make_float2(0.f, 0.f), l1 = l0;
-    if (v < N) {
-      if (k0 < K) l0 = L[(long long)k0 * N + v];
-      if (k0 + 1 < K) l1 = L[(long long)(k0 + 1) * N + v];
-    }
-    bf16x4 a = {bf16_bits(l0.x), bf16_bits(l0.y), bf16_bits(l1.x), bf16_bits(l1.y)};
-    // the two voxels this lane writes: 4kq + {0,1} (even n) or 4kq + {2,3} (odd n)
-    const long long w0 = t * 16 + 4 * kq + ((n & 1) ? 2 : 0);
-    bool cov[2];
+  const long long v0 = (long long)blockIdx.x * PB_TV;
+  const long long v = v0 + 16 * wave + n;
+  const int k0 = 2 * kq;
+  float2 l0 = make_float2(0.f, 0.f), l1 = l0;
+  if (v < N) {
+    if (k0 < K) l0 = L[(long long)k0 * N + v];
+    if (k0 + 1 < K) l1 = L[(long long)(k0 + 1) * N + v];
+  }
+  const bf16x4 a = {bf16_bits(l0.x), bf16_bits(l0.y), bf16_bits(l1.x), bf16_bits(l1.y)};
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      long long w = w0 + e;
-      cov[e] = false;
-      if (w < N) {
-        int x = (int)(w % nx), y = (int)((w / nx) % ny), z = (int)(w / ((long long)nx * ny));
-        int rx = x - sx, ry = y - sy, rz = z - sz;
-        if (rx < 0) rx += nx;
-        if (ry < 0) ry += ny;
-        if (rz < 0) rz += nz;
-        cov[e] = rx < cx && ry < cy && rz < cz;
-      }
-    }
+  for (int t = 0; t < MS; ++t) {
+    if (t < NT) {
+      const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+      const f32x4 acc = mfma16(a, bol[t], mfma16(a, bop[t], z4));
+      // D[4kq + j][n]: voxel 16 wave + 4kq + j, column n = (volume 8t + n/2, part n & 1)
+      const int q = 8 * t + (n >> 1);
+      float* o = (float*)(s_o + (long long)q * PB_TV + 16 * wave + 4 * kq) + (n & 1);
 #pragma unroll
-    for (int tt = 0; tt < MAX_STEPS; ++tt) {
-      if (tt < NT) {
-        f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
-        f32x4 acc = mfma16(a, bol[tt], mfma16(a, bop[tt], z4));
-        float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
-        float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
-        const int q = 8 * tt + (n >> 1);
-        if (q < nq) {
-          float2 c0, c1;
-          if ((n & 1) == 0) {
-            c0 = make_float2(acc[0], p0);
-            c1 = make_float2(acc[1], p1);
-          } else {
-            c0 = make_float2(p2, acc[2]);
-            c1 = make_float2(p3, acc[3]);
-          }
-          float2* dst = X + (long long)q * N + w0;
-          if (cov[0]) dst[0] = c0;
-          if (cov[1]) dst[1] = c1;
-        }
-      }
+      for (int j = 0; j < 4; ++j) o[2 * j] = acc[j];
     }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < 8 * MS * PB_TV / 256; ++it) {
+    const int e = tid + 256 * it, vi = e % PB_TV, q = e / PB_TV;
+    const long long w = v0 + vi;
+    if (q >= nq || w >= N) continue;
+    const int x = (int)(w % nx), y = (int)((w / nx) % ny), z = (int)(w / ((long long)nx * ny));
+    int rx = x - sx, ry = y - sy, rz = z - sz;
+    if (rx < 0) rx += nx;
+    if (ry < 0) ry += ny;
+    if (rz < 0) rz += nz;
+    if (rx < cx && ry < cy && rz < cz) X[(long long)q * N + w] = s_o[q * PB_TV + vi];
   }
 }
 
@@ -415,9 +425,16 @@ namespace llr {
 
 llr_status project_bf16_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K) {
   if (nq > 8 * MAX_STEPS) return fail(LLR_ERR_UNSUPPORTED, "BF16 projection handles at most 128 volumes");
-  unsigned grid = grid_for((N + 63) / 64, 1, 16384u);
-#define PB_CASE(KK) \
-  case KK: hipLaunchKernelGGL(k_project_bf16<KK>, dim3(grid), dim3(256), 0, ctx->stream, X, Cp, S, (long long)N, nq); break;
+  const unsigned grid = (unsigned)((N + PB_TV - 1) / PB_TV);
+  const bool small = 2 * nq <= 16 * 9;
+#define PB_CASE(KK)                                                                                              \
+  case KK:                                                                                                       \
+    if (small)                                                                                                   \
+      hipLaunchKernelGGL((k_project_bf16<KK, 9>), dim3(grid), dim3(256), 0, ctx->stream, X, Cp, S, (long long)N, nq); \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_project_bf16<KK, MAX_STEPS>), dim3(grid), dim3(256), 0, ctx->stream, X, Cp, S,          \
+                         (long long)N, nq);                                                                      \
+    break;
   switch (K) {
     PB_CASE(1) PB_CASE(2) PB_CASE(3) PB_CASE(4) PB_CASE(5) PB_CASE(6) PB_CASE(7) PB_CASE(8)
     default: return fail(LLR_ERR_UNSUPPORTED, "BF16 projection handles subspace ranks up to 8");
@@ -437,11 +454,16 @@ llr_status backproject_bf16_device(llr_ctx* ctx, const float2* C, float2* X, con
     cov[a] = (int)(partial ? vol[a] : (vol[a] / blk[a]) * blk[a]);
     sh[a] = (int)(shift[a] % vol[a]);
   }
-  unsigned grid = grid_for((N + 63) / 64, 1, 16384u);
-#define BB_CASE(KK)                                                                                                  \
-  case KK:                                                                                                           \
-    hipLaunchKernelGGL(k_backproject_bf16<KK>, dim3(grid), dim3(256), 0, ctx->stream, C, X, S, (int)vol[0],         \
-                       (int)vol[1], (int)vol[2], cov[0], cov[1], cov[2], sh[0], sh[1], sh[2], nq);                   \
+  const unsigned grid = (unsigned)((N + PB_TV - 1) / PB_TV);
+  const bool small = 2 * nq <= 16 * 9;
+#define BB_CASE(KK)                                                                                              \
+  case KK:                                                                                                       \
+    if (small)                                                                                                   \
+      hipLaunchKernelGGL((k_backproject_bf16<KK, 9>), dim3(grid), dim3(256), 0, ctx->stream, C, X, S, (int)vol[0], \
+                         (int)vol[1], (int)vol[2], cov[0], cov[1], cov[2], sh[0], sh[1], sh[2], nq);             \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_backproject_bf16<KK, MAX_STEPS>), dim3(grid), dim3(256), 0, ctx->stream, C, X, S,     \
+                         (int)vol[0], (int)vol[1], (int)vol[2], cov[0], cov[1], cov[2], sh[0], sh[1], sh[2], nq); \
     break;
   switch (K) {
     BB_CASE(1) BB_CASE(2) BB_CASE(3) BB_CASE(4) BB_CASE(5) BB_CASE(6) BB_CASE(7) BB_CASE(8)

@@ -9,22 +9,21 @@
 // to the finer level's length (pywt waverecn). Every detail subband and the
 // deepest approximation are soft-thresholded (shrink_complex, lib.rs:460-465).
 //
-// Each level is two passes over its data; a workgroup owns one x-y column of
-// tiles and marches along z, keeping the planes consecutive steps share in LDS:
-//   k_wav_an3: input planes (with their 2-sample x-y halo) are filtered along x
-//     and y into a ring of planes, the z filter turns the ring into all eight
-//     subbands of the step's coefficient planes; the seven detail bands (and
-//     the approximation at the deepest level) are shrunk and everything is
-//     written to the level's band box B (2px x 2ncy x 2ncz; px = ncx rounded
-//     up to 16 so band rows start on 128-byte lines; detail halves at px,
-//     ncy, ncz).
-//   k_wav_syn3: the eight bands' coefficient planes are kept in a ring and
-//     synthesised along z, y and x into the step's output planes.
-// The input of level l+1 is the approximation box (ncx x ncy x ncz) inside
-// level l's band box, read and written in place through its pitches: level
-// l+1's synthesis leaves the reconstructed approximation exactly where level
-// l's synthesis reads it, so no other buffer is needed. Levels run fine to
-// coarse for the analysis and coarse to fine for the synthesis.
+// One level takes three passes over its data instead of one pass per axis and
+// direction:
+//   1. k_wav_xy_an: an LDS tile (with its 2-sample halo) of one z-slice is
+//      filtered along x then y, giving the four xy-bands of the level, stored
+//      with z untouched (band buffer B, 2ncx x 2ncy x nz).
+//   2. k_wav_z: a tile of z-lines of B is analysed along z, shrunk and
+//      synthesised back in place while it sits in LDS. For the lines of the LL
+//      band the z-approximation (LLL) is the next level's input; it is written
+//      out and the line keeps only the synthesis of its shrunk detail, which
+//      is valid because synthesis is linear: syn(a', d') = syn(a', 0) + syn(0, d').
+//   3. k_wav_xy_syn: the xy-bands are synthesised along y then x from an LDS
+//      tile; the LL band picks up syn_z(a', 0) of the reconstructed deeper level
+//      while it is loaded.
+// Levels run coarse-to-fine on the way back; the deepest level handles its
+// LLL band completely in pass 2.
 #include <hip/hip_runtime.h>
 
 #include <array>
@@ -72,271 +71,302 @@ __device__ __forceinline__ void fma2(float2& acc, float w, float2 v) {
   acc.y += w * v.y;
 }
 
-// coefficient tile of the analysis (per band) and sample tile of the synthesis
-constexpr int TCX = 16, TCY = 8;          // coefficients along x, y
-constexpr int IW = 2 * TCX + 2, IH = 2 * TCY + 2;
-constexpr int TX = 32, TY = 16;           // samples along x, y
-constexpr int NX = TX / 2 + 1, NY = TY / 2 + 1;
+constexpr int AN_TCX = 32, AN_TCY = 16;              // coefficient tile of pass 1
+constexpr int AN_W = 2 * AN_TCX + 2, AN_H = 2 * AN_TCY + 2;
+constexpr int SY_TX = 64, SY_TY = 16;                // sample tile of pass 3
+constexpr int SY_NX = SY_TX / 2 + 1, SY_NY = SY_TY / 2 + 1;
+constexpr int ZT = 16;                               // z-lines per workgroup in pass 2
 
-// A box of complex samples or coefficients: element (x, y, z) of volume v at
-// base + v*vp + z*p2 + y*p1 + x.
-struct Box {
-  float2* p;
-  long long p1, p2, vp;
+constexpr int TPW = 4;  // tiles per workgroup (the next tile is prefetched into registers)
+
+struct XYAnArgs {
+  const float2* in;
+  long long ip1, ip2, ivp;  // input pitches (y, z) and volume pitch; x pitch 1
+  float2* B;
+  long long bx, by, bvp;    // band buffer row pitch (2 px), rows (2ncy) and volume pitch
+  int px;                   // padded band width (ncx rounded up to 16): the detail band starts at x = px
+  int ax, ay, az, ncx, ncy, tiles_x, groups_x;
+  Db2F f;
 };
 
-struct AnArgs {
-  Box in;
-  int ax, ay, az;            // input extent
-  Box B;                     // band box of this level
-  int px, ncx, ncy, ncz;     // detail offsets (x: px) and coefficients per band
-  int tiles_x, tiles_y;
+constexpr int AN_NLD = (AN_W * AN_H + 255) / 256;
+
+// grid.x = groups_x * tiles_y (a group is TPW consecutive x tiles of one tile row),
+// grid.y = z * volumes
+__global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
+  __shared__ float2 s_in[AN_H * AN_W];
+  __shared__ float2 s_x[AN_H * 2 * AN_TCX];
+  const int tid = threadIdx.x;
+  const int gx = blockIdx.x % a.groups_x;
+  const int cy0 = (blockIdx.x / a.groups_x) * AN_TCY;
+  const int z = blockIdx.y % a.az;
+  const long long v = blockIdx.y / a.az;
+  const float2* src = a.in + v * a.ivp + (long long)z * a.ip2;
+  float2* dst = a.B + v * a.bvp + (long long)z * a.bx * a.by;
+  const int t0 = gx * TPW;
+  const int nt = min(TPW, a.tiles_x - t0);
+  float2 xn[AN_NLD];
+  auto load = [&](int cx0) {
+#pragma unroll
+    for (int q = 0; q < AN_NLD; ++q) {
+      int e = tid + 256 * q;
+      if (e < AN_W * AN_H) {
+        int i = e % AN_W, j = e / AN_W;
+        xn[q] = src[(long long)sym(2 * cy0 - 2 + j, a.ay) * a.ip1 + sym(2 * cx0 - 2 + i, a.ax)];
+      }
+    }
+  };
+  load(t0 * AN_TCX);
+  for (int it = 0; it < nt; ++it) {
+    const int cx0 = (t0 + it) * AN_TCX;
+#pragma unroll
+    for (int q = 0; q < AN_NLD; ++q) {
+      int e = tid + 256 * q;
+      if (e < AN_W * AN_H) s_in[e] = xn[q];
+    }
+    __syncthreads();
+    // along x: local sample 2c+3-k <-> global 2(cx0+c)+1-k
+    for (int e = tid; e < AN_H * AN_TCX; e += 256) {
+      int c = e % AN_TCX, j = e / AN_TCX;
+      float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float2 x = s_in[j * AN_W + 2 * c + 3 - k];
+        fma2(sa, a.f.dlo[k], x);
+        fma2(sd, a.f.dhi[k], x);
+      }
+      s_x[j * 2 * AN_TCX + c] = sa;
+      s_x[j * 2 * AN_TCX + AN_TCX + c] = sd;
+    }
+    __syncthreads();
+    if (it + 1 < nt) load(cx0 + AN_TCX);
+    for (int e = tid; e < AN_TCY * 2 * AN_TCX; e += 256) {
+      int c2 = e % (2 * AN_TCX), cy = e / (2 * AN_TCX);
+      float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float2 x = s_x[(2 * cy + 3 - k) * 2 * AN_TCX + c2];
+        fma2(sa, a.f.dlo[k], x);
+        fma2(sd, a.f.dhi[k], x);
+      }
+      int cxl = c2 < AN_TCX ? c2 : c2 - AN_TCX;
+      if (cx0 + cxl < a.ncx && cy0 + cy < a.ncy) {
+        long long gxo = (c2 < AN_TCX) ? cx0 + cxl : a.px + cx0 + cxl;
+        dst[(long long)(cy0 + cy) * a.bx + gxo] = sa;
+        dst[(long long)(a.ncy + cy0 + cy) * a.bx + gxo] = sd;
+      }
+    }
+  }
+}
+
+struct ZArgs {
+  float2* B;
+  long long bx, by, bvp;
+  int az, ncz, ncx, ncy, px, tiles_x, groups_x;
+  float2* A;  // next-level input (ncx x ncy x ncz, row pitch pa) or null at the deepest level
+  long long avp;
+  int pa;
   int deepest;
   float alpha;
   Db2F f;
 };
 
-// Analysis, marching along z. A workgroup owns one (TCX x TCY) column of
-// coefficient tiles of one volume and steps through z, TCZ coefficient planes
-// per step. Each input plane is loaded once, filtered along x and y once, and
-// kept in a ring of 2*TCZ + 2 y-filtered planes (plane L in slot (L + 2) mod
-// ring), from which the z filter reads the 4 planes each coefficient plane
-// needs; the next step's 2*TCZ input planes are loaded into registers while
-// the current step filters along z and stores.
-constexpr int AN_TCZ = 2;
-constexpr int AN_NEW = 2 * AN_TCZ;             // input planes loaded per step
-constexpr int AN_RING = 2 * AN_TCZ + 2;        // y-filtered planes kept
-constexpr int AN_PLANE = IH * IW;              // staged input plane
-constexpr int AN_NLD = (AN_NEW * AN_PLANE + 255) / 256;
-constexpr int AN_SMEM = AN_NEW * AN_PLANE + AN_NEW * IH * 2 * TCX + AN_RING * 2 * TCY * 2 * TCX;
-
-// grid.x = tiles_x * tiles_y, grid.y = volumes
-__global__ void __launch_bounds__(256) k_wav_an3(AnArgs a) {
-  extern __shared__ float2 sm[];
-  float2* s_in = sm;                                 // [AN_NEW][IH][IW]
-  float2* s_x = s_in + AN_NEW * AN_PLANE;            // [AN_NEW][IH][2 TCX]: x-lo | x-hi
-  float2* s_y = s_x + AN_NEW * IH * 2 * TCX;         // ring [AN_RING][2 TCY][2 TCX]: y-lo | y-hi
+// grid.x = groups_x * by (TPW consecutive tiles of ZT x-neighbour z-lines at one Y), grid.y = volumes.
+// NZR >= ceil(az / 16) values of a line are prefetched per thread.
+template <int NZR>
+__global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
+  extern __shared__ float2 zs[];
+  float2* L = zs;                    // az x ZT
+  float2* Cf = zs + a.az * ZT;       // 2ncz x ZT
   const int tid = threadIdx.x;
-  const int tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x;
-  const int cx0 = tx * TCX, cy0 = ty * TCY;
+  const int t = tid % ZT, r = tid / ZT;
+  const int gx = blockIdx.x % a.groups_x;
+  const int Y = blockIdx.x / a.groups_x;
   const long long v = blockIdx.y;
-  const float2* src = a.in.p + v * a.in.vp;
-  float2* dst = a.B.p + v * a.B.vp;
-  float2 ld[AN_NLD];
-  // planes L0 .. L0 + n - 1 (logical z, reflected on load) into registers
-  auto load = [&](int L0, int n) {
+  const long long zp = a.bx * a.by;
+  float2* row = a.B + v * a.bvp + (long long)Y * a.bx;
+  const int t0 = gx * TPW;
+  const int nt = min(TPW, a.tiles_x - t0);
+  float2 xn[NZR];
+  auto load = [&](int X) {
+    const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx;
 #pragma unroll
-    for (int q = 0; q < AN_NLD; ++q) {
-      const int e = tid + 256 * q;
-      if (e < n * AN_PLANE) {
-        const int i = e % IW, r = e / IW;
-        const int j = r % IH, k = r / IH;
-        const int gx = sym(2 * cx0 - 2 + i, a.ax), gy = sym(2 * cy0 - 2 + j, a.ay), gz = sym(L0 + k, a.az);
-        ld[q] = src[(long long)gz * a.in.p2 + (long long)gy * a.in.p1 + gx];
-      }
+    for (int q = 0; q < NZR; ++q) {
+      int z = r + (256 / ZT) * q;
+      if (z < a.az) xn[q] = act ? row[X + z * zp] : make_float2(0.f, 0.f);
     }
   };
-  // registers -> LDS, then x and y filters of the n planes into ring slots of L0 ..
-  auto filter_xy = [&](int L0, int n) {
+  load(t0 * ZT + t);
+  for (int it = 0; it < nt; ++it) {
+    const int X = (t0 + it) * ZT + t;
+    const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx;
 #pragma unroll
-    for (int q = 0; q < AN_NLD; ++q) {
-      const int e = tid + 256 * q;
-      if (e < n * AN_PLANE) s_in[e] = ld[q];
+    for (int q = 0; q < NZR; ++q) {
+      int z = r + (256 / ZT) * q;
+      if (z < a.az) L[z * ZT + t] = xn[q];
     }
     __syncthreads();
-    for (int e = tid; e < n * IH * TCX; e += 256) {
-      const int c = e % TCX, row = e / TCX;
+    const bool ll = X < a.ncx && Y < a.ncy;
+    for (int c = r; c < a.ncz; c += 256 / ZT) {
       float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const float2 x = s_in[row * IW + 2 * c + 3 - k];
+        float2 x = L[sym(2 * c + 1 - k, a.az) * ZT + t];
         fma2(sa, a.f.dlo[k], x);
         fma2(sd, a.f.dhi[k], x);
       }
-      s_x[row * 2 * TCX + c] = sa;
-      s_x[row * 2 * TCX + TCX + c] = sd;
-    }
-    __syncthreads();
-    for (int e = tid; e < n * TCY * 2 * TCX; e += 256) {
-      const int c2 = e % (2 * TCX), r = e / (2 * TCX);
-      const int cy = r % TCY, k = r / TCY;
-      float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float2 x = s_x[(k * IH + 2 * cy + 3 - q) * 2 * TCX + c2];
-        fma2(sa, a.f.dlo[q], x);
-        fma2(sd, a.f.dhi[q], x);
-      }
-      const int slot = (L0 + k + 2) % AN_RING;
-      s_y[(slot * 2 * TCY + cy) * 2 * TCX + c2] = sa;
-      s_y[(slot * 2 * TCY + TCY + cy) * 2 * TCX + c2] = sd;
-    }
-    __syncthreads();
-  };
-  // the two planes below the first step (the z halo)
-  load(-2, 2);
-  filter_xy(-2, 2);
-  const int steps = (a.ncz + AN_TCZ - 1) / AN_TCZ;
-  load(0, AN_NEW);
-  for (int st = 0; st < steps; ++st) {
-    const int cz0 = st * AN_TCZ;
-    filter_xy(2 * cz0, AN_NEW);
-    if (st + 1 < steps) load(2 * cz0 + AN_NEW, AN_NEW);
-    // z: coefficient plane cz reads planes 2cz-2 .. 2cz+1 (slots 2cz .. 2cz+3 mod ring)
-    for (int e = tid; e < AN_TCZ * 2 * TCY * 2 * TCX; e += 256) {
-      const int c2 = e % (2 * TCX), r = e / (2 * TCX);
-      const int r2 = r % (2 * TCY), cz = r / (2 * TCY);
-      const int czz = cz0 + cz;
-      float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int slot = (2 * czz + 3 - q) % AN_RING;
-        const float2 x = s_y[(slot * 2 * TCY + r2) * 2 * TCX + c2];
-        fma2(sa, a.f.dlo[q], x);
-        fma2(sd, a.f.dhi[q], x);
-      }
-      const bool xh = c2 >= TCX, yh = r2 >= TCY;
-      const int cx = cx0 + (xh ? c2 - TCX : c2), cy = cy0 + (yh ? r2 - TCY : r2);
-      if (cx >= a.ncx || cy >= a.ncy || czz >= a.ncz) continue;
-      if (xh || yh || a.deepest) sa = shrink(sa, a.alpha);
       sd = shrink(sd, a.alpha);
-      const long long o = (long long)(yh ? a.ncy + cy : cy) * a.B.p1 + (xh ? a.px + cx : cx);
-      dst[(long long)czz * a.B.p2 + o] = sa;
-      dst[(long long)(a.ncz + czz) * a.B.p2 + o] = sd;
+      if (!ll || a.deepest) {
+        sa = shrink(sa, a.alpha);
+      } else {
+        a.A[v * a.avp + (long long)c * a.pa * a.ncy + (long long)Y * a.pa + X] = sa;
+        sa = make_float2(0.f, 0.f);
+      }
+      Cf[c * ZT + t] = sa;
+      Cf[(a.ncz + c) * ZT + t] = sd;
     }
-    // the ring slots the next step overwrites were read above
     __syncthreads();
+    if (it + 1 < nt) load(X + ZT);
+    for (int s = r; s < a.az; s += 256 / ZT) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int idx = s + 2 - j;
+        if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0) {
+          int tt = idx >> 1;
+          fma2(acc, a.f.rlo[j], Cf[tt * ZT + t]);
+          fma2(acc, a.f.rhi[j], Cf[(a.ncz + tt) * ZT + t]);
+        }
+      }
+      if (act) row[X + s * zp] = acc;
+    }
+    __syncthreads();  // L and Cf are rewritten by the next tile
   }
 }
 
-struct SynArgs {
-  Box B;                     // band box of this level
-  int px, ncx, ncy, ncz;
-  Box out;                   // samples of this level's input box
-  int ax, ay, az;
-  int tiles_x, tiles_y;
+struct XYSynArgs {
+  const float2* B;
+  long long bx, by, bvp;
+  int ncx, ncy, px;
+  const float2* Ap;  // reconstructed next level (ncx x ncy x ncz, row pitch pa) or null
+  long long apvp;
+  int pa;
+  int ncz;           // this level's z coefficients per band (Ap's z length)
+  float2* out;
+  long long op1, op2, ovp;
+  int ax, ay, az, tiles_x, groups_x;
   Db2F f;
 };
 
-// Synthesis, marching along z: a workgroup owns one (TX x TY) column of sample
-// tiles of one volume and steps through z, SYN_TZ samples per step. The
-// coefficient planes of the eight bands are kept in a ring of SYN_NZ planes
-// (plane c in slot c mod ring); a step needs planes z0/2 .. z0/2 + SYN_TZ/2,
-// of which SYN_TZ/2 are new, loaded into registers during the previous step.
-constexpr int SYN_TZ = 4;
-constexpr int SYN_NZ = SYN_TZ / 2 + 1;
-constexpr int SYN_NEW = SYN_TZ / 2;
-constexpr int SYN_PLANE = NY * NX;                         // one band's coefficient plane of a tile
-constexpr int SYN_NLD = (8 * SYN_NEW * SYN_PLANE + 255) / 256;
-constexpr int SYN_SMEM = 8 * SYN_NZ * SYN_PLANE + 4 * SYN_TZ * SYN_PLANE + 2 * SYN_TZ * TY * NX;
+constexpr int SY_NLD = (4 * SY_NY * SY_NX + 255) / 256;
 
-// grid.x = tiles_x * tiles_y, grid.y = volumes
-__global__ void __launch_bounds__(256) k_wav_syn3(SynArgs a) {
-  extern __shared__ float2 sm[];
-  float2* s_b = sm;                                  // ring [band][slot][iy][ix], band = xb + 2 yb + 4 zb
-  float2* s_z = s_b + 8 * SYN_NZ * SYN_PLANE;        // [xb + 2 yb][sz][iy][ix]
-  float2* s_y = s_z + 4 * SYN_TZ * SYN_PLANE;        // [xb][sz][sy][ix]
+__global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
+  __shared__ float2 band[4 * SY_NY * SY_NX];   // [xb + 2*yb][iy][ix]
+  __shared__ float2 yrow[2 * SY_TY * SY_NX];   // [xb][sy][ix]
   const int tid = threadIdx.x;
-  const int tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x;
-  const int x0 = tx * TX, y0 = ty * TY;
-  const int cx0 = x0 / 2, cy0 = y0 / 2;
-  const long long v = blockIdx.y;
-  const float2* B = a.B.p + v * a.B.vp;
-  float2 ld[SYN_NLD];
-  // coefficient planes c0 .. c0 + n - 1 of all eight bands into registers
-  auto load = [&](int c0, int n) {
+  const int gx = blockIdx.x % a.groups_x;
+  const int y0 = (blockIdx.x / a.groups_x) * SY_TY;
+  const int z = blockIdx.y % a.az;
+  const long long v = blockIdx.y / a.az;
+  const int ty0 = y0 / 2;
+  const float2* src = a.B + v * a.bvp + (long long)z * a.bx * a.by;
+  const float2* ap = a.Ap ? a.Ap + v * a.apvp : nullptr;
+  // the LL band of this z slice picks up syn_z(Ap, 0): taps j with z+2-j even and in range
+  int za = -1, zb = -1;
+  float wa = 0.f, wb = 0.f;
+  if (ap) {
 #pragma unroll
-    for (int q = 0; q < SYN_NLD; ++q) {
-      const int e = tid + 256 * q;
-      ld[q] = make_float2(0.f, 0.f);
-      if (e < 8 * n * SYN_PLANE) {
-        const int ix = e % NX, r = e / NX;
-        const int iy = r % NY, r2 = r / NY;
-        const int k = r2 % n, band = r2 / n;
-        const int cx = cx0 + ix, cy = cy0 + iy, cz = c0 + k;
-        if (cx < a.ncx && cy < a.ncy && cz < a.ncz) {
-          const long long gx = (band & 1) ? a.px + cx : cx;
-          const long long gy = (band & 2) ? a.ncy + cy : cy;
-          const long long gz = (band & 4) ? a.ncz + cz : cz;
-          ld[q] = B[gz * a.B.p2 + gy * a.B.p1 + gx];
+    for (int j = 0; j < 4; ++j) {
+      int idx = z + 2 - j;
+      if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0) {
+        if (za < 0) {
+          za = idx >> 1;
+          wa = a.f.rlo[j];
+        } else {
+          zb = idx >> 1;
+          wb = a.f.rlo[j];
+        }
+      }
+    }
+  }
+  const int t0 = gx * TPW;
+  const int nt = min(TPW, a.tiles_x - t0);
+  float2 bn[SY_NLD], an[SY_NLD], cn[SY_NLD];
+  auto load = [&](int x0) {
+    const int tx0 = x0 / 2;
+#pragma unroll
+    for (int q = 0; q < SY_NLD; ++q) {
+      int e = tid + 256 * q;
+      bn[q] = make_float2(0.f, 0.f);
+      an[q] = make_float2(0.f, 0.f);
+      cn[q] = make_float2(0.f, 0.f);
+      if (e < 4 * SY_NY * SY_NX) {
+        int ix = e % SY_NX, rem = e / SY_NX;
+        int iy = rem % SY_NY, b = rem / SY_NY;
+        int cx = tx0 + ix, cy = ty0 + iy;
+        if (cx < a.ncx && cy < a.ncy) {
+          long long gxo = (b & 1) ? a.px + cx : cx;
+          long long gyo = (b >> 1) ? a.ncy + cy : cy;
+          bn[q] = src[gyo * a.bx + gxo];
+          if (b == 0 && ap) {
+            long long o = (long long)cy * a.pa + cx;
+            if (za >= 0) an[q] = ap[(long long)za * a.pa * a.ncy + o];
+            if (zb >= 0) cn[q] = ap[(long long)zb * a.pa * a.ncy + o];
+          }
         }
       }
     }
   };
-  auto store = [&](int c0, int n) {
+  load(t0 * SY_TX);
+  float2* dst = a.out + v * a.ovp + (long long)z * a.op2;
+  for (int it = 0; it < nt; ++it) {
+    const int x0 = (t0 + it) * SY_TX, tx0 = x0 / 2;
 #pragma unroll
-    for (int q = 0; q < SYN_NLD; ++q) {
-      const int e = tid + 256 * q;
-      if (e < 8 * n * SYN_PLANE) {
-        const int ixy = e % SYN_PLANE, r2 = e / SYN_PLANE;
-        const int k = r2 % n, band = r2 / n;
-        s_b[(band * SYN_NZ + (c0 + k) % SYN_NZ) * SYN_PLANE + ixy] = ld[q];
+    for (int q = 0; q < SY_NLD; ++q) {
+      int e = tid + 256 * q;
+      if (e < 4 * SY_NY * SY_NX) {
+        float2 val = bn[q];
+        fma2(val, wa, an[q]);
+        fma2(val, wb, cn[q]);
+        band[e] = val;
       }
     }
-  };
-  float2* dst = a.out.p + v * a.out.vp;
-  const int steps = (a.az + SYN_TZ - 1) / SYN_TZ;
-  load(0, 1);
-  store(0, 1);
-  load(1, SYN_NEW);
-  for (int st = 0; st < steps; ++st) {
-    const int z0 = st * SYN_TZ, cz0 = z0 / 2;
-    store(cz0 + 1, SYN_NEW);
     __syncthreads();
-    if (st + 1 < steps) load(cz0 + 1 + SYN_NEW, SYN_NEW);
-    // along z: sample z0+sz from coefficient planes idx/2, idx = z0+sz+2-j even, inside [0, 2ncz)
-    for (int e = tid; e < 4 * SYN_TZ * SYN_PLANE; e += 256) {
-      const int ixy = e % SYN_PLANE, r = e / SYN_PLANE;
-      const int sz = r % SYN_TZ, pair = r / SYN_TZ;
+    if (it + 1 < nt) load(x0 + SY_TX);
+    // along y: yrow[xb][sy][ix] from the (xb, yb = 0 / 1) bands
+    for (int e = tid; e < 2 * SY_TY * SY_NX; e += 256) {
+      int ix = e % SY_NX, rem = e / SY_NX;
+      int sy = rem % SY_TY, xb = rem / SY_TY;
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int idx = z0 + sz + 2 - j;
-        if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0) {
-          const int slot = (idx >> 1) % SYN_NZ;
-          fma2(acc, a.f.rlo[j], s_b[(pair * SYN_NZ + slot) * SYN_PLANE + ixy]);
-          fma2(acc, a.f.rhi[j], s_b[((pair + 4) * SYN_NZ + slot) * SYN_PLANE + ixy]);
-        }
-      }
-      s_z[e] = acc;
-    }
-    __syncthreads();
-    for (int e = tid; e < 2 * SYN_TZ * TY * NX; e += 256) {
-      const int ix = e % NX, r = e / NX;
-      const int sy = r % TY, r2 = r / TY;
-      const int sz = r2 % SYN_TZ, xb = r2 / SYN_TZ;
-      float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int idx = y0 + sy + 2 - j;
+        int idx = y0 + sy + 2 - j;
         if (idx >= 0 && idx < 2 * a.ncy && (idx & 1) == 0) {
-          const int iy = (idx >> 1) - cy0;
-          fma2(acc, a.f.rlo[j], s_z[((xb * SYN_TZ + sz) * NY + iy) * NX + ix]);
-          fma2(acc, a.f.rhi[j], s_z[(((xb + 2) * SYN_TZ + sz) * NY + iy) * NX + ix]);
+          int tt = (idx >> 1) - ty0;
+          fma2(acc, a.f.rlo[j], band[(xb * SY_NY + tt) * SY_NX + ix]);
+          fma2(acc, a.f.rhi[j], band[((xb + 2) * SY_NY + tt) * SY_NX + ix]);
         }
       }
-      s_y[e] = acc;
+      yrow[e] = acc;
     }
     __syncthreads();
-    for (int e = tid; e < SYN_TZ * TY * TX; e += 256) {
-      const int sx = e % TX, r = e / TX;
-      const int sy = r % TY, sz = r / TY;
-      const int gx = x0 + sx, gy = y0 + sy, gz = z0 + sz;
-      if (gx >= a.ax || gy >= a.ay || gz >= a.az) continue;
+    for (int e = tid; e < SY_TY * SY_TX; e += 256) {
+      int sx = e % SY_TX, sy = e / SY_TX;
+      int gxo = x0 + sx, gyo = y0 + sy;
+      if (gxo >= a.ax || gyo >= a.ay) continue;
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int idx = gx + 2 - j;
+        int idx = gxo + 2 - j;
         if (idx >= 0 && idx < 2 * a.ncx && (idx & 1) == 0) {
-          const int ix = (idx >> 1) - cx0;
-          fma2(acc, a.f.rlo[j], s_y[((0 * SYN_TZ + sz) * TY + sy) * NX + ix]);
-          fma2(acc, a.f.rhi[j], s_y[((1 * SYN_TZ + sz) * TY + sy) * NX + ix]);
+          int tt = (idx >> 1) - tx0;
+          fma2(acc, a.f.rlo[j], yrow[sy * SY_NX + tt]);
+          fma2(acc, a.f.rhi[j], yrow[(SY_TY + sy) * SY_NX + tt]);
         }
       }
-      dst[(long long)gz * a.out.p2 + (long long)gy * a.out.p1 + gx] = acc;
+      dst[(long long)gyo * a.op1 + gxo] = acc;
     }
-    // the ring slot the next step overwrites and s_z / s_y are free again
-    __syncthreads();
+    __syncthreads();  // band and yrow are rewritten by the next tile
   }
 }
 
@@ -352,7 +382,7 @@ int wavelet_max_level(const size_t* dims) {
   return (int)std::floor(std::log2((double)s / 3.0));
 }
 
-// Level geometry: in_dims[l] (input box) and cdims[l] (coefficient box, 2nc per axis).
+// Level geometry: in_dims[l] (input box) and cdims[l] (coefficient box).
 void wavelet_geometry(const size_t* dims, int L, std::vector<std::array<size_t, 3>>& in_dims,
                       std::vector<std::array<size_t, 3>>& cdims) {
   in_dims.resize(L);
@@ -367,14 +397,21 @@ void wavelet_geometry(const size_t* dims, int L, std::vector<std::array<size_t, 
   }
 }
 
+
 static inline size_t pad16(size_t n) { return (n + 15) / 16 * 16; }
 
-// Per-volume workspace: the band box of every level, 2*pad16(ncx) x 2ncy x 2ncz.
+// Per-volume workspace, with rows padded to 16 elements (128 B) so every band
+// row segment is cache-line aligned: the band buffer of every level
+// (2*pad16(ncx) x 2ncy x nz) and the next-level inputs of levels 1..L-1
+// (pad16(ncx) x ncy x ncz of the level above).
 size_t wavelet_workspace_elems(const size_t* dims, int L, size_t* coef_elems, size_t* tmp_elems) {
   std::vector<std::array<size_t, 3>> in_dims, cdims;
   wavelet_geometry(dims, L, in_dims, cdims);
   size_t c = 0;
-  for (int l = 0; l < L; ++l) c += 2 * pad16(cdims[l][0] / 2) * cdims[l][1] * cdims[l][2];
+  for (int l = 0; l < L; ++l) {
+    c += 2 * pad16(cdims[l][0] / 2) * cdims[l][1] * in_dims[l][2];
+    if (l > 0) c += pad16(in_dims[l][0]) * in_dims[l][1] * in_dims[l][2];
+  }
   *coef_elems = c;
   *tmp_elems = 0;
   return c;
@@ -387,83 +424,118 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
   (void)t1;
   (void)t2;
   Db2F f = db2_filters();
-  const size_t N = dims[0] * dims[1] * dims[2];
+  size_t N = dims[0] * dims[1] * dims[2];
   if (L <= 0) return shrink_all_device(ctx, vols, N * nvols, alpha);  // the volume is its own approximation
   std::vector<std::array<size_t, 3>> in_dims, cdims;
   wavelet_geometry(dims, L, in_dims, cdims);
-  std::vector<Box> band(L), input(L);
-  std::vector<int> px(L);
+  std::vector<float2*> Bp(L), Ap(L, nullptr);
+  std::vector<size_t> bvol(L), avol(L, 0), px(L), pa(L, 0);
   {
     float2* p = coef;
     for (int l = 0; l < L; ++l) {
-      px[l] = (int)pad16(cdims[l][0] / 2);
-      Box b;
-      b.p = p;
-      b.p1 = 2 * (long long)px[l];
-      b.p2 = b.p1 * (long long)cdims[l][1];
-      b.vp = b.p2 * (long long)cdims[l][2];
-      band[l] = b;
-      p += b.vp * nvols;
+      px[l] = pad16(cdims[l][0] / 2);
+      bvol[l] = 2 * px[l] * cdims[l][1] * in_dims[l][2];
+      Bp[l] = p;
+      p += bvol[l] * nvols;
     }
-    Box x0;
-    x0.p = vols;
-    x0.p1 = (long long)dims[0];
-    x0.p2 = x0.p1 * (long long)dims[1];
-    x0.vp = x0.p2 * (long long)dims[2];
-    input[0] = x0;
-    for (int l = 1; l < L; ++l) input[l] = band[l - 1];  // the approximation box sits at the origin
-  }
-  const size_t an_smem = (size_t)AN_SMEM * sizeof(float2);
-  const size_t syn_smem = (size_t)SYN_SMEM * sizeof(float2);
-  static bool attr_set = false;
-  if (!attr_set) {
-    LLR_HIP(hipFuncSetAttribute((const void*)k_wav_an3, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)an_smem));
-    LLR_HIP(hipFuncSetAttribute((const void*)k_wav_syn3, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)syn_smem));
-    attr_set = true;
+    for (int l = 1; l < L; ++l) {
+      pa[l] = pad16(in_dims[l][0]);
+      avol[l] = pa[l] * in_dims[l][1] * in_dims[l][2];
+      Ap[l] = p;
+      p += avol[l] * nvols;
+    }
   }
   hipStream_t s = ctx->stream;
   for (int l = 0; l < L; ++l) {
     const size_t* in = in_dims[l].data();
     const size_t* cd = cdims[l].data();
-    AnArgs a{};
-    a.in = input[l];
-    a.ax = (int)in[0];
-    a.ay = (int)in[1];
-    a.az = (int)in[2];
-    a.B = band[l];
-    a.px = px[l];
-    a.ncx = (int)cd[0] / 2;
-    a.ncy = (int)cd[1] / 2;
-    a.ncz = (int)cd[2] / 2;
-    a.tiles_x = (a.ncx + TCX - 1) / TCX;
-    a.tiles_y = (a.ncy + TCY - 1) / TCY;
-    a.deepest = l == L - 1;
-    a.alpha = alpha;
-    a.f = f;
-    dim3 g((unsigned)(a.tiles_x * a.tiles_y), (unsigned)nvols);
-    hipLaunchKernelGGL(k_wav_an3, g, dim3(256), an_smem, s, a);
+    int ncx = (int)cd[0] / 2, ncy = (int)cd[1] / 2, ncz = (int)cd[2] / 2;
+    const long long bx = 2 * (long long)px[l], by = (long long)cd[1];
+    // pass 1
+    XYAnArgs a1{};
+    a1.in = l == 0 ? vols : Ap[l];
+    a1.ip1 = l == 0 ? (long long)in[0] : (long long)pa[l];
+    a1.ip2 = a1.ip1 * (long long)in[1];
+    a1.ivp = a1.ip2 * (long long)in[2];
+    a1.B = Bp[l];
+    a1.bx = bx;
+    a1.by = by;
+    a1.bvp = (long long)bvol[l];
+    a1.px = (int)px[l];
+    a1.ax = (int)in[0];
+    a1.ay = (int)in[1];
+    a1.az = (int)in[2];
+    a1.ncx = ncx;
+    a1.ncy = ncy;
+    a1.tiles_x = (ncx + AN_TCX - 1) / AN_TCX;
+    a1.groups_x = (a1.tiles_x + TPW - 1) / TPW;
+    a1.f = f;
+    int tiles_y = (ncy + AN_TCY - 1) / AN_TCY;
+    dim3 g1((unsigned)(a1.groups_x * tiles_y), (unsigned)(in[2] * nvols));
+    hipLaunchKernelGGL(k_wav_xy_an, g1, dim3(256), 0, s, a1);
+    LLR_HIP(hipGetLastError());
+    // pass 2
+    ZArgs a2{};
+    a2.B = Bp[l];
+    a2.bx = bx;
+    a2.by = by;
+    a2.bvp = (long long)bvol[l];
+    a2.az = (int)in[2];
+    a2.ncz = ncz;
+    a2.ncx = ncx;
+    a2.ncy = ncy;
+    a2.px = (int)px[l];
+    a2.tiles_x = (int)((bx + ZT - 1) / ZT);
+    a2.groups_x = (a2.tiles_x + TPW - 1) / TPW;
+    a2.deepest = l == L - 1;
+    a2.A = a2.deepest ? nullptr : Ap[l + 1];
+    a2.avp = a2.deepest ? 0 : (long long)avol[l + 1];
+    a2.pa = a2.deepest ? 0 : (int)pa[l + 1];
+    a2.alpha = alpha;
+    a2.f = f;
+    size_t smem = (size_t)(in[2] + 2 * ncz) * ZT * sizeof(float2);
+    if (smem > 160 * 1024 || in[2] > 512) return fail(LLR_ERR_UNSUPPORTED, "z length too large for the wavelet z pass");
+    dim3 g2((unsigned)(a2.groups_x * by), (unsigned)nvols);
+    int nzr = (int)((in[2] + (256 / ZT) - 1) / (256 / ZT));
+#define WZ_CASE(NN)                                                                                       \
+  if (nzr <= NN) {                                                                                        \
+    if (smem > 64 * 1024)                                                                                 \
+      LLR_HIP(hipFuncSetAttribute((const void*)k_wav_z<NN>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                  (int)smem));                                                            \
+    hipLaunchKernelGGL(k_wav_z<NN>, g2, dim3(256), smem, s, a2);                                          \
+  } else
+    WZ_CASE(1) WZ_CASE(2) WZ_CASE(4) WZ_CASE(8) WZ_CASE(16) WZ_CASE(32) {}
+#undef WZ_CASE
     LLR_HIP(hipGetLastError());
   }
   for (int l = L - 1; l >= 0; --l) {
     const size_t* od = in_dims[l].data();
     const size_t* cd = cdims[l].data();
-    SynArgs a{};
-    a.B = band[l];
-    a.px = px[l];
-    a.ncx = (int)cd[0] / 2;
-    a.ncy = (int)cd[1] / 2;
-    a.ncz = (int)cd[2] / 2;
-    a.out = input[l];
-    a.ax = (int)od[0];
-    a.ay = (int)od[1];
-    a.az = (int)od[2];
-    a.tiles_x = (a.ax + TX - 1) / TX;
-    a.tiles_y = (a.ay + TY - 1) / TY;
-    a.f = f;
-    dim3 g((unsigned)(a.tiles_x * a.tiles_y), (unsigned)nvols);
-    hipLaunchKernelGGL(k_wav_syn3, g, dim3(256), syn_smem, s, a);
+    XYSynArgs a3{};
+    a3.B = Bp[l];
+    a3.bx = 2 * (long long)px[l];
+    a3.by = (long long)cd[1];
+    a3.bvp = (long long)bvol[l];
+    a3.ncx = (int)cd[0] / 2;
+    a3.ncy = (int)cd[1] / 2;
+    a3.px = (int)px[l];
+    a3.ncz = (int)cd[2] / 2;
+    a3.Ap = (l < L - 1) ? Ap[l + 1] : nullptr;
+    a3.apvp = (l < L - 1) ? (long long)avol[l + 1] : 0;
+    a3.pa = (l < L - 1) ? (int)pa[l + 1] : 0;
+    a3.out = l == 0 ? vols : Ap[l];
+    a3.op1 = l == 0 ? (long long)od[0] : (long long)pa[l];
+    a3.op2 = a3.op1 * (long long)od[1];
+    a3.ovp = a3.op2 * (long long)od[2];
+    a3.ax = (int)od[0];
+    a3.ay = (int)od[1];
+    a3.az = (int)od[2];
+    a3.tiles_x = (int)((od[0] + SY_TX - 1) / SY_TX);
+    a3.groups_x = (a3.tiles_x + TPW - 1) / TPW;
+    a3.f = f;
+    int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
+    dim3 g3((unsigned)(a3.groups_x * tiles_y), (unsigned)(od[2] * nvols));
+    hipLaunchKernelGGL(k_wav_xy_syn, g3, dim3(256), 0, s, a3);
     LLR_HIP(hipGetLastError());
   }
   return LLR_OK;
