@@ -34,11 +34,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(stage, ds):
+def measured_traffic(stage, ds, form):
     """HBM bytes per launch of a stage from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary of the same workload (copy-calibrated), or None if there is none."""
+    summary of the same workload and form (copy-calibrated), or None if there is none."""
     v = ds["vol"]
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{v[0]}x{v[1]}x{v[2]}x{ds['nq']}_plain_fp32.json")
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{v[0]}x{v[1]}x{v[2]}x{ds['nq']}_{form}.json")
     try:
         with open(path) as f:
             return json.load(f)["stages"][stage]["bytes_per_launch"]
@@ -87,20 +87,24 @@ def build_measurements(preset, torch, llr, keep_truth=False):
     return dict(y=x, vol=vol, nq=nq, cfg=cfg, subspace=sub, shifts=shifts, truth=truth, bvals=bvals, bvecs=bvecs)
 
 
-def stage_bytes(ds):
-    """Algorithmic HBM bytes per launch of each timed kernel."""
+def stage_bytes(ds, eb=8):
+    """Algorithmic HBM bytes per launch of each timed kernel; eb = bytes per stored
+    complex value of the iterate (8 complex64, 4 half pairs). With a subspace the
+    LLR step works on the K coefficient planes in place (complex64) and the
+    wavelet's first level reads them and writes the iterate (the back-projection
+    is folded into it)."""
     N = int(ds["vol"][0] * ds["vol"][1] * ds["vol"][2])
     Q = ds["nq"]
+    sub = bool(ds["cfg"]["srank"])
     K = ds["cfg"]["srank"] or Q
-    D = N * Q * 8
+    D = N * Q * eb
     return {
-        # gather K coefficient planes, write Q image planes (with a subspace) or K planes in place
-        "llr_block": N * K * 8 + N * (Q if ds["cfg"]["srank"] else K) * 8,
-        "subspace_project": D + N * K * 8,  # read X, write C
-        "wavelet": 2 * D,                    # one call: read and write the iterate once
+        "llr_block": 2 * N * K * 8 if sub else 2 * D,
+        "subspace_project": D + N * K * 8,               # read X, write C
+        "wavelet": N * K * 8 + D if sub else 2 * D,      # read C (or X), write X
         "fft_x": 2 * D,
         "fft_y": 2 * D,
-        "fft_z_project": 3 * D,              # read X, read Y, write X
+        "fft_z_project": 3 * D,                          # read X, read Y, write X
     }
 
 
@@ -196,6 +200,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stage-timing", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="time the plain FP32 form only")
+    ap.add_argument("--storage", default="fp16", choices=["fp16", "fp32"],
+                    help="storage of the learned form's iterate, measurements copy and wavelet workspace")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (counter calibration) and exit")
     args = ap.parse_args()
@@ -271,7 +277,7 @@ def main():
         to_net, from_net = W.tensor_mixing(ds["bvals"], ds["bvecs"], ds["subspace"])
         rl = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode="svt",
                          lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
-                         subspace=ds["subspace"], precision="mixed")
+                         subspace=ds["subspace"], precision="mixed", storage=args.storage)
         rl.set_measurements(device_ptr=ds["y"].data_ptr())
         s_learned, (hl, fell_back) = timed(lambda: rl.run_learned(net, r_stop, to_net, from_net))
         rl.image_to_device(img.data_ptr())
@@ -281,8 +287,9 @@ def main():
             f"residual {hl[0]:.4e} -> {hl[-1]:.4e}; NRMSE vs truth {e_learned:.5f} "
             f"({e_learned / e_plain:.4f} x plain); FP64 eigen-solves {fb64}/{nb} blocks")
         if not args.no_stage_timing:
-            stages["learned_bf16"] = stage_record(rl, ds, lambda: rl.run_learned(net, r_stop, to_net, from_net))
-        learned = dict(s_per_recon=round(s_learned, 5), iterations=len(hl), fell_back=bool(fell_back),
+            stages["learned_bf16"] = stage_record(rl, ds, lambda: rl.run_learned(net, r_stop, to_net, from_net),
+                                                  eb=4 if args.storage == "fp16" else 8)
+        learned = dict(s_per_recon=round(s_learned, 5), iterations=len(hl), fell_back=bool(fell_back), storage=args.storage,
                        nrmse_vs_truth=round(e_learned, 6), residual_final=float(hl[-1]),
                        fp64_eigen_blocks=int(fb64), weights=os.path.relpath(weights, ROOT),
                        unet_training=dict(pairs=int(net.meta.get("pairs", 0)), epochs=int(net.meta.get("epochs", 0)),
@@ -298,9 +305,11 @@ def main():
         kernels = [k for k in ("fft_z_project", "fft_y", "fft_x", "llr_block", "subspace_project") if k in st]
         dom = max(kernels, key=lambda k: st[k]["total_ms"])
         ach = st[dom]["alg_GBps"]
+        eb = 4 if (learned and args.storage == "fp16") else 8
+        form = ("learned_fp16" if eb == 4 else "learned_fp32") if learned else "plain_fp32"
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, ds),
-                "alg_bytes_per_launch": stage_bytes(ds)[dom], "avg_launch_ms": st[dom]["avg_ms"]}
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, ds, form),
+                "alg_bytes_per_launch": stage_bytes(ds, eb)[dom], "avg_launch_ms": st[dom]["avg_ms"]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -340,13 +349,13 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def stage_record(r, ds, run=None):
+def stage_record(r, ds, run=None, eb=8):
     """Per-stage HIP-event timing of one more run of `r` (library stream)."""
     r.enable_timing(True)
     (run or r.run)()
     st = r.stage_times()
     r.enable_timing(False)
-    nbytes = stage_bytes(ds)
+    nbytes = stage_bytes(ds, eb)
     rec = {}
     for name, (ms, n) in st.items():
         if n:

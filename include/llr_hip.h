@@ -224,6 +224,12 @@ llr_status llr_dti_scalars_device(llr_ctx* ctx, const llr_cf32* d_volumes, size_
 #define LLR_PREC_MIXED 1 /* BF16 MFMA Gram and projection (FP32 accumulation), one FP32 Jacobi
                           refinement sweep, FP64 eigen-solve for blocks with Gram condition > 1e6 */
 
+#define LLR_STORE_FP32 0 /* complex64 */
+#define LLR_STORE_FP16 1 /* two IEEE half-precision numbers per complex value, FP32 arithmetic in
+                            every kernel, data scaled by sqrt(voxels / measurement energy) while
+                            stored (outputs, coefficients and thresholds in the caller's units);
+                            LLR_PREC_MIXED with a subspace only (halves the HBM bytes per pass) */
+
 typedef struct {
   size_t vol[3];         /* nx, ny, nz */
   size_t nq;             /* volumes */
@@ -243,6 +249,8 @@ typedef struct {
   int partial_blocks;    /* 1: ceil(vol/blk) block grid whose edge blocks are truncated at the
                             volume end (extension; the reference grid is floor(vol/blk),
                             block.rs:112-118, leaving the remainder voxels out of the LLR step) */
+  int storage;           /* LLR_STORE_*: how the iterate, the measurements' working copy and the
+                            wavelet workspace are held on the device (extension) */
 } llr_recon_params;
 
 /* Create a reconstruction (allocates the iterate, the measurements and all
@@ -287,8 +295,9 @@ llr_status llr_recon_mixed_stats(llr_recon* r, unsigned long long* fp64_blocks, 
 /* Stage timing with HIP events on the context's stream (for roofline reports).
  * Stages: 0 LLR block kernel, 1 subspace projection, 2 wavelet, 3 FFT x pass,
  * 4 FFT y pass, 5 FFT z pass + hard projection, 6 residual reduction, 7 run set-up
- * (statistics, corrections, zero-filled adjoint, threshold). */
-#define LLR_NUM_STAGES 8
+ * (statistics, corrections, zero-filled adjoint, threshold), 8 warm-start U-Net
+ * prediction (packing, network, unpacking). */
+#define LLR_NUM_STAGES 9
 llr_status llr_recon_enable_timing(llr_recon* r, int enable);
 llr_status llr_recon_stage_times(llr_recon* r, double* total_ms, size_t* launches);
 

@@ -114,16 +114,19 @@ static llr_status check_fft_dims(const size_t* dims) {
 }
 
 // signal model (signal_model.rs:34-79) on device data; src -> dst (may alias)
-static llr_status model_device(llr_ctx* ctx, const float2* src, float2* dst, const size_t* dims, size_t nv,
-                               const RampTables& t, int dir) {
-  const float2* in = src;
+// (src and dst both complex64, or both half-precision pairs when h16 is set)
+static llr_status model_device(llr_ctx* ctx, const void* src, void* dst, const size_t* dims, size_t nv,
+                               const RampTables& t, int dir, bool h16 = false) {
+  const void* in = src;
   for (int a = 0; a < 3; ++a) {
     float s = (float)(1.0 / std::sqrt((double)dims[a]));
     llr_status st;
     if (dir == LLR_FORWARD)
-      st = fft_axis_pass(ctx, in, dst, dims, nv, a, false, t.pk[a].as<float2>(), false, t.pi[a].as<float2>(), true, s);
+      st = fft_axis_pass(ctx, in, dst, dims, nv, a, false, t.pk[a].as<float2>(), false, t.pi[a].as<float2>(), true, s,
+                         h16);
     else
-      st = fft_axis_pass(ctx, in, dst, dims, nv, a, true, t.pi[a].as<float2>(), false, t.pk[a].as<float2>(), true, s);
+      st = fft_axis_pass(ctx, in, dst, dims, nv, a, true, t.pi[a].as<float2>(), false, t.pk[a].as<float2>(), true, s,
+                         h16);
     if (st) return st;
     in = dst;
   }
@@ -146,7 +149,7 @@ using namespace llr;
 extern "C" {
 
 const char* llr_last_error(void) { return g_err.c_str(); }
-int llr_abi_version(void) { return 102; }
+int llr_abi_version(void) { return 103; }
 
 llr_status llr_ctx_create(int device, llr_ctx** out) {
   if (!out) return fail(LLR_ERR_ARG, "null output pointer");
@@ -578,6 +581,20 @@ struct llr_recon {
   int nc = 0;       // columns of the LLR matrices (srank or nq)
   int wL = 0;       // wavelet levels
   DevBuf X, Y, Cb, S, coef, t1, t2, partials, stats, hist, smax, best, counters, werr;
+  // half storage (LLR_STORE_FP16): X, coef hold half-precision pairs; Yh is the
+  // copy of the measurements made by the set-up; Xin holds an initial image
+  // (complex64) until the set-up converts it. Stored data are the caller's
+  // times hs = sqrt(N / energy): no value can exceed sqrt(N) <= 2^14 (one value
+  // is at most the norm of its volume, and every transform here preserves the
+  // norm), and a typical image value is O(1).
+  bool s16 = false, xin_pending = false;
+  double hs = 1.0;
+  // the block grid covers every voxel and the wavelet is on: level 0 of the
+  // wavelet reads the LLR step's coefficient planes (coefK: their xy-bands)
+  // and the back-projection is folded into it
+  bool fuse_wavelet = false;
+  DevBuf coefK;
+  DevBuf Yh, Xin;
   RampTables tabs;
   unsigned long long llr_blocks = 0;  // blocks processed by the LLR step (mixed-form statistics)
   size_t nparts = 0;
@@ -610,7 +627,10 @@ struct StageTimer {
   }
 };
 
-llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, unsigned* smax) {
+// backproject = false leaves the new coefficients in Cb (with a subspace) for
+// the fused wavelet to expand.
+llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, unsigned* smax,
+                     bool backproject = true) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
   const bool partial = p.partial_blocks != 0;
@@ -624,7 +644,8 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
     float2* F = p.srank ? r->Cb.as<float2>() : r->X.as<float2>();
     if (p.srank) {
       StageTimer t(r, 1);
-      st = project_bf16_device(ctx, r->X.as<float2>(), F, r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank);
+      st = r->s16 ? project_s16_device(ctx, r->X.p, F, r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank, 1.f)
+                  : project_bf16_device(ctx, r->X.as<float2>(), F, r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank);
       if (st) return st;
     }
     {
@@ -632,8 +653,11 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
       st = llr_block_run_mixed(ctx, F, p.vol, p.blk, shift, r->nc, 0, 0, nb, mode, (int)p.rank, lambda, smax, nullptr,
                                mode != 2 ? fp64n : nullptr, partial);
     }
-    if (st || mode == 2 || !p.srank) return st;
+    if (st || mode == 2 || !p.srank || !backproject) return st;
     StageTimer t(r, 1);
+    if (r->s16)
+      return backproject_device(ctx, F, r->X.p, r->S.as<float2>(), p.vol, p.blk, shift, (int)r->nq, (int)p.srank,
+                                partial, true, 1.f);
     return backproject_bf16_device(ctx, F, r->X.as<float2>(), r->S.as<float2>(), p.vol, p.blk, shift, (int)r->nq,
                                    (int)p.srank, partial);
   }
@@ -650,7 +674,7 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
       st = llr_block_run(ctx, r->Cb.as<float2>(), r->Cb.as<float2>(), nullptr, 0, p.vol, p.blk, shift, r->nc, 0, 0, nb,
                          mode, (int)p.rank, lambda, smax, nullptr, partial);
     }
-    if (st || mode == 2) return st;
+    if (st || mode == 2 || !backproject) return st;
     StageTimer t(r, 1);
     return backproject_device(ctx, r->Cb.as<float2>(), r->X.as<float2>(), r->S.as<float2>(), p.vol, p.blk, shift,
                               (int)r->nq, (int)p.srank, partial);
@@ -666,34 +690,40 @@ llr_status data_consistency(llr_recon* r, double* err) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
   llr_status st;
-  float2* X = r->X.as<float2>();
+  void* X = r->X.p;
+  const bool h = r->s16;
+  const void* Ym = h ? r->Yh.p : r->Y.p;
   const RampTables& T = r->tabs;
   float sx = (float)(1.0 / std::sqrt((double)p.vol[0])), sy = (float)(1.0 / std::sqrt((double)p.vol[1]));
   {
     StageTimer t(r, 3);
-    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, false, T.pk[0].as<float2>(), false, T.pi[0].as<float2>(), true, sx);
+    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, false, T.pk[0].as<float2>(), false, T.pi[0].as<float2>(), true, sx,
+                       h);
   }
   if (st) return st;
   {
     StageTimer t(r, 4);
-    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, false, T.pk[1].as<float2>(), false, T.pi[1].as<float2>(), true, sy);
+    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, false, T.pk[1].as<float2>(), false, T.pi[1].as<float2>(), true, sy,
+                       h);
   }
   if (st) return st;
   {
     StageTimer t(r, 5);
     size_t np = 0;
-    st = fft_zproj(ctx, X, r->Y.as<float2>(), p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(),
-                   r->partials.as<double>(), &np);
+    st = fft_zproj(ctx, X, Ym, p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(), r->partials.as<double>(), &np,
+                   h);
   }
   if (st) return st;
   {
     StageTimer t(r, 4);
-    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, true, T.pi[1].as<float2>(), false, T.pk[1].as<float2>(), true, sy);
+    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, true, T.pi[1].as<float2>(), false, T.pk[1].as<float2>(), true, sy,
+                       h);
   }
   if (st) return st;
   {
     StageTimer t(r, 3);
-    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false, T.pk[0].as<float2>(), true, sx);
+    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false, T.pk[0].as<float2>(), true, sx,
+                       h);
   }
   if (st) return st;
   StageTimer t(r, 6);
@@ -704,12 +734,17 @@ llr_status data_consistency(llr_recon* r, double* err) {
 llr_status recon_iteration(llr_recon* r, size_t it) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
-  llr_status st = recon_llr(r, &r->shifts[3 * it], p.llr_mode, (float)r->lambda, nullptr);
+  llr_status st = recon_llr(r, &r->shifts[3 * it], p.llr_mode, (float)r->lambda, nullptr, !r->fuse_wavelet);
   if (st) return st;
   if (p.wavelet_alpha > 0) {
     StageTimer t(r, 2);
-    st = wavelet_denoise_device(ctx, r->X.as<float2>(), r->nq, p.vol, (float)p.wavelet_alpha, r->wL, r->coef.as<float2>(),
-                                r->t1.as<float2>(), r->t2.as<float2>());
+    const float alpha = (float)(p.wavelet_alpha * r->hs);
+    if (r->fuse_wavelet) {
+      const WaveletFromSubspace sub{r->Cb.as<float2>(), r->S.as<float2>(), (int)p.srank, r->coefK.as<float2>()};
+      st = wavelet_denoise_subspace(ctx, r->X.p, r->nq, p.vol, alpha, r->wL, r->coef.p, sub, r->s16);
+    } else {
+      st = wavelet_denoise_device(ctx, r->X.p, r->nq, p.vol, alpha, r->wL, r->coef.p, r->t1.p, r->t2.p, r->s16);
+    }
     if (st) return st;
   }
   return data_consistency(r, r->hist.as<double>() + it);
@@ -742,6 +777,9 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
       return fail(LLR_ERR_UNSUPPORTED, "block too large for the mixed-precision LLR kernel");
     if (p.srank && p.nq > 128) return fail(LLR_ERR_UNSUPPORTED, "mixed form handles at most 128 volumes");
   }
+  if (p.storage != LLR_STORE_FP32 && p.storage != LLR_STORE_FP16) return fail(LLR_ERR_ARG, "unknown storage");
+  if (p.storage == LLR_STORE_FP16 && (p.precision != LLR_PREC_MIXED || !p.srank))
+    return fail(LLR_ERR_UNSUPPORTED, "half storage needs the mixed-precision form and a subspace");
   LLR_HIP(hipSetDevice(ctx->device));
   llr_recon* r = new llr_recon();
   r->ctx = ctx;
@@ -750,6 +788,8 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
   r->N = p.vol[0] * p.vol[1] * p.vol[2];
   r->D = r->N * p.nq;
   r->nc = nc;
+  r->s16 = p.storage == LLR_STORE_FP16;
+  const size_t eb = r->s16 ? 4 : 8;  // bytes per stored complex value
   r->shifts.assign(p.shifts, p.shifts + 3 * p.n_iter);
   r->p.shifts = nullptr;
   r->p.subspace = nullptr;
@@ -757,8 +797,9 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
     delete r;
     return s;
   };
-  if ((st = upload(r->X, nullptr, r->D * 8))) return bail(st);
+  if ((st = upload(r->X, nullptr, r->D * eb))) return bail(st);
   if ((st = upload(r->Y, nullptr, r->D * 8))) return bail(st);
+  if (r->s16 && (st = upload(r->Yh, nullptr, r->D * 4))) return bail(st);
   if (p.srank) {
     if ((st = upload(r->Cb, nullptr, r->N * p.srank * 8))) return bail(st);
     if ((st = upload(r->S, p.subspace, p.nq * p.srank * 8))) return bail(st);
@@ -767,12 +808,20 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
     r->wL = p.wavelet_levels < 0 ? wavelet_max_level(p.vol) : p.wavelet_levels;
     size_t ce = 0, te = 0;
     wavelet_workspace_elems(p.vol, r->wL, &ce, &te);
-    if ((st = upload(r->coef, nullptr, ce * p.nq * 8))) return bail(st);
+    if ((st = upload(r->coef, nullptr, ce * p.nq * eb))) return bail(st);
+    bool cover = p.partial_blocks != 0;
+    if (!cover) {
+      cover = true;
+      for (int a = 0; a < 3; ++a) cover = cover && p.vol[a] % p.blk[a] == 0;
+    }
+    r->fuse_wavelet = p.srank && cover && wavelet_subspace_supported(p.vol, r->wL, (int)p.srank);
+    if (r->fuse_wavelet && (st = upload(r->coefK, nullptr, wavelet_subspace_elems(p.vol, r->wL, (int)p.srank) * 8)))
+      return bail(st);
     if ((st = upload(r->t1, nullptr, te * p.nq * 8))) return bail(st);
     if ((st = upload(r->t2, nullptr, te * p.nq * 8))) return bail(st);
   }
   size_t np = 0;
-  fft_zproj(ctx, nullptr, nullptr, p.vol, p.nq, nullptr, nullptr, nullptr, &np);
+  fft_zproj(ctx, nullptr, nullptr, p.vol, p.nq, nullptr, nullptr, nullptr, &np, r->s16);
   r->nparts = np;
   if ((st = upload(r->partials, nullptr, std::max<size_t>(np, 2048) * 8))) return bail(st);
   if ((st = upload(r->stats, nullptr, 16))) return bail(st);
@@ -812,8 +861,13 @@ llr_status llr_recon_set_initial(llr_recon* r, const llr_cf32* x, size_t len, in
   if (!r || !x) return fail(LLR_ERR_ARG, "null argument");
   if (len != r->D) return fail(LLR_ERR_ARG, "initial image must hold nx*ny*nz*nq samples");
   LLR_HIP(hipSetDevice(r->ctx->device));
-  LLR_HIP(hipMemcpyAsync(r->X.p, x, len * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                         r->ctx->stream));
+  if (r->s16 && !r->Xin.p) {
+    llr_status st = upload(r->Xin, nullptr, r->D * 8);
+    if (st) return st;
+  }
+  LLR_HIP(hipMemcpyAsync(r->s16 ? r->Xin.p : r->X.p, x, len * 8,
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, r->ctx->stream));
+  r->xin_pending = r->s16;
   LLR_HIP(hipStreamSynchronize(r->ctx->stream));
   r->have_init = true;
   return LLR_OK;
@@ -844,6 +898,7 @@ static llr_status recon_setup(llr_recon* r) {
   LLR_HIP(hipMemcpy(best.data(), r->best.p, r->nq * 8, hipMemcpyDeviceToHost));
   LLR_HIP(hipMemcpy(stats, r->stats.p, 16, hipMemcpyDeviceToHost));
   r->energy = stats[0];
+  r->hs = (r->s16 && r->energy > 0) ? std::sqrt((double)r->N / r->energy) : 1.0;
   r->ksp_corr.assign(3 * r->nq, 0.f);
   for (size_t q = 0; q < r->nq; ++q) {
     size_t idx = (size_t)(0xFFFFFFFFu - (unsigned)(best[q] & 0xFFFFFFFFull));
@@ -851,15 +906,24 @@ static llr_status recon_setup(llr_recon* r) {
     for (int a = 0; a < 3; ++a) r->ksp_corr[3 * q + a] = (float)((long long)s[a] - ((long long)p.vol[a] + 1) / 2);
   }
   if ((st = make_tables(r->tabs, p.vol, r->nq, r->ksp_corr.data(), nullptr))) return st;
+  if (r->s16) {
+    // the working copy of the measurements, and the initial image if one was set
+    StageTimer t(r, 7);
+    if ((st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs))) return st;
+    if (r->have_init && r->xin_pending && (st = to_h16_device(ctx, r->Xin.as<float2>(), r->X.p, r->D, (float)r->hs)))
+      return st;
+    r->xin_pending = false;
+  }
   // zero-filled adjoint (lib.rs:233-240) unless warm-started
   if (!r->have_init) {
     StageTimer t(r, 7);
-    if ((st = model_device(ctx, r->Y.as<float2>(), r->X.as<float2>(), p.vol, r->nq, r->tabs, LLR_INVERSE))) return st;
+    if ((st = model_device(ctx, r->s16 ? r->Yh.p : r->Y.p, r->X.p, p.vol, r->nq, r->tabs, LLR_INVERSE, r->s16)))
+      return st;
   }
   // threshold: lambda_factor * sigma_max of the starting blocks at shift 0
   if (p.llr_mode == LLR_MODE_SVT) {
     if (p.svt_lambda > 0) {
-      r->lambda = p.svt_lambda;
+      r->lambda = p.svt_lambda * r->hs;  // thresholds act on stored (scaled) values
     } else {
       LLR_HIP(hipMemsetAsync(r->smax.p, 0, 8, ctx->stream));
       size_t zero[3] = {0, 0, 0};
@@ -875,11 +939,19 @@ static llr_status recon_setup(llr_recon* r) {
   return LLR_OK;
 }
 
-// X = C S^H for every voxel (one block covering the whole volume)
-static llr_status backproject_all(llr_recon* r, const float2* C) {
+// C = X S of the current iterate (FP32 VALU from complex64, BF16 MFMA from half storage)
+// (out_scale multiplies C: 1 / hs gives the caller's units from half storage)
+static llr_status project_current(llr_recon* r, float2* C, float out_scale = 1.f) {
+  if (r->s16)
+    return project_s16_device(r->ctx, r->X.p, C, r->S.as<float2>(), r->N, (int)r->nq, (int)r->p.srank, out_scale);
+  return project_device(r->ctx, r->X.as<float2>(), C, r->S.as<float2>(), r->N, (int)r->nq, (int)r->p.srank);
+}
+
+// X = scale * C S^H for every voxel (one block covering the whole volume)
+static llr_status backproject_all(llr_recon* r, const float2* C, float scale = 1.f) {
   const size_t zero[3] = {0, 0, 0};
-  return backproject_device(r->ctx, C, r->X.as<float2>(), r->S.as<float2>(), r->p.vol, r->p.vol, zero, (int)r->nq,
-                            (int)r->p.srank, false);
+  return backproject_device(r->ctx, C, r->X.p, r->S.as<float2>(), r->p.vol, r->p.vol, zero, (int)r->nq,
+                            (int)r->p.srank, false, r->s16, scale);
 }
 
 // Iterations it = 0 .. n_max-1 of the prepared run; stop (if > 0) ends the run
@@ -894,7 +966,7 @@ static llr_status run_iterations(llr_recon* r, size_t n_max, double stop, double
       double e = 0;
       LLR_HIP(hipStreamSynchronize(ctx->stream));
       LLR_HIP(hipMemcpy(&e, r->hist.as<double>() + it, 8, hipMemcpyDeviceToHost));
-      if (e / r->energy <= stop) {
+      if (e / (r->energy * r->hs * r->hs) <= stop) {
         ++it;
         break;
       }
@@ -904,7 +976,7 @@ static llr_status run_iterations(llr_recon* r, size_t n_max, double stop, double
   if (err_hist && it) {
     std::vector<double> h(it);
     LLR_HIP(hipMemcpy(h.data(), r->hist.p, it * 8, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < it; ++i) err_hist[i] = h[i] / r->energy;
+    for (size_t i = 0; i < it; ++i) err_hist[i] = h[i] / (r->energy * r->hs * r->hs);
   }
   if (iters_done) *iters_done = it;
   return LLR_OK;
@@ -922,15 +994,24 @@ static llr_status warm_start(llr_recon* r, llr_unet* net, const float* to_net, c
   if ((st = recon_setup(r))) return st;
   const int K = (int)r->p.srank;
   float2* C = r->Cb.as<float2>();
-  if ((st = project_device(r->ctx, r->X.as<float2>(), C, r->S.as<float2>(), r->N, (int)r->nq, K))) return st;
+  {
+    StageTimer t(r, 1);
+    if ((st = project_current(r, C))) return st;
+  }
   double scale = 0;
-  if ((st = unet_coef_scale(r->ctx, C, K, r->N, to_net, &scale))) return st;
-  if (!(scale > 0) || !std::isfinite(scale)) return fail(LLR_ERR_ARG, "zero-filled adjoint has no signal");
-  llr_ctx* nctx = unet_context(net);
-  if (nctx != r->ctx) LLR_HIP(hipStreamSynchronize(r->ctx->stream));
-  if ((st = unet_predict_device(net, C, K, r->p.vol, (float)scale, to_net, from_net, C))) return st;
-  if (nctx != r->ctx) LLR_HIP(hipStreamSynchronize(nctx->stream));
-  if ((st = backproject_all(r, C))) return st;
+  {
+    StageTimer t(r, 8);
+    if ((st = unet_coef_scale(r->ctx, C, K, r->N, to_net, &scale))) return st;
+    if (!(scale > 0) || !std::isfinite(scale)) return fail(LLR_ERR_ARG, "zero-filled adjoint has no signal");
+    llr_ctx* nctx = unet_context(net);
+    if (nctx != r->ctx) LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+    if ((st = unet_predict_device(net, C, K, r->p.vol, (float)scale, to_net, from_net, C))) return st;
+    if (nctx != r->ctx) LLR_HIP(hipStreamSynchronize(nctx->stream));
+  }
+  {
+    StageTimer t(r, 1);
+    if ((st = backproject_all(r, C))) return st;
+  }
   if ((st = data_consistency(r, r->werr.as<double>()))) return st;
   r->have_init = true;
   r->prepared = true;
@@ -939,7 +1020,7 @@ static llr_status warm_start(llr_recon* r, llr_unet* net, const float* to_net, c
     double e = 0;
     LLR_HIP(hipStreamSynchronize(r->ctx->stream));
     LLR_HIP(hipMemcpy(&e, r->werr.p, 8, hipMemcpyDeviceToHost));
-    *residual_out = e / r->energy;
+    *residual_out = e / (r->energy * r->hs * r->hs);
   }
   return LLR_OK;
 }
@@ -976,7 +1057,7 @@ llr_status llr_recon_get_coefficients(llr_recon* r, llr_cf32* out, size_t len, i
   if (len != r->N * r->p.srank) return fail(LLR_ERR_ARG, "output must hold nx*ny*nz*srank samples");
   LLR_HIP(hipSetDevice(r->ctx->device));
   float2* dst = on_device ? (float2*)out : r->Cb.as<float2>();
-  llr_status st = project_device(r->ctx, r->X.as<float2>(), dst, r->S.as<float2>(), r->N, (int)r->nq, (int)r->p.srank);
+  llr_status st = project_current(r, dst, (float)(1.0 / r->hs));
   if (st) return st;
   if (!on_device) return download(out, r->Cb, len * 8, r->ctx);
   LLR_HIP(hipStreamSynchronize(r->ctx->stream));
@@ -993,10 +1074,11 @@ llr_status llr_recon_set_coefficients(llr_recon* r, const llr_cf32* c, size_t le
     LLR_HIP(hipMemcpyAsync(r->Cb.p, c, len * 8, hipMemcpyHostToDevice, r->ctx->stream));
     src = r->Cb.as<float2>();
   }
-  llr_status st = backproject_all(r, src);
+  llr_status st = backproject_all(r, src, (float)r->hs);
   if (st) return st;
   LLR_HIP(hipStreamSynchronize(r->ctx->stream));
   r->have_init = true;
+  r->xin_pending = false;
   return LLR_OK;
 }
 
@@ -1045,6 +1127,23 @@ llr_status llr_recon_get_image(llr_recon* r, llr_cf32* out, size_t len, int on_d
   if (!r || !out) return fail(LLR_ERR_ARG, "null argument");
   if (len != r->D) return fail(LLR_ERR_ARG, "output must hold nx*ny*nz*nq samples");
   LLR_HIP(hipSetDevice(r->ctx->device));
+  if (r->s16) {
+    // widen to complex64: straight into a device output, else through a
+    // temporary device buffer
+    if (on_device) {
+      llr_status st = from_h16_device(r->ctx, r->X.p, (float2*)out, len, (float)(1.0 / r->hs));
+      if (st) return st;
+    } else {
+      DevBuf tmp;
+      LLR_HIP(tmp.alloc(len * 8));
+      llr_status st = from_h16_device(r->ctx, r->X.p, tmp.as<float2>(), len, (float)(1.0 / r->hs));
+      if (st) return st;
+      LLR_HIP(hipMemcpyAsync(out, tmp.p, len * 8, hipMemcpyDeviceToHost, r->ctx->stream));
+      LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+    }
+    LLR_HIP(hipStreamSynchronize(r->ctx->stream));
+    return LLR_OK;
+  }
   LLR_HIP(hipMemcpyAsync(out, r->X.p, len * 8, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                          r->ctx->stream));
   LLR_HIP(hipStreamSynchronize(r->ctx->stream));
@@ -1053,7 +1152,7 @@ llr_status llr_recon_get_image(llr_recon* r, llr_cf32* out, size_t len, int on_d
 
 llr_status llr_recon_info(llr_recon* r, double* lambda, float* ksp_corr) {
   if (!r) return fail(LLR_ERR_ARG, "null recon");
-  if (lambda) *lambda = r->lambda;
+  if (lambda) *lambda = r->lambda / r->hs;
   if (ksp_corr && !r->ksp_corr.empty()) std::memcpy(ksp_corr, r->ksp_corr.data(), r->ksp_corr.size() * sizeof(float));
   return LLR_OK;
 }
@@ -1256,14 +1355,21 @@ static llr_status batch_post(llr_recon* r, const unsigned* iter, double* tmp) {
   const llr_recon_params& p = r->p;
   const size_t zero[3] = {0, 0, 0};
   llr_status st;
-  if (p.srank &&
+  if (p.srank && !r->fuse_wavelet &&
       (st = backproject_device(ctx, r->Cb.as<float2>(), r->X.as<float2>(), r->S.as<float2>(), p.vol, p.blk, zero,
                                (int)r->nq, (int)p.srank, true)))
     return st;
-  if (p.wavelet_alpha > 0 &&
-      (st = wavelet_denoise_device(ctx, r->X.as<float2>(), r->nq, p.vol, (float)p.wavelet_alpha, r->wL,
-                                   r->coef.as<float2>(), r->t1.as<float2>(), r->t2.as<float2>())))
-    return st;
+  if (p.wavelet_alpha > 0) {
+    if (r->fuse_wavelet) {
+      // the members' grids have partial edge blocks: every voxel is covered
+      const WaveletFromSubspace sub{r->Cb.as<float2>(), r->S.as<float2>(), (int)p.srank, r->coefK.as<float2>()};
+      st = wavelet_denoise_subspace(ctx, r->X.p, r->nq, p.vol, (float)p.wavelet_alpha, r->wL, r->coef.p, sub, false);
+    } else {
+      st = wavelet_denoise_device(ctx, r->X.p, r->nq, p.vol, (float)p.wavelet_alpha, r->wL, r->coef.p, r->t1.p,
+                                  r->t2.p);
+    }
+    if (st) return st;
+  }
   float2* X = r->X.as<float2>();
   const RampTables& T = r->tabs;
   float sx = (float)(1.0 / std::sqrt((double)p.vol[0])), sy = (float)(1.0 / std::sqrt((double)p.vol[1]));

@@ -29,6 +29,38 @@ __host__ __device__ inline float2 cscale(float2 a, float s) { return make_float2
 __host__ __device__ inline float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 __host__ __device__ inline float cnorm(float2 a) { return a.x * a.x + a.y * a.y; }
 
+// ------------------------------------------------------- complex storage
+// Large buffers hold complex values either as complex64 (float2) or as two
+// IEEE half-precision numbers in one 32-bit word (low half: real part).
+// Kernels templated on the storage load into FP32 registers (exactly) and
+// round once, to nearest even, when storing; all arithmetic is FP32. Data in
+// half storage is kept at a scale where every value is below 2^14 in
+// magnitude (the caller's job), far inside the format's range.
+typedef unsigned int ch16;
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+struct StoreF32 {
+  using T = float2;
+  static constexpr int bytes = 8;
+  __device__ static inline float2 ld(const float2* p) { return *p; }
+  __device__ static inline void st(float2* p, float2 v) { *p = v; }
+};
+
+struct StoreF16 {
+  using T = ch16;
+  static constexpr int bytes = 4;
+  __device__ static inline float2 unpack(ch16 u) {
+    const half2v h = __builtin_bit_cast(half2v, u);
+    return make_float2((float)h.x, (float)h.y);
+  }
+  __device__ static inline ch16 pack(float2 v) {
+    const half2v h = {(_Float16)v.x, (_Float16)v.y};
+    return __builtin_bit_cast(ch16, h);
+  }
+  __device__ static inline float2 ld(const ch16* p) { return unpack(*p); }
+  __device__ static inline void st(ch16* p, float2 v) { *p = pack(v); }
+};
+
 // ------------------------------------------------------------- host context
 struct llr_ctx {
   int device = 0;

@@ -166,8 +166,8 @@ __device__ __forceinline__ void dft(float2* v, bool inv) {
 }
 
 struct PassArgs {
-  const float2* src;
-  float2* dst;
+  const void* src;    // complex values in the pass's storage (ST::T)
+  void* dst;
   int stride;         // element stride along the transformed axis
   int U;              // number of lines along the tiled (neighbour) dimension
   int ustride;        // element stride between neighbouring lines
@@ -184,11 +184,15 @@ struct PassArgs {
 
 constexpr int pow2_floor(int x) { return x >= 256 ? 256 : x >= 128 ? 128 : x >= 64 ? 64 : x >= 32 ? 32 : x >= 16 ? 16 : x >= 8 ? 8 : x >= 4 ? 4 : x >= 2 ? 2 : 1; }
 
-template <int R1, int R2>
+// MINT: at least this many lines per workgroup while it keeps 512 threads or
+// fewer (half storage takes 32 so a strided pass moves 128-byte runs).
+template <int R1, int R2, int MINT = 1>
 struct Layout {
   static constexpr int N = R1 * R2;
   static constexpr int P = R1 > R2 ? R1 : R2;
-  static constexpr int T = pow2_floor(256 / P);  // lines per workgroup (power of two: aligned x runs)
+  static constexpr int T0 = pow2_floor(256 / P);
+  static constexpr int TM = MINT < 512 / P ? MINT : 512 / P;
+  static constexpr int T = T0 > TM ? T0 : TM;  // lines per workgroup (power of two: aligned x runs)
   static constexpr int R2P = R2 + 1;
   static constexpr int LS = R1 * R2P + 1;
   static constexpr int THREADS = T * P;
@@ -199,9 +203,10 @@ struct Layout {
 // direction, conjugation and 1/sqrt(n) folded in) are staged in LDS once, and
 // the next tile's loads are issued before the current tile's second stage so
 // they overlap its compute and stores. Offsets inside a volume are 32-bit.
-template <int R1, int R2, bool TFAST, bool INV, bool RAMPS>
-__global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_pass(PassArgs a) {
-  using L = Layout<R1, R2>;
+template <int R1, int R2, bool TFAST, bool INV, bool RAMPS, class ST, int MINT>
+__global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_pass(PassArgs a) {
+  using L = Layout<R1, R2, MINT>;
+  using E = typename ST::T;
   constexpr int N = L::N, P = L::P, T = L::T, TH = L::THREADS;
   __shared__ float2 lds[T * L::LS];
   __shared__ float2 s_tw[N];
@@ -222,18 +227,18 @@ __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_pass(PassArgs
     }
   }
   const long long row = v * a.vstride + (long long)blockIdx.y * a.wstride;
-  const float2* __restrict__ src = a.src + row;
-  float2* __restrict__ dst = a.dst + row;
+  const E* __restrict__ src = (const E*)a.src + row;
+  E* __restrict__ dst = (E*)a.dst + row;
   const int tile0 = blockIdx.x * a.tiles_per_wg;
   const int ntiles = min(a.tiles_per_wg, a.tiles - tile0);
   float2 xn[R1];
   auto load = [&](int tile, float2* x) {
     int u = tile * T + t;
     if (u < a.U && j < R2) {
-      const float2* p = src + (unsigned)(u * a.ustride + j * a.stride);
+      const E* p = src + (unsigned)(u * a.ustride + j * a.stride);
       unsigned step = (unsigned)(R2 * a.stride);
 #pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1) x[j1] = p[j1 * step];
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = ST::ld(p + j1 * step);
     }
   };
   load(tile0, xn);
@@ -267,14 +272,14 @@ __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_pass(PassArgs
         if (INV) y[j2].y = -y[j2].y;
       }
       SmallDFT<R2>::run(y);
-      float2* p = dst + (unsigned)(u * a.ustride + j * a.stride);
+      E* p = dst + (unsigned)(u * a.ustride + j * a.stride);
       unsigned step = (unsigned)(R1 * a.stride);
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) {
         float2 val = y[k2];
         if (INV) val.y = -val.y;
         val = RAMPS ? cmul(val, s_post[j + R1 * k2]) : cscale(val, a.scale);
-        p[k2 * step] = val;
+        ST::st(p + k2 * step, val);
       }
     }
     __syncthreads();
@@ -282,8 +287,8 @@ __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_pass(PassArgs
 }
 
 struct ZProjArgs {
-  float2* data;        // image-space iterate after the forward x and y passes
-  const float2* meas;  // measurements (k-space), same layout
+  void* data;          // image-space iterate after the forward x and y passes (ST::T)
+  const void* meas;    // measurements (k-space), same layout and storage
   int stride, ustride;
   long long wstride, vstride;
   int U, tiles, tiles_per_wg;
@@ -298,9 +303,10 @@ struct ZProjArgs {
 // The forward output ramp conj(pi) and the adjoint input ramp pi cancel on
 // unsampled entries, so the projection compares x*scale with y*pi and, where
 // sampled, substitutes y*pi (|pi| = 1, so |x*conj(pi) - y| = |x - y*pi|).
-template <int R1, int R2>
-__global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_z_project(ZProjArgs a) {
-  using L = Layout<R1, R2>;
+template <int R1, int R2, class ST, int MINT>
+__global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_z_project(ZProjArgs a) {
+  using L = Layout<R1, R2, MINT>;
+  using E = typename ST::T;
   constexpr int N = L::N, T = L::T, TH = L::THREADS;
   __shared__ float2 lds[T * L::LS];
   __shared__ float2 s_tw[N], s_pk[N], s_pi[N];
@@ -315,8 +321,8 @@ __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_z_project(ZPr
     s_pi[i] = a.pi[v * N + i];
   }
   const long long row = v * a.vstride + (long long)blockIdx.y * a.wstride;
-  float2* __restrict__ data = a.data + row;
-  const float2* __restrict__ meas = a.meas + row;
+  E* __restrict__ data = (E*)a.data + row;
+  const E* __restrict__ meas = (const E*)a.meas + row;
   const int tile0 = blockIdx.x * a.tiles_per_wg;
   const int ntiles = min(a.tiles_per_wg, a.tiles - tile0);
   double err = 0.0;
@@ -324,10 +330,10 @@ __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_z_project(ZPr
   auto load = [&](int tile, float2* x) {
     int u = tile * T + t;
     if (u < a.U && j < R2) {
-      const float2* p = data + (unsigned)(u * a.ustride + j * a.stride);
+      const E* p = data + (unsigned)(u * a.ustride + j * a.stride);
       unsigned step = (unsigned)(R2 * a.stride);
 #pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1) x[j1] = p[j1 * step];
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = ST::ld(p + j1 * step);
     }
   };
   load(tile0, xn);
@@ -350,14 +356,14 @@ __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_z_project(ZPr
 #pragma unroll
       for (int j2 = 0; j2 < R2; ++j2) y[j2] = lds[t * L::LS + j * L::R2P + j2];
       SmallDFT<R2>::run(y);
-      const float2* mp = meas + (unsigned)(u * a.ustride + j * a.stride);
+      const E* mp = meas + (unsigned)(u * a.ustride + j * a.stride);
       unsigned step = (unsigned)(R1 * a.stride);
       // k-space sample k = j + R1*k2: project, then start the adjoint
       // (R2-point inverse DFT over k2 for the fixed k1 = j)
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) {
         float2 xv = cscale(y[k2], a.scale);
-        float2 m = mp[k2 * step];
+        float2 m = ST::ld(mp + k2 * step);
         if (m.x != 0.0f || m.y != 0.0f) {
           float2 mpv = cmul(m, s_pi[j + R1 * k2]);
           float2 d = csub(xv, mpv);
@@ -383,13 +389,13 @@ __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_z_project(ZPr
         x[k1] = make_float2(q.x, -q.y);
       }
       SmallDFT<R1>::run(x);
-      float2* p = data + (unsigned)(u * a.ustride + j * a.stride);
+      E* p = data + (unsigned)(u * a.ustride + j * a.stride);
       unsigned step = (unsigned)(R2 * a.stride);
 #pragma unroll
       for (int j1 = 0; j1 < R1; ++j1) {
         int idx = j + R2 * j1;
         float2 r = make_float2(x[j1].x, -x[j1].y);
-        p[j1 * step] = cscale(cmulc(r, s_pk[idx]), a.scale);
+        ST::st(p + j1 * step, cscale(cmulc(r, s_pk[idx]), a.scale));
       }
     }
     __syncthreads();
@@ -407,16 +413,30 @@ __global__ void __launch_bounds__((Layout<R1, R2>::THREADS)) k_fft_z_project(ZPr
   }
 }
 
-template <int R1, int R2>
-hipError_t launch_pass(PassArgs a, long long W, long long nvols, bool tfast, bool inv, hipStream_t s) {
-  using L = Layout<R1, R2>;
+template <int R1, int R2, bool TF, bool IV, bool RP, class ST, int MINT>
+hipError_t launch_one(PassArgs a, long long W, long long nvols, hipStream_t s) {
+  using L = Layout<R1, R2, MINT>;
   a.tiles = (a.U + L::T - 1) / L::T;
   const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;  // tiles per workgroup (tunable)
   a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
   dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
+  hipLaunchKernelGGL((k_fft_pass<R1, R2, TF, IV, RP, ST, MINT>), grid, dim3(L::THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int R1, int R2>
+hipError_t launch_pass(PassArgs a, long long W, long long nvols, bool tfast, bool inv, bool h16, hipStream_t s) {
   const bool ramps = a.pre != nullptr;
-#define FFT_LAUNCH(TF, IV, RP) \
-  hipLaunchKernelGGL((k_fft_pass<R1, R2, TF, IV, RP>), grid, dim3(L::THREADS), 0, s, a)
+  if (h16) {
+    // half storage exists for the signal model's ramped passes only; the
+    // strided (tfast) passes take 32 x-neighbour lines per workgroup
+    if (!ramps) return hipErrorInvalidValue;
+    if (tfast) return inv ? launch_one<R1, R2, true, true, true, StoreF16, 32>(a, W, nvols, s)
+                          : launch_one<R1, R2, true, false, true, StoreF16, 32>(a, W, nvols, s);
+    return inv ? launch_one<R1, R2, false, true, true, StoreF16, 1>(a, W, nvols, s)
+               : launch_one<R1, R2, false, false, true, StoreF16, 1>(a, W, nvols, s);
+  }
+#define FFT_LAUNCH(TF, IV, RP) return launch_one<R1, R2, TF, IV, RP, StoreF32, 1>(a, W, nvols, s)
   if (tfast) {
     if (inv) { if (ramps) FFT_LAUNCH(true, true, true); else FFT_LAUNCH(true, true, false); }
     else { if (ramps) FFT_LAUNCH(true, false, true); else FFT_LAUNCH(true, false, false); }
@@ -425,20 +445,25 @@ hipError_t launch_pass(PassArgs a, long long W, long long nvols, bool tfast, boo
     else { if (ramps) FFT_LAUNCH(false, false, true); else FFT_LAUNCH(false, false, false); }
   }
 #undef FFT_LAUNCH
-  return hipGetLastError();
 }
 
-template <int R1, int R2>
-hipError_t launch_zproj(ZProjArgs a, long long W, long long nvols, hipStream_t s, size_t* nparts) {
-  using L = Layout<R1, R2>;
+template <int R1, int R2, class ST, int MINT>
+hipError_t launch_zproj_as(ZProjArgs a, long long W, long long nvols, hipStream_t s, size_t* nparts) {
+  using L = Layout<R1, R2, MINT>;
   a.tiles = (a.U + L::T - 1) / L::T;
   const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;
   a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
   dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
   *nparts = (size_t)grid.x * grid.y * grid.z;
   if (a.partial == nullptr) return hipSuccess;  // size query
-  hipLaunchKernelGGL((k_fft_z_project<R1, R2>), grid, dim3(L::THREADS), 0, s, a);
+  hipLaunchKernelGGL((k_fft_z_project<R1, R2, ST, MINT>), grid, dim3(L::THREADS), 0, s, a);
   return hipGetLastError();
+}
+
+template <int R1, int R2>
+hipError_t launch_zproj(ZProjArgs a, long long W, long long nvols, bool h16, hipStream_t s, size_t* nparts) {
+  return h16 ? launch_zproj_as<R1, R2, StoreF16, 32>(a, W, nvols, s, nparts)
+             : launch_zproj_as<R1, R2, StoreF32, 1>(a, W, nvols, s, nparts);
 }
 
 }  // namespace
@@ -456,39 +481,40 @@ bool fft_size_supported(size_t n) {
 }
 
 // Dispatch an axis pass for line length n.
-static hipError_t dispatch_pass(int n, const PassArgs& a, long long W, long long nv, bool tfast, bool inv,
+static hipError_t dispatch_pass(int n, const PassArgs& a, long long W, long long nv, bool tfast, bool inv, bool h16,
                                 hipStream_t s) {
   switch (n) {
-    case 2: return launch_pass<2, 1>(a, W, nv, tfast, inv, s);
-    case 4: return launch_pass<2, 2>(a, W, nv, tfast, inv, s);
-    case 8: return launch_pass<4, 2>(a, W, nv, tfast, inv, s);
-    case 16: return launch_pass<4, 4>(a, W, nv, tfast, inv, s);
-    case 32: return launch_pass<8, 4>(a, W, nv, tfast, inv, s);
-    case 64: return launch_pass<8, 8>(a, W, nv, tfast, inv, s);
-    case 128: return launch_pass<16, 8>(a, W, nv, tfast, inv, s);
-    case 256: return launch_pass<16, 16>(a, W, nv, tfast, inv, s);
-    case 512: return launch_pass<32, 16>(a, W, nv, tfast, inv, s);
-    case 3: return launch_pass<3, 1>(a, W, nv, tfast, inv, s);
-    case 6: return launch_pass<3, 2>(a, W, nv, tfast, inv, s);
-    case 12: return launch_pass<6, 2>(a, W, nv, tfast, inv, s);
-    case 24: return launch_pass<6, 4>(a, W, nv, tfast, inv, s);
-    case 48: return launch_pass<12, 4>(a, W, nv, tfast, inv, s);
-    case 96: return launch_pass<12, 8>(a, W, nv, tfast, inv, s);
-    case 192: return launch_pass<12, 16>(a, W, nv, tfast, inv, s);
-    case 384: return launch_pass<12, 32>(a, W, nv, tfast, inv, s);
+    case 2: return launch_pass<2, 1>(a, W, nv, tfast, inv, h16, s);
+    case 4: return launch_pass<2, 2>(a, W, nv, tfast, inv, h16, s);
+    case 8: return launch_pass<4, 2>(a, W, nv, tfast, inv, h16, s);
+    case 16: return launch_pass<4, 4>(a, W, nv, tfast, inv, h16, s);
+    case 32: return launch_pass<8, 4>(a, W, nv, tfast, inv, h16, s);
+    case 64: return launch_pass<8, 8>(a, W, nv, tfast, inv, h16, s);
+    case 128: return launch_pass<16, 8>(a, W, nv, tfast, inv, h16, s);
+    case 256: return launch_pass<16, 16>(a, W, nv, tfast, inv, h16, s);
+    case 512: return launch_pass<32, 16>(a, W, nv, tfast, inv, h16, s);
+    case 3: return launch_pass<3, 1>(a, W, nv, tfast, inv, h16, s);
+    case 6: return launch_pass<3, 2>(a, W, nv, tfast, inv, h16, s);
+    case 12: return launch_pass<6, 2>(a, W, nv, tfast, inv, h16, s);
+    case 24: return launch_pass<6, 4>(a, W, nv, tfast, inv, h16, s);
+    case 48: return launch_pass<12, 4>(a, W, nv, tfast, inv, h16, s);
+    case 96: return launch_pass<12, 8>(a, W, nv, tfast, inv, h16, s);
+    case 192: return launch_pass<12, 16>(a, W, nv, tfast, inv, h16, s);
+    case 384: return launch_pass<12, 32>(a, W, nv, tfast, inv, h16, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 // One pass along `axis` of nvols volumes of size dims; src may equal dst.
 // pre and post must both be given (ramped signal-model pass) or both be null.
-llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const size_t* dims, size_t nvols, int axis,
+llr_status fft_axis_pass(llr_ctx* ctx, const void* src, void* dst, const size_t* dims, size_t nvols, int axis,
                          bool inverse, const float2* pre, bool pre_conj, const float2* post, bool post_conj,
-                         float scale) {
+                         float scale, bool h16) {
   int n = (int)dims[axis];
   if (!fft_size_supported((size_t)n))
     return fail(LLR_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " is not supported (2^a * 3^b with a radix of at most 32, up to 512)");
   if ((pre == nullptr) != (post == nullptr)) return fail(LLR_ERR_ARG, "ramp tables must be given together");
+  if (h16 && pre == nullptr) return fail(LLR_ERR_UNSUPPORTED, "half storage is for the signal model's passes only");
   if ((double)dims[0] * dims[1] * dims[2] >= 4294967296.0)
     return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
   const float2* tw = nullptr;
@@ -530,7 +556,7 @@ llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const siz
     tfast = true;
   }
   a.tiles_per_wg = ctx->fft_tpw[axis];
-  hipError_t e = dispatch_pass(n, a, W, (long long)nvols, tfast, inverse, ctx->stream);
+  hipError_t e = dispatch_pass(n, a, W, (long long)nvols, tfast, inverse, h16, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "fft axis pass launch");
   return LLR_OK;
 }
@@ -538,8 +564,8 @@ llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const siz
 // Fused forward-z / hard projection / adjoint-z. Writes one partial error sum
 // per workgroup into `partial` (size returned in nparts; partial == null only
 // queries the size).
-llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_t* dims, size_t nvols,
-                     const float2* pk, const float2* pi, double* partial, size_t* nparts) {
+llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* dims, size_t nvols,
+                     const float2* pk, const float2* pi, double* partial, size_t* nparts, bool h16) {
   int n = (int)dims[2];
   const float2* tw = nullptr;
   if (partial) {
@@ -563,23 +589,23 @@ llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_
   a.tiles_per_wg = ctx->fft_tpw[2];
   hipError_t e;
   switch (n) {
-    case 2: e = launch_zproj<2, 1>(a, ny, nvols, ctx->stream, nparts); break;
-    case 4: e = launch_zproj<2, 2>(a, ny, nvols, ctx->stream, nparts); break;
-    case 8: e = launch_zproj<4, 2>(a, ny, nvols, ctx->stream, nparts); break;
-    case 16: e = launch_zproj<4, 4>(a, ny, nvols, ctx->stream, nparts); break;
-    case 32: e = launch_zproj<8, 4>(a, ny, nvols, ctx->stream, nparts); break;
-    case 64: e = launch_zproj<8, 8>(a, ny, nvols, ctx->stream, nparts); break;
-    case 128: e = launch_zproj<16, 8>(a, ny, nvols, ctx->stream, nparts); break;
-    case 256: e = launch_zproj<16, 16>(a, ny, nvols, ctx->stream, nparts); break;
-    case 512: e = launch_zproj<32, 16>(a, ny, nvols, ctx->stream, nparts); break;
-    case 3: e = launch_zproj<3, 1>(a, ny, nvols, ctx->stream, nparts); break;
-    case 6: e = launch_zproj<3, 2>(a, ny, nvols, ctx->stream, nparts); break;
-    case 12: e = launch_zproj<6, 2>(a, ny, nvols, ctx->stream, nparts); break;
-    case 24: e = launch_zproj<6, 4>(a, ny, nvols, ctx->stream, nparts); break;
-    case 48: e = launch_zproj<12, 4>(a, ny, nvols, ctx->stream, nparts); break;
-    case 96: e = launch_zproj<12, 8>(a, ny, nvols, ctx->stream, nparts); break;
-    case 192: e = launch_zproj<12, 16>(a, ny, nvols, ctx->stream, nparts); break;
-    case 384: e = launch_zproj<12, 32>(a, ny, nvols, ctx->stream, nparts); break;
+    case 2: e = launch_zproj<2, 1>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 4: e = launch_zproj<2, 2>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 8: e = launch_zproj<4, 2>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 16: e = launch_zproj<4, 4>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 32: e = launch_zproj<8, 4>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 64: e = launch_zproj<8, 8>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 128: e = launch_zproj<16, 8>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 256: e = launch_zproj<16, 16>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 512: e = launch_zproj<32, 16>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 3: e = launch_zproj<3, 1>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 6: e = launch_zproj<3, 2>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 12: e = launch_zproj<6, 2>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 24: e = launch_zproj<6, 4>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 48: e = launch_zproj<12, 4>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 96: e = launch_zproj<12, 8>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 192: e = launch_zproj<12, 16>(a, ny, nvols, h16, ctx->stream, nparts); break;
+    case 384: e = launch_zproj<12, 32>(a, ny, nvols, h16, ctx->stream, nparts); break;
     default: return fail(LLR_ERR_UNSUPPORTED, "z length " + std::to_string(n) + " is not a supported FFT length");
   }
   if (e != hipSuccess) return hip_fail(e, "fft z projection launch");

@@ -10,11 +10,13 @@ namespace llr {
 
 // fft.hip
 bool fft_size_supported(size_t n);
-llr_status fft_axis_pass(llr_ctx* ctx, const float2* src, float2* dst, const size_t* dims, size_t nvols, int axis,
+// src/dst/data/meas hold complex64, or half-precision pairs (ch16) when h16 is set
+// (ramped signal-model passes only).
+llr_status fft_axis_pass(llr_ctx* ctx, const void* src, void* dst, const size_t* dims, size_t nvols, int axis,
                          bool inverse, const float2* pre, bool pre_conj, const float2* post, bool post_conj,
-                         float scale);
-llr_status fft_zproj(llr_ctx* ctx, float2* data, const float2* meas, const size_t* dims, size_t nvols,
-                     const float2* pk, const float2* pi, double* partial, size_t* nparts);
+                         float scale, bool h16 = false);
+llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* dims, size_t nvols,
+                     const float2* pk, const float2* pi, double* partial, size_t* nparts, bool h16 = false);
 
 // llr.hip
 // One reconstruction's share of the persistent multi-reconstruction LLR launch.
@@ -43,8 +45,10 @@ llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols,
                        const size_t* blk, const size_t* shift, size_t start, size_t n, bool unlift);
 llr_status project_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K);
 // X[q][v] = sum_k C[k][v] conj(S[q][k]) for voxels inside the block grid of `shift`
-llr_status backproject_device(llr_ctx* ctx, const float2* C, float2* X, const float2* S, const size_t* vol,
-                              const size_t* blk, const size_t* shift, int nq, int K, bool partial = false);
+// (X as complex64, or half-precision pairs when h16 is set)
+llr_status backproject_device(llr_ctx* ctx, const float2* C, void* X, const float2* S, const size_t* vol,
+                              const size_t* blk, const size_t* shift, int nq, int K, bool partial = false,
+                              bool h16 = false, float scale = 1.f);
 llr_status llr_block_run(llr_ctx* ctx, const float2* F, float2* out, const float2* S, int nq, const size_t* vol,
                          const size_t* blk, const size_t* shift, int nc, int layout, long long block0,
                          long long nblocks, int mode, int rank, float lambda, unsigned* smax_bits, float* svals,
@@ -58,6 +62,10 @@ void wavelet_geometry(const size_t* dims, int L, std::vector<std::array<size_t, 
                       std::vector<std::array<size_t, 3>>& cdims);
 size_t wavelet_workspace_elems(const size_t* dims, int L, size_t* coef_elems, size_t* tmp_elems);
 llr_status project_bf16_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K);
+// C = X S on BF16 MFMA (hi + lo split of each half value) for an iterate held in
+// half storage (ch16), times out_scale
+llr_status project_s16_device(llr_ctx* ctx, const void* X, float2* Cp, const float2* S, size_t N, int nq, int K,
+                              float out_scale);
 llr_status backproject_bf16_device(llr_ctx* ctx, const float2* C, float2* X, const float2* S, const size_t* vol,
                                    const size_t* blk, const size_t* shift, int nq, int K, bool partial);
 llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const size_t* blk, const size_t* shift,
@@ -66,8 +74,23 @@ llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const
 size_t llr_block_mixed_smem(long long m, int nc);
 llr_status dti_scalars_device(llr_ctx* ctx, const float2* vols, size_t n_vox, size_t nq, const float* bvals,
                               const float* bvecs, float b0_tol, float* out);
-llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, const size_t* dims, float alpha, int L,
-                                  float2* coef, float2* t1, float2* t2);
+// Level 0 of the wavelet taken from subspace coefficients: the iterate is
+// X_q = sum_k C_k conj(S[q][k]) at every voxel (K planes C, FP32); BK is FP32
+// workspace of wavelet_subspace_elems values.
+struct WaveletFromSubspace {
+  const float2* C;
+  const float2* S;
+  int K;
+  float2* BK;
+};
+size_t wavelet_subspace_elems(const size_t* dims, int L, int K);
+bool wavelet_subspace_supported(const size_t* dims, int L, int K);
+// The wavelet soft threshold of X_q = C S^H written into vols (every voxel).
+llr_status wavelet_denoise_subspace(llr_ctx* ctx, void* vols, size_t nvols, const size_t* dims, float alpha, int L,
+                                    void* coef, const WaveletFromSubspace& sub, bool h16);
+// vols and coef hold complex64, or half-precision pairs (ch16) when h16 is set
+llr_status wavelet_denoise_device(llr_ctx* ctx, void* vols, size_t nvols, const size_t* dims, float alpha, int L,
+                                  void* coef, void* t1, void* t2, bool h16 = false);
 
 // unet.hip
 llr_status unet_predict_device(llr_unet* U, const float2* coef, int K, const size_t* vol, float scale,
@@ -77,6 +100,9 @@ llr_ctx* unet_context(llr_unet* U);
 
 // misc.hip
 llr_status shrink_all_device(llr_ctx* ctx, float2* x, size_t n, float alpha);
+// complex64 -> half-precision pairs (x s) and back (x s); n complex values, n even
+llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s);
+llr_status from_h16_device(llr_ctx* ctx, const void* src, float2* dst, size_t n, float s);
 llr_status energy_partials(llr_ctx* ctx, const float2* y, size_t n, double* partials, size_t nparts);
 llr_status reduce_strided(llr_ctx* ctx, const double* partials, size_t n, size_t stride, double* out);
 llr_status reduce_at(llr_ctx* ctx, const double* partials, size_t n, double* out, const unsigned* at, double* tmp);

@@ -90,18 +90,23 @@ __global__ void __launch_bounds__(256) k_project(const float2* __restrict__ X, f
 // block grid (the back-projection of dti_back_project + conj_transpose,
 // lib.rs:288-289), streamed in voxel order; voxels the grid leaves out keep
 // their value (block.rs:112-118 skips the remainder).
-template <int K>
-__global__ void __launch_bounds__(256) k_backproject(const float2* __restrict__ L, float2* __restrict__ X,
+// X is stored as ST (complex64 or half-precision pairs); `full` skips the coverage test
+// when the grid covers every voxel.
+template <int K, class ST>
+__global__ void __launch_bounds__(256) k_backproject(const float2* __restrict__ L, typename ST::T* __restrict__ X,
                                                      const float2* __restrict__ S, int nx, int ny, int nz, int cx,
-                                                     int cy, int cz, int sx, int sy, int sz, int nq) {
+                                                     int cy, int cz, int sx, int sy, int sz, int nq, int full,
+                                                     float scale) {
   const unsigned N = (unsigned)nx * ny * nz;
   for (unsigned v = blockIdx.x * blockDim.x + threadIdx.x; v < N; v += gridDim.x * blockDim.x) {
-    int x = (int)(v % (unsigned)nx), y = (int)((v / (unsigned)nx) % (unsigned)ny), z = (int)(v / ((unsigned)nx * ny));
-    int rx = x - sx, ry = y - sy, rz = z - sz;
-    rx += rx < 0 ? nx : 0;
-    ry += ry < 0 ? ny : 0;
-    rz += rz < 0 ? nz : 0;
-    if (rx >= cx || ry >= cy || rz >= cz) continue;
+    if (!full) {
+      int x = (int)(v % (unsigned)nx), y = (int)((v / (unsigned)nx) % (unsigned)ny), z = (int)(v / ((unsigned)nx * ny));
+      int rx = x - sx, ry = y - sy, rz = z - sz;
+      rx += rx < 0 ? nx : 0;
+      ry += ry < 0 ? ny : 0;
+      rz += rz < 0 ? nz : 0;
+      if (rx >= cx || ry >= cy || rz >= cz) continue;
+    }
     float2 c[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) c[k] = L[(size_t)k * N + v];
@@ -113,7 +118,7 @@ __global__ void __launch_bounds__(256) k_backproject(const float2* __restrict__ 
         acc.x += c[k].x * s.x + c[k].y * s.y;
         acc.y += c[k].y * s.x - c[k].x * s.y;
       }
-      X[(size_t)q * N + v] = acc;
+      ST::st(X + (size_t)q * N + v, cscale(acc, scale));
     }
   }
 }
@@ -438,8 +443,9 @@ llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols,
   return LLR_OK;
 }
 
-llr_status backproject_device(llr_ctx* ctx, const float2* C, float2* X, const float2* S, const size_t* vol,
-                              const size_t* blk, const size_t* shift, int nq, int K, bool partial) {
+llr_status backproject_device(llr_ctx* ctx, const float2* C, void* X, const float2* S, const size_t* vol,
+                              const size_t* blk, const size_t* shift, int nq, int K, bool partial, bool h16,
+                              float scale) {
   size_t N = vol[0] * vol[1] * vol[2];
   if (N >= 0xFFFFFFFFull) return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
   int cov[3], sh[3];
@@ -448,11 +454,16 @@ llr_status backproject_device(llr_ctx* ctx, const float2* C, float2* X, const fl
     cov[a] = (int)(partial ? vol[a] : (vol[a] / blk[a]) * blk[a]);
     sh[a] = (int)(shift[a] % vol[a]);
   }
+  const int full = cov[0] == (int)vol[0] && cov[1] == (int)vol[1] && cov[2] == (int)vol[2];
   unsigned grid = grid_for(N, 256, 65536u);
 #define BP_CASE(KK)                                                                                                 \
   case KK:                                                                                                          \
-    hipLaunchKernelGGL(k_backproject<KK>, dim3(grid), dim3(256), 0, ctx->stream, C, X, S, (int)vol[0], (int)vol[1], \
-                       (int)vol[2], cov[0], cov[1], cov[2], sh[0], sh[1], sh[2], nq);                               \
+    if (h16)                                                                                                       \
+      hipLaunchKernelGGL((k_backproject<KK, StoreF16>), dim3(grid), dim3(256), 0, ctx->stream, C, (ch16*)X, S,    \
+                         (int)vol[0], (int)vol[1], (int)vol[2], cov[0], cov[1], cov[2], sh[0], sh[1], sh[2], nq, full, scale); \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_backproject<KK, StoreF32>), dim3(grid), dim3(256), 0, ctx->stream, C, (float2*)X, S,    \
+                         (int)vol[0], (int)vol[1], (int)vol[2], cov[0], cov[1], cov[2], sh[0], sh[1], sh[2], nq, full, scale); \
     break;
   switch (K) {
     BP_CASE(1) BP_CASE(2) BP_CASE(3) BP_CASE(4) BP_CASE(5) BP_CASE(6) BP_CASE(7) BP_CASE(8)

@@ -133,6 +133,24 @@ __global__ void k_shrink_all(float2* x, long long n, float alpha) {
   }
 }
 
+// complex64 <-> half-precision-pair storage, scaled by s on the way (two values per thread)
+__global__ void __launch_bounds__(256) k_to_h16(const float4* __restrict__ src, uint2* __restrict__ dst, long long n2,
+                                                 float s) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2; i += (long long)gridDim.x * blockDim.x) {
+    float4 v = src[i];
+    dst[i] = make_uint2(StoreF16::pack(make_float2(v.x * s, v.y * s)), StoreF16::pack(make_float2(v.z * s, v.w * s)));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_from_h16(const uint2* __restrict__ src, float4* __restrict__ dst, long long n2,
+                                                   float s) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2; i += (long long)gridDim.x * blockDim.x) {
+    uint2 u = src[i];
+    float2 a = StoreF16::unpack(u.x), b = StoreF16::unpack(u.y);
+    dst[i] = make_float4(a.x * s, a.y * s, b.x * s, b.y * s);
+  }
+}
+
 __global__ void k_under_sample(float2* data, long long nx, long long pe, long long nq, const float2* mask) {
   long long total = nx * pe * nq;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
@@ -232,6 +250,22 @@ __global__ void k_transpose(const float2* __restrict__ in, float2* __restrict__ 
 }  // namespace
 
 namespace llr {
+
+llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s) {
+  if (n % 2) return fail(LLR_ERR_UNSUPPORTED, "half-precision conversion needs an even number of values");
+  hipLaunchKernelGGL(k_to_h16, dim3(grid_for(n / 2, 256, 65536u)), dim3(256), 0, ctx->stream, (const float4*)src,
+                     (uint2*)dst, (long long)(n / 2), s);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+llr_status from_h16_device(llr_ctx* ctx, const void* src, float2* dst, size_t n, float s) {
+  if (n % 2) return fail(LLR_ERR_UNSUPPORTED, "half-precision conversion needs an even number of values");
+  hipLaunchKernelGGL(k_from_h16, dim3(grid_for(n / 2, 256, 65536u)), dim3(256), 0, ctx->stream, (const uint2*)src,
+                     (float4*)dst, (long long)(n / 2), s);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
 
 llr_status shrink_all_device(llr_ctx* ctx, float2* x, size_t n, float alpha) {
   hipLaunchKernelGGL(k_shrink_all, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, x, (long long)n, alpha);

@@ -419,9 +419,129 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   }
 }
 
+// Projection C = X*S of an iterate held in half storage. An FP16 value has 11
+// significant bits, so it is exactly the sum of two BF16 numbers hi + lo; the
+// product is taken as hi*S_hi + hi*S_lo + lo*S_hi on BF16 MFMA with FP32
+// accumulation (the dropped lo*S_lo term is below 2^-16 relative), so it adds
+// no rounding beyond the storage itself. Tiles of P16_TV = 128 voxels (two
+// 16-voxel MFMA row blocks per wavefront); plane runs of 512 bytes are staged
+// in LDS as rows [voxel][plane] of 32-bit half pairs.
+constexpr int P16_TV = 128;
+
+template <int K, int MS>
+__global__ void __launch_bounds__(256) k_project_s16(const ch16* __restrict__ X, float2* __restrict__ Cp,
+                                                     const float2* __restrict__ S, long long N, int nq,
+                                                     float out_scale) {
+  constexpr int ROW = 8 * MS + 2;  // 32-bit words per staged voxel row (even: 8-byte aligned reads)
+  __shared__ __align__(16) unsigned s_x[P16_TV * ROW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = lane & 15, kq = lane >> 4;
+  const int NS = (2 * nq + 15) / 16;
+  bf16x4 bop[MS], bol[MS];
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kk = 16 * s + 4 * kq + j, q = kk >> 1, ic = kk & 1;
+      const int k = n >> 1, oc = n & 1;
+      float val = 0.f;
+      if (s < NS && q < nq && k < K) {
+        float2 sv = S[q + (long long)k * nq];
+        val = oc == 0 ? (ic == 0 ? sv.x : -sv.y) : (ic == 0 ? sv.y : sv.x);
+      }
+      bop[s][j] = bf16_bits(val);
+      bol[s][j] = bf16_bits(val - bf16_round(val));
+    }
+  }
+  const long long v0 = (long long)blockIdx.x * P16_TV;
+  constexpr int NLD = 8 * MS * P16_TV / 256;
+  unsigned xl[NLD];
+#pragma unroll
+  for (int it = 0; it < NLD; ++it) {
+    const int e = tid + 256 * it, vi = e % P16_TV, q = e / P16_TV;
+    xl[it] = 0u;
+    if (q < nq && v0 + vi < N) xl[it] = X[(long long)q * N + v0 + vi];
+  }
+#pragma unroll
+  for (int it = 0; it < NLD; ++it) {
+    const int e = tid + 256 * it, vi = e % P16_TV, q = e / P16_TV;
+    if (q < 8 * NS) s_x[vi * ROW + q] = xl[it];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const unsigned* row = s_x + (32 * wave + 16 * rb + n) * ROW + 2 * kq;
+#pragma unroll
+    for (int s = 0; s < MS; ++s) {
+      if (s < NS) {
+        const uint2 u = *(const uint2*)(row + 8 * s);
+        const float2 x0 = StoreF16::unpack(u.x), x1 = StoreF16::unpack(u.y);
+        const float xv[4] = {x0.x, x0.y, x1.x, x1.y};
+        bf16x4 ah, al;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          ah[j] = bf16_bits(xv[j]);
+          al[j] = bf16_bits(xv[j] - bf16_round(xv[j]));
+        }
+        acc = mfma16(ah, bop[s], acc);
+        acc = mfma16(ah, bol[s], acc);
+        acc = mfma16(al, bop[s], acc);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[j] *= out_scale;
+    const float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
+    const float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
+    const int k = n >> 1;
+    if (k < K) {
+      const long long w0 = v0 + 32 * wave + 16 * rb + 4 * kq + ((n & 1) ? 2 : 0);
+      float2 c0, c1;
+      if ((n & 1) == 0) {
+        c0 = make_float2(acc[0], p0);
+        c1 = make_float2(acc[1], p1);
+      } else {
+        c0 = make_float2(p2, acc[2]);
+        c1 = make_float2(p3, acc[3]);
+      }
+      float2* dst = Cp + (long long)k * N + w0;
+      if (w0 + 1 < N) {
+        dst[0] = c0;
+        dst[1] = c1;
+      } else if (w0 < N) {
+        dst[0] = c0;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 namespace llr {
+
+llr_status project_s16_device(llr_ctx* ctx, const void* X, float2* Cp, const float2* S, size_t N, int nq, int K,
+                              float out_scale) {
+  if (nq > 8 * MAX_STEPS) return fail(LLR_ERR_UNSUPPORTED, "BF16 projection handles at most 128 volumes");
+  const unsigned grid = (unsigned)((N + P16_TV - 1) / P16_TV);  // 128-voxel tiles
+  const bool small = 2 * nq <= 16 * 9;
+  const ch16* Xs = (const ch16*)X;
+#define PS_CASE(KK)                                                                                               \
+  case KK:                                                                                                        \
+    if (small)                                                                                                    \
+      hipLaunchKernelGGL((k_project_s16<KK, 9>), dim3(grid), dim3(256), 0, ctx->stream, Xs, Cp, S, (long long)N, nq, \
+                         out_scale);                                                                              \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_project_s16<KK, MAX_STEPS>), dim3(grid), dim3(256), 0, ctx->stream, Xs, Cp, S,          \
+                         (long long)N, nq, out_scale);                                                            \
+    break;
+  switch (K) {
+    PS_CASE(1) PS_CASE(2) PS_CASE(3) PS_CASE(4) PS_CASE(5) PS_CASE(6) PS_CASE(7) PS_CASE(8)
+    default: return fail(LLR_ERR_UNSUPPORTED, "BF16 projection handles subspace ranks up to 8");
+  }
+#undef PS_CASE
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
 
 llr_status project_bf16_device(llr_ctx* ctx, const float2* X, float2* Cp, const float2* S, size_t N, int nq, int K) {
   if (nq > 8 * MAX_STEPS) return fail(LLR_ERR_UNSUPPORTED, "BF16 projection handles at most 128 volumes");

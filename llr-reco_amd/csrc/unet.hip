@@ -79,6 +79,8 @@ struct ConvArgs {
   int D, H, W, cin, cout, k;
   int mirror;           // read offset kk-1-o (backward data)
   int relu, accumulate;
+  const bf16_t* up;     // 3^3 BF16 forward only: half-resolution source of input channels [0, cup), or null
+  int cup;
 };
 
 // out[v][co] = sum_{o, ci} in[v + off(o)][ci] w[o][co][ci] (+ bias, ReLU, mask, +=)
@@ -148,25 +150,39 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
   }
 }
 
-// 3^3 convolution with the input staged in LDS. A workgroup owns CV_M
-// consecutive voxels (each wavefront CV_MT m-tiles of 16); for every chunk of
-// 16 input channels it stages the nine (dz, dy) runs of CV_M + 2 voxels its
-// 27 offsets read, so each input value is fetched from memory about 9 / 3
-// times per workgroup instead of once per offset and lane. FP32 inputs (the
-// backward-data gradients) are rounded to BF16 while staged. Neighbours
-// outside the volume are masked per lane, as in k_conv.
+// 3^3 convolution with the input and the weights staged in LDS. A workgroup
+// owns CV_M consecutive voxels (each wavefront CV_MT m-tiles of 16); for every
+// chunk of 16 input channels it stages the nine (dz, dy) runs of CV_M + 2
+// voxels its 27 offsets read, so each input value is fetched from memory about
+// 9 / 3 times per workgroup instead of once per offset and lane, and the
+// chunk's 27 x cout x 16 weights, so the MFMA loop reads only LDS. Rows are
+// padded to CV_ROW values (40 bytes) so the 16 lanes of an operand read hit
+// distinct banks. FP32 inputs (the backward-data gradients) are rounded to
+// BF16 while staged. With a.up set, input channels [0, a.cup) are the nearest
+// upsampling of the half-resolution tensor a.up and channels [a.cup, cin) come
+// from a.in (the decoder's [upsampled, skip] concatenation, never stored).
+// Neighbours outside the volume are masked per lane, as in k_conv.
 constexpr int CV_MT = 2;
 constexpr int CV_M = 64 * CV_MT;
 constexpr int CV_RUN = CV_M + 2;
+constexpr int CV_ROW = 20;
+
+__device__ __forceinline__ bf16x4 lds4(const bf16_t* p) {
+  const uint2 u = *(const uint2*)p;
+  return bf16x4{(short)(u.x & 0xFFFF), (short)(u.x >> 16), (short)(u.y & 0xFFFF), (short)(u.y >> 16)};
+}
 
 template <typename TIn, typename TOut, int NTM>
 __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
-  __shared__ __align__(16) bf16_t s_run[9 * CV_RUN * 16];
+  __shared__ __align__(16) bf16_t s_run[9 * CV_RUN * CV_ROW];
+  __shared__ __align__(16) bf16_t s_w[27 * 16 * NTM * CV_ROW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = lane & 15, kq = lane >> 4;
   const long long HW = (long long)a.H * a.W, V = HW * a.D;
   const long long vb = (long long)blockIdx.x * CV_M;
-  const int nt = a.cout / 16;
+  const int cb = blockIdx.y * 16 * NTM;             // this workgroup's output channels [cb, cb + 16 nt)
+  const int nt = min(NTM, (a.cout - cb) / 16);
+  const int cin2 = a.cin - a.cup;  // channels per voxel of a.in
   int vx[CV_MT], vy[CV_MT], vz[CV_MT];
   bool vin[CV_MT];
 #pragma unroll
@@ -183,13 +199,20 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
 #pragma unroll
     for (int t = 0; t < NTM; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < a.cin; c0 += 16) {
+    const bool from_up = a.up != nullptr && c0 < a.cup;
     for (int e = tid; e < 9 * CV_RUN * 2; e += 256) {
       const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
       const long long idx = vb - 1 + i + (long long)(r / 3 - 1) * HW + (long long)(r % 3 - 1) * a.W;
       uint4 u = make_uint4(0u, 0u, 0u, 0u);
       if (idx >= 0 && idx < V) {
         if constexpr (sizeof(TIn) == 2) {
-          u = *(const uint4*)((const bf16_t*)a.in + idx * a.cin + c0 + 8 * half);
+          if (from_up) {
+            const int x = (int)(idx % a.W), y = (int)((idx / a.W) % a.H), z = (int)(idx / HW);
+            const long long ci = ((long long)(z >> 1) * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
+            u = *(const uint4*)(a.up + ci * a.cup + c0 + 8 * half);
+          } else {
+            u = *(const uint4*)((const bf16_t*)a.in + idx * cin2 + (c0 - a.cup) + 8 * half);
+          }
         } else {
           const float* q = (const float*)a.in + idx * a.cin + c0 + 8 * half;
           const float4 f0 = *(const float4*)q, f1 = *(const float4*)(q + 4);
@@ -199,9 +222,22 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
           u.w = (unsigned)f2bf(f1.z) | ((unsigned)f2bf(f1.w) << 16);
         }
       }
-      *(uint4*)(s_run + (size_t)(r * CV_RUN + i) * 16 + 8 * half) = u;
+      uint2* d = (uint2*)(s_run + (size_t)(r * CV_RUN + i) * CV_ROW + 8 * half);
+      d[0] = make_uint2(u.x, u.y);
+      d[1] = make_uint2(u.z, u.w);
+    }
+    // this chunk's weights: s_w[o][co - cb][0..16) = w[o'][co][c0..c0+16), o' = o or 26 - o
+    const int ncw = 16 * nt;
+    for (int e = tid; e < 27 * ncw * 2; e += 256) {
+      const int half = e & 1, col = (e >> 1) % ncw, o = (e >> 1) / ncw;
+      const int ow = a.mirror ? 26 - o : o;
+      const uint4 u = *(const uint4*)(a.w + ((size_t)ow * a.cout + cb + col) * a.cin + c0 + 8 * half);
+      uint2* d = (uint2*)(s_w + (size_t)(o * 16 * NTM + col) * CV_ROW + 8 * half);
+      d[0] = make_uint2(u.x, u.y);
+      d[1] = make_uint2(u.z, u.w);
     }
     __syncthreads();
+#pragma unroll 3
     for (int o = 0; o < 27; ++o) {
       const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1, r = o / 3;
       bf16x4 av[CV_MT];
@@ -211,14 +247,13 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
         const bool valid = vin[mt] && xx >= 0 && xx < a.W && yy >= 0 && yy < a.H && zz >= 0 && zz < a.D;
         const int pos = (wave * CV_MT + mt) * 16 + n + dx + 1;
         av[mt] = bf16x4{0, 0, 0, 0};
-        if (valid) av[mt] = load4(s_run + (size_t)(r * CV_RUN + pos) * 16 + 4 * kq);
+        if (valid) av[mt] = lds4(s_run + (size_t)(r * CV_RUN + pos) * CV_ROW + 4 * kq);
       }
-      const int ow = a.mirror ? 26 - o : o;
-      const bf16_t* wo = a.w + (size_t)ow * a.cout * a.cin;
+      const bf16_t* wo = s_w + (size_t)o * 16 * NTM * CV_ROW;
 #pragma unroll
       for (int t = 0; t < NTM; ++t) {
         if (t < nt) {
-          const bf16x4 bv = load4(wo + (size_t)(16 * t + n) * a.cin + c0 + 4 * kq);
+          const bf16x4 bv = lds4(wo + (size_t)(16 * t + n) * CV_ROW + 4 * kq);
 #pragma unroll
           for (int mt = 0; mt < CV_MT; ++mt) acc[mt][t] = mfma16(av[mt], bv, acc[mt][t]);
         }
@@ -232,7 +267,7 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
 #pragma unroll
     for (int t = 0; t < NTM; ++t) {
       if (t < nt) {
-        const int co = 16 * t + n;
+        const int co = cb + 16 * t + n;
         const float b = a.bias ? a.bias[co] : 0.f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -689,9 +724,13 @@ static llr_status ensure_workspace(llr_unet* U, const size_t* dims) {
 // dims are (nx, ny, nz) = (W, H, D)
 template <typename TIn, typename TOut>
 static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int lvl, const TIn* in, TOut* out,
-                       bool backward_data, const bf16_t* mask, bool accumulate) {
+                       bool backward_data, const bf16_t* mask, bool accumulate, const bf16_t* up = nullptr,
+                       int cup = 0) {
   ConvArgs a{};
   a.in = in;
+  a.up = up;
+  a.cup = up ? cup : 0;
+  if (up && (u.k != 3 || backward_data || cup % 16)) return fail(LLR_ERR_ARG, "fused upsampling: 3^3 forward only");
   a.w = (backward_data ? U->Wt.as<bf16_t>() : U->Wf.as<bf16_t>()) + u.wb_off;
   a.bias = backward_data ? nullptr : U->P.as<float>() + u.b_off;
   a.mask = mask;
@@ -708,19 +747,17 @@ static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int 
   if (a.cout > 16 * MAX_CO_TILES || a.cin % 16 || a.cout % 16) return fail(LLR_ERR_UNSUPPORTED, "U-Net channel count");
   const long long V = (long long)a.D * a.H * a.W;
   if (a.k == 3) {
+    // output channels in slices of at most 64 per workgroup (grid.y), so the
+    // staged weights fit in LDS beside the input runs
     const int nt = a.cout / 16;
-    dim3 g((unsigned)((V + CV_M - 1) / CV_M));
+    const unsigned gx = (unsigned)((V + CV_M - 1) / CV_M);
     hipStream_t s = U->ctx->stream;
     if (nt <= 1)
-      hipLaunchKernelGGL((k_conv3<TIn, TOut, 1>), g, dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_conv3<TIn, TOut, 1>), dim3(gx), dim3(256), 0, s, a);
     else if (nt <= 2)
-      hipLaunchKernelGGL((k_conv3<TIn, TOut, 2>), g, dim3(256), 0, s, a);
-    else if (nt <= 4)
-      hipLaunchKernelGGL((k_conv3<TIn, TOut, 4>), g, dim3(256), 0, s, a);
-    else if (nt <= 8)
-      hipLaunchKernelGGL((k_conv3<TIn, TOut, 8>), g, dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_conv3<TIn, TOut, 2>), dim3(gx), dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL((k_conv3<TIn, TOut, MAX_CO_TILES>), g, dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_conv3<TIn, TOut, 4>), dim3(gx, (unsigned)((nt + 3) / 4)), dim3(256), 0, s, a);
   } else {
     unsigned grid = (unsigned)((V + 63) / 64);
     hipLaunchKernelGGL((k_conv<TIn, TOut>), dim3(grid), dim3(256), 0, U->ctx->stream, a);
@@ -754,8 +791,10 @@ static llr_status wgrad(llr_unet* U, const UnetLayer& u, const size_t* dims, int
 
 static unsigned ew_grid(size_t n) { return grid_for(n, 256, 16384); }
 
-// forward of the packed input U->xb / U->xf; prediction (input + net) in U->y
-static llr_status unet_forward(llr_unet* U, const size_t* dims) {
+// forward of the packed input U->xb / U->xf; prediction (input + net) in U->y.
+// keep_cat: store the decoder's [upsampled, skip] concatenations (the weight
+// gradients read them); inference reads the two halves in the convolution.
+static llr_status unet_forward(llr_unet* U, const size_t* dims, bool keep_cat = true) {
   llr_status st;
   const int L = U->levels;
   const size_t W = dims[0], H = dims[1], D = dims[2];
@@ -778,12 +817,17 @@ static llr_status unet_forward(llr_unet* U, const size_t* dims) {
   for (int l = L - 2; l >= 0; --l) {
     const bf16_t* coarse = (l == L - 2) ? U->b[L - 1]->as<bf16_t>() : U->db[l + 1]->as<bf16_t>();
     const int c = U->base << l, cu = U->base << (l + 1);
-    size_t n = (W >> l) * (H >> l) * (D >> l) * (size_t)(c + cu);
-    hipLaunchKernelGGL(k_concat_fwd, dim3(ew_grid(n)), dim3(256), 0, s, coarse, U->b[l]->as<bf16_t>(),
-                       U->cat[l]->as<bf16_t>(), (int)(D >> l), (int)(H >> l), (int)(W >> l), cu, c);
-    if ((st = conv(U, layer(U, "dec" + std::to_string(l) + "a"), dims, l, U->cat[l]->as<bf16_t>(),
-                   U->da[l]->as<bf16_t>(), false, nullptr, false)))
+    if (keep_cat) {
+      size_t n = (W >> l) * (H >> l) * (D >> l) * (size_t)(c + cu);
+      hipLaunchKernelGGL(k_concat_fwd, dim3(ew_grid(n)), dim3(256), 0, s, coarse, U->b[l]->as<bf16_t>(),
+                         U->cat[l]->as<bf16_t>(), (int)(D >> l), (int)(H >> l), (int)(W >> l), cu, c);
+      if ((st = conv(U, layer(U, "dec" + std::to_string(l) + "a"), dims, l, U->cat[l]->as<bf16_t>(),
+                     U->da[l]->as<bf16_t>(), false, nullptr, false)))
+        return st;
+    } else if ((st = conv(U, layer(U, "dec" + std::to_string(l) + "a"), dims, l, U->b[l]->as<bf16_t>(),
+                          U->da[l]->as<bf16_t>(), false, nullptr, false, coarse, cu))) {
       return st;
+    }
     if ((st = conv(U, layer(U, "dec" + std::to_string(l) + "b"), dims, l, U->da[l]->as<bf16_t>(),
                    U->db[l]->as<bf16_t>(), false, nullptr, false)))
       return st;
@@ -879,7 +923,7 @@ llr_status unet_predict_device(llr_unet* U, const float2* coef, int K, const siz
   hipLaunchKernelGGL(k_pack_coef<float>, dim3(vox_grid((size_t)N)), dim3(256), 0, U->ctx->stream, coef,
                      U->xf.as<float>(), N, K, U->cin, make_mix(to_net, K), 1.0f / scale);
   if ((st = unet_set_input(U, vol))) return st;
-  if ((st = unet_forward(U, vol))) return st;
+  if ((st = unet_forward(U, vol, false))) return st;
   hipLaunchKernelGGL(k_unpack_coef, dim3(vox_grid((size_t)N)), dim3(256), 0, U->ctx->stream, U->y.as<float>(), out,
                      N, K, U->cin, make_mix(from_net, K), scale);
   LLR_HIP(hipGetLastError());

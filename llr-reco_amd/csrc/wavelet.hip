@@ -66,6 +66,14 @@ __device__ __forceinline__ float2 shrink(float2 x, float alpha) {
   return make_float2(x.x * s, x.y * s);
 }
 
+// x <- shrink(x) for every stored value (a level-0 transform: the volume is its
+// own approximation)
+template <class ST>
+__global__ void __launch_bounds__(256) k_shrink_store(typename ST::T* x, long long n, float alpha) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    ST::st(x + i, shrink(ST::ld(x + i), alpha));
+}
+
 __device__ __forceinline__ void fma2(float2& acc, float w, float2 v) {
   acc.x += w * v.x;
   acc.y += w * v.y;
@@ -80,9 +88,9 @@ constexpr int ZT = 16;                               // z-lines per workgroup in
 constexpr int TPW = 4;  // tiles per workgroup (the next tile is prefetched into registers)
 
 struct XYAnArgs {
-  const float2* in;
+  const void* in;           // storage ST::T, as every buffer of the wavelet kernels
   long long ip1, ip2, ivp;  // input pitches (y, z) and volume pitch; x pitch 1
-  float2* B;
+  void* B;
   long long bx, by, bvp;    // band buffer row pitch (2 px), rows (2ncy) and volume pitch
   int px;                   // padded band width (ncx rounded up to 16): the detail band starts at x = px
   int ax, ay, az, ncx, ncy, tiles_x, groups_x;
@@ -93,7 +101,9 @@ constexpr int AN_NLD = (AN_W * AN_H + 255) / 256;
 
 // grid.x = groups_x * tiles_y (a group is TPW consecutive x tiles of one tile row),
 // grid.y = z * volumes
+template <class ST>
 __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
+  using E = typename ST::T;
   __shared__ float2 s_in[AN_H * AN_W];
   __shared__ float2 s_x[AN_H * 2 * AN_TCX];
   const int tid = threadIdx.x;
@@ -101,8 +111,8 @@ __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
   const int cy0 = (blockIdx.x / a.groups_x) * AN_TCY;
   const int z = blockIdx.y % a.az;
   const long long v = blockIdx.y / a.az;
-  const float2* src = a.in + v * a.ivp + (long long)z * a.ip2;
-  float2* dst = a.B + v * a.bvp + (long long)z * a.bx * a.by;
+  const E* src = (const E*)a.in + v * a.ivp + (long long)z * a.ip2;
+  E* dst = (E*)a.B + v * a.bvp + (long long)z * a.bx * a.by;
   const int t0 = gx * TPW;
   const int nt = min(TPW, a.tiles_x - t0);
   float2 xn[AN_NLD];
@@ -112,7 +122,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
       int e = tid + 256 * q;
       if (e < AN_W * AN_H) {
         int i = e % AN_W, j = e / AN_W;
-        xn[q] = src[(long long)sym(2 * cy0 - 2 + j, a.ay) * a.ip1 + sym(2 * cx0 - 2 + i, a.ax)];
+        xn[q] = ST::ld(src + (long long)sym(2 * cy0 - 2 + j, a.ay) * a.ip1 + sym(2 * cx0 - 2 + i, a.ax));
       }
     }
   };
@@ -152,18 +162,18 @@ __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
       int cxl = c2 < AN_TCX ? c2 : c2 - AN_TCX;
       if (cx0 + cxl < a.ncx && cy0 + cy < a.ncy) {
         long long gxo = (c2 < AN_TCX) ? cx0 + cxl : a.px + cx0 + cxl;
-        dst[(long long)(cy0 + cy) * a.bx + gxo] = sa;
-        dst[(long long)(a.ncy + cy0 + cy) * a.bx + gxo] = sd;
+        ST::st(dst + (long long)(cy0 + cy) * a.bx + gxo, sa);
+        ST::st(dst + (long long)(a.ncy + cy0 + cy) * a.bx + gxo, sd);
       }
     }
   }
 }
 
 struct ZArgs {
-  float2* B;
+  void* B;
   long long bx, by, bvp;
   int az, ncz, ncx, ncy, px, tiles_x, groups_x;
-  float2* A;  // next-level input (ncx x ncy x ncz, row pitch pa) or null at the deepest level
+  void* A;    // next-level input (ncx x ncy x ncz, row pitch pa) or null at the deepest level
   long long avp;
   int pa;
   int deepest;
@@ -173,8 +183,9 @@ struct ZArgs {
 
 // grid.x = groups_x * by (TPW consecutive tiles of ZT x-neighbour z-lines at one Y), grid.y = volumes.
 // NZR >= ceil(az / 16) values of a line are prefetched per thread.
-template <int NZR>
+template <int NZR, class ST>
 __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
+  using E = typename ST::T;
   extern __shared__ float2 zs[];
   float2* L = zs;                    // az x ZT
   float2* Cf = zs + a.az * ZT;       // 2ncz x ZT
@@ -184,7 +195,7 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
   const int Y = blockIdx.x / a.groups_x;
   const long long v = blockIdx.y;
   const long long zp = a.bx * a.by;
-  float2* row = a.B + v * a.bvp + (long long)Y * a.bx;
+  E* row = (E*)a.B + v * a.bvp + (long long)Y * a.bx;
   const int t0 = gx * TPW;
   const int nt = min(TPW, a.tiles_x - t0);
   float2 xn[NZR];
@@ -193,7 +204,7 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
 #pragma unroll
     for (int q = 0; q < NZR; ++q) {
       int z = r + (256 / ZT) * q;
-      if (z < a.az) xn[q] = act ? row[X + z * zp] : make_float2(0.f, 0.f);
+      if (z < a.az) xn[q] = act ? ST::ld(row + X + z * zp) : make_float2(0.f, 0.f);
     }
   };
   load(t0 * ZT + t);
@@ -219,7 +230,7 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
       if (!ll || a.deepest) {
         sa = shrink(sa, a.alpha);
       } else {
-        a.A[v * a.avp + (long long)c * a.pa * a.ncy + (long long)Y * a.pa + X] = sa;
+        ST::st((E*)a.A + v * a.avp + (long long)c * a.pa * a.ncy + (long long)Y * a.pa + X, sa);
         sa = make_float2(0.f, 0.f);
       }
       Cf[c * ZT + t] = sa;
@@ -238,21 +249,115 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
           fma2(acc, a.f.rhi[j], Cf[(a.ncz + tt) * ZT + t]);
         }
       }
-      if (act) row[X + s * zp] = acc;
+      if (act) ST::st(row + X + s * zp, acc);
     }
     __syncthreads();  // L and Cf are rewritten by the next tile
   }
 }
 
+// Level-0 z pass of an iterate that is X_q = sum_k C_k conj(S[q][k]) at every
+// voxel (the LLR step's output when its block grid covers the volume). The
+// wavelet's analysis is linear with real filters, so the xy-bands of X_q are
+// the same combination of the xy-bands of the K coefficient planes (pass 1 run
+// on the planes, FP32). A workgroup keeps its ZT z-lines of all K plane bands
+// in registers and, for every volume in turn, expands them (FP32), analyses
+// the lines along z, shrinks and synthesises them back, writing the volume's
+// bands (and the next level's input) exactly as k_wav_z would have: the
+// back-projection and the per-volume xy analysis never touch HBM.
+struct ZXArgs {
+  const float2* BK;  // K planes of xy-bands, layout of B (FP32), plane pitch bkvp
+  long long bkvp;
+  const float2* S;   // nq x K, column-major
+  int K, nq;
+  ZArgs z;           // per-volume band buffer B (ST), next-level input A, geometry
+};
+
+constexpr int ZX_KM = 8;  // coefficient planes held per thread (K <= 8)
+
+// grid.x = tiles_x * by (one tile of ZT x-neighbour z-lines at one Y); NZR >= ceil(az / 16)
+template <int NZR, class ST>
+__global__ void __launch_bounds__(256) k_wav_z_expand(ZXArgs g) {
+  using E = typename ST::T;
+  const ZArgs& a = g.z;
+  extern __shared__ float2 zs[];
+  float2* L = zs;               // az x ZT
+  float2* Cf = zs + a.az * ZT;  // 2ncz x ZT
+  const int tid = threadIdx.x;
+  const int t = tid % ZT, r = tid / ZT;
+  const int Y = blockIdx.x / a.tiles_x;
+  const int X = (blockIdx.x % a.tiles_x) * ZT + t;
+  const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx;
+  const long long zp = a.bx * a.by;
+  float2 kv[ZX_KM][NZR];
+#pragma unroll
+  for (int k = 0; k < ZX_KM; ++k)
+#pragma unroll
+    for (int q = 0; q < NZR; ++q) {
+      const int z = r + (256 / ZT) * q;
+      kv[k][q] = (k < g.K && z < a.az && act) ? g.BK[k * g.bkvp + (long long)Y * a.bx + X + z * zp]
+                                              : make_float2(0.f, 0.f);
+    }
+  const bool ll = X < a.ncx && Y < a.ncy;
+  for (int v = 0; v < g.nq; ++v) {
+    float2 sc[ZX_KM];
+#pragma unroll
+    for (int k = 0; k < ZX_KM; ++k) sc[k] = k < g.K ? g.S[v + (long long)k * g.nq] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int q = 0; q < NZR; ++q) {
+      const int z = r + (256 / ZT) * q;
+      if (z < a.az) {
+        float2 x = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < ZX_KM; ++k) x = cadd(x, cmulc(kv[k][q], sc[k]));
+        L[z * ZT + t] = x;
+      }
+    }
+    __syncthreads();  // L ready; the previous volume's synthesis is done with Cf
+    E* row = (E*)a.B + v * a.bvp + (long long)Y * a.bx;
+    for (int c = r; c < a.ncz; c += 256 / ZT) {
+      float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float2 x = L[sym(2 * c + 1 - k, a.az) * ZT + t];
+        fma2(sa, a.f.dlo[k], x);
+        fma2(sd, a.f.dhi[k], x);
+      }
+      sd = shrink(sd, a.alpha);
+      if (!ll || a.deepest) {
+        sa = shrink(sa, a.alpha);
+      } else {
+        ST::st((E*)a.A + v * a.avp + (long long)c * a.pa * a.ncy + (long long)Y * a.pa + X, sa);
+        sa = make_float2(0.f, 0.f);
+      }
+      Cf[c * ZT + t] = sa;
+      Cf[(a.ncz + c) * ZT + t] = sd;
+    }
+    __syncthreads();  // Cf ready; L is free for the next volume
+    for (int s = r; s < a.az; s += 256 / ZT) {
+      float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int idx = s + 2 - j;
+        if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0) {
+          int tt = idx >> 1;
+          fma2(acc, a.f.rlo[j], Cf[tt * ZT + t]);
+          fma2(acc, a.f.rhi[j], Cf[(a.ncz + tt) * ZT + t]);
+        }
+      }
+      if (act) ST::st(row + X + s * zp, acc);
+    }
+  }
+}
+
 struct XYSynArgs {
-  const float2* B;
+  const void* B;
   long long bx, by, bvp;
   int ncx, ncy, px;
-  const float2* Ap;  // reconstructed next level (ncx x ncy x ncz, row pitch pa) or null
+  const void* Ap;    // reconstructed next level (ncx x ncy x ncz, row pitch pa) or null
   long long apvp;
   int pa;
   int ncz;           // this level's z coefficients per band (Ap's z length)
-  float2* out;
+  void* out;
   long long op1, op2, ovp;
   int ax, ay, az, tiles_x, groups_x;
   Db2F f;
@@ -260,7 +365,9 @@ struct XYSynArgs {
 
 constexpr int SY_NLD = (4 * SY_NY * SY_NX + 255) / 256;
 
+template <class ST>
 __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
+  using E = typename ST::T;
   __shared__ float2 band[4 * SY_NY * SY_NX];   // [xb + 2*yb][iy][ix]
   __shared__ float2 yrow[2 * SY_TY * SY_NX];   // [xb][sy][ix]
   const int tid = threadIdx.x;
@@ -269,8 +376,8 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   const int z = blockIdx.y % a.az;
   const long long v = blockIdx.y / a.az;
   const int ty0 = y0 / 2;
-  const float2* src = a.B + v * a.bvp + (long long)z * a.bx * a.by;
-  const float2* ap = a.Ap ? a.Ap + v * a.apvp : nullptr;
+  const E* src = (const E*)a.B + v * a.bvp + (long long)z * a.bx * a.by;
+  const E* ap = a.Ap ? (const E*)a.Ap + v * a.apvp : nullptr;
   // the LL band of this z slice picks up syn_z(Ap, 0): taps j with z+2-j even and in range
   int za = -1, zb = -1;
   float wa = 0.f, wb = 0.f;
@@ -307,18 +414,18 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
         if (cx < a.ncx && cy < a.ncy) {
           long long gxo = (b & 1) ? a.px + cx : cx;
           long long gyo = (b >> 1) ? a.ncy + cy : cy;
-          bn[q] = src[gyo * a.bx + gxo];
+          bn[q] = ST::ld(src + gyo * a.bx + gxo);
           if (b == 0 && ap) {
             long long o = (long long)cy * a.pa + cx;
-            if (za >= 0) an[q] = ap[(long long)za * a.pa * a.ncy + o];
-            if (zb >= 0) cn[q] = ap[(long long)zb * a.pa * a.ncy + o];
+            if (za >= 0) an[q] = ST::ld(ap + (long long)za * a.pa * a.ncy + o);
+            if (zb >= 0) cn[q] = ST::ld(ap + (long long)zb * a.pa * a.ncy + o);
           }
         }
       }
     }
   };
   load(t0 * SY_TX);
-  float2* dst = a.out + v * a.ovp + (long long)z * a.op2;
+  E* dst = (E*)a.out + v * a.ovp + (long long)z * a.op2;
   for (int it = 0; it < nt; ++it) {
     const int x0 = (t0 + it) * SY_TX, tx0 = x0 / 2;
 #pragma unroll
@@ -364,7 +471,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
           fma2(acc, a.f.rhi[j], yrow[(SY_TY + sy) * SY_NX + tt]);
         }
       }
-      dst[(long long)gyo * a.op1 + gxo] = acc;
+      ST::st(dst + (long long)gyo * a.op1 + gxo, acc);
     }
     __syncthreads();  // band and yrow are rewritten by the next tile
   }
@@ -417,21 +524,25 @@ size_t wavelet_workspace_elems(const size_t* dims, int L, size_t* coef_elems, si
   return c;
 }
 
-// In-place wavelet soft threshold of nvols volumes; coef holds nvols *
-// coef_elems elements (t1, t2 unused).
-llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, const size_t* dims, float alpha, int L,
-                                  float2* coef, float2* t1, float2* t2) {
-  (void)t1;
-  (void)t2;
+template <class ST>
+static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const size_t* dims, float alpha, int L,
+                              void* coef_v, const WaveletFromSubspace* sub) {
+  using E = typename ST::T;
+  E* vols = (E*)vols_v;
   Db2F f = db2_filters();
   size_t N = dims[0] * dims[1] * dims[2];
-  if (L <= 0) return shrink_all_device(ctx, vols, N * nvols, alpha);  // the volume is its own approximation
+  if (L <= 0) {  // the volume is its own approximation
+    hipLaunchKernelGGL(k_shrink_store<ST>, dim3(grid_for(N * nvols, 256)), dim3(256), 0, ctx->stream, vols,
+                       (long long)(N * nvols), alpha);
+    LLR_HIP(hipGetLastError());
+    return LLR_OK;
+  }
   std::vector<std::array<size_t, 3>> in_dims, cdims;
   wavelet_geometry(dims, L, in_dims, cdims);
-  std::vector<float2*> Bp(L), Ap(L, nullptr);
+  std::vector<E*> Bp(L), Ap(L, nullptr);
   std::vector<size_t> bvol(L), avol(L, 0), px(L), pa(L, 0);
   {
-    float2* p = coef;
+    E* p = (E*)coef_v;
     for (int l = 0; l < L; ++l) {
       px[l] = pad16(cdims[l][0] / 2);
       bvol[l] = 2 * px[l] * cdims[l][1] * in_dims[l][2];
@@ -451,13 +562,14 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     const size_t* cd = cdims[l].data();
     int ncx = (int)cd[0] / 2, ncy = (int)cd[1] / 2, ncz = (int)cd[2] / 2;
     const long long bx = 2 * (long long)px[l], by = (long long)cd[1];
-    // pass 1
+    const bool fused = l == 0 && sub != nullptr;
+    // pass 1 (on the K coefficient planes, FP32, when fused)
     XYAnArgs a1{};
-    a1.in = l == 0 ? vols : Ap[l];
+    a1.in = fused ? (const void*)sub->C : l == 0 ? (const void*)vols : (const void*)Ap[l];
     a1.ip1 = l == 0 ? (long long)in[0] : (long long)pa[l];
     a1.ip2 = a1.ip1 * (long long)in[1];
     a1.ivp = a1.ip2 * (long long)in[2];
-    a1.B = Bp[l];
+    a1.B = fused ? (void*)sub->BK : (void*)Bp[l];
     a1.bx = bx;
     a1.by = by;
     a1.bvp = (long long)bvol[l];
@@ -471,8 +583,11 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     a1.groups_x = (a1.tiles_x + TPW - 1) / TPW;
     a1.f = f;
     int tiles_y = (ncy + AN_TCY - 1) / AN_TCY;
-    dim3 g1((unsigned)(a1.groups_x * tiles_y), (unsigned)(in[2] * nvols));
-    hipLaunchKernelGGL(k_wav_xy_an, g1, dim3(256), 0, s, a1);
+    dim3 g1((unsigned)(a1.groups_x * tiles_y), (unsigned)(in[2] * (fused ? (size_t)sub->K : nvols)));
+    if (fused)
+      hipLaunchKernelGGL(k_wav_xy_an<StoreF32>, g1, dim3(256), 0, s, a1);
+    else
+      hipLaunchKernelGGL(k_wav_xy_an<ST>, g1, dim3(256), 0, s, a1);
     LLR_HIP(hipGetLastError());
     // pass 2
     ZArgs a2{};
@@ -493,16 +608,35 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     a2.pa = a2.deepest ? 0 : (int)pa[l + 1];
     a2.alpha = alpha;
     a2.f = f;
-    size_t smem = (size_t)(in[2] + 2 * ncz) * ZT * sizeof(float2);
+    size_t smem = (size_t)(in[2] + 2 * ncz) * ZT * sizeof(float2);  // LDS holds FP32 values
     if (smem > 160 * 1024 || in[2] > 512) return fail(LLR_ERR_UNSUPPORTED, "z length too large for the wavelet z pass");
-    dim3 g2((unsigned)(a2.groups_x * by), (unsigned)nvols);
     int nzr = (int)((in[2] + (256 / ZT) - 1) / (256 / ZT));
+    if (fused) {
+      if (nzr > 8 || sub->K > ZX_KM) return fail(LLR_ERR_UNSUPPORTED, "fused wavelet: z length <= 128, rank <= 8");
+      ZXArgs ax{};
+      ax.BK = sub->BK;
+      ax.bkvp = (long long)bvol[0];
+      ax.S = sub->S;
+      ax.K = sub->K;
+      ax.nq = (int)nvols;
+      ax.z = a2;
+      dim3 gx((unsigned)(a2.tiles_x * by));
+#define ZX_CASE(NN)                                                                  \
+  if (nzr <= NN) {                                                                   \
+    hipLaunchKernelGGL((k_wav_z_expand<NN, ST>), gx, dim3(256), smem, s, ax);        \
+  } else
+      ZX_CASE(1) ZX_CASE(2) ZX_CASE(4) ZX_CASE(8) {}
+#undef ZX_CASE
+      LLR_HIP(hipGetLastError());
+      continue;
+    }
+    dim3 g2((unsigned)(a2.groups_x * by), (unsigned)nvols);
 #define WZ_CASE(NN)                                                                                       \
   if (nzr <= NN) {                                                                                        \
     if (smem > 64 * 1024)                                                                                 \
-      LLR_HIP(hipFuncSetAttribute((const void*)k_wav_z<NN>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+      LLR_HIP(hipFuncSetAttribute((const void*)k_wav_z<NN, ST>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                   (int)smem));                                                            \
-    hipLaunchKernelGGL(k_wav_z<NN>, g2, dim3(256), smem, s, a2);                                          \
+    hipLaunchKernelGGL((k_wav_z<NN, ST>), g2, dim3(256), smem, s, a2);                                    \
   } else
     WZ_CASE(1) WZ_CASE(2) WZ_CASE(4) WZ_CASE(8) WZ_CASE(16) WZ_CASE(32) {}
 #undef WZ_CASE
@@ -535,10 +669,38 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, float2* vols, size_t nvols, cons
     a3.f = f;
     int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
     dim3 g3((unsigned)(a3.groups_x * tiles_y), (unsigned)(od[2] * nvols));
-    hipLaunchKernelGGL(k_wav_xy_syn, g3, dim3(256), 0, s, a3);
+    hipLaunchKernelGGL(k_wav_xy_syn<ST>, g3, dim3(256), 0, s, a3);
     LLR_HIP(hipGetLastError());
   }
   return LLR_OK;
+}
+
+// In-place wavelet soft threshold of nvols volumes; coef holds nvols *
+// coef_elems elements of the same storage (t1, t2 unused).
+llr_status wavelet_denoise_device(llr_ctx* ctx, void* vols, size_t nvols, const size_t* dims, float alpha, int L,
+                                  void* coef, void* t1, void* t2, bool h16) {
+  (void)t1;
+  (void)t2;
+  return h16 ? wavelet_run<StoreF16>(ctx, vols, nvols, dims, alpha, L, coef, nullptr)
+              : wavelet_run<StoreF32>(ctx, vols, nvols, dims, alpha, L, coef, nullptr);
+}
+
+size_t wavelet_subspace_elems(const size_t* dims, int L, int K) {
+  if (L <= 0) return 0;
+  std::vector<std::array<size_t, 3>> in_dims, cdims;
+  wavelet_geometry(dims, L, in_dims, cdims);
+  return 2 * pad16(cdims[0][0] / 2) * cdims[0][1] * in_dims[0][2] * (size_t)K;
+}
+
+bool wavelet_subspace_supported(const size_t* dims, int L, int K) {
+  return L >= 1 && K >= 1 && K <= ZX_KM && dims[2] <= 128;
+}
+
+llr_status wavelet_denoise_subspace(llr_ctx* ctx, void* vols, size_t nvols, const size_t* dims, float alpha, int L,
+                                    void* coef, const WaveletFromSubspace& sub, bool h16) {
+  if (!wavelet_subspace_supported(dims, L, sub.K)) return fail(LLR_ERR_UNSUPPORTED, "fused subspace wavelet");
+  return h16 ? wavelet_run<StoreF16>(ctx, vols, nvols, dims, alpha, L, coef, &sub)
+             : wavelet_run<StoreF32>(ctx, vols, nvols, dims, alpha, L, coef, &sub);
 }
 
 }  // namespace llr

@@ -22,8 +22,12 @@ LLR_MODE_RANK = 0
 LLR_MODE_SVT = 1
 LLR_PREC_FP32 = 0
 LLR_PREC_MIXED = 1
-LLR_NUM_STAGES = 8
-STAGE_NAMES = ["llr_block", "subspace_project", "wavelet", "fft_x", "fft_y", "fft_z_project", "reductions", "setup"]
+LLR_STORE_FP32 = 0
+LLR_STORE_FP16 = 1
+ABI_VERSION = 103
+LLR_NUM_STAGES = 9
+STAGE_NAMES = ["llr_block", "subspace_project", "wavelet", "fft_x", "fft_y", "fft_z_project", "reductions", "setup",
+               "unet"]
 
 # every symbol include/llr_hip.h declares
 EXPORTS = [
@@ -86,6 +90,7 @@ class ReconParams(ctypes.Structure):
         ("shifts", ctypes.c_void_p),
         ("residual_stop", ctypes.c_double),
         ("partial_blocks", ctypes.c_int),
+        ("storage", ctypes.c_int),
     ]
 
 

@@ -56,7 +56,7 @@ def wavelet_denoise(volumes, n_volumes, vol_size, alpha, levels=-1, ctx=None):
 
 def make_params(vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0, lambda_factor=0.02,
                 wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0, precision="fp32",
-                partial_blocks=False):
+                partial_blocks=False, storage="fp32"):
     """llr_recon_params for the C ABI, plus the arrays it points at (keep them alive)."""
     vol = [int(v) for v in vol]
     nq = int(nq)
@@ -87,6 +87,7 @@ def make_params(vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lamb
     p.shifts = ctypes.c_void_p(sh.ctypes.data) if sh.shape[0] else None
     p.residual_stop = float(residual_stop)
     p.partial_blocks = 1 if partial_blocks else 0
+    p.storage = {"fp32": _lib.LLR_STORE_FP32, "fp16": _lib.LLR_STORE_FP16}[storage]
     return p, (sh, sub)
 
 
@@ -98,14 +99,19 @@ class Recon:
     truncation to `rank`, low_rank.rs:10-69).
     partial_blocks: block grid ceil(vol/blk) with edge blocks truncated at the
     volume end, instead of floor(vol/blk) (block.rs:112-118).
+    storage: "fp32" (complex64) or "fp16" (two IEEE half-precision numbers per
+    complex value for the iterate, the measurements' working copy and the
+    wavelet workspace, held at a scale of sqrt(voxels / measurement energy);
+    precision "mixed" with a subspace only). Arithmetic is FP32 either way and
+    every output is in the caller's units.
     """
 
     def __init__(self, vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0,
                  lambda_factor=0.02, wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0,
-                 precision="fp32", partial_blocks=False, ctx=None):
+                 precision="fp32", partial_blocks=False, storage="fp32", ctx=None):
         self.ctx = ctx or _lib.default_context()
         p, self._keep = make_params(vol, nq, shifts, blk, llr_mode, rank, svt_lambda, lambda_factor, wavelet_alpha,
-                                    wavelet_levels, subspace, residual_stop, precision, partial_blocks)
+                                    wavelet_levels, subspace, residual_stop, precision, partial_blocks, storage)
         self.vol = [int(v) for v in vol]
         self.nq = int(nq)
         self.D = int(np.prod(self.vol)) * self.nq

@@ -8,6 +8,14 @@ FP32 sweep refines, so the image stays at FP32 accuracy (<= 1e-4); with a
 subspace the BF16 projection and back-projection round every coefficient to
 8 bits of mantissa, bounded here by 5e-3. Config 3 also reports the per-voxel
 FA / MD error of the tensor fit (diffusion_models.rs:21-129) inside the object.
+
+Half storage (iterate, measurements' working copy and wavelet workspace held
+as IEEE FP16 pairs at the scale sqrt(voxels / energy), every kernel computing
+in FP32): each pass rounds what it stores to 11 significant bits (relative
+2^-11), about ten roundings per iteration, none accumulating past the next
+data-consistency projection. The mixed form's BF16 back-projection dominates
+its error either way; bounded by 5e-3 relative NRMSE vs the FP64 oracle and
+2e-2 on the residuals.
 """
 import numpy as np
 import pytest
@@ -18,13 +26,13 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def run(ds, precision, **kw):
+def run(ds, precision, storage="fp32", **kw):
     from llr_hip import recon
 
     cfg = ds["cfg"]
     r = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode=kw.get("llr_mode", "svt"),
                     rank=kw.get("rank", 0), lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
-                    subspace=ds["subspace"], precision=precision)
+                    subspace=ds["subspace"], precision=precision, storage=storage)
     r.set_measurements(ds["meas"])
     hist = r.run()
     img = r.image()
@@ -121,3 +129,68 @@ def test_mixed_fp64_fallback_on_rank_deficient_blocks(llr):
     print(f"[mixed rank-one blocks] NRMSE vs fp64 {e:.3e}, fp64 eigen-solves {nf}/{nb}", flush=True)
     assert nf > nb // 4
     assert e <= 1e-5
+
+
+@pytest.mark.parametrize("vol", [(32, 32, 16), (48, 24, 16)])
+def test_fp16_storage_matches_fp64(llr, vol):
+    """Mixed form with half storage against the FP64 oracle, radix-2 and radix-3 sizes."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, vol=vol))
+    print(f"[fp16 storage {vol} x{ds['nq']}] running", flush=True)
+    img, hist, lam, _ = run(ds, "mixed", storage="fp16")
+    x64, h64, lam64, _ = fp64(ds)
+    img32 = run(ds, "mixed")[0]
+    e, e32 = H.nrmse(img, x64), H.nrmse(img32, x64)
+    print(f"[fp16 storage {vol} x{ds['nq']}] NRMSE vs fp64 {e:.3e} (fp32 storage {e32:.3e}); "
+          f"residual {hist[-1]:.5e} vs {h64[-1]:.5e}", flush=True)
+    assert np.all(np.isfinite(img))
+    assert abs(lam - lam64) <= 2e-2 * lam64
+    assert np.allclose(hist, h64, rtol=2e-2)
+    assert e <= 5e-3
+
+
+@pytest.mark.parametrize("gain", [1.0, 3.0e4, 1.0e-4])
+def test_fp16_storage_round_trips(llr, gain):
+    """Initial image in, image out at any data scale: half-precision rounding
+    only (relative 2^-11 per part, the stored data being scaled to O(1));
+    coefficients of the stored iterate match an FP64 projection of the rounded
+    image (the hi + lo BF16 MFMA projection carries X and S to ~2^-16) and come
+    back in the caller's units; set_coefficients scales them back in."""
+    from llr_hip import recon
+
+    ds = H.make(dict(H.SMALL_SUBSPACE, n_iter=1))
+    cfg = ds["cfg"]
+    vol, nq, K = ds["vol"], ds["nq"], cfg["srank"]
+    N = int(np.prod(vol))
+    rng = np.random.default_rng(5)
+    x = (gain * (rng.standard_normal(N * nq) + 1j * rng.standard_normal(N * nq))).astype(np.complex64)
+    r = recon.Recon(vol, nq, ds["shifts"], blk=cfg["blk"], lambda_factor=cfg["lambda_factor"],
+                    wavelet_alpha=cfg["wavelet_alpha"], subspace=ds["subspace"], precision="mixed", storage="fp16")
+    r.set_measurements((gain * ds["meas"]).astype(np.complex64))  # sets the storage scale
+    r.set_initial(x)
+    r.prepare()
+    out = r.image()
+    # stored values are x * hs, hs = sqrt(voxels / measurement energy); parts that
+    # land among half's subnormals (below 2^-14 once scaled) keep an absolute
+    # precision of 2^-24 / hs instead of a relative one
+    meas = (gain * ds["meas"]).astype(np.complex64)
+    hs = np.sqrt(N / np.sum(np.abs(meas.astype(np.complex128)) ** 2))
+    xr, xi = np.abs(x.real) * hs, np.abs(x.imag) * hs
+    er, ei = np.abs(out.real - x.real) * hs, np.abs(out.imag - x.imag) * hs
+    normal = np.concatenate([xr >= 2.0 ** -14, xi >= 2.0 ** -14])
+    err = np.concatenate([er, ei])
+    mag = np.concatenate([xr, xi])
+    rel = err[normal] / mag[normal]
+    assert np.all(np.isfinite(out))
+    assert rel.max() <= 2.0 ** -11 * 1.01 and err[~normal].max(initial=0.0) <= 2.0 ** -24
+    c = r.coefficients().reshape(K, N)
+    S = np.asarray(ds["subspace"], np.complex128)  # nq x K
+    ref = (S.T @ out.astype(np.complex128).reshape(nq, N))  # C[k][v] = sum_q X[q][v] S[q][k]
+    e = H.nrmse(c, ref)
+    # coefficients in, image out: X = C S^H, in the caller's units
+    r.set_coefficients(c.ravel())
+    back = r.image()
+    r.close()
+    e2 = H.nrmse(back, (np.conj(S) @ c.astype(np.complex128)).ravel())
+    print(f"[fp16 storage round trip, gain {gain:g}] max rel rounding {rel.max():.3e}; projection NRMSE {e:.3e}; "
+          f"back-projection NRMSE {e2:.3e}", flush=True)
+    assert e <= 1e-4 and e2 <= 1e-3
