@@ -820,10 +820,12 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
     if ((st = upload(r->t1, nullptr, te * p.nq * 8))) return bail(st);
     if ((st = upload(r->t2, nullptr, te * p.nq * 8))) return bail(st);
   }
+  if ((st = make_tables(r->tabs, p.vol, p.nq, nullptr, nullptr))) return bail(st);  // pk is redone by every set-up
   size_t np = 0;
   fft_zproj(ctx, nullptr, nullptr, p.vol, p.nq, nullptr, nullptr, nullptr, &np, r->s16);
   r->nparts = np;
-  if ((st = upload(r->partials, nullptr, std::max<size_t>(np, 2048) * 8))) return bail(st);
+  if ((st = upload(r->partials, nullptr, std::max<size_t>(np, 2 * energy_argmax_parts(r->N, p.nq)) * 8)))
+    return bail(st);
   if ((st = upload(r->stats, nullptr, 16))) return bail(st);
   if ((st = upload(r->hist, nullptr, std::max<size_t>(p.n_iter, 1) * 8))) return bail(st);
   if ((st = upload(r->smax, nullptr, 8))) return bail(st);
@@ -883,15 +885,24 @@ static llr_status recon_setup(llr_recon* r) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
   llr_status st;
-  // total energy (lib.rs:224, 409-417)
+  // total energy (lib.rs:224, 409-417) and the arg-max behind the linear
+  // k-space corrections (lib.rs:229-231) in one pass over the measurements
   {
     StageTimer t(r, 7);
-    if ((st = energy_partials(ctx, r->Y.as<float2>(), r->D, r->partials.as<double>(), 1024))) return st;
-    if ((st = reduce_strided(ctx, r->partials.as<double>(), 1024, 2, r->stats.as<double>()))) return st;
+    LLR_HIP(hipMemsetAsync(r->best.p, 0, r->nq * 8, ctx->stream));
+    if ((st = energy_argmax_device(ctx, r->Y.as<float2>(), r->N, r->nq, r->partials.as<double>(),
+                                   r->best.as<unsigned long long>())))
+      return st;
+    if ((st = reduce_strided(ctx, r->partials.as<double>(), energy_argmax_parts(r->N, r->nq), 2,
+                             r->stats.as<double>())))
+      return st;
   }
-  // linear k-space corrections (lib.rs:229-231)
-  LLR_HIP(hipMemsetAsync(r->best.p, 0, r->nq * 8, ctx->stream));
-  if ((st = argmax_volumes(ctx, r->Y.as<float2>(), r->N, r->nq, r->best.as<unsigned long long>()))) return st;
+  // the k-space ramp tables follow from the arg-max on the device; the image
+  // ramps (zero corrections) were made when the reconstruction was created
+  for (int a = 0; a < 3; ++a)
+    if ((st = ramp_from_argmax_device(ctx, r->tabs.pk[a].as<float2>(), p.vol, a, r->nq,
+                                      r->best.as<unsigned long long>())))
+      return st;
   std::vector<unsigned long long> best(r->nq);
   double stats[2];
   LLR_HIP(hipStreamSynchronize(ctx->stream));
@@ -905,7 +916,6 @@ static llr_status recon_setup(llr_recon* r) {
     size_t s[3] = {idx % p.vol[0], (idx / p.vol[0]) % p.vol[1], idx / (p.vol[0] * p.vol[1])};
     for (int a = 0; a < 3; ++a) r->ksp_corr[3 * q + a] = (float)((long long)s[a] - ((long long)p.vol[a] + 1) / 2);
   }
-  if ((st = make_tables(r->tabs, p.vol, r->nq, r->ksp_corr.data(), nullptr))) return st;
   if (r->s16) {
     // the working copy of the measurements, and the initial image if one was set
     StageTimer t(r, 7);

@@ -326,8 +326,11 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_z_proje
   const int tile0 = blockIdx.x * a.tiles_per_wg;
   const int ntiles = min(a.tiles_per_wg, a.tiles - tile0);
   double err = 0.0;
+  // the next tile's iterate (stage-1 lanes) and measurements (stage-2 lanes:
+  // k-space sample j + R1*k2) are loaded while the current tile computes
   float2 xn[R1];
-  auto load = [&](int tile, float2* x) {
+  typename ST::T mn[R2];
+  auto load = [&](int tile, float2* x, typename ST::T* m) {
     int u = tile * T + t;
     if (u < a.U && j < R2) {
       const E* p = data + (unsigned)(u * a.ustride + j * a.stride);
@@ -335,8 +338,14 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_z_proje
 #pragma unroll
       for (int j1 = 0; j1 < R1; ++j1) x[j1] = ST::ld(p + j1 * step);
     }
+    if (u < a.U && j < R1) {
+      const E* mp = meas + (unsigned)(u * a.ustride + j * a.stride);
+      unsigned step = (unsigned)(R1 * a.stride);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) m[k2] = mp[k2 * step];
+    }
   };
-  load(tile0, xn);
+  load(tile0, xn, mn);
   __syncthreads();
   for (int it = 0; it < ntiles; ++it) {
     const int u = (tile0 + it) * T + t;
@@ -349,21 +358,22 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_z_proje
 #pragma unroll
       for (int k1 = 0; k1 < R1; ++k1) lds[t * L::LS + k1 * L::R2P + j] = cmul(x[k1], s_tw[j * k1]);
     }
+    typename ST::T mc[R2];
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) mc[k2] = mn[k2];
     __syncthreads();
-    if (it + 1 < ntiles) load(tile0 + it + 1, xn);
+    if (it + 1 < ntiles) load(tile0 + it + 1, xn, mn);
     float2 y[R2];
     if (active && j < R1) {
 #pragma unroll
       for (int j2 = 0; j2 < R2; ++j2) y[j2] = lds[t * L::LS + j * L::R2P + j2];
       SmallDFT<R2>::run(y);
-      const E* mp = meas + (unsigned)(u * a.ustride + j * a.stride);
-      unsigned step = (unsigned)(R1 * a.stride);
       // k-space sample k = j + R1*k2: project, then start the adjoint
       // (R2-point inverse DFT over k2 for the fixed k1 = j)
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) {
         float2 xv = cscale(y[k2], a.scale);
-        float2 m = ST::ld(mp + k2 * step);
+        float2 m = ST::ld(&mc[k2]);
         if (m.x != 0.0f || m.y != 0.0f) {
           float2 mpv = cmul(m, s_pi[j + R1 * k2]);
           float2 d = csub(xv, mpv);

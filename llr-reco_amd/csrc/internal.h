@@ -100,6 +100,10 @@ llr_ctx* unet_context(llr_unet* U);
 
 // misc.hip
 llr_status shrink_all_device(llr_ctx* ctx, float2* x, size_t n, float alpha);
+// k-space ramp table of one axis (nvols x dims[axis]) from the per-volume
+// arg-max found by argmax_volumes (integer corrections, exact reduction)
+llr_status ramp_from_argmax_device(llr_ctx* ctx, float2* tab, const size_t* dims, int axis, size_t nvols,
+                                   const unsigned long long* best);
 // complex64 -> half-precision pairs (x s) and back (x s); n complex values, n even
 llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s);
 llr_status from_h16_device(llr_ctx* ctx, const void* src, float2* dst, size_t n, float s);
@@ -110,6 +114,11 @@ llr_status advance_counter(hipStream_t s, unsigned* c);
 llr_status reduce_sum(llr_ctx* ctx, const double* partials, size_t nparts, double* out, double* tmp, size_t tmp_n);
 llr_status hard_project_partials(llr_ctx* ctx, float2* x, const float2* y, size_t n, double* partials, size_t nparts);
 llr_status argmax_volumes(llr_ctx* ctx, const float2* y, size_t N, size_t nvols, unsigned long long* best);
+// energy partials (sum |y|^2, nonzero count; 2 per part) and per-volume arg-max
+// keys in one pass; energy_argmax_parts(N, nvols) parts; best zeroed by the caller
+size_t energy_argmax_parts(size_t N, size_t nvols);
+llr_status energy_argmax_device(llr_ctx* ctx, const float2* y, size_t N, size_t nvols, double* partials,
+                                unsigned long long* best);
 llr_status under_sample_device(llr_ctx* ctx, float2* data, const size_t* dims, size_t nq, const float2* mask);
 llr_status circshift_device(llr_ctx* ctx, const void* src, void* dst, size_t eb, const size_t* dims, const long long* sh);
 llr_status phase_shift3_device(llr_ctx* ctx, float2* x, const size_t* dims, int sign);

@@ -7,6 +7,8 @@
 // repeated runs give identical sums.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 
 #include "common.h"
@@ -104,6 +106,51 @@ __global__ void __launch_bounds__(RB) k_hard_project(float2* __restrict__ x, con
   if (threadIdx.x == 0) partials[blockIdx.x] = e;
 }
 
+// One pass over the measurements for the set-up: per block, the sum of |y|^2
+// and the count of nonzero samples (partials[2b], partials[2b + 1], b =
+// volume * gridDim.x + blockIdx.x) and the per-volume arg-max key of k_argmax.
+// Four independent loads per thread and step keep enough bytes in flight.
+__global__ void __launch_bounds__(RB) k_energy_argmax(const float2* __restrict__ y, long long N, double* partials,
+                                                      unsigned long long* best) {
+  __shared__ double sh[RB / 64];
+  const long long v = blockIdx.y;
+  const float2* p = y + v * N;
+  const long long step = (long long)gridDim.x * RB;
+  double e = 0.0, c = 0.0;
+  unsigned long long b = 0;
+  for (long long i0 = blockIdx.x * (long long)RB + threadIdx.x; i0 < N; i0 += 4 * step) {
+    float2 s[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0 + u * step;
+      s[u] = i < N ? p[i] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long long i = i0 + u * step;
+      const float en = s[u].x * s[u].x + s[u].y * s[u].y;
+      e += (double)en;
+      c += (s[u].x != 0.f || s[u].y != 0.f) ? 1.0 : 0.0;
+      const unsigned long long key =
+          ((unsigned long long)__float_as_uint(en) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)i);
+      if (i < N && key > b) b = key;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_down(b, off, 64);
+    b = o > b ? o : b;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(best + v, b);
+  e = block_sum(e, sh);
+  c = block_sum(c, sh);
+  if (threadIdx.x == 0) {
+    const long long pb = v * gridDim.x + blockIdx.x;
+    partials[2 * pb] = e;
+    partials[2 * pb + 1] = c;
+  }
+}
+
 // arg-max of |y|^2 per volume: pack (value bits, ~index) so that max picks the
 // largest value and, among equal values, the lowest index.
 __global__ void __launch_bounds__(RB) k_argmax(const float2* __restrict__ y, long long N, unsigned long long* best) {
@@ -130,6 +177,28 @@ __global__ void k_shrink_all(float2* x, long long n, float alpha) {
     float m = sqrtf(v.x * v.x + v.y * v.y);
     float s = (m == 0.f) ? 0.f : fmaxf(m - alpha, 0.f) / m;
     x[i] = make_float2(v.x * s, v.y * s);
+  }
+}
+
+// Signal-model ramp exp(+i 2 pi frac((i - n/2)(c + n/2)/n)) of every volume
+// along one axis, with the integer k-space correction c found by the arg-max
+// (argmax_volumes packs ~index in the low 32 bits): c = s_axis - (n + 1)/2.
+// (i - n/2)(c + n/2)/n = (2i - n)(2c + n) / (4n) is reduced exactly in integers
+// and the angle taken with FP64 sincospi, so every entry is the FP32 rounding
+// of the exact value (as the long-double host tables).
+__global__ void __launch_bounds__(256) k_ramp_argmax(float2* __restrict__ tab, int n, long long stride, int nax,
+                                                     const unsigned long long* __restrict__ best, int nvols) {
+  const int v = blockIdx.y;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const unsigned long long idx = 0xFFFFFFFFull - (best[v] & 0xFFFFFFFFull);
+    const long long sa = (long long)((idx / (unsigned long long)stride) % (unsigned long long)nax);
+    const long long c = sa - ((long long)nax + 1) / 2;
+    const long long M = 4LL * n;
+    long long m = ((2LL * i - n) * (2 * c + n)) % M;
+    if (m < 0) m += M;
+    double sn, cs;
+    sincospi(2.0 * (double)m / (double)M, &sn, &cs);
+    tab[(long long)v * n + i] = make_float2((float)cs, (float)sn);
   }
 }
 
@@ -251,6 +320,16 @@ __global__ void k_transpose(const float2* __restrict__ in, float2* __restrict__ 
 
 namespace llr {
 
+llr_status ramp_from_argmax_device(llr_ctx* ctx, float2* tab, const size_t* dims, int axis, size_t nvols,
+                                   const unsigned long long* best) {
+  const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)dims[0] : (long long)(dims[0] * dims[1]);
+  const int n = (int)dims[axis];
+  hipLaunchKernelGGL(k_ramp_argmax, dim3((unsigned)((n + 255) / 256), (unsigned)nvols), dim3(256), 0, ctx->stream,
+                     tab, n, stride, n, best, (int)nvols);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
 llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s) {
   if (n % 2) return fail(LLR_ERR_UNSUPPORTED, "half-precision conversion needs an even number of values");
   hipLaunchKernelGGL(k_to_h16, dim3(grid_for(n / 2, 256, 65536u)), dim3(256), 0, ctx->stream, (const float4*)src,
@@ -341,6 +420,21 @@ llr_status hard_project_partials(llr_ctx* ctx, float2* x, const float2* y, size_
 }
 
 // best[v] must be zeroed; receives (value bits << 32 | ~index) per volume
+size_t energy_argmax_parts(size_t N, size_t nvols) {
+  size_t gx = std::max<size_t>(1, std::min<size_t>((N + 4 * RB - 1) / (4 * RB), 2048 / std::max<size_t>(nvols, 1)));
+  return gx * nvols;
+}
+
+llr_status energy_argmax_device(llr_ctx* ctx, const float2* y, size_t N, size_t nvols, double* partials,
+                                unsigned long long* best) {
+  if (N > 0xFFFFFFFFull) return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
+  const unsigned gx = (unsigned)(energy_argmax_parts(N, nvols) / nvols);
+  hipLaunchKernelGGL(k_energy_argmax, dim3(gx, (unsigned)nvols), dim3(RB), 0, ctx->stream, y, (long long)N, partials,
+                     best);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
 llr_status argmax_volumes(llr_ctx* ctx, const float2* y, size_t N, size_t nvols, unsigned long long* best) {
   if (N > 0xFFFFFFFFull) return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
   unsigned gx = grid_for(N, RB, 1024);

@@ -198,12 +198,21 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
   for (int mt = 0; mt < CV_MT; ++mt)
 #pragma unroll
     for (int t = 0; t < NTM; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // every load of a chunk is issued before the first LDS store (the loads
+  // are independent; issued one at a time their latencies would add up)
+  constexpr int NRL = (9 * CV_RUN * 2 + 255) / 256;
+  constexpr int NWL = (27 * 16 * NTM * 2 + 255) / 256;
   for (int c0 = 0; c0 < a.cin; c0 += 16) {
     const bool from_up = a.up != nullptr && c0 < a.cup;
-    for (int e = tid; e < 9 * CV_RUN * 2; e += 256) {
+    uint4 ur[NRL];
+#pragma unroll
+    for (int q = 0; q < NRL; ++q) {
+      const int e = tid + 256 * q;
+      ur[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (e >= 9 * CV_RUN * 2) continue;
       const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
       const long long idx = vb - 1 + i + (long long)(r / 3 - 1) * HW + (long long)(r % 3 - 1) * a.W;
-      uint4 u = make_uint4(0u, 0u, 0u, 0u);
+      uint4& u = ur[q];
       if (idx >= 0 && idx < V) {
         if constexpr (sizeof(TIn) == 2) {
           if (from_up) {
@@ -214,27 +223,44 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
             u = *(const uint4*)((const bf16_t*)a.in + idx * cin2 + (c0 - a.cup) + 8 * half);
           }
         } else {
-          const float* q = (const float*)a.in + idx * a.cin + c0 + 8 * half;
-          const float4 f0 = *(const float4*)q, f1 = *(const float4*)(q + 4);
+          const float* fp = (const float*)a.in + idx * a.cin + c0 + 8 * half;
+          const float4 f0 = *(const float4*)fp, f1 = *(const float4*)(fp + 4);
           u.x = (unsigned)f2bf(f0.x) | ((unsigned)f2bf(f0.y) << 16);
           u.y = (unsigned)f2bf(f0.z) | ((unsigned)f2bf(f0.w) << 16);
           u.z = (unsigned)f2bf(f1.x) | ((unsigned)f2bf(f1.y) << 16);
           u.w = (unsigned)f2bf(f1.z) | ((unsigned)f2bf(f1.w) << 16);
         }
       }
-      uint2* d = (uint2*)(s_run + (size_t)(r * CV_RUN + i) * CV_ROW + 8 * half);
-      d[0] = make_uint2(u.x, u.y);
-      d[1] = make_uint2(u.z, u.w);
     }
     // this chunk's weights: s_w[o][co - cb][0..16) = w[o'][co][c0..c0+16), o' = o or 26 - o
     const int ncw = 16 * nt;
-    for (int e = tid; e < 27 * ncw * 2; e += 256) {
+    uint4 uw[NWL];
+#pragma unroll
+    for (int q = 0; q < NWL; ++q) {
+      const int e = tid + 256 * q;
+      uw[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (e >= 27 * ncw * 2) continue;
       const int half = e & 1, col = (e >> 1) % ncw, o = (e >> 1) / ncw;
       const int ow = a.mirror ? 26 - o : o;
-      const uint4 u = *(const uint4*)(a.w + ((size_t)ow * a.cout + cb + col) * a.cin + c0 + 8 * half);
+      uw[q] = *(const uint4*)(a.w + ((size_t)ow * a.cout + cb + col) * a.cin + c0 + 8 * half);
+    }
+#pragma unroll
+    for (int q = 0; q < NRL; ++q) {
+      const int e = tid + 256 * q;
+      if (e >= 9 * CV_RUN * 2) continue;
+      const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
+      uint2* d = (uint2*)(s_run + (size_t)(r * CV_RUN + i) * CV_ROW + 8 * half);
+      d[0] = make_uint2(ur[q].x, ur[q].y);
+      d[1] = make_uint2(ur[q].z, ur[q].w);
+    }
+#pragma unroll
+    for (int q = 0; q < NWL; ++q) {
+      const int e = tid + 256 * q;
+      if (e >= 27 * ncw * 2) continue;
+      const int half = e & 1, col = (e >> 1) % ncw, o = (e >> 1) / ncw;
       uint2* d = (uint2*)(s_w + (size_t)(o * 16 * NTM + col) * CV_ROW + 8 * half);
-      d[0] = make_uint2(u.x, u.y);
-      d[1] = make_uint2(u.z, u.w);
+      d[0] = make_uint2(uw[q].x, uw[q].y);
+      d[1] = make_uint2(uw[q].z, uw[q].w);
     }
     __syncthreads();
 #pragma unroll 3

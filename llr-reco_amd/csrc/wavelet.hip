@@ -274,77 +274,120 @@ struct ZXArgs {
 
 constexpr int ZX_KM = 8;  // coefficient planes held per thread (K <= 8)
 
-// grid.x = tiles_x * by (one tile of ZT x-neighbour z-lines at one Y); NZR >= ceil(az / 16)
-template <int NZR, class ST>
-__global__ void __launch_bounds__(256) k_wav_z_expand(ZXArgs g) {
+// grid.x = tiles_x * by: one tile of ZT x-neighbour z-lines at one Y, ZT * NCH
+// threads. Thread (t = line, r = chunk) owns the CH consecutive samples z in
+// [CH r, CH r + CH) of its line (CH even, NCH CH >= az, az a multiple of CH)
+// and the coefficients c with 2c in that range (the last chunk also owns
+// c = az/2): analysis needs the two samples before the chunk and synthesis the
+// first coefficient after it, which come from the neighbouring chunks through
+// LDS; everything else stays in registers.
+template <int CH, int NCH, class ST>
+__global__ void __launch_bounds__(ZT * NCH) k_wav_z_expand(ZXArgs g) {
   using E = typename ST::T;
   const ZArgs& a = g.z;
-  extern __shared__ float2 zs[];
-  float2* L = zs;               // az x ZT
-  float2* Cf = zs + a.az * ZT;  // 2ncz x ZT
+  constexpr int NC = CH / 2 + 1;                 // coefficient slots per thread (the last one: c = az/2)
+  __shared__ float2 h_x[NCH][2][ZT];             // last two samples of each chunk
+  __shared__ float2 h_c[NCH][2][ZT];             // first (lo, hi) coefficient of each chunk
   const int tid = threadIdx.x;
   const int t = tid % ZT, r = tid / ZT;
   const int Y = blockIdx.x / a.tiles_x;
   const int X = (blockIdx.x % a.tiles_x) * ZT + t;
   const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx;
   const long long zp = a.bx * a.by;
-  float2 kv[ZX_KM][NZR];
+  const int z0 = CH * r;
+  const int nch = a.az / CH;                     // chunks holding samples (az is a multiple of CH)
+  const bool live = r < nch;
+  const bool last = r == nch - 1;
+  float2 kv[ZX_KM][CH];
 #pragma unroll
   for (int k = 0; k < ZX_KM; ++k)
 #pragma unroll
-    for (int q = 0; q < NZR; ++q) {
-      const int z = r + (256 / ZT) * q;
-      kv[k][q] = (k < g.K && z < a.az && act) ? g.BK[k * g.bkvp + (long long)Y * a.bx + X + z * zp]
-                                              : make_float2(0.f, 0.f);
-    }
+    for (int i = 0; i < CH; ++i)
+      kv[k][i] = (k < g.K && live && act) ? g.BK[k * g.bkvp + (long long)Y * a.bx + X + (long long)(z0 + i) * zp]
+                                          : make_float2(0.f, 0.f);
   const bool ll = X < a.ncx && Y < a.ncy;
   for (int v = 0; v < g.nq; ++v) {
     float2 sc[ZX_KM];
 #pragma unroll
     for (int k = 0; k < ZX_KM; ++k) sc[k] = k < g.K ? g.S[v + (long long)k * g.nq] : make_float2(0.f, 0.f);
+    // 1. this volume's samples of the chunk: x = sum_k C_k conj(S[v][k])
+    float2 x[CH];
 #pragma unroll
-    for (int q = 0; q < NZR; ++q) {
-      const int z = r + (256 / ZT) * q;
-      if (z < a.az) {
-        float2 x = make_float2(0.f, 0.f);
+    for (int i = 0; i < CH; ++i) {
+      float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-        for (int k = 0; k < ZX_KM; ++k) x = cadd(x, cmulc(kv[k][q], sc[k]));
-        L[z * ZT + t] = x;
-      }
+      for (int k = 0; k < ZX_KM; ++k) acc = cadd(acc, cmulc(kv[k][i], sc[k]));
+      x[i] = acc;
     }
-    __syncthreads();  // L ready; the previous volume's synthesis is done with Cf
-    E* row = (E*)a.B + v * a.bvp + (long long)Y * a.bx;
-    for (int c = r; c < a.ncz; c += 256 / ZT) {
+    h_x[r][0][t] = x[CH - 2];
+    h_x[r][1][t] = x[CH - 1];
+    __syncthreads();  // halos of samples ready (and the previous volume's coefficient halos consumed)
+    // samples z0 - 2 .. z0 + CH + 1 (symmetric extension at both ends)
+    float2 w[CH + 4];
+    if (r == 0) {
+      w[0] = x[1];
+      w[1] = x[0];
+    } else {
+      w[0] = h_x[r - 1][0][t];
+      w[1] = h_x[r - 1][1][t];
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) w[2 + i] = x[i];
+    w[CH + 2] = x[CH - 1];  // used only by the last chunk: sym(az) = az - 1, sym(az + 1) = az - 2
+    w[CH + 3] = x[CH - 2];
+    // 2. analysis of c = z0/2 + m (m < CH/2), and c = az/2 for the last chunk:
+    //    a[c] = sum_k dlo[k] x[sym(2c + 1 - k)], local sample index 2m + 3 - k
+    float2 ca[NC], cd[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
       float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        float2 x = L[sym(2 * c + 1 - k, a.az) * ZT + t];
-        fma2(sa, a.f.dlo[k], x);
-        fma2(sd, a.f.dhi[k], x);
+        const float2 xv = w[2 * m + 3 - k];
+        fma2(sa, a.f.dlo[k], xv);
+        fma2(sd, a.f.dhi[k], xv);
       }
+      const int c = z0 / 2 + m;
       sd = shrink(sd, a.alpha);
       if (!ll || a.deepest) {
         sa = shrink(sa, a.alpha);
       } else {
-        ST::st((E*)a.A + v * a.avp + (long long)c * a.pa * a.ncy + (long long)Y * a.pa + X, sa);
+        if (live && (m < NC - 1 || last) && act)
+          ST::st((E*)a.A + v * a.avp + (long long)c * a.pa * a.ncy + (long long)Y * a.pa + X, sa);
         sa = make_float2(0.f, 0.f);
       }
-      Cf[c * ZT + t] = sa;
-      Cf[(a.ncz + c) * ZT + t] = sd;
+      ca[m] = sa;
+      cd[m] = sd;
     }
-    __syncthreads();  // Cf ready; L is free for the next volume
-    for (int s = r; s < a.az; s += 256 / ZT) {
-      float2 acc = make_float2(0.f, 0.f);
+    h_c[r][0][t] = ca[0];
+    h_c[r][1][t] = cd[0];
+    __syncthreads();  // coefficient halos ready; sample halos consumed
+    // 3. synthesis of the chunk's samples: out[s] uses coefficients s/2 and s/2 + 1
+    //    (the latter from the next chunk for the chunk's last two samples)
+    if (live && !last) {  // r + 1 < nch <= NCH
+      ca[CH / 2] = h_c[r + 1][0][t];
+      cd[CH / 2] = h_c[r + 1][1][t];
+    }
+    if (live && act) {
+      E* row = (E*)a.B + v * a.bvp + (long long)Y * a.bx + X;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int idx = s + 2 - j;
-        if (idx >= 0 && idx < 2 * a.ncz && (idx & 1) == 0) {
-          int tt = idx >> 1;
-          fma2(acc, a.f.rlo[j], Cf[tt * ZT + t]);
-          fma2(acc, a.f.rhi[j], Cf[(a.ncz + tt) * ZT + t]);
+      for (int i = 0; i < CH; ++i) {
+        // s = z0 + i; even s: taps j = 2 (c = s/2) and j = 0 (c = s/2 + 1); odd s: j = 3 and j = 1
+        const int m = i / 2;
+        float2 acc = make_float2(0.f, 0.f);
+        if ((i & 1) == 0) {
+          fma2(acc, a.f.rlo[2], ca[m]);
+          fma2(acc, a.f.rhi[2], cd[m]);
+          fma2(acc, a.f.rlo[0], ca[m + 1]);
+          fma2(acc, a.f.rhi[0], cd[m + 1]);
+        } else {
+          fma2(acc, a.f.rlo[3], ca[m]);
+          fma2(acc, a.f.rhi[3], cd[m]);
+          fma2(acc, a.f.rlo[1], ca[m + 1]);
+          fma2(acc, a.f.rhi[1], cd[m + 1]);
         }
+        ST::st(row + (long long)(z0 + i) * zp, acc);
       }
-      if (act) ST::st(row + X + s * zp, acc);
     }
   }
 }
@@ -476,6 +519,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
     __syncthreads();  // band and yrow are rewritten by the next tile
   }
 }
+
 
 }  // namespace
 
@@ -612,7 +656,7 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
     if (smem > 160 * 1024 || in[2] > 512) return fail(LLR_ERR_UNSUPPORTED, "z length too large for the wavelet z pass");
     int nzr = (int)((in[2] + (256 / ZT) - 1) / (256 / ZT));
     if (fused) {
-      if (nzr > 8 || sub->K > ZX_KM) return fail(LLR_ERR_UNSUPPORTED, "fused wavelet: z length <= 128, rank <= 8");
+      if (sub->K > ZX_KM) return fail(LLR_ERR_UNSUPPORTED, "fused wavelet: rank <= 8");
       ZXArgs ax{};
       ax.BK = sub->BK;
       ax.bkvp = (long long)bvol[0];
@@ -621,12 +665,18 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
       ax.nq = (int)nvols;
       ax.z = a2;
       dim3 gx((unsigned)(a2.tiles_x * by));
-#define ZX_CASE(NN)                                                                  \
-  if (nzr <= NN) {                                                                   \
-    hipLaunchKernelGGL((k_wav_z_expand<NN, ST>), gx, dim3(256), smem, s, ax);        \
-  } else
-      ZX_CASE(1) ZX_CASE(2) ZX_CASE(4) ZX_CASE(8) {}
-#undef ZX_CASE
+      // chunks of 2 (az <= 32 with 16 chunks) or 4 samples (16 or 32 chunks)
+      const int nch = in[2] <= 64 ? 16 : 32;
+      const int ch = 2 * (int)((in[2] + 2 * nch - 1) / (2 * nch));
+      if (in[2] % 2 || in[2] % ch || ch > 4) return fail(LLR_ERR_UNSUPPORTED, "fused wavelet: z length");
+      if (nch == 16 && ch == 2)
+        hipLaunchKernelGGL((k_wav_z_expand<2, 16, ST>), gx, dim3(ZT * 16), 0, s, ax);
+      else if (nch == 16)
+        hipLaunchKernelGGL((k_wav_z_expand<4, 16, ST>), gx, dim3(ZT * 16), 0, s, ax);
+      else if (ch == 2)
+        hipLaunchKernelGGL((k_wav_z_expand<2, 32, ST>), gx, dim3(ZT * 32), 0, s, ax);
+      else
+        hipLaunchKernelGGL((k_wav_z_expand<4, 32, ST>), gx, dim3(ZT * 32), 0, s, ax);
       LLR_HIP(hipGetLastError());
       continue;
     }
@@ -693,7 +743,8 @@ size_t wavelet_subspace_elems(const size_t* dims, int L, int K) {
 }
 
 bool wavelet_subspace_supported(const size_t* dims, int L, int K) {
-  return L >= 1 && K >= 1 && K <= ZX_KM && dims[2] <= 128;
+  const size_t nz = dims[2], nch = nz <= 64 ? 16 : 32, ch = 2 * ((nz + 2 * nch - 1) / (2 * nch));
+  return L >= 1 && K >= 1 && K <= ZX_KM && nz >= 4 && nz <= 128 && nz % 2 == 0 && nz % ch == 0;
 }
 
 llr_status wavelet_denoise_subspace(llr_ctx* ctx, void* vols, size_t nvols, const size_t* dims, float alpha, int L,
