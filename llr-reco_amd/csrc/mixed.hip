@@ -31,12 +31,8 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// FP32 -> BF16 bit pattern, round to nearest even (finite inputs)
-__device__ __forceinline__ short bf16_bits(float f) {
-  unsigned u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
+// FP32 -> BF16 bit pattern, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ short bf16_bits(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
 // D = A*B + C for 16x16 tiles, K = 16. Lane l supplies A[l%16][4(l/16)+j] and
 // B[4(l/16)+j][l%16] (j = 0..3) and receives D[4(l/16)+j][l%16].

@@ -59,10 +59,13 @@ __device__ __forceinline__ int sym(int i, int n) {
   return i;
 }
 
+// x * max(|x| - alpha, 0) / |x| = x * max(1 - alpha / |x|, 0), with 1/|x| from
+// the hardware reciprocal square root (about 1 ulp); |x| = 0 gives rsq = inf,
+// 1 - inf -> factor 0, so zero maps to zero (shrink_complex, lib.rs:460-465,
+// returns NaN there).
 __device__ __forceinline__ float2 shrink(float2 x, float alpha) {
-  float m = sqrtf(x.x * x.x + x.y * x.y);
-  if (m == 0.f) return make_float2(0.f, 0.f);
-  float s = fmaxf(m - alpha, 0.f) / m;
+  const float m2 = x.x * x.x + x.y * x.y;
+  const float s = fmaxf(1.f - alpha * __builtin_amdgcn_rsqf(m2), 0.f);
   return make_float2(x.x * s, x.y * s);
 }
 
@@ -281,7 +284,7 @@ constexpr int ZX_KM = 8;  // coefficient planes held per thread (K <= 8)
 // c = az/2): analysis needs the two samples before the chunk and synthesis the
 // first coefficient after it, which come from the neighbouring chunks through
 // LDS; everything else stays in registers.
-template <int CH, int NCH, class ST>
+template <int CH, int NCH, int KK, class ST>
 __global__ void __launch_bounds__(ZT * NCH) k_wav_z_expand(ZXArgs g) {
   using E = typename ST::T;
   const ZArgs& a = g.z;
@@ -298,25 +301,26 @@ __global__ void __launch_bounds__(ZT * NCH) k_wav_z_expand(ZXArgs g) {
   const int nch = a.az / CH;                     // chunks holding samples (az is a multiple of CH)
   const bool live = r < nch;
   const bool last = r == nch - 1;
-  float2 kv[ZX_KM][CH];
+  // KK planes are held (KK = K, or 8 with planes K..7 zero)
+  float2 kv[KK][CH];
 #pragma unroll
-  for (int k = 0; k < ZX_KM; ++k)
+  for (int k = 0; k < KK; ++k)
 #pragma unroll
     for (int i = 0; i < CH; ++i)
       kv[k][i] = (k < g.K && live && act) ? g.BK[k * g.bkvp + (long long)Y * a.bx + X + (long long)(z0 + i) * zp]
                                           : make_float2(0.f, 0.f);
   const bool ll = X < a.ncx && Y < a.ncy;
   for (int v = 0; v < g.nq; ++v) {
-    float2 sc[ZX_KM];
+    float2 sc[KK];
 #pragma unroll
-    for (int k = 0; k < ZX_KM; ++k) sc[k] = k < g.K ? g.S[v + (long long)k * g.nq] : make_float2(0.f, 0.f);
+    for (int k = 0; k < KK; ++k) sc[k] = k < g.K ? g.S[v + (long long)k * g.nq] : make_float2(0.f, 0.f);
     // 1. this volume's samples of the chunk: x = sum_k C_k conj(S[v][k])
     float2 x[CH];
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int k = 0; k < ZX_KM; ++k) acc = cadd(acc, cmulc(kv[k][i], sc[k]));
+      for (int k = 0; k < KK; ++k) acc = cadd(acc, cmulc(kv[k][i], sc[k]));
       x[i] = acc;
     }
     h_x[r][0][t] = x[CH - 2];
@@ -338,8 +342,11 @@ __global__ void __launch_bounds__(ZT * NCH) k_wav_z_expand(ZXArgs g) {
     // 2. analysis of c = z0/2 + m (m < CH/2), and c = az/2 for the last chunk:
     //    a[c] = sum_k dlo[k] x[sym(2c + 1 - k)], local sample index 2m + 3 - k
     float2 ca[NC], cd[NC];
+    ca[NC - 1] = cd[NC - 1] = make_float2(0.f, 0.f);
+    // slot NC - 1 (c = az/2) only for the last chunk; the others take it from the next chunk
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
+      if (m == NC - 1 && !last) break;
       float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -407,7 +414,15 @@ struct XYSynArgs {
 };
 
 constexpr int SY_NLD = (4 * SY_NY * SY_NX + 255) / 256;
+constexpr int SY_NYS = (2 * SY_TY * SY_NX + 255) / 256;  // y-step outputs per thread
+constexpr int SY_NXS = SY_TY * SY_TX / 256;                // x-step outputs per thread
 
+// Every index a thread touches is fixed by its thread id except for the tile's
+// x origin, so the decompositions are done once per workgroup. Coefficients
+// that do not exist (past ncx / ncy) are staged as zeros, so the synthesis
+// reads the two taps of an output without range checks: output s uses
+// coefficients floor(s/2) (tap 2 if s is even, 3 if odd) and floor(s/2) + 1
+// (tap 0 / 1), tile origins being even.
 template <class ST>
 __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   using E = typename ST::T;
@@ -439,30 +454,57 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
       }
     }
   }
+  const int pza = za * a.pa * a.ncy, pzb = zb * a.pa * a.ncy;
+  // load slots: band element e = tid + 256 q -> row offset in the slice (or -1) and column
+  int lofs[SY_NLD], lix[SY_NLD], aofs[SY_NLD];
+#pragma unroll
+  for (int q = 0; q < SY_NLD; ++q) {
+    const int e = tid + 256 * q;
+    lofs[q] = -1;
+    aofs[q] = -1;
+    lix[q] = 0;
+    if (e < 4 * SY_NY * SY_NX) {
+      const int ix = e % SY_NX, rem = e / SY_NX;
+      const int iy = rem % SY_NY, b = rem / SY_NY;
+      const int cy = ty0 + iy;
+      lix[q] = ix;
+      if (cy < a.ncy) {
+        lofs[q] = ((b >> 1) ? a.ncy + cy : cy) * (int)a.bx + ((b & 1) ? a.px : 0);
+        if (b == 0 && ap) aofs[q] = cy * a.pa;
+      }
+    }
+  }
+  // y step: output e -> (xb, sy, ix); its two taps read band rows t_a, t_a + 1
+  // with the weights of the output's parity (bit q of yodd)
+  int yo[SY_NYS];
+  unsigned yodd = 0;
+#pragma unroll
+  for (int q = 0; q < SY_NYS; ++q) {
+    const int e = min(tid + 256 * q, 2 * SY_TY * SY_NX - 1);
+    const int ix = e % SY_NX, rem = e / SY_NX;
+    const int sy = rem % SY_TY, xb = rem / SY_TY;
+    yo[q] = (xb * SY_NY + (sy >> 1)) * SY_NX + ix;
+    yodd |= (unsigned)(sy & 1) << q;
+  }
+  constexpr int HB = 2 * SY_NY * SY_NX;  // lo band rows -> hi band rows
+  // x step: output e = tid + 256 q -> (sy = tid/64 + 4q, sx = tid % 64)
+  const int sx = tid % SY_TX, sy0 = tid / SY_TX, xodd = sx & 1;
+  const float xl0 = a.f.rlo[xodd ? 3 : 2], xl1 = a.f.rlo[xodd ? 1 : 0];
+  const float xh0 = a.f.rhi[xodd ? 3 : 2], xh1 = a.f.rhi[xodd ? 1 : 0];
   const int t0 = gx * TPW;
   const int nt = min(TPW, a.tiles_x - t0);
-  float2 bn[SY_NLD], an[SY_NLD], cn[SY_NLD];
+  float2 bn[SY_NLD];
   auto load = [&](int x0) {
     const int tx0 = x0 / 2;
 #pragma unroll
     for (int q = 0; q < SY_NLD; ++q) {
-      int e = tid + 256 * q;
+      const int cx = tx0 + lix[q];
       bn[q] = make_float2(0.f, 0.f);
-      an[q] = make_float2(0.f, 0.f);
-      cn[q] = make_float2(0.f, 0.f);
-      if (e < 4 * SY_NY * SY_NX) {
-        int ix = e % SY_NX, rem = e / SY_NX;
-        int iy = rem % SY_NY, b = rem / SY_NY;
-        int cx = tx0 + ix, cy = ty0 + iy;
-        if (cx < a.ncx && cy < a.ncy) {
-          long long gxo = (b & 1) ? a.px + cx : cx;
-          long long gyo = (b >> 1) ? a.ncy + cy : cy;
-          bn[q] = ST::ld(src + gyo * a.bx + gxo);
-          if (b == 0 && ap) {
-            long long o = (long long)cy * a.pa + cx;
-            if (za >= 0) an[q] = ST::ld(ap + (long long)za * a.pa * a.ncy + o);
-            if (zb >= 0) cn[q] = ST::ld(ap + (long long)zb * a.pa * a.ncy + o);
-          }
+      if (lofs[q] >= 0 && cx < a.ncx) {
+        bn[q] = ST::ld(src + lofs[q] + cx);
+        if (aofs[q] >= 0) {
+          if (za >= 0) fma2(bn[q], wa, ST::ld(ap + pza + aofs[q] + cx));
+          if (zb >= 0) fma2(bn[q], wb, ST::ld(ap + pzb + aofs[q] + cx));
         }
       }
     }
@@ -470,56 +512,47 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   load(t0 * SY_TX);
   E* dst = (E*)a.out + v * a.ovp + (long long)z * a.op2;
   for (int it = 0; it < nt; ++it) {
-    const int x0 = (t0 + it) * SY_TX, tx0 = x0 / 2;
+    const int x0 = (t0 + it) * SY_TX;
 #pragma unroll
     for (int q = 0; q < SY_NLD; ++q) {
-      int e = tid + 256 * q;
-      if (e < 4 * SY_NY * SY_NX) {
-        float2 val = bn[q];
-        fma2(val, wa, an[q]);
-        fma2(val, wb, cn[q]);
-        band[e] = val;
-      }
+      const int e = tid + 256 * q;
+      if (e < 4 * SY_NY * SY_NX) band[e] = bn[q];
     }
     __syncthreads();
     if (it + 1 < nt) load(x0 + SY_TX);
-    // along y: yrow[xb][sy][ix] from the (xb, yb = 0 / 1) bands
-    for (int e = tid; e < 2 * SY_TY * SY_NX; e += 256) {
-      int ix = e % SY_NX, rem = e / SY_NX;
-      int sy = rem % SY_TY, xb = rem / SY_TY;
-      float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int idx = y0 + sy + 2 - j;
-        if (idx >= 0 && idx < 2 * a.ncy && (idx & 1) == 0) {
-          int tt = (idx >> 1) - ty0;
-          fma2(acc, a.f.rlo[j], band[(xb * SY_NY + tt) * SY_NX + ix]);
-          fma2(acc, a.f.rhi[j], band[((xb + 2) * SY_NY + tt) * SY_NX + ix]);
-        }
+    for (int q = 0; q < SY_NYS; ++q) {
+      const int e = tid + 256 * q;
+      if (e < 2 * SY_TY * SY_NX) {
+        const bool odd = (yodd >> q) & 1;
+        const float l0 = odd ? a.f.rlo[3] : a.f.rlo[2], l1 = odd ? a.f.rlo[1] : a.f.rlo[0];
+        const float h0 = odd ? a.f.rhi[3] : a.f.rhi[2], h1 = odd ? a.f.rhi[1] : a.f.rhi[0];
+        const int o = yo[q];
+        float2 acc = make_float2(0.f, 0.f);
+        fma2(acc, l0, band[o]);
+        fma2(acc, h0, band[o + HB]);
+        fma2(acc, l1, band[o + SY_NX]);
+        fma2(acc, h1, band[o + HB + SY_NX]);
+        yrow[e] = acc;
       }
-      yrow[e] = acc;
     }
     __syncthreads();
-    for (int e = tid; e < SY_TY * SY_TX; e += 256) {
-      int sx = e % SY_TX, sy = e / SY_TX;
-      int gxo = x0 + sx, gyo = y0 + sy;
-      if (gxo >= a.ax || gyo >= a.ay) continue;
-      float2 acc = make_float2(0.f, 0.f);
+    const int gxo = x0 + sx;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        int idx = gxo + 2 - j;
-        if (idx >= 0 && idx < 2 * a.ncx && (idx & 1) == 0) {
-          int tt = (idx >> 1) - tx0;
-          fma2(acc, a.f.rlo[j], yrow[sy * SY_NX + tt]);
-          fma2(acc, a.f.rhi[j], yrow[(SY_TY + sy) * SY_NX + tt]);
-        }
-      }
+    for (int q = 0; q < SY_NXS; ++q) {
+      const int sy = sy0 + 4 * q, gyo = y0 + sy;
+      if (gxo >= a.ax || gyo >= a.ay) continue;
+      const int o = sy * SY_NX + (sx >> 1);
+      float2 acc = make_float2(0.f, 0.f);
+      fma2(acc, xl0, yrow[o]);
+      fma2(acc, xh0, yrow[o + SY_TY * SY_NX]);
+      fma2(acc, xl1, yrow[o + 1]);
+      fma2(acc, xh1, yrow[o + 1 + SY_TY * SY_NX]);
       ST::st(dst + (long long)gyo * a.op1 + gxo, acc);
     }
     __syncthreads();  // band and yrow are rewritten by the next tile
   }
 }
-
 
 }  // namespace
 
@@ -669,14 +702,21 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
       const int nch = in[2] <= 64 ? 16 : 32;
       const int ch = 2 * (int)((in[2] + 2 * nch - 1) / (2 * nch));
       if (in[2] % 2 || in[2] % ch || ch > 4) return fail(LLR_ERR_UNSUPPORTED, "fused wavelet: z length");
-      if (nch == 16 && ch == 2)
-        hipLaunchKernelGGL((k_wav_z_expand<2, 16, ST>), gx, dim3(ZT * 16), 0, s, ax);
-      else if (nch == 16)
-        hipLaunchKernelGGL((k_wav_z_expand<4, 16, ST>), gx, dim3(ZT * 16), 0, s, ax);
-      else if (ch == 2)
-        hipLaunchKernelGGL((k_wav_z_expand<2, 32, ST>), gx, dim3(ZT * 32), 0, s, ax);
-      else
-        hipLaunchKernelGGL((k_wav_z_expand<4, 32, ST>), gx, dim3(ZT * 32), 0, s, ax);
+#define ZX_LAUNCH(C, N)                                                                   \
+  if (sub->K == 7)                                                                        \
+    hipLaunchKernelGGL((k_wav_z_expand<C, N, 7, ST>), gx, dim3(ZT * N), 0, s, ax);       \
+  else                                                                                    \
+    hipLaunchKernelGGL((k_wav_z_expand<C, N, ZX_KM, ST>), gx, dim3(ZT * N), 0, s, ax);
+      if (nch == 16 && ch == 2) {
+        ZX_LAUNCH(2, 16)
+      } else if (nch == 16) {
+        ZX_LAUNCH(4, 16)
+      } else if (ch == 2) {
+        ZX_LAUNCH(2, 32)
+      } else {
+        ZX_LAUNCH(4, 32)
+      }
+#undef ZX_LAUNCH
       LLR_HIP(hipGetLastError());
       continue;
     }
