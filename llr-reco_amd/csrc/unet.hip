@@ -175,20 +175,26 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
   __shared__ __align__(16) bf16_t s_w[27 * 16 * NTM * CV_ROW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = lane & 15, kq = lane >> 4;
-  const long long HW = (long long)a.H * a.W, V = HW * a.D;
-  const long long vb = (long long)blockIdx.x * CV_M;
+  // voxel indices are 32-bit (the host checks D*H*W*channels < 2^31)
+  const int HW = a.H * a.W, V = HW * a.D;
+  const int vb = blockIdx.x * CV_M;
   const int cb = blockIdx.y * 16 * NTM;             // this workgroup's output channels [cb, cb + 16 nt)
   const int nt = min(NTM, (a.cout - cb) / 16);
   const int cin2 = a.cin - a.cup;  // channels per voxel of a.in
-  int vx[CV_MT], vy[CV_MT], vz[CV_MT];
-  bool vin[CV_MT];
+  // per m-tile lane voxel: the 27-bit set of offsets o whose neighbour lies inside the volume
+  unsigned vmask[CV_MT];
 #pragma unroll
   for (int mt = 0; mt < CV_MT; ++mt) {
-    const long long v = vb + (wave * CV_MT + mt) * 16 + n;
-    vin[mt] = v < V;
-    vx[mt] = (int)(v % a.W);
-    vy[mt] = (int)((v / a.W) % a.H);
-    vz[mt] = (int)(v / HW);
+    const int v = vb + (wave * CV_MT + mt) * 16 + n;
+    const int x = v % a.W, y = (v / a.W) % a.H, z = v / HW;
+    // bit d + 1 of an axis mask: the step d in {-1, 0, 1} stays inside
+    const unsigned mx = (x > 0 ? 1u : 0u) | 2u | (x < a.W - 1 ? 4u : 0u);
+    const unsigned my = (y > 0 ? 1u : 0u) | 2u | (y < a.H - 1 ? 4u : 0u);
+    const unsigned mz = (z > 0 ? 1u : 0u) | 2u | (z < a.D - 1 ? 4u : 0u);
+    unsigned m = 0;
+#pragma unroll
+    for (int o = 0; o < 27; ++o) m |= ((mz >> (o / 9)) & (my >> ((o / 3) % 3)) & (mx >> (o % 3)) & 1u) << o;
+    vmask[mt] = v < V ? m : 0u;
   }
   f32x4 acc[CV_MT][NTM];
 #pragma unroll
@@ -208,19 +214,19 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
       ur[q] = make_uint4(0u, 0u, 0u, 0u);
       if (e >= 9 * CV_RUN * 2) continue;
       const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
-      const long long idx = vb - 1 + i + (long long)(r / 3 - 1) * HW + (long long)(r % 3 - 1) * a.W;
+      const int idx = vb - 1 + i + (r / 3 - 1) * HW + (r % 3 - 1) * a.W;
       uint4& u = ur[q];
       if (idx >= 0 && idx < V) {
         if constexpr (sizeof(TIn) == 2) {
           if (from_up) {
-            const int x = (int)(idx % a.W), y = (int)((idx / a.W) % a.H), z = (int)(idx / HW);
-            const long long ci = ((long long)(z >> 1) * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
-            u = *(const uint4*)(a.up + ci * a.cup + c0 + 8 * half);
+            const int x = idx % a.W, y = (idx / a.W) % a.H, z = idx / HW;
+            const int ci = ((z >> 1) * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
+            u = *(const uint4*)(a.up + (size_t)ci * a.cup + c0 + 8 * half);
           } else {
-            u = *(const uint4*)((const bf16_t*)a.in + idx * cin2 + (c0 - a.cup) + 8 * half);
+            u = *(const uint4*)((const bf16_t*)a.in + (size_t)idx * cin2 + (c0 - a.cup) + 8 * half);
           }
         } else {
-          const float* fp = (const float*)a.in + idx * a.cin + c0 + 8 * half;
+          const float* fp = (const float*)a.in + (size_t)idx * a.cin + c0 + 8 * half;
           const float4 f0 = *(const float4*)fp, f1 = *(const float4*)(fp + 4);
           u.x = (unsigned)f2bf(f0.x) | ((unsigned)f2bf(f0.y) << 16);
           u.y = (unsigned)f2bf(f0.z) | ((unsigned)f2bf(f0.w) << 16);
@@ -262,12 +268,11 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
     __syncthreads();
 #pragma unroll 3
     for (int o = 0; o < 27; ++o) {
-      const int dz = o / 9 - 1, dy = (o / 3) % 3 - 1, dx = o % 3 - 1, r = o / 3;
+      const int dx = o % 3 - 1, r = o / 3;
       bf16x4 av[CV_MT];
 #pragma unroll
       for (int mt = 0; mt < CV_MT; ++mt) {
-        const int xx = vx[mt] + dx, yy = vy[mt] + dy, zz = vz[mt] + dz;
-        const bool valid = vin[mt] && xx >= 0 && xx < a.W && yy >= 0 && yy < a.H && zz >= 0 && zz < a.D;
+        const bool valid = (vmask[mt] >> o) & 1u;
         const int pos = (wave * CV_MT + mt) * 16 + n + dx + 1;
         av[mt] = bf16x4{0, 0, 0, 0};
         if (valid) av[mt] = lds4(s_run + (size_t)(r * CV_RUN + pos) * CV_ROW + 4 * kq);
@@ -769,6 +774,8 @@ static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int 
   a.accumulate = accumulate ? 1 : 0;
   if (a.cout > 16 * MAX_CO_TILES || a.cin % 16 || a.cout % 16) return fail(LLR_ERR_UNSUPPORTED, "U-Net channel count");
   const long long V = (long long)a.D * a.H * a.W;
+  if ((double)V * std::max(a.cin, a.cout) >= 2147483647.0)
+    return fail(LLR_ERR_UNSUPPORTED, "U-Net tensor larger than 2^31 values");
   if (a.k == 3) {
     // output channels in slices of at most 64 per workgroup (grid.y), so the
     // staged weights fit in LDS beside the input runs

@@ -85,7 +85,7 @@ __device__ __forceinline__ void fma2(float2& acc, float w, float2 v) {
 constexpr int AN_TCX = 32, AN_TCY = 16;              // coefficient tile of pass 1
 constexpr int AN_W = 2 * AN_TCX + 2, AN_H = 2 * AN_TCY + 2;
 constexpr int SY_TX = 64, SY_TY = 16;                // sample tile of pass 3
-constexpr int SY_NX = SY_TX / 2 + 1, SY_NY = SY_TY / 2 + 1;
+constexpr int SY_NY = SY_TY / 2 + 1;
 constexpr int ZT = 16;                               // z-lines per workgroup in pass 2
 
 constexpr int TPW = 4;  // tiles per workgroup (the next tile is prefetched into registers)
@@ -410,24 +410,26 @@ struct XYSynArgs {
   void* out;
   long long op1, op2, ovp;
   int ax, ay, az, tiles_x, groups_x;
+  int tpw;  // x tiles per workgroup
   Db2F f;
 };
 
-constexpr int SY_NLD = (4 * SY_NY * SY_NX + 255) / 256;
-constexpr int SY_NYS = (2 * SY_TY * SY_NX + 255) / 256;  // y-step outputs per thread
-constexpr int SY_NXS = SY_TY * SY_TX / 256;                // x-step outputs per thread
-
-// Every index a thread touches is fixed by its thread id except for the tile's
+// Tiles are TX x SY_TY samples. Every index a thread touches is fixed by its thread id except for the tile's
 // x origin, so the decompositions are done once per workgroup. Coefficients
 // that do not exist (past ncx / ncy) are staged as zeros, so the synthesis
 // reads the two taps of an output without range checks: output s uses
 // coefficients floor(s/2) (tap 2 if s is even, 3 if odd) and floor(s/2) + 1
 // (tap 0 / 1), tile origins being even.
-template <class ST>
+template <class ST, int TX>
 __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   using E = typename ST::T;
-  __shared__ float2 band[4 * SY_NY * SY_NX];   // [xb + 2*yb][iy][ix]
-  __shared__ float2 yrow[2 * SY_TY * SY_NX];   // [xb][sy][ix]
+  constexpr int NX = TX / 2 + 1;                      // staged coefficient columns
+  constexpr int NLD = (4 * SY_NY * NX + 255) / 256;   // band loads per thread
+  constexpr int NYS = (2 * SY_TY * NX + 255) / 256;   // y-step outputs per thread
+  constexpr int NXS = SY_TY * TX / 256;               // x-step outputs per thread
+  constexpr int RP = 256 / TX;                        // output rows per x-step pass
+  __shared__ float2 band[4 * SY_NY * NX];   // [xb + 2*yb][iy][ix]
+  __shared__ float2 yrow[2 * SY_TY * NX];   // [xb][sy][ix]
   const int tid = threadIdx.x;
   const int gx = blockIdx.x % a.groups_x;
   const int y0 = (blockIdx.x / a.groups_x) * SY_TY;
@@ -456,15 +458,15 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   }
   const int pza = za * a.pa * a.ncy, pzb = zb * a.pa * a.ncy;
   // load slots: band element e = tid + 256 q -> row offset in the slice (or -1) and column
-  int lofs[SY_NLD], lix[SY_NLD], aofs[SY_NLD];
+  int lofs[NLD], lix[NLD], aofs[NLD];
 #pragma unroll
-  for (int q = 0; q < SY_NLD; ++q) {
+  for (int q = 0; q < NLD; ++q) {
     const int e = tid + 256 * q;
     lofs[q] = -1;
     aofs[q] = -1;
     lix[q] = 0;
-    if (e < 4 * SY_NY * SY_NX) {
-      const int ix = e % SY_NX, rem = e / SY_NX;
+    if (e < 4 * SY_NY * NX) {
+      const int ix = e % NX, rem = e / NX;
       const int iy = rem % SY_NY, b = rem / SY_NY;
       const int cy = ty0 + iy;
       lix[q] = ix;
@@ -476,28 +478,28 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   }
   // y step: output e -> (xb, sy, ix); its two taps read band rows t_a, t_a + 1
   // with the weights of the output's parity (bit q of yodd)
-  int yo[SY_NYS];
+  int yo[NYS];
   unsigned yodd = 0;
 #pragma unroll
-  for (int q = 0; q < SY_NYS; ++q) {
-    const int e = min(tid + 256 * q, 2 * SY_TY * SY_NX - 1);
-    const int ix = e % SY_NX, rem = e / SY_NX;
+  for (int q = 0; q < NYS; ++q) {
+    const int e = min(tid + 256 * q, 2 * SY_TY * NX - 1);
+    const int ix = e % NX, rem = e / NX;
     const int sy = rem % SY_TY, xb = rem / SY_TY;
-    yo[q] = (xb * SY_NY + (sy >> 1)) * SY_NX + ix;
+    yo[q] = (xb * SY_NY + (sy >> 1)) * NX + ix;
     yodd |= (unsigned)(sy & 1) << q;
   }
-  constexpr int HB = 2 * SY_NY * SY_NX;  // lo band rows -> hi band rows
-  // x step: output e = tid + 256 q -> (sy = tid/64 + 4q, sx = tid % 64)
-  const int sx = tid % SY_TX, sy0 = tid / SY_TX, xodd = sx & 1;
+  constexpr int HB = 2 * SY_NY * NX;  // lo band rows -> hi band rows
+  // x step: output e = tid + 256 q -> (sy = tid/TX + RP q, sx = tid % TX)
+  const int sx = tid % TX, sy0 = tid / TX, xodd = sx & 1;
   const float xl0 = a.f.rlo[xodd ? 3 : 2], xl1 = a.f.rlo[xodd ? 1 : 0];
   const float xh0 = a.f.rhi[xodd ? 3 : 2], xh1 = a.f.rhi[xodd ? 1 : 0];
-  const int t0 = gx * TPW;
-  const int nt = min(TPW, a.tiles_x - t0);
-  float2 bn[SY_NLD];
+  const int t0 = gx * a.tpw;
+  const int nt = min(a.tpw, a.tiles_x - t0);
+  float2 bn[NLD];
   auto load = [&](int x0) {
     const int tx0 = x0 / 2;
 #pragma unroll
-    for (int q = 0; q < SY_NLD; ++q) {
+    for (int q = 0; q < NLD; ++q) {
       const int cx = tx0 + lix[q];
       bn[q] = make_float2(0.f, 0.f);
       if (lofs[q] >= 0 && cx < a.ncx) {
@@ -509,21 +511,21 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
       }
     }
   };
-  load(t0 * SY_TX);
+  load(t0 * TX);
   E* dst = (E*)a.out + v * a.ovp + (long long)z * a.op2;
   for (int it = 0; it < nt; ++it) {
-    const int x0 = (t0 + it) * SY_TX;
+    const int x0 = (t0 + it) * TX;
 #pragma unroll
-    for (int q = 0; q < SY_NLD; ++q) {
+    for (int q = 0; q < NLD; ++q) {
       const int e = tid + 256 * q;
-      if (e < 4 * SY_NY * SY_NX) band[e] = bn[q];
+      if (e < 4 * SY_NY * NX) band[e] = bn[q];
     }
     __syncthreads();
-    if (it + 1 < nt) load(x0 + SY_TX);
+    if (it + 1 < nt) load(x0 + TX);
 #pragma unroll
-    for (int q = 0; q < SY_NYS; ++q) {
+    for (int q = 0; q < NYS; ++q) {
       const int e = tid + 256 * q;
-      if (e < 2 * SY_TY * SY_NX) {
+      if (e < 2 * SY_TY * NX) {
         const bool odd = (yodd >> q) & 1;
         const float l0 = odd ? a.f.rlo[3] : a.f.rlo[2], l1 = odd ? a.f.rlo[1] : a.f.rlo[0];
         const float h0 = odd ? a.f.rhi[3] : a.f.rhi[2], h1 = odd ? a.f.rhi[1] : a.f.rhi[0];
@@ -531,23 +533,23 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
         float2 acc = make_float2(0.f, 0.f);
         fma2(acc, l0, band[o]);
         fma2(acc, h0, band[o + HB]);
-        fma2(acc, l1, band[o + SY_NX]);
-        fma2(acc, h1, band[o + HB + SY_NX]);
+        fma2(acc, l1, band[o + NX]);
+        fma2(acc, h1, band[o + HB + NX]);
         yrow[e] = acc;
       }
     }
     __syncthreads();
     const int gxo = x0 + sx;
 #pragma unroll
-    for (int q = 0; q < SY_NXS; ++q) {
-      const int sy = sy0 + 4 * q, gyo = y0 + sy;
+    for (int q = 0; q < NXS; ++q) {
+      const int sy = sy0 + RP * q, gyo = y0 + sy;
       if (gxo >= a.ax || gyo >= a.ay) continue;
-      const int o = sy * SY_NX + (sx >> 1);
+      const int o = sy * NX + (sx >> 1);
       float2 acc = make_float2(0.f, 0.f);
       fma2(acc, xl0, yrow[o]);
-      fma2(acc, xh0, yrow[o + SY_TY * SY_NX]);
+      fma2(acc, xh0, yrow[o + SY_TY * NX]);
       fma2(acc, xl1, yrow[o + 1]);
-      fma2(acc, xh1, yrow[o + 1 + SY_TY * SY_NX]);
+      fma2(acc, xh1, yrow[o + 1 + SY_TY * NX]);
       ST::st(dst + (long long)gyo * a.op1 + gxo, acc);
     }
     __syncthreads();  // band and yrow are rewritten by the next tile
@@ -754,12 +756,14 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
     a3.ax = (int)od[0];
     a3.ay = (int)od[1];
     a3.az = (int)od[2];
-    a3.tiles_x = (int)((od[0] + SY_TX - 1) / SY_TX);
-    a3.groups_x = (a3.tiles_x + TPW - 1) / TPW;
+    constexpr int TXS = SY_TX;  // (128 for half storage measured slower)
+    a3.tiles_x = (int)((od[0] + TXS - 1) / TXS);
+    a3.tpw = TPW;
+    a3.groups_x = (a3.tiles_x + a3.tpw - 1) / a3.tpw;
     a3.f = f;
     int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
     dim3 g3((unsigned)(a3.groups_x * tiles_y), (unsigned)(od[2] * nvols));
-    hipLaunchKernelGGL(k_wav_xy_syn<ST>, g3, dim3(256), 0, s, a3);
+    hipLaunchKernelGGL((k_wav_xy_syn<ST, TXS>), g3, dim3(256), 0, s, a3);
     LLR_HIP(hipGetLastError());
   }
   return LLR_OK;
