@@ -594,6 +594,9 @@ struct llr_recon {
   // and the back-projection is folded into it
   bool fuse_wavelet = false;
   DevBuf coefK;
+  // Cb holds C = X S of the current iterate as project_current computes it
+  // (left there by the set-up's threshold step)
+  bool cb_current = false;
   DevBuf Yh, Xin;
   RampTables tabs;
   unsigned long long llr_blocks = 0;  // blocks processed by the LLR step (mixed-form statistics)
@@ -885,6 +888,7 @@ static llr_status recon_setup(llr_recon* r) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
   llr_status st;
+  r->cb_current = false;
   // total energy (lib.rs:224, 409-417) and the arg-max behind the linear
   // k-space corrections (lib.rs:229-231) in one pass over the measurements
   {
@@ -938,6 +942,9 @@ static llr_status recon_setup(llr_recon* r) {
       LLR_HIP(hipMemsetAsync(r->smax.p, 0, 8, ctx->stream));
       size_t zero[3] = {0, 0, 0};
       if ((st = recon_llr(r, zero, 2, 0.f, r->smax.as<unsigned>()))) return st;
+      // singular values only: the blocks are not written back, and the
+      // projection is project_current's unless the mixed form rounds X to BF16
+      r->cb_current = p.srank && (p.precision != LLR_PREC_MIXED || r->s16);
       unsigned bits = 0;
       LLR_HIP(hipStreamSynchronize(ctx->stream));
       LLR_HIP(hipMemcpy(&bits, r->smax.p, 4, hipMemcpyDeviceToHost));
@@ -1004,7 +1011,7 @@ static llr_status warm_start(llr_recon* r, llr_unet* net, const float* to_net, c
   if ((st = recon_setup(r))) return st;
   const int K = (int)r->p.srank;
   float2* C = r->Cb.as<float2>();
-  {
+  if (!r->cb_current) {
     StageTimer t(r, 1);
     if ((st = project_current(r, C))) return st;
   }

@@ -78,6 +78,7 @@ struct ConvArgs {
   int relu, accumulate;
   const bf16_t* up;     // 3^3 BF16 forward only: half-resolution source of input channels [0, cup), or null
   int cup;
+  const float* addend;  // 1^3 forward only: out = conv + addend (same layout as out), or null
 };
 
 // out[v][co] = sum_{o, ci} in[v + off(o)][ci] w[o][co][ci] (+ bias, ReLU, mask, +=)
@@ -137,6 +138,7 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
           if (a.mask && !(bf2f(a.mask[idx]) > 0.f)) r = 0.f;
           if constexpr (sizeof(TOut) == 4) {
             if (a.accumulate) r += ((float*)out)[idx];
+            if (a.addend) r += a.addend[idx];
             ((float*)out)[idx] = r;
           } else {
             ((bf16_t*)out)[idx] = f2bf(r);
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
 constexpr int CV_MT = 2;
 constexpr int CV_M = 64 * CV_MT;
 constexpr int CV_RUN = CV_M + 2;
-constexpr int CV_ROW = 20;
+constexpr int CV_ROW = 16;
 
 __device__ __forceinline__ bf16x4 lds4(const bf16_t* p) {
   const uint2 u = *(const uint2*)p;
@@ -753,9 +755,11 @@ static llr_status ensure_workspace(llr_unet* U, const size_t* dims) {
 template <typename TIn, typename TOut>
 static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int lvl, const TIn* in, TOut* out,
                        bool backward_data, const bf16_t* mask, bool accumulate, const bf16_t* up = nullptr,
-                       int cup = 0) {
+                       int cup = 0, const float* addend = nullptr) {
   ConvArgs a{};
   a.in = in;
+  a.addend = addend;
+  if (addend && (u.k != 1 || sizeof(TOut) != 4)) return fail(LLR_ERR_ARG, "addend: 1^3 FP32 output only");
   a.up = up;
   a.cup = up ? cup : 0;
   if (up && (u.k != 3 || backward_data || cup % 16)) return fail(LLR_ERR_ARG, "fused upsampling: 3^3 forward only");
@@ -862,9 +866,9 @@ static llr_status unet_forward(llr_unet* U, const size_t* dims, bool keep_cat = 
                    U->db[l]->as<bf16_t>(), false, nullptr, false)))
       return st;
   }
-  const size_t V = W * H * D;
-  LLR_HIP(hipMemcpyAsync(U->y.p, U->xf.p, V * U->cin * 4, hipMemcpyDeviceToDevice, s));  // residual
-  return conv(U, layer(U, "out"), dims, 0, U->db[0]->as<bf16_t>(), U->y.as<float>(), false, nullptr, true);
+  // prediction = input + 1^3 output convolution (residual), in one pass
+  return conv(U, layer(U, "out"), dims, 0, U->db[0]->as<bf16_t>(), U->y.as<float>(), false, nullptr, false, nullptr, 0,
+              U->xf.as<float>());
 }
 
 // gradients of sum(gy * prediction) into U->G (zeroed first)
