@@ -36,6 +36,9 @@ __host__ __device__ inline float cnorm(float2 a) { return a.x * a.x + a.y * a.y;
 // round once, to nearest even, when storing; all arithmetic is FP32. Data in
 // half storage is kept at a scale where every value is below 2^14 in
 // magnitude (the caller's job), far inside the format's range.
+// Prefetching kernels hold the raw words (T, from zero() or a plain load) and
+// widen them with cvt() only where the values are used: a widening next to
+// the load would make the wave wait for the load at once.
 typedef unsigned int ch16;
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
@@ -44,6 +47,8 @@ struct StoreF32 {
   static constexpr int bytes = 8;
   __device__ static inline float2 ld(const float2* p) { return *p; }
   __device__ static inline void st(float2* p, float2 v) { *p = v; }
+  __device__ static inline float2 cvt(float2 v) { return v; }
+  __device__ static inline float2 zero() { return make_float2(0.f, 0.f); }
 };
 
 struct StoreF16 {
@@ -59,6 +64,8 @@ struct StoreF16 {
   }
   __device__ static inline float2 ld(const ch16* p) { return unpack(*p); }
   __device__ static inline void st(ch16* p, float2 v) { *p = pack(v); }
+  __device__ static inline float2 cvt(ch16 u) { return unpack(u); }
+  __device__ static inline ch16 zero() { return 0u; }
 };
 
 // ------------------------------------------------------------- host context

@@ -45,6 +45,31 @@ constexpr int MAX_STEPS = 16;  // 2*nq <= 256
 // BF16 value nearest to f, as float
 __device__ __forceinline__ float bf16_round(float f) { return __uint_as_float(((unsigned)(unsigned short)bf16_bits(f)) << 16); }
 
+// B fragments of the projection's real operand built from S (nq x K,
+// column-major): row kk = 2q + ic, column n = 2k + oc, entries
+// Re C += Re X Re S - Im X Im S, Im C += Re X Im S + Im X Re S, as BF16 hi + lo
+// so the subspace the product uses is S to ~2^-17. S is read at clamped
+// indices with no guard around the load, so a lane's loads are issued back to
+// back and waited for once; entries outside the matrix are zeroed afterwards.
+template <int K, int MS>
+__device__ __forceinline__ void projection_operand(const float2* __restrict__ S, int nq, int n, int kq,
+                                                   bf16x4 (&bop)[MS], bf16x4 (&bol)[MS]) {
+  const int k = n >> 1, oc = n & 1;
+  const int kc = k < K ? k : K - 1;
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kk = 16 * s + 4 * kq + j, q = kk >> 1, ic = kk & 1;
+      const float2 sv = S[(q < nq ? q : nq - 1) + (long long)kc * nq];
+      float val = oc == 0 ? (ic == 0 ? sv.x : -sv.y) : (ic == 0 ? sv.y : sv.x);
+      if (q >= nq || k >= K) val = 0.f;
+      bop[s][j] = bf16_bits(val);
+      bol[s][j] = bf16_bits(val - bf16_round(val));
+    }
+  }
+}
+
 // Both projections work on tiles of PB_TV = 64 consecutive voxels (one
 // 16-voxel MFMA row block per wavefront) and move the nq-plane side of the
 // product through LDS, so every global access of a plane is a 512-byte run
@@ -64,35 +89,21 @@ __global__ void __launch_bounds__(256) k_project_bf16(const float2* __restrict__
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = lane & 15, kq = lane >> 4;
   const int NS = (2 * nq + 15) / 16;
-  // S as a sum of two BF16 operands (hi + lo), so the subspace the projection
-  // uses is S to ~2^-17 and only the streamed data carries BF16 rounding
+  // only the streamed data carries BF16 rounding
   bf16x4 bop[MS], bol[MS];
-#pragma unroll
-  for (int s = 0; s < MS; ++s) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int kk = 16 * s + 4 * kq + j, q = kk >> 1, ic = kk & 1;
-      const int k = n >> 1, oc = n & 1;
-      float val = 0.f;
-      if (s < NS && q < nq && k < K) {
-        float2 sv = S[q + (long long)k * nq];
-        // Re C += Re X Re S - Im X Im S ; Im C += Re X Im S + Im X Re S
-        val = oc == 0 ? (ic == 0 ? sv.x : -sv.y) : (ic == 0 ? sv.y : sv.x);
-      }
-      bop[s][j] = bf16_bits(val);
-      bol[s][j] = bf16_bits(val - bf16_round(val));
-    }
-  }
+  projection_operand<K, MS>(S, nq, n, kq, bop, bol);
   const long long v0 = (long long)blockIdx.x * PB_TV;
   // stage: plane q, voxel vi -> row vi, columns 2q, 2q+1 (zeros past nq, up to
   // 16 NS); every load of the tile is issued before the first LDS store
   constexpr int NLD = 8 * MS * PB_TV / 256;
+  // (clamped, unguarded loads: a guarded load was waited for on its own)
   float2 xl[NLD];
 #pragma unroll
   for (int it = 0; it < NLD; ++it) {
     const int e = tid + 256 * it, vi = e % PB_TV, q = e / PB_TV;
-    xl[it] = make_float2(0.f, 0.f);
-    if (q < nq && v0 + vi < N) xl[it] = X[(long long)q * N + v0 + vi];
+    const long long vc = v0 + vi < N ? v0 + vi : N - 1;
+    const float2 x = X[(long long)(q < nq ? q : nq - 1) * N + vc];
+    xl[it] = (q < nq && v0 + vi < N) ? x : make_float2(0.f, 0.f);
   }
 #pragma unroll
   for (int it = 0; it < NLD; ++it) {
@@ -158,12 +169,11 @@ __global__ void __launch_bounds__(256) k_backproject_bf16(const float2* __restri
     for (int j = 0; j < 4; ++j) {
       const int kk = 4 * kq + j, k = kk >> 1, ic = kk & 1;
       const int nn = 16 * t + n, q = nn >> 1, oc = nn & 1;
-      float val = 0.f;
-      if (t < NT && k < K && q < nq) {
-        float2 sv = S[q + (long long)k * nq];
-        // Re X += Re L Re S + Im L Im S ; Im X += Im L Re S - Re L Im S
-        val = oc == 0 ? (ic == 0 ? sv.x : sv.y) : (ic == 0 ? -sv.y : sv.x);
-      }
+      // clamped, unguarded load (issued with the others); zeroed outside S
+      const float2 sv = S[(q < nq ? q : nq - 1) + (long long)(k < K ? k : K - 1) * nq];
+      // Re X += Re L Re S + Im L Im S ; Im X += Im L Re S - Re L Im S
+      float val = oc == 0 ? (ic == 0 ? sv.x : sv.y) : (ic == 0 ? -sv.y : sv.x);
+      if (t >= NT || k >= K || q >= nq) val = 0.f;
       bop[t][j] = bf16_bits(val);
       bol[t][j] = bf16_bits(val - bf16_round(val));
     }
@@ -423,89 +433,88 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
 // 16-voxel MFMA row blocks per wavefront); plane runs of 512 bytes are staged
 // in LDS as rows [voxel][plane] of 32-bit half pairs.
 constexpr int P16_TV = 128;
+constexpr int P16_TPW = 4;  // tiles per workgroup (large volumes)
 
 template <int K, int MS>
 __global__ void __launch_bounds__(256) k_project_s16(const ch16* __restrict__ X, float2* __restrict__ Cp,
                                                      const float2* __restrict__ S, long long N, int nq,
-                                                     float out_scale) {
+                                                     float out_scale, int tpw) {
   constexpr int ROW = 8 * MS + 2;  // 32-bit words per staged voxel row (even: 8-byte aligned reads)
   __shared__ __align__(16) unsigned s_x[P16_TV * ROW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = lane & 15, kq = lane >> 4;
   const int NS = (2 * nq + 15) / 16;
   bf16x4 bop[MS], bol[MS];
-#pragma unroll
-  for (int s = 0; s < MS; ++s) {
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int kk = 16 * s + 4 * kq + j, q = kk >> 1, ic = kk & 1;
-      const int k = n >> 1, oc = n & 1;
-      float val = 0.f;
-      if (s < NS && q < nq && k < K) {
-        float2 sv = S[q + (long long)k * nq];
-        val = oc == 0 ? (ic == 0 ? sv.x : -sv.y) : (ic == 0 ? sv.y : sv.x);
-      }
-      bop[s][j] = bf16_bits(val);
-      bol[s][j] = bf16_bits(val - bf16_round(val));
-    }
-  }
-  const long long v0 = (long long)blockIdx.x * P16_TV;
+  projection_operand<K, MS>(S, nq, n, kq, bop, bol);
+  // tpw consecutive tiles per workgroup; the next tile's words are loaded
+  // into registers while the current one is multiplied
+  const long long tiles = (N + P16_TV - 1) / P16_TV;
+  const long long t0 = (long long)blockIdx.x * tpw;
+  const int nt = (int)min((long long)tpw, tiles - t0);
   constexpr int NLD = 8 * MS * P16_TV / 256;
   unsigned xl[NLD];
+  auto load = [&](long long v0) {
 #pragma unroll
-  for (int it = 0; it < NLD; ++it) {
-    const int e = tid + 256 * it, vi = e % P16_TV, q = e / P16_TV;
-    xl[it] = 0u;
-    if (q < nq && v0 + vi < N) xl[it] = X[(long long)q * N + v0 + vi];
-  }
-#pragma unroll
-  for (int it = 0; it < NLD; ++it) {
-    const int e = tid + 256 * it, vi = e % P16_TV, q = e / P16_TV;
-    if (q < 8 * NS) s_x[vi * ROW + q] = xl[it];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb) {
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const unsigned* row = s_x + (32 * wave + 16 * rb + n) * ROW + 2 * kq;
-#pragma unroll
-    for (int s = 0; s < MS; ++s) {
-      if (s < NS) {
-        const uint2 u = *(const uint2*)(row + 8 * s);
-        const float2 x0 = StoreF16::unpack(u.x), x1 = StoreF16::unpack(u.y);
-        const float xv[4] = {x0.x, x0.y, x1.x, x1.y};
-        bf16x4 ah, al;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          ah[j] = bf16_bits(xv[j]);
-          al[j] = bf16_bits(xv[j] - bf16_round(xv[j]));
-        }
-        acc = mfma16(ah, bop[s], acc);
-        acc = mfma16(ah, bol[s], acc);
-        acc = mfma16(al, bop[s], acc);
-      }
+    for (int it = 0; it < NLD; ++it) {
+      const int e = tid + 256 * it, vi = e % P16_TV, q = e / P16_TV;
+      xl[it] = 0u;
+      if (q < nq && v0 + vi < N) xl[it] = X[(long long)q * N + v0 + vi];
     }
+  };
+  load(t0 * P16_TV);
+  for (int ti = 0; ti < nt; ++ti) {
+    const long long v0 = (t0 + ti) * P16_TV;
+    if (ti > 0) __syncthreads();  // the previous tile's LDS reads are done
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[j] *= out_scale;
-    const float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
-    const float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
-    const int k = n >> 1;
-    if (k < K) {
-      const long long w0 = v0 + 32 * wave + 16 * rb + 4 * kq + ((n & 1) ? 2 : 0);
-      float2 c0, c1;
-      if ((n & 1) == 0) {
-        c0 = make_float2(acc[0], p0);
-        c1 = make_float2(acc[1], p1);
-      } else {
-        c0 = make_float2(p2, acc[2]);
-        c1 = make_float2(p3, acc[3]);
+    for (int it = 0; it < NLD; ++it) {
+      const int e = tid + 256 * it, vi = e % P16_TV, q = e / P16_TV;
+      if (q < 8 * NS) s_x[vi * ROW + q] = xl[it];
+    }
+    __syncthreads();
+    if (ti + 1 < nt) load(v0 + P16_TV);
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      const unsigned* row = s_x + (32 * wave + 16 * rb + n) * ROW + 2 * kq;
+#pragma unroll
+      for (int s = 0; s < MS; ++s) {
+        if (s < NS) {
+          const uint2 u = *(const uint2*)(row + 8 * s);
+          const float2 x0 = StoreF16::unpack(u.x), x1 = StoreF16::unpack(u.y);
+          const float xv[4] = {x0.x, x0.y, x1.x, x1.y};
+          bf16x4 ah, al;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            ah[j] = bf16_bits(xv[j]);
+            al[j] = bf16_bits(xv[j] - bf16_round(xv[j]));
+          }
+          acc = mfma16(ah, bop[s], acc);
+          acc = mfma16(ah, bol[s], acc);
+          acc = mfma16(al, bop[s], acc);
+        }
       }
-      float2* dst = Cp + (long long)k * N + w0;
-      if (w0 + 1 < N) {
-        dst[0] = c0;
-        dst[1] = c1;
-      } else if (w0 < N) {
-        dst[0] = c0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] *= out_scale;
+      const float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
+      const float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
+      const int k = n >> 1;
+      if (k < K) {
+        const long long w0 = v0 + 32 * wave + 16 * rb + 4 * kq + ((n & 1) ? 2 : 0);
+        float2 c0, c1;
+        if ((n & 1) == 0) {
+          c0 = make_float2(acc[0], p0);
+          c1 = make_float2(acc[1], p1);
+        } else {
+          c0 = make_float2(p2, acc[2]);
+          c1 = make_float2(p3, acc[3]);
+        }
+        float2* dst = Cp + (long long)k * N + w0;
+        if (w0 + 1 < N) {
+          dst[0] = c0;
+          dst[1] = c1;
+        } else if (w0 < N) {
+          dst[0] = c0;
+        }
       }
     }
   }
@@ -518,17 +527,21 @@ namespace llr {
 llr_status project_s16_device(llr_ctx* ctx, const void* X, float2* Cp, const float2* S, size_t N, int nq, int K,
                               float out_scale) {
   if (nq > 8 * MAX_STEPS) return fail(LLR_ERR_UNSUPPORTED, "BF16 projection handles at most 128 volumes");
-  const unsigned grid = (unsigned)((N + P16_TV - 1) / P16_TV);  // 128-voxel tiles
+  // 128-voxel tiles, P16_TPW per workgroup while that leaves at least 4096 workgroups
+  const size_t tiles = (N + P16_TV - 1) / P16_TV;
+  int tpw = P16_TPW;
+  while (tpw > 1 && (tiles + tpw - 1) / tpw < 4096) tpw /= 2;
+  const unsigned grid = (unsigned)((tiles + tpw - 1) / tpw);
   const bool small = 2 * nq <= 16 * 9;
   const ch16* Xs = (const ch16*)X;
 #define PS_CASE(KK)                                                                                               \
   case KK:                                                                                                        \
     if (small)                                                                                                    \
       hipLaunchKernelGGL((k_project_s16<KK, 9>), dim3(grid), dim3(256), 0, ctx->stream, Xs, Cp, S, (long long)N, nq, \
-                         out_scale);                                                                              \
+                         out_scale, tpw);                                                                         \
     else                                                                                                          \
       hipLaunchKernelGGL((k_project_s16<KK, MAX_STEPS>), dim3(grid), dim3(256), 0, ctx->stream, Xs, Cp, S,          \
-                         (long long)N, nq, out_scale);                                                            \
+                         (long long)N, nq, out_scale, tpw);                                                       \
     break;
   switch (K) {
     PS_CASE(1) PS_CASE(2) PS_CASE(3) PS_CASE(4) PS_CASE(5) PS_CASE(6) PS_CASE(7) PS_CASE(8)

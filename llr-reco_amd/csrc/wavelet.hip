@@ -118,14 +118,14 @@ __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
   E* dst = (E*)a.B + v * a.bvp + (long long)z * a.bx * a.by;
   const int t0 = gx * TPW;
   const int nt = min(TPW, a.tiles_x - t0);
-  float2 xn[AN_NLD];
+  E xn[AN_NLD];
   auto load = [&](int cx0) {
 #pragma unroll
     for (int q = 0; q < AN_NLD; ++q) {
       int e = tid + 256 * q;
       if (e < AN_W * AN_H) {
         int i = e % AN_W, j = e / AN_W;
-        xn[q] = ST::ld(src + (long long)sym(2 * cy0 - 2 + j, a.ay) * a.ip1 + sym(2 * cx0 - 2 + i, a.ax));
+        xn[q] = src[(long long)sym(2 * cy0 - 2 + j, a.ay) * a.ip1 + sym(2 * cx0 - 2 + i, a.ax)];
       }
     }
   };
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_an(XYAnArgs a) {
 #pragma unroll
     for (int q = 0; q < AN_NLD; ++q) {
       int e = tid + 256 * q;
-      if (e < AN_W * AN_H) s_in[e] = xn[q];
+      if (e < AN_W * AN_H) s_in[e] = ST::cvt(xn[q]);
     }
     __syncthreads();
     // along x: local sample 2c+3-k <-> global 2(cx0+c)+1-k
@@ -201,13 +201,13 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
   E* row = (E*)a.B + v * a.bvp + (long long)Y * a.bx;
   const int t0 = gx * TPW;
   const int nt = min(TPW, a.tiles_x - t0);
-  float2 xn[NZR];
+  E xn[NZR];
   auto load = [&](int X) {
     const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx;
 #pragma unroll
     for (int q = 0; q < NZR; ++q) {
       int z = r + (256 / ZT) * q;
-      if (z < a.az) xn[q] = act ? ST::ld(row + X + z * zp) : make_float2(0.f, 0.f);
+      if (z < a.az) xn[q] = act ? row[X + z * zp] : ST::zero();
     }
   };
   load(t0 * ZT + t);
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
 #pragma unroll
     for (int q = 0; q < NZR; ++q) {
       int z = r + (256 / ZT) * q;
-      if (z < a.az) L[z * ZT + t] = xn[q];
+      if (z < a.az) L[z * ZT + t] = ST::cvt(xn[q]);
     }
     __syncthreads();
     const bool ll = X < a.ncx && Y < a.ncy;
@@ -428,6 +428,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   constexpr int NYS = (2 * SY_TY * NX + 255) / 256;   // y-step outputs per thread
   constexpr int NXS = SY_TY * TX / 256;               // x-step outputs per thread
   constexpr int RP = 256 / TX;                        // output rows per x-step pass
+  constexpr int NAQ = (SY_NY * NX + 255) / 256;       // load slots that can hold the LL band (b = 0)
   __shared__ float2 band[4 * SY_NY * NX];   // [xb + 2*yb][iy][ix]
   __shared__ float2 yrow[2 * SY_TY * NX];   // [xb][sy][ix]
   const int tid = threadIdx.x;
@@ -495,18 +496,24 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   const float xh0 = a.f.rhi[xodd ? 3 : 2], xh1 = a.f.rhi[xodd ? 1 : 0];
   const int t0 = gx * a.tpw;
   const int nt = min(a.tpw, a.tiles_x - t0);
-  float2 bn[NLD];
+  // raw words of the next tile: the band value and, for LL slots (q < NAQ),
+  // the two taps of the next level (zero where absent: weight x 0 adds 0)
+  E bn[NLD], ba[NAQ], bb[NAQ];
   auto load = [&](int x0) {
     const int tx0 = x0 / 2;
 #pragma unroll
     for (int q = 0; q < NLD; ++q) {
       const int cx = tx0 + lix[q];
-      bn[q] = make_float2(0.f, 0.f);
+      bn[q] = ST::zero();
+      if (q < NAQ) {
+        ba[q] = ST::zero();
+        bb[q] = ST::zero();
+      }
       if (lofs[q] >= 0 && cx < a.ncx) {
-        bn[q] = ST::ld(src + lofs[q] + cx);
-        if (aofs[q] >= 0) {
-          if (za >= 0) fma2(bn[q], wa, ST::ld(ap + pza + aofs[q] + cx));
-          if (zb >= 0) fma2(bn[q], wb, ST::ld(ap + pzb + aofs[q] + cx));
+        bn[q] = src[lofs[q] + cx];
+        if (q < NAQ && aofs[q] >= 0) {
+          if (za >= 0) ba[q] = ap[pza + aofs[q] + cx];
+          if (zb >= 0) bb[q] = ap[pzb + aofs[q] + cx];
         }
       }
     }
@@ -518,7 +525,14 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
 #pragma unroll
     for (int q = 0; q < NLD; ++q) {
       const int e = tid + 256 * q;
-      if (e < 4 * SY_NY * NX) band[e] = bn[q];
+      if (e < 4 * SY_NY * NX) {
+        float2 b = ST::cvt(bn[q]);
+        if (q < NAQ) {
+          fma2(b, wa, ST::cvt(ba[q]));
+          fma2(b, wb, ST::cvt(bb[q]));
+        }
+        band[e] = b;
+      }
     }
     __syncthreads();
     if (it + 1 < nt) load(x0 + TX);
