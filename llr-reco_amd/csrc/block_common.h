@@ -16,19 +16,68 @@ struct GeomArgs {
   long long m;  // voxels per block
 };
 
-// Voxel address of row i of block b, or -1 for a row past the end of the
-// volume (possible only on the ceil(vol/blk) grid of partial edge blocks).
-// Block b's corner is sub2ind(b, grid) * blk + shift with periodic wrap
-// (block.rs:8-28, 122-133).
+// floor(i / d) for 0 <= i < 2^20 and 1 <= d < 2^20, rd = 1.f / d: the float
+// quotient is within 0.2 of i / d, so one step either way corrects it.
+__device__ __forceinline__ int small_div(int i, int d, float rd) {
+  int q = (int)((float)i * rd);
+  q += ((q + 1) * d <= i) ? 1 : 0;
+  q -= (q * d > i) ? 1 : 0;
+  return q;
+}
+
+// Rows of one block. Block b's corner is sub2ind(b, grid) * blk + shift with
+// periodic wrap (block.rs:8-28, 122-133); row i is the voxel i = ix + bx (iy +
+// by iz) inside it. The corner is found once per block; a row then costs two
+// small divisions and three conditional wraps (shift < vol, set by make_geom
+// and the shift tables). addr(i) is -1 for a row past the end of the volume
+// (possible only on the ceil(vol/blk) grid of partial edge blocks).
+struct BlockRows {
+  int cx, cy, cz, b0, b01, v0, v1, v2, s0, s1, s2;
+  float rb0, rb01;
+  __device__ __forceinline__ BlockRows(const GeomArgs& g, long long b) {
+    const unsigned g0 = (unsigned)g.grid[0], g01 = (unsigned)(g.grid[0] * g.grid[1]);
+    const unsigned ub = (unsigned)b, bz = ub / g01, rr = ub - bz * g01, by = rr / g0, bx = rr - by * g0;
+    b0 = (int)g.blk[0];
+    b01 = (int)(g.blk[0] * g.blk[1]);
+    cx = (int)bx * b0;
+    cy = (int)by * (int)g.blk[1];
+    cz = (int)bz * (int)g.blk[2];
+    v0 = (int)g.vol[0];
+    v1 = (int)g.vol[1];
+    v2 = (int)g.vol[2];
+    s0 = (int)g.shift[0];
+    s1 = (int)g.shift[1];
+    s2 = (int)g.shift[2];
+    rb0 = 1.f / (float)b0;
+    rb01 = 1.f / (float)b01;
+  }
+  __device__ __forceinline__ long long addr(int i) const {
+    const int iz = small_div(i, b01, rb01), r = i - iz * b01;
+    const int iy = small_div(r, b0, rb0), ix = r - iy * b0;
+    int x = cx + ix, y = cy + iy, z = cz + iz;
+    if (x >= v0 || y >= v1 || z >= v2) return -1;
+    x += s0;
+    y += s1;
+    z += s2;
+    if (x >= v0) x -= v0;
+    if (y >= v1) y -= v1;
+    if (z >= v2) z -= v2;
+    return (long long)x + (long long)v0 * ((long long)y + (long long)v1 * z);
+  }
+};
+
+// Voxel address of row i of block b, or -1 (one-off uses; kernels that touch a
+// block's rows many times keep a BlockRows).
 __device__ __forceinline__ long long block_voxel_addr(const GeomArgs& g, long long b, long long i) {
-  long long bx = b % g.grid[0], by = (b / g.grid[0]) % g.grid[1], bz = b / (g.grid[0] * g.grid[1]);
-  long long ix = i % g.blk[0], iy = (i / g.blk[0]) % g.blk[1], iz = i / (g.blk[0] * g.blk[1]);
-  long long sx = ix + bx * g.blk[0], sy = iy + by * g.blk[1], sz = iz + bz * g.blk[2];
-  if (sx >= g.vol[0] || sy >= g.vol[1] || sz >= g.vol[2]) return -1;
-  long long x = (sx + g.shift[0]) % g.vol[0];
-  long long y = (sy + g.shift[1]) % g.vol[1];
-  long long z = (sz + g.shift[2]) % g.vol[2];
-  return x + g.vol[0] * (y + g.vol[1] * z);
+  return BlockRows(g, b).addr((int)i);
+}
+
+// Host-side limits of the 32-bit row arithmetic above.
+inline bool geom_in_range(const GeomArgs& g) {
+  long long blocks = g.grid[0] * g.grid[1] * g.grid[2];
+  return g.m < (1LL << 20) && blocks < (1LL << 31) && g.vol[0] < (1LL << 30) && g.vol[1] < (1LL << 30) &&
+         g.vol[2] < (1LL << 30) && g.grid[0] * g.blk[0] < (1LL << 30) && g.grid[1] * g.blk[1] < (1LL << 30) &&
+         g.grid[2] * g.blk[2] < (1LL << 30);
 }
 
 // Block grid floor(vol/blk) (block.rs:112-118), or ceil(vol/blk) with
@@ -39,7 +88,7 @@ inline GeomArgs make_geom(const size_t* vol, const size_t* blk, const size_t* sh
     g.vol[a] = (long long)vol[a];
     g.blk[a] = (long long)blk[a];
     g.grid[a] = (long long)(partial ? (vol[a] + blk[a] - 1) / blk[a] : vol[a] / blk[a]);
-    g.shift[a] = (long long)shift[a];
+    g.shift[a] = (long long)(shift[a] % vol[a]);
   }
   g.N = g.vol[0] * g.vol[1] * g.vol[2];
   g.m = g.blk[0] * g.blk[1] * g.blk[2];
@@ -224,14 +273,23 @@ struct BlockArgs {
   unsigned* fp64_count; // mixed form: blocks whose eigen-solve fell back to FP64 (optional)
 };
 
-// element address of row i, column col of block b; -1 for a row outside the volume
-__device__ __forceinline__ long long elem_addr(const BlockArgs& A, long long b, long long i, int col) {
+// element address of row i, column col of block b (rows `br` of b when
+// layout 0); -1 for a row outside the volume
+__device__ __forceinline__ long long elem_addr(const BlockArgs& A, const BlockRows& br, long long b, long long i,
+                                               int col) {
   if (A.layout == 1) return b * A.g.m * A.nc + (long long)col * A.g.m + i;
-  long long v = block_voxel_addr(A.g, b, i);
+  long long v = br.addr((int)i);
   return v < 0 ? -1 : (long long)col * A.g.N + v;
 }
-__device__ __forceinline__ float2 load_elem(const BlockArgs& A, long long b, long long i, int col) {
-  long long e = elem_addr(A, b, i, col);
+// element of column 0 of row i (a voxel address for layout 0, -1 outside the
+// volume); column col is at row_base + col * col_stride
+__device__ __forceinline__ long long row_base(const BlockArgs& A, const BlockRows& br, long long b, int i) {
+  return A.layout == 1 ? b * A.g.m * A.nc + i : br.addr(i);
+}
+__device__ __forceinline__ long long col_stride(const BlockArgs& A) { return A.layout == 1 ? A.g.m : A.g.N; }
+__device__ __forceinline__ float2 load_elem(const BlockArgs& A, const BlockRows& br, long long b, long long i,
+                                            int col) {
+  long long e = elem_addr(A, br, b, i, col);
   return e < 0 ? make_float2(0.f, 0.f) : A.F[e];
 }
 

@@ -128,19 +128,20 @@ __global__ void __launch_bounds__(256) k_backproject(const float2* __restrict__ 
 __device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem) {
   const int tid = threadIdx.x;
   const int nc = A.nc;
-  const long long m = A.g.m;
+  const int m = (int)A.g.m;  // < 2^20 (geom_in_range)
   const long long b = A.block0 + slot;
-  const long long mp = m + 1;  // padded column stride in LDS
+  const int mp = m + 1;  // padded column stride in LDS
   float2* Fs = smem;                                      // nc * mp (when staged)
   float2* Gp = smem + (A.stage_lds ? nc * mp : 0);        // 256 partial Gram entries
   float2* Ws = Gp + 256;                                  // 64: W
   float2* Ps = Ws + 64;                                   // nc * nq: W S^H
 
+  const BlockRows br(A.g, b);
+  const long long cs = col_stride(A);
   if (A.stage_lds) {
-    for (long long e = tid; e < m * nc; e += 256) {
-      long long i = e % m;
-      int col = (int)(e / m);
-      Fs[col * mp + i] = load_elem(A, b, i, col);
+    for (int i = tid; i < m; i += 256) {
+      const long long base = row_base(A, br, b, i);
+      for (int col = 0; col < nc; ++col) Fs[col * mp + i] = base < 0 ? make_float2(0.f, 0.f) : A.F[base + col * cs];
     }
     __syncthreads();
   }
@@ -163,14 +164,14 @@ __device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem)
     }
     float2 acc = make_float2(0.f, 0.f);
     if (h < NH) {
-      for (long long r = h; r < m; r += NH) {
+      for (int r = h; r < m; r += NH) {
         float2 fi, fj;
         if (A.stage_lds) {
           fi = Fs[i * mp + r];
           fj = Fs[j * mp + r];
         } else {
-          fi = load_elem(A, b, r, i);
-          fj = load_elem(A, b, r, j);
+          fi = load_elem(A, br, b, r, i);
+          fj = load_elem(A, br, b, r, j);
         }
         acc.x += fi.x * fj.x + fi.y * fj.y;
         acc.y += fi.x * fj.y - fi.y * fj.x;
@@ -234,18 +235,21 @@ __device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem)
     __syncthreads();
   }
   // apply: each thread handles rows r = tid, tid+256, ...
-  for (long long rr = tid; rr < m; rr += 256) {
-    const long long vaddr = A.layout == 0 ? block_voxel_addr(A.g, b, rr) : 0;
-    if (vaddr < 0) continue;  // outside a partial edge block
+  for (int rr = tid; rr < m; rr += 256) {
+    const long long base = row_base(A, br, b, rr);
+    if (base < 0) continue;  // outside a partial edge block
+    // (two loops: a load through a pointer chosen between LDS and global
+    // memory is a generic load)
     float2 f[8];
-    for (int k = 0; k < 8; ++k) {
-      if (k < nc)
-        f[k] = A.stage_lds ? Fs[k * mp + rr] : A.F[elem_addr(A, b, rr, k)];
-      else
-        f[k] = make_float2(0.f, 0.f);
+    if (A.stage_lds) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) f[k] = k < nc ? Fs[k * mp + rr] : make_float2(0.f, 0.f);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) f[k] = k < nc ? A.F[base + k * cs] : make_float2(0.f, 0.f);
     }
     if (A.S) {
-      const long long addr = vaddr;
+      const long long addr = base;  // layout 0 (a subspace output needs voxel planes)
       for (int q = 0; q < A.nq; ++q) {
         float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
@@ -259,7 +263,7 @@ __device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem)
 #pragma unroll
         for (int k = 0; k < 8; ++k)
           if (k < nc) acc = cadd(acc, cmul(f[k], Ws[k + 8 * cc]));
-        A.out[A.layout == 0 ? (long long)cc * A.g.N + vaddr : elem_addr(A, b, rr, cc)] = acc;
+        A.out[base + cc * cs] = acc;
       }
     }
   }
@@ -415,6 +419,7 @@ namespace llr {
 llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols, size_t eb, const size_t* vol,
                        const size_t* blk, const size_t* shift, size_t start, size_t n, bool unlift) {
   GeomArgs g = make_geom(vol, blk, shift);
+  if (!geom_in_range(g)) return fail(LLR_ERR_UNSUPPORTED, "block geometry beyond the lift kernel's index range");
   long long total = (long long)n * (long long)nvols * g.m;
   if (total == 0) return LLR_OK;
   unsigned grid = grid_for((size_t)total, 256);
@@ -503,6 +508,8 @@ llr_status llr_block_run(llr_ctx* ctx, const float2* F, float2* out, const float
                          bool partial) {
   GeomArgs g = make_geom(vol, blk, shift, partial);
   if (nc < 1 || nc > 8) return fail(LLR_ERR_UNSUPPORTED, "fused LLR kernel handles 1..8 columns");
+  if (!geom_in_range(g)) return fail(LLR_ERR_UNSUPPORTED, "block geometry beyond the LLR kernel's index range");
+  if (S && layout != 0) return fail(LLR_ERR_ARG, "a subspace output needs the voxel-plane layout");
   if (nblocks <= 0) return LLR_OK;
   BlockArgs A{};
   A.F = F;
@@ -540,6 +547,7 @@ llr_status llr_multi_plan(MultiPlan& plan, const std::vector<MemberSpec>& member
     if (ms.nc < 1 || ms.nc > 8) return fail(LLR_ERR_UNSUPPORTED, "fused LLR kernel handles 1..8 columns");
     size_t zero[3] = {0, 0, 0};
     GeomArgs g = make_geom(ms.vol, ms.blk, zero, ms.partial);
+    if (!geom_in_range(g)) return fail(LLR_ERR_UNSUPPORTED, "block geometry beyond the LLR kernel's index range");
     BlockArgs A{};
     A.F = ms.F;
     A.out = ms.F;

@@ -220,8 +220,8 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   __shared__ int s_fp64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nc = A.nc;
-  const long long m = A.g.m, mp = m + 1;
-  const long long fsz = (nc * mp + 1) & ~1LL;      // keeps what follows 16-byte aligned
+  const int m = (int)A.g.m, mp = m + 1;  // m < 2^20 (geom_in_range)
+  const int fsz = (nc * mp + 1) & ~1;     // keeps what follows 16-byte aligned
   const long long slot = xcd_slot(A.per_xcd);
   if (slot >= A.nblocks) return;
   const long long b = A.block0 + slot;
@@ -232,10 +232,11 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   float2* Vs = Gp + 256;                           // 64: V0
   float2* Ms = Vs + 64;                            // 64: J g J^H V0^H
 
-  for (long long e = tid; e < m * nc; e += 256) {
-    long long i = e % m;
-    int col = (int)(e / m);
-    Fs[col * mp + i] = load_elem(A, b, i, col);
+  const BlockRows br(A.g, b);
+  const long long cs = col_stride(A);
+  for (int i = tid; i < m; i += 256) {
+    const long long base = row_base(A, br, b, i);
+    for (int col = 0; col < nc; ++col) Fs[col * mp + i] = base < 0 ? make_float2(0.f, 0.f) : A.F[base + col * cs];
   }
   __syncthreads();
 
@@ -245,12 +246,12 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
     const bool valid = cidx < 2 * nc, re = cidx < nc;
     const float2* colp = Fs + (long long)(valid ? (re ? cidx : cidx - nc) : 0) * mp;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    const long long tiles = (m + 15) / 16;
-    for (long long t = wave; t < tiles; t += 4) {
+    const int tiles = (m + 15) / 16;
+    for (int t = wave; t < tiles; t += 4) {
       bf16x4 v;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        long long row = t * 16 + 4 * kq + j;
+        int row = t * 16 + 4 * kq + j;
         float x = 0.f;
         if (valid && row < m) {
           float2 f = colp[row];
@@ -287,7 +288,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   __syncthreads();
 
   // 3. B = F V0 in place (each thread owns whole rows)
-  for (long long rr = tid; rr < m; rr += 256) {
+  for (int rr = tid; rr < m; rr += 256) {
     float2 f[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) f[k] = k < nc ? Fs[k * mp + rr] : make_float2(0.f, 0.f);
@@ -318,7 +319,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
     const int h = tid / NE;
     float2 acc = make_float2(0.f, 0.f);
     if (h < NH)
-      for (long long row = h; row < m; row += NH) acc = cadd(acc, cconjmul(Fs[gi * mp + row], Fs[gj * mp + row]));
+      for (int row = h; row < m; row += NH) acc = cadd(acc, cconjmul(Fs[gi * mp + row], Fs[gj * mp + row]));
     Gp[tid] = acc;
   }
   __syncthreads();
@@ -354,7 +355,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
       const int h = tid / NE;
       double2 acc = make_double2(0.0, 0.0);
       if (h < NH)
-        for (long long row = h; row < m; row += NH) {
+        for (int row = h; row < m; row += NH) {
           float2 x = Fs[gi * mp + row], y = Fs[gj * mp + row];
           double2 xd = make_double2(x.x, x.y), yd = make_double2(y.x, y.y);
           acc = cadd(acc, cconjmul(xd, yd));
@@ -409,9 +410,9 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   __syncthreads();
 
   // 8. out = B M (rows outside a partial edge block are not written)
-  for (long long rr = tid; rr < m; rr += 256) {
-    const long long vaddr = A.layout == 0 ? block_voxel_addr(A.g, b, rr) : 0;
-    if (vaddr < 0) continue;
+  for (int rr = tid; rr < m; rr += 256) {
+    const long long base = row_base(A, br, b, rr);
+    if (base < 0) continue;
     float2 f[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) f[k] = k < nc ? Fs[k * mp + rr] : make_float2(0.f, 0.f);
@@ -420,7 +421,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (k < nc) acc = cadd(acc, cmul(f[k], Ms[k + 8 * cc]));
-      A.out[A.layout == 0 ? (long long)cc * A.g.N + vaddr : elem_addr(A, b, rr, cc)] = acc;
+      A.out[base + cc * cs] = acc;
     }
   }
 }
@@ -613,6 +614,7 @@ llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const
                                float lambda, unsigned* smax_bits, float* svals, unsigned* fp64_count, bool partial) {
   GeomArgs g = make_geom(vol, blk, shift, partial);
   if (nc < 1 || nc > 8) return fail(LLR_ERR_UNSUPPORTED, "mixed LLR kernel handles 1..8 columns");
+  if (!geom_in_range(g)) return fail(LLR_ERR_UNSUPPORTED, "block geometry beyond the LLR kernel's index range");
   if (nblocks <= 0) return LLR_OK;
   size_t smem = llr_block_mixed_smem(g.m, nc);
   if (smem > 64 * 1024) return fail(LLR_ERR_UNSUPPORTED, "block too large for the mixed LLR kernel");
