@@ -171,6 +171,42 @@ __device__ __forceinline__ Rot<R> rot_params(R app, R aqq, typename Cx<R>::T apq
   return r;
 }
 
+// FP32 rotation from the hardware reciprocal and reciprocal square root
+// (about 1 ulp each) instead of IEEE-exact division and square root: the
+// rotation stays unitary to a few ulp (cs = rsq(1 + t^2), sn = t cs; u from
+// rsq of a modulus in [1, 2]) and only the angle carries the approximation,
+// which the next rotations of the sweep absorb. Same skip rule and scaling as
+// the generic form.
+template <>
+__device__ __forceinline__ Rot<float> rot_params<float>(float app, float aqq, float2 apq) {
+  Rot<float> r;
+  const float s = fmaxf(fabsf(apq.x), fabsf(apq.y));
+  if (s == 0.f || s <= 1e-9f * (fabsf(app) + fabsf(aqq))) {
+    r.cs = 1.f;
+    r.sn = 0.f;
+    r.u = make_float2(1.f, 0.f);
+    return r;
+  }
+  const float is = __builtin_amdgcn_rcpf(s);
+  const float xs = apq.x * is, ys = apq.y * is;
+  const float q = xs * xs + ys * ys;  // in [1, 2]
+  const float rn = __builtin_amdgcn_rsqf(q);
+  const float a = s * (q * rn);       // |apq|
+  r.u = make_float2(xs * rn, ys * rn);
+  const float theta = (aqq - app) * (0.5f * __builtin_amdgcn_rcpf(a));
+  const float at = fabsf(theta);
+  float t;
+  if (at > 1e18f) {
+    t = 0.5f * __builtin_amdgcn_rcpf(theta);
+  } else {
+    const float h = theta * theta + 1.f;
+    t = copysignf(__builtin_amdgcn_rcpf(at + h * __builtin_amdgcn_rsqf(h)), theta);
+  }
+  r.cs = __builtin_amdgcn_rsqf(t * t + 1.f);
+  r.sn = t * r.cs;
+  return r;
+}
+
 // Entries of the 2x2 rotation J for the pair containing index i: J[i][i] and
 // J[partner][i].
 template <class R>
