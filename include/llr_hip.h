@@ -251,6 +251,11 @@ typedef struct {
                             block.rs:112-118, leaving the remainder voxels out of the LLR step) */
   int storage;           /* LLR_STORE_*: how the iterate, the measurements' working copy and the
                             wavelet workspace are held on the device (extension) */
+  int no_hybrid;         /* 0: when every x-line of the measurements is fully measured or empty
+                            (a phase-encode mask, as under_sample_kspace makes), the set-up takes
+                            them through the inverse x transform once and the iterations skip the
+                            x passes, which commute with the hard projection; 1: always the full
+                            3D forward model and adjoint, as the reference (extension) */
 } llr_recon_params;
 
 /* Create a reconstruction (allocates the iterate, the measurements and all
@@ -286,6 +291,10 @@ llr_status llr_recon_set_coefficients(llr_recon* r, const llr_cf32* c, size_t le
 llr_status llr_recon_get_image(llr_recon* r, llr_cf32* out, size_t len, int on_device);
 /* Threshold used by the SVT step and the per-volume k-space corrections found. */
 llr_status llr_recon_info(llr_recon* r, double* lambda, float* ksp_corr);
+
+/* 1 in *hybrid when the last set-up chose the (x, ky, kz) iteration (see
+ * llr_recon_params.no_hybrid), else 0. No counterpart in the reference. */
+llr_status llr_recon_hybrid(llr_recon* r, int* hybrid);
 
 /* Mixed-precision statistics: LLR blocks whose K x K eigen-solve fell back to
  * FP64 (Gram condition number above 1e6) and blocks processed, since creation.

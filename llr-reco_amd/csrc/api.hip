@@ -115,10 +115,11 @@ static llr_status check_fft_dims(const size_t* dims) {
 
 // signal model (signal_model.rs:34-79) on device data; src -> dst (may alias)
 // (src and dst both complex64, or both half-precision pairs when h16 is set)
+// (axes a0..2: a0 = 1 leaves the x axis alone)
 static llr_status model_device(llr_ctx* ctx, const void* src, void* dst, const size_t* dims, size_t nv,
-                               const RampTables& t, int dir, bool h16 = false) {
+                               const RampTables& t, int dir, bool h16 = false, int a0 = 0) {
   const void* in = src;
-  for (int a = 0; a < 3; ++a) {
+  for (int a = a0; a < 3; ++a) {
     float s = (float)(1.0 / std::sqrt((double)dims[a]));
     llr_status st;
     if (dir == LLR_FORWARD)
@@ -149,7 +150,7 @@ using namespace llr;
 extern "C" {
 
 const char* llr_last_error(void) { return g_err.c_str(); }
-int llr_abi_version(void) { return 103; }
+int llr_abi_version(void) { return 104; }
 
 llr_status llr_ctx_create(int device, llr_ctx** out) {
   if (!out) return fail(LLR_ERR_ARG, "null output pointer");
@@ -598,6 +599,13 @@ struct llr_recon {
   // (left there by the set-up's threshold step)
   bool cb_current = false;
   DevBuf Yh, Xin;
+  // hybrid: the measurements are a mask constant along x (every x-line is
+  // fully measured or empty, before and after the transform), so the x-axis
+  // transform commutes with the hard projection and the iteration works in
+  // (x, ky, kz): the measurements are taken through the inverse x pass once
+  // (Yx, complex64, or Yh in half storage) and the x passes are skipped
+  bool hybrid = false;
+  DevBuf Yx, flags;
   RampTables tabs;
   unsigned long long llr_blocks = 0;  // blocks processed by the LLR step (mixed-form statistics)
   size_t nparts = 0;
@@ -629,6 +637,13 @@ struct StageTimer {
     r->stage_n[stage] += 1;
   }
 };
+
+// measurements the hard projection reads: half storage (Yh), the x-transformed
+// copy in hybrid mode (Yx), or the caller's k-space data (Y)
+const void* meas_buffer(const llr_recon* r) {
+  if (r->s16) return r->Yh.p;
+  return r->hybrid ? r->Yx.p : r->Y.p;
+}
 
 // backproject = false leaves the new coefficients in Cb (with a subspace) for
 // the fused wavelet to expand.
@@ -695,10 +710,10 @@ llr_status data_consistency(llr_recon* r, double* err) {
   llr_status st;
   void* X = r->X.p;
   const bool h = r->s16;
-  const void* Ym = h ? r->Yh.p : r->Y.p;
+  const void* Ym = meas_buffer(r);
   const RampTables& T = r->tabs;
   float sx = (float)(1.0 / std::sqrt((double)p.vol[0])), sy = (float)(1.0 / std::sqrt((double)p.vol[1]));
-  {
+  if (!r->hybrid) {
     StageTimer t(r, 3);
     st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, false, T.pk[0].as<float2>(), false, T.pi[0].as<float2>(), true, sx,
                        h);
@@ -723,7 +738,7 @@ llr_status data_consistency(llr_recon* r, double* err) {
                        h);
   }
   if (st) return st;
-  {
+  if (!r->hybrid) {
     StageTimer t(r, 3);
     st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false, T.pk[0].as<float2>(), true, sx,
                        h);
@@ -907,11 +922,20 @@ static llr_status recon_setup(llr_recon* r) {
     if ((st = ramp_from_argmax_device(ctx, r->tabs.pk[a].as<float2>(), p.vol, a, r->nq,
                                       r->best.as<unsigned long long>())))
       return st;
+  // is the sampling a mask constant along x (the hybrid iteration's premise)?
+  unsigned flags[2] = {0, 0};
+  {
+    StageTimer t(r, 7);
+    if (!r->flags.p && (st = upload(r->flags, nullptr, 8))) return st;
+    LLR_HIP(hipMemsetAsync(r->flags.p, 0, 8, ctx->stream));
+    if ((st = line_check_device(ctx, r->Y.p, false, p.vol[0], r->D / p.vol[0], r->flags.as<unsigned>()))) return st;
+  }
   std::vector<unsigned long long> best(r->nq);
   double stats[2];
   LLR_HIP(hipStreamSynchronize(ctx->stream));
   LLR_HIP(hipMemcpy(best.data(), r->best.p, r->nq * 8, hipMemcpyDeviceToHost));
   LLR_HIP(hipMemcpy(stats, r->stats.p, 16, hipMemcpyDeviceToHost));
+  LLR_HIP(hipMemcpy(flags, r->flags.p, 4, hipMemcpyDeviceToHost));
   r->energy = stats[0];
   r->hs = (r->s16 && r->energy > 0) ? std::sqrt((double)r->N / r->energy) : 1.0;
   r->ksp_corr.assign(3 * r->nq, 0.f);
@@ -920,18 +944,52 @@ static llr_status recon_setup(llr_recon* r) {
     size_t s[3] = {idx % p.vol[0], (idx / p.vol[0]) % p.vol[1], idx / (p.vol[0] * p.vol[1])};
     for (int a = 0; a < 3; ++a) r->ksp_corr[3 * q + a] = (float)((long long)s[a] - ((long long)p.vol[a] + 1) / 2);
   }
-  if (r->s16) {
-    // the working copy of the measurements, and the initial image if one was set
+  // The measurements the iteration reads. With the sampling constant along x,
+  // they go through the inverse x pass here (in half storage after the
+  // conversion: one more rounding to half) and the x-lines are checked again
+  // on the stored result: the hard projection's measured-sample test must
+  // still pick whole lines.
+  r->hybrid = false;
+  const bool try_hybrid = flags[0] == 0 && !p.no_hybrid;
+  {
     StageTimer t(r, 7);
-    if ((st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs))) return st;
-    if (r->have_init && r->xin_pending && (st = to_h16_device(ctx, r->Xin.as<float2>(), r->X.p, r->D, (float)r->hs)))
+    const float sx = (float)(1.0 / std::sqrt((double)p.vol[0]));
+    const RampTables& T = r->tabs;
+    if (r->s16) {
+      if ((st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs))) return st;
+      if (try_hybrid) {
+        if ((st = fft_axis_pass(ctx, r->Yh.p, r->Yh.p, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false,
+                                T.pk[0].as<float2>(), true, sx, true)))
+          return st;
+        if ((st = line_check_device(ctx, r->Yh.p, true, p.vol[0], r->D / p.vol[0], r->flags.as<unsigned>() + 1)))
+          return st;
+      }
+    } else if (try_hybrid) {
+      if (!r->Yx.p && (st = upload(r->Yx, nullptr, r->D * 8))) return st;
+      if ((st = fft_axis_pass(ctx, r->Y.p, r->Yx.p, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false,
+                              T.pk[0].as<float2>(), true, sx)))
+        return st;
+      if ((st = line_check_device(ctx, r->Yx.p, false, p.vol[0], r->D / p.vol[0], r->flags.as<unsigned>() + 1)))
+        return st;
+    }
+    if (try_hybrid) {
+      LLR_HIP(hipStreamSynchronize(ctx->stream));
+      LLR_HIP(hipMemcpy(flags + 1, r->flags.as<unsigned>() + 1, 4, hipMemcpyDeviceToHost));
+      r->hybrid = flags[1] == 0;
+      if (!r->hybrid && r->s16 && (st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs)))
+        return st;
+    }
+    // the initial image, if one was set
+    if (r->s16 && r->have_init && r->xin_pending &&
+        (st = to_h16_device(ctx, r->Xin.as<float2>(), r->X.p, r->D, (float)r->hs)))
       return st;
     r->xin_pending = false;
   }
   // zero-filled adjoint (lib.rs:233-240) unless warm-started
   if (!r->have_init) {
     StageTimer t(r, 7);
-    if ((st = model_device(ctx, r->s16 ? r->Yh.p : r->Y.p, r->X.p, p.vol, r->nq, r->tabs, LLR_INVERSE, r->s16)))
+    if ((st = model_device(ctx, meas_buffer(r), r->X.p, p.vol, r->nq, r->tabs, LLR_INVERSE, r->s16,
+                           r->hybrid ? 1 : 0)))
       return st;
   }
   // threshold: lambda_factor * sigma_max of the starting blocks at shift 0
@@ -1174,6 +1232,12 @@ llr_status llr_recon_info(llr_recon* r, double* lambda, float* ksp_corr) {
   return LLR_OK;
 }
 
+llr_status llr_recon_hybrid(llr_recon* r, int* hybrid) {
+  if (!r || !hybrid) return fail(LLR_ERR_ARG, "null argument");
+  *hybrid = r->hybrid ? 1 : 0;
+  return LLR_OK;
+}
+
 llr_status llr_recon_mixed_stats(llr_recon* r, unsigned long long* fp64_blocks, unsigned long long* blocks) {
   if (!r) return fail(LLR_ERR_ARG, "null recon");
   LLR_HIP(hipSetDevice(r->ctx->device));
@@ -1390,21 +1454,21 @@ static llr_status batch_post(llr_recon* r, const unsigned* iter, double* tmp) {
   float2* X = r->X.as<float2>();
   const RampTables& T = r->tabs;
   float sx = (float)(1.0 / std::sqrt((double)p.vol[0])), sy = (float)(1.0 / std::sqrt((double)p.vol[1]));
-  if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, false, T.pk[0].as<float2>(), false, T.pi[0].as<float2>(), true,
-                          sx)))
+  if (!r->hybrid && (st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, false, T.pk[0].as<float2>(), false,
+                                        T.pi[0].as<float2>(), true, sx)))
     return st;
   if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, false, T.pk[1].as<float2>(), false, T.pi[1].as<float2>(), true,
                           sy)))
     return st;
   size_t np = 0;
-  if ((st = fft_zproj(ctx, X, r->Y.as<float2>(), p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(),
+  if ((st = fft_zproj(ctx, X, meas_buffer(r), p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(),
                       r->partials.as<double>(), &np)))
     return st;
   if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, true, T.pi[1].as<float2>(), false, T.pk[1].as<float2>(), true,
                           sy)))
     return st;
-  if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false, T.pk[0].as<float2>(), true,
-                          sx)))
+  if (!r->hybrid && (st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false,
+                                        T.pk[0].as<float2>(), true, sx)))
     return st;
   return reduce_at(ctx, r->partials.as<double>(), np, r->hist.as<double>(), iter, tmp);
 }

@@ -104,6 +104,9 @@ llr_status shrink_all_device(llr_ctx* ctx, float2* x, size_t n, float alpha);
 // arg-max found by argmax_volumes (integer corrections, exact reduction)
 llr_status ramp_from_argmax_device(llr_ctx* ctx, float2* tab, const size_t* dims, int axis, size_t nvols,
                                    const unsigned long long* best);
+// *bad |= 1 when some line of nx contiguous values (nlines of them) holds
+// measured samples (!= 0) without being fully measured; *bad is not cleared
+llr_status line_check_device(llr_ctx* ctx, const void* y, bool h16, size_t nx, size_t nlines, unsigned* bad);
 // complex64 -> half-precision pairs (x s) and back (x s); n complex values, n even
 llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s);
 llr_status from_h16_device(llr_ctx* ctx, const void* src, float2* dst, size_t n, float s);

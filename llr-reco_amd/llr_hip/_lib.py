@@ -24,7 +24,7 @@ LLR_PREC_FP32 = 0
 LLR_PREC_MIXED = 1
 LLR_STORE_FP32 = 0
 LLR_STORE_FP16 = 1
-ABI_VERSION = 103
+ABI_VERSION = 104
 LLR_NUM_STAGES = 9
 STAGE_NAMES = ["llr_block", "subspace_project", "wavelet", "fft_x", "fft_y", "fft_z_project", "reductions", "setup",
                "unet"]
@@ -40,6 +40,7 @@ EXPORTS = [
     "llr_under_sample_kspace", "llr_dti_scalars", "llr_dti_scalars_device", "llr_recon_create",
     "llr_recon_destroy", "llr_recon_set_measurements",
     "llr_recon_set_initial", "llr_recon_run", "llr_recon_get_image", "llr_recon_info", "llr_recon_mixed_stats",
+    "llr_recon_hybrid",
     "llr_recon_enable_timing",
     "llr_recon_stage_times", "llr_recon_batch_create", "llr_recon_batch_destroy",
     "llr_recon_batch_set_measurements", "llr_recon_batch_run", "llr_recon_batch_get_image", "llr_recon_batch_tuning",
@@ -91,6 +92,7 @@ class ReconParams(ctypes.Structure):
         ("residual_stop", ctypes.c_double),
         ("partial_blocks", ctypes.c_int),
         ("storage", ctypes.c_int),
+        ("no_hybrid", ctypes.c_int),
     ]
 
 

@@ -56,7 +56,7 @@ def wavelet_denoise(volumes, n_volumes, vol_size, alpha, levels=-1, ctx=None):
 
 def make_params(vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0, lambda_factor=0.02,
                 wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0, precision="fp32",
-                partial_blocks=False, storage="fp32"):
+                partial_blocks=False, storage="fp32", hybrid=True):
     """llr_recon_params for the C ABI, plus the arrays it points at (keep them alive)."""
     vol = [int(v) for v in vol]
     nq = int(nq)
@@ -88,6 +88,7 @@ def make_params(vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lamb
     p.residual_stop = float(residual_stop)
     p.partial_blocks = 1 if partial_blocks else 0
     p.storage = {"fp32": _lib.LLR_STORE_FP32, "fp16": _lib.LLR_STORE_FP16}[storage]
+    p.no_hybrid = 0 if hybrid else 1
     return p, (sh, sub)
 
 
@@ -104,14 +105,19 @@ class Recon:
     wavelet workspace, held at a scale of sqrt(voxels / measurement energy);
     precision "mixed" with a subspace only). Arithmetic is FP32 either way and
     every output is in the caller's units.
+    hybrid: when the measurements are a phase-encode mask (every x-line fully
+    measured or empty), iterate in (x, ky, kz) without the x transforms, which
+    commute with the hard projection (same result up to rounding); False keeps
+    the full 3D forward model and adjoint in every iteration.
     """
 
     def __init__(self, vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0,
                  lambda_factor=0.02, wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0,
-                 precision="fp32", partial_blocks=False, storage="fp32", ctx=None):
+                 precision="fp32", partial_blocks=False, storage="fp32", hybrid=True, ctx=None):
         self.ctx = ctx or _lib.default_context()
         p, self._keep = make_params(vol, nq, shifts, blk, llr_mode, rank, svt_lambda, lambda_factor, wavelet_alpha,
-                                    wavelet_levels, subspace, residual_stop, precision, partial_blocks, storage)
+                                    wavelet_levels, subspace, residual_stop, precision, partial_blocks, storage,
+                                    hybrid)
         self.vol = [int(v) for v in vol]
         self.nq = int(nq)
         self.D = int(np.prod(self.vol)) * self.nq
@@ -210,6 +216,12 @@ class Recon:
         kc = np.zeros(3 * self.nq, np.float32)
         _lib.check(_lib.load().llr_recon_info(self.handle, ctypes.byref(lam), _lib.ptr(kc)))
         return lam.value, kc.reshape(self.nq, 3)
+
+    def hybrid(self):
+        """True when the last set-up chose the (x, ky, kz) iteration."""
+        h = ctypes.c_int()
+        _lib.check(_lib.load().llr_recon_hybrid(self.handle, ctypes.byref(h)))
+        return bool(h.value)
 
     def mixed_stats(self):
         """(blocks whose eigen-solve fell back to FP64, blocks processed) of the mixed form."""
