@@ -707,7 +707,7 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
 llr_status data_consistency(llr_recon* r, double* err) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
-  llr_status st;
+  llr_status st = LLR_OK;
   void* X = r->X.p;
   const bool h = r->s16;
   const void* Ym = meas_buffer(r);
@@ -922,20 +922,11 @@ static llr_status recon_setup(llr_recon* r) {
     if ((st = ramp_from_argmax_device(ctx, r->tabs.pk[a].as<float2>(), p.vol, a, r->nq,
                                       r->best.as<unsigned long long>())))
       return st;
-  // is the sampling a mask constant along x (the hybrid iteration's premise)?
-  unsigned flags[2] = {0, 0};
-  {
-    StageTimer t(r, 7);
-    if (!r->flags.p && (st = upload(r->flags, nullptr, 8))) return st;
-    LLR_HIP(hipMemsetAsync(r->flags.p, 0, 8, ctx->stream));
-    if ((st = line_check_device(ctx, r->Y.p, false, p.vol[0], r->D / p.vol[0], r->flags.as<unsigned>()))) return st;
-  }
   std::vector<unsigned long long> best(r->nq);
   double stats[2];
   LLR_HIP(hipStreamSynchronize(ctx->stream));
   LLR_HIP(hipMemcpy(best.data(), r->best.p, r->nq * 8, hipMemcpyDeviceToHost));
   LLR_HIP(hipMemcpy(stats, r->stats.p, 16, hipMemcpyDeviceToHost));
-  LLR_HIP(hipMemcpy(flags, r->flags.p, 4, hipMemcpyDeviceToHost));
   r->energy = stats[0];
   r->hs = (r->s16 && r->energy > 0) ? std::sqrt((double)r->N / r->energy) : 1.0;
   r->ksp_corr.assign(3 * r->nq, 0.f);
@@ -944,41 +935,33 @@ static llr_status recon_setup(llr_recon* r) {
     size_t s[3] = {idx % p.vol[0], (idx / p.vol[0]) % p.vol[1], idx / (p.vol[0] * p.vol[1])};
     for (int a = 0; a < 3; ++a) r->ksp_corr[3 * q + a] = (float)((long long)s[a] - ((long long)p.vol[a] + 1) / 2);
   }
-  // The measurements the iteration reads. With the sampling constant along x,
-  // they go through the inverse x pass here (in half storage after the
-  // conversion: one more rounding to half) and the x-lines are checked again
-  // on the stored result: the hard projection's measured-sample test must
-  // still pick whole lines.
+  // The measurements the iteration reads. Unless the caller asks for the full
+  // 3D iteration, they go through the inverse x pass once (into half storage
+  // directly when the iterate is held in half), which also checks that every
+  // x-line is fully measured or empty before and after (as stored): the hard
+  // projection's measured-sample test then picks whole lines, and the x
+  // transform commutes with it.
   r->hybrid = false;
-  const bool try_hybrid = flags[0] == 0 && !p.no_hybrid;
   {
     StageTimer t(r, 7);
     const float sx = (float)(1.0 / std::sqrt((double)p.vol[0]));
     const RampTables& T = r->tabs;
-    if (r->s16) {
-      if ((st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs))) return st;
-      if (try_hybrid) {
-        if ((st = fft_axis_pass(ctx, r->Yh.p, r->Yh.p, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false,
-                                T.pk[0].as<float2>(), true, sx, true)))
-          return st;
-        if ((st = line_check_device(ctx, r->Yh.p, true, p.vol[0], r->D / p.vol[0], r->flags.as<unsigned>() + 1)))
-          return st;
-      }
-    } else if (try_hybrid) {
-      if (!r->Yx.p && (st = upload(r->Yx, nullptr, r->D * 8))) return st;
-      if ((st = fft_axis_pass(ctx, r->Y.p, r->Yx.p, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false,
-                              T.pk[0].as<float2>(), true, sx)))
+    bool convert = r->s16;
+    if (!p.no_hybrid) {
+      if (!r->flags.p && (st = upload(r->flags, nullptr, 8))) return st;
+      if (!r->s16 && !r->Yx.p && (st = upload(r->Yx, nullptr, r->D * 8))) return st;
+      LLR_HIP(hipMemsetAsync(r->flags.p, 0, 8, ctx->stream));
+      if ((st = fft_x_inverse_checked(ctx, r->Y.as<float2>(), r->s16 ? r->Yh.p : r->Yx.p, r->s16, p.vol, r->nq,
+                                      T.pi[0].as<float2>(), T.pk[0].as<float2>(),
+                                      (float)(sx * (r->s16 ? r->hs : 1.0)), r->flags.as<unsigned>())))
         return st;
-      if ((st = line_check_device(ctx, r->Yx.p, false, p.vol[0], r->D / p.vol[0], r->flags.as<unsigned>() + 1)))
-        return st;
-    }
-    if (try_hybrid) {
+      unsigned bad = 0;
       LLR_HIP(hipStreamSynchronize(ctx->stream));
-      LLR_HIP(hipMemcpy(flags + 1, r->flags.as<unsigned>() + 1, 4, hipMemcpyDeviceToHost));
-      r->hybrid = flags[1] == 0;
-      if (!r->hybrid && r->s16 && (st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs)))
-        return st;
+      LLR_HIP(hipMemcpy(&bad, r->flags.p, 4, hipMemcpyDeviceToHost));
+      r->hybrid = bad == 0;
+      convert = r->s16 && !r->hybrid;
     }
+    if (convert && (st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs))) return st;
     // the initial image, if one was set
     if (r->s16 && r->have_init && r->xin_pending &&
         (st = to_h16_device(ctx, r->Xin.as<float2>(), r->X.p, r->D, (float)r->hs)))

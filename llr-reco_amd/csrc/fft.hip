@@ -180,7 +180,16 @@ struct PassArgs {
   const float2* tw;   // exp(-2*pi*i*m/N)
   int tiles;          // tiles of T lines along U
   int tiles_per_wg;
+  unsigned* flag;     // line check (x passes): set to 1 when a line is partly measured
 };
+
+// The value a stored complex holds is nonzero (the hard projection's
+// measured-sample test, applied to what a later pass will read back).
+template <class SO>
+__device__ __forceinline__ bool stored_nonzero(float2 v) {
+  if constexpr (SO::bytes == 4) return (SO::pack(v) & 0x7fff7fffu) != 0u;
+  return v.x != 0.f || v.y != 0.f;
+}
 
 constexpr int pow2_floor(int x) { return x >= 256 ? 256 : x >= 128 ? 128 : x >= 64 ? 64 : x >= 32 ? 32 : x >= 16 ? 16 : x >= 8 ? 8 : x >= 4 ? 4 : x >= 2 ? 2 : 1; }
 
@@ -203,15 +212,21 @@ struct Layout {
 // direction, conjugation and 1/sqrt(n) folded in) are staged in LDS once, and
 // the next tile's loads are issued before the current tile's second stage so
 // they overlap its compute and stores. Offsets inside a volume are 32-bit.
-template <int R1, int R2, bool TFAST, bool INV, bool RAMPS, class ST, int MINT>
+// SO: storage of the output (the input's by default). CHECK (lines along the
+// contiguous axis only): count each line's nonzero inputs and stored outputs
+// and set *a.flag when either count is neither 0 nor N.
+template <int R1, int R2, bool TFAST, bool INV, bool RAMPS, class ST, int MINT, class SO = ST, bool CHECK = false>
 __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_pass(PassArgs a) {
   using L = Layout<R1, R2, MINT>;
   using E = typename ST::T;
+  using EO = typename SO::T;
   constexpr int N = L::N, P = L::P, T = L::T, TH = L::THREADS;
+  static_assert(!CHECK || !TFAST, "line checks are for contiguous lines");
   __shared__ float2 lds[T * L::LS];
   __shared__ float2 s_tw[N];
   __shared__ float2 s_pre[RAMPS ? N : 1];
   __shared__ float2 s_post[RAMPS ? N : 1];
+  __shared__ int s_cnt[CHECK ? 2 * 2 * T : 1];  // [tile parity][in, out][line]
   const int tid = threadIdx.x;
   const int t = TFAST ? (tid % T) : (tid / P);
   const int j = TFAST ? (tid / T) : (tid % P);
@@ -226,9 +241,11 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_pass(Pa
       s_post[i] = cscale(a.post_conj ? cconj(q) : q, a.scale);
     }
   }
+  if constexpr (CHECK)
+    for (int i = tid; i < 4 * T; i += TH) s_cnt[i] = 0;
   const long long row = v * a.vstride + (long long)blockIdx.y * a.wstride;
   const E* __restrict__ src = (const E*)a.src + row;
-  E* __restrict__ dst = (E*)a.dst + row;
+  EO* __restrict__ dst = (EO*)a.dst + row;
   const int tile0 = blockIdx.x * a.tiles_per_wg;
   const int ntiles = min(a.tiles_per_wg, a.tiles - tile0);
   float2 xn[R1];
@@ -246,7 +263,14 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_pass(Pa
   for (int it = 0; it < ntiles; ++it) {
     const int u = (tile0 + it) * T + t;
     const bool active = u < a.U;
+    int* cnt = s_cnt + (CHECK ? (it & 1) * 2 * T : 0);
     if (active && j < R2) {
+      if constexpr (CHECK) {
+        int c = 0;
+#pragma unroll
+        for (int j1 = 0; j1 < R1; ++j1) c += (xn[j1].x != 0.f || xn[j1].y != 0.f) ? 1 : 0;
+        atomicAdd(&cnt[t], c);
+      }
       float2 x[R1];
 #pragma unroll
       for (int j1 = 0; j1 < R1; ++j1) x[j1] = RAMPS ? cmul(xn[j1], s_pre[j1 * R2 + j]) : xn[j1];
@@ -272,17 +296,30 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_pass(Pa
         if (INV) y[j2].y = -y[j2].y;
       }
       SmallDFT<R2>::run(y);
-      E* p = dst + (unsigned)(u * a.ustride + j * a.stride);
+      EO* p = dst + (unsigned)(u * a.ustride + j * a.stride);
       unsigned step = (unsigned)(R1 * a.stride);
+      int c = 0;
 #pragma unroll
       for (int k2 = 0; k2 < R2; ++k2) {
         float2 val = y[k2];
         if (INV) val.y = -val.y;
         val = RAMPS ? cmul(val, s_post[j + R1 * k2]) : cscale(val, a.scale);
-        ST::st(p + k2 * step, val);
+        SO::st(p + k2 * step, val);
+        if constexpr (CHECK) c += stored_nonzero<SO>(val) ? 1 : 0;
       }
+      if constexpr (CHECK) atomicAdd(&cnt[T + t], c);
     }
     __syncthreads();
+    if constexpr (CHECK) {
+      // this tile's counts are complete; the next tile counts into the other
+      // parity, and this one is cleared before the tile after next uses it
+      if (active && j == 0) {
+        const int ci = cnt[t], co = cnt[T + t];
+        if ((ci != 0 && ci != N) || (co != 0 && co != N)) atomicOr(a.flag, 1u);
+        cnt[t] = 0;
+        cnt[T + t] = 0;
+      }
+    }
   }
 }
 
@@ -434,6 +471,23 @@ hipError_t launch_one(PassArgs a, long long W, long long nvols, hipStream_t s) {
   return hipGetLastError();
 }
 
+// x-line checked inverse pass from complex64 (into complex64 or half storage)
+template <int R1, int R2>
+hipError_t launch_checked(PassArgs a, long long W, long long nvols, bool h16_out, hipStream_t s) {
+  using L = Layout<R1, R2, 1>;
+  a.tiles = (a.U + L::T - 1) / L::T;
+  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;
+  a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
+  dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
+  if (h16_out)
+    hipLaunchKernelGGL((k_fft_pass<R1, R2, false, true, true, StoreF32, 1, StoreF16, true>), grid, dim3(L::THREADS), 0,
+                       s, a);
+  else
+    hipLaunchKernelGGL((k_fft_pass<R1, R2, false, true, true, StoreF32, 1, StoreF32, true>), grid, dim3(L::THREADS), 0,
+                       s, a);
+  return hipGetLastError();
+}
+
 template <int R1, int R2>
 hipError_t launch_pass(PassArgs a, long long W, long long nvols, bool tfast, bool inv, bool h16, hipStream_t s) {
   const bool ramps = a.pre != nullptr;
@@ -515,6 +569,29 @@ static hipError_t dispatch_pass(int n, const PassArgs& a, long long W, long long
   }
 }
 
+static hipError_t dispatch_checked(int n, const PassArgs& a, long long W, long long nv, bool h16_out, hipStream_t s) {
+  switch (n) {
+    case 2: return launch_checked<2, 1>(a, W, nv, h16_out, s);
+    case 4: return launch_checked<2, 2>(a, W, nv, h16_out, s);
+    case 8: return launch_checked<4, 2>(a, W, nv, h16_out, s);
+    case 16: return launch_checked<4, 4>(a, W, nv, h16_out, s);
+    case 32: return launch_checked<8, 4>(a, W, nv, h16_out, s);
+    case 64: return launch_checked<8, 8>(a, W, nv, h16_out, s);
+    case 128: return launch_checked<16, 8>(a, W, nv, h16_out, s);
+    case 256: return launch_checked<16, 16>(a, W, nv, h16_out, s);
+    case 512: return launch_checked<32, 16>(a, W, nv, h16_out, s);
+    case 3: return launch_checked<3, 1>(a, W, nv, h16_out, s);
+    case 6: return launch_checked<3, 2>(a, W, nv, h16_out, s);
+    case 12: return launch_checked<6, 2>(a, W, nv, h16_out, s);
+    case 24: return launch_checked<6, 4>(a, W, nv, h16_out, s);
+    case 48: return launch_checked<12, 4>(a, W, nv, h16_out, s);
+    case 96: return launch_checked<12, 8>(a, W, nv, h16_out, s);
+    case 192: return launch_checked<12, 16>(a, W, nv, h16_out, s);
+    case 384: return launch_checked<12, 32>(a, W, nv, h16_out, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // One pass along `axis` of nvols volumes of size dims; src may equal dst.
 // pre and post must both be given (ramped signal-model pass) or both be null.
 llr_status fft_axis_pass(llr_ctx* ctx, const void* src, void* dst, const size_t* dims, size_t nvols, int axis,
@@ -568,6 +645,39 @@ llr_status fft_axis_pass(llr_ctx* ctx, const void* src, void* dst, const size_t*
   a.tiles_per_wg = ctx->fft_tpw[axis];
   hipError_t e = dispatch_pass(n, a, W, (long long)nvols, tfast, inverse, h16, ctx->stream);
   if (e != hipSuccess) return hip_fail(e, "fft axis pass launch");
+  return LLR_OK;
+}
+
+llr_status fft_x_inverse_checked(llr_ctx* ctx, const float2* src, void* dst, bool h16_out, const size_t* dims,
+                                 size_t nvols, const float2* pre, const float2* post, float scale, unsigned* flag) {
+  const int n = (int)dims[0];
+  if (!fft_size_supported((size_t)n))
+    return fail(LLR_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " is not supported");
+  if (!pre || !post || !flag) return fail(LLR_ERR_ARG, "checked x pass needs ramp tables and a flag");
+  if ((double)dims[0] * dims[1] * dims[2] >= 4294967296.0)
+    return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
+  const float2* tw = nullptr;
+  llr_status st = get_twiddles(ctx, n, &tw);
+  if (st) return st;
+  const long long nx = dims[0], ny = dims[1], nz = dims[2];
+  PassArgs a{};
+  a.src = src;
+  a.dst = dst;
+  a.vstride = nx * ny * nz;
+  a.pre = pre;
+  a.post = post;
+  a.pre_conj = 0;
+  a.post_conj = 1;
+  a.scale = scale;
+  a.tw = tw;
+  a.stride = 1;
+  a.U = (int)ny;
+  a.ustride = (int)nx;
+  a.wstride = nx * ny;
+  a.tiles_per_wg = ctx->fft_tpw[0];
+  a.flag = flag;
+  hipError_t e = dispatch_checked(n, a, nz, (long long)nvols, h16_out, ctx->stream);
+  if (e != hipSuccess) return hip_fail(e, "checked x pass launch");
   return LLR_OK;
 }
 

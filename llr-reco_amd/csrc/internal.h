@@ -15,6 +15,12 @@ bool fft_size_supported(size_t n);
 llr_status fft_axis_pass(llr_ctx* ctx, const void* src, void* dst, const size_t* dims, size_t nvols, int axis,
                          bool inverse, const float2* pre, bool pre_conj, const float2* post, bool post_conj,
                          float scale, bool h16 = false);
+// Inverse x pass (ramps pre, conj(post), times scale) of complex64 src into
+// dst (complex64, or half-precision pairs when h16_out), setting *flag to 1 if
+// some x-line of src or of the stored dst is partly zero (neither all nonzero
+// nor all zero). *flag is not cleared.
+llr_status fft_x_inverse_checked(llr_ctx* ctx, const float2* src, void* dst, bool h16_out, const size_t* dims,
+                                 size_t nvols, const float2* pre, const float2* post, float scale, unsigned* flag);
 llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* dims, size_t nvols,
                      const float2* pk, const float2* pi, double* partial, size_t* nparts, bool h16 = false);
 
@@ -104,9 +110,6 @@ llr_status shrink_all_device(llr_ctx* ctx, float2* x, size_t n, float alpha);
 // arg-max found by argmax_volumes (integer corrections, exact reduction)
 llr_status ramp_from_argmax_device(llr_ctx* ctx, float2* tab, const size_t* dims, int axis, size_t nvols,
                                    const unsigned long long* best);
-// *bad |= 1 when some line of nx contiguous values (nlines of them) holds
-// measured samples (!= 0) without being fully measured; *bad is not cleared
-llr_status line_check_device(llr_ctx* ctx, const void* y, bool h16, size_t nx, size_t nlines, unsigned* bad);
 // complex64 -> half-precision pairs (x s) and back (x s); n complex values, n even
 llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s);
 llr_status from_h16_device(llr_ctx* ctx, const void* src, float2* dst, size_t n, float s);

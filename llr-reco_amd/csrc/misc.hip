@@ -220,30 +220,6 @@ __global__ void __launch_bounds__(256) k_from_h16(const uint2* __restrict__ src,
   }
 }
 
-// Lines of nx contiguous values (one wavefront per line): sets *bad when a
-// line's count of measured samples (values != 0, the hard projection's test)
-// is neither 0 nor nx, i.e. when the sampling is not a mask constant along x.
-template <class ST>
-__global__ void __launch_bounds__(256) k_line_check(const typename ST::T* __restrict__ y, int nx, long long nlines,
-                                                     unsigned* __restrict__ bad) {
-  const int lane = threadIdx.x & 63;
-  const long long w0 = (blockIdx.x * (long long)blockDim.x + threadIdx.x) >> 6;
-  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
-  unsigned mixed = 0;
-  for (long long l = w0; l < nlines; l += nw) {
-    const typename ST::T* row = y + l * nx;
-    int cnt = 0;
-    for (int x = lane; x < nx; x += 64) {
-      const float2 v = ST::cvt(row[x]);
-      cnt += (v.x != 0.f || v.y != 0.f) ? 1 : 0;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if (cnt != 0 && cnt != nx) mixed = 1;
-  }
-  if (mixed && lane == 0) atomicOr(bad, 1u);
-}
-
 __global__ void k_under_sample(float2* data, long long nx, long long pe, long long nq, const float2* mask) {
   long long total = nx * pe * nq;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
@@ -350,19 +326,6 @@ llr_status ramp_from_argmax_device(llr_ctx* ctx, float2* tab, const size_t* dims
   const int n = (int)dims[axis];
   hipLaunchKernelGGL(k_ramp_argmax, dim3((unsigned)((n + 255) / 256), (unsigned)nvols), dim3(256), 0, ctx->stream,
                      tab, n, stride, n, best, (int)nvols);
-  LLR_HIP(hipGetLastError());
-  return LLR_OK;
-}
-
-llr_status line_check_device(llr_ctx* ctx, const void* y, bool h16, size_t nx, size_t nlines, unsigned* bad) {
-  if (nx == 0 || nlines == 0) return LLR_OK;
-  const unsigned grid = grid_for(nlines * 64, 256, 65536u);
-  if (h16)
-    hipLaunchKernelGGL(k_line_check<StoreF16>, dim3(grid), dim3(256), 0, ctx->stream, (const ch16*)y, (int)nx,
-                       (long long)nlines, bad);
-  else
-    hipLaunchKernelGGL(k_line_check<StoreF32>, dim3(grid), dim3(256), 0, ctx->stream, (const float2*)y, (int)nx,
-                       (long long)nlines, bad);
   LLR_HIP(hipGetLastError());
   return LLR_OK;
 }

@@ -79,6 +79,7 @@ struct ConvArgs {
   const bf16_t* up;     // 3^3 BF16 forward only: half-resolution source of input channels [0, cup), or null
   int cup;
   const float* addend;  // 1^3 forward only: out = conv + addend (same layout as out), or null
+  int per_xcd;          // 3^3: voxel runs per XCD (grid.x = 8 per_xcd; see k_conv3)
 };
 
 // out[v][co] = sum_{o, ci} in[v + off(o)][ci] w[o][co][ci] (+ bias, ReLU, mask, +=)
@@ -179,7 +180,12 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
   const int n = lane & 15, kq = lane >> 4;
   // voxel indices are 32-bit (the host checks D*H*W*channels < 2^31)
   const int HW = a.H * a.W, V = HW * a.D;
-  const int vb = blockIdx.x * CV_M;
+  // workgroups are dealt round-robin to the 8 XCDs; XCD j takes the contiguous
+  // runs [j per_xcd, (j + 1) per_xcd), so the (dz, dy) neighbour rows a run
+  // stages were mostly fetched into the same L2 by the runs before it
+  const int run = (int)(blockIdx.x % 8) * a.per_xcd + (int)(blockIdx.x / 8);
+  if (run * CV_M >= V) return;
+  const int vb = run * CV_M;
   const int cb = blockIdx.y * 16 * NTM;             // this workgroup's output channels [cb, cb + 16 nt)
   const int nt = min(NTM, (a.cout - cb) / 16);
   const int cin2 = a.cin - a.cup;  // channels per voxel of a.in
@@ -784,7 +790,9 @@ static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int 
     // output channels in slices of at most 64 per workgroup (grid.y), so the
     // staged weights fit in LDS beside the input runs
     const int nt = a.cout / 16;
-    const unsigned gx = (unsigned)((V + CV_M - 1) / CV_M);
+    const long long runs = (V + CV_M - 1) / CV_M;
+    a.per_xcd = (int)((runs + 7) / 8);
+    const unsigned gx = (unsigned)(8 * a.per_xcd);
     hipStream_t s = U->ctx->stream;
     if (nt <= 1)
       hipLaunchKernelGGL((k_conv3<TIn, TOut, 1>), dim3(gx), dim3(256), 0, s, a);
