@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--plain-only", action="store_true", help="time the plain FP32 form only")
     ap.add_argument("--storage", default="fp16", choices=["fp16", "fp32"],
                     help="storage of the learned form's iterate, measurements copy and wavelet workspace")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "llr-reco_amd", "llr_hip", "weights", "warm_start_unet.npz"),
+                    help="trained warm-start U-Net (.npz written by llr_hip.warm_start)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (counter calibration) and exit")
     args = ap.parse_args()
@@ -270,7 +272,7 @@ def main():
 
     # ---- learned warm start + BF16 form, to the plain run's residual level
     learned = None
-    weights = os.path.join(ROOT, "llr-reco_amd", "llr_hip", "weights", "warm_start_unet.npz")
+    weights = os.path.abspath(args.weights)
     have_net = os.path.exists(weights) and not args.plain_only and cfg["srank"]
     if have_net:
         from llr_hip import warm_start as W

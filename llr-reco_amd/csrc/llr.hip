@@ -182,7 +182,8 @@ __device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem)
   __syncthreads();
   // wave 0: reduce partials, Jacobi, weights, W
   if (tid < 64) {
-    const int r = tid & 7, c = tid >> 3;
+    const int lane = tid;
+    const int r = lane & 7, c = lane >> 3;
     float2 a = make_float2(0.f, 0.f);
     if (r < nc && c < nc) {
       int i = min(r, c), j = max(r, c);
@@ -193,7 +194,7 @@ __device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem)
       if (r == c) a.y = 0.f;
     }
     float2 vv;
-    wave_jacobi8<float>(a, vv, tid, 16, 2.5e-7f);
+    wave_jacobi8<float>(a, vv, lane, 16, 2.5e-7f);
     float lam[8];
 #pragma unroll
     for (int l = 0; l < 8; ++l) lam[l] = __shfl(a.x, 9 * l, 64);
@@ -201,11 +202,11 @@ __device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem)
 #pragma unroll
     for (int l = 0; l < 8; ++l)
       if (l < nc) lmax = fmaxf(lmax, lam[l]);
-    if (tid == 0) {
+    if (lane == 0) {
       float smax = sqrtf(fmaxf(lmax, 0.f));
       if (A.smax_bits) atomicMax(A.smax_bits, __float_as_uint(smax));
     }
-    if (A.svals && tid < 8) A.svals[slot * 8 + tid] = sqrtf(fmaxf(__shfl(a.x, 9 * tid, 64), 0.f));
+    if (A.svals && lane < 8) A.svals[slot * 8 + lane] = sqrtf(fmaxf(__shfl(a.x, 9 * lane, 64), 0.f));
     if (A.mode != 2) {
       float gl[8];
 #pragma unroll
@@ -219,7 +220,7 @@ __device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem)
         w.x += gl[l] * t.x;
         w.y += gl[l] * t.y;
       }
-      Ws[tid] = w;  // W[r][c]
+      Ws[lane] = w;  // W[r][c]
     }
   }
   if (A.mode == 2) return;

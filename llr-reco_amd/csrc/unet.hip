@@ -161,18 +161,19 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
 // BF16 while staged. With a.up set, input channels [0, a.cup) are the nearest
 // upsampling of the half-resolution tensor a.up and channels [a.cup, cin) come
 // from a.in (the decoder's [upsampled, skip] concatenation, never stored).
-// Neighbours outside the volume are masked per lane, as in k_conv.
+// Neighbours outside the volume: with ROWS (the row length W a multiple of
+// CV_M, so a run lies inside one x-row) every staged row is inside or outside
+// the volume as a whole and only the run's two end values can fall off the
+// row, so the staging writes zeros there and the MFMA loop reads LDS without
+// masks; otherwise they are masked per lane, as in k_conv.
 constexpr int CV_MT = 2;
 constexpr int CV_M = 64 * CV_MT;
 constexpr int CV_RUN = CV_M + 2;
 constexpr int CV_ROW = 16;
 
-__device__ __forceinline__ bf16x4 lds4(const bf16_t* p) {
-  const uint2 u = *(const uint2*)p;
-  return bf16x4{(short)(u.x & 0xFFFF), (short)(u.x >> 16), (short)(u.y & 0xFFFF), (short)(u.y >> 16)};
-}
+__device__ __forceinline__ bf16x4 lds4(const bf16_t* p) { return __builtin_bit_cast(bf16x4, *(const uint2*)p); }
 
-template <typename TIn, typename TOut, int NTM>
+template <typename TIn, typename TOut, int NTM, bool ROWS>
 __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
   __shared__ __align__(16) bf16_t s_run[9 * CV_RUN * CV_ROW];
   __shared__ __align__(16) bf16_t s_w[27 * 16 * NTM * CV_ROW];
@@ -193,6 +194,8 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
   unsigned vmask[CV_MT];
 #pragma unroll
   for (int mt = 0; mt < CV_MT; ++mt) {
+    vmask[mt] = 0u;
+    if constexpr (ROWS) continue;
     const int v = vb + (wave * CV_MT + mt) * 16 + n;
     const int x = v % a.W, y = (v / a.W) % a.H, z = v / HW;
     // bit d + 1 of an axis mask: the step d in {-1, 0, 1} stays inside
@@ -213,6 +216,9 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
   // are independent; issued one at a time their latencies would add up)
   constexpr int NRL = (9 * CV_RUN * 2 + 255) / 256;
   constexpr int NWL = (27 * 16 * NTM * 2 + 255) / 256;
+  // ROWS: the run's row (x0 .. x0 + CV_M - 1, y0, z0), found once
+  const int yz0 = ROWS ? vb / a.W : 0;
+  const int x0 = vb - yz0 * a.W, y0 = ROWS ? yz0 % a.H : 0, z0 = ROWS ? yz0 / a.H : 0;
   for (int c0 = 0; c0 < a.cin; c0 += 16) {
     const bool from_up = a.up != nullptr && c0 < a.cup;
     uint4 ur[NRL];
@@ -222,12 +228,21 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
       ur[q] = make_uint4(0u, 0u, 0u, 0u);
       if (e >= 9 * CV_RUN * 2) continue;
       const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
-      const int idx = vb - 1 + i + (r / 3 - 1) * HW + (r % 3 - 1) * a.W;
+      // neighbour (xx, yy, zz) of the staged value, or its flat index
+      const int xx = x0 - 1 + i, yy = y0 + r % 3 - 1, zz = z0 + r / 3 - 1;
+      const int idx = ROWS ? (zz * a.H + yy) * a.W + xx : vb - 1 + i + (r / 3 - 1) * HW + (r % 3 - 1) * a.W;
+      const bool inside = ROWS ? (xx >= 0 && xx < a.W && yy >= 0 && yy < a.H && zz >= 0 && zz < a.D)
+                               : (idx >= 0 && idx < V);
       uint4& u = ur[q];
-      if (idx >= 0 && idx < V) {
+      if (inside) {
         if constexpr (sizeof(TIn) == 2) {
           if (from_up) {
-            const int x = idx % a.W, y = (idx / a.W) % a.H, z = idx / HW;
+            int x = xx, y = yy, z = zz;
+            if constexpr (!ROWS) {
+              x = idx % a.W;
+              y = (idx / a.W) % a.H;
+              z = idx / HW;
+            }
             const int ci = ((z >> 1) * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
             u = *(const uint4*)(a.up + (size_t)ci * a.cup + c0 + 8 * half);
           } else {
@@ -244,16 +259,17 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
       }
     }
     // this chunk's weights: s_w[o][co - cb][0..16) = w[o'][co][c0..c0+16), o' = o or 26 - o
+    // (zeros for the tiles t >= nt, so every tile's MFMAs run without a branch)
     const int ncw = 16 * nt;
     uint4 uw[NWL];
 #pragma unroll
     for (int q = 0; q < NWL; ++q) {
       const int e = tid + 256 * q;
       uw[q] = make_uint4(0u, 0u, 0u, 0u);
-      if (e >= 27 * ncw * 2) continue;
-      const int half = e & 1, col = (e >> 1) % ncw, o = (e >> 1) / ncw;
+      if (e >= 27 * 16 * NTM * 2) continue;
+      const int half = e & 1, col = (e >> 1) % (16 * NTM), o = (e >> 1) / (16 * NTM);
       const int ow = a.mirror ? 26 - o : o;
-      uw[q] = *(const uint4*)(a.w + ((size_t)ow * a.cout + cb + col) * a.cin + c0 + 8 * half);
+      if (col < ncw) uw[q] = *(const uint4*)(a.w + ((size_t)ow * a.cout + cb + col) * a.cin + c0 + 8 * half);
     }
 #pragma unroll
     for (int q = 0; q < NRL; ++q) {
@@ -267,32 +283,34 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
 #pragma unroll
     for (int q = 0; q < NWL; ++q) {
       const int e = tid + 256 * q;
-      if (e >= 27 * ncw * 2) continue;
-      const int half = e & 1, col = (e >> 1) % ncw, o = (e >> 1) / ncw;
+      if (e >= 27 * 16 * NTM * 2) continue;
+      const int half = e & 1, col = (e >> 1) % (16 * NTM), o = (e >> 1) / (16 * NTM);
       uint2* d = (uint2*)(s_w + (size_t)(o * 16 * NTM + col) * CV_ROW + 8 * half);
       d[0] = make_uint2(uw[q].x, uw[q].y);
       d[1] = make_uint2(uw[q].z, uw[q].w);
     }
     __syncthreads();
-#pragma unroll 3
+#pragma unroll
     for (int o = 0; o < 27; ++o) {
       const int dx = o % 3 - 1, r = o / 3;
       bf16x4 av[CV_MT];
 #pragma unroll
       for (int mt = 0; mt < CV_MT; ++mt) {
-        const bool valid = (vmask[mt] >> o) & 1u;
         const int pos = (wave * CV_MT + mt) * 16 + n + dx + 1;
-        av[mt] = bf16x4{0, 0, 0, 0};
-        if (valid) av[mt] = lds4(s_run + (size_t)(r * CV_RUN + pos) * CV_ROW + 4 * kq);
+        if constexpr (ROWS) {
+          av[mt] = lds4(s_run + (size_t)(r * CV_RUN + pos) * CV_ROW + 4 * kq);
+        } else {
+          const bool valid = (vmask[mt] >> o) & 1u;
+          av[mt] = bf16x4{0, 0, 0, 0};
+          if (valid) av[mt] = lds4(s_run + (size_t)(r * CV_RUN + pos) * CV_ROW + 4 * kq);
+        }
       }
       const bf16_t* wo = s_w + (size_t)o * 16 * NTM * CV_ROW;
 #pragma unroll
       for (int t = 0; t < NTM; ++t) {
-        if (t < nt) {
-          const bf16x4 bv = lds4(wo + (size_t)(16 * t + n) * CV_ROW + 4 * kq);
+        const bf16x4 bv = lds4(wo + (size_t)(16 * t + n) * CV_ROW + 4 * kq);
 #pragma unroll
-          for (int mt = 0; mt < CV_MT; ++mt) acc[mt][t] = mfma16(av[mt], bv, acc[mt][t]);
-        }
+        for (int mt = 0; mt < CV_MT; ++mt) acc[mt][t] = mfma16(av[mt], bv, acc[mt][t]);
       }
     }
     __syncthreads();
@@ -794,12 +812,22 @@ static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int 
     a.per_xcd = (int)((runs + 7) / 8);
     const unsigned gx = (unsigned)(8 * a.per_xcd);
     hipStream_t s = U->ctx->stream;
-    if (nt <= 1)
-      hipLaunchKernelGGL((k_conv3<TIn, TOut, 1>), dim3(gx), dim3(256), 0, s, a);
-    else if (nt <= 2)
-      hipLaunchKernelGGL((k_conv3<TIn, TOut, 2>), dim3(gx), dim3(256), 0, s, a);
-    else
-      hipLaunchKernelGGL((k_conv3<TIn, TOut, 4>), dim3(gx, (unsigned)((nt + 3) / 4)), dim3(256), 0, s, a);
+    const dim3 g4(gx, (unsigned)((nt + 3) / 4));
+    if (a.W % CV_M == 0) {
+      if (nt <= 1)
+        hipLaunchKernelGGL((k_conv3<TIn, TOut, 1, true>), dim3(gx), dim3(256), 0, s, a);
+      else if (nt <= 2)
+        hipLaunchKernelGGL((k_conv3<TIn, TOut, 2, true>), dim3(gx), dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_conv3<TIn, TOut, 4, true>), g4, dim3(256), 0, s, a);
+    } else {
+      if (nt <= 1)
+        hipLaunchKernelGGL((k_conv3<TIn, TOut, 1, false>), dim3(gx), dim3(256), 0, s, a);
+      else if (nt <= 2)
+        hipLaunchKernelGGL((k_conv3<TIn, TOut, 2, false>), dim3(gx), dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_conv3<TIn, TOut, 4, false>), g4, dim3(256), 0, s, a);
+    }
   } else {
     unsigned grid = (unsigned)((V + 63) / 64);
     hipLaunchKernelGGL((k_conv<TIn, TOut>), dim3(grid), dim3(256), 0, U->ctx->stream, a);

@@ -44,7 +44,9 @@ def test_unet_param_layout_round_trip(llr):
     net.close()
 
 
-@pytest.mark.parametrize("dims", [(16, 16, 16), (8, 24, 16)])
+# (8, 8, 256): rows of 256 and 128 at levels 0 and 1 take the 3^3 kernel's
+# row-aligned path (neighbour validity from the staging, no per-lane masks)
+@pytest.mark.parametrize("dims", [(16, 16, 16), (8, 24, 16), (8, 8, 256)])
 def test_unet_forward_vs_oracle(llr, dims):
     net, P, rng = setup_net()
     x = rng.standard_normal(dims + (16,)).astype(np.float32)
@@ -77,7 +79,7 @@ def gpu_cache(net, x, levels=LEVELS):
     return cache
 
 
-@pytest.mark.parametrize("dims", [(16, 16, 16), (24, 32, 48)])
+@pytest.mark.parametrize("dims", [(16, 16, 16), (24, 32, 48), (8, 16, 128)])
 def test_unet_gradients_vs_oracle(llr, dims):
     """Backward kernels against the oracle's backward on the GPU's own forward
     activations (with independent forwards, BF16 rounding flips change ReLU
