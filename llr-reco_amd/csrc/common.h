@@ -39,8 +39,11 @@ __host__ __device__ inline float cnorm(float2 a) { return a.x * a.x + a.y * a.y;
 // Prefetching kernels hold the raw words (T, from zero() or a plain load) and
 // widen them with cvt() only where the values are used: a widening next to
 // the load would make the wave wait for the load at once.
+// ldnt / rawnt: non-temporal loads (streamed data read once per pass, far
+// larger than the caches).
 typedef unsigned int ch16;
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
 
 struct StoreF32 {
   using T = float2;
@@ -49,6 +52,11 @@ struct StoreF32 {
   __device__ static inline void st(float2* p, float2 v) { *p = v; }
   __device__ static inline float2 cvt(float2 v) { return v; }
   __device__ static inline float2 zero() { return make_float2(0.f, 0.f); }
+  __device__ static inline float2 rawnt(const float2* p) {
+    const f32x2v v = __builtin_nontemporal_load((const f32x2v*)p);
+    return make_float2(v.x, v.y);
+  }
+  __device__ static inline float2 ldnt(const float2* p) { return rawnt(p); }
 };
 
 struct StoreF16 {
@@ -66,6 +74,8 @@ struct StoreF16 {
   __device__ static inline void st(ch16* p, float2 v) { *p = pack(v); }
   __device__ static inline float2 cvt(ch16 u) { return unpack(u); }
   __device__ static inline ch16 zero() { return 0u; }
+  __device__ static inline ch16 rawnt(const ch16* p) { return __builtin_nontemporal_load(p); }
+  __device__ static inline float2 ldnt(const ch16* p) { return unpack(rawnt(p)); }
 };
 
 // ------------------------------------------------------------- host context

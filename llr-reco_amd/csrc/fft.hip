@@ -255,7 +255,7 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_pass(Pa
       const E* p = src + (unsigned)(u * a.ustride + j * a.stride);
       unsigned step = (unsigned)(R2 * a.stride);
 #pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1) x[j1] = ST::ld(p + j1 * step);
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = ST::ldnt(p + j1 * step);
     }
   };
   load(tile0, xn);
@@ -373,13 +373,13 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_z_proje
       const E* p = data + (unsigned)(u * a.ustride + j * a.stride);
       unsigned step = (unsigned)(R2 * a.stride);
 #pragma unroll
-      for (int j1 = 0; j1 < R1; ++j1) x[j1] = ST::ld(p + j1 * step);
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = ST::ldnt(p + j1 * step);
     }
     if (u < a.U && j < R1) {
       const E* mp = meas + (unsigned)(u * a.ustride + j * a.stride);
       unsigned step = (unsigned)(R1 * a.stride);
 #pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2) m[k2] = mp[k2 * step];
+      for (int k2 = 0; k2 < R2; ++k2) m[k2] = ST::rawnt(mp + k2 * step);
     }
   };
   load(tile0, xn, mn);
