@@ -464,7 +464,9 @@ template <int R1, int R2, bool TF, bool IV, bool RP, class ST, int MINT>
 hipError_t launch_one(PassArgs a, long long W, long long nvols, hipStream_t s) {
   using L = Layout<R1, R2, MINT>;
   a.tiles = (a.U + L::T - 1) / L::T;
-  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;  // tiles per workgroup (tunable)
+  // tiles per workgroup (tunable): 4 for half storage's 32-line tiles, 2 for
+  // complex64 (measured on the 256-point y pass at 256x256x128x67)
+  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : (MINT > 1 ? 4 : 2);
   a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
   dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
   hipLaunchKernelGGL((k_fft_pass<R1, R2, TF, IV, RP, ST, MINT>), grid, dim3(L::THREADS), 0, s, a);

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256) k_project(const float2* __restrict__ X, f
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = make_float2(0.f, 0.f);
     for (int q = 0; q < nq; ++q) {
-      float2 x = X[(long long)q * N + v];
+      float2 x = StoreF32::ldnt(X + (long long)q * N + v);
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         float2 s = S[q + (long long)k * nq];

@@ -459,7 +459,7 @@ __global__ void __launch_bounds__(256) k_project_s16(const ch16* __restrict__ X,
     for (int it = 0; it < NLD; ++it) {
       const int e = tid + 256 * it, vi = e % P16_TV, q = e / P16_TV;
       xl[it] = 0u;
-      if (q < nq && v0 + vi < N) xl[it] = X[(long long)q * N + v0 + vi];
+      if (q < nq && v0 + vi < N) xl[it] = StoreF16::rawnt(X + (long long)q * N + v0 + vi);
     }
   };
   load(t0 * P16_TV);

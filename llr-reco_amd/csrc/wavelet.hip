@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
         bb[q] = ST::zero();
       }
       if (lofs[q] >= 0 && cx < a.ncx) {
-        bn[q] = src[lofs[q] + cx];
+        bn[q] = src[lofs[q] + cx];  // (cached: neighbouring tiles share band rows and columns)
         if (q < NAQ && aofs[q] >= 0) {
           if (za >= 0) ba[q] = ap[pza + aofs[q] + cx];
           if (zb >= 0) bb[q] = ap[pzb + aofs[q] + cx];
