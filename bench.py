@@ -137,6 +137,44 @@ def cpu_baseline(ds, torch):
                        f"extrapolated to {cfg['n_iter']} iterations")}
 
 
+def gather_scatter(ds, torch):
+    """Block gather (lift_data_set) and scatter (un_lift_data_set, block.rs:147-184) of every
+    block of all volumes, on device buffers (the measurements as the volumes; the scatter
+    goes to a scratch copy). Per launch each moves D bytes in and D bytes out."""
+    import numpy as np
+
+    from llr_hip import block
+
+    vol, nq, blk = ds["vol"], ds["nq"], ds["cfg"]["blk"]
+    n = int(np.prod(block.grid_dim(vol, blk)))
+    m = int(np.prod(blk))
+    shift = tuple(int(s) for s in ds["shifts"][0])
+    y = ds["y"]
+    mats = torch.empty(n * nq * m, dtype=torch.complex64, device=y.device)
+    scratch = torch.empty_like(y)
+    torch.cuda.synchronize()
+    res = {}
+    for name, fn, src, dst in (("lift", block.lift_data_set_device, y, mats),
+                               ("unlift", block.un_lift_data_set_device, scratch, mats)):
+        fn(src.data_ptr(), nq, 8, dst.data_ptr(), mats.numel(), vol, blk, shift, 0, n)
+        ts = []
+        for _ in range(5):
+            t = time.perf_counter()
+            fn(src.data_ptr(), nq, 8, dst.data_ptr(), mats.numel(), vol, blk, shift, 0, n)
+            ts.append(time.perf_counter() - t)
+        ms = 1e3 * min(ts)
+        nbytes = 2 * n * m * nq * 8
+        res[name] = {"ms": round(ms, 4), "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "frac_of_8TBps": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    res["bytes_per_launch"] = 2 * n * m * nq * 8
+    res["what"] = (f"{n} blocks of {blk[0]}x{blk[1]}x{blk[2]} x {nq} volumes, complex64, device buffers; "
+                   "best of 5, host clock around the call (which waits for the copy)")
+    del mats, scratch
+    torch.cuda.empty_cache()
+    log("block gather/scatter: " + json.dumps(res))
+    return res
+
+
 def bench_batch(args, torch, llr_hip):
     """16 reconstructions of uneven size (seeds 100-115: sizes from 96x96x64 to
     256x256x128, 13/31/67 volumes, R 4/6/8) in one batch with graph-replayed
@@ -315,6 +353,8 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, ds, form),
                 "alg_bytes_per_launch": stage_bytes(ds, eb)[dom], "avg_launch_ms": st[dom]["avg_ms"]}
 
+    gs = gather_scatter(ds, torch)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(ds, torch)
@@ -348,6 +388,7 @@ def main():
                    "equal_nrmse_within_1pct": (learned["nrmse_vs_truth"] <= 1.01 * e_plain) if learned else None,
                    "stages": stages},
         "roofline": roof,
+        "gather_scatter": gs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

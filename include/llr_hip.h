@@ -87,6 +87,19 @@ llr_status llr_un_lift_data_set(llr_ctx* ctx, void* volumes, size_t n_volumes, s
                                 const size_t block_size[3], const size_t shift[3], size_t matrix_idx_start,
                                 size_t n_matrices);
 
+/* lift_data_set / un_lift_data_set on buffers already on the context's device
+ * (volumes and matrix_data are device pointers; same arguments and checks;
+ * returns after the copy has finished). No counterpart in the reference, whose
+ * buffers are host slices: for callers that keep the data resident. */
+llr_status llr_lift_data_set_device(llr_ctx* ctx, const void* volumes, size_t n_volumes, size_t elem_bytes,
+                                    void* matrix_data, size_t matrix_len, const size_t vol_size[3],
+                                    const size_t block_size[3], const size_t shift[3], size_t matrix_idx_start,
+                                    size_t n_matrices);
+llr_status llr_un_lift_data_set_device(llr_ctx* ctx, void* volumes, size_t n_volumes, size_t elem_bytes,
+                                       const void* matrix_data, size_t matrix_len, const size_t vol_size[3],
+                                       const size_t block_size[3], const size_t shift[3], size_t matrix_idx_start,
+                                       size_t n_matrices);
+
 /* --------------------------------------------------------- dti_subspace.rs */
 /* dti_project (dti_subspace.rs:141-166): compressed (srank x nvox per matrix)
  * = S^H X^H for each of batch_size image matrices X (nvox x nq); S is nq x srank.

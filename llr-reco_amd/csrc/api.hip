@@ -150,7 +150,7 @@ using namespace llr;
 extern "C" {
 
 const char* llr_last_error(void) { return g_err.c_str(); }
-int llr_abi_version(void) { return 104; }
+int llr_abi_version(void) { return 105; }
 
 llr_status llr_ctx_create(int device, llr_ctx** out) {
   if (!out) return fail(LLR_ERR_ARG, "null output pointer");
@@ -201,6 +201,9 @@ static llr_status lift_common(llr_ctx* ctx, const void* volumes, void* vol_out, 
   size_t m = blk[0] * blk[1] * blk[2];
   size_t need = m * nvols * n;
   if (matrix_len < need) return fail(LLR_ERR_ARG, "matrix data buffer is too small");
+  size_t grid[3];
+  llr_grid_dim(vol, blk, grid);
+  if (start + n > grid[0] * grid[1] * grid[2]) return fail(LLR_ERR_ARG, "matrix range past the block grid");
   size_t N = vol[0] * vol[1] * vol[2];
   LLR_HIP(hipSetDevice(ctx->device));
   DevBuf dv, dm;
@@ -227,6 +230,41 @@ llr_status llr_un_lift_data_set(llr_ctx* ctx, void* volumes, size_t n_volumes, s
                                 size_t n_matrices) {
   return lift_common(ctx, volumes, volumes, n_volumes, elem_bytes, matrix_data, nullptr, matrix_len, vol_size,
                      block_size, shift, matrix_idx_start, n_matrices, true);
+}
+
+static llr_status lift_device_common(llr_ctx* ctx, const void* volumes, size_t nvols, size_t eb, void* mats,
+                                     size_t matrix_len, const size_t* vol, const size_t* blk, const size_t* shift,
+                                     size_t start, size_t n, bool unlift) {
+  if (!ctx) return fail(LLR_ERR_ARG, "null context");
+  if (eb == 0) return fail(LLR_ERR_ARG, "element size must be positive");
+  if (!volumes || !mats) return fail(LLR_ERR_ARG, "null buffer");
+  llr_status st = check_blocks(vol, blk);
+  if (st) return st;
+  const size_t m = blk[0] * blk[1] * blk[2];
+  if (matrix_len < m * nvols * n) return fail(LLR_ERR_ARG, "matrix data buffer is too small");
+  size_t grid[3];
+  llr_grid_dim(vol, blk, grid);
+  if (start + n > grid[0] * grid[1] * grid[2]) return fail(LLR_ERR_ARG, "matrix range past the block grid");
+  LLR_HIP(hipSetDevice(ctx->device));
+  if ((st = lift_device(ctx, volumes, mats, nvols, eb, vol, blk, shift, start, n, unlift))) return st;
+  LLR_HIP(hipStreamSynchronize(ctx->stream));
+  return LLR_OK;
+}
+
+llr_status llr_lift_data_set_device(llr_ctx* ctx, const void* volumes, size_t n_volumes, size_t elem_bytes,
+                                    void* matrix_data, size_t matrix_len, const size_t vol_size[3],
+                                    const size_t block_size[3], const size_t shift[3], size_t matrix_idx_start,
+                                    size_t n_matrices) {
+  return lift_device_common(ctx, volumes, n_volumes, elem_bytes, matrix_data, matrix_len, vol_size, block_size, shift,
+                            matrix_idx_start, n_matrices, false);
+}
+
+llr_status llr_un_lift_data_set_device(llr_ctx* ctx, void* volumes, size_t n_volumes, size_t elem_bytes,
+                                       const void* matrix_data, size_t matrix_len, const size_t vol_size[3],
+                                       const size_t block_size[3], const size_t shift[3], size_t matrix_idx_start,
+                                       size_t n_matrices) {
+  return lift_device_common(ctx, volumes, n_volumes, elem_bytes, (void*)matrix_data, matrix_len, vol_size,
+                            block_size, shift, matrix_idx_start, n_matrices, true);
 }
 
 // ----------------------------------------------------------- dti_subspace.rs

@@ -26,25 +26,75 @@
 namespace {
 
 
-template <class E>
-__global__ void k_lift(const E* __restrict__ vols, E* __restrict__ mats, GeomArgs g, long long nvols,
-                       long long start, long long total) {
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    long long i = e % g.m, rest = e / g.m;
-    long long q = rest % nvols, c = rest / nvols;
-    mats[e] = vols[q * g.N + block_voxel_addr(g, start + c, i)];
-  }
-}
-
-template <class E>
-__global__ void k_unlift(E* __restrict__ vols, const E* __restrict__ mats, GeomArgs g, long long nvols,
-                         long long start, long long total) {
-  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    long long i = e % g.m, rest = e / g.m;
-    long long q = rest % nvols, c = rest / nvols;
-    vols[q * g.N + block_voxel_addr(g, start + c, i)] = mats[e];
+// Gather (lift, block.rs:147-164) or scatter (un_lift, block.rs:166-184) of
+// whole blocks, by rows of the block grid. Workgroup (row, y, z) owns the
+// grid_x blocks of one grid row (the matrices c = row grid_x + bx that fall in
+// [start, start + n)), the volumes q = 4y .. 4y+3 and a run of block rows
+// r = iy + blk[1] iz (blockIdx.z). Thread t owns the voxel column x = t,
+// t + 256, ... of the row, so every volume row the workgroup touches is one
+// contiguous run of grid_x blk[0] voxels, and a thread's matrix offsets
+// advance by blk[0] per block row (a block's matrix fills contiguously in
+// L2). Four block rows of four volumes are loaded before the first value is
+// stored. Matrix c, volume q, row i is at ((c - start) nvols + q) m + i.
+template <class E, bool UNLIFT>
+__global__ void __launch_bounds__(256) k_lift_rows(const E* __restrict__ src, E* __restrict__ dst, GeomArgs g,
+                                                    long long nvols, long long start, long long n) {
+  constexpr int QU = 4, RU = 4;
+  const int gx = (int)g.grid[0], gyy = (int)g.grid[1];
+  const int b0 = (int)g.blk[0], b1 = (int)g.blk[1], b2 = (int)g.blk[2], m = (int)g.m;
+  const int nx = (int)g.vol[0], ny = (int)g.vol[1], nz = (int)g.vol[2];
+  const long long row = blockIdx.x + start / gx;  // grid row (by, bz)
+  const int by = (int)(row % gyy), bz = (int)(row / gyy);
+  const int nr = b1 * b2;
+  // this workgroup's block rows [rlo, rhi): grid.z splits them in runs of RU
+  const int rper = ((nr + (int)gridDim.z - 1) / (int)gridDim.z + RU - 1) / RU * RU;
+  const int rlo = blockIdx.z * rper, rhi = min(nr, rlo + rper);
+  const float rb0 = 1.f / (float)b0;
+  for (int xl = threadIdx.x; xl < gx * b0; xl += blockDim.x) {
+    const int bx = small_div(xl, b0, rb0), ix = xl - bx * b0;
+    const long long c = row * gx + bx;
+    if (c < start || c >= start + n) continue;
+    int x = bx * b0 + ix + (int)g.shift[0];
+    if (x >= nx) x -= nx;
+    const long long mbase = (c - start) * nvols * m + ix;
+    for (long long q0 = (long long)blockIdx.y * QU; q0 < nvols; q0 += (long long)gridDim.y * QU) {
+      for (int r0 = rlo; r0 < rhi; r0 += RU) {
+        E val[RU][QU];
+        long long va[RU];
+#pragma unroll
+        for (int rr = 0; rr < RU; ++rr) {
+          const int r = r0 + rr;
+          va[rr] = -1;
+          if (r < rhi) {
+            const int iz = r / b1, iy = r - iz * b1;
+            int y = by * b1 + iy + (int)g.shift[1], z = bz * b2 + iz + (int)g.shift[2];
+            if (y >= ny) y -= ny;
+            if (z >= nz) z -= nz;
+            va[rr] = x + (long long)nx * (y + (long long)ny * z);
+          }
+        }
+#pragma unroll
+        for (int rr = 0; rr < RU; ++rr)
+#pragma unroll
+          for (int u = 0; u < QU; ++u) {
+            const long long q = q0 + u;
+            if (q < nvols && va[rr] >= 0)
+              val[rr][u] = UNLIFT ? src[mbase + q * m + (long long)(r0 + rr) * b0] : src[q * g.N + va[rr]];
+          }
+#pragma unroll
+        for (int rr = 0; rr < RU; ++rr)
+#pragma unroll
+          for (int u = 0; u < QU; ++u) {
+            const long long q = q0 + u;
+            if (q < nvols && va[rr] >= 0) {
+              if (UNLIFT)
+                dst[q * g.N + va[rr]] = val[rr][u];
+              else
+                dst[mbase + q * m + (long long)(r0 + rr) * b0] = val[rr][u];
+            }
+          }
+      }
+    }
   }
 }
 
@@ -423,15 +473,21 @@ llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols,
   if (!geom_in_range(g)) return fail(LLR_ERR_UNSUPPORTED, "block geometry beyond the lift kernel's index range");
   long long total = (long long)n * (long long)nvols * g.m;
   if (total == 0) return LLR_OK;
-  unsigned grid = grid_for((size_t)total, 256);
   hipStream_t s = ctx->stream;
-#define LIFT_CASE(E)                                                                                        \
-  if (unlift)                                                                                               \
-    hipLaunchKernelGGL(k_unlift<E>, dim3(grid), dim3(256), 0, s, (E*)vols, (const E*)mats, g, (long long)nvols, \
-                       (long long)start, total);                                                            \
-  else                                                                                                      \
-    hipLaunchKernelGGL(k_lift<E>, dim3(grid), dim3(256), 0, s, (const E*)vols, (E*)mats, g, (long long)nvols, \
-                       (long long)start, total);
+#define LIFT_CASE(E)                                                                                          \
+  if (unlift)                                                                                                 \
+    hipLaunchKernelGGL((k_lift_rows<E, true>), grid, dim3(256), 0, s, (const E*)mats, (E*)(void*)vols, g,     \
+                       (long long)nvols, (long long)start, (long long)n);                                    \
+  else                                                                                                        \
+    hipLaunchKernelGGL((k_lift_rows<E, false>), grid, dim3(256), 0, s, (const E*)vols, (E*)mats, g,           \
+                       (long long)nvols, (long long)start, (long long)n);
+  // grid.x: the block-grid rows the range touches; grid.y: volume groups of 4;
+  // grid.z: runs of block rows (about 16 rows each)
+  const long long gx = g.grid[0];
+  const long long rows = (long long)((start + n - 1) / gx) - (long long)(start / gx) + 1;
+  const long long nr = g.blk[1] * g.blk[2];
+  const dim3 grid((unsigned)rows, (unsigned)std::min<size_t>((nvols + 3) / 4, 65535),
+                  (unsigned)std::max<long long>(1, std::min<long long>(nr / 16, 64)));
   switch (eb) {
     case 1: LIFT_CASE(uint8_t); break;
     case 2: LIFT_CASE(uint16_t); break;

@@ -24,7 +24,7 @@ LLR_PREC_FP32 = 0
 LLR_PREC_MIXED = 1
 LLR_STORE_FP32 = 0
 LLR_STORE_FP16 = 1
-ABI_VERSION = 104
+ABI_VERSION = 105
 LLR_NUM_STAGES = 9
 STAGE_NAMES = ["llr_block", "subspace_project", "wavelet", "fft_x", "fft_y", "fft_z_project", "reductions", "setup",
                "unet"]
@@ -32,7 +32,8 @@ STAGE_NAMES = ["llr_block", "subspace_project", "wavelet", "fft_x", "fft_y", "ff
 # every symbol include/llr_hip.h declares
 EXPORTS = [
     "llr_last_error", "llr_abi_version", "llr_ctx_create", "llr_ctx_destroy", "llr_ctx_synchronize",
-    "llr_ctx_device", "llr_grid_dim", "llr_lift_data_set", "llr_un_lift_data_set", "llr_dti_project",
+    "llr_ctx_device", "llr_grid_dim", "llr_lift_data_set", "llr_un_lift_data_set", "llr_lift_data_set_device",
+    "llr_un_lift_data_set_device", "llr_dti_project",
     "llr_dti_back_project", "llr_conj_transpose_matrix", "llr_subspace_constraint", "llr_low_rank_approx_batch",
     "llr_svt_batch", "llr_cgemm_strided_batched", "llr_rhs_cdgmm_batched", "llr_signal_model",
     "llr_signal_model_device", "llr_estimate_linear_correction", "llr_fftn_batch", "llr_fftshift", "llr_ifftshift",

@@ -51,3 +51,23 @@ def un_lift_data_set(volumes, n_volumes, matrix_data, vol_size, block_size, shif
         _lib.ctx_handle(ctx), _lib.ptr(volumes), ctypes.c_size_t(n_volumes), ctypes.c_size_t(volumes.itemsize),
         _lib.ptr(matrix_data), ctypes.c_size_t(matrix_data.size), _lib.sz3(vol_size), _lib.sz3(block_size),
         _lib.sz3(shift), ctypes.c_size_t(matrix_idx_start), ctypes.c_size_t(n_matrices)))
+
+
+def lift_data_set_device(volumes_ptr, n_volumes, elem_bytes, matrix_ptr, matrix_len, vol_size, block_size, shift,
+                         matrix_idx_start, n_matrices, ctx=None):
+    """lift_data_set on device buffers (pointers on the context's device; matrix_len in elements)."""
+    _lib.check(_lib.load().llr_lift_data_set_device(
+        _lib.ctx_handle(ctx), ctypes.c_void_p(int(volumes_ptr)), ctypes.c_size_t(n_volumes),
+        ctypes.c_size_t(elem_bytes), ctypes.c_void_p(int(matrix_ptr)), ctypes.c_size_t(matrix_len),
+        _lib.sz3(vol_size), _lib.sz3(block_size), _lib.sz3(shift), ctypes.c_size_t(matrix_idx_start),
+        ctypes.c_size_t(n_matrices)))
+
+
+def un_lift_data_set_device(volumes_ptr, n_volumes, elem_bytes, matrix_ptr, matrix_len, vol_size, block_size, shift,
+                            matrix_idx_start, n_matrices, ctx=None):
+    """un_lift_data_set on device buffers (pointers on the context's device; matrix_len in elements)."""
+    _lib.check(_lib.load().llr_un_lift_data_set_device(
+        _lib.ctx_handle(ctx), ctypes.c_void_p(int(volumes_ptr)), ctypes.c_size_t(n_volumes),
+        ctypes.c_size_t(elem_bytes), ctypes.c_void_p(int(matrix_ptr)), ctypes.c_size_t(matrix_len),
+        _lib.sz3(vol_size), _lib.sz3(block_size), _lib.sz3(shift), ctypes.c_size_t(matrix_idx_start),
+        ctypes.c_size_t(n_matrices)))

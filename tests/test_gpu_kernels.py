@@ -63,6 +63,37 @@ def test_lift_errors(llr):
         block.lift_data_set(np.zeros(8, np.float32), 1, np.zeros(64, np.float32), (2, 2, 2), (4, 1, 1), (0, 0, 0), 0, 1)
     with pytest.raises(llr.LlrError):  # matrix buffer too small (block.rs:152)
         block.lift_data_set(np.zeros(64, np.float32), 1, np.zeros(7, np.float32), (4, 4, 4), (2, 2, 2), (0, 0, 0), 0, 1)
+    with pytest.raises(llr.LlrError):  # matrices past the block grid (8 blocks)
+        block.lift_data_set(np.zeros(64, np.float32), 1, np.zeros(64, np.float32), (4, 4, 4), (2, 2, 2), (0, 0, 0), 7, 2)
+
+
+@pytest.mark.parametrize("vol,blk,shift,start,n", [((64, 64, 64), (8, 8, 8), (61, 3, 40), 100, 300),
+                                                    ((9, 7, 5), (3, 2, 2), (8, 6, 4), 3, 7)])
+def test_lift_device_matches_host(llr, vol, blk, shift, start, n):
+    """The device-pointer entry points give the host entry points' bytes."""
+    import torch
+
+    from llr_hip import block
+
+    nvols = 5
+    N = int(np.prod(vol))
+    m = int(np.prod(blk))
+    vols = (rng.standard_normal(N * nvols) + 1j * rng.standard_normal(N * nvols)).astype(np.complex64)
+    ref = np.zeros(n * nvols * m, np.complex64)
+    block.lift_data_set(vols, nvols, ref, vol, blk, shift, start, n)
+    dev = torch.device("cuda", 0)
+    dv = torch.from_numpy(vols).to(dev)
+    dm = torch.zeros(n * nvols * m, dtype=torch.complex64, device=dev)
+    torch.cuda.synchronize()
+    block.lift_data_set_device(dv.data_ptr(), nvols, 8, dm.data_ptr(), dm.numel(), vol, blk, shift, start, n)
+    assert dm.cpu().numpy().tobytes() == ref.tobytes()
+    out = np.ascontiguousarray(vols[::-1])
+    refv = out.copy()
+    block.un_lift_data_set(refv, nvols, ref, vol, blk, shift, start, n)
+    do = torch.from_numpy(out).to(dev)
+    torch.cuda.synchronize()
+    block.un_lift_data_set_device(do.data_ptr(), nvols, 8, dm.data_ptr(), dm.numel(), vol, blk, shift, start, n)
+    assert do.cpu().numpy().tobytes() == refv.tobytes()
 
 
 # ----------------------------------------------------- cgemm / subspace / diag
