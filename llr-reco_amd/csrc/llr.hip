@@ -98,6 +98,126 @@ __global__ void __launch_bounds__(256) k_lift_rows(const E* __restrict__ src, E*
   }
 }
 
+// Gather/scatter through LDS, one slab at a time. Workgroup (row, y, z) owns
+// grid row `row`, the block rows r0 .. r0 + LB_RC - 1 (r0 = LB_RC blockIdx.z)
+// and the volumes q = y, y + gridDim.y, ... For one volume the slab is LB_RC
+// volume rows of grid_x blk[0] voxels; for each block of the row the same
+// slab is LB_RC blk[0] consecutive matrix elements starting at row r0 blk[0].
+// Lift reads the volume rows whole (coalesced along x) into the LDS tile and
+// writes each block's run contiguously; un_lift does the reverse. Tile rows
+// are padded by one element so that the block-wise side does not hit one
+// LDS bank with every row. Matrix c, volume q, row i is at
+// ((c - start) nvols + q) m + i (block.rs:147-184).
+constexpr int LB_RC = 8;
+template <class E, bool UNLIFT>
+__global__ void __launch_bounds__(256) k_lift_slab(const E* __restrict__ src, E* __restrict__ dst, GeomArgs g,
+                                                    long long nvols, long long start, long long n) {
+  extern __shared__ unsigned char lift_lds[];
+  E* tile = reinterpret_cast<E*>(lift_lds);
+  __shared__ long long rowoff[LB_RC];
+  constexpr int NB = 8;  // elements per thread loaded before the first store
+  const int gx = (int)g.grid[0], gyy = (int)g.grid[1];
+  const int b0 = (int)g.blk[0], b1 = (int)g.blk[1], m = (int)g.m;
+  const int nx = (int)g.vol[0], ny = (int)g.vol[1], nz = (int)g.vol[2];
+  const int WP = gx * b0 + 1;
+  const long long row = blockIdx.x + start / gx;
+  const long long c0 = row * gx;
+  const int by = (int)(row % gyy), bz = (int)(row / gyy);
+  const int r0 = (int)blockIdx.z * LB_RC;
+  const int rc = min(LB_RC, b1 * (int)g.blk[2] - r0);
+  const int bxlo = (int)max(0LL, start - c0), bxhi = (int)min((long long)gx, start + n - c0);
+  if (threadIdx.x < rc) {
+    const int r = r0 + (int)threadIdx.x, iz = r / b1, iy = r - iz * b1;
+    int y = by * b1 + iy + (int)g.shift[1], z = bz * (int)g.blk[2] + iz + (int)g.shift[2];
+    if (y >= ny) y -= ny;
+    if (z >= nz) z -= nz;
+    rowoff[threadIdx.x] = (long long)nx * (y + (long long)ny * z);
+  }
+  __syncthreads();
+  const int wx = (bxhi - bxlo) * b0;  // slab row length
+  const int per = rc * b0;             // one block's run
+  const int tot = rc * wx;
+  const float rwx = 1.f / (float)wx, rper = 1.f / (float)per, rb0 = 1.f / (float)b0;
+  const int sx = (int)g.shift[0], xlo = bxlo * b0;
+  const long long mrow = (long long)r0 * b0;
+  for (long long q = blockIdx.y; q < nvols; q += gridDim.y) {
+    const long long vq = q * g.N;
+    const long long mq = (c0 + bxlo - start) * nvols * m + q * m + mrow;
+    // volume side: element j of the slab is row rr = j / wx, x = xlo + j % wx
+    // block side: element j is block bxlo + j / per, run offset j % per
+    for (int j0 = threadIdx.x; j0 < tot; j0 += 256 * NB) {
+      E v[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int j = j0 + u * 256;
+        if (j < tot) {
+          if (!UNLIFT) {
+            const int rr = small_div(j, wx, rwx);
+            int x = xlo + (j - rr * wx) + sx;
+            if (x >= nx) x -= nx;
+            v[u] = src[vq + rowoff[rr] + x];
+          } else {
+            const int bb = small_div(j, per, rper);
+            v[u] = src[mq + (long long)bb * nvols * m + (j - bb * per)];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int j = j0 + u * 256;
+        if (j < tot) {
+          int t;
+          if (!UNLIFT) {
+            const int rr = small_div(j, wx, rwx);
+            t = rr * WP + (j - rr * wx);
+          } else {
+            const int bb = small_div(j, per, rper), e = j - bb * per;
+            const int rr = small_div(e, b0, rb0);
+            t = rr * WP + bb * b0 + (e - rr * b0);
+          }
+          tile[t] = v[u];
+        }
+      }
+    }
+    __syncthreads();
+    for (int j0 = threadIdx.x; j0 < tot; j0 += 256 * NB) {
+      E v[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int j = j0 + u * 256;
+        if (j < tot) {
+          int t;
+          if (!UNLIFT) {
+            const int bb = small_div(j, per, rper), e = j - bb * per;
+            const int rr = small_div(e, b0, rb0);
+            t = rr * WP + bb * b0 + (e - rr * b0);
+          } else {
+            const int rr = small_div(j, wx, rwx);
+            t = rr * WP + (j - rr * wx);
+          }
+          v[u] = tile[t];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int j = j0 + u * 256;
+        if (j < tot) {
+          if (!UNLIFT) {
+            const int bb = small_div(j, per, rper);
+            dst[mq + (long long)bb * nvols * m + (j - bb * per)] = v[u];
+          } else {
+            const int rr = small_div(j, wx, rwx);
+            int x = xlo + (j - rr * wx) + sx;
+            if (x >= nx) x -= nx;
+            dst[vq + rowoff[rr] + x] = v[u];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // byte-granular fallback for element sizes other than 1/2/4/8/16
 __global__ void k_lift_bytes(const unsigned char* __restrict__ vols, unsigned char* __restrict__ mats, GeomArgs g,
                              long long nvols, long long start, long long total, long long eb, int dir) {
@@ -488,6 +608,27 @@ llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols,
   const long long nr = g.blk[1] * g.blk[2];
   const dim3 grid((unsigned)rows, (unsigned)std::min<size_t>((nvols + 3) / 4, 65535),
                   (unsigned)std::max<long long>(1, std::min<long long>(nr / 16, 64)));
+  // slab kernel while its tile fits in 32 KiB of LDS
+  const size_t slab_lds = (size_t)LB_RC * (size_t)(g.grid[0] * g.blk[0] + 1) * eb;
+  if ((eb == 4 || eb == 8 || eb == 16) && slab_lds <= 32768) {
+    const dim3 sgrid((unsigned)rows, (unsigned)std::min<size_t>((nvols + 3) / 4, 65535),
+                     (unsigned)((nr + LB_RC - 1) / LB_RC));
+#define SLAB_CASE(E)                                                                                        \
+  if (unlift)                                                                                               \
+    hipLaunchKernelGGL((k_lift_slab<E, true>), sgrid, dim3(256), slab_lds, s, (const E*)mats, (E*)(void*)vols, \
+                       g, (long long)nvols, (long long)start, (long long)n);                               \
+  else                                                                                                      \
+    hipLaunchKernelGGL((k_lift_slab<E, false>), sgrid, dim3(256), slab_lds, s, (const E*)vols, (E*)mats, g,  \
+                       (long long)nvols, (long long)start, (long long)n);
+    switch (eb) {
+      case 4: SLAB_CASE(uint32_t); break;
+      case 8: SLAB_CASE(uint2); break;
+      default: SLAB_CASE(uint4); break;
+    }
+#undef SLAB_CASE
+    LLR_HIP(hipGetLastError());
+    return LLR_OK;
+  }
   switch (eb) {
     case 1: LIFT_CASE(uint8_t); break;
     case 2: LIFT_CASE(uint16_t); break;
