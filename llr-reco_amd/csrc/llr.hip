@@ -98,34 +98,40 @@ __global__ void __launch_bounds__(256) k_lift_rows(const E* __restrict__ src, E*
   }
 }
 
-// Gather/scatter through LDS, one slab at a time. Workgroup (row, y, z) owns
-// grid row `row`, the block rows r0 .. r0 + LB_RC - 1 (r0 = LB_RC blockIdx.z)
-// and the volumes q = y, y + gridDim.y, ... For one volume the slab is LB_RC
-// volume rows of grid_x blk[0] voxels; for each block of the row the same
-// slab is LB_RC blk[0] consecutive matrix elements starting at row r0 blk[0].
-// Lift reads the volume rows whole (coalesced along x) into the LDS tile and
-// writes each block's run contiguously; un_lift does the reverse. Tile rows
-// are padded by one element so that the block-wise side does not hit one
-// LDS bank with every row. Matrix c, volume q, row i is at
+// Gather/scatter through LDS, one slab at a time. Workgroup (x, y, z) owns
+// the blocks [xc xbw, xc xbw + xbw) of grid row `row` (blockIdx.x = row nxc +
+// xc), the block rows r0 .. r0 + RC - 1 (r0 = RC blockIdx.z) and the volumes
+// q = y, y + gridDim.y, ... For one volume the slab is RC volume runs of
+// xbw blk[0] voxels; for each of its blocks the same slab is RC blk[0]
+// consecutive matrix elements starting at row r0 blk[0]. Lift reads the
+// volume runs (coalesced along x) into the LDS tile and writes each block's
+// run contiguously; un_lift does the reverse. Tile rows are padded by 8
+// elements, so the 8 x 8 elements a wavefront touches on the block side fall
+// on distinct LDS banks. Matrix c, volume q, row i is at
 // ((c - start) nvols + q) m + i (block.rs:147-184).
-constexpr int LB_RC = 8;
-template <class E, bool UNLIFT>
+constexpr int LB_PAD = 8;
+template <class E, bool UNLIFT, int RC>
 __global__ void __launch_bounds__(256) k_lift_slab(const E* __restrict__ src, E* __restrict__ dst, GeomArgs g,
-                                                    long long nvols, long long start, long long n) {
+                                                    long long nvols, long long start, long long n, int nxc, int xbw) {
+  static_assert(RC <= 256, "row offsets are filled by one thread each");
   extern __shared__ unsigned char lift_lds[];
   E* tile = reinterpret_cast<E*>(lift_lds);
-  __shared__ long long rowoff[LB_RC];
+  __shared__ long long rowoff[RC];
   constexpr int NB = 8;  // elements per thread loaded before the first store
   const int gx = (int)g.grid[0], gyy = (int)g.grid[1];
   const int b0 = (int)g.blk[0], b1 = (int)g.blk[1], m = (int)g.m;
   const int nx = (int)g.vol[0], ny = (int)g.vol[1], nz = (int)g.vol[2];
-  const int WP = gx * b0 + 1;
-  const long long row = blockIdx.x + start / gx;
+  const int WP = min(gx, xbw) * b0 + LB_PAD;
+  const long long rowi = blockIdx.x / nxc;
+  const int xc = (int)(blockIdx.x - rowi * nxc);
+  const long long row = rowi + start / gx;
   const long long c0 = row * gx;
   const int by = (int)(row % gyy), bz = (int)(row / gyy);
-  const int r0 = (int)blockIdx.z * LB_RC;
-  const int rc = min(LB_RC, b1 * (int)g.blk[2] - r0);
-  const int bxlo = (int)max(0LL, start - c0), bxhi = (int)min((long long)gx, start + n - c0);
+  const int r0 = (int)blockIdx.z * RC;
+  const int rc = min(RC, b1 * (int)g.blk[2] - r0);
+  const int bxlo = (int)max((long long)xc * xbw, start - c0);
+  const int bxhi = (int)min((long long)min(gx, xc * xbw + xbw), start + n - c0);
+  if (bxlo >= bxhi) return;  // the whole workgroup, before any barrier
   if (threadIdx.x < rc) {
     const int r = r0 + (int)threadIdx.x, iz = r / b1, iy = r - iz * b1;
     int y = by * b1 + iy + (int)g.shift[1], z = bz * (int)g.blk[2] + iz + (int)g.shift[2];
@@ -608,24 +614,45 @@ llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols,
   const long long nr = g.blk[1] * g.blk[2];
   const dim3 grid((unsigned)rows, (unsigned)std::min<size_t>((nvols + 3) / 4, 65535),
                   (unsigned)std::max<long long>(1, std::min<long long>(nr / 16, 64)));
-  // slab kernel while its tile fits in 32 KiB of LDS
-  const size_t slab_lds = (size_t)LB_RC * (size_t)(g.grid[0] * g.blk[0] + 1) * eb;
-  if ((eb == 4 || eb == 8 || eb == 16) && slab_lds <= 32768) {
-    const dim3 sgrid((unsigned)rows, (unsigned)std::min<size_t>((nvols + 3) / 4, 65535),
-                     (unsigned)((nr + LB_RC - 1) / LB_RC));
-#define SLAB_CASE(E)                                                                                        \
-  if (unlift)                                                                                               \
-    hipLaunchKernelGGL((k_lift_slab<E, true>), sgrid, dim3(256), slab_lds, s, (const E*)mats, (E*)(void*)vols, \
-                       g, (long long)nvols, (long long)start, (long long)n);                               \
-  else                                                                                                      \
-    hipLaunchKernelGGL((k_lift_slab<E, false>), sgrid, dim3(256), slab_lds, s, (const E*)vols, (E*)mats, g,  \
-                       (long long)nvols, (long long)start, (long long)n);
+  // Slab shapes: the side a direction writes gets the whole runs. Lift writes
+  // the block side and takes 64 block rows of 8 blocks (a whole 4 KB block
+  // run per volume for 8^3 complex64 blocks); un_lift writes the volume side
+  // and takes 8 block rows of the whole grid row (whole volume rows). Either
+  // shape falls back to the other, then to the grid-row kernel, when its LDS
+  // tile would exceed 64 KiB.
+  auto slab_bytes = [&](int rc, long long xb) {
+    return (size_t)rc * (size_t)(std::min<long long>(xb, g.grid[0]) * g.blk[0] + LB_PAD) * eb;
+  };
+  const bool tall_ok = slab_bytes(64, 8) <= 65536, wide_ok = slab_bytes(8, g.grid[0]) <= 65536;
+  int shape = 0;  // 1: 64 block rows x 8 blocks; 2: 8 block rows x the grid row
+  if (eb == 4 || eb == 8 || eb == 16) shape = unlift ? (wide_ok ? 2 : tall_ok ? 1 : 0) : (tall_ok ? 1 : wide_ok ? 2 : 0);
+  if (shape) {
+    const int rc = shape == 1 ? 64 : 8;
+    const int xbw = shape == 1 ? (int)std::min<long long>(8, g.grid[0]) : (int)g.grid[0];
+    const int nxc = (int)((g.grid[0] + xbw - 1) / xbw);
+    const size_t slab_lds = slab_bytes(rc, xbw);
+    const dim3 sgrid((unsigned)(rows * nxc), (unsigned)std::min<size_t>((nvols + 3) / 4, 65535),
+                     (unsigned)((nr + rc - 1) / rc));
+#define SLAB_LAUNCH(E, RC)                                                                                     \
+  if (unlift)                                                                                                  \
+    hipLaunchKernelGGL((k_lift_slab<E, true, RC>), sgrid, dim3(256), slab_lds, s, (const E*)mats,              \
+                       (E*)(void*)vols, g, (long long)nvols, (long long)start, (long long)n, nxc, xbw);        \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_lift_slab<E, false, RC>), sgrid, dim3(256), slab_lds, s, (const E*)vols, (E*)mats, g, \
+                       (long long)nvols, (long long)start, (long long)n, nxc, xbw);
+#define SLAB_CASE(E)     \
+  if (shape == 1) {      \
+    SLAB_LAUNCH(E, 64)   \
+  } else {               \
+    SLAB_LAUNCH(E, 8)    \
+  }
     switch (eb) {
       case 4: SLAB_CASE(uint32_t); break;
       case 8: SLAB_CASE(uint2); break;
       default: SLAB_CASE(uint4); break;
     }
 #undef SLAB_CASE
+#undef SLAB_LAUNCH
     LLR_HIP(hipGetLastError());
     return LLR_OK;
   }
