@@ -27,7 +27,10 @@ def rel(a, b):
 @pytest.mark.parametrize("vol,blk,shift,start,n", [((10, 12, 14), (1, 1, 4), (0, 0, 12), 0, None),
                                                     ((64, 64, 64), (8, 8, 8), (61, 3, 40), 100, 300),
                                                     ((9, 7, 5), (3, 2, 2), (8, 6, 4), 3, None),
-                                                    ((256, 256, 128), (8, 8, 8), (255, 129, 127), 16000, 384)])
+                                                    ((256, 256, 128), (8, 8, 8), (255, 129, 127), 16000, 384),
+                                                    # 1024-voxel rows: 16-byte elements outgrow both LDS
+                                                    # slab shapes and take the grid-row kernel
+                                                    ((1024, 4, 4), (8, 2, 2), (5, 1, 3), 10, 200)])
 def test_lift_unlift_bit_exact(llr, dtype, vol, blk, shift, start, n):
     from llr_hip import block
 
