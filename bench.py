@@ -49,43 +49,9 @@ def measured_traffic(stage, ds, form):
 
 def build_measurements(preset, torch, llr, keep_truth=False):
     """Synthetic data set of the preset with its measurements on the GPU."""
-    import numpy as np
+    from llr_hip import phantom
 
-    from llr_hip import phantom, signal_model
-
-    cfg = dict(phantom.PRESETS[preset]) if isinstance(preset, str) else dict(preset)
-    s_ph, s_noise, s_mask, s_shift = cfg["seeds"]
-    dirs = phantom.six_directions() if cfg["dirs"] == "six" else phantom.repulsion_directions(cfg["n_dirs"], s_ph + 1000)
-    bvals, bvecs = phantom.b_table(cfg["n_b0"], dirs, cfg["bval"])
-    nq = len(bvals)
-    vol = cfg["vol"]
-    N = int(np.prod(vol))
-    t = time.time()
-    img, geo = phantom.phantom_image(vol, bvals, bvecs, s_ph, cfg["n_ell"], return_truth=True)
-    s0_obj = float(np.mean(np.abs(img[:N][geo["labels"].ravel(order="F") > 0])))
-    log(f"phantom {vol} x{nq}: {time.time() - t:.1f} s")
-    dev = torch.device("cuda", 0)
-    x = torch.from_numpy(img).to(dev)
-    truth = x.clone() if keep_truth else None
-    del img
-    torch.cuda.synchronize()
-    zero = np.zeros((nq, 3), np.float32)
-    signal_model.signal_model_device(x.data_ptr(), x.numel(), vol, nq, zero, zero, signal_model.ModelDirection.Forward)
-    g = torch.Generator(device=dev)
-    g.manual_seed(s_noise)
-    sigma = s0_obj / 30.0
-    noise = torch.randn(2 * x.numel(), device=dev, dtype=torch.float32, generator=g) * (sigma / np.sqrt(2.0))
-    x += torch.view_as_complex(noise.view(-1, 2))
-    del noise
-    mask = phantom.pe_mask(vol[1], vol[2], nq, cfg["R"], s_mask)  # (ny, nz, nq)
-    m = torch.from_numpy(np.ascontiguousarray(np.real(mask).transpose(2, 1, 0))).to(dev)  # (nq, nz, ny)
-    xv = x.view(nq, vol[2], vol[1], vol[0])
-    xv *= (m != 0).to(torch.complex64)[..., None]
-    torch.cuda.synchronize()
-    sub = phantom.loglinear_subspace(bvals, bvecs, cfg["srank"]) if cfg["srank"] else None
-    shifts = phantom.shift_sequence(vol, cfg["n_iter"], s_shift)
-    log(f"measurements on device: {time.time() - t:.1f} s, undersampling R={cfg['R']}")
-    return dict(y=x, vol=vol, nq=nq, cfg=cfg, subspace=sub, shifts=shifts, truth=truth, bvals=bvals, bvecs=bvecs)
+    return phantom.device_dataset(preset, torch, keep_truth=keep_truth, log=log)
 
 
 def stage_bytes(ds, eb=8):

@@ -189,7 +189,7 @@ llr_status llr_fftn_batch(llr_ctx* ctx, const size_t dims[3], size_t batches, in
  * ceil(d/2) along each of the (up to 3) dims, generic element size. */
 llr_status llr_fftshift(llr_ctx* ctx, void* data, size_t elem_bytes, const size_t dims[3]);
 llr_status llr_ifftshift(llr_ctx* ctx, void* data, size_t elem_bytes, const size_t dims[3]);
-/* phase_shift3 (fftshift.rs:293-318): x *= exp(i*sign*pi*sum_a (i_a - n_a/2)),
+/* phase_shift3 (fftshift.rs:42-74): x *= exp(i*sign*pi*sum_a (i_a - n_a/2)),
  * sign -1 for LLR_FORWARD, +1 for LLR_INVERSE, f32 phase as the reference. */
 llr_status llr_phase_shift3(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t dims[3], int direction);
 

@@ -646,7 +646,7 @@ struct llr_recon {
   DevBuf Yx, flags;
   RampTables tabs;
   unsigned long long llr_blocks = 0;  // blocks processed by the LLR step (mixed-form statistics)
-  size_t nparts = 0;
+  size_t parts_cap = 0;  // doubles in partials
   bool have_meas = false, have_init = false;
   bool prepared = false;  // set-up done (llr_recon_prepare / warm start) and not yet consumed by a run
   double lambda = 0.0, energy = 0.0;
@@ -763,11 +763,11 @@ llr_status data_consistency(llr_recon* r, double* err) {
                        h);
   }
   if (st) return st;
+  size_t np = 0;
   {
     StageTimer t(r, 5);
-    size_t np = 0;
-    st = fft_zproj(ctx, X, Ym, p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(), r->partials.as<double>(), &np,
-                   h);
+    st = fft_zproj(ctx, X, Ym, p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(), r->partials.as<double>(),
+                   r->parts_cap, &np, h);
   }
   if (st) return st;
   {
@@ -783,7 +783,7 @@ llr_status data_consistency(llr_recon* r, double* err) {
   }
   if (st) return st;
   StageTimer t(r, 6);
-  return reduce_strided(ctx, r->partials.as<double>(), r->nparts, 1, err);
+  return reduce_strided(ctx, r->partials.as<double>(), np, 1, err);
 }
 
 
@@ -836,6 +836,8 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
   if (p.storage != LLR_STORE_FP32 && p.storage != LLR_STORE_FP16) return fail(LLR_ERR_ARG, "unknown storage");
   if (p.storage == LLR_STORE_FP16 && (p.precision != LLR_PREC_MIXED || !p.srank))
     return fail(LLR_ERR_UNSUPPORTED, "half storage needs the mixed-precision form and a subspace");
+  if (p.storage == LLR_STORE_FP16 && (p.vol[0] * p.vol[1] * p.vol[2] * p.nq) % 2)
+    return fail(LLR_ERR_UNSUPPORTED, "half storage needs an even number of samples (nx*ny*nz*nq)");
   LLR_HIP(hipSetDevice(ctx->device));
   llr_recon* r = new llr_recon();
   r->ctx = ctx;
@@ -877,11 +879,12 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
     if ((st = upload(r->t2, nullptr, te * p.nq * 8))) return bail(st);
   }
   if ((st = make_tables(r->tabs, p.vol, p.nq, nullptr, nullptr))) return bail(st);  // pk is redone by every set-up
+  // partial sums: the fused z pass's, at any tiles-per-workgroup setting (the
+  // batch autotuner may change it), or the set-up's energy/arg-max pass
   size_t np = 0;
-  fft_zproj(ctx, nullptr, nullptr, p.vol, p.nq, nullptr, nullptr, nullptr, &np, r->s16);
-  r->nparts = np;
-  if ((st = upload(r->partials, nullptr, std::max<size_t>(np, 2 * energy_argmax_parts(r->N, p.nq)) * 8)))
-    return bail(st);
+  fft_zproj(ctx, nullptr, nullptr, p.vol, p.nq, nullptr, nullptr, nullptr, 0, &np, r->s16);
+  r->parts_cap = std::max<size_t>(np, 2 * energy_argmax_parts(r->N, p.nq));
+  if ((st = upload(r->partials, nullptr, r->parts_cap * 8))) return bail(st);
   if ((st = upload(r->stats, nullptr, 16))) return bail(st);
   if ((st = upload(r->hist, nullptr, std::max<size_t>(p.n_iter, 1) * 8))) return bail(st);
   if ((st = upload(r->smax, nullptr, 8))) return bail(st);
@@ -925,7 +928,15 @@ llr_status llr_recon_set_initial(llr_recon* r, const llr_cf32* x, size_t len, in
   }
   LLR_HIP(hipMemcpyAsync(r->s16 ? r->Xin.p : r->X.p, x, len * 8,
                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, r->ctx->stream));
-  r->xin_pending = r->s16;
+  // Half storage holds values at the scale of the last set-up. Before a set-up
+  // the start waits in complex64 and the set-up converts it at its scale;
+  // after llr_recon_prepare (the next run skips the set-up) it is converted now
+  // at the scale in force.
+  r->xin_pending = r->s16 && !r->prepared;
+  if (r->s16 && r->prepared) {
+    llr_status st = to_h16_device(r->ctx, r->Xin.as<float2>(), r->X.p, r->D, (float)r->hs);
+    if (st) return st;
+  }
   LLR_HIP(hipStreamSynchronize(r->ctx->stream));
   r->have_init = true;
   return LLR_OK;
@@ -999,7 +1010,7 @@ static llr_status recon_setup(llr_recon* r) {
       r->hybrid = bad == 0;
       convert = r->s16 && !r->hybrid;
     }
-    if (convert && (st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs))) return st;
+    if (convert && (st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs, true))) return st;
     // the initial image, if one was set
     if (r->s16 && r->have_init && r->xin_pending &&
         (st = to_h16_device(ctx, r->Xin.as<float2>(), r->X.p, r->D, (float)r->hs)))
@@ -1170,11 +1181,23 @@ llr_status llr_recon_set_coefficients(llr_recon* r, const llr_cf32* c, size_t le
     LLR_HIP(hipMemcpyAsync(r->Cb.p, c, len * 8, hipMemcpyHostToDevice, r->ctx->stream));
     src = r->Cb.as<float2>();
   }
-  llr_status st = backproject_all(r, src, (float)r->hs);
+  llr_status st;
+  if (r->s16 && !r->prepared) {
+    // half storage before a set-up: the scale is not known yet, so the start
+    // C S^H waits in complex64 and the set-up converts it at its scale (as
+    // llr_recon_set_initial)
+    if (!r->Xin.p && (st = upload(r->Xin, nullptr, r->D * 8))) return st;
+    const size_t zero[3] = {0, 0, 0};
+    st = backproject_device(r->ctx, src, r->Xin.p, r->S.as<float2>(), r->p.vol, r->p.vol, zero, (int)r->nq,
+                            (int)r->p.srank, false, false, 1.f);
+    r->xin_pending = true;
+  } else {
+    st = backproject_all(r, src, (float)r->hs);
+    r->xin_pending = false;
+  }
   if (st) return st;
   LLR_HIP(hipStreamSynchronize(r->ctx->stream));
   r->have_init = true;
-  r->xin_pending = false;
   return LLR_OK;
 }
 
@@ -1483,7 +1506,7 @@ static llr_status batch_post(llr_recon* r, const unsigned* iter, double* tmp) {
     return st;
   size_t np = 0;
   if ((st = fft_zproj(ctx, X, meas_buffer(r), p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(),
-                      r->partials.as<double>(), &np)))
+                      r->partials.as<double>(), r->parts_cap, &np)))
     return st;
   if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, true, T.pi[1].as<float2>(), false, T.pk[1].as<float2>(), true,
                           sy)))

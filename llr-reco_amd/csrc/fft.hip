@@ -334,6 +334,7 @@ struct ZProjArgs {
   float scale;         // 1/sqrt(nz)
   const float2* tw;
   double* partial;     // one partial sum of |x - y|^2 per workgroup
+  long long cap;       // doubles available at partial (host-side check, not read by the kernel)
 };
 
 // Forward z pass -> hard projection -> adjoint z pass on tiles of z lines.
@@ -517,11 +518,13 @@ template <int R1, int R2, class ST, int MINT>
 hipError_t launch_zproj_as(ZProjArgs a, long long W, long long nvols, hipStream_t s, size_t* nparts) {
   using L = Layout<R1, R2, MINT>;
   a.tiles = (a.U + L::T - 1) / L::T;
-  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;
+  // size query: one tile per workgroup gives the most workgroups of any setting
+  const int want = a.partial == nullptr ? 1 : a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;
   a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
   dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
   *nparts = (size_t)grid.x * grid.y * grid.z;
-  if (a.partial == nullptr) return hipSuccess;  // size query
+  if (a.partial == nullptr) return hipSuccess;
+  if ((long long)*nparts > a.cap) return hipErrorInvalidValue;  // partial sums would not fit
   hipLaunchKernelGGL((k_fft_z_project<R1, R2, ST, MINT>), grid, dim3(L::THREADS), 0, s, a);
   return hipGetLastError();
 }
@@ -687,7 +690,7 @@ llr_status fft_x_inverse_checked(llr_ctx* ctx, const float2* src, void* dst, boo
 // per workgroup into `partial` (size returned in nparts; partial == null only
 // queries the size).
 llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* dims, size_t nvols,
-                     const float2* pk, const float2* pi, double* partial, size_t* nparts, bool h16) {
+                     const float2* pk, const float2* pi, double* partial, size_t cap, size_t* nparts, bool h16) {
   int n = (int)dims[2];
   const float2* tw = nullptr;
   if (partial) {
@@ -708,6 +711,7 @@ llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* d
   a.scale = (float)(1.0 / std::sqrt((double)nz));
   a.tw = tw;
   a.partial = partial;
+  a.cap = (long long)cap;
   a.tiles_per_wg = ctx->fft_tpw[2];
   hipError_t e;
   switch (n) {
@@ -730,6 +734,9 @@ llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* d
     case 384: e = launch_zproj<12, 32>(a, ny, nvols, h16, ctx->stream, nparts); break;
     default: return fail(LLR_ERR_UNSUPPORTED, "z length " + std::to_string(n) + " is not a supported FFT length");
   }
+  if (e == hipErrorInvalidValue && partial && *nparts > cap)
+    return fail(LLR_ERR_ARG, "fused z pass needs " + std::to_string(*nparts) + " partial sums, buffer holds " +
+                                 std::to_string(cap));
   if (e != hipSuccess) return hip_fail(e, "fft z projection launch");
   return LLR_OK;
 }

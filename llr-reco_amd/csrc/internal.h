@@ -21,8 +21,13 @@ llr_status fft_axis_pass(llr_ctx* ctx, const void* src, void* dst, const size_t*
 // nor all zero). *flag is not cleared.
 llr_status fft_x_inverse_checked(llr_ctx* ctx, const float2* src, void* dst, bool h16_out, const size_t* dims,
                                  size_t nvols, const float2* pre, const float2* post, float scale, unsigned* flag);
+// Fused forward-z / hard projection / adjoint-z: one partial error sum per
+// workgroup into partial (cap doubles; the launch fails if it would need
+// more). partial == null queries the largest count any tiles-per-workgroup
+// setting of ctx can produce (the buffer size to allocate).
 llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* dims, size_t nvols,
-                     const float2* pk, const float2* pi, double* partial, size_t* nparts, bool h16 = false);
+                     const float2* pk, const float2* pi, double* partial, size_t cap, size_t* nparts,
+                     bool h16 = false);
 
 // llr.hip
 // One reconstruction's share of the persistent multi-reconstruction LLR launch.
@@ -110,8 +115,10 @@ llr_status shrink_all_device(llr_ctx* ctx, float2* x, size_t n, float alpha);
 // arg-max found by argmax_volumes (integer corrections, exact reduction)
 llr_status ramp_from_argmax_device(llr_ctx* ctx, float2* tab, const size_t* dims, int axis, size_t nvols,
                                    const unsigned long long* best);
-// complex64 -> half-precision pairs (x s) and back (x s); n complex values, n even
-llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s);
+// complex64 -> half-precision pairs (x s) and back (x s); n complex values, n even.
+// keep_nonzero: a nonzero value never becomes zero (measurements: the hard
+// projection's measured set is the set of nonzero samples)
+llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s, bool keep_nonzero = false);
 llr_status from_h16_device(llr_ctx* ctx, const void* src, float2* dst, size_t n, float s);
 llr_status energy_partials(llr_ctx* ctx, const float2* y, size_t n, double* partials, size_t nparts);
 llr_status reduce_strided(llr_ctx* ctx, const double* partials, size_t n, size_t stride, double* out);

@@ -49,9 +49,10 @@ __global__ void __launch_bounds__(256) k_lift_rows(const E* __restrict__ src, E*
   // this workgroup's block rows [rlo, rhi): grid.z splits them in runs of RU
   const int rper = ((nr + (int)gridDim.z - 1) / (int)gridDim.z + RU - 1) / RU * RU;
   const int rlo = blockIdx.z * rper, rhi = min(nr, rlo + rper);
-  const float rb0 = 1.f / (float)b0;
+  // rows may be wider than small_div's exact range (2^20): an integer divide,
+  // once per voxel column
   for (int xl = threadIdx.x; xl < gx * b0; xl += blockDim.x) {
-    const int bx = small_div(xl, b0, rb0), ix = xl - bx * b0;
+    const int bx = xl / b0, ix = xl - bx * b0;
     const long long c = row * gx + bx;
     if (c < start || c >= start + n) continue;
     int x = bx * b0 + ix + (int)g.shift[0];

@@ -202,12 +202,29 @@ __global__ void __launch_bounds__(256) k_ramp_argmax(float2* __restrict__ tab, i
   }
 }
 
+// One complex value to a half pair. KEEP_NZ: a nonzero value whose two parts
+// both round to zero is stored as the smallest subnormal half (2^-24, sign
+// kept) in its larger part, so a sample that the hard projection treats as
+// measured (y != 0, lib.rs:396) stays measured in half storage.
+template <bool KEEP_NZ>
+__device__ __forceinline__ unsigned pack_h16(float re, float im) {
+  unsigned u = StoreF16::pack(make_float2(re, im));
+  if (KEEP_NZ && (u & 0x7FFF7FFFu) == 0u && (re != 0.f || im != 0.f)) {
+    const bool use_re = fabsf(re) >= fabsf(im);
+    const float v = use_re ? re : im;
+    const unsigned h = (v < 0.f ? 0x8001u : 0x0001u);
+    u = use_re ? h : (h << 16);
+  }
+  return u;
+}
+
 // complex64 <-> half-precision-pair storage, scaled by s on the way (two values per thread)
+template <bool KEEP_NZ>
 __global__ void __launch_bounds__(256) k_to_h16(const float4* __restrict__ src, uint2* __restrict__ dst, long long n2,
                                                  float s) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2; i += (long long)gridDim.x * blockDim.x) {
     float4 v = src[i];
-    dst[i] = make_uint2(StoreF16::pack(make_float2(v.x * s, v.y * s)), StoreF16::pack(make_float2(v.z * s, v.w * s)));
+    dst[i] = make_uint2(pack_h16<KEEP_NZ>(v.x * s, v.y * s), pack_h16<KEEP_NZ>(v.z * s, v.w * s));
   }
 }
 
@@ -251,7 +268,7 @@ __global__ void k_circshift_bytes(const unsigned char* __restrict__ src, unsigne
   }
 }
 
-// fftshift.rs:285-318: phase pi*(i - n/2) per axis in f32, summed x + y + z
+// fftshift.rs:42-74: phase pi*(i - n/2) per axis in f32, summed x + y + z
 __global__ void k_phase_shift3(float2* x, long long d0, long long d1, long long d2, float sign) {
   const float PI_F = 3.14159265358979323846f;
   long long total = d0 * d1 * d2;
@@ -330,10 +347,14 @@ llr_status ramp_from_argmax_device(llr_ctx* ctx, float2* tab, const size_t* dims
   return LLR_OK;
 }
 
-llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s) {
+llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s, bool keep_nonzero) {
   if (n % 2) return fail(LLR_ERR_UNSUPPORTED, "half-precision conversion needs an even number of values");
-  hipLaunchKernelGGL(k_to_h16, dim3(grid_for(n / 2, 256, 65536u)), dim3(256), 0, ctx->stream, (const float4*)src,
-                     (uint2*)dst, (long long)(n / 2), s);
+  if (keep_nonzero)
+    hipLaunchKernelGGL(k_to_h16<true>, dim3(grid_for(n / 2, 256, 65536u)), dim3(256), 0, ctx->stream,
+                       (const float4*)src, (uint2*)dst, (long long)(n / 2), s);
+  else
+    hipLaunchKernelGGL(k_to_h16<false>, dim3(grid_for(n / 2, 256, 65536u)), dim3(256), 0, ctx->stream,
+                       (const float4*)src, (uint2*)dst, (long long)(n / 2), s);
   LLR_HIP(hipGetLastError());
   return LLR_OK;
 }

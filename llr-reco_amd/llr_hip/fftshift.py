@@ -43,7 +43,7 @@ def ifftshift(dims, data, ctx=None):
 
 
 def phase_shift3(dims, x, direction, ctx=None):
-    """fftshift.rs:293-318: FOV/2 phase modulation, in place."""
+    """fftshift.rs:42-74: FOV/2 phase modulation, in place."""
     X = _lib.as_cf32(x, "x")
     d = _dims3(dims)
     _lib.check(_lib.load().llr_phase_shift3(_lib.ctx_handle(ctx), _lib.ptr(X), ctypes.c_size_t(X.size), _lib.sz3(d),

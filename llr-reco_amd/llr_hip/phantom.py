@@ -274,3 +274,51 @@ def dataset(preset, forward, vol=None, nq_override=None):
     shifts = shift_sequence(v, cfg["n_iter"], s_shift)
     return dict(meas=ksp, truth=img, subspace=sub, shifts=shifts, mask=mask, bvals=bvals, bvecs=bvecs, cfg=cfg,
                 nq=nq, vol=v)
+
+
+def device_dataset(preset, torch, keep_truth=False, log=None):
+    """The measurement set of `preset` built on GPU 0, for workloads too large
+    for the numpy path: the numpy phantom image, the library's forward signal
+    model (signal_model.rs:34-79, zero corrections) on the device, complex
+    Gaussian noise from a seeded torch generator, the phase-encode mask.
+    Returns dict(y (device complex64), vol, nq, cfg, subspace, shifts, truth
+    (device, if keep_truth), bvals, bvecs, mask)."""
+    import time
+
+    from . import signal_model
+
+    say = log or (lambda *a: None)
+    cfg = dict(PRESETS[preset]) if isinstance(preset, str) else dict(preset)
+    s_ph, s_noise, s_mask, s_shift = cfg["seeds"]
+    dirs = six_directions() if cfg["dirs"] == "six" else repulsion_directions(cfg["n_dirs"], s_ph + 1000)
+    bvals, bvecs = b_table(cfg["n_b0"], dirs, cfg["bval"])
+    nq = len(bvals)
+    vol = tuple(cfg["vol"])
+    N = int(np.prod(vol))
+    t = time.time()
+    img, geo = phantom_image(vol, bvals, bvecs, s_ph, cfg["n_ell"], return_truth=True)
+    s0_obj = float(np.mean(np.abs(img[:N][geo["labels"].ravel(order="F") > 0])))
+    say(f"phantom {vol} x{nq}: {time.time() - t:.1f} s")
+    dev = torch.device("cuda", 0)
+    x = torch.from_numpy(img).to(dev)
+    truth = x.clone() if keep_truth else None
+    del img
+    torch.cuda.synchronize()
+    zero = np.zeros((nq, 3), np.float32)
+    signal_model.signal_model_device(x.data_ptr(), x.numel(), vol, nq, zero, zero, signal_model.ModelDirection.Forward)
+    g = torch.Generator(device=dev)
+    g.manual_seed(s_noise)
+    sigma = s0_obj / 30.0
+    noise = torch.randn(2 * x.numel(), device=dev, dtype=torch.float32, generator=g) * (sigma / np.sqrt(2.0))
+    x += torch.view_as_complex(noise.view(-1, 2))
+    del noise
+    mask = pe_mask(vol[1], vol[2], nq, cfg["R"], s_mask)  # (ny, nz, nq)
+    m = torch.from_numpy(np.ascontiguousarray(np.real(mask).transpose(2, 1, 0))).to(dev)  # (nq, nz, ny)
+    xv = x.view(nq, vol[2], vol[1], vol[0])
+    xv *= (m != 0).to(torch.complex64)[..., None]
+    torch.cuda.synchronize()
+    sub = loglinear_subspace(bvals, bvecs, cfg["srank"]) if cfg["srank"] else None
+    shifts = shift_sequence(vol, cfg["n_iter"], s_shift)
+    say(f"measurements on device: {time.time() - t:.1f} s, undersampling R={cfg['R']}")
+    return dict(y=x, vol=vol, nq=nq, cfg=cfg, subspace=sub, shifts=shifts, truth=truth, bvals=bvals, bvecs=bvecs,
+                mask=mask)

@@ -12,9 +12,9 @@
 //                which is how cuSOLVER gesvda forms its approximate SVD.
 //
 // Pinning (see tests/test_oracle_*.py): brute-force index formulas, the
-// reference's KATs (low_rank.rs:387-461 cgemm, cublas.rs:139-183 diag,
+// reference's KATs (low_rank.rs:107-180 cgemm, cublas.rs:139-183 diag,
 // dti_subspace.rs:239-261 identity subspace, block.rs:285-343 round trip,
-// fftshift.rs:370-378 wrap, cufft.rs:415-422 FFT round trip), numpy FP64
+// fftshift.rs:127-134 wrap, cufft.rs:415-422 FFT round trip), numpy FP64
 // FFT/SVD, closed-form db2 filters and perfect reconstruction.
 //
 // Parity status: every function is pinned except the multi-level wavelet's
@@ -738,7 +738,7 @@ static void circshift3(E* data, const size_t* dims, const long long* sh) {
       }
 }
 
-// fftshift.rs:285-318 phase_shift3: phase pi*(i - n/2) per axis, f32 as the reference
+// fftshift.rs:42-74 phase_shift3: phase pi*(i - n/2) per axis, f32 as the reference
 static void phase_shift3_f32(C<float>* x, const size_t* dims, int sign) {
   const float PI_F = 3.14159265358979323846f;
   for (size_t iz = 0; iz < dims[2]; ++iz)
@@ -772,6 +772,8 @@ struct orc_recon_params {
   const void* subspace;  // nq x srank complex, col-major (T precision)
   const size_t* shifts;  // n_iter x 3
   int partial_blocks;    // 1: ceil(vol/blk) grid with truncated edge blocks (extension)
+  size_t snap_iter;      // > 0: copy the iterate into snap after this many iterations
+  void* snap;            // N*nq complex (T precision), or null
 };
 
 namespace orc {
@@ -922,6 +924,7 @@ static int recon(const orc_recon_params* p, const C<T>* meas, C<T>* x, double* e
     double e = hard_project<T>(x, meas, N * nq);
     if (err_hist) err_hist[it] = e / energy;
     signal_model<T>(x, vol, nq, kc.data(), ic.data(), 1, p->phase_mode);
+    if (p->snap && p->snap_iter == it + 1) std::memcpy(p->snap, x, sizeof(C<T>) * N * nq);
   }
   return 0;
 }

@@ -262,11 +262,14 @@ class ReconParams(ctypes.Structure):
         ("subspace", ctypes.c_void_p),
         ("shifts", ctypes.POINTER(ctypes.c_size_t)),
         ("partial_blocks", ctypes.c_int),
+        ("snap_iter", ctypes.c_size_t),
+        ("snap", ctypes.c_void_p),
     ]
 
 
 def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda_factor=0.02,
-          wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, phase_mode=None, partial_blocks=False):
+          wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, phase_mode=None, partial_blocks=False,
+          snapshot_after=0):
     """Run the reconstruction loop of llr_recon_exec (lib.rs:227-341) on CPU.
 
     partial_blocks: ceil(vol/blk) block grid with truncated edge blocks (an
@@ -274,6 +277,8 @@ def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda
 
     meas: flat k-space measurements (complex64 -> FP32 path, complex128 -> FP64 oracle).
     shifts: (n_iter, 3) integer block shifts. Returns (image, err_hist, lambda, ksp_corr).
+    snapshot_after > 0: also return the iterate after that many iterations
+    (a fifth element).
     """
     s = _suf(meas)
     if phase_mode is None:
@@ -301,6 +306,11 @@ def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda
         p.subspace = None
     p.shifts = shifts.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t))
     p.partial_blocks = 1 if partial_blocks else 0
+    snap = None
+    if snapshot_after:
+        snap = np.zeros_like(meas)
+        p.snap_iter = int(snapshot_after)
+        p.snap = ctypes.c_void_p(snap.ctypes.data)
     x = np.zeros_like(meas)
     hist = np.zeros(max(1, shifts.shape[0]), np.float64)
     lam = ctypes.c_double()
@@ -308,6 +318,8 @@ def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda
     r = getattr(lib(), "orc_recon_" + s)(ctypes.byref(p), _p(meas), _p(x), _p(hist), ctypes.byref(lam), _p(kc))
     if r:
         raise ValueError(f"oracle recon rejected the parameters (code {r})")
+    if snap is not None:
+        return x, hist[:shifts.shape[0]], lam.value, kc.reshape(nq, 3), snap
     return x, hist[:shifts.shape[0]], lam.value, kc.reshape(nq, 3)
 
 

@@ -27,3 +27,16 @@ SMALL_PLAIN = dict(vol=(32, 32, 32), n_b0=1, dirs="six", n_dirs=6, bval=1000.0, 
 
 def make(preset):
     return phantom.dataset(preset, oracle_forward)
+
+
+def nrmse_big(a, b, chunk=1 << 24):
+    """nrmse for arrays of hundreds of millions of samples, in FP64 chunks."""
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
+    num = den = 0.0
+    for i in range(0, a.size, chunk):
+        bb = b[i:i + chunk].astype(np.complex128)
+        d = a[i:i + chunk].astype(np.complex128) - bb
+        num += float(np.vdot(d, d).real)
+        den += float(np.vdot(bb, bb).real)
+    return float(np.sqrt(num / den))

@@ -159,6 +159,24 @@ def test_identity_subspace_kat(llr):
     assert np.array_equal(flat, X.ravel())
 
 
+def test_subspace_constraint_vs_oracle(llr):
+    """dti_subspace.rs:89-134 with a complex, non-orthonormal subspace and complex
+    matrices: X' = conj(X) conj(S) S^T (the reference's final plain transpose),
+    against the oracle, whose form is pinned by that closed form."""
+    from llr_hip import dti_subspace
+
+    for nq, srank, nvox, b in ((67, 7, 512, 5), (13, 3, 1000, 4)):
+        S = crandn(nq * srank)
+        X = crandn(b * nq * nvox)
+        g = X.copy()
+        dti_subspace.subspace_constraint(S, g, b, nq, nvox, srank)
+        r = X.astype(np.complex128)
+        O.subspace_constraint(S.astype(np.complex128), r, b, nq, nvox, srank)
+        e = rel(g, r)
+        print(f"[subspace_constraint nq {nq} srank {srank} nvox {nvox} x{b}] rel {e:.2e}", flush=True)
+        assert e < 3e-6
+
+
 def test_project_backproject_transpose_vs_oracle(llr):
     from llr_hip import dti_subspace
 
@@ -311,7 +329,7 @@ def test_phase_shift3(llr):
 
 
 # ------------------------------------------------------------------ wavelet
-@pytest.mark.parametrize("dims", [(16, 12, 10), (15, 13, 11), (64, 64, 64), (256, 64, 32)])
+@pytest.mark.parametrize("dims", [(16, 12, 10), (15, 13, 11), (64, 64, 64), (256, 64, 32), (256, 256, 128)])
 @pytest.mark.parametrize("alpha", [0.0, 0.3])
 def test_wavelet_vs_oracle(llr, dims, alpha):
     from llr_hip import recon

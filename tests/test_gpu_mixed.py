@@ -131,7 +131,7 @@ def test_mixed_fp64_fallback_on_rank_deficient_blocks(llr):
     assert e <= 1e-5
 
 
-@pytest.mark.parametrize("vol", [(32, 32, 16), (48, 24, 16)])
+@pytest.mark.parametrize("vol", [(32, 32, 16), (48, 24, 16), (96, 96, 96)])
 def test_fp16_storage_matches_fp64(llr, vol):
     """Mixed form with half storage against the FP64 oracle, radix-2 and radix-3 sizes."""
     ds = H.make(dict(H.SMALL_SUBSPACE, vol=vol))
@@ -194,3 +194,119 @@ def test_fp16_storage_round_trips(llr, gain):
     print(f"[fp16 storage round trip, gain {gain:g}] max rel rounding {rel.max():.3e}; projection NRMSE {e:.3e}; "
           f"back-projection NRMSE {e2:.3e}", flush=True)
     assert e <= 1e-4 and e2 <= 1e-3
+
+
+def _fp16_recon(ds, storage="fp16", hybrid=True, n_iter=None):
+    from llr_hip import recon
+
+    cfg = ds["cfg"]
+    sh = ds["shifts"] if n_iter is None else ds["shifts"][:n_iter]
+    return recon.Recon(ds["vol"], ds["nq"], sh, blk=cfg["blk"], lambda_factor=cfg["lambda_factor"],
+                       wavelet_alpha=cfg["wavelet_alpha"], subspace=ds["subspace"], precision="mixed",
+                       storage=storage, hybrid=hybrid)
+
+
+def test_fp16_set_coefficients_before_prepare(llr):
+    """Coefficients set before any set-up (the storage scale is not known yet)
+    wait in complex64 and are converted at the set-up's scale: the image after
+    prepare() is C S^H to half rounding, in the caller's units."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, n_iter=1))
+    vol, nq, K = ds["vol"], ds["nq"], ds["cfg"]["srank"]
+    N = int(np.prod(vol))
+    rng = np.random.default_rng(8)
+    c = (30.0 * (rng.standard_normal(K * N) + 1j * rng.standard_normal(K * N))).astype(np.complex64)
+    S = np.asarray(ds["subspace"], np.complex128)
+    expect = (np.conj(S) @ c.astype(np.complex128).reshape(K, N)).ravel()
+    for storage in ("fp16", "fp32"):
+        r = _fp16_recon(ds, storage)
+        r.set_measurements(ds["meas"])
+        r.set_coefficients(c)
+        r.prepare()
+        img = r.image()
+        r.close()
+        e = H.nrmse(img, expect)
+        print(f"[set_coefficients before prepare, {storage}] NRMSE vs C S^H {e:.3e}", flush=True)
+        assert e <= (1e-3 if storage == "fp16" else 1e-6)
+
+
+def test_fp16_set_initial_after_prepare(llr):
+    """An initial image set after prepare() is the start of the next run (which
+    skips the set-up) in half storage as in complex64 storage."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, n_iter=2))
+    N = int(np.prod(ds["vol"])) * ds["nq"]
+    rng = np.random.default_rng(9)
+    x = (0.3 * (rng.standard_normal(N) + 1j * rng.standard_normal(N))).astype(np.complex64)
+    out = {}
+    for storage in ("fp16", "fp32"):
+        r = _fp16_recon(ds, storage)
+        r.set_measurements(ds["meas"])
+        r.prepare()
+        r.set_initial(x)
+        start = r.image()
+        r.run()
+        out[storage] = r.image()
+        r.close()
+        e0 = H.nrmse(start, x)
+        print(f"[set_initial after prepare, {storage}] start vs x {e0:.3e}", flush=True)
+        assert e0 <= (1e-3 if storage == "fp16" else 0.0)
+    cold = _fp16_recon(ds, "fp32")
+    cold.set_measurements(ds["meas"])
+    cold.run()
+    img_cold = cold.image()
+    cold.close()
+    e = H.nrmse(out["fp16"], out["fp32"])
+    ec = H.nrmse(out["fp32"], img_cold)
+    print(f"[set_initial after prepare] fp16 vs fp32 storage {e:.3e}; warm vs cold start {ec:.3e}", flush=True)
+    assert e <= 5e-3 and ec > 10 * e
+
+
+def test_fp16_full_3d_matches_fp64(llr):
+    """Half storage with the full 3D iteration: asked for (no_hybrid), and forced by
+    measurements whose x-lines are partly measured (single samples zeroed)."""
+    ds = H.make(H.SMALL_SUBSPACE)
+    x64 = fp64(ds)[0]
+    r = _fp16_recon(ds, hybrid=False)
+    r.set_measurements(ds["meas"])
+    h = r.run()
+    img = r.image()
+    r.close()
+    e = H.nrmse(img, x64)
+    print(f"[fp16 full 3D, no_hybrid] NRMSE vs fp64 {e:.3e}", flush=True)
+    assert e <= 5e-3
+    meas = ds["meas"].copy()
+    rng = np.random.default_rng(10)
+    meas[rng.random(meas.size) < 0.1] = 0
+    ds2 = dict(ds, meas=meas)
+    x64b = fp64(ds2)[0]
+    r = _fp16_recon(ds2)
+    r.set_measurements(meas)
+    r.run()
+    used = r.hybrid()
+    img = r.image()
+    r.close()
+    e2 = H.nrmse(img, x64b)
+    print(f"[fp16 full 3D, samples zeroed inside lines] hybrid {used}; NRMSE vs fp64 {e2:.3e}", flush=True)
+    assert not used and e2 <= 5e-3
+
+
+def test_fp16_keeps_tiny_measured_samples(llr):
+    """A measured sample whose half-precision value would round to zero stays
+    measured (y != 0 is the hard projection's sampled set, lib.rs:396): 5 % of
+    the measured samples scaled by 1e-9, full 3D iteration, half against
+    complex64 storage of the same mixed form."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, n_iter=3))
+    meas = ds["meas"].copy()
+    rng = np.random.default_rng(11)
+    nz = np.flatnonzero(meas)
+    tiny = nz[rng.random(nz.size) < 0.05]
+    meas[tiny] *= np.float32(1e-9)
+    imgs = {}
+    for storage in ("fp16", "fp32"):
+        r = _fp16_recon(dict(ds, meas=meas), storage, hybrid=False)
+        r.set_measurements(meas)
+        r.run()
+        imgs[storage] = r.image()
+        r.close()
+    e = H.nrmse(imgs["fp16"], imgs["fp32"])
+    print(f"[fp16 tiny measured samples, {tiny.size} of {nz.size}] NRMSE fp16 vs fp32 storage {e:.3e}", flush=True)
+    assert e <= 5e-3

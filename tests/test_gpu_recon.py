@@ -130,3 +130,12 @@ def test_recon_partial_equals_whole_when_divisible(llr):
     a = run_gpu(ds)[0]
     b = run_gpu(ds, partial_blocks=True)[0]
     assert np.array_equal(a, b)
+
+
+@pytest.mark.timeout(600)
+def test_recon_five_level_fused_subspace_wavelet(llr):
+    """96^3 x13 with a rank-7 subspace and db2 0.005: the wavelet runs
+    floor(log2(96/3)) = 5 levels, level 0 fused with the back-projection from the
+    subspace coefficients (the 256x256x128 workload's path), 2 iterations."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, vol=(96, 96, 96), n_iter=2))
+    check(ds, "96^3 x13 subspace7 wavelet 5 levels")

@@ -75,14 +75,14 @@ def test_lift_rejects_oversized_block():
 
 
 def test_coord_wrap_kat():
-    # fftshift.rs:370-378
+    # fftshift.rs:127-134
     idx = O.coord_to_col_maj_index([0, 0, -1], [256, 128, 256])
     assert idx == 255 * 256 * 128
 
 
 # ------------------------------------------------------ cgemm / subspace KATs
 def test_cgemm_kat():
-    # low_rank.rs:387-461: A(i,j) = i + ij (4x3), B(i,j) = j + ii (3x4), batch 2
+    # low_rank.rs:107-180: A(i,j) = i + ij (4x3), B(i,j) = j + ii (3x4), batch 2
     m, k, n, b = 4, 3, 4, 2
     A = np.zeros((b, k, m), np.complex64)
     B = np.zeros((b, n, k), np.complex64)
@@ -125,6 +125,25 @@ def test_identity_subspace_kat():
     flat = X.ravel().copy()
     O.subspace_constraint(S, flat, batch, nq, nvox, rank)
     assert np.array_equal(flat, X.ravel())
+
+
+def test_subspace_constraint_closed_form():
+    """dti_subspace.rs:89-134: C = S^H X^H (srank x nvox), B = S C (nq x nvox),
+    then the plain (non-conjugating) transpose of :128-133 gives X' = B^T =
+    conj(X) conj(S) S^T. Complex S and X: a dropped conjugation or a
+    conjugating final transpose fails here (the identity KAT with real X
+    cannot tell them apart)."""
+    nq, srank, nvox, b = 11, 4, 37, 3
+    S = crandn(nq, srank)
+    X = crandn(b, nq, nvox)  # [b][col q][row v]: matrix nvox x nq col-major
+    flat = X.ravel().copy()
+    O.subspace_constraint(S.ravel(order="F"), flat, b, nq, nvox, srank)
+    for bb in range(b):
+        Xm = X[bb].T  # nvox x nq
+        got = flat[bb * nq * nvox:(bb + 1) * nq * nvox].reshape(nq, nvox).T
+        expect = Xm.conj() @ S.conj() @ S.T
+        assert np.allclose(got, expect, atol=1e-12)
+        assert not np.allclose(got, Xm @ S @ S.conj().T, atol=1e-6)
 
 
 def test_project_backproject_is_x_s_sh():
