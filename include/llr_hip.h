@@ -315,11 +315,15 @@ llr_status llr_recon_hybrid(llr_recon* r, int* hybrid);
 llr_status llr_recon_mixed_stats(llr_recon* r, unsigned long long* fp64_blocks, unsigned long long* blocks);
 
 /* Stage timing with HIP events on the context's stream (for roofline reports).
- * Stages: 0 LLR block kernel, 1 subspace projection, 2 wavelet, 3 FFT x pass,
- * 4 FFT y pass, 5 FFT z pass + hard projection, 6 residual reduction, 7 run set-up
- * (statistics, corrections, zero-filled adjoint, threshold), 8 warm-start U-Net
- * prediction (packing, network, unpacking). */
-#define LLR_NUM_STAGES 9
+ * Stages: 0 LLR block kernel, 1 subspace projection, 2 wavelet (all its
+ * kernels), 3 FFT x pass (forward), 4 FFT y pass (forward), 5 FFT z pass + hard
+ * projection, 6 residual reduction, 7 run set-up (statistics, corrections,
+ * zero-filled adjoint, threshold), 8 warm-start U-Net prediction (packing,
+ * network, unpacking), 9 FFT y pass (adjoint), 10 FFT x pass (adjoint), and
+ * the wavelet's kernels, also counted in stage 2: 11 the level-0 kernel that
+ * reads the iterate or its subspace coefficients, 12 the level-0 kernels in
+ * between, 13 the level-0 kernel that writes the iterate, 14 levels >= 1. */
+#define LLR_NUM_STAGES 15
 llr_status llr_recon_enable_timing(llr_recon* r, int enable);
 llr_status llr_recon_stage_times(llr_recon* r, double* total_ms, size_t* launches);
 

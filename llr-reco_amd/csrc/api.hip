@@ -150,7 +150,7 @@ using namespace llr;
 extern "C" {
 
 const char* llr_last_error(void) { return g_err.c_str(); }
-int llr_abi_version(void) { return 105; }
+int llr_abi_version(void) { return 106; }
 
 llr_status llr_ctx_create(int device, llr_ctx** out) {
   if (!out) return fail(LLR_ERR_ARG, "null output pointer");
@@ -175,6 +175,8 @@ void llr_ctx_destroy(llr_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto& p : ctx->twiddles) (void)hipFree(p.second);
   if (ctx->d_red) (void)hipFree(ctx->d_red);
+  for (hipEvent_t e : ctx->kt_ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -771,13 +773,13 @@ llr_status data_consistency(llr_recon* r, double* err) {
   }
   if (st) return st;
   {
-    StageTimer t(r, 4);
+    StageTimer t(r, 9);
     st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, true, T.pi[1].as<float2>(), false, T.pk[1].as<float2>(), true, sy,
                        h);
   }
   if (st) return st;
   if (!r->hybrid) {
-    StageTimer t(r, 3);
+    StageTimer t(r, 10);
     st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false, T.pk[0].as<float2>(), true, sx,
                        h);
   }
@@ -902,6 +904,10 @@ void llr_recon_destroy(llr_recon* r) {
   if (!r) return;
   (void)hipSetDevice(r->ctx->device);
   (void)hipStreamSynchronize(r->ctx->stream);
+  if (r->ctx->kt_ms == r->stage_ms) {
+    r->ctx->kt_ms = nullptr;
+    r->ctx->kt_n = nullptr;
+  }
   if (r->ev0) (void)hipEventDestroy(r->ev0);
   if (r->ev1) (void)hipEventDestroy(r->ev1);
   delete r;
@@ -1296,6 +1302,14 @@ llr_status llr_recon_mixed_stats(llr_recon* r, unsigned long long* fp64_blocks, 
 llr_status llr_recon_enable_timing(llr_recon* r, int enable) {
   if (!r) return fail(LLR_ERR_ARG, "null recon");
   r->timing = enable != 0;
+  llr_ctx* c = r->ctx;
+  if (r->timing && !c->kt_ev[0]) {
+    LLR_HIP(hipSetDevice(c->device));
+    LLR_HIP(hipEventCreate(&c->kt_ev[0]));
+    LLR_HIP(hipEventCreate(&c->kt_ev[1]));
+  }
+  c->kt_ms = r->timing ? r->stage_ms : nullptr;
+  c->kt_n = r->timing ? r->stage_n : nullptr;
   for (int i = 0; i < LLR_NUM_STAGES; ++i) {
     r->stage_ms[i] = 0;
     r->stage_n[i] = 0;

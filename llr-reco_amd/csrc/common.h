@@ -90,6 +90,11 @@ struct llr_ctx {
   // launch-shape overrides chosen by the autotuner (0 / -1: built-in default)
   int fft_tpw[3] = {0, 0, 0};  // FFT tiles per workgroup, per axis (z: the fused projection pass)
   int llr_stage = -1;          // LLR block kernel: 1 stage the block in LDS, 0 read it from HBM twice
+  // per-kernel timing inside multi-kernel stages: while a reconstruction's
+  // stage timing is on these point at its stage tables (null otherwise)
+  double* kt_ms = nullptr;
+  size_t* kt_n = nullptr;
+  hipEvent_t kt_ev[2] = {nullptr, nullptr};
 };
 
 namespace llr {
@@ -147,5 +152,27 @@ inline unsigned grid_for(size_t total, unsigned block, unsigned cap = 65535u * 8
 
 // Reduction scratch of `n` doubles on the context's device.
 llr_status red_scratch(llr_ctx* ctx, size_t n, double** out);
+
+// Times the launches made in its scope into stage `id` of the context's
+// timing tables when they are set (HIP events on the context's stream; waits
+// for the launches, so only for timing runs).
+struct KernelTimer {
+  llr_ctx* c;
+  int id;
+  KernelTimer(llr_ctx* c_, int id_) : c(c_), id(id_) {
+    if (c->kt_ms) (void)hipEventRecord(c->kt_ev[0], c->stream);
+  }
+  ~KernelTimer() {
+    if (!c->kt_ms) return;
+    (void)hipEventRecord(c->kt_ev[1], c->stream);
+    (void)hipEventSynchronize(c->kt_ev[1]);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->kt_ev[0], c->kt_ev[1]);
+    c->kt_ms[id] += ms;
+    c->kt_n[id] += 1;
+  }
+};
+// stage ids of the per-kernel timers (see llr_recon_stage_times)
+constexpr int kStageWavIn = 11, kStageWavMid = 12, kStageWavOut = 13, kStageWavDeep = 14;
 
 }  // namespace llr

@@ -677,10 +677,13 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
     a1.f = f;
     int tiles_y = (ncy + AN_TCY - 1) / AN_TCY;
     dim3 g1((unsigned)(a1.groups_x * tiles_y), (unsigned)(in[2] * (fused ? (size_t)sub->K : nvols)));
-    if (fused)
-      hipLaunchKernelGGL(k_wav_xy_an<StoreF32>, g1, dim3(256), 0, s, a1);
-    else
-      hipLaunchKernelGGL(k_wav_xy_an<ST>, g1, dim3(256), 0, s, a1);
+    {
+      KernelTimer kt(ctx, l == 0 ? kStageWavIn : kStageWavDeep);
+      if (fused)
+        hipLaunchKernelGGL(k_wav_xy_an<StoreF32>, g1, dim3(256), 0, s, a1);
+      else
+        hipLaunchKernelGGL(k_wav_xy_an<ST>, g1, dim3(256), 0, s, a1);
+    }
     LLR_HIP(hipGetLastError());
     // pass 2
     ZArgs a2{};
@@ -718,6 +721,7 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
       const int nch = in[2] <= 64 ? 16 : 32;
       const int ch = 2 * (int)((in[2] + 2 * nch - 1) / (2 * nch));
       if (in[2] % 2 || in[2] % ch || ch > 4) return fail(LLR_ERR_UNSUPPORTED, "fused wavelet: z length");
+      KernelTimer kt(ctx, kStageWavMid);
 #define ZX_LAUNCH(C, N)                                                                   \
   if (sub->K == 7)                                                                        \
     hipLaunchKernelGGL((k_wav_z_expand<C, N, 7, ST>), gx, dim3(ZT * N), 0, s, ax);       \
@@ -737,6 +741,7 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
       continue;
     }
     dim3 g2((unsigned)(a2.groups_x * by), (unsigned)nvols);
+    KernelTimer kt(ctx, l == 0 ? kStageWavMid : kStageWavDeep);
 #define WZ_CASE(NN)                                                                                       \
   if (nzr <= NN) {                                                                                        \
     if (smem > 64 * 1024)                                                                                 \
@@ -777,7 +782,10 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
     a3.f = f;
     int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
     dim3 g3((unsigned)(a3.groups_x * tiles_y), (unsigned)(od[2] * nvols));
-    hipLaunchKernelGGL((k_wav_xy_syn<ST, TXS>), g3, dim3(256), 0, s, a3);
+    {
+      KernelTimer kt(ctx, l == 0 ? kStageWavOut : kStageWavDeep);
+      hipLaunchKernelGGL((k_wav_xy_syn<ST, TXS>), g3, dim3(256), 0, s, a3);
+    }
     LLR_HIP(hipGetLastError());
   }
   return LLR_OK;
