@@ -174,6 +174,20 @@ llr_status llr_signal_model_device(llr_ctx* ctx, llr_cf32* x, size_t len, const 
                                    size_t batch_size, const float* lin_ksp_corr, const float* lin_img_corr,
                                    int direction);
 
+/* llr_signal_model_device without any device allocation. No counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc): the twiddle and
+ * phase-ramp tables are built in a fixed 2 MiB device array of the library
+ * (262144 complex values, one per GPU, kept until the process ends) instead of
+ * buffers allocated per call. Same arguments, semantics and completion as
+ * llr_signal_model_device; sizes whose tables need more than
+ * batch_size * (2 (nx+ny+nz) + 3) + nx+ny+nz <= 262144 entries are rejected
+ * with LLR_ERR_UNSUPPORTED. Calls that overlap on one GPU (several threads or
+ * contexts) share the array and race. */
+llr_status llr_signal_model_device_noalloc(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t image_size[3],
+                                           size_t batch_size, const float* lin_ksp_corr, const float* lin_img_corr,
+                                           int direction);
+
 /* estimate_linear_correction (signal_model.rs:100-106 via max_coord,
  * cufft.rs:329-337): arg-max of |k|^2 (lowest index on ties) minus (n+1)/2. */
 llr_status llr_estimate_linear_correction(llr_ctx* ctx, const llr_cf32* ksp, size_t len,

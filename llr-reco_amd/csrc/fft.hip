@@ -601,7 +601,7 @@ static hipError_t dispatch_checked(int n, const PassArgs& a, long long W, long l
 // pre and post must both be given (ramped signal-model pass) or both be null.
 llr_status fft_axis_pass(llr_ctx* ctx, const void* src, void* dst, const size_t* dims, size_t nvols, int axis,
                          bool inverse, const float2* pre, bool pre_conj, const float2* post, bool post_conj,
-                         float scale, bool h16) {
+                         float scale, bool h16, const float2* twiddles) {
   int n = (int)dims[axis];
   if (!fft_size_supported((size_t)n))
     return fail(LLR_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " is not supported (2^a * 3^b with a radix of at most 32, up to 512)");
@@ -609,9 +609,11 @@ llr_status fft_axis_pass(llr_ctx* ctx, const void* src, void* dst, const size_t*
   if (h16 && pre == nullptr) return fail(LLR_ERR_UNSUPPORTED, "half storage is for the signal model's passes only");
   if ((double)dims[0] * dims[1] * dims[2] >= 4294967296.0)
     return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
-  const float2* tw = nullptr;
-  llr_status st = get_twiddles(ctx, n, &tw);
-  if (st) return st;
+  const float2* tw = twiddles;
+  if (!tw) {
+    llr_status st = get_twiddles(ctx, n, &tw);
+    if (st) return st;
+  }
   long long nx = dims[0], ny = dims[1], nz = dims[2];
   PassArgs a{};
   a.src = src;

@@ -12,9 +12,10 @@ namespace llr {
 bool fft_size_supported(size_t n);
 // src/dst/data/meas hold complex64, or half-precision pairs (ch16) when h16 is set
 // (ramped signal-model passes only).
+// twiddles (exp(-2 pi i m / n), m < n) may be given; else the context's cached table
 llr_status fft_axis_pass(llr_ctx* ctx, const void* src, void* dst, const size_t* dims, size_t nvols, int axis,
                          bool inverse, const float2* pre, bool pre_conj, const float2* post, bool post_conj,
-                         float scale, bool h16 = false);
+                         float scale, bool h16 = false, const float2* twiddles = nullptr);
 // Inverse x pass (ramps pre, conj(post), times scale) of complex64 src into
 // dst (complex64, or half-precision pairs when h16_out), setting *flag to 1 if
 // some x-line of src or of the stored dst is partly zero (neither all nonzero

@@ -13,6 +13,12 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libllr_hip.so")
 
+LLR_OK = 0
+LLR_ERR_ARG = 1
+LLR_ERR_HIP = 2
+LLR_ERR_NOMEM = 3
+LLR_ERR_UNSUPPORTED = 4
+LLR_ERR_NUMERIC = 5
 LLR_FORWARD = 0
 LLR_INVERSE = 1
 LLR_NORM_NONE = 0
@@ -52,7 +58,7 @@ EXPORTS = [
     "llr_unet_forward", "llr_unet_gradients", "llr_unet_train_step", "llr_unet_predict_coefficients",
     "llr_unet_pack_coefficients", "llr_unet_get_activation", "llr_unet_coefficient_scale", "llr_unet_train_step_crop",
     "llr_recon_prepare", "llr_recon_get_coefficients", "llr_recon_set_coefficients", "llr_recon_warm_start",
-    "llr_recon_run_learned",
+    "llr_recon_run_learned", "llr_signal_model_device_noalloc",
 ]
 
 

@@ -47,6 +47,18 @@ def signal_model_device(x_dev_ptr, n_elems, image_size, batch_size, lin_ksp_corr
                                                    _lib.ptr(ic), int(direction)))
 
 
+def signal_model_device_noalloc(x_dev_ptr, n_elems, image_size, batch_size, lin_ksp_corr, lin_img_corr, direction,
+                                ctx=None):
+    """signal_model_device without any device allocation: the twiddle and ramp
+    tables are built in the library's fixed 2 MiB per-GPU array (calls that
+    overlap on one GPU race on it)."""
+    kc = _corr(lin_ksp_corr, batch_size)
+    ic = _corr(lin_img_corr, batch_size)
+    _lib.check(_lib.load().llr_signal_model_device_noalloc(_lib.ctx_handle(ctx), ctypes.c_void_p(int(x_dev_ptr)),
+                                                           _sz(n_elems), _lib.sz3(image_size), _sz(batch_size),
+                                                           _lib.ptr(kc), _lib.ptr(ic), int(direction)))
+
+
 def signal_model_batched(x, image_size, nq, batch_size, lin_ksp_corr, lin_img_corr, model_direction, ctx=None):
     """signal_model.rs:83-94: chunks of batch_size volumes. Each chunk gets its
     own slice of both correction lists (the reference passes the whole image
