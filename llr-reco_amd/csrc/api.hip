@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -649,6 +650,7 @@ struct llr_recon {
   RampTables tabs;
   unsigned long long llr_blocks = 0;  // blocks processed by the LLR step (mixed-form statistics)
   size_t parts_cap = 0;  // doubles in partials
+  size_t vgroup = 0;     // volumes per group in the data-consistency passes (0: all at once)
   bool have_meas = false, have_init = false;
   bool prepared = false;  // set-up done (llr_recon_prepare / warm start) and not yet consumed by a run
   double lambda = 0.0, energy = 0.0;
@@ -664,7 +666,8 @@ namespace {
 struct StageTimer {
   llr_recon* r;
   int stage;
-  StageTimer(llr_recon* r_, int s) : r(r_), stage(s) {
+  bool count;  // false: add the time to the stage's current launch (a volume group after the first)
+  StageTimer(llr_recon* r_, int s, bool count_ = true) : r(r_), stage(s), count(count_) {
     if (r->timing) (void)hipEventRecord(r->ev0, r->ctx->stream);
   }
   ~StageTimer() {
@@ -674,7 +677,7 @@ struct StageTimer {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, r->ev0, r->ev1);
     r->stage_ms[stage] += ms;
-    r->stage_n[stage] += 1;
+    if (count) r->stage_n[stage] += 1;
   }
 };
 
@@ -743,49 +746,59 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
 }
 
 // Forward model -> hard projection with its FP64 residual into *err -> adjoint
-// (lib.rs:311-333), in place on the iterate.
+// (lib.rs:311-333), in place on the iterate. The volumes are independent in
+// these passes: they run in groups of r->vgroup volumes (all at once when 0),
+// each group through all its passes before the next, so a group's
+// intermediate data can stay in the Infinity Cache between the passes.
 llr_status data_consistency(llr_recon* r, double* err) {
   llr_ctx* ctx = r->ctx;
   const llr_recon_params& p = r->p;
   llr_status st = LLR_OK;
-  void* X = r->X.p;
   const bool h = r->s16;
-  const void* Ym = meas_buffer(r);
+  const size_t eb = h ? 4 : 8;  // stored bytes per complex value (iterate and measurements alike)
   const RampTables& T = r->tabs;
-  float sx = (float)(1.0 / std::sqrt((double)p.vol[0])), sy = (float)(1.0 / std::sqrt((double)p.vol[1]));
-  if (!r->hybrid) {
-    StageTimer t(r, 3);
-    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, false, T.pk[0].as<float2>(), false, T.pi[0].as<float2>(), true, sx,
-                       h);
+  const float sx = (float)(1.0 / std::sqrt((double)p.vol[0])), sy = (float)(1.0 / std::sqrt((double)p.vol[1]));
+  const size_t G = (r->vgroup && r->vgroup < r->nq) ? r->vgroup : r->nq;
+  size_t np_total = 0;
+  for (size_t g0 = 0; g0 < r->nq; g0 += G) {
+    const size_t ng = std::min(G, r->nq - g0);
+    const bool first = g0 == 0;
+    void* X = (char*)r->X.p + g0 * r->N * eb;
+    const void* Ym = (const char*)meas_buffer(r) + g0 * r->N * eb;
+    const float2 *pk0 = T.pk[0].as<float2>() + g0 * p.vol[0], *pi0 = T.pi[0].as<float2>() + g0 * p.vol[0];
+    const float2 *pk1 = T.pk[1].as<float2>() + g0 * p.vol[1], *pi1 = T.pi[1].as<float2>() + g0 * p.vol[1];
+    const float2 *pk2 = T.pk[2].as<float2>() + g0 * p.vol[2], *pi2 = T.pi[2].as<float2>() + g0 * p.vol[2];
+    if (!r->hybrid) {
+      StageTimer t(r, 3, first);
+      st = fft_axis_pass(ctx, X, X, p.vol, ng, 0, false, pk0, false, pi0, true, sx, h);
+    }
+    if (st) return st;
+    {
+      StageTimer t(r, 4, first);
+      st = fft_axis_pass(ctx, X, X, p.vol, ng, 1, false, pk1, false, pi1, true, sy, h);
+    }
+    if (st) return st;
+    {
+      StageTimer t(r, 5, first);
+      size_t np = 0;
+      st = fft_zproj(ctx, X, Ym, p.vol, ng, pk2, pi2, r->partials.as<double>() + np_total, r->parts_cap - np_total,
+                     &np, h);
+      np_total += np;
+    }
+    if (st) return st;
+    {
+      StageTimer t(r, 9, first);
+      st = fft_axis_pass(ctx, X, X, p.vol, ng, 1, true, pi1, false, pk1, true, sy, h);
+    }
+    if (st) return st;
+    if (!r->hybrid) {
+      StageTimer t(r, 10, first);
+      st = fft_axis_pass(ctx, X, X, p.vol, ng, 0, true, pi0, false, pk0, true, sx, h);
+    }
+    if (st) return st;
   }
-  if (st) return st;
-  {
-    StageTimer t(r, 4);
-    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, false, T.pk[1].as<float2>(), false, T.pi[1].as<float2>(), true, sy,
-                       h);
-  }
-  if (st) return st;
-  size_t np = 0;
-  {
-    StageTimer t(r, 5);
-    st = fft_zproj(ctx, X, Ym, p.vol, r->nq, T.pk[2].as<float2>(), T.pi[2].as<float2>(), r->partials.as<double>(),
-                   r->parts_cap, &np, h);
-  }
-  if (st) return st;
-  {
-    StageTimer t(r, 9);
-    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, true, T.pi[1].as<float2>(), false, T.pk[1].as<float2>(), true, sy,
-                       h);
-  }
-  if (st) return st;
-  if (!r->hybrid) {
-    StageTimer t(r, 10);
-    st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 0, true, T.pi[0].as<float2>(), false, T.pk[0].as<float2>(), true, sx,
-                       h);
-  }
-  if (st) return st;
   StageTimer t(r, 6);
-  return reduce_strided(ctx, r->partials.as<double>(), np, 1, err);
+  return reduce_strided(ctx, r->partials.as<double>(), np_total, 1, err);
 }
 
 
@@ -851,6 +864,7 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
   r->s16 = p.storage == LLR_STORE_FP16;
   const size_t eb = r->s16 ? 4 : 8;  // bytes per stored complex value
   r->shifts.assign(p.shifts, p.shifts + 3 * p.n_iter);
+  if (const char* g = std::getenv("LLR_VOLUME_GROUP")) r->vgroup = (size_t)std::strtoull(g, nullptr, 10);
   r->p.shifts = nullptr;
   r->p.subspace = nullptr;
   auto bail = [&](llr_status s) {

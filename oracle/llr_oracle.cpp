@@ -11,18 +11,26 @@
 //                caller asks for exact ramps; Gram + Jacobi eigen-solve in FP32,
 //                which is how cuSOLVER gesvda forms its approximate SVD.
 //
-// Pinning (see tests/test_oracle_*.py): brute-force index formulas, the
-// reference's KATs (low_rank.rs:107-180 cgemm, cublas.rs:139-183 diag,
-// dti_subspace.rs:239-261 identity subspace, block.rs:285-343 round trip,
-// fftshift.rs:127-134 wrap, cufft.rs:415-422 FFT round trip), numpy FP64
-// FFT/SVD, closed-form db2 filters and perfect reconstruction.
+// Pinning (tests/test_oracle.py, tests/test_oracle_pinning.py): brute-force
+// index formulas, the reference's KATs (low_rank.rs:107-180 cgemm,
+// cublas.rs:139-183 diag, dti_subspace.rs:239-261 identity subspace,
+// block.rs:285-343 round trip, fftshift.rs:127-134 wrap, cufft.rs:415-422 FFT
+// round trip), numpy FP64 FFT/SVD, closed-form db2 filters and perfect
+// reconstruction, the subspace_constraint closed form conj(X) conj(S) S^T, and
+// for the loop (lib.rs:262-341) an independent numpy FP64 restatement of two
+// iterations plus two closed forms (fully sampled: A^H y after every
+// iteration; identity proximal step: the zero-filled adjoint is a fixed point
+// with zero residual). The multi-level wavelet is pinned against explicit
+// analysis/synthesis matrices built from the single-level formulas of the
+// reference's CUDA kernels (kernels.cu:155-268): level rule, subband set and
+// odd-length bookkeeping.
 //
-// Parity status: every function is pinned except the multi-level wavelet's
-// level-count/boundary bookkeeping, which lives in the un-vendored `dwt` crate
-// (lib.rs:439-454). That part follows pywt's conventions (max level, symmetric
-// extension, trim on reconstruction) and is "parity unpinned" beyond the
-// perfect-reconstruction invariant and the single-level formulas of the
-// reference's own CUDA kernels (kernels.cu:155-268).
+// Parity status: parity unpinned only where the un-vendored `dwt` crate
+// (lib.rs:439-454, git HEAD, no revision) fixes a convention: its level count
+// (taken here as pywt's floor(log2(min dim / 3))), whether the deepest
+// approximation is shrunk (here: yes, every band of `subbands`), and its
+// trim on reconstruction. The checks above pin the oracle to that reading,
+// not to the crate itself.
 
 #include <algorithm>
 #include <array>
