@@ -438,6 +438,82 @@ typedef struct {
 llr_status llr_recon_run_learned(llr_recon* r, llr_unet* net, const llr_warm_params* wp, double* err_hist,
                                  size_t* iters_done, int* fell_back);
 
+/* ------------------------------------------------- data ingest and egress
+ * Host-only (no device work): what llr_recon_exec reads and writes around the
+ * reconstruction. Paths may be given with or without their extension. */
+
+/* BART .cfl/.hdr (the cfl crate's get_dims / read_to_buffer / write_buffer,
+ * data_import.rs:37-185, lib.rs:354-362): the .hdr holds "# Dimensions" and up
+ * to 16 sizes, the .cfl the column-major complex64 samples, little endian.
+ * llr_cfl_dims returns the sizes written (dims[i] = 1 beyond *ndims); len must
+ * equal their product. */
+llr_status llr_cfl_dims(const char* path, size_t dims[16], size_t* ndims);
+llr_status llr_cfl_read(const char* path, llr_cf32* data, size_t len);
+llr_status llr_cfl_write(const char* path, const size_t* dims, size_t ndims, const llr_cf32* data, size_t len);
+
+/* DtiDataSetBuilder::with_b_table's parser (data_import.rs:110-145): the .txt
+ * file holds whitespace-separated [b, gx, gy, gz] rows; *n receives the row
+ * count (b_vals and b_vecs both null: size query). b_vecs is n x 3. */
+llr_status llr_read_b_table(const char* path, float* b_vals, float* b_vecs, size_t capacity, size_t* n);
+/* b0_mask (data_import.rs:322-328): mask[i] = |g_i| <= tolerance (tolerance >= 0). */
+llr_status llr_b0_mask(const float* b_vecs, size_t n, float tolerance, int* mask);
+
+/* DtiDataSetBuilder (data_import.rs:13-273) over a directory of per-volume
+ * .cfl files: the files matching <dir>/<pattern>.cfl (glob), sorted
+ * ascending, every one with the same 3 non-singleton dimensions; expected = 0
+ * accepts any count (from_cfl_volume_dir, :67-108). */
+#define LLR_VOLUMES_DTI 0 /* b-table rows with |g| > tolerance (load_dti_set, dti_indices) */
+#define LLR_VOLUMES_B0 1  /* b0 rows (load_b0_set, b0_indices) */
+#define LLR_VOLUMES_ALL 2 /* every file (load_all) */
+typedef struct llr_dataset llr_dataset;
+llr_status llr_dataset_from_cfl_dir(const char* dir, const char* pattern, size_t expected, llr_dataset** out);
+void llr_dataset_destroy(llr_dataset* ds);
+/* with_b_table (:110-145): one row per file, b0 volumes by b0_tol. */
+llr_status llr_dataset_with_b_table(llr_dataset* ds, const char* path, float b0_tol);
+/* with_sample_mask (:187-198): an (ny, nz, n_files) .cfl mask. */
+llr_status llr_dataset_with_sample_mask(llr_dataset* ds, const char* path);
+llr_status llr_dataset_info(const llr_dataset* ds, size_t vol[3], size_t* n_volumes, size_t* n_dti, size_t* n_b0);
+/* name of file `index` (NUL-terminated, cap bytes) */
+llr_status llr_dataset_file(const llr_dataset* ds, size_t index, char* out, size_t cap);
+/* file indices of a set (dti_indices / b0_indices, :253-265); out null: count only */
+llr_status llr_dataset_indices(const llr_dataset* ds, int which, size_t* out, size_t cap, size_t* n);
+/* b-values (n_files) and vectors (n_files x 3) of every file */
+llr_status llr_dataset_b_table(const llr_dataset* ds, float* b_vals, float* b_vecs);
+/* the selected volumes, contiguous (load_volumes, :173-185); len = nx*ny*nz*count */
+llr_status llr_dataset_load(const llr_dataset* ds, int which, llr_cf32* out, size_t len);
+/* the selected volumes' masks (dti_sample_mask / b0_sample_mask, :204-250): ny*nz*count */
+llr_status llr_dataset_sample_mask(const llr_dataset* ds, int which, llr_cf32* out, size_t len);
+
+/* LlrReconParams (lib.rs:47-80) as its TOML file holds it, plus keys this
+ * library adds (all optional): llr_mode = "rank" | "svt", svt_lambda_factor,
+ * precision = "fp32" | "mixed", storage = "fp32" | "fp16", shift_seed. */
+typedef struct {
+  size_t llr_block_size[3];
+  int has_max_rand_shift;
+  size_t max_rand_shift[3];
+  size_t fft_max_batch_size;     /* accepted; the resident reconstruction does not batch */
+  size_t cas_mat_max_batch_size; /* accepted; all blocks of an iteration run in one launch */
+  size_t llr_block_rank;
+  float wavelet_shrink_factor;
+  size_t n_iter;
+  int has_subspace;
+  char subspace_projection_mat[4096];
+  int llr_mode;            /* LLR_MODE_RANK (the reference's) or LLR_MODE_SVT */
+  double lambda_factor;    /* SVT threshold factor (of sigma_max of the zero-filled blocks) */
+  int precision;           /* LLR_PREC_* */
+  int storage;             /* LLR_STORE_* */
+  unsigned long long seed; /* block-shift generator seed (the reference's shifts are unseeded) */
+} llr_toml_params;
+/* the defaults of lib.rs:67-80 (and of the added keys: rank mode, 0.02, FP32, seed 0) */
+void llr_recon_params_default(llr_toml_params* p);
+/* read <path>.toml (bin/llr-recon.rs:8-12): the reference's required fields must be
+ * there; unknown keys are ignored */
+llr_status llr_recon_params_read(const char* path, llr_toml_params* p);
+/* write <path>.toml with the reference's fields (bin/llr-reco-params.rs:14-23) */
+llr_status llr_recon_params_write(const char* path, const llr_toml_params* p);
+/* error_history.json (lib.rs:345-351): the relative residuals as a pretty JSON array */
+llr_status llr_write_error_history(const char* path, const double* hist, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,7 +59,36 @@ EXPORTS = [
     "llr_unet_pack_coefficients", "llr_unet_get_activation", "llr_unet_coefficient_scale", "llr_unet_train_step_crop",
     "llr_recon_prepare", "llr_recon_get_coefficients", "llr_recon_set_coefficients", "llr_recon_warm_start",
     "llr_recon_run_learned", "llr_signal_model_device_noalloc",
+    "llr_cfl_dims", "llr_cfl_read", "llr_cfl_write", "llr_read_b_table", "llr_b0_mask", "llr_dataset_from_cfl_dir",
+    "llr_dataset_destroy", "llr_dataset_with_b_table", "llr_dataset_with_sample_mask", "llr_dataset_info",
+    "llr_dataset_file", "llr_dataset_indices", "llr_dataset_b_table", "llr_dataset_load", "llr_dataset_sample_mask",
+    "llr_recon_params_default", "llr_recon_params_read", "llr_recon_params_write", "llr_write_error_history",
 ]
+
+LLR_VOLUMES_DTI = 0
+LLR_VOLUMES_B0 = 1
+LLR_VOLUMES_ALL = 2
+
+
+class TomlParams(ctypes.Structure):
+    """llr_toml_params (include/llr_hip.h)."""
+    _fields_ = [
+        ("llr_block_size", ctypes.c_size_t * 3),
+        ("has_max_rand_shift", ctypes.c_int),
+        ("max_rand_shift", ctypes.c_size_t * 3),
+        ("fft_max_batch_size", ctypes.c_size_t),
+        ("cas_mat_max_batch_size", ctypes.c_size_t),
+        ("llr_block_rank", ctypes.c_size_t),
+        ("wavelet_shrink_factor", ctypes.c_float),
+        ("n_iter", ctypes.c_size_t),
+        ("has_subspace", ctypes.c_int),
+        ("subspace_projection_mat", ctypes.c_char * 4096),
+        ("llr_mode", ctypes.c_int),
+        ("lambda_factor", ctypes.c_double),
+        ("precision", ctypes.c_int),
+        ("storage", ctypes.c_int),
+        ("seed", ctypes.c_ulonglong),
+    ]
 
 
 class LlrError(RuntimeError):
@@ -136,6 +165,9 @@ def load():
             lib.llr_recon_destroy.argtypes = [ctypes.c_void_p]
             lib.llr_recon_destroy.restype = None
             lib.llr_grid_dim.restype = None
+            lib.llr_dataset_destroy.argtypes = [ctypes.c_void_p]
+            lib.llr_dataset_destroy.restype = None
+            lib.llr_recon_params_default.restype = None
             lib.llr_svt_batch.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_float,
                                           ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
             lib.llr_wavelet_denoise.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
