@@ -198,6 +198,22 @@ llr_status llr_estimate_linear_correction(llr_ctx* ctx, const llr_cf32* ksp, siz
 llr_status llr_fftn_batch(llr_ctx* ctx, const size_t dims[3], size_t batches, int direction, int norm,
                           llr_cf32* data, size_t len);
 
+/* Centered unitary 3-D FFT of the batch_size col-major volumes in x, in place
+ * (fft3c / ifft3c / fft3c_chunked / ifft3c_chunked, cufft.rs:340-402):
+ * phase_shift3(Inverse) -> unitary FFT (LLR_FORWARD or LLR_INVERSE) ->
+ * phase_shift3(Forward), the FOV/2 modulation pi (i - n/2) per axis evaluated
+ * exactly. vols_per_batch > 0 moves that many volumes through the device at
+ * a time (the chunked forms); 0 moves all at once. */
+llr_status llr_fft3c(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t dims[3], size_t batch_size, int direction,
+                     size_t vols_per_batch);
+/* cu_fft3 (cufft.rs:213-267): one volume, unnormalised forward, 1/N on the inverse. */
+llr_status llr_cu_fft3(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t dims[3], int direction);
+
+/* read_affine_translation (signal_model.rs:119-176): for each of n text files,
+ * the first line containing "Parameters:" must hold 12 numbers (a 3x4 affine
+ * transform); out (n x 3) receives its last three, times multiplier. */
+llr_status llr_read_affine_translation(const char* const* files, size_t n, float multiplier, float* out);
+
 /* ---------------------------------------------------------------- fftshift.rs */
 /* fftshift / ifftshift (fftshift.rs:103-116): circular shift by floor(d/2) /
  * ceil(d/2) along each of the (up to 3) dims, generic element size. */
@@ -214,6 +230,37 @@ llr_status llr_phase_shift3(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t 
  * 0 where the reference produces NaN. */
 llr_status llr_wavelet_denoise(llr_ctx* ctx, llr_cf32* volumes, size_t n_volumes, const size_t vol_size[3],
                                float alpha, int levels);
+
+/* wavelet_denoise of every volume moved circularly by shift (sample i of the
+ * moved volume is sample i + shift, each axis modulo its length), then moved
+ * back: shift-invariant (cycle-spinning) shrinkage, the use the reference's
+ * shift hook (kernels.cu:80-95, 194, 264) was built for. */
+llr_status llr_wavelet_denoise_shifted(llr_ctx* ctx, llr_cf32* volumes, size_t n_volumes, const size_t vol_size[3],
+                                       float alpha, int levels, const int shift[3]);
+
+/* cu_dwt3_axis (cudwt3.rs:250-298; kernels.cu:155-212, 270-310): one-level
+ * analysis along `axis` of a col-major vol_size box into a decomp_size box
+ * (decomp_size[axis] = 2 nc, nc = (n + F - 1) / 2; the other sizes equal):
+ * symmetric extension, approximation at [0, nc) and detail at [nc, 2 nc).
+ * Any even filter length F <= 32 (lo_d, hi_d on the host). shift (may be NULL)
+ * moves every volume-side read circularly, each axis modulo its own length
+ * (the reference uses the first dimension's length for all three,
+ * kernels.cu:85-92). */
+llr_status llr_dwt3_axis(llr_ctx* ctx, const llr_cf32* vol, size_t vol_len, llr_cf32* decomp, size_t decomp_len,
+                         const size_t vol_size[3], const size_t decomp_size[3], int axis, const float* lo_d,
+                         const float* hi_d, size_t filter_len, const int shift[3]);
+/* cu_idwt3_axis (cudwt3.rs:300-351; kernels.cu:215-268, 312-348): the
+ * synthesis, keeping the central 2 nc - F + 2 samples; shift moves the
+ * volume-side writes. With equal shifts it inverts llr_dwt3_axis. */
+llr_status llr_idwt3_axis(llr_ctx* ctx, llr_cf32* vol, size_t vol_len, const llr_cf32* decomp, size_t decomp_len,
+                          const size_t vol_size[3], const size_t decomp_size[3], int axis, const float* lo_r,
+                          const float* hi_r, size_t filter_len, const int shift[3]);
+/* cu_dwt_denoise (cudwt3.rs:102-247): analysis along x, y, z then synthesis
+ * along z, y, x of one volume, in place, each stage with the shift hook on its
+ * input box; no shrinkage (a perfect-reconstruction round trip). */
+llr_status llr_dwt_denoise(llr_ctx* ctx, llr_cf32* vol, const size_t vol_size[3], const float* lo_d,
+                           const float* hi_d, const float* lo_r, const float* hi_r, size_t filter_len,
+                           const int shift[3]);
 
 /* hard_project (lib.rs:392-406): where y != 0, err += |x - y|^2 and x = y. */
 llr_status llr_hard_project(llr_ctx* ctx, llr_cf32* x, const llr_cf32* y, size_t len, double* sum_sq_error);
@@ -283,6 +330,10 @@ typedef struct {
                             them through the inverse x transform once and the iterations skip the
                             x passes, which commute with the hard projection; 1: always the full
                             3D forward model and adjoint, as the reference (extension) */
+  const float* lin_img_corr; /* nq x 3 linear image corrections of the signal model
+                            (signal_model.rs:34-79: a sub-voxel image shift per volume, e.g. from
+                            llr_read_affine_translation); host pointer, copied; NULL: zero, as
+                            lib.rs:148 fixes them */
 } llr_recon_params;
 
 /* Create a reconstruction (allocates the iterate, the measurements and all

@@ -493,6 +493,38 @@ llr_status llr_fftn_batch(llr_ctx* ctx, const size_t dims[3], size_t batches, in
   return download(data, d, len * 8, ctx);
 }
 
+// Centered unitary FFT (cufft.rs:340-402): phase_shift3(Inverse) -> unitary FFT ->
+// phase_shift3(Forward). The modulation pi (i - n/2) per axis is the signal
+// model's ramp with zero corrections (s = n/2), so this is the signal model's
+// pass sequence with zero-correction tables, a chunk of volumes at a time.
+llr_status llr_fft3c(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t dims[3], size_t batch_size, int direction,
+                     size_t vols_per_batch) {
+  if (!ctx || !x || !dims) return fail(LLR_ERR_ARG, "null argument");
+  const size_t N = dims[0] * dims[1] * dims[2];
+  if (len != N * batch_size) return fail(LLR_ERR_ARG, "data layout and data length inconsistency");
+  if (direction != LLR_FORWARD && direction != LLR_INVERSE) return fail(LLR_ERR_ARG, "bad direction");
+  llr_status st = check_fft_dims(dims);
+  if (st) return st;
+  if (batch_size == 0) return LLR_OK;
+  const size_t chunk = vols_per_batch ? std::min(vols_per_batch, batch_size) : batch_size;
+  LLR_HIP(hipSetDevice(ctx->device));
+  RampTables t;
+  if ((st = make_tables(t, dims, chunk, nullptr, nullptr))) return st;
+  DevBuf d;
+  for (size_t v0 = 0; v0 < batch_size; v0 += chunk) {
+    const size_t nv = std::min(chunk, batch_size - v0);
+    if ((st = upload(d, x + v0 * N, nv * N * 8))) return st;
+    if ((st = model_device(ctx, d.p, d.p, dims, nv, t, direction))) return st;
+    if ((st = download(x + v0 * N, d, nv * N * 8, ctx))) return st;
+  }
+  return LLR_OK;
+}
+
+llr_status llr_cu_fft3(llr_ctx* ctx, llr_cf32* x, size_t len, const size_t dims[3], int direction) {
+  if (direction != LLR_FORWARD && direction != LLR_INVERSE) return fail(LLR_ERR_ARG, "bad direction");
+  return llr_fftn_batch(ctx, dims, 1, direction, LLR_NORM_INVERSE, x, len);
+}
+
 // ---------------------------------------------------------------- fftshift.rs
 static llr_status shift_common(llr_ctx* ctx, void* data, size_t eb, const size_t* dims, bool inverse) {
   size_t N = dims[0] * dims[1] * dims[2];
@@ -894,7 +926,9 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
     if ((st = upload(r->t1, nullptr, te * p.nq * 8))) return bail(st);
     if ((st = upload(r->t2, nullptr, te * p.nq * 8))) return bail(st);
   }
-  if ((st = make_tables(r->tabs, p.vol, p.nq, nullptr, nullptr))) return bail(st);  // pk is redone by every set-up
+  // the image ramps (lin_img_corr) are fixed for the reconstruction; pk is redone by every set-up
+  if ((st = make_tables(r->tabs, p.vol, p.nq, nullptr, p.lin_img_corr))) return bail(st);
+  r->p.lin_img_corr = nullptr;
   // partial sums: the fused z pass's, at any tiles-per-workgroup setting (the
   // batch autotuner may change it), or the set-up's energy/arg-max pass
   size_t np = 0;

@@ -591,6 +591,33 @@ llr_status llr_recon_params_write(const char* path, const llr_toml_params* p) {
   return LLR_OK;
 }
 
+llr_status llr_read_affine_translation(const char* const* files, size_t n, float multiplier, float* out) {
+  if ((!files || !out) && n) return fail(LLR_ERR_ARG, "null argument");
+  for (size_t i = 0; i < n; ++i) {
+    if (!files[i]) return fail(LLR_ERR_ARG, "null file name");
+    std::ifstream f(files[i]);
+    if (!f) return fail(LLR_ERR_ARG, std::string("failed to open file ") + files[i]);
+    std::string line;
+    bool found = false;
+    while (!found && std::getline(f, line)) {
+      if (line.find("Parameters:") == std::string::npos) continue;
+      std::vector<float> v;
+      std::istringstream is(line);
+      std::string tok;
+      while (is >> tok) {  // the numeric tokens of the line (split_ascii_whitespace + parse::<f32>)
+        char* end = nullptr;
+        const float x = std::strtof(tok.c_str(), &end);
+        if (end != tok.c_str() && *end == '\0') v.push_back(x);
+      }
+      if (v.size() != 12) return fail(LLR_ERR_ARG, std::string("expected 12 matrix entries in ") + files[i]);
+      for (int a = 0; a < 3; ++a) out[3 * i + a] = v[9 + a] * multiplier;
+      found = true;
+    }
+    if (!found) return fail(LLR_ERR_ARG, std::string("failed to find Parameter field in affine file ") + files[i]);
+  }
+  return LLR_OK;
+}
+
 llr_status llr_write_error_history(const char* path, const double* hist, size_t n) {
   if (!path || (!hist && n)) return fail(LLR_ERR_ARG, "null argument");
   std::ofstream f(path);

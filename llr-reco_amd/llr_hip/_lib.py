@@ -63,6 +63,8 @@ EXPORTS = [
     "llr_dataset_destroy", "llr_dataset_with_b_table", "llr_dataset_with_sample_mask", "llr_dataset_info",
     "llr_dataset_file", "llr_dataset_indices", "llr_dataset_b_table", "llr_dataset_load", "llr_dataset_sample_mask",
     "llr_recon_params_default", "llr_recon_params_read", "llr_recon_params_write", "llr_write_error_history",
+    "llr_fft3c", "llr_cu_fft3", "llr_read_affine_translation", "llr_wavelet_denoise_shifted", "llr_dwt3_axis",
+    "llr_idwt3_axis", "llr_dwt_denoise",
 ]
 
 LLR_VOLUMES_DTI = 0
@@ -132,6 +134,7 @@ class ReconParams(ctypes.Structure):
         ("partial_blocks", ctypes.c_int),
         ("storage", ctypes.c_int),
         ("no_hybrid", ctypes.c_int),
+        ("lin_img_corr", ctypes.c_void_p),
     ]
 
 

@@ -56,7 +56,7 @@ def wavelet_denoise(volumes, n_volumes, vol_size, alpha, levels=-1, ctx=None):
 
 def make_params(vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0, lambda_factor=0.02,
                 wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0, precision="fp32",
-                partial_blocks=False, storage="fp32", hybrid=True):
+                partial_blocks=False, storage="fp32", hybrid=True, img_corr=None):
     """llr_recon_params for the C ABI, plus the arrays it points at (keep them alive)."""
     vol = [int(v) for v in vol]
     nq = int(nq)
@@ -89,7 +89,13 @@ def make_params(vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lamb
     p.partial_blocks = 1 if partial_blocks else 0
     p.storage = {"fp32": _lib.LLR_STORE_FP32, "fp16": _lib.LLR_STORE_FP16}[storage]
     p.no_hybrid = 0 if hybrid else 1
-    return p, (sh, sub)
+    ic = None
+    if img_corr is not None:
+        ic = np.ascontiguousarray(np.asarray(img_corr, np.float32).reshape(-1))
+        if ic.size != 3 * nq:
+            raise ValueError("the number of linear corrections must be equal to the number of volumes")
+        p.lin_img_corr = ctypes.c_void_p(ic.ctypes.data)
+    return p, (sh, sub, ic)
 
 
 class Recon:
@@ -109,15 +115,17 @@ class Recon:
     measured or empty), iterate in (x, ky, kz) without the x transforms, which
     commute with the hard projection (same result up to rounding); False keeps
     the full 3D forward model and adjoint in every iteration.
+    img_corr: (nq, 3) linear image corrections of the signal model (a
+    sub-voxel image shift per volume; signal_model.rs:34-79), default zero.
     """
 
     def __init__(self, vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0,
                  lambda_factor=0.02, wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0,
-                 precision="fp32", partial_blocks=False, storage="fp32", hybrid=True, ctx=None):
+                 precision="fp32", partial_blocks=False, storage="fp32", hybrid=True, img_corr=None, ctx=None):
         self.ctx = ctx or _lib.default_context()
         p, self._keep = make_params(vol, nq, shifts, blk, llr_mode, rank, svt_lambda, lambda_factor, wavelet_alpha,
                                     wavelet_levels, subspace, residual_stop, precision, partial_blocks, storage,
-                                    hybrid)
+                                    hybrid, img_corr)
         self.vol = [int(v) for v in vol]
         self.nq = int(nq)
         self.D = int(np.prod(self.vol)) * self.nq
@@ -324,3 +332,12 @@ class ReconBatch:
             self.close()
         except Exception:
             pass
+
+
+def wavelet_denoise_shifted(volumes, n_volumes, vol_size, alpha, shift, levels=-1, ctx=None):
+    """Cycle-spinning wavelet shrinkage: wavelet_denoise of the volumes moved
+    circularly by shift (sample i reads i + shift), moved back. In place."""
+    V = _lib.as_cf32(volumes, "volumes")
+    sh = (ctypes.c_int * 3)(*[int(s) for s in shift])
+    _lib.check(_lib.load().llr_wavelet_denoise_shifted(_lib.ctx_handle(ctx), _lib.ptr(V), _sz(n_volumes),
+                                                       _lib.sz3(vol_size), ctypes.c_float(alpha), int(levels), sh))

@@ -89,3 +89,53 @@ def cu_fftn_batch(dims, batches, fft_direction, norm, data, ctx=None):
     X = _lib.as_cf32(data, "data")
     _lib.check(_lib.load().llr_fftn_batch(_lib.ctx_handle(ctx), _lib.sz3(dims), _sz(batches), int(fft_direction),
                                           int(norm), _lib.ptr(X), _sz(X.size)))
+
+
+def _fft3c(x, dims, batch_size, direction, vols_per_batch, ctx):
+    X = _lib.as_cf32(x, "x")
+    if X.size != int(np.prod(dims)) * batch_size:
+        raise ValueError("dims must agree with the size of x")
+    _lib.check(_lib.load().llr_fft3c(_lib.ctx_handle(ctx), _lib.ptr(X), _sz(X.size), _lib.sz3(dims), _sz(batch_size),
+                                     int(direction), _sz(vols_per_batch)))
+
+
+def fft3c(x, dims, batch_size=1, ctx=None):
+    """cufft.rs:340-346: unitary forward centered FFT of each volume, in place."""
+    _fft3c(x, dims, batch_size, _lib.LLR_FORWARD, 0, ctx)
+
+
+def ifft3c(x, dims, batch_size=1, ctx=None):
+    """cufft.rs:348-353: unitary inverse centered FFT of each volume, in place."""
+    _fft3c(x, dims, batch_size, _lib.LLR_INVERSE, 0, ctx)
+
+
+def fft3c_chunked(x, dims, batch_size, vols_per_batch, ctx=None):
+    """cufft.rs:385-387 (vols_per_batch volumes through the device at a time)."""
+    _fft3c(x, dims, batch_size, _lib.LLR_FORWARD, vols_per_batch, ctx)
+
+
+def ifft3c_chunked(x, dims, batch_size, vols_per_batch, ctx=None):
+    """cufft.rs:389-391."""
+    _fft3c(x, dims, batch_size, _lib.LLR_INVERSE, vols_per_batch, ctx)
+
+
+def cu_fft3(x, dims, direction, ctx=None):
+    """cufft.rs:213-267: one volume, unnormalised forward, 1/N on the inverse."""
+    X = _lib.as_cf32(x, "x")
+    _lib.check(_lib.load().llr_cu_fft3(_lib.ctx_handle(ctx), _lib.ptr(X), _sz(X.size), _lib.sz3(dims), int(direction)))
+
+
+def read_affine_translation(files, multiplier=1.0):
+    """signal_model.rs:149-176: the translation (last 3 of the 12 'Parameters:'
+    entries) of each affine text file, times multiplier; (n, 3) float32."""
+    names = [str(f).encode() for f in files]
+    arr = (ctypes.c_char_p * len(names))(*names)
+    out = np.zeros((len(names), 3), np.float32)
+    _lib.check(_lib.load().llr_read_affine_translation(arr, _sz(len(names)), ctypes.c_float(multiplier),
+                                                       _lib.ptr(out)))
+    return out
+
+
+def read_affine_translation_range(n_range, file_pattern, multiplier=1.0):
+    """signal_model.rs:149-161: file_pattern(i) for i in n_range."""
+    return read_affine_translation([file_pattern(i) for i in n_range], multiplier)

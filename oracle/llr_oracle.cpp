@@ -782,6 +782,7 @@ struct orc_recon_params {
   int partial_blocks;    // 1: ceil(vol/blk) grid with truncated edge blocks (extension)
   size_t snap_iter;      // > 0: copy the iterate into snap after this many iterations
   void* snap;            // N*nq complex (T precision), or null
+  const float* img_corr; // nq x 3 image corrections (signal_model.rs:34-79), or null (zero, lib.rs:148)
 };
 
 namespace orc {
@@ -870,6 +871,7 @@ static int recon(const orc_recon_params* p, const C<T>* meas, C<T>* x, double* e
   double us, energy;
   calc_undersampling<T>(meas, N * nq, &us, &energy);
   std::vector<float> kc(3 * nq), ic(3 * nq, 0.f);
+  if (p->img_corr) std::memcpy(ic.data(), p->img_corr, sizeof(float) * 3 * nq);
   for (size_t q = 0; q < nq; ++q) estimate_linear_correction<T>(meas + q * N, vol, &kc[3 * q]);
   if (out_ksp_corr) std::memcpy(out_ksp_corr, kc.data(), sizeof(float) * 3 * nq);
   std::memcpy(x, meas, sizeof(C<T>) * N * nq);

@@ -264,12 +264,13 @@ class ReconParams(ctypes.Structure):
         ("partial_blocks", ctypes.c_int),
         ("snap_iter", ctypes.c_size_t),
         ("snap", ctypes.c_void_p),
+        ("img_corr", ctypes.c_void_p),
     ]
 
 
 def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda_factor=0.02,
           wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, phase_mode=None, partial_blocks=False,
-          snapshot_after=0):
+          snapshot_after=0, img_corr=None):
     """Run the reconstruction loop of llr_recon_exec (lib.rs:227-341) on CPU.
 
     partial_blocks: ceil(vol/blk) block grid with truncated edge blocks (an
@@ -306,6 +307,11 @@ def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda
         p.subspace = None
     p.shifts = shifts.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t))
     p.partial_blocks = 1 if partial_blocks else 0
+    ic = None
+    if img_corr is not None:
+        ic = np.ascontiguousarray(img_corr, np.float32).reshape(-1)
+        assert ic.size == 3 * nq
+        p.img_corr = ctypes.c_void_p(ic.ctypes.data)
     snap = None
     if snapshot_after:
         snap = np.zeros_like(meas)

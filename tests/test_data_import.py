@@ -238,3 +238,27 @@ def test_llr_recon_arguments_and_input_errors(tmp_path):
     assert r.returncode == 1 and "failed to open header" in r.stderr
     r = run(args + ["x"])
     assert r.returncode == 1 and "not a number" in r.stderr
+
+
+def test_read_affine_translation(tmp_path):
+    """signal_model.rs:119-176: the 'Parameters:' line of an affine text file (ITK
+    style) holds a 3x3 matrix and a translation; the translation is returned
+    times the multiplier."""
+    from llr_hip import signal_model as sm
+
+    files = []
+    for i, t in enumerate([(1.5, -2.0, 0.25), (0.0, 3.0, -1.0)]):
+        f = tmp_path / f"aff{i}.txt"
+        f.write_text("#Insight Transform File V1.0\nTransform: AffineTransform_double_3_3\n"
+                     f"Parameters: 1 0 0 0 1 0 0 0 1 {t[0]} {t[1]} {t[2]}\nFixedParameters: 0 0 0\n")
+        files.append(str(f))
+    out = sm.read_affine_translation(files, multiplier=2.0)
+    assert np.allclose(out, [[3.0, -4.0, 0.5], [0.0, 6.0, -2.0]])
+    out = sm.read_affine_translation_range(range(2), lambda i: str(tmp_path / f"aff{i}.txt"))
+    assert np.allclose(out[1], [0.0, 3.0, -1.0])
+    (tmp_path / "bad.txt").write_text("Parameters: 1 2 3\n")
+    with pytest.raises(_lib.LlrError, match="expected 12 matrix entries"):
+        sm.read_affine_translation([str(tmp_path / "bad.txt")])
+    (tmp_path / "none.txt").write_text("nothing here\n")
+    with pytest.raises(_lib.LlrError, match="failed to find Parameter field"):
+        sm.read_affine_translation([str(tmp_path / "none.txt")])

@@ -218,7 +218,7 @@ def wavelet_denoise(vol_data, vol, alpha, levels=None):
 
 # ------------------------------------------------------------ the loop
 def recon(y, vol, nq, blk, shifts, mode="svt", lam_factor=0.02, rank=0, alpha=0.0, S=None, order="llr_first",
-          normalize=True):
+          normalize=True, img_corr=None):
     """lib.rs:224-341: energy, corrections, zero-filled adjoint, threshold from the
     adjoint's blocks at shift 0, then per iteration LLR -> wavelet -> forward ->
     hard projection (residual / total energy) -> adjoint."""
@@ -226,7 +226,7 @@ def recon(y, vol, nq, blk, shifts, mode="svt", lam_factor=0.02, rank=0, alpha=0.
     y = y.astype(np.complex128)
     energy = float(np.sum(np.abs(y) ** 2))
     kc = linear_corrections(y, vol, nq)
-    ic = np.zeros((nq, 3))
+    ic = np.zeros((nq, 3)) if img_corr is None else np.asarray(img_corr, np.float64).reshape(nq, 3)
     x = signal_model(y, vol, nq, kc, ic, forward=False)
     lam = lam_factor * llr_step(x.copy(), vol, nq, blk, (0, 0, 0), S, mode, 0.0, rank, modify=False)
     hist = []
@@ -260,9 +260,9 @@ def measurements(vol, nq, frac, seed):
     return y
 
 
-def oracle_recon(y, vol, nq, blk, shifts, mode="svt", lam_factor=0.02, rank=0, alpha=0.0, S=None):
+def oracle_recon(y, vol, nq, blk, shifts, mode="svt", lam_factor=0.02, rank=0, alpha=0.0, S=None, img_corr=None):
     return O.recon(y.astype(np.complex128), vol, nq, blk, shifts, llr_mode={"svt": 1, "rank": 0}[mode], rank=rank,
-                   lambda_factor=lam_factor, wavelet_alpha=alpha, subspace=S)
+                   lambda_factor=lam_factor, wavelet_alpha=alpha, subspace=S, img_corr=img_corr)
 
 
 def rel(a, b):
@@ -306,6 +306,21 @@ def test_loop_rank_mode_matches_independent_restatement():
     x_o, h_o, _, _ = oracle_recon(y, vol, nq, blk, shifts, mode="rank", rank=2)
     assert np.allclose(h_o, h_np, rtol=1e-9, atol=0)
     assert rel(x_o, x_np) <= 1e-9
+
+
+def test_loop_with_image_corrections_matches_independent_restatement():
+    """Sub-voxel image corrections (lin_img_corr of signal_model.rs:34-79, fixed at
+    zero by lib.rs:148 but an input of the signal model) in both directions."""
+    vol, nq, blk = (8, 12, 8), 3, (4, 4, 4)
+    y = measurements(vol, nq, 0.5, 17)
+    ic = np.array([[0.3, -1.25, 2.0], [0.0, 0.5, -0.75], [1.5, 0.0, 0.25]], np.float32)
+    shifts = np.array([[1, 2, 3], [6, 5, 4]])
+    x_np, h_np, _, _ = recon(y, vol, nq, blk, shifts, lam_factor=0.3, alpha=0.4, img_corr=ic)
+    x_o, h_o, _, _ = oracle_recon(y, vol, nq, blk, shifts, lam_factor=0.3, alpha=0.4, img_corr=ic)
+    x_0, _, _, _ = oracle_recon(y, vol, nq, blk, shifts, lam_factor=0.3, alpha=0.4)
+    assert np.allclose(h_o, h_np, rtol=1e-9, atol=0)
+    assert rel(x_o, x_np) <= 1e-9
+    assert rel(x_o, x_0) > 1e-3
 
 
 def test_fully_sampled_loop_returns_the_adjoint():
