@@ -461,6 +461,140 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_z_proje
   }
 }
 
+// The same fused pass with R2 lanes per line instead of max(R1, R2), for
+// R1 = q R2: stage 1 (R1-point DFTs, one per lane) keeps every lane busy and
+// each lane takes q of the R1 R2-point DFTs of stage 2 (k1 = j + R2 u). With
+// half the lanes per line, T = 32 lines make a 256-thread workgroup and three
+// of them fit a CU: half storage then has the loads in flight that complex64
+// storage gets from the wide layout (whose stage 1 leaves half the lanes idle).
+template <int R1, int R2, class ST, int T>
+__global__ void __launch_bounds__(T * R2) k_fft_z_project_nw(ZProjArgs a) {
+  static_assert(R1 % R2 == 0 && R1 > R2, "narrow layout: R1 a multiple of R2");
+  constexpr int N = R1 * R2, TH = T * R2, Q = R1 / R2;
+  constexpr int R2P = R2 + 1, LS = R1 * R2P + 1;
+  using E = typename ST::T;
+  __shared__ float2 lds[T * LS];
+  __shared__ float2 s_tw[N], s_pk[N], s_pi[N];
+  __shared__ double red[(TH + 63) / 64];  // (a 32-lane workgroup is one partial wave)
+  const int tid = threadIdx.x;
+  const int t = tid % T;
+  const int j = tid / T;  // 0 .. R2-1
+  const long long v = blockIdx.z;
+  for (int i = tid; i < N; i += TH) {
+    s_tw[i] = a.tw[i];
+    s_pk[i] = a.pk[v * N + i];
+    s_pi[i] = a.pi[v * N + i];
+  }
+  const long long row = v * a.vstride + (long long)blockIdx.y * a.wstride;
+  E* __restrict__ data = (E*)a.data + row;
+  const E* __restrict__ meas = (const E*)a.meas + row;
+  const int tile0 = blockIdx.x * a.tiles_per_wg;
+  const int ntiles = min(a.tiles_per_wg, a.tiles - tile0);
+  double err = 0.0;
+  // next tile: iterate samples j + R2 j1 (stage 1) and measurements at
+  // k = k1 + R1 k2 for this lane's stage-2 rows k1 = j + R2 u
+  float2 xn[R1];
+  E mn[Q][R2];
+  // (the measurements of the next tile are loaded once stage 2 has used the
+  // current ones: no second register copy of them)
+  auto loadx = [&](int tile) {
+    const int u = tile * T + t;
+    if (u < a.U) {
+      const E* p = data + (unsigned)(u * a.ustride + j * a.stride);
+      const unsigned step = (unsigned)(R2 * a.stride);
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) xn[j1] = ST::ldnt(p + j1 * step);
+    }
+  };
+  auto loadm = [&](int tile) {
+    const int u = tile * T + t;
+    if (u < a.U) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const E* mp = meas + (unsigned)(u * a.ustride + (j + R2 * q) * a.stride);
+        const unsigned ms = (unsigned)(R1 * a.stride);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) mn[q][k2] = ST::rawnt(mp + k2 * ms);
+      }
+    }
+  };
+  loadx(tile0);
+  loadm(tile0);
+  __syncthreads();
+  for (int it = 0; it < ntiles; ++it) {
+    const int u = (tile0 + it) * T + t;
+    const bool active = u < a.U;
+    if (active) {
+      float2 x[R1];
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) x[j1] = cmul(xn[j1], s_pk[j1 * R2 + j]);
+      SmallDFT<R1>::run(x);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) lds[t * LS + k1 * R2P + j] = cmul(x[k1], s_tw[j * k1]);
+    }
+    __syncthreads();
+    if (it + 1 < ntiles) loadx(tile0 + it + 1);
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int k1 = j + R2 * q;
+        float2 y[R2];
+#pragma unroll
+        for (int j2 = 0; j2 < R2; ++j2) y[j2] = lds[t * LS + k1 * R2P + j2];
+        SmallDFT<R2>::run(y);
+        // k-space sample k = k1 + R1 k2: project, then the adjoint's R2-point DFT
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+          float2 xv = cscale(y[k2], a.scale);
+          const float2 m = ST::ld(&mn[q][k2]);
+          if (m.x != 0.0f || m.y != 0.0f) {
+            const float2 mpv = cmul(m, s_pi[k1 + R1 * k2]);
+            const float2 d = csub(xv, mpv);
+            err += (double)cnorm(d);
+            xv = mpv;
+          }
+          y[k2] = make_float2(xv.x, -xv.y);  // conjugated: inverse DFT via the forward kernel
+        }
+        SmallDFT<R2>::run(y);
+        // each lane rewrites only the rows k1 it read (no other lane reads them)
+#pragma unroll
+        for (int j2 = 0; j2 < R2; ++j2)
+          lds[t * LS + k1 * R2P + j2] = cmul(make_float2(y[j2].x, -y[j2].y), cconj(s_tw[k1 * j2]));
+      }
+    }
+    if (it + 1 < ntiles) loadm(tile0 + it + 1);
+    __syncthreads();
+    if (active) {
+      float2 x[R1];
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        const float2 q = lds[t * LS + k1 * R2P + j];
+        x[k1] = make_float2(q.x, -q.y);
+      }
+      SmallDFT<R1>::run(x);
+      E* p = data + (unsigned)(u * a.ustride + j * a.stride);
+      const unsigned step = (unsigned)(R2 * a.stride);
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) {
+        const int idx = j + R2 * j1;
+        const float2 r = make_float2(x[j1].x, -x[j1].y);
+        ST::st(p + j1 * step, cscale(cmulc(r, s_pk[idx]), a.scale));
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) err += __shfl_down(err, off, 64);
+  if ((tid & 63) == 0) red[tid >> 6] = err;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int w = 0; w < (TH + 63) / 64; ++w) s += red[w];
+    const size_t bid = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    a.partial[bid] = s;
+  }
+}
+
 template <int R1, int R2, bool TF, bool IV, bool RP, class ST, int MINT>
 hipError_t launch_one(PassArgs a, long long W, long long nvols, hipStream_t s) {
   using L = Layout<R1, R2, MINT>;
@@ -529,8 +663,31 @@ hipError_t launch_zproj_as(ZProjArgs a, long long W, long long nvols, hipStream_
   return hipGetLastError();
 }
 
+// narrow-layout launch: 32 lines per workgroup, 4 tiles per workgroup by
+// default (at 256x256x128x67 in half storage: 1.27 ms per launch against 1.66
+// ms for the wide layout; 64 lines per 512-thread workgroup takes 1.33-1.39 ms,
+// its 139 registers per lane allowing one workgroup per CU, or spilling when
+// held to 128)
+template <int R1, int R2, class ST>
+hipError_t launch_zproj_nw(ZProjArgs a, long long W, long long nvols, hipStream_t s, size_t* nparts) {
+  constexpr int T = 32;
+  a.tiles = (a.U + T - 1) / T;
+  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;
+  a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
+  dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
+  *nparts = (size_t)grid.x * grid.y * grid.z;
+  if ((long long)*nparts > a.cap) return hipErrorInvalidValue;  // partial sums would not fit
+  hipLaunchKernelGGL((k_fft_z_project_nw<R1, R2, ST, T>), grid, dim3(T * R2), 0, s, a);
+  return hipGetLastError();
+}
+
 template <int R1, int R2>
 hipError_t launch_zproj(ZProjArgs a, long long W, long long nvols, bool h16, hipStream_t s, size_t* nparts) {
+  // half storage with R1 a multiple of R2 takes the narrow layout; the size
+  // query stays with the wide layout, whose workgroup count bounds both
+  if constexpr (R1 > R2 && R1 % R2 == 0 && R2 >= 2 && R2 <= 8) {  // whole waves: 32 R2 >= 64 threads
+    if (h16 && a.partial != nullptr) return launch_zproj_nw<R1, R2, StoreF16>(a, W, nvols, s, nparts);
+  }
   return h16 ? launch_zproj_as<R1, R2, StoreF16, 32>(a, W, nvols, s, nparts)
              : launch_zproj_as<R1, R2, StoreF32, 1>(a, W, nvols, s, nparts);
 }
