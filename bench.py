@@ -309,7 +309,8 @@ def main():
                          lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
                          subspace=ds["subspace"], precision="mixed", storage=args.storage)
         rl.set_measurements(device_ptr=ds["y"].data_ptr())
-        s_learned, (hl, fell_back) = timed(lambda: rl.run_learned(net, r_stop, to_net, from_net))
+        run_kw = dict(momentum=W.LEARNED_MOMENTUM, min_iter=W.LEARNED_MIN_ITER)
+        s_learned, (hl, fell_back) = timed(lambda: rl.run_learned(net, r_stop, to_net, from_net, **run_kw))
         rl.image_to_device(img.data_ptr())
         e_learned = nrmse(img)
         fb64, nb = rl.mixed_stats()
@@ -317,7 +318,7 @@ def main():
             f"residual {hl[0]:.4e} -> {hl[-1]:.4e}; NRMSE vs truth {e_learned:.5f} "
             f"({e_learned / e_plain:.4f} x plain); FP64 eigen-solves {fb64}/{nb} blocks")
         if not args.no_stage_timing:
-            stages["learned_bf16"] = stage_record(rl, ds, lambda: rl.run_learned(net, r_stop, to_net, from_net),
+            stages["learned_bf16"] = stage_record(rl, ds, lambda: rl.run_learned(net, r_stop, to_net, from_net, **run_kw),
                                                   eb=4 if args.storage == "fp16" else 8)
         other = None
         if not args.no_other_storage:
@@ -326,7 +327,7 @@ def main():
                              lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
                              subspace=ds["subspace"], precision="mixed", storage=alt)
             ra.set_measurements(device_ptr=ds["y"].data_ptr())
-            s_alt, (ha, fb_a) = timed(lambda: ra.run_learned(net, r_stop, to_net, from_net))
+            s_alt, (ha, fb_a) = timed(lambda: ra.run_learned(net, r_stop, to_net, from_net, **run_kw))
             ra.image_to_device(img.data_ptr())
             e_alt = nrmse(img)
             ra.close()
@@ -336,7 +337,7 @@ def main():
             log(f"learned BF16 with {alt} storage: {s_alt:.4f} s/recon; {len(ha)} iterations (fallback {fb_a}); "
                 f"NRMSE vs truth {e_alt:.5f} ({e_alt / e_plain:.4f} x plain)")
         learned = dict(s_per_recon=round(s_learned, 5), iterations=len(hl), fell_back=bool(fell_back), storage=args.storage,
-                       other_storage=other,
+                       momentum=W.LEARNED_MOMENTUM, min_iter=W.LEARNED_MIN_ITER, other_storage=other,
                        nrmse_vs_truth=round(e_learned, 6), residual_final=float(hl[-1]),
                        fp64_eigen_blocks=int(fb64), weights=os.path.relpath(weights, ROOT),
                        unet_training=dict(pairs=int(net.meta.get("pairs", 0)), epochs=int(net.meta.get("epochs", 0)),

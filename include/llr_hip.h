@@ -481,11 +481,16 @@ typedef struct {
   const float* from_net; /* K x K (NULL = identity) */
   double residual_stop;  /* the iterations stop once the relative residual falls to this value */
   size_t max_iter;       /* iteration budget of the warm-started run (0 or > n_iter: n_iter) */
+  double momentum;       /* b in [0, 1): the LLR step of the warm-started run acts on
+                            x_k + b (x_k - x_{k-1}) (needs the fused wavelet: a subspace and a
+                            block grid covering every voxel); 0: plain iterations */
+  size_t min_iter;       /* the residual_stop test starts after this many iterations */
 } llr_warm_params;
-/* Warm start, then iterations until residual_stop. If the warm-started run
- * does not reach residual_stop within max_iter iterations the cold start is
- * run instead (zero-filled adjoint, n_iter iterations) and kept: fell_back = 1.
- * err_hist (n_iter entries) receives the kept run's relative residuals. */
+/* Warm start, then iterations (at least min_iter) until residual_stop. If the
+ * warm-started run does not reach residual_stop within max_iter iterations the
+ * cold start is run instead (zero-filled adjoint, n_iter plain iterations) and
+ * kept: fell_back = 1. err_hist (n_iter entries) receives the kept run's
+ * relative residuals. */
 llr_status llr_recon_run_learned(llr_recon* r, llr_unet* net, const llr_warm_params* wp, double* err_hist,
                                  size_t* iters_done, int* fell_back);
 

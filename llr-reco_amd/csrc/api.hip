@@ -151,7 +151,7 @@ using namespace llr;
 extern "C" {
 
 const char* llr_last_error(void) { return g_err.c_str(); }
-int llr_abi_version(void) { return 106; }
+int llr_abi_version(void) { return 107; }
 
 llr_status llr_ctx_create(int device, llr_ctx** out) {
   if (!out) return fail(LLR_ERR_ARG, "null output pointer");
@@ -655,6 +655,9 @@ struct llr_recon {
   int nc = 0;       // columns of the LLR matrices (srank or nq)
   int wL = 0;       // wavelet levels
   DevBuf X, Y, Cb, S, coef, t1, t2, partials, stats, hist, smax, best, counters, werr;
+  DevBuf Cprev;            // previous iteration's subspace coefficients (momentum)
+  double momentum = 0.0;   // > 0: the LLR step acts on C(x_k) + momentum (C(x_k) - C(x_{k-1}))
+  bool mom_first = false;  // the next LLR step has no previous coefficients yet
   // half storage (LLR_STORE_FP16): X, coef hold half-precision pairs; Yh is the
   // copy of the measurements made by the set-up; Xin holds an initial image
   // (complex64) until the set-up converts it. Stored data are the caller's
@@ -720,6 +723,21 @@ const void* meas_buffer(const llr_recon* r) {
   return r->hybrid ? r->Yx.p : r->Y.p;
 }
 
+// Momentum (extrapolation of the iterate entering the LLR step) in
+// coefficient space: with the fused wavelet the LLR step sees the iterate only
+// through its coefficients C = X S, and C is linear, so the extrapolated
+// iterate x_k + b (x_k - x_{k-1}) has coefficients (1 + b) C(x_k) - b C(x_{k-1}).
+// Cprev keeps C(x_{k-1}); the first step after a set-up only stores it.
+llr_status apply_momentum(llr_recon* r, int mode, bool backproject) {
+  if (!(r->momentum > 0) || mode == 2 || backproject) return LLR_OK;
+  const size_t n = r->N * r->p.srank;
+  if (!r->Cprev.p) LLR_HIP(r->Cprev.alloc(n * 8));
+  llr_status st = momentum_device(r->ctx, r->Cb.as<float2>(), r->Cprev.as<float2>(), n, (float)r->momentum,
+                                  r->mom_first);
+  r->mom_first = false;
+  return st;
+}
+
 // backproject = false leaves the new coefficients in Cb (with a subspace) for
 // the fused wavelet to expand.
 llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, unsigned* smax,
@@ -740,6 +758,7 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
       st = r->s16 ? project_s16_device(ctx, r->X.p, F, r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank, 1.f)
                   : project_bf16_device(ctx, r->X.as<float2>(), F, r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank);
       if (st) return st;
+      if ((st = apply_momentum(r, mode, backproject))) return st;
     }
     {
       StageTimer t(r, 0);
@@ -758,6 +777,7 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
     {
       StageTimer t(r, 1);
       st = project_device(ctx, r->X.as<float2>(), r->Cb.as<float2>(), r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank);
+      if (!st) st = apply_momentum(r, mode, backproject);
     }
     if (st) return st;
     {
@@ -1117,13 +1137,14 @@ static llr_status backproject_all(llr_recon* r, const float2* C, float scale = 1
 
 // Iterations it = 0 .. n_max-1 of the prepared run; stop (if > 0) ends the run
 // once the relative residual falls to that value. Relative residuals out.
-static llr_status run_iterations(llr_recon* r, size_t n_max, double stop, double* err_hist, size_t* iters_done) {
+static llr_status run_iterations(llr_recon* r, size_t n_max, double stop, double* err_hist, size_t* iters_done,
+                                 size_t min_iter = 0) {
   llr_ctx* ctx = r->ctx;
   llr_status st;
   size_t it = 0;
   for (; it < n_max; ++it) {
     if ((st = recon_iteration(r, it))) return st;
-    if (stop > 0) {
+    if (stop > 0 && it + 1 >= min_iter) {
       double e = 0;
       LLR_HIP(hipStreamSynchronize(ctx->stream));
       LLR_HIP(hipMemcpy(&e, r->hist.as<double>() + it, 8, hipMemcpyDeviceToHost));
@@ -1275,11 +1296,18 @@ llr_status llr_recon_run_learned(llr_recon* r, llr_unet* net, const llr_warm_par
   if (!(wp->residual_stop > 0)) return fail(LLR_ERR_ARG, "residual_stop must be positive");
   const size_t budget = wp->max_iter ? std::min(wp->max_iter, r->p.n_iter) : r->p.n_iter;
   LLR_HIP(hipSetDevice(r->ctx->device));
+  if (wp->momentum < 0 || wp->momentum >= 1) return fail(LLR_ERR_ARG, "momentum must be in [0, 1)");
+  if (wp->momentum > 0 && !r->fuse_wavelet)
+    return fail(LLR_ERR_UNSUPPORTED, "momentum needs the wavelet fused with the subspace back-projection");
   llr_status st = warm_start(r, net, wp->to_net, wp->from_net, nullptr, nullptr);
   if (st) return st;
   size_t done = 0;
   std::vector<double> h(std::max<size_t>(r->p.n_iter, 1));
-  if ((st = run_iterations(r, budget, wp->residual_stop, h.data(), &done))) return st;
+  r->momentum = wp->momentum;
+  r->mom_first = true;
+  st = run_iterations(r, budget, wp->residual_stop, h.data(), &done, wp->min_iter);
+  r->momentum = 0.0;
+  if (st) return st;
   const bool reached = done > 0 && h[done - 1] <= wp->residual_stop;
   if (fell_back) *fell_back = reached ? 0 : 1;
   if (!reached) {

@@ -121,6 +121,8 @@ llr_status ramp_from_argmax_device(llr_ctx* ctx, float2* tab, const size_t* dims
 // projection's measured set is the set of nonzero samples)
 llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, float s, bool keep_nonzero = false);
 llr_status from_h16_device(llr_ctx* ctx, const void* src, float2* dst, size_t n, float s);
+// C <- (1 + beta) C - beta P, P <- the old C (first: P <- C only); n complex values, n even
+llr_status momentum_device(llr_ctx* ctx, float2* C, float2* P, size_t n, float beta, bool first);
 llr_status energy_partials(llr_ctx* ctx, const float2* y, size_t n, double* partials, size_t nparts);
 llr_status reduce_strided(llr_ctx* ctx, const double* partials, size_t n, size_t stride, double* out);
 llr_status reduce_at(llr_ctx* ctx, const double* partials, size_t n, double* out, const unsigned* at, double* tmp);

@@ -237,6 +237,21 @@ __global__ void __launch_bounds__(256) k_from_h16(const uint2* __restrict__ src,
   }
 }
 
+// Momentum on subspace coefficients: c = C; C = (1 + beta) c - beta P; P = c
+// (first: P = C only), two complex values per thread
+__global__ void __launch_bounds__(256) k_momentum(float4* __restrict__ C, float4* __restrict__ P, long long n2,
+                                                   float beta, int first) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n2; i += (long long)gridDim.x * blockDim.x) {
+    const float4 c = C[i];
+    if (!first) {
+      const float4 q = P[i];
+      const float a = 1.f + beta;
+      C[i] = make_float4(a * c.x - beta * q.x, a * c.y - beta * q.y, a * c.z - beta * q.z, a * c.w - beta * q.w);
+    }
+    P[i] = c;
+  }
+}
+
 __global__ void k_under_sample(float2* data, long long nx, long long pe, long long nq, const float2* mask) {
   long long total = nx * pe * nq;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total; i += (long long)gridDim.x * blockDim.x) {
@@ -355,6 +370,14 @@ llr_status to_h16_device(llr_ctx* ctx, const float2* src, void* dst, size_t n, f
   else
     hipLaunchKernelGGL(k_to_h16<false>, dim3(grid_for(n / 2, 256, 65536u)), dim3(256), 0, ctx->stream,
                        (const float4*)src, (uint2*)dst, (long long)(n / 2), s);
+  LLR_HIP(hipGetLastError());
+  return LLR_OK;
+}
+
+llr_status momentum_device(llr_ctx* ctx, float2* C, float2* P, size_t n, float beta, bool first) {
+  if (n % 2) return fail(LLR_ERR_UNSUPPORTED, "momentum needs an even number of coefficients");
+  hipLaunchKernelGGL(k_momentum, dim3(grid_for(n / 2, 256, 65536u)), dim3(256), 0, ctx->stream, (float4*)C, (float4*)P,
+                     (long long)(n / 2), beta, first ? 1 : 0);
   LLR_HIP(hipGetLastError());
   return LLR_OK;
 }

@@ -30,7 +30,7 @@ LLR_PREC_FP32 = 0
 LLR_PREC_MIXED = 1
 LLR_STORE_FP32 = 0
 LLR_STORE_FP16 = 1
-ABI_VERSION = 106
+ABI_VERSION = 107
 LLR_NUM_STAGES = 15
 # llr_recon_stage_times order; fft_x/fft_y are the forward passes, *_adjoint the
 # inverse ones; wavelet_* are the wavelet's kernels (also inside "wavelet")
@@ -111,6 +111,8 @@ class WarmParams(ctypes.Structure):
         ("from_net", ctypes.c_void_p),
         ("residual_stop", ctypes.c_double),
         ("max_iter", ctypes.c_size_t),
+        ("momentum", ctypes.c_double),
+        ("min_iter", ctypes.c_size_t),
     ]
 
 

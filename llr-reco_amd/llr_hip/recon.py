@@ -186,8 +186,9 @@ class Recon:
                                                     ctypes.byref(res)))
         return sc.value, res.value
 
-    def run_learned(self, net, residual_stop, to_net=None, from_net=None, max_iter=0):
-        """Warm start + iterations to residual_stop, cold-start fallback (llr_recon_run_learned).
+    def run_learned(self, net, residual_stop, to_net=None, from_net=None, max_iter=0, momentum=0.0, min_iter=0):
+        """Warm start + iterations (at least min_iter, with momentum in coefficient space
+        when momentum > 0) to residual_stop, cold-start fallback (llr_recon_run_learned).
         Returns (relative residual history, fell_back)."""
         from .unet import _mix, _mptr
         K = self.srank
@@ -197,6 +198,8 @@ class Recon:
         wp.from_net = ctypes.c_void_p(fn.ctypes.data) if fn is not None else None
         wp.residual_stop = float(residual_stop)
         wp.max_iter = int(max_iter)
+        wp.momentum = float(momentum)
+        wp.min_iter = int(min_iter)
         hist = np.zeros(max(1, self.n_iter), np.float64)
         done = ctypes.c_size_t()
         fb = ctypes.c_int()

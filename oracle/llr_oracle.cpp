@@ -20,10 +20,11 @@
 // for the loop (lib.rs:262-341) an independent numpy FP64 restatement of two
 // iterations plus two closed forms (fully sampled: A^H y after every
 // iteration; identity proximal step: the zero-filled adjoint is a fixed point
-// with zero residual). The multi-level wavelet is pinned against explicit
-// analysis/synthesis matrices built from the single-level formulas of the
-// reference's CUDA kernels (kernels.cu:155-268): level rule, subband set and
-// odd-length bookkeeping.
+// with zero residual), and the same restatement with a given first iterate
+// and momentum (the extension used by the learned warm start). The
+// multi-level wavelet is pinned against explicit analysis/synthesis matrices
+// built from the single-level formulas of the reference's CUDA kernels
+// (kernels.cu:155-268): level rule, subband set and odd-length bookkeeping.
 //
 // Parity status: parity unpinned only where the un-vendored `dwt` crate
 // (lib.rs:439-454, git HEAD, no revision) fixes a convention: its level count
@@ -783,6 +784,8 @@ struct orc_recon_params {
   size_t snap_iter;      // > 0: copy the iterate into snap after this many iterations
   void* snap;            // N*nq complex (T precision), or null
   const float* img_corr; // nq x 3 image corrections (signal_model.rs:34-79), or null (zero, lib.rs:148)
+  const void* x0;        // N*nq complex (T precision): the first iterate instead of the zero-filled adjoint, or null
+  double momentum;       // > 0: LLR acts on x_k + momentum (x_k - x_{k-1}) (extension, image space)
 };
 
 namespace orc {
@@ -924,9 +927,29 @@ static int recon(const orc_recon_params* p, const C<T>* meas, C<T>* x, double* e
     }
   }
   if (out_lambda) *out_lambda = (double)lam;
+  // the threshold always comes from the zero-filled adjoint; a given first
+  // iterate replaces it only afterwards
+  if (p->x0) std::memcpy(x, p->x0, sizeof(C<T>) * N * nq);
 
+  // Momentum (an extension with no counterpart in the reference): the LLR
+  // step receives x_k + b (x_k - x_{k-1}), where x_k is the iterate the loop
+  // hands over (after the adjoint) and x_{k-1} the previous one, both
+  // un-extrapolated; the first iteration only records x_0.
+  std::vector<C<T>> xprev;
+  const T beta = (T)p->momentum;
   for (size_t it = 0; it < p->n_iter; ++it) {
     const size_t* shift = p->shifts + 3 * it;
+    if (p->momentum > 0) {
+      if (it == 0) {
+        xprev.assign(x, x + N * nq);
+      } else {
+        for (size_t i = 0; i < N * nq; ++i) {
+          const C<T> cur = x[i];
+          x[i] = cur + beta * (cur - xprev[i]);
+          xprev[i] = cur;
+        }
+      }
+    }
     llr(shift, p->llr_mode, lam, true);
     if (p->wavelet_alpha > 0)
       for (size_t q = 0; q < nq; ++q) wavelet_denoise<T>(x + q * N, vol, (T)p->wavelet_alpha, p->wavelet_levels);

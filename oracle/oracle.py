@@ -265,12 +265,14 @@ class ReconParams(ctypes.Structure):
         ("snap_iter", ctypes.c_size_t),
         ("snap", ctypes.c_void_p),
         ("img_corr", ctypes.c_void_p),
+        ("x0", ctypes.c_void_p),
+        ("momentum", ctypes.c_double),
     ]
 
 
 def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda_factor=0.02,
           wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, phase_mode=None, partial_blocks=False,
-          snapshot_after=0, img_corr=None):
+          snapshot_after=0, img_corr=None, x0=None, momentum=0.0):
     """Run the reconstruction loop of llr_recon_exec (lib.rs:227-341) on CPU.
 
     partial_blocks: ceil(vol/blk) block grid with truncated edge blocks (an
@@ -280,6 +282,11 @@ def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda
     shifts: (n_iter, 3) integer block shifts. Returns (image, err_hist, lambda, ksp_corr).
     snapshot_after > 0: also return the iterate after that many iterations
     (a fifth element).
+    x0: the first iterate (flat, same dtype as meas) in place of the
+    zero-filled adjoint; the threshold still comes from the adjoint.
+    momentum > 0: the LLR step acts on x_k + momentum (x_k - x_{k-1}) (an
+    extension with no counterpart in the reference; the first iteration only
+    records x_0).
     """
     s = _suf(meas)
     if phase_mode is None:
@@ -312,6 +319,12 @@ def recon(meas, vol, nq, blk, shifts, llr_mode=1, rank=0, svt_lambda=0.0, lambda
         ic = np.ascontiguousarray(img_corr, np.float32).reshape(-1)
         assert ic.size == 3 * nq
         p.img_corr = ctypes.c_void_p(ic.ctypes.data)
+    xs = None
+    if x0 is not None:
+        xs = np.ascontiguousarray(x0, meas.dtype).reshape(-1)
+        assert xs.size == meas.size
+        p.x0 = ctypes.c_void_p(xs.ctypes.data)
+    p.momentum = float(momentum)
     snap = None
     if snapshot_after:
         snap = np.zeros_like(meas)

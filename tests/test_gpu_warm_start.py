@@ -203,3 +203,47 @@ def test_trained_warm_start_gains_over_cold_start(llr):
     assert e_learned <= 1.01 * e_cold
     assert e_learned < e_same
     net.close()
+
+
+def test_learned_momentum_matches_cpu_path(llr):
+    """Momentum of the learned run (coefficient space: (1 + b) C(x_k) - b C(x_{k-1})
+    entering the LLR step) against the CPU FP32 reference path's image-space
+    form x_k + b (x_k - x_{k-1}) from the same warm-start iterate, three
+    iterations, bar 1e-4 relative NRMSE; the same run without momentum matches
+    the plain CPU loop from that start, and the two differ by far more than
+    the bar."""
+    import oracle as O
+    import torch
+
+    from llr_hip import unet
+    from llr_hip import warm_start as W
+
+    ds = small_ds(46)
+    cfg = ds["cfg"]
+    net = unet.UNet(16, 16, 4, seed=2)
+    y = ds["y"].cpu().numpy()
+    img = torch.empty_like(ds["y"])
+    r = W.recon_for(ds)
+    r.warm_start(net, ds["to_net"], ds["from_net"])
+    r.image_to_device(img.data_ptr())
+    x0 = img.cpu().numpy()
+    lam, _ = r.info()
+    got = {}
+    for beta in (0.0, 0.6):
+        # an always-reached stop after exactly three iterations: no fallback
+        h, fell_back = r.run_learned(net, 1e30, ds["to_net"], ds["from_net"], max_iter=3, momentum=beta, min_iter=3)
+        assert not fell_back and len(h) == 3
+        r.image_to_device(img.data_ptr())
+        got[beta] = (img.cpu().numpy(), h)
+    r.close()
+    net.close()
+    for beta in (0.0, 0.6):
+        x32, h32, _, _ = O.recon(y, ds["vol"], ds["nq"], cfg["blk"], ds["shifts"][:3], llr_mode=1, svt_lambda=lam,
+                                 wavelet_alpha=cfg["wavelet_alpha"], subspace=ds["subspace"], x0=x0, momentum=beta)
+        e = float(np.linalg.norm(got[beta][0] - x32) / np.linalg.norm(x32))
+        print(f"[learned momentum {beta}] NRMSE vs cpu32 {e:.3e}; residuals {got[beta][1]} / {h32}", flush=True)
+        assert e <= 1e-4
+        assert np.allclose(got[beta][1], h32, rtol=1e-3)
+    d = float(np.linalg.norm(got[0.6][0] - got[0.0][0]) / np.linalg.norm(got[0.0][0]))
+    print(f"[learned momentum] 0.6 vs 0 differ by {d:.3e}", flush=True)
+    assert d > 1e-3

@@ -119,7 +119,8 @@ def test_c5_learned_fp16_inference_shape(llr):
     net = W.load_net(weights)
     to_net, from_net = W.tensor_mixing(ds["bvals"], ds["bvecs"], ds["subspace"])
     rl = _recon(ds, ds["cfg"]["n_iter"], precision="mixed", storage="fp16")
-    hl, fell_back = rl.run_learned(net, stop, to_net, from_net)
+    hl, fell_back = rl.run_learned(net, stop, to_net, from_net, momentum=W.LEARNED_MOMENTUM,
+                                   min_iter=W.LEARNED_MIN_ITER)
     rl.image_to_device(img.data_ptr())
     rl.close()
     net.close()
@@ -127,7 +128,7 @@ def test_c5_learned_fp16_inference_shape(llr):
     _log(f"[c5 learned fp16, seed 9001] {len(hl)} iterations (fallback {fell_back}), NRMSE vs truth {e_learned:.5f} "
          f"vs plain {e_plain:.5f} ({e_learned / e_plain:.4f} x); stop value {stop:.5e}")
     assert not fell_back
-    assert len(hl) < ds["cfg"]["n_iter"] and hl[-1] <= stop
+    assert W.LEARNED_MIN_ITER <= len(hl) < ds["cfg"]["n_iter"] // 2 and hl[-1] <= stop
     assert e_learned <= 1.01 * e_plain
     del img, ds
     torch.cuda.empty_cache()

@@ -218,10 +218,15 @@ def wavelet_denoise(vol_data, vol, alpha, levels=None):
 
 # ------------------------------------------------------------ the loop
 def recon(y, vol, nq, blk, shifts, mode="svt", lam_factor=0.02, rank=0, alpha=0.0, S=None, order="llr_first",
-          normalize=True, img_corr=None):
+          normalize=True, img_corr=None, x0=None, momentum=0.0, mom_prev="handed_over"):
     """lib.rs:224-341: energy, corrections, zero-filled adjoint, threshold from the
     adjoint's blocks at shift 0, then per iteration LLR -> wavelet -> forward ->
-    hard projection (residual / total energy) -> adjoint."""
+    hard projection (residual / total energy) -> adjoint.
+
+    x0 replaces the first iterate after the threshold is taken. momentum > 0
+    (the warm start's extension): the LLR step receives
+    x_k + momentum (x_k - x_{k-1}); mom_prev="extrapolated" keeps the
+    extrapolated iterate as x_{k-1} instead (a planted mistake)."""
     N = int(np.prod(vol))
     y = y.astype(np.complex128)
     energy = float(np.sum(np.abs(y) ** 2))
@@ -229,8 +234,17 @@ def recon(y, vol, nq, blk, shifts, mode="svt", lam_factor=0.02, rank=0, alpha=0.
     ic = np.zeros((nq, 3)) if img_corr is None else np.asarray(img_corr, np.float64).reshape(nq, 3)
     x = signal_model(y, vol, nq, kc, ic, forward=False)
     lam = lam_factor * llr_step(x.copy(), vol, nq, blk, (0, 0, 0), S, mode, 0.0, rank, modify=False)
+    if x0 is not None:
+        x = np.asarray(x0, np.complex128).copy()
     hist = []
+    prev = None
     for it in range(len(shifts)):
+        if momentum > 0:
+            cur = x.copy()
+            if prev is not None:
+                x[:] = cur + momentum * (cur - prev)
+            prev = x.copy() if mom_prev == "extrapolated" else cur
+
         def llr():
             llr_step(x, vol, nq, blk, tuple(int(s) for s in shifts[it]), S, mode, lam, rank)
 
@@ -296,6 +310,31 @@ def test_loop_matches_independent_restatement(vol, nq, blk, lf, alpha, srank):
     assert rel(x_o, x_sw) > 1e-4 and not np.allclose(h_o, h_sw, rtol=1e-6)
     _, h_raw, _, _ = recon(y, vol, nq, blk, shifts, lam_factor=lf, alpha=alpha, S=S, normalize=False)
     assert not np.allclose(h_o, h_raw, rtol=1e-3)
+
+
+def test_loop_with_first_iterate_and_momentum_matches_independent_restatement():
+    """The oracle's momentum (image space, first iteration only records x_0)
+    and given first iterate against the numpy restatement, three iterations;
+    the bar must reject momentum that keeps the extrapolated iterate as the
+    previous one, and the plain loop from the same start."""
+    vol, nq, blk, S = (8, 8, 8), 4, (4, 4, 2), np.linalg.qr(crandn(4, 2))[0]
+    y = measurements(vol, nq, 0.5, 11)
+    shifts = np.array([[3, 1, 2], [5, 6, 7], [1, 0, 4]]) % np.array(vol)
+    x0 = crandn(int(np.prod(vol)) * nq)
+    kw = dict(lam_factor=0.3, alpha=0.3, S=S)
+    x_np, h_np, lam_np, _ = recon(y, vol, nq, blk, shifts, x0=x0, momentum=0.6, **kw)
+    x_o, h_o, lam_o, _ = O.recon(y.astype(np.complex128), vol, nq, blk, shifts, llr_mode=1, lambda_factor=0.3,
+                                 wavelet_alpha=0.3, subspace=S, x0=x0, momentum=0.6)
+    assert abs(lam_o - lam_np) <= 1e-10 * lam_np
+    assert np.allclose(h_o, h_np, rtol=1e-9, atol=0)
+    assert rel(x_o, x_np) <= 1e-9
+    x_bad, _, _, _ = recon(y, vol, nq, blk, shifts, x0=x0, momentum=0.6, mom_prev="extrapolated", **kw)
+    x_plain, _, _, _ = recon(y, vol, nq, blk, shifts, x0=x0, **kw)
+    assert rel(x_o, x_bad) > 1e-4 and rel(x_o, x_plain) > 1e-4
+    # momentum 0 with a given start is the plain loop from that start
+    x_o0, _, _, _ = O.recon(y.astype(np.complex128), vol, nq, blk, shifts, llr_mode=1, lambda_factor=0.3,
+                            wavelet_alpha=0.3, subspace=S, x0=x0)
+    assert rel(x_o0, x_plain) <= 1e-9
 
 
 def test_loop_rank_mode_matches_independent_restatement():
