@@ -55,23 +55,35 @@ def build_measurements(preset, torch, llr, keep_truth=False):
 
 
 def stage_bytes(ds, eb=8):
-    """Algorithmic HBM bytes per launch of each timed kernel; eb = bytes per stored
-    complex value of the iterate (8 complex64, 4 half pairs). With a subspace the
-    LLR step works on the K coefficient planes in place (complex64) and the
-    wavelet's first level reads them and writes the iterate (the back-projection
-    is folded into it): its compulsory traffic is that read and that write, so
-    the kernels in between (level-0 z pass, deeper levels) carry none."""
+    """Algorithmic HBM bytes per launch of each timed stage; eb = bytes per stored
+    complex value of the iterate (8 complex64, 4 half pairs), D = N Q eb.
+
+    Operators (the whole-path accounting): the LLR step works on the K coefficient
+    planes in place (complex64) when there is a subspace; the wavelet reads those
+    planes (or X) and writes X, its compulsory traffic; the FFT passes read and
+    write the iterate, the z pass also reads the measurements.
+
+    Single kernels of the wavelet (their own compulsory inputs and outputs, the
+    intermediates of the wavelet's design included): with a subspace, level 0 is
+    the K planes' analysis (read C, write their coefficients), the per-volume z
+    synthesis (read the K planes' coefficients and the reconstructed level 1,
+    write every volume's xy-bands) and the xy synthesis (read the bands, write
+    X); without one, the xy analysis (read X, write the bands), the z pass (read
+    and write the bands, write level 1's input) and the xy synthesis (read the
+    bands and level 1, write X)."""
     N = int(ds["vol"][0] * ds["vol"][1] * ds["vol"][2])
     Q = ds["nq"]
     sub = bool(ds["cfg"]["srank"])
     K = ds["cfg"]["srank"] or Q
     D = N * Q * eb
+    CK = N * K * 8
     return {
-        "llr_block": 2 * N * K * 8 if sub else 2 * D,
-        "subspace_project": D + N * K * 8,               # read X, write C
-        "wavelet": N * K * 8 + D if sub else 2 * D,      # read C (or X), write X
-        "wavelet_level0_in": N * K * 8 if sub else D,    # reads C (or X)
-        "wavelet_level0_out": D,                         # writes X
+        "llr_block": 2 * CK if sub else 2 * D,
+        "subspace_project": D + CK,                      # read X, write C
+        "wavelet": CK + D if sub else 2 * D,             # read C (or X), write X
+        "wavelet_level0_in": 2 * CK if sub else 2 * D,
+        "wavelet_level0_mid": CK + D // 8 + D if sub else 2 * D + D // 8,
+        "wavelet_level0_out": 2 * D if sub else 2 * D + D // 8,
         "fft_x": 2 * D,
         "fft_x_adjoint": 2 * D,
         "fft_y": 2 * D,
@@ -80,7 +92,8 @@ def stage_bytes(ds, eb=8):
     }
 
 
-# stages that are one kernel each (candidates for the roofline's dominant kernel)
+# stages that are one kernel each (candidates for the roofline's dominant kernel;
+# with a subspace the level-0 analysis stage is two kernels on the K planes)
 KERNEL_STAGES = {
     "llr_block": "k_llr_block / k_llr_block_mixed",
     "subspace_project": "k_project / k_project_s16",
@@ -89,8 +102,8 @@ KERNEL_STAGES = {
     "fft_y": "k_fft_pass (y, forward)",
     "fft_y_adjoint": "k_fft_pass (y, adjoint)",
     "fft_z_project": "k_fft_z_project",
-    "wavelet_level0_in": "k_wav_xy_an (level 0)",
-    "wavelet_level0_mid": "k_wav_z_expand / k_wav_z (level 0)",
+    "wavelet_level0_in": "k_wav_xy_an + k_wav_zan_k on the K planes / k_wav_xy_an (level 0)",
+    "wavelet_level0_mid": "k_wav_zsyn_k4 / k_wav_z (level 0)",
     "wavelet_level0_out": "k_wav_xy_syn (level 0)",
 }
 
@@ -362,9 +375,12 @@ def main():
                 "traffic": measured_traffic(dom, ds, form), "alg_bytes_per_launch": nbytes,
                 "avg_launch_ms": st[dom]["avg_ms"], "total_ms": st[dom]["total_ms"],
                 "ranking": sorted(((k, st[k]["total_ms"]) for k in cands), key=lambda t: -t[1])[:6]}
-        wt = measured_traffic("wavelet", ds, form)
-        if wt and "wavelet" in st:
-            roof["wavelet_traffic_over_alg"] = round(wt / stage_bytes(ds, eb)["wavelet"], 2)
+        if "wavelet" in st:
+            wb = stage_bytes(ds, eb)["wavelet"]
+            roof["wavelet_stage_alg_GBps"] = round(wb / (st["wavelet"]["avg_ms"] * 1e-3) / 1e9, 1)
+            wt = measured_traffic("wavelet", ds, form)
+            if wt:
+                roof["wavelet_traffic_over_alg"] = round(wt / wb, 2)
 
     gs = gather_scatter(ds, torch)
 

@@ -86,9 +86,9 @@ llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const
 size_t llr_block_mixed_smem(long long m, int nc);
 llr_status dti_scalars_device(llr_ctx* ctx, const float2* vols, size_t n_vox, size_t nq, const float* bvals,
                               const float* bvecs, float b0_tol, float* out);
-// Level 0 of the wavelet taken from subspace coefficients: the iterate is
-// X_q = sum_k C_k conj(S[q][k]) at every voxel (K planes C, FP32); BK is FP32
-// workspace of wavelet_subspace_elems values.
+// The wavelet of an iterate given by subspace coefficients: X_q = sum_k C_k
+// conj(S[q][k]) at every voxel (K planes C, FP32). Every analysis runs on the
+// K planes; BK is their FP32 workspace of wavelet_subspace_elems values.
 struct WaveletFromSubspace {
   const float2* C;
   const float2* S;

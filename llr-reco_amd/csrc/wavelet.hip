@@ -258,150 +258,276 @@ __global__ void __launch_bounds__(256) k_wav_z(ZArgs a) {
   }
 }
 
-// Level-0 z pass of an iterate that is X_q = sum_k C_k conj(S[q][k]) at every
-// voxel (the LLR step's output when its block grid covers the volume). The
-// wavelet's analysis is linear with real filters, so the xy-bands of X_q are
-// the same combination of the xy-bands of the K coefficient planes (pass 1 run
-// on the planes, FP32). A workgroup keeps its ZT z-lines of all K plane bands
-// in registers and, for every volume in turn, expands them (FP32), analyses
-// the lines along z, shrinks and synthesises them back, writing the volume's
-// bands (and the next level's input) exactly as k_wav_z would have: the
-// back-projection and the per-volume xy analysis never touch HBM.
-struct ZXArgs {
-  const float2* BK;  // K planes of xy-bands, layout of B (FP32), plane pitch bkvp
+// Wavelet of an iterate that is X_q = sum_k C_k conj(S[q][k]) at every voxel
+// (the LLR step's output when its block grid covers the volume). Analysis is
+// linear with real filters, so every level's coefficients of X_q are the same
+// combination of the coefficients of the K planes C_k: every analysis pass
+// runs on the K planes only (FP32), and each volume is only synthesised. The
+// per-volume z synthesis expands its coefficients from the K planes'
+// (cmulc with the volume's row of S), shrinks them and synthesises the z-lines
+// of the xy-bands; the xy synthesis then runs as in the plain path. Levels are
+// synthesised coarse to fine, so a level's LL band reads the reconstructed
+// next level as its z-approximation (not shrunk) and the xy synthesis has no
+// next-level taps to add. The back-projection and every per-volume analysis
+// never touch HBM.
+
+// K-plane z analysis of one level: the xy-bands BK (box bx x by x az, plane
+// pitch bkvp) -> z coefficients WK (box bx x by x 2ncz: approximation at c,
+// detail at ncz + c; plane pitch wvp). Above the deepest level the LL band's
+// z-approximation is the next level's input; it goes to AK (ncx x ncy x ncz,
+// row pitch pa, plane pitch akvp) instead of WK.
+struct ZAnKArgs {
+  const float2* BK;
   long long bkvp;
-  const float2* S;   // nq x K, column-major
-  int K, nq;
-  ZArgs z;           // per-volume band buffer B (ST), next-level input A, geometry
+  float2* WK;
+  long long wvp;
+  float2* AK;
+  long long akvp;
+  int pa;
+  long long bx, by;
+  int az, ncz, ncx, ncy, px, deepest, K;
+  Db2F f;
 };
 
-constexpr int ZX_KM = 8;  // coefficient planes held per thread (K <= 8)
-
-// grid.x = tiles_x * by: one tile of ZT x-neighbour z-lines at one Y, ZT * NCH
-// threads. Thread (t = line, r = chunk) owns the CH consecutive samples z in
-// [CH r, CH r + CH) of its line (CH even, NCH CH >= az, az a multiple of CH)
-// and the coefficients c with 2c in that range (the last chunk also owns
-// c = az/2): analysis needs the two samples before the chunk and synthesis the
-// first coefficient after it, which come from the neighbouring chunks through
-// LDS; everything else stays in registers.
-template <int CH, int NCH, int KK, class ST>
-__global__ void __launch_bounds__(ZT * NCH) k_wav_z_expand(ZXArgs g) {
-  using E = typename ST::T;
-  const ZArgs& a = g.z;
-  constexpr int NC = CH / 2 + 1;                 // coefficient slots per thread (the last one: c = az/2)
-  __shared__ float2 h_x[NCH][2][ZT];             // last two samples of each chunk
-  __shared__ float2 h_c[NCH][2][ZT];             // first (lo, hi) coefficient of each chunk
-  const int tid = threadIdx.x;
-  const int t = tid % ZT, r = tid / ZT;
-  const int Y = blockIdx.x / a.tiles_x;
-  const int X = (blockIdx.x % a.tiles_x) * ZT + t;
-  const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx;
+// block = 64 columns x 4 coefficients; grid = (ceil(bx / 64), by, K ceil(ncz / 4))
+__global__ void __launch_bounds__(256) k_wav_zan_k(ZAnKArgs a) {
+  const int X = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+  const int Y = blockIdx.y;
+  const int cq = (a.ncz + 3) / 4;
+  const int k = blockIdx.z / cq;
+  const int c = (blockIdx.z % cq) * 4 + (int)(threadIdx.x >> 6);
+  if (X >= a.bx || c >= a.ncz || k >= a.K) return;
+  if ((X < a.px ? X : X - a.px) >= a.ncx) return;  // padding column of the band box
   const long long zp = a.bx * a.by;
-  const int z0 = CH * r;
-  const int nch = a.az / CH;                     // chunks holding samples (az is a multiple of CH)
-  const bool live = r < nch;
-  const bool last = r == nch - 1;
-  // KK planes are held (KK = K, or 8 with planes K..7 zero)
-  float2 kv[KK][CH];
+  const float2* col = a.BK + k * a.bkvp + (long long)Y * a.bx + X;
+  float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
 #pragma unroll
-  for (int k = 0; k < KK; ++k)
+  for (int j = 0; j < 4; ++j) {
+    const float2 x = col[(long long)sym(2 * c + 1 - j, a.az) * zp];
+    fma2(sa, a.f.dlo[j], x);
+    fma2(sd, a.f.dhi[j], x);
+  }
+  float2* w = a.WK + k * a.wvp + (long long)Y * a.bx + X;
+  if (X < a.ncx && Y < a.ncy && !a.deepest)
+    a.AK[k * a.akvp + ((long long)c * a.ncy + Y) * a.pa + X] = sa;
+  else
+    w[(long long)c * zp] = sa;
+  w[(long long)(a.ncz + c) * zp] = sd;
+}
+
+// Per-volume z synthesis of one level from the K planes' z coefficients.
+struct ZSynArgs {
+  const float2* WK;  // K planes of z coefficients (FP32), box bx x by x 2ncz, plane pitch wvp
+  long long wvp;
+  const float2* S;   // nq x K, column-major
+  int K, nq;
+  const void* Ar;    // reconstructed next level per volume (ST; ncx x ncy x ncz, row pitch pa), null at the deepest level
+  long long arvp;
+  int pa;
+  void* B;           // per-volume xy-bands (ST), box bx x by x az, volume pitch bvp
+  long long bvp;
+  long long bx, by;
+  int az, ncz, ncx, ncy, px, deepest;
+  int zmid;          // B rows ordered (Y, z) instead of (z, Y): element (X, Y, z) at (Y az + z) bx + X
+  int vpg;           // volumes per workgroup: grid.y walks the volume groups
+  float alpha;
+  Db2F f;
+};
+
+// The coefficient pair (lo, hi) of slot c for volume v: expanded from the K
+// planes and shrunk, except the LL band's approximation above the deepest
+// level, which is the reconstructed next level as it is.
+// sum_k w[k] conj(sc[k]) as four FMAs per term
+template <int KK>
+__device__ __forceinline__ float2 expand(const float2 (&w)[KK], const float2 (&sc)[KK]) {
+  float re = w[0].x * sc[0].x, im = w[0].y * sc[0].x;
+  re = fmaf(w[0].y, sc[0].y, re);
+  im = fmaf(-w[0].x, sc[0].y, im);
 #pragma unroll
-    for (int i = 0; i < CH; ++i)
-      kv[k][i] = (k < g.K && live && act) ? g.BK[k * g.bkvp + (long long)Y * a.bx + X + (long long)(z0 + i) * zp]
-                                          : make_float2(0.f, 0.f);
-  const bool ll = X < a.ncx && Y < a.ncy;
-  for (int v = 0; v < g.nq; ++v) {
+  for (int k = 1; k < KK; ++k) {
+    re = fmaf(w[k].x, sc[k].x, re);
+    re = fmaf(w[k].y, sc[k].y, re);
+    im = fmaf(w[k].y, sc[k].x, im);
+    im = fmaf(-w[k].x, sc[k].y, im);
+  }
+  return make_float2(re, im);
+}
+
+// out[s] from the slots floor(s/2) (lo0, hi0) and floor(s/2) + 1 (lo1, hi1):
+// taps 2 and 0 for even s, 3 and 1 for odd s (kernels.cu:230-261)
+__device__ __forceinline__ float2 zsyn_tap(const Db2F& f, bool odd, float2 lo0, float2 hi0, float2 lo1, float2 hi1) {
+  float2 acc = make_float2(0.f, 0.f);
+  fma2(acc, f.rlo[odd ? 3 : 2], lo0);
+  fma2(acc, f.rhi[odd ? 3 : 2], hi0);
+  fma2(acc, f.rlo[odd ? 1 : 0], lo1);
+  fma2(acc, f.rhi[odd ? 1 : 0], hi1);
+  return acc;
+}
+
+// Any z length: thread = (column X, sample pair m) holding the K-plane
+// coefficients of slots m and m + 1 for every volume in turn (the slot shared
+// by two pairs is expanded by both). block = 32 columns x 8 pairs; grid =
+// (ceil(bx / 32), by, ceil(ceil(az / 2) / 8)).
+template <int KK, class ST>
+__global__ void __launch_bounds__(256) k_wav_zsyn_k(ZSynArgs a) {
+  using E = typename ST::T;
+  const int pgs = ((a.az + 1) / 2 + 7) / 8;
+  const int X = blockIdx.x * 32 + (int)(threadIdx.x & 31);
+  const int Y = blockIdx.z / pgs;
+  const int m = (blockIdx.z % pgs) * 8 + (int)(threadIdx.x >> 5);
+  if (X >= a.bx || m >= (a.az + 1) / 2) return;
+  if ((X < a.px ? X : X - a.px) >= a.ncx) return;
+  const bool lo_ar = X < a.ncx && Y < a.ncy && !a.deepest;
+  const long long zp = a.bx * a.by;
+  const long long xy = (long long)Y * a.bx + X;
+  float2 wl[2][KK], wh[2][KK];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int k = 0; k < KK; ++k) {
+      const bool on = k < a.K;
+      wl[j][k] = (on && !lo_ar) ? a.WK[k * a.wvp + (long long)(m + j) * zp + xy] : make_float2(0.f, 0.f);
+      wh[j][k] = on ? a.WK[k * a.wvp + (long long)(a.ncz + m + j) * zp + xy] : make_float2(0.f, 0.f);
+    }
+  const E* ar = lo_ar ? (const E*)a.Ar + ((long long)m * a.ncy + Y) * a.pa + X : nullptr;
+  const long long arz = (long long)a.pa * a.ncy;
+  const long long bz = a.zmid ? a.bx : zp;  // z step of B
+  E* out = (E*)a.B + (a.zmid ? (long long)Y * a.az * a.bx + X : xy) + (long long)(2 * m) * bz;
+  const bool two = 2 * m + 1 < a.az;
+  const int vend = min(a.nq, (int)blockIdx.y * a.vpg + a.vpg);
+  for (int v = blockIdx.y * a.vpg; v < vend; ++v) {
     float2 sc[KK];
 #pragma unroll
-    for (int k = 0; k < KK; ++k) sc[k] = k < g.K ? g.S[v + (long long)k * g.nq] : make_float2(0.f, 0.f);
-    // 1. this volume's samples of the chunk: x = sum_k C_k conj(S[v][k])
-    float2 x[CH];
+    for (int k = 0; k < KK; ++k) sc[k] = k < a.K ? a.S[v + (long long)k * a.nq] : make_float2(0.f, 0.f);
+    float2 lo[2], hi[2];
 #pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int k = 0; k < KK; ++k) acc = cadd(acc, cmulc(kv[k][i], sc[k]));
-      x[i] = acc;
+    for (int j = 0; j < 2; ++j) {
+      hi[j] = shrink(expand<KK>(wh[j], sc), a.alpha);
+      lo[j] = lo_ar ? ST::cvt(ar[v * a.arvp + j * arz]) : shrink(expand<KK>(wl[j], sc), a.alpha);
     }
-    h_x[r][0][t] = x[CH - 2];
-    h_x[r][1][t] = x[CH - 1];
-    __syncthreads();  // halos of samples ready (and the previous volume's coefficient halos consumed)
-    // samples z0 - 2 .. z0 + CH + 1 (symmetric extension at both ends)
-    float2 w[CH + 4];
-    if (r == 0) {
-      w[0] = x[1];
-      w[1] = x[0];
-    } else {
-      w[0] = h_x[r - 1][0][t];
-      w[1] = h_x[r - 1][1][t];
+    E* o = out + v * a.bvp;
+    ST::st(o, zsyn_tap(a.f, false, lo[0], hi[0], lo[1], hi[1]));
+    if (two) ST::st(o + bz, zsyn_tap(a.f, true, lo[0], hi[0], lo[1], hi[1]));
+  }
+}
+
+// z length az <= 4 NCH: thread (t = column, r = chunk) owns the samples
+// [4r, min(4r + 4, az)) and the slots 2r, 2r + 1 (always < ncz), holding their
+// K-plane coefficients while it walks the volumes in groups of G; each slot is
+// expanded once per volume. Samples 4r + 2, 4r + 3 also need slot 2r + 2: the
+// next chunk's first slot, passed through LDS for the G volumes of a group at
+// once (double-buffered: one barrier per group), or, for the last chunk when
+// that slot exists (2r + 2 <= ncz - 1), its own expansion of K-plane
+// coefficients kept in LDS. Rows of ZTL columns are 128 B of complex64.
+template <int ZTL, int NCH, int KK, int G, class ST>
+__global__ void __launch_bounds__(ZTL * NCH, 4) k_wav_zsyn_c(ZSynArgs a, const float2* __restrict__ S) {
+  using E = typename ST::T;
+  __shared__ float2 xch[2][G][2][NCH][ZTL];  // (lo, hi) of each chunk's first slot, G volumes
+  __shared__ float2 lastw[2][KK][ZTL];       // K-plane (lo, hi) of slot 2r + 2 of the last chunk
+  const int tid = threadIdx.x;
+  const int t = tid % ZTL, r = tid / ZTL;
+  const int tiles = (int)((a.bx + ZTL - 1) / ZTL);
+  const int Y = blockIdx.x / tiles;
+  const int X = (blockIdx.x % tiles) * ZTL + t;
+  const int nch = (a.az + 3) / 4;
+  const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx && r < nch;
+  const bool last = r == nch - 1;
+  const bool extra = act && last && 2 * r + 2 <= a.ncz - 1;
+  const int ns = act ? min(4, a.az - 4 * r) : 0;
+  const bool lo_ar = X < a.ncx && Y < a.ncy && !a.deepest;
+  const long long zp = a.bx * a.by;
+  const long long xy = (long long)Y * a.bx + X;
+  // planes K..KK-1 (KK = 8 for K < 8) hold zeros; their S column is clamped to K - 1
+  const float2* wk = a.WK + xy;
+  float2 wl[2][KK], wh[2][KK];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int k = 0; k < KK; ++k) {
+      const bool on = act && k < a.K;
+      wl[j][k] = (on && !lo_ar) ? wk[k * a.wvp + (long long)(2 * r + j) * zp] : make_float2(0.f, 0.f);
+      wh[j][k] = on ? wk[k * a.wvp + (long long)(a.ncz + 2 * r + j) * zp] : make_float2(0.f, 0.f);
     }
+  if (extra) {
 #pragma unroll
-    for (int i = 0; i < CH; ++i) w[2 + i] = x[i];
-    w[CH + 2] = x[CH - 1];  // used only by the last chunk: sym(az) = az - 1, sym(az + 1) = az - 2
-    w[CH + 3] = x[CH - 2];
-    // 2. analysis of c = z0/2 + m (m < CH/2), and c = az/2 for the last chunk:
-    //    a[c] = sum_k dlo[k] x[sym(2c + 1 - k)], local sample index 2m + 3 - k
-    float2 ca[NC], cd[NC];
-    ca[NC - 1] = cd[NC - 1] = make_float2(0.f, 0.f);
-    // slot NC - 1 (c = az/2) only for the last chunk; the others take it from the next chunk
+    for (int k = 0; k < KK; ++k) {
+      const bool on = k < a.K;
+      lastw[0][k][t] = (on && !lo_ar) ? wk[k * a.wvp + (long long)(2 * r + 2) * zp] : make_float2(0.f, 0.f);
+      lastw[1][k][t] = on ? wk[k * a.wvp + (long long)(a.ncz + 2 * r + 2) * zp] : make_float2(0.f, 0.f);
+    }
+  }
+  const long long arz = (long long)a.pa * a.ncy;
+  const E* ar = (act && lo_ar) ? (const E*)a.Ar + ((long long)(2 * r) * a.ncy + Y) * a.pa + X : nullptr;
+  const long long bz = a.zmid ? a.bx : zp;  // z step of B
+  E* out = (E*)a.B + (a.zmid ? (long long)Y * a.az * a.bx + X : xy) + (long long)(4 * r) * bz;
+  const int kmax = a.K - 1;
+  const int vbeg = blockIdx.y * a.vpg, vend = min(a.nq, vbeg + a.vpg);
+  for (int v0 = vbeg; v0 < vend; v0 += G) {
+    const int b = ((v0 - vbeg) / G) & 1;
+    E an[G][3];
+    if (ar) {
 #pragma unroll
-    for (int m = 0; m < NC; ++m) {
-      if (m == NC - 1 && !last) break;
-      float2 sa = make_float2(0.f, 0.f), sd = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float2 xv = w[2 * m + 3 - k];
-        fma2(sa, a.f.dlo[k], xv);
-        fma2(sd, a.f.dhi[k], xv);
+      for (int g = 0; g < G; ++g) {
+        const E* p = ar + (long long)min(v0 + g, vend - 1) * a.arvp;
+        an[g][0] = p[0];
+        an[g][1] = p[arz];
+        an[g][2] = extra ? p[2 * arz] : ST::zero();
       }
-      const int c = z0 / 2 + m;
-      sd = shrink(sd, a.alpha);
-      if (!ll || a.deepest) {
-        sa = shrink(sa, a.alpha);
-      } else {
-        if (live && (m < NC - 1 || last) && act)
-          ST::st((E*)a.A + v * a.avp + (long long)c * a.pa * a.ncy + (long long)Y * a.pa + X, sa);
-        sa = make_float2(0.f, 0.f);
-      }
-      ca[m] = sa;
-      cd[m] = sd;
     }
-    h_c[r][0][t] = ca[0];
-    h_c[r][1][t] = cd[0];
-    __syncthreads();  // coefficient halos ready; sample halos consumed
-    // 3. synthesis of the chunk's samples: out[s] uses coefficients s/2 and s/2 + 1
-    //    (the latter from the next chunk for the chunk's last two samples)
-    if (live && !last) {  // r + 1 < nch <= NCH
-      ca[CH / 2] = h_c[r + 1][0][t];
-      cd[CH / 2] = h_c[r + 1][1][t];
-    }
-    if (live && act) {
-      E* row = (E*)a.B + v * a.bvp + (long long)Y * a.bx + X;
+    float2 lo1[G], hi1[G];
 #pragma unroll
-      for (int i = 0; i < CH; ++i) {
-        // s = z0 + i; even s: taps j = 2 (c = s/2) and j = 0 (c = s/2 + 1); odd s: j = 3 and j = 1
-        const int m = i / 2;
-        float2 acc = make_float2(0.f, 0.f);
-        if ((i & 1) == 0) {
-          fma2(acc, a.f.rlo[2], ca[m]);
-          fma2(acc, a.f.rhi[2], cd[m]);
-          fma2(acc, a.f.rlo[0], ca[m + 1]);
-          fma2(acc, a.f.rhi[0], cd[m + 1]);
+    for (int g = 0; g < G; ++g) {
+      const int v = min(v0 + g, vend - 1);
+      float2 sc[KK];
+#pragma unroll
+      for (int k = 0; k < KK; ++k) sc[k] = S[v + (long long)min(k, kmax) * a.nq];
+      const float2 hi0 = shrink(expand<KK>(wh[0], sc), a.alpha);
+      hi1[g] = shrink(expand<KK>(wh[1], sc), a.alpha);
+      const float2 lo0 = ar ? ST::cvt(an[g][0]) : shrink(expand<KK>(wl[0], sc), a.alpha);
+      lo1[g] = ar ? ST::cvt(an[g][1]) : shrink(expand<KK>(wl[1], sc), a.alpha);
+      xch[b][g][0][r][t] = lo0;
+      xch[b][g][1][r][t] = hi0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int v = v0 + g;
+      if (v >= vend || !act) continue;
+      const float2 lo0 = xch[b][g][0][r][t], hi0 = xch[b][g][1][r][t];
+      float2 lo2 = make_float2(0.f, 0.f), hi2 = make_float2(0.f, 0.f);
+      if (!last) {  // r + 1 < nch <= NCH
+        lo2 = xch[b][g][0][r + 1][t];
+        hi2 = xch[b][g][1][r + 1][t];
+      } else if (extra) {
+        float2 sc[KK], w[KK];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) sc[k] = S[v + (long long)min(k, kmax) * a.nq];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) w[k] = lastw[1][k][t];
+        hi2 = shrink(expand<KK>(w, sc), a.alpha);
+        if (ar) {
+          lo2 = ST::cvt(an[g][2]);
         } else {
-          fma2(acc, a.f.rlo[3], ca[m]);
-          fma2(acc, a.f.rhi[3], cd[m]);
-          fma2(acc, a.f.rlo[1], ca[m + 1]);
-          fma2(acc, a.f.rhi[1], cd[m + 1]);
+#pragma unroll
+          for (int k = 0; k < KK; ++k) w[k] = lastw[0][k][t];
+          lo2 = shrink(expand<KK>(w, sc), a.alpha);
         }
-        ST::st(row + (long long)(z0 + i) * zp, acc);
       }
+      E* o = out + v * a.bvp;
+      ST::st(o, zsyn_tap(a.f, false, lo0, hi0, lo1[g], hi1[g]));
+      if (ns > 1) ST::st(o + bz, zsyn_tap(a.f, true, lo0, hi0, lo1[g], hi1[g]));
+      if (ns > 2) ST::st(o + 2 * bz, zsyn_tap(a.f, false, lo1[g], hi1[g], lo2, hi2));
+      if (ns > 3) ST::st(o + 3 * bz, zsyn_tap(a.f, true, lo1[g], hi1[g], lo2, hi2));
     }
   }
 }
 
+constexpr int ZS_KM = 8;  // coefficient planes held per thread (K <= 8)
+
 struct XYSynArgs {
   const void* B;
   long long bx, by, bvp;
+  long long bzp, bry;  // z-plane and row pitch of B (bx by and bx, or bx and az bx)
   int ncx, ncy, px;
   const void* Ap;    // reconstructed next level (ncx x ncy x ncz, row pitch pa) or null
   long long apvp;
@@ -437,7 +563,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   const int z = blockIdx.y % a.az;
   const long long v = blockIdx.y / a.az;
   const int ty0 = y0 / 2;
-  const E* src = (const E*)a.B + v * a.bvp + (long long)z * a.bx * a.by;
+  const E* src = (const E*)a.B + v * a.bvp + (long long)z * a.bzp;
   const E* ap = a.Ap ? (const E*)a.Ap + v * a.apvp : nullptr;
   // the LL band of this z slice picks up syn_z(Ap, 0): taps j with z+2-j even and in range
   int za = -1, zb = -1;
@@ -472,7 +598,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
       const int cy = ty0 + iy;
       lix[q] = ix;
       if (cy < a.ncy) {
-        lofs[q] = ((b >> 1) ? a.ncy + cy : cy) * (int)a.bx + ((b & 1) ? a.px : 0);
+        lofs[q] = ((b >> 1) ? a.ncy + cy : cy) * (int)a.bry + ((b & 1) ? a.px : 0);
         if (b == 0 && ap) aofs[q] = cy * a.pa;
       }
     }
@@ -617,9 +743,99 @@ size_t wavelet_workspace_elems(const size_t* dims, int L, size_t* coef_elems, si
   return c;
 }
 
+// Per-level band-box geometry shared by the passes.
+struct LevelBox {
+  size_t px, bvol, pa, avol;  // band row half-width (ncx padded to 16), band box, next-input pitch and box
+  int ncx, ncy, ncz;
+  long long bx, by;
+};
+
+static std::vector<LevelBox> level_boxes(const std::vector<std::array<size_t, 3>>& in_dims,
+                                         const std::vector<std::array<size_t, 3>>& cdims, int L) {
+  std::vector<LevelBox> lb(L);
+  for (int l = 0; l < L; ++l) {
+    LevelBox& q = lb[l];
+    q.ncx = (int)cdims[l][0] / 2;
+    q.ncy = (int)cdims[l][1] / 2;
+    q.ncz = (int)cdims[l][2] / 2;
+    q.px = pad16(cdims[l][0] / 2);
+    q.bx = 2 * (long long)q.px;
+    q.by = (long long)cdims[l][1];
+    q.bvol = 2 * q.px * cdims[l][1] * in_dims[l][2];
+    q.pa = l > 0 ? pad16(in_dims[l][0]) : 0;
+    q.avol = l > 0 ? q.pa * in_dims[l][1] * in_dims[l][2] : 0;
+  }
+  return lb;
+}
+
+// pass 1 of a level: xy analysis of nv volumes (or planes) `in` (row pitch ip1,
+// ax x ay x az) into the band box B
+template <class ST>
+static void launch_xy_an(hipStream_t s, const void* in, long long ip1, const std::array<size_t, 3>& in_dims,
+                         const LevelBox& q, void* B, size_t nv, const Db2F& f) {
+  XYAnArgs a1{};
+  a1.in = in;
+  a1.ip1 = ip1;
+  a1.ip2 = ip1 * (long long)in_dims[1];
+  a1.ivp = a1.ip2 * (long long)in_dims[2];
+  a1.B = B;
+  a1.bx = q.bx;
+  a1.by = q.by;
+  a1.bvp = (long long)q.bvol;
+  a1.px = (int)q.px;
+  a1.ax = (int)in_dims[0];
+  a1.ay = (int)in_dims[1];
+  a1.az = (int)in_dims[2];
+  a1.ncx = q.ncx;
+  a1.ncy = q.ncy;
+  a1.tiles_x = (q.ncx + AN_TCX - 1) / AN_TCX;
+  a1.groups_x = (a1.tiles_x + TPW - 1) / TPW;
+  a1.f = f;
+  const int tiles_y = (q.ncy + AN_TCY - 1) / AN_TCY;
+  dim3 g1((unsigned)(a1.groups_x * tiles_y), (unsigned)(in_dims[2] * nv));
+  hipLaunchKernelGGL(k_wav_xy_an<ST>, g1, dim3(256), 0, s, a1);
+}
+
+// pass 3 of a level: xy synthesis of the band box B (plus syn_z(Ap, 0) on the
+// LL band when Ap is set) into `out` (row pitch op1, the level's input box)
+template <class ST>
+static void launch_xy_syn(hipStream_t s, const void* B, const LevelBox& q, const void* Ap, long long apvp, int pa,
+                          void* out, long long op1, const std::array<size_t, 3>& od, size_t nv, const Db2F& f,
+                          bool zmid = false) {
+  XYSynArgs a3{};
+  a3.B = B;
+  a3.bx = q.bx;
+  a3.by = q.by;
+  a3.bzp = zmid ? q.bx : q.bx * q.by;
+  a3.bry = zmid ? q.bx * (long long)od[2] : q.bx;
+  a3.bvp = (long long)q.bvol;
+  a3.ncx = q.ncx;
+  a3.ncy = q.ncy;
+  a3.px = (int)q.px;
+  a3.ncz = q.ncz;
+  a3.Ap = Ap;
+  a3.apvp = Ap ? apvp : 0;
+  a3.pa = Ap ? pa : 0;
+  a3.out = out;
+  a3.op1 = op1;
+  a3.op2 = a3.op1 * (long long)od[1];
+  a3.ovp = a3.op2 * (long long)od[2];
+  a3.ax = (int)od[0];
+  a3.ay = (int)od[1];
+  a3.az = (int)od[2];
+  constexpr int TXS = SY_TX;  // (128 for half storage measured slower)
+  a3.tiles_x = (int)((od[0] + TXS - 1) / TXS);
+  a3.tpw = TPW;
+  a3.groups_x = (a3.tiles_x + a3.tpw - 1) / a3.tpw;
+  a3.f = f;
+  const int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
+  dim3 g3((unsigned)(a3.groups_x * tiles_y), (unsigned)(od[2] * nv));
+  hipLaunchKernelGGL((k_wav_xy_syn<ST, TXS>), g3, dim3(256), 0, s, a3);
+}
+
 template <class ST>
 static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const size_t* dims, float alpha, int L,
-                              void* coef_v, const WaveletFromSubspace* sub) {
+                              void* coef_v) {
   using E = typename ST::T;
   E* vols = (E*)vols_v;
   Db2F f = db2_filters();
@@ -632,115 +848,52 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
   }
   std::vector<std::array<size_t, 3>> in_dims, cdims;
   wavelet_geometry(dims, L, in_dims, cdims);
+  const std::vector<LevelBox> lb = level_boxes(in_dims, cdims, L);
   std::vector<E*> Bp(L), Ap(L, nullptr);
-  std::vector<size_t> bvol(L), avol(L, 0), px(L), pa(L, 0);
   {
     E* p = (E*)coef_v;
     for (int l = 0; l < L; ++l) {
-      px[l] = pad16(cdims[l][0] / 2);
-      bvol[l] = 2 * px[l] * cdims[l][1] * in_dims[l][2];
       Bp[l] = p;
-      p += bvol[l] * nvols;
+      p += lb[l].bvol * nvols;
     }
     for (int l = 1; l < L; ++l) {
-      pa[l] = pad16(in_dims[l][0]);
-      avol[l] = pa[l] * in_dims[l][1] * in_dims[l][2];
       Ap[l] = p;
-      p += avol[l] * nvols;
+      p += lb[l].avol * nvols;
     }
   }
   hipStream_t s = ctx->stream;
   for (int l = 0; l < L; ++l) {
     const size_t* in = in_dims[l].data();
-    const size_t* cd = cdims[l].data();
-    int ncx = (int)cd[0] / 2, ncy = (int)cd[1] / 2, ncz = (int)cd[2] / 2;
-    const long long bx = 2 * (long long)px[l], by = (long long)cd[1];
-    const bool fused = l == 0 && sub != nullptr;
-    // pass 1 (on the K coefficient planes, FP32, when fused)
-    XYAnArgs a1{};
-    a1.in = fused ? (const void*)sub->C : l == 0 ? (const void*)vols : (const void*)Ap[l];
-    a1.ip1 = l == 0 ? (long long)in[0] : (long long)pa[l];
-    a1.ip2 = a1.ip1 * (long long)in[1];
-    a1.ivp = a1.ip2 * (long long)in[2];
-    a1.B = fused ? (void*)sub->BK : (void*)Bp[l];
-    a1.bx = bx;
-    a1.by = by;
-    a1.bvp = (long long)bvol[l];
-    a1.px = (int)px[l];
-    a1.ax = (int)in[0];
-    a1.ay = (int)in[1];
-    a1.az = (int)in[2];
-    a1.ncx = ncx;
-    a1.ncy = ncy;
-    a1.tiles_x = (ncx + AN_TCX - 1) / AN_TCX;
-    a1.groups_x = (a1.tiles_x + TPW - 1) / TPW;
-    a1.f = f;
-    int tiles_y = (ncy + AN_TCY - 1) / AN_TCY;
-    dim3 g1((unsigned)(a1.groups_x * tiles_y), (unsigned)(in[2] * (fused ? (size_t)sub->K : nvols)));
+    const LevelBox& q = lb[l];
     {
       KernelTimer kt(ctx, l == 0 ? kStageWavIn : kStageWavDeep);
-      if (fused)
-        hipLaunchKernelGGL(k_wav_xy_an<StoreF32>, g1, dim3(256), 0, s, a1);
-      else
-        hipLaunchKernelGGL(k_wav_xy_an<ST>, g1, dim3(256), 0, s, a1);
+      launch_xy_an<ST>(s, l == 0 ? (const void*)vols : (const void*)Ap[l], l == 0 ? (long long)in[0] : (long long)q.pa,
+                       in_dims[l], q, Bp[l], nvols, f);
     }
     LLR_HIP(hipGetLastError());
     // pass 2
     ZArgs a2{};
     a2.B = Bp[l];
-    a2.bx = bx;
-    a2.by = by;
-    a2.bvp = (long long)bvol[l];
+    a2.bx = q.bx;
+    a2.by = q.by;
+    a2.bvp = (long long)q.bvol;
     a2.az = (int)in[2];
-    a2.ncz = ncz;
-    a2.ncx = ncx;
-    a2.ncy = ncy;
-    a2.px = (int)px[l];
-    a2.tiles_x = (int)((bx + ZT - 1) / ZT);
+    a2.ncz = q.ncz;
+    a2.ncx = q.ncx;
+    a2.ncy = q.ncy;
+    a2.px = (int)q.px;
+    a2.tiles_x = (int)((q.bx + ZT - 1) / ZT);
     a2.groups_x = (a2.tiles_x + TPW - 1) / TPW;
     a2.deepest = l == L - 1;
     a2.A = a2.deepest ? nullptr : Ap[l + 1];
-    a2.avp = a2.deepest ? 0 : (long long)avol[l + 1];
-    a2.pa = a2.deepest ? 0 : (int)pa[l + 1];
+    a2.avp = a2.deepest ? 0 : (long long)lb[l + 1].avol;
+    a2.pa = a2.deepest ? 0 : (int)lb[l + 1].pa;
     a2.alpha = alpha;
     a2.f = f;
-    size_t smem = (size_t)(in[2] + 2 * ncz) * ZT * sizeof(float2);  // LDS holds FP32 values
+    size_t smem = (size_t)(in[2] + 2 * q.ncz) * ZT * sizeof(float2);  // LDS holds FP32 values
     if (smem > 160 * 1024 || in[2] > 512) return fail(LLR_ERR_UNSUPPORTED, "z length too large for the wavelet z pass");
     int nzr = (int)((in[2] + (256 / ZT) - 1) / (256 / ZT));
-    if (fused) {
-      if (sub->K > ZX_KM) return fail(LLR_ERR_UNSUPPORTED, "fused wavelet: rank <= 8");
-      ZXArgs ax{};
-      ax.BK = sub->BK;
-      ax.bkvp = (long long)bvol[0];
-      ax.S = sub->S;
-      ax.K = sub->K;
-      ax.nq = (int)nvols;
-      ax.z = a2;
-      dim3 gx((unsigned)(a2.tiles_x * by));
-      // chunks of 2 (az <= 32 with 16 chunks) or 4 samples (16 or 32 chunks)
-      const int nch = in[2] <= 64 ? 16 : 32;
-      const int ch = 2 * (int)((in[2] + 2 * nch - 1) / (2 * nch));
-      if (in[2] % 2 || in[2] % ch || ch > 4) return fail(LLR_ERR_UNSUPPORTED, "fused wavelet: z length");
-      KernelTimer kt(ctx, kStageWavMid);
-#define ZX_LAUNCH(C, N)                                                                   \
-  if (sub->K == 7)                                                                        \
-    hipLaunchKernelGGL((k_wav_z_expand<C, N, 7, ST>), gx, dim3(ZT * N), 0, s, ax);       \
-  else                                                                                    \
-    hipLaunchKernelGGL((k_wav_z_expand<C, N, ZX_KM, ST>), gx, dim3(ZT * N), 0, s, ax);
-      if (nch == 16 && ch == 2) {
-        ZX_LAUNCH(2, 16)
-      } else if (nch == 16) {
-        ZX_LAUNCH(4, 16)
-      } else if (ch == 2) {
-        ZX_LAUNCH(2, 32)
-      } else {
-        ZX_LAUNCH(4, 32)
-      }
-#undef ZX_LAUNCH
-      LLR_HIP(hipGetLastError());
-      continue;
-    }
-    dim3 g2((unsigned)(a2.groups_x * by), (unsigned)nvols);
+    dim3 g2((unsigned)(a2.groups_x * q.by), (unsigned)nvols);
     KernelTimer kt(ctx, l == 0 ? kStageWavMid : kStageWavDeep);
 #define WZ_CASE(NN)                                                                                       \
   if (nzr <= NN) {                                                                                        \
@@ -754,37 +907,175 @@ static llr_status wavelet_run(llr_ctx* ctx, void* vols_v, size_t nvols, const si
     LLR_HIP(hipGetLastError());
   }
   for (int l = L - 1; l >= 0; --l) {
-    const size_t* od = in_dims[l].data();
-    const size_t* cd = cdims[l].data();
-    XYSynArgs a3{};
-    a3.B = Bp[l];
-    a3.bx = 2 * (long long)px[l];
-    a3.by = (long long)cd[1];
-    a3.bvp = (long long)bvol[l];
-    a3.ncx = (int)cd[0] / 2;
-    a3.ncy = (int)cd[1] / 2;
-    a3.px = (int)px[l];
-    a3.ncz = (int)cd[2] / 2;
-    a3.Ap = (l < L - 1) ? Ap[l + 1] : nullptr;
-    a3.apvp = (l < L - 1) ? (long long)avol[l + 1] : 0;
-    a3.pa = (l < L - 1) ? (int)pa[l + 1] : 0;
-    a3.out = l == 0 ? vols : Ap[l];
-    a3.op1 = l == 0 ? (long long)od[0] : (long long)pa[l];
-    a3.op2 = a3.op1 * (long long)od[1];
-    a3.ovp = a3.op2 * (long long)od[2];
-    a3.ax = (int)od[0];
-    a3.ay = (int)od[1];
-    a3.az = (int)od[2];
-    constexpr int TXS = SY_TX;  // (128 for half storage measured slower)
-    a3.tiles_x = (int)((od[0] + TXS - 1) / TXS);
-    a3.tpw = TPW;
-    a3.groups_x = (a3.tiles_x + a3.tpw - 1) / a3.tpw;
-    a3.f = f;
-    int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
-    dim3 g3((unsigned)(a3.groups_x * tiles_y), (unsigned)(od[2] * nvols));
+    const LevelBox& q = lb[l];
     {
       KernelTimer kt(ctx, l == 0 ? kStageWavOut : kStageWavDeep);
-      hipLaunchKernelGGL((k_wav_xy_syn<ST, TXS>), g3, dim3(256), 0, s, a3);
+      launch_xy_syn<ST>(s, Bp[l], q, l < L - 1 ? Ap[l + 1] : nullptr, l < L - 1 ? (long long)lb[l + 1].avol : 0,
+                        l < L - 1 ? (int)lb[l + 1].pa : 0, l == 0 ? (void*)vols : (void*)Ap[l],
+                        l == 0 ? (long long)in_dims[0][0] : (long long)q.pa, in_dims[l], nvols, f);
+    }
+    LLR_HIP(hipGetLastError());
+  }
+  return LLR_OK;
+}
+
+// K-plane workspace of the subspace wavelet (FP32 values): the xy-band box of
+// level 0 (reused by every level), the z coefficients of every level and the
+// next-level inputs of levels 1..L-1, each for K planes.
+struct SubspaceWork {
+  float2* BK;
+  std::vector<float2*> WK, AK;
+  std::vector<size_t> wvol;
+};
+
+static size_t subspace_work(const std::vector<std::array<size_t, 3>>& in_dims, const std::vector<LevelBox>& lb, int L,
+                            int K, float2* base, SubspaceWork* w) {
+  size_t off = lb[0].bvol * (size_t)K;
+  if (w) {
+    w->BK = base;
+    w->WK.assign(L, nullptr);
+    w->AK.assign(L, nullptr);
+    w->wvol.assign(L, 0);
+  }
+  for (int l = 0; l < L; ++l) {
+    const size_t wv = (size_t)lb[l].bx * (size_t)lb[l].by * 2 * (size_t)lb[l].ncz;
+    if (w) {
+      w->wvol[l] = wv;
+      w->WK[l] = base + off;
+    }
+    off += wv * (size_t)K;
+  }
+  for (int l = 1; l < L; ++l) {
+    if (w) w->AK[l] = base + off;
+    off += lb[l].avol * (size_t)K;
+  }
+  (void)in_dims;
+  return off;
+}
+
+template <class ST>
+static llr_status wavelet_run_subspace(llr_ctx* ctx, void* vols_v, size_t nvols, const size_t* dims, float alpha,
+                                       int L, void* coef_v, const WaveletFromSubspace& sub) {
+  using E = typename ST::T;
+  if (sub.K < 1 || sub.K > ZS_KM) return fail(LLR_ERR_UNSUPPORTED, "fused wavelet: rank <= 8");
+  Db2F f = db2_filters();
+  std::vector<std::array<size_t, 3>> in_dims, cdims;
+  wavelet_geometry(dims, L, in_dims, cdims);
+  const std::vector<LevelBox> lb = level_boxes(in_dims, cdims, L);
+  SubspaceWork w;
+  subspace_work(in_dims, lb, L, sub.K, sub.BK, &w);
+  std::vector<E*> Bp(L), Ap(L, nullptr);
+  {
+    E* p = (E*)coef_v;
+    for (int l = 0; l < L; ++l) {
+      Bp[l] = p;
+      p += lb[l].bvol * nvols;
+    }
+    for (int l = 1; l < L; ++l) {
+      Ap[l] = p;
+      p += lb[l].avol * nvols;
+    }
+  }
+  hipStream_t s = ctx->stream;
+  // analysis of the K planes, every level
+  for (int l = 0; l < L; ++l) {
+    const LevelBox& q = lb[l];
+    KernelTimer kt(ctx, l == 0 ? kStageWavIn : kStageWavDeep);
+    launch_xy_an<StoreF32>(s, l == 0 ? (const void*)sub.C : (const void*)w.AK[l],
+                           l == 0 ? (long long)in_dims[0][0] : (long long)q.pa, in_dims[l], q, w.BK, (size_t)sub.K, f);
+    LLR_HIP(hipGetLastError());
+    ZAnKArgs z{};
+    z.BK = w.BK;
+    z.bkvp = (long long)q.bvol;
+    z.WK = w.WK[l];
+    z.wvp = (long long)w.wvol[l];
+    const bool deepest = l == L - 1;
+    z.AK = deepest ? nullptr : w.AK[l + 1];
+    z.akvp = deepest ? 0 : (long long)lb[l + 1].avol;
+    z.pa = deepest ? 0 : (int)lb[l + 1].pa;
+    z.bx = q.bx;
+    z.by = q.by;
+    z.az = (int)in_dims[l][2];
+    z.ncz = q.ncz;
+    z.ncx = q.ncx;
+    z.ncy = q.ncy;
+    z.px = (int)q.px;
+    z.deepest = deepest;
+    z.K = sub.K;
+    z.f = f;
+    dim3 gz((unsigned)((q.bx + 63) / 64), (unsigned)q.by, (unsigned)(sub.K * ((q.ncz + 3) / 4)));
+    hipLaunchKernelGGL(k_wav_zan_k, gz, dim3(256), 0, s, z);
+    LLR_HIP(hipGetLastError());
+  }
+  // synthesis of every volume, coarse to fine
+  for (int l = L - 1; l >= 0; --l) {
+    const LevelBox& q = lb[l];
+    const int az = (int)in_dims[l][2];
+    ZSynArgs a{};
+    a.WK = w.WK[l];
+    a.wvp = (long long)w.wvol[l];
+    a.S = sub.S;
+    a.K = sub.K;
+    a.nq = (int)nvols;
+    const bool deepest = l == L - 1;
+    a.Ar = deepest ? nullptr : (const void*)Ap[l + 1];
+    a.arvp = deepest ? 0 : (long long)lb[l + 1].avol;
+    a.pa = deepest ? 0 : (int)lb[l + 1].pa;
+    a.B = Bp[l];
+    a.bvp = (long long)q.bvol;
+    a.bx = q.bx;
+    a.by = q.by;
+    a.az = az;
+    a.ncz = q.ncz;
+    a.ncx = q.ncx;
+    a.ncy = q.ncy;
+    a.px = (int)q.px;
+    a.deepest = deepest;
+    a.zmid = 1;
+    a.alpha = alpha;
+    a.f = f;
+    // volume groups so that a launch has about 2048 workgroups or more (the deep
+    // levels have few columns; each workgroup walks its volumes in turn)
+    auto vgroups = [&](size_t wgs) {
+      size_t vg = (2048 + wgs - 1) / wgs;
+      vg = vg < 1 ? 1 : (vg > nvols ? nvols : vg);
+      a.vpg = (int)((nvols + vg - 1) / vg);
+      return (unsigned)((nvols + a.vpg - 1) / a.vpg);
+    };
+    {
+      KernelTimer kt(ctx, l == 0 ? kStageWavMid : kStageWavDeep);
+      const int nch = (az + 3) / 4;
+      if (nch <= 32) {
+        constexpr int ZTL = 16, GV = 2;
+        const unsigned tiles = (unsigned)(((q.bx + ZTL - 1) / ZTL) * q.by);
+        const dim3 grid(tiles, vgroups(tiles));
+#define ZSC_LAUNCH(NN)                                                                                \
+  if (sub.K == 7)                                                                                     \
+    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, 7, GV, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);      \
+  else                                                                                                \
+    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, ZS_KM, GV, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);
+        if (nch <= 8) {
+          ZSC_LAUNCH(8)
+        } else if (nch <= 16) {
+          ZSC_LAUNCH(16)
+        } else {
+          ZSC_LAUNCH(32)
+        }
+#undef ZSC_LAUNCH
+      } else {
+        const unsigned tx = (unsigned)((q.bx + 31) / 32), tz = (unsigned)(q.by * (((az + 1) / 2 + 7) / 8));
+        dim3 g(tx, vgroups((size_t)tx * tz), tz);
+        if (sub.K == 7)
+          hipLaunchKernelGGL((k_wav_zsyn_k<7, ST>), g, dim3(256), 0, s, a);
+        else
+          hipLaunchKernelGGL((k_wav_zsyn_k<ZS_KM, ST>), g, dim3(256), 0, s, a);
+      }
+    }
+    LLR_HIP(hipGetLastError());
+    {
+      KernelTimer kt(ctx, l == 0 ? kStageWavOut : kStageWavDeep);
+      launch_xy_syn<ST>(s, Bp[l], q, nullptr, 0, 0, l == 0 ? vols_v : (void*)Ap[l],
+                        l == 0 ? (long long)in_dims[0][0] : (long long)q.pa, in_dims[l], nvols, f, true);
     }
     LLR_HIP(hipGetLastError());
   }
@@ -797,27 +1088,27 @@ llr_status wavelet_denoise_device(llr_ctx* ctx, void* vols, size_t nvols, const 
                                   void* coef, void* t1, void* t2, bool h16) {
   (void)t1;
   (void)t2;
-  return h16 ? wavelet_run<StoreF16>(ctx, vols, nvols, dims, alpha, L, coef, nullptr)
-              : wavelet_run<StoreF32>(ctx, vols, nvols, dims, alpha, L, coef, nullptr);
+  return h16 ? wavelet_run<StoreF16>(ctx, vols, nvols, dims, alpha, L, coef)
+              : wavelet_run<StoreF32>(ctx, vols, nvols, dims, alpha, L, coef);
 }
 
 size_t wavelet_subspace_elems(const size_t* dims, int L, int K) {
   if (L <= 0) return 0;
   std::vector<std::array<size_t, 3>> in_dims, cdims;
   wavelet_geometry(dims, L, in_dims, cdims);
-  return 2 * pad16(cdims[0][0] / 2) * cdims[0][1] * in_dims[0][2] * (size_t)K;
+  return subspace_work(in_dims, level_boxes(in_dims, cdims, L), L, K, nullptr, nullptr);
 }
 
 bool wavelet_subspace_supported(const size_t* dims, int L, int K) {
-  const size_t nz = dims[2], nch = nz <= 64 ? 16 : 32, ch = 2 * ((nz + 2 * nch - 1) / (2 * nch));
-  return L >= 1 && K >= 1 && K <= ZX_KM && nz >= 4 && nz <= 128 && nz % 2 == 0 && nz % ch == 0;
+  (void)dims;
+  return L >= 1 && K >= 1 && K <= ZS_KM;
 }
 
 llr_status wavelet_denoise_subspace(llr_ctx* ctx, void* vols, size_t nvols, const size_t* dims, float alpha, int L,
                                     void* coef, const WaveletFromSubspace& sub, bool h16) {
   if (!wavelet_subspace_supported(dims, L, sub.K)) return fail(LLR_ERR_UNSUPPORTED, "fused subspace wavelet");
-  return h16 ? wavelet_run<StoreF16>(ctx, vols, nvols, dims, alpha, L, coef, &sub)
-             : wavelet_run<StoreF32>(ctx, vols, nvols, dims, alpha, L, coef, &sub);
+  return h16 ? wavelet_run_subspace<StoreF16>(ctx, vols, nvols, dims, alpha, L, coef, sub)
+             : wavelet_run_subspace<StoreF32>(ctx, vols, nvols, dims, alpha, L, coef, sub);
 }
 
 }  // namespace llr
