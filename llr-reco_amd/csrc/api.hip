@@ -657,6 +657,7 @@ struct llr_recon {
   DevBuf X, Y, Cb, S, coef, t1, t2, partials, stats, hist, smax, best, counters, werr;
   DevBuf Cprev;            // previous iteration's subspace coefficients (momentum)
   double momentum = 0.0;   // > 0: the LLR step acts on C(x_k) + momentum (C(x_k) - C(x_{k-1}))
+  bool s_real = false;     // every subspace entry has a zero imaginary part
   bool mom_first = false;  // the next LLR step has no previous coefficients yet
   // half storage (LLR_STORE_FP16): X, coef hold half-precision pairs; Yh is the
   // copy of the measurements made by the set-up; Xin holds an initial image
@@ -863,7 +864,7 @@ llr_status recon_iteration(llr_recon* r, size_t it) {
     StageTimer t(r, 2);
     const float alpha = (float)(p.wavelet_alpha * r->hs);
     if (r->fuse_wavelet) {
-      const WaveletFromSubspace sub{r->Cb.as<float2>(), r->S.as<float2>(), (int)p.srank, r->coefK.as<float2>()};
+      const WaveletFromSubspace sub{r->Cb.as<float2>(), r->S.as<float2>(), (int)p.srank, r->coefK.as<float2>(), r->s_real};
       st = wavelet_denoise_subspace(ctx, r->X.p, r->nq, p.vol, alpha, r->wL, r->coef.p, sub, r->s16);
     } else {
       st = wavelet_denoise_device(ctx, r->X.p, r->nq, p.vol, alpha, r->wL, r->coef.p, r->t1.p, r->t2.p, r->s16);
@@ -929,6 +930,11 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
   if (p.srank) {
     if ((st = upload(r->Cb, nullptr, r->N * p.srank * 8))) return bail(st);
     if ((st = upload(r->S, p.subspace, p.nq * p.srank * 8))) return bail(st);
+    // a real subspace (zero imaginary parts, as the log-linear DTI basis) takes
+    // the wavelet's two-FMA expansion
+    r->s_real = true;
+    for (size_t i = 0; i < p.nq * p.srank; ++i)
+      if (p.subspace[i].im != 0.f) r->s_real = false;
   }
   if (p.wavelet_alpha > 0) {
     r->wL = p.wavelet_levels < 0 ? wavelet_max_level(p.vol) : p.wavelet_levels;
@@ -1577,7 +1583,7 @@ static llr_status batch_post(llr_recon* r, const unsigned* iter, double* tmp) {
   if (p.wavelet_alpha > 0) {
     if (r->fuse_wavelet) {
       // the members' grids have partial edge blocks: every voxel is covered
-      const WaveletFromSubspace sub{r->Cb.as<float2>(), r->S.as<float2>(), (int)p.srank, r->coefK.as<float2>()};
+      const WaveletFromSubspace sub{r->Cb.as<float2>(), r->S.as<float2>(), (int)p.srank, r->coefK.as<float2>(), r->s_real};
       st = wavelet_denoise_subspace(ctx, r->X.p, r->nq, p.vol, (float)p.wavelet_alpha, r->wL, r->coef.p, sub, false);
     } else {
       st = wavelet_denoise_device(ctx, r->X.p, r->nq, p.vol, (float)p.wavelet_alpha, r->wL, r->coef.p, r->t1.p,

@@ -39,8 +39,8 @@ __host__ __device__ inline float cnorm(float2 a) { return a.x * a.x + a.y * a.y;
 // Prefetching kernels hold the raw words (T, from zero() or a plain load) and
 // widen them with cvt() only where the values are used: a widening next to
 // the load would make the wave wait for the load at once.
-// ldnt / rawnt: non-temporal loads (streamed data read once per pass, far
-// larger than the caches).
+// ldnt / rawnt / stnt: non-temporal loads and stores (streamed data read or
+// written once per pass, far larger than the caches).
 typedef unsigned int ch16;
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef float f32x2v __attribute__((ext_vector_type(2)));
@@ -57,6 +57,10 @@ struct StoreF32 {
     return make_float2(v.x, v.y);
   }
   __device__ static inline float2 ldnt(const float2* p) { return rawnt(p); }
+  __device__ static inline void stnt(float2* p, float2 v) {
+    const f32x2v w = {v.x, v.y};
+    __builtin_nontemporal_store(w, (f32x2v*)p);
+  }
 };
 
 struct StoreF16 {
@@ -76,6 +80,7 @@ struct StoreF16 {
   __device__ static inline ch16 zero() { return 0u; }
   __device__ static inline ch16 rawnt(const ch16* p) { return __builtin_nontemporal_load(p); }
   __device__ static inline float2 ldnt(const ch16* p) { return unpack(rawnt(p)); }
+  __device__ static inline void stnt(ch16* p, float2 v) { __builtin_nontemporal_store(pack(v), p); }
 };
 
 // ------------------------------------------------------------- host context

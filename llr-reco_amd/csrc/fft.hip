@@ -304,7 +304,7 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_pass(Pa
         float2 val = y[k2];
         if (INV) val.y = -val.y;
         val = RAMPS ? cmul(val, s_post[j + R1 * k2]) : cscale(val, a.scale);
-        SO::st(p + k2 * step, val);
+        SO::stnt(p + k2 * step, val);
         if constexpr (CHECK) c += stored_nonzero<SO>(val) ? 1 : 0;
       }
       if constexpr (CHECK) atomicAdd(&cnt[T + t], c);
@@ -443,7 +443,7 @@ __global__ void __launch_bounds__((Layout<R1, R2, MINT>::THREADS)) k_fft_z_proje
       for (int j1 = 0; j1 < R1; ++j1) {
         int idx = j + R2 * j1;
         float2 r = make_float2(x[j1].x, -x[j1].y);
-        ST::st(p + j1 * step, cscale(cmulc(r, s_pk[idx]), a.scale));
+        ST::stnt(p + j1 * step, cscale(cmulc(r, s_pk[idx]), a.scale));
       }
     }
     __syncthreads();
@@ -578,7 +578,7 @@ __global__ void __launch_bounds__(T * R2) k_fft_z_project_nw(ZProjArgs a) {
       for (int j1 = 0; j1 < R1; ++j1) {
         const int idx = j + R2 * j1;
         const float2 r = make_float2(x[j1].x, -x[j1].y);
-        ST::st(p + j1 * step, cscale(cmulc(r, s_pk[idx]), a.scale));
+        ST::stnt(p + j1 * step, cscale(cmulc(r, s_pk[idx]), a.scale));
       }
     }
     __syncthreads();

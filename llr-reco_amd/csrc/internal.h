@@ -94,6 +94,7 @@ struct WaveletFromSubspace {
   const float2* S;
   int K;
   float2* BK;
+  bool s_real;  // S has zero imaginary parts: two FMAs per expansion term
 };
 size_t wavelet_subspace_elems(const size_t* dims, int L, int K);
 bool wavelet_subspace_supported(const size_t* dims, int L, int K);

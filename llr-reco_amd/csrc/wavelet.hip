@@ -337,18 +337,23 @@ struct ZSynArgs {
 // The coefficient pair (lo, hi) of slot c for volume v: expanded from the K
 // planes and shrunk, except the LL band's approximation above the deepest
 // level, which is the reconstructed next level as it is.
-// sum_k w[k] conj(sc[k]) as four FMAs per term
-template <int KK>
+// sum_k w[k] conj(sc[k]): four FMAs per term, or two when every sc[k] is
+// real (RS: the imaginary parts are zero and not read)
+template <int KK, bool RS = false>
 __device__ __forceinline__ float2 expand(const float2 (&w)[KK], const float2 (&sc)[KK]) {
   float re = w[0].x * sc[0].x, im = w[0].y * sc[0].x;
-  re = fmaf(w[0].y, sc[0].y, re);
-  im = fmaf(-w[0].x, sc[0].y, im);
+  if (!RS) {
+    re = fmaf(w[0].y, sc[0].y, re);
+    im = fmaf(-w[0].x, sc[0].y, im);
+  }
 #pragma unroll
   for (int k = 1; k < KK; ++k) {
     re = fmaf(w[k].x, sc[k].x, re);
-    re = fmaf(w[k].y, sc[k].y, re);
     im = fmaf(w[k].y, sc[k].x, im);
-    im = fmaf(-w[k].x, sc[k].y, im);
+    if (!RS) {
+      re = fmaf(w[k].y, sc[k].y, re);
+      im = fmaf(-w[k].x, sc[k].y, im);
+    }
   }
   return make_float2(re, im);
 }
@@ -368,7 +373,7 @@ __device__ __forceinline__ float2 zsyn_tap(const Db2F& f, bool odd, float2 lo0, 
 // coefficients of slots m and m + 1 for every volume in turn (the slot shared
 // by two pairs is expanded by both). block = 32 columns x 8 pairs; grid =
 // (ceil(bx / 32), by, ceil(ceil(az / 2) / 8)).
-template <int KK, class ST>
+template <int KK, bool RS, class ST>
 __global__ void __launch_bounds__(256) k_wav_zsyn_k(ZSynArgs a) {
   using E = typename ST::T;
   const int pgs = ((a.az + 1) / 2 + 7) / 8;
@@ -402,8 +407,8 @@ __global__ void __launch_bounds__(256) k_wav_zsyn_k(ZSynArgs a) {
     float2 lo[2], hi[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      hi[j] = shrink(expand<KK>(wh[j], sc), a.alpha);
-      lo[j] = lo_ar ? ST::cvt(ar[v * a.arvp + j * arz]) : shrink(expand<KK>(wl[j], sc), a.alpha);
+      hi[j] = shrink(expand<KK, RS>(wh[j], sc), a.alpha);
+      lo[j] = lo_ar ? ST::cvt(ar[v * a.arvp + j * arz]) : shrink(expand<KK, RS>(wl[j], sc), a.alpha);
     }
     E* o = out + v * a.bvp;
     ST::st(o, zsyn_tap(a.f, false, lo[0], hi[0], lo[1], hi[1]));
@@ -419,7 +424,7 @@ __global__ void __launch_bounds__(256) k_wav_zsyn_k(ZSynArgs a) {
 // once (double-buffered: one barrier per group), or, for the last chunk when
 // that slot exists (2r + 2 <= ncz - 1), its own expansion of K-plane
 // coefficients kept in LDS. Rows of ZTL columns are 128 B of complex64.
-template <int ZTL, int NCH, int KK, int G, class ST>
+template <int ZTL, int NCH, int KK, int G, bool RS, class ST>
 __global__ void __launch_bounds__(ZTL * NCH, 4) k_wav_zsyn_c(ZSynArgs a, const float2* __restrict__ S) {
   using E = typename ST::T;
   __shared__ float2 xch[2][G][2][NCH][ZTL];  // (lo, hi) of each chunk's first slot, G volumes
@@ -481,10 +486,10 @@ __global__ void __launch_bounds__(ZTL * NCH, 4) k_wav_zsyn_c(ZSynArgs a, const f
       float2 sc[KK];
 #pragma unroll
       for (int k = 0; k < KK; ++k) sc[k] = S[v + (long long)min(k, kmax) * a.nq];
-      const float2 hi0 = shrink(expand<KK>(wh[0], sc), a.alpha);
-      hi1[g] = shrink(expand<KK>(wh[1], sc), a.alpha);
-      const float2 lo0 = ar ? ST::cvt(an[g][0]) : shrink(expand<KK>(wl[0], sc), a.alpha);
-      lo1[g] = ar ? ST::cvt(an[g][1]) : shrink(expand<KK>(wl[1], sc), a.alpha);
+      const float2 hi0 = shrink(expand<KK, RS>(wh[0], sc), a.alpha);
+      hi1[g] = shrink(expand<KK, RS>(wh[1], sc), a.alpha);
+      const float2 lo0 = ar ? ST::cvt(an[g][0]) : shrink(expand<KK, RS>(wl[0], sc), a.alpha);
+      lo1[g] = ar ? ST::cvt(an[g][1]) : shrink(expand<KK, RS>(wl[1], sc), a.alpha);
       xch[b][g][0][r][t] = lo0;
       xch[b][g][1][r][t] = hi0;
     }
@@ -504,13 +509,13 @@ __global__ void __launch_bounds__(ZTL * NCH, 4) k_wav_zsyn_c(ZSynArgs a, const f
         for (int k = 0; k < KK; ++k) sc[k] = S[v + (long long)min(k, kmax) * a.nq];
 #pragma unroll
         for (int k = 0; k < KK; ++k) w[k] = lastw[1][k][t];
-        hi2 = shrink(expand<KK>(w, sc), a.alpha);
+        hi2 = shrink(expand<KK, RS>(w, sc), a.alpha);
         if (ar) {
           lo2 = ST::cvt(an[g][2]);
         } else {
 #pragma unroll
           for (int k = 0; k < KK; ++k) w[k] = lastw[0][k][t];
-          lo2 = shrink(expand<KK>(w, sc), a.alpha);
+          lo2 = shrink(expand<KK, RS>(w, sc), a.alpha);
         }
       }
       E* o = out + v * a.bvp;
@@ -1049,11 +1054,13 @@ static llr_status wavelet_run_subspace(llr_ctx* ctx, void* vols_v, size_t nvols,
         constexpr int ZTL = 16, GV = 2;
         const unsigned tiles = (unsigned)(((q.bx + ZTL - 1) / ZTL) * q.by);
         const dim3 grid(tiles, vgroups(tiles));
-#define ZSC_LAUNCH(NN)                                                                                \
-  if (sub.K == 7)                                                                                     \
-    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, 7, GV, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);      \
-  else                                                                                                \
-    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, ZS_KM, GV, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);
+#define ZSC_LAUNCH(NN)                                                                                   \
+  if (sub.K == 7 && sub.s_real)                                                                          \
+    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, 7, GV, true, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);   \
+  else if (sub.K == 7)                                                                                   \
+    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, 7, GV, false, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);  \
+  else                                                                                                   \
+    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, ZS_KM, GV, false, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);
         if (nch <= 8) {
           ZSC_LAUNCH(8)
         } else if (nch <= 16) {
@@ -1065,10 +1072,12 @@ static llr_status wavelet_run_subspace(llr_ctx* ctx, void* vols_v, size_t nvols,
       } else {
         const unsigned tx = (unsigned)((q.bx + 31) / 32), tz = (unsigned)(q.by * (((az + 1) / 2 + 7) / 8));
         dim3 g(tx, vgroups((size_t)tx * tz), tz);
-        if (sub.K == 7)
-          hipLaunchKernelGGL((k_wav_zsyn_k<7, ST>), g, dim3(256), 0, s, a);
+        if (sub.K == 7 && sub.s_real)
+          hipLaunchKernelGGL((k_wav_zsyn_k<7, true, ST>), g, dim3(256), 0, s, a);
+        else if (sub.K == 7)
+          hipLaunchKernelGGL((k_wav_zsyn_k<7, false, ST>), g, dim3(256), 0, s, a);
         else
-          hipLaunchKernelGGL((k_wav_zsyn_k<ZS_KM, ST>), g, dim3(256), 0, s, a);
+          hipLaunchKernelGGL((k_wav_zsyn_k<ZS_KM, false, ST>), g, dim3(256), 0, s, a);
       }
     }
     LLR_HIP(hipGetLastError());

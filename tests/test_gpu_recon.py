@@ -142,3 +142,23 @@ def test_recon_five_level_fused_subspace_wavelet(llr):
     subspace coefficients (the 256x256x128 workload's path), 2 iterations."""
     ds = H.make(dict(H.SMALL_SUBSPACE, vol=(96, 96, 96), n_iter=2))
     check(ds, "96^3 x13 subspace7 wavelet 5 levels")
+
+
+def test_recon_complex_subspace_matches_real(llr):
+    """A complex subspace S Q (Q a seeded random 7x7 unitary) takes the general
+    four-FMA expansion of the subspace wavelet; the real S takes the two-FMA
+    one. X S Q and the block SVT commute with Q, and (C Q)(S Q)^H = C S^H, so
+    both give the same image: each against the CPU path, and the two against
+    each other."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, n_iter=3))
+    rng = np.random.default_rng(77)
+    K = ds["subspace"].shape[1]
+    q, _ = np.linalg.qr(rng.standard_normal((K, K)) + 1j * rng.standard_normal((K, K)))
+    dq = dict(ds, subspace=np.asfortranarray((ds["subspace"].astype(np.complex128) @ q).astype(np.complex64)))
+    assert np.abs(dq["subspace"].imag).max() > 0.1
+    e_c = check(dq, "32x32x16 x13 complex subspace")
+    img_r = run_gpu(ds)[0]
+    img_c = run_gpu(dq)[0]
+    e = H.nrmse(img_c, img_r)
+    print(f"[complex vs real subspace] NRMSE {e:.3e} (complex vs cpu32 {e_c:.3e})", flush=True)
+    assert e <= 1e-5
