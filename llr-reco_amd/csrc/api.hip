@@ -1541,7 +1541,7 @@ static llr_status batch_autotune(llr_recon_batch* B) {
     for (int a = 0; a < 3; ++a) {
       float bt = 1e30f;
       int bw = 4;
-      for (int w : {1, 2, 4, 8}) {
+      for (int w : {1, 2, 3, 4, 6, 8}) {
         ctx->fft_tpw[a] = w;
         float ms = 0.f;
         float sc = (float)(1.0 / std::sqrt((double)p.vol[a]));

@@ -599,9 +599,10 @@ template <int R1, int R2, bool TF, bool IV, bool RP, class ST, int MINT>
 hipError_t launch_one(PassArgs a, long long W, long long nvols, hipStream_t s) {
   using L = Layout<R1, R2, MINT>;
   a.tiles = (a.U + L::T - 1) / L::T;
-  // tiles per workgroup (tunable): 4 for half storage's 32-line tiles, 2 for
-  // complex64 (measured on the 256-point y pass at 256x256x128x67)
-  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : (MINT > 1 ? 4 : 2);
+  // tiles per workgroup (tunable): 3 for half storage's 32-line tiles, 2 for
+  // complex64 (256-point y passes at 256x256x128x67, swept over 1-8: half
+  // storage 5.72 TB/s at 3 against 5.29 at 2 and 4; complex64 5.41 at 2)
+  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : (MINT > 1 ? 3 : 2);
   a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
   dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
   hipLaunchKernelGGL((k_fft_pass<R1, R2, TF, IV, RP, ST, MINT>), grid, dim3(L::THREADS), 0, s, a);
@@ -652,8 +653,10 @@ template <int R1, int R2, class ST, int MINT>
 hipError_t launch_zproj_as(ZProjArgs a, long long W, long long nvols, hipStream_t s, size_t* nparts) {
   using L = Layout<R1, R2, MINT>;
   a.tiles = (a.U + L::T - 1) / L::T;
-  // size query: one tile per workgroup gives the most workgroups of any setting
-  const int want = a.partial == nullptr ? 1 : a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;
+  // size query: one tile per workgroup gives the most workgroups of any setting;
+  // 6 by default (complex64 at 256x256x128x67, swept over 1-8: 5.57 TB/s
+  // against 5.42 at 4)
+  const int want = a.partial == nullptr ? 1 : a.tiles_per_wg > 0 ? a.tiles_per_wg : 6;
   a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
   dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
   *nparts = (size_t)grid.x * grid.y * grid.z;
@@ -663,16 +666,16 @@ hipError_t launch_zproj_as(ZProjArgs a, long long W, long long nvols, hipStream_
   return hipGetLastError();
 }
 
-// narrow-layout launch: 32 lines per workgroup, 4 tiles per workgroup by
-// default (at 256x256x128x67 in half storage: 1.27 ms per launch against 1.66
-// ms for the wide layout; 64 lines per 512-thread workgroup takes 1.33-1.39 ms,
-// its 139 registers per lane allowing one workgroup per CU, or spilling when
-// held to 128)
+// narrow-layout launch: 32 lines per workgroup, 3 tiles per workgroup by
+// default (at 256x256x128x67 in half storage: 1.17 ms per launch, against 1.25
+// at 2 or 4 tiles and 1.66 ms for the wide layout; 64 lines per 512-thread
+// workgroup takes 1.33-1.39 ms, its 139 registers per lane allowing one
+// workgroup per CU, or spilling when held to 128)
 template <int R1, int R2, class ST>
 hipError_t launch_zproj_nw(ZProjArgs a, long long W, long long nvols, hipStream_t s, size_t* nparts) {
   constexpr int T = 32;
   a.tiles = (a.U + T - 1) / T;
-  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 4;
+  const int want = a.tiles_per_wg > 0 ? a.tiles_per_wg : 3;
   a.tiles_per_wg = a.tiles >= want ? want : a.tiles;
   dim3 grid((a.tiles + a.tiles_per_wg - 1) / a.tiles_per_wg, (unsigned)W, (unsigned)nvols);
   *nparts = (size_t)grid.x * grid.y * grid.z;
