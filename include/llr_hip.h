@@ -128,7 +128,9 @@ llr_status llr_subspace_constraint(llr_ctx* ctx, const llr_cf32* subspace, size_
 
 /* ------------------------------------------------ low_rank.rs / cublas.rs */
 /* cu_low_rank_approx_batch (low_rank.rs:10-69): replace each m x n matrix
- * (m > n) by its rank-`rank` truncation U_r diag(s_r) V_r^H. */
+ * (m > n, n <= 128) by its rank-`rank` truncation U_r diag(s_r) V_r^H.
+ * n <= 8: one fused kernel per matrix; 8 < n <= 128: Gram, workgroup Jacobi
+ * and apply kernels (one workgroup per matrix each). */
 llr_status llr_low_rank_approx_batch(llr_ctx* ctx, size_t m, size_t n, size_t rank, size_t batch_size,
                                      llr_cf32* matrix_data, size_t len);
 
@@ -316,7 +318,8 @@ typedef struct {
   double lambda_factor;  /* ... else lambda_factor * sigma_max of the zero-filled blocks at shift 0 */
   double wavelet_alpha;  /* wavelet_shrink_factor (lib.rs:60); 0 disables the wavelet step */
   int wavelet_levels;    /* < 0: floor(log2(min dim / 3)) */
-  size_t srank;          /* subspace rank; 0: no subspace (lib.rs:281-292) */
+  size_t srank;          /* subspace rank; 0: no subspace (lib.rs:281-292): the LLR step then works
+                            on all nq volumes, up to 8 in every form, up to 128 in the FP32 form */
   const llr_cf32* subspace; /* nq x srank, col-major (lib.rs:210-219); host pointer, copied */
   const size_t* shifts;  /* n_iter x 3 block shifts (lib.rs:266); host pointer, copied */
   double residual_stop;  /* > 0: stop once the relative residual falls to this value */

@@ -658,6 +658,7 @@ struct llr_recon {
   DevBuf Cprev;            // previous iteration's subspace coefficients (momentum)
   double momentum = 0.0;   // > 0: the LLR step acts on C(x_k) + momentum (C(x_k) - C(x_{k-1}))
   bool s_real = false;     // every subspace entry has a zero imaginary part
+  DevBuf wide;             // more than 8 columns: per-block Gram / W of the wide LLR kernels
   bool mom_first = false;  // the next LLR step has no previous coefficients yet
   // half storage (LLR_STORE_FP16): X, coef hold half-precision pairs; Yh is the
   // copy of the measurements made by the set-up; Xin holds an initial image
@@ -794,6 +795,9 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
                               (int)r->nq, (int)p.srank, partial);
   }
   StageTimer t(r, 0);
+  if (r->nc > 8)
+    return llr_block_run_wide(ctx, r->X.as<float2>(), p.vol, p.blk, shift, r->nc, 0, 0, nb, mode, (int)p.rank, lambda,
+                              smax, partial, r->wide.as<float2>());
   return llr_block_run(ctx, r->X.as<float2>(), r->X.as<float2>(), nullptr, 0, p.vol, p.blk, shift, r->nc, 0, 0, nb, mode,
                        (int)p.rank, lambda, smax, nullptr, partial);
 }
@@ -892,7 +896,10 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
     return fail(LLR_ERR_ARG, "rank must be in 1..=columns");  // lib.rs:205-207
   if (p.srank && p.rank > p.srank && p.llr_mode == LLR_MODE_RANK)
     return fail(LLR_ERR_ARG, "llr rank must be less than or equal to subspace rank");  // lib.rs:217
-  if (nc > 8) return fail(LLR_ERR_UNSUPPORTED, "the fused LLR kernel handles up to 8 matrix columns");
+  // more than 8 columns: the wide LLR kernels, for the FP32 form without a subspace (lib.rs:290-292)
+  if (nc > 8 && (p.srank || p.precision != LLR_PREC_FP32))
+    return fail(LLR_ERR_UNSUPPORTED, "more than 8 matrix columns need the FP32 form without a subspace");
+  if (nc > 128) return fail(LLR_ERR_UNSUPPORTED, "the LLR kernels handle up to 128 matrix columns");
   if (p.srank && !p.subspace) return fail(LLR_ERR_ARG, "subspace rank given without a subspace matrix");
   if (p.n_iter && !p.shifts) return fail(LLR_ERR_ARG, "shift sequence required");
   if (p.precision != LLR_PREC_FP32 && p.precision != LLR_PREC_MIXED) return fail(LLR_ERR_ARG, "unknown precision form");
@@ -926,6 +933,12 @@ llr_status llr_recon_create(llr_ctx* ctx, const llr_recon_params* params, llr_re
   };
   if ((st = upload(r->X, nullptr, r->D * eb))) return bail(st);
   if ((st = upload(r->Y, nullptr, r->D * 8))) return bail(st);
+  if (nc > 8) {  // Gram / W per block of the wide LLR kernels
+    long long nb = 1;
+    for (int a = 0; a < 3; ++a)
+      nb *= (long long)(p.partial_blocks ? (p.vol[a] + p.blk[a] - 1) / p.blk[a] : p.vol[a] / p.blk[a]);
+    if ((st = upload(r->wide, nullptr, llr_wide_work_elems(nc, nb) * 8))) return bail(st);
+  }
   if (r->s16 && (st = upload(r->Yh, nullptr, r->D * 4))) return bail(st);
   if (p.srank) {
     if ((st = upload(r->Cb, nullptr, r->N * p.srank * 8))) return bail(st);

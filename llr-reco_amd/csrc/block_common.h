@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include "common.h"
 
 namespace {
@@ -181,7 +183,8 @@ template <>
 __device__ __forceinline__ Rot<float> rot_params<float>(float app, float aqq, float2 apq) {
   Rot<float> r;
   const float s = fmaxf(fabsf(apq.x), fabsf(apq.y));
-  if (s == 0.f || s <= 1e-9f * (fabsf(app) + fabsf(aqq))) {
+  // below 16 FLT_MIN the hardware reciprocal of s overflows (denormal inputs)
+  if (s < 16.f * FLT_MIN || s <= 1e-9f * (fabsf(app) + fabsf(aqq))) {
     r.cs = 1.f;
     r.sn = 0.f;
     r.u = make_float2(1.f, 0.f);

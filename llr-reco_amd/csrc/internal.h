@@ -30,6 +30,13 @@ llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* d
                      const float2* pk, const float2* pi, double* partial, size_t cap, size_t* nparts,
                      bool h16 = false);
 
+// llr_wide.hip: LLR prox of blocks with more than 8 columns (up to 128).
+// work: llr_wide_work_elems(nc, nblocks) complex values, or null to allocate.
+size_t llr_wide_work_elems(int nc, long long nblocks);
+llr_status llr_block_run_wide(llr_ctx* ctx, float2* F, const size_t* vol, const size_t* blk, const size_t* shift,
+                              int nc, int layout, long long block0, long long nblocks, int mode, int rank,
+                              float lambda, unsigned* smax_bits, bool partial, float2* work);
+
 // llr.hip
 // One reconstruction's share of the persistent multi-reconstruction LLR launch.
 struct MemberSpec {

@@ -483,113 +483,6 @@ __global__ void __launch_bounds__(256) k_llr_block_multi(const MemberBlocks* __r
   }
 }
 
-// ----------------------------------------------- generic path (n > 8 columns)
-// Gram of contiguous m x n matrices: G[b][i + n*j] for all i, j.
-__global__ void __launch_bounds__(256) k_gram_generic(const float2* __restrict__ Am, float2* __restrict__ G, long long m,
-                                                      int n) {
-  long long b = blockIdx.x;
-  const float2* a = Am + b * m * n;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    int i = e % n, j = e / n;
-    if (i > j) continue;
-    float2 acc = make_float2(0.f, 0.f);
-    for (long long r = 0; r < m; ++r) acc = cadd(acc, cconjmul(a[(long long)i * m + r], a[(long long)j * m + r]));
-    if (i == j) acc.y = 0.f;
-    G[b * n * n + i + (long long)n * j] = acc;
-    G[b * n * n + j + (long long)n * i] = cconj(acc);
-  }
-}
-
-// Serial cyclic Jacobi per matrix (one thread per matrix); G is overwritten
-// with the eigenvalue matrix, V receives the eigenvectors, W the weights matrix.
-__global__ void k_eig_serial(float2* __restrict__ G, float2* __restrict__ V, float2* __restrict__ W, long long batch,
-                             int n, int mode, int rank, float lambda, unsigned* smax_bits) {
-  long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (b >= batch) return;
-  float2* A = G + b * n * n;
-  float2* Vb = V + b * n * n;
-  float2* Wb = W + b * n * n;
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) Vb[i + n * j] = make_float2(i == j ? 1.f : 0.f, 0.f);
-  const float eps = FLT_EPSILON;
-  for (int sweep = 0; sweep < 30; ++sweep) {
-    float off = 0.f, nrm = 0.f;
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i) {
-        float v = cnorm(A[i + n * j]);
-        nrm += v;
-        if (i != j) off += v;
-      }
-    if (off <= eps * eps * nrm || nrm == 0.f) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        float2 apq = A[p + n * q];
-        Rot R = rot_params(A[p + n * p].x, A[q + n * q].x, apq);
-        if (R.sn == 0.f) continue;
-        float2 cu = cconj(R.u);
-        for (int i = 0; i < n; ++i) {
-          float2 xp = A[i + n * p], xq = A[i + n * q];
-          A[i + n * p] = csub(cscale(xp, R.cs), cscale(cmul(cu, xq), R.sn));
-          A[i + n * q] = cadd(cscale(xp, R.sn), cscale(cmul(cu, xq), R.cs));
-        }
-        for (int j = 0; j < n; ++j) {
-          float2 xp = A[p + n * j], xq = A[q + n * j];
-          A[p + n * j] = csub(cscale(xp, R.cs), cscale(cmul(R.u, xq), R.sn));
-          A[q + n * j] = cadd(cscale(xp, R.sn), cscale(cmul(R.u, xq), R.cs));
-        }
-        A[p + n * q] = make_float2(0.f, 0.f);
-        A[q + n * p] = make_float2(0.f, 0.f);
-        A[p + n * p].y = 0.f;
-        A[q + n * q].y = 0.f;
-        for (int i = 0; i < n; ++i) {
-          float2 xp = Vb[i + n * p], xq = Vb[i + n * q];
-          Vb[i + n * p] = csub(cscale(xp, R.cs), cscale(cmul(cu, xq), R.sn));
-          Vb[i + n * q] = cadd(cscale(xp, R.sn), cscale(cmul(cu, xq), R.cs));
-        }
-      }
-  }
-  float lmax = 0.f;
-  for (int l = 0; l < n; ++l) lmax = fmaxf(lmax, A[l + n * l].x);
-  if (smax_bits) atomicMax(smax_bits, __float_as_uint(sqrtf(fmaxf(lmax, 0.f))));
-  // weights into the diagonal of A's imaginary slot is not needed; compute W
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < n; ++i) {
-      float2 acc = make_float2(0.f, 0.f);
-      for (int l = 0; l < n; ++l) {
-        float g;
-        if (mode == 0) {
-          int pos = 0;
-          for (int o = 0; o < n; ++o)
-            pos += (A[o + n * o].x > A[l + n * l].x || (A[o + n * o].x == A[l + n * l].x && o < l)) ? 1 : 0;
-          g = pos < rank ? 1.f : 0.f;
-        } else {
-          float s = sqrtf(fmaxf(A[l + n * l].x, 0.f));
-          g = (s > lambda && s > 0.f) ? (s - lambda) / s : 0.f;
-        }
-        acc = cadd(acc, cscale(cmulc(Vb[i + n * l], Vb[j + n * l]), g));
-      }
-      Wb[i + n * j] = acc;
-    }
-}
-
-// out = A W for contiguous m x n matrices (in place allowed: rows are read
-// into registers before being written, one row per thread).
-__global__ void __launch_bounds__(256) k_apply_generic(float2* __restrict__ Am, const float2* __restrict__ W, long long m,
-                                                       int n) {
-  long long b = blockIdx.y;
-  float2* a = Am + b * m * n;
-  const float2* w = W + b * n * n;
-  for (long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x; r < m; r += (long long)gridDim.x * blockDim.x) {
-    float2 row[128];
-    for (int k = 0; k < n; ++k) row[k] = a[(long long)k * m + r];
-    for (int j = 0; j < n; ++j) {
-      float2 acc = make_float2(0.f, 0.f);
-      for (int k = 0; k < n; ++k) acc = cadd(acc, cmul(row[k], w[k + n * j]));
-      a[(long long)j * m + r] = acc;
-    }
-  }
-}
-
 }  // namespace
 
 namespace llr {
@@ -632,7 +525,10 @@ llr_status lift_device(llr_ctx* ctx, const void* vols, void* mats, size_t nvols,
     const int xbw = shape == 1 ? (int)std::min<long long>(8, g.grid[0]) : (int)g.grid[0];
     const int nxc = (int)((g.grid[0] + xbw - 1) / xbw);
     const size_t slab_lds = slab_bytes(rc, xbw);
-    const dim3 sgrid((unsigned)(rows * nxc), (unsigned)std::min<size_t>((nvols + 3) / 4, 65535),
+    // volumes per workgroup: 1 by default (256x256x128x67 complex64, 8^3 blocks,
+    // swept over 1-67: lift 4.93 / un_lift 5.28 TB/s at 1 against 4.73 / 5.04 at 4)
+    const size_t vpw = ctx->lift_vpw > 0 ? (size_t)ctx->lift_vpw : 1;
+    const dim3 sgrid((unsigned)(rows * nxc), (unsigned)std::min<size_t>((nvols + vpw - 1) / vpw, 65535),
                      (unsigned)((nr + rc - 1) / rc));
 #define SLAB_LAUNCH(E, RC)                                                                                     \
   if (unlift)                                                                                                  \
@@ -816,25 +712,12 @@ llr_status llr_multi_launch(hipStream_t s, const MultiPlan& plan, const unsigned
   return LLR_OK;
 }
 
-// Generic low-rank/SVT of contiguous m x n matrices (any n <= 128).
+// Low-rank/SVT of contiguous m x n matrices with 8 < n <= 128 (the wide LLR kernels).
 llr_status lowrank_generic_device(llr_ctx* ctx, float2* mats, long long m, int n, long long batch, int mode, int rank,
                                   float lambda, unsigned* smax_bits) {
   if (n > 128) return fail(LLR_ERR_UNSUPPORTED, "more than 128 columns");
-  DevBuf G, V, W;
-  size_t nn = (size_t)n * n * (size_t)batch * sizeof(float2);
-  LLR_HIP(G.alloc(nn));
-  LLR_HIP(V.alloc(nn));
-  LLR_HIP(W.alloc(nn));
-  hipLaunchKernelGGL(k_gram_generic, dim3((unsigned)batch), dim3(256), 0, ctx->stream, mats, G.as<float2>(), m, n);
-  LLR_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_eig_serial, dim3(grid_for((size_t)batch, 64)), dim3(64), 0, ctx->stream, G.as<float2>(),
-                     V.as<float2>(), W.as<float2>(), batch, n, mode, rank, lambda, smax_bits);
-  LLR_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_apply_generic, dim3(grid_for((size_t)m, 256, 1024), (unsigned)batch), dim3(256), 0, ctx->stream,
-                     mats, W.as<float2>(), m, n);
-  LLR_HIP(hipGetLastError());
-  LLR_HIP(hipStreamSynchronize(ctx->stream));
-  return LLR_OK;
+  size_t vol[3] = {(size_t)m, 1, 1}, zero[3] = {0, 0, 0};
+  return llr_block_run_wide(ctx, mats, vol, vol, zero, n, 1, 0, batch, mode, rank, lambda, smax_bits, false, nullptr);
 }
 
 }  // namespace llr

@@ -209,7 +209,9 @@ def test_rank_one_ones(llr):
     assert np.abs(a - 1).max() < 1e-5
 
 
-@pytest.mark.parametrize("m,n,r", [(512, 7, 3), (512, 8, 5), (343, 5, 5), (4096, 7, 2), (300, 12, 4)])
+# n > 8 takes the wide kernels (Gram, workgroup Jacobi, apply); n > 96 keeps V in the work buffer
+@pytest.mark.parametrize("m,n,r", [(512, 7, 3), (512, 8, 5), (343, 5, 5), (4096, 7, 2), (300, 12, 4), (512, 67, 7),
+                                   (343, 16, 16), (300, 100, 10), (200, 128, 5)])
 def test_low_rank_vs_oracle(llr, m, n, r):
     from llr_hip import low_rank
 
@@ -224,7 +226,7 @@ def test_low_rank_vs_oracle(llr, m, n, r):
     assert rel(g, ref) < 2e-5
 
 
-@pytest.mark.parametrize("m,n", [(512, 7), (512, 1), (729, 8), (200, 16)])
+@pytest.mark.parametrize("m,n", [(512, 7), (512, 1), (729, 8), (200, 16), (512, 67), (1000, 97), (131, 9)])
 def test_svt_vs_oracle(llr, m, n):
     from llr_hip import low_rank
 

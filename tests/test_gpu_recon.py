@@ -162,3 +162,24 @@ def test_recon_complex_subspace_matches_real(llr):
     e = H.nrmse(img_c, img_r)
     print(f"[complex vs real subspace] NRMSE {e:.3e} (complex vs cpu32 {e_c:.3e})", flush=True)
     assert e <= 1e-5
+
+
+WIDE = dict(H.SMALL_PLAIN, n_b0=7, dirs="repulsion", n_dirs=60, bval=3000.0, n_iter=3)
+
+
+@pytest.mark.parametrize("mode,wav", [("svt", 0.0), ("svt", 0.005), ("rank", 0.0)])
+def test_recon_no_subspace_67_volumes(llr, mode, wav):
+    """The reference's default call without a subspace (lib.rs:290-292): the LLR
+    step on 512 x 67 blocks of all volumes, through the wide kernels (Gram,
+    workgroup Jacobi, apply), 32^3 x 67, against the CPU path."""
+    ds = H.make(dict(WIDE, wavelet_alpha=wav))
+    assert ds["nq"] == 67 and ds["subspace"] is None
+    kw = dict(llr_mode="rank", rank=10) if mode == "rank" else {}
+    check(ds, f"32^3 x67 no subspace {mode} wavelet {wav}", **kw)
+
+
+def test_recon_no_subspace_partial_blocks_wide(llr):
+    """Wide blocks with partial edge blocks (rows outside the volume are zero in
+    the Gram and left alone by the apply)."""
+    ds = H.make(dict(WIDE, vol=(48, 24, 32), blk=(10, 10, 10), n_iter=2))
+    check(ds, "48x24x32 x67 no subspace partial 10^3", partial_blocks=True)
