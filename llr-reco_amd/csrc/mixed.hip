@@ -13,10 +13,12 @@
 //  - k_llr_block_mixed: per shifted block, the Gram matrix of the real
 //    m x 2n matrix [Re F | Im F] on BF16 MFMA (both MFMA operands are the
 //    same registers), an FP32 Jacobi eigen-solve of it giving V0, then the
-//    FP32 refinement: B = F*V0 (FP32), H = B^H B (FP32) and one Jacobi sweep
-//    on H. If H's eigenvalues have a condition number above 1e6 the n x n
-//    eigen-solve is redone in FP64 (Gram of B in FP64 and a full FP64
-//    Jacobi). The block is replaced by B * J diag(g) J^H * V0^H = F W.
+//    FP32 refinement: H = V0^H G V0 = (F V0)^H (F V0) and one Jacobi sweep on
+//    H. G comes from the same MFMAs with every value split exactly into three
+//    BF16 pieces (Gram to ~2^-24, FP32 accumulation), so F V0 is never formed.
+//    If H's eigenvalues have a condition number above 1e6 the n x n
+//    eigen-solve is redone in FP64 (Gram of F in FP64 and a full FP64
+//    Jacobi). The block is replaced by F W, W = V0 J diag(g) J^H V0^H.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -225,12 +227,12 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   const long long slot = xcd_slot(A.per_xcd);
   if (slot >= A.nblocks) return;
   const long long b = A.block0 + slot;
-  float2* Fs = (float2*)smem_d;                    // nc * mp: F, then B = F V0
-  float* Rp = (float*)(Fs + fsz);                  // 4 x 256 floats: per-wave BF16 Gram partials
+  float2* Fs = (float2*)smem_d;                    // nc * mp: the block F
+  float* Rp = (float*)(Fs + fsz);                  // 4 x 256 floats: per-wave BF16 Gram partials (leading term)
+  float* Rq = Rp + 1024;                           // 4 x 256 floats: the split Gram's remaining terms
   double2* Gp64 = (double2*)Rp;                    // 256 double2 (FP64 fallback, aliases Rp)
-  float2* Gp = Fs + fsz + 512;                     // 256 FP32 Gram partials
-  float2* Vs = Gp + 256;                           // 64: V0
-  float2* Ms = Vs + 64;                            // 64: J g J^H V0^H
+  float2* Vs = (float2*)(Rq + 1024);               // 64: V0
+  float2* Ws = Vs + 64;                            // 64: W = V0 J diag(g) J^H V0^H
 
   const BlockRows br(A.g, b);
   const long long cs = col_stride(A);
@@ -240,15 +242,19 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   }
   __syncthreads();
 
-  // 1. BF16 MFMA Gram of the real m x 2nc matrix [Re F | Im F] (16 rows per MFMA)
+  // 1. Gram of the real m x 2nc matrix [Re F | Im F] on BF16 MFMA (16 rows per
+  //    MFMA). Every value is split exactly into three BF16 pieces x = h + u + l;
+  //    the leading term h h^T is the BF16 Gram that seeds the eigenvectors, and
+  //    with h u^T + u h^T + h l^T + l h^T + u u^T it is the Gram to ~2^-24 (the
+  //    dropped u l^T, l u^T, l l^T are smaller), accumulated in FP32.
   {
     const int cidx = lane & 15, kq = lane >> 4;
     const bool valid = cidx < 2 * nc, re = cidx < nc;
     const float2* colp = Fs + (long long)(valid ? (re ? cidx : cidx - nc) : 0) * mp;
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f}, acc3 = {0.f, 0.f, 0.f, 0.f};
     const int tiles = (m + 15) / 16;
     for (int t = wave; t < tiles; t += 4) {
-      bf16x4 v;
+      bf16x4 vh, vu, vl;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         int row = t * 16 + 4 * kq + j;
@@ -257,84 +263,62 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
           float2 f = colp[row];
           x = re ? f.x : f.y;
         }
-        v[j] = bf16_bits(x);
+        const float h = bf16_round(x), r1 = x - h, u = bf16_round(r1);
+        vh[j] = bf16_bits(h);
+        vu[j] = bf16_bits(u);
+        vl[j] = bf16_bits(r1 - u);
       }
-      acc = mfma16(v, v, acc);
+      acc = mfma16(vh, vh, acc);
+      acc3 = mfma16(vh, vu, acc3);
+      acc3 = mfma16(vu, vh, acc3);
+      acc3 = mfma16(vh, vl, acc3);
+      acc3 = mfma16(vl, vh, acc3);
+      acc3 = mfma16(vu, vu, acc3);
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) Rp[wave * 256 + (4 * kq + j) * 16 + cidx] = acc[j];
+    for (int j = 0; j < 4; ++j) {
+      Rp[wave * 256 + (4 * kq + j) * 16 + cidx] = acc[j];
+      Rq[wave * 256 + (4 * kq + j) * 16 + cidx] = acc3[j];
+    }
   }
   __syncthreads();
 
-  // 2. wave 0: complex Gram G = F^H F from the real one, FP32 Jacobi -> V0
+  // 2. wave 0: the complex Gram G = F^H F from the real one: FP32 Jacobi of the
+  //    BF16 Gram -> V0; H = V0^H G V0 from the split Gram; one FP32 Jacobi
+  //    sweep on H -> J, its eigenvalues; condition test
   const int r = lane & 7, c = lane >> 3;
+  float2 hv = make_float2(0.f, 0.f), jv = make_float2(0.f, 0.f), v0 = make_float2(0.f, 0.f);
   if (wave == 0) {
-    float2 a = make_float2(0.f, 0.f);
+    float2 a = make_float2(0.f, 0.f), g3 = make_float2(0.f, 0.f);
     if (r < nc && c < nc) {
-      float rr = 0.f, ii = 0.f, ri = 0.f, ir = 0.f;
+      float rr = 0.f, ii = 0.f, ri = 0.f, ir = 0.f, rr3 = 0.f, ii3 = 0.f, ri3 = 0.f, ir3 = 0.f;
       for (int w = 0; w < 4; ++w) {
         const float* R = Rp + w * 256;
+        const float* Q = Rq + w * 256;
         rr += R[r * 16 + c];
         ii += R[(nc + r) * 16 + nc + c];
         ri += R[r * 16 + nc + c];
         ir += R[(nc + r) * 16 + c];
+        rr3 += Q[r * 16 + c];
+        ii3 += Q[(nc + r) * 16 + nc + c];
+        ri3 += Q[r * 16 + nc + c];
+        ir3 += Q[(nc + r) * 16 + c];
       }
       a = make_float2(rr + ii, r == c ? 0.f : ri - ir);
+      g3 = make_float2((rr + rr3) + (ii + ii3), r == c ? 0.f : (ri + ri3) - (ir + ir3));
     }
-    float2 v0;
     wave_jacobi8<float>(a, v0, lane, 16, 2.5e-7f);
     Vs[lane] = v0;  // V0[r][c]
-  }
-  __syncthreads();
-
-  // 3. B = F V0 in place (each thread owns whole rows)
-  for (int rr = tid; rr < m; rr += 256) {
-    float2 f[8];
+    // T = G V0, then H = V0^H T, made exactly Hermitian
+    float2 t = make_float2(0.f, 0.f);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) f[k] = k < nc ? Fs[k * mp + rr] : make_float2(0.f, 0.f);
-    for (int k = 0; k < nc; ++k) {
-      float2 acc = make_float2(0.f, 0.f);
+    for (int j = 0; j < 8; ++j) t = cadd(t, cmul(shfl2(g3, r + 8 * j), shfl2(v0, j + 8 * c)));
+    float2 h = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int cc = 0; cc < 8; ++cc)
-        if (cc < nc) acc = cadd(acc, cmul(f[cc], Vs[cc + 8 * k]));
-      Fs[k * mp + rr] = acc;
-    }
-  }
-  __syncthreads();
-
-  // 4. FP32 Gram of B: entry e = (i <= j) for thread t % NE, rows interleaved by t / NE
-  const int NE = nc * (nc + 1) / 2;
-  const int NH = 256 / NE;
-  int gi = 0, gj = 0;
-  {
-    int rem = tid % NE;
-    for (int jj = 0; jj < nc; ++jj) {
-      if (rem <= jj) {
-        gi = rem;
-        gj = jj;
-        break;
-      }
-      rem -= jj + 1;
-    }
-    const int h = tid / NE;
-    float2 acc = make_float2(0.f, 0.f);
-    if (h < NH)
-      for (int row = h; row < m; row += NH) acc = cadd(acc, cconjmul(Fs[gi * mp + row], Fs[gj * mp + row]));
-    Gp[tid] = acc;
-  }
-  __syncthreads();
-
-  // 5. wave 0: one FP32 Jacobi sweep on H = B^H B; condition test
-  float2 hv = make_float2(0.f, 0.f), jv = make_float2(0.f, 0.f);
-  if (wave == 0) {
-    if (r < nc && c < nc) {
-      int i = min(r, c), j = max(r, c);
-      int e = j * (j + 1) / 2 + i;
-      float2 s = make_float2(0.f, 0.f);
-      for (int h = 0; h < NH; ++h) s = cadd(s, Gp[h * NE + e]);
-      hv = (r <= c) ? s : cconj(s);
-      if (r == c) hv.y = 0.f;
-    }
+    for (int i = 0; i < 8; ++i) h = cadd(h, cconjmul(shfl2(v0, i + 8 * r), shfl2(t, i + 8 * c)));
+    const float2 ht = shfl2(h, c + 8 * r);
+    hv = (r < nc && c < nc) ? make_float2(0.5f * (h.x + ht.x), r == c ? 0.f : 0.5f * (h.y - ht.y))
+                            : make_float2(0.f, 0.f);
     wave_jacobi8<float>(hv, jv, lane, 1, 0.f);
     float lmax = 0.f, lmin = FLT_MAX;
 #pragma unroll
@@ -349,13 +333,27 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   }
   __syncthreads();
 
-  // 6. FP64 fallback for ill-conditioned blocks: Gram of B and Jacobi in FP64
-  if (s_fp64) {
+  // 3. FP64 fallback for ill-conditioned blocks: Gram of F and Jacobi in FP64;
+  //    its eigenvectors replace V0 J
+  const bool fb = s_fp64 != 0;
+  if (fb) {
+    const int NE = nc * (nc + 1) / 2;
+    const int NH = 256 / NE;
+    int gi = 0, gj = 0;
     {
-      const int h = tid / NE;
+      int rem = tid % NE;
+      for (int jj = 0; jj < nc; ++jj) {
+        if (rem <= jj) {
+          gi = rem;
+          gj = jj;
+          break;
+        }
+        rem -= jj + 1;
+      }
+      const int hh = tid / NE;
       double2 acc = make_double2(0.0, 0.0);
-      if (h < NH)
-        for (int row = h; row < m; row += NH) {
+      if (hh < NH)
+        for (int row = hh; row < m; row += NH) {
           float2 x = Fs[gi * mp + row], y = Fs[gj * mp + row];
           double2 xd = make_double2(x.x, x.y), yd = make_double2(y.x, y.y);
           acc = cadd(acc, cconjmul(xd, yd));
@@ -368,9 +366,9 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
       if (r < nc && c < nc) {
         int i = min(r, c), j = max(r, c);
         int e = j * (j + 1) / 2 + i;
-        double2 s = make_double2(0.0, 0.0);
-        for (int h = 0; h < NH; ++h) s = cadd(s, Gp64[h * NE + e]);
-        a64 = (r <= c) ? s : make_double2(s.x, -s.y);
+        double2 sd = make_double2(0.0, 0.0);
+        for (int hh = 0; hh < NH; ++hh) sd = cadd(sd, Gp64[hh * NE + e]);
+        a64 = (r <= c) ? sd : make_double2(sd.x, -sd.y);
         if (r == c) a64.y = 0.0;
       }
       wave_jacobi8<double>(a64, v64, lane, 30, 1e-15);
@@ -380,7 +378,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
     }
   }
 
-  // 7. wave 0: weights, M = J diag(g) J^H V0^H
+  // 4. wave 0: weights, W = V0 J diag(g) J^H V0^H (J diag(g) J^H in the fallback)
   if (wave == 0) {
     float lam[8];
 #pragma unroll
@@ -392,7 +390,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
     if (lane == 0 && A.smax_bits) atomicMax(A.smax_bits, __float_as_uint(sqrtf(fmaxf(lmax, 0.f))));
     if (A.svals && lane < 8) A.svals[slot * 8 + lane] = sqrtf(fmaxf(lam[lane], 0.f));
     if (A.mode != 2) {
-      float2 w = make_float2(0.f, 0.f);
+      float2 w = make_float2(0.f, 0.f);  // J diag(g) J^H
 #pragma unroll
       for (int l = 0; l < 8; ++l) {
         float g = shrink_weight(lam, l, nc, A.mode, A.rank, A.lambda);
@@ -400,16 +398,22 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
         w.x += g * t.x;
         w.y += g * t.y;
       }
-      float2 mm = make_float2(0.f, 0.f);
+      if (!fb) {
+        float2 mm = make_float2(0.f, 0.f);  // (J g J^H) V0^H
 #pragma unroll
-      for (int l = 0; l < 8; ++l) mm = cadd(mm, cmulc(shfl2(w, r + 8 * l), Vs[c + 8 * l]));
-      Ms[lane] = mm;  // M[r][c]
+        for (int l = 0; l < 8; ++l) mm = cadd(mm, cmulc(shfl2(w, r + 8 * l), Vs[c + 8 * l]));
+        float2 wv = make_float2(0.f, 0.f);  // V0 (J g J^H V0^H)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) wv = cadd(wv, cmul(Vs[r + 8 * k], shfl2(mm, k + 8 * c)));
+        w = wv;
+      }
+      Ws[lane] = w;  // W[r][c]
     }
   }
   if (A.mode == 2) return;
   __syncthreads();
 
-  // 8. out = B M (rows outside a partial edge block are not written)
+  // 5. out = F W (rows outside a partial edge block are not written)
   for (int rr = tid; rr < m; rr += 256) {
     const long long base = row_base(A, br, b, rr);
     if (base < 0) continue;
@@ -420,7 +424,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
       float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if (k < nc) acc = cadd(acc, cmul(f[k], Ms[k + 8 * cc]));
+        if (k < nc) acc = cadd(acc, cmul(f[k], Ws[k + 8 * cc]));
       A.out[base + cc * cs] = acc;
     }
   }
@@ -606,7 +610,7 @@ llr_status backproject_bf16_device(llr_ctx* ctx, const float2* C, float2* X, con
 
 size_t llr_block_mixed_smem(long long m, int nc) {
   size_t fsz = ((size_t)nc * (size_t)(m + 1) + 1) & ~(size_t)1;
-  return (fsz + 512 + 256 + 64 + 64) * sizeof(float2);
+  return (fsz + 512 + 512 + 64 + 64) * sizeof(float2);
 }
 
 llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const size_t* blk, const size_t* shift,
