@@ -1480,7 +1480,7 @@ static llr_status time_min(hipStream_t s, int reps, F fn, float* ms_out) {
 // Autotuner, per member, after its set-up: block size from {6,8,10}^3 (those
 // that fit the volume with more rows than columns), by the time of one LLR
 // step per voxel; LDS staging of the block kernel; FFT tiles per workgroup per
-// axis from {1, 2, 4, 8}. The LLR step is timed on the coefficient planes
+// axis from {1, 2, 3, 4, 6, 8}. The LLR step is timed on the coefficient planes
 // (subspace) or in singular-value-only mode (no subspace), so the iterate is
 // not modified; FFT passes are timed out of place into a scratch buffer.
 static llr_status batch_autotune(llr_recon_batch* B) {
