@@ -760,12 +760,19 @@ llr_status recon_llr(llr_recon* r, const size_t* shift, int mode, float lambda, 
       st = r->s16 ? project_s16_device(ctx, r->X.p, F, r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank, 1.f)
                   : project_bf16_device(ctx, r->X.as<float2>(), F, r->S.as<float2>(), r->N, (int)r->nq, (int)p.srank);
       if (st) return st;
-      if ((st = apply_momentum(r, mode, backproject))) return st;
+    }
+    // momentum (learned warm start) is applied by the block kernel as it loads C
+    float2* prev = nullptr;
+    const bool mom_first = r->mom_first;
+    if (p.srank && r->momentum > 0 && mode != 2 && !backproject) {
+      if (!r->Cprev.p) LLR_HIP(r->Cprev.alloc(r->N * p.srank * 8));
+      prev = r->Cprev.as<float2>();
+      r->mom_first = false;
     }
     {
       StageTimer t(r, 0);
       st = llr_block_run_mixed(ctx, F, p.vol, p.blk, shift, r->nc, 0, 0, nb, mode, (int)p.rank, lambda, smax, nullptr,
-                               mode != 2 ? fp64n : nullptr, partial);
+                               mode != 2 ? fp64n : nullptr, partial, prev, (float)r->momentum, mom_first);
     }
     if (st || mode == 2 || !p.srank || !backproject) return st;
     StageTimer t(r, 1);

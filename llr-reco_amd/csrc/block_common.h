@@ -310,6 +310,9 @@ struct BlockArgs {
   float* svals;         // optional [nblocks][8] singular values (unsorted)
   int stage_lds;        // 1: F block staged in LDS
   unsigned* fp64_count; // mixed form: blocks whose eigen-solve fell back to FP64 (optional)
+  float2* prev;         // mixed form: momentum on load, F <- (1 + mom) F - mom prev, prev <- F (null: none)
+  float mom;
+  int mom_first;        // 1: only store prev (the first step after a set-up)
 };
 
 // element address of row i, column col of block b (rows `br` of b when

@@ -87,9 +87,13 @@ llr_status project_s16_device(llr_ctx* ctx, const void* X, float2* Cp, const flo
                               float out_scale);
 llr_status backproject_bf16_device(llr_ctx* ctx, const float2* C, float2* X, const float2* S, const size_t* vol,
                                    const size_t* blk, const size_t* shift, int nq, int K, bool partial);
+// prev (optional, layout 0 with a block grid covering every voxel): the
+// momentum extrapolation F <- (1 + mom) F - mom prev is applied as each block is
+// loaded and prev <- F stored (only stored when mom_first).
 llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const size_t* blk, const size_t* shift,
                                int nc, int layout, long long block0, long long nblocks, int mode, int rank,
-                               float lambda, unsigned* smax_bits, float* svals, unsigned* fp64_count, bool partial);
+                               float lambda, unsigned* smax_bits, float* svals, unsigned* fp64_count, bool partial,
+                               float2* prev = nullptr, float mom = 0.f, bool mom_first = false);
 size_t llr_block_mixed_smem(long long m, int nc);
 llr_status dti_scalars_device(llr_ctx* ctx, const float2* vols, size_t n_vox, size_t nq, const float* bvals,
                               const float* bvecs, float b0_tol, float* out);

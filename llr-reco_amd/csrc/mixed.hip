@@ -238,7 +238,19 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   const long long cs = col_stride(A);
   for (int i = tid; i < m; i += 256) {
     const long long base = row_base(A, br, b, i);
-    for (int col = 0; col < nc; ++col) Fs[col * mp + i] = base < 0 ? make_float2(0.f, 0.f) : A.F[base + col * cs];
+    for (int col = 0; col < nc; ++col) {
+      float2 v = make_float2(0.f, 0.f);
+      if (base >= 0) {
+        const long long e = base + col * cs;
+        v = A.F[e];
+        if (A.prev) {  // momentum: every coefficient belongs to exactly one block
+          const float2 q = A.mom_first ? v : A.prev[e];
+          A.prev[e] = v;
+          v = make_float2((1.f + A.mom) * v.x - A.mom * q.x, (1.f + A.mom) * v.y - A.mom * q.y);
+        }
+      }
+      Fs[col * mp + i] = v;
+    }
   }
   __syncthreads();
 
@@ -615,7 +627,8 @@ size_t llr_block_mixed_smem(long long m, int nc) {
 
 llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const size_t* blk, const size_t* shift,
                                int nc, int layout, long long block0, long long nblocks, int mode, int rank,
-                               float lambda, unsigned* smax_bits, float* svals, unsigned* fp64_count, bool partial) {
+                               float lambda, unsigned* smax_bits, float* svals, unsigned* fp64_count, bool partial,
+                               float2* prev, float mom, bool mom_first) {
   GeomArgs g = make_geom(vol, blk, shift, partial);
   if (nc < 1 || nc > 8) return fail(LLR_ERR_UNSUPPORTED, "mixed LLR kernel handles 1..8 columns");
   if (!geom_in_range(g)) return fail(LLR_ERR_UNSUPPORTED, "block geometry beyond the LLR kernel's index range");
@@ -640,6 +653,10 @@ llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const
   A.svals = svals;
   A.stage_lds = 1;
   A.fp64_count = fp64_count;
+  if (prev && layout != 0) return fail(LLR_ERR_ARG, "momentum on load needs the voxel-plane layout");
+  A.prev = prev;
+  A.mom = mom;
+  A.mom_first = mom_first ? 1 : 0;
   hipLaunchKernelGGL(k_llr_block_mixed, dim3((unsigned)(8 * A.per_xcd)), dim3(256), smem, ctx->stream, A);
   LLR_HIP(hipGetLastError());
   return LLR_OK;

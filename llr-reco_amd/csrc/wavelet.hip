@@ -828,14 +828,23 @@ static void launch_xy_syn(hipStream_t s, const void* B, const LevelBox& q, const
   a3.ax = (int)od[0];
   a3.ay = (int)od[1];
   a3.az = (int)od[2];
-  constexpr int TXS = SY_TX;  // (128 for half storage measured slower)
+  // 64-sample x tiles (128 for half storage measured slower), or 32 for rows
+  // of at most 96 samples where they pad the row less (the deep levels' odd
+  // lengths: 65 -> 96, not 128); 32-wide tiles of a 129-sample row take
+  // 357 us against 231 us for 64-wide at 256x256x128x67's level 1
+  const size_t ax = od[0];
+  const bool narrow = ax <= 96 && (ax + 31) / 32 * 32 < (ax + SY_TX - 1) / SY_TX * SY_TX;
+  const int TXS = narrow ? 32 : SY_TX;
   a3.tiles_x = (int)((od[0] + TXS - 1) / TXS);
   a3.tpw = TPW;
   a3.groups_x = (a3.tiles_x + a3.tpw - 1) / a3.tpw;
   a3.f = f;
   const int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
   dim3 g3((unsigned)(a3.groups_x * tiles_y), (unsigned)(od[2] * nv));
-  hipLaunchKernelGGL((k_wav_xy_syn<ST, TXS>), g3, dim3(256), 0, s, a3);
+  if (narrow)
+    hipLaunchKernelGGL((k_wav_xy_syn<ST, 32>), g3, dim3(256), 0, s, a3);
+  else
+    hipLaunchKernelGGL((k_wav_xy_syn<ST, SY_TX>), g3, dim3(256), 0, s, a3);
 }
 
 template <class ST>
@@ -1061,10 +1070,21 @@ static llr_status wavelet_run_subspace(llr_ctx* ctx, void* vols_v, size_t nvols,
     hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, 7, GV, false, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);  \
   else                                                                                                   \
     hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, ZS_KM, GV, false, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);
-        if (nch <= 8) {
+        // chunks per column rounded up to a workgroup shape: few idle chunks at
+        // the deep levels' odd z lengths (level 1 of a 128-long z axis has 65
+        // samples, 17 chunks)
+        if (nch <= 4) {
+          ZSC_LAUNCH(4)
+        } else if (nch <= 8) {
           ZSC_LAUNCH(8)
+        } else if (nch <= 12) {
+          ZSC_LAUNCH(12)
         } else if (nch <= 16) {
           ZSC_LAUNCH(16)
+        } else if (nch <= 20) {
+          ZSC_LAUNCH(20)
+        } else if (nch <= 24) {
+          ZSC_LAUNCH(24)
         } else {
           ZSC_LAUNCH(32)
         }
