@@ -96,6 +96,8 @@ struct llr_ctx {
   int fft_tpw[3] = {0, 0, 0};  // FFT tiles per workgroup, per axis (z: the fused projection pass)
   int llr_stage = -1;          // LLR block kernel: 1 stage the block in LDS, 0 read it from HBM twice
   int lift_vpw = 0;           // gather/scatter slab kernels: volumes per workgroup (0: built-in default)
+  int proj_tpw = 0;           // half-storage projection: 128-voxel tiles per workgroup (0: built-in default)
+  int wav_tpw = 0;            // wavelet xy synthesis: x tiles per workgroup (0: built-in default)
   // per-kernel timing inside multi-kernel stages: while a reconstruction's
   // stage timing is on these point at its stage tables (null otherwise)
   double* kt_ms = nullptr;

@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
 // 16-voxel MFMA row blocks per wavefront); plane runs of 512 bytes are staged
 // in LDS as rows [voxel][plane] of 32-bit half pairs.
 constexpr int P16_TV = 128;
-constexpr int P16_TPW = 4;  // tiles per workgroup (large volumes)
+constexpr int P16_TPW = 8;  // tiles per workgroup (large volumes; at 256x256x128x67, swept over 1-16: 0.638 ms at 8, 0.686 at 4)
 
 template <int K, int MS>
 __global__ void __launch_bounds__(256) k_project_s16(const ch16* __restrict__ X, float2* __restrict__ Cp,
@@ -548,6 +548,7 @@ llr_status project_s16_device(llr_ctx* ctx, const void* X, float2* Cp, const flo
   const size_t tiles = (N + P16_TV - 1) / P16_TV;
   int tpw = P16_TPW;
   while (tpw > 1 && (tiles + tpw - 1) / tpw < 4096) tpw /= 2;
+  if (ctx->proj_tpw > 0) tpw = ctx->proj_tpw;
   const unsigned grid = (unsigned)((tiles + tpw - 1) / tpw);
   const bool small = 2 * nq <= 16 * 9;
   const ch16* Xs = (const ch16*)X;

@@ -806,7 +806,7 @@ static void launch_xy_an(hipStream_t s, const void* in, long long ip1, const std
 template <class ST>
 static void launch_xy_syn(hipStream_t s, const void* B, const LevelBox& q, const void* Ap, long long apvp, int pa,
                           void* out, long long op1, const std::array<size_t, 3>& od, size_t nv, const Db2F& f,
-                          bool zmid = false) {
+                          bool zmid = false, int tpw = 0) {
   XYSynArgs a3{};
   a3.B = B;
   a3.bx = q.bx;
@@ -836,7 +836,7 @@ static void launch_xy_syn(hipStream_t s, const void* B, const LevelBox& q, const
   const bool narrow = ax <= 96 && (ax + 31) / 32 * 32 < (ax + SY_TX - 1) / SY_TX * SY_TX;
   const int TXS = narrow ? 32 : SY_TX;
   a3.tiles_x = (int)((od[0] + TXS - 1) / TXS);
-  a3.tpw = TPW;
+  a3.tpw = tpw > 0 ? tpw : TPW;
   a3.groups_x = (a3.tiles_x + a3.tpw - 1) / a3.tpw;
   a3.f = f;
   const int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
@@ -1104,7 +1104,7 @@ static llr_status wavelet_run_subspace(llr_ctx* ctx, void* vols_v, size_t nvols,
     {
       KernelTimer kt(ctx, l == 0 ? kStageWavOut : kStageWavDeep);
       launch_xy_syn<ST>(s, Bp[l], q, nullptr, 0, 0, l == 0 ? vols_v : (void*)Ap[l],
-                        l == 0 ? (long long)in_dims[0][0] : (long long)q.pa, in_dims[l], nvols, f, true);
+                        l == 0 ? (long long)in_dims[0][0] : (long long)q.pa, in_dims[l], nvols, f, true, ctx->wav_tpw);
     }
     LLR_HIP(hipGetLastError());
   }
