@@ -70,7 +70,8 @@ def stage_bytes(ds, eb=8):
     write every volume's xy-bands) and the xy synthesis (read the bands, write
     X); without one, the xy analysis (read X, write the bands), the z pass (read
     and write the bands, write level 1's input) and the xy synthesis (read the
-    bands and level 1, write X)."""
+    bands and level 1, write X). The adjoint's y pass fused with the projection
+    reads the z pass's output and writes C (the image is not stored)."""
     N = int(ds["vol"][0] * ds["vol"][1] * ds["vol"][2])
     Q = ds["nq"]
     sub = bool(ds["cfg"]["srank"])
@@ -89,6 +90,7 @@ def stage_bytes(ds, eb=8):
         "fft_y": 2 * D,
         "fft_y_adjoint": 2 * D,
         "fft_z_project": 3 * D,                          # read X, read Y, write X
+        "fft_y_adjoint_project": D + CK,                 # read the z pass's output, write C
     }
 
 
@@ -102,8 +104,9 @@ KERNEL_STAGES = {
     "fft_y": "k_fft_pass (y, forward)",
     "fft_y_adjoint": "k_fft_pass (y, adjoint)",
     "fft_z_project": "k_fft_z_project",
+    "fft_y_adjoint_project": "k_fft_y_adjoint_project",
     "wavelet_level0_in": "k_wav_xy_an + k_wav_zan_k on the K planes / k_wav_xy_an (level 0)",
-    "wavelet_level0_mid": "k_wav_zsyn_k4 / k_wav_z (level 0)",
+    "wavelet_level0_mid": "k_wav_zsyn_c / k_wav_z (level 0)",
     "wavelet_level0_out": "k_wav_xy_syn (level 0)",
 }
 

@@ -337,6 +337,13 @@ typedef struct {
                             (signal_model.rs:34-79: a sub-voxel image shift per volume, e.g. from
                             llr_read_affine_translation); host pointer, copied; NULL: zero, as
                             lib.rs:148 fixes them */
+  int fused_projection;  /* where the subspace wavelet is fused into level 0 (a subspace, every
+                            voxel in a block, the wavelet on) and the iteration is hybrid, the
+                            adjoint's inverse y pass can hand the next LLR step the subspace
+                            coefficients C = X S directly, the image X being formed once when
+                            the run ends: 0 with complex64 storage (where it is faster), 1 with
+                            either storage, -1 never (every iteration stores X and projects it,
+                            as the reference) (extension) */
 } llr_recon_params;
 
 /* Create a reconstruction (allocates the iterate, the measurements and all
@@ -390,8 +397,9 @@ llr_status llr_recon_mixed_stats(llr_recon* r, unsigned long long* fp64_blocks, 
  * network, unpacking), 9 FFT y pass (adjoint), 10 FFT x pass (adjoint), and
  * the wavelet's kernels, also counted in stage 2: 11 the level-0 kernel that
  * reads the iterate or its subspace coefficients, 12 the level-0 kernels in
- * between, 13 the level-0 kernel that writes the iterate, 14 levels >= 1. */
-#define LLR_NUM_STAGES 15
+ * between, 13 the level-0 kernel that writes the iterate, 14 levels >= 1, and
+ * 15 the adjoint's y pass fused with the subspace projection. */
+#define LLR_NUM_STAGES 16
 llr_status llr_recon_enable_timing(llr_recon* r, int enable);
 llr_status llr_recon_stage_times(llr_recon* r, double* total_ms, size_t* launches);
 

@@ -595,6 +595,193 @@ __global__ void __launch_bounds__(T * R2) k_fft_z_project_nw(ZProjArgs a) {
   }
 }
 
+// Adjoint y pass fused with the subspace projection C = X S (k_project's
+// C[k][n] = sum_q X[q][n] S[q][k]): the adjoint image is not stored, only its
+// K coefficient planes. A workgroup owns YP_T x-neighbour y-lines of one z
+// slice and walks the volumes VG at a time: each group of T*P threads runs
+// one volume's inverse line FFTs exactly as k_fft_pass does (same stage order,
+// twiddles and ramps: the values are those the unfused pass would store),
+// leaving the outputs in LDS, and then every thread adds the VG volumes'
+// contributions to the K coefficients of its VPT voxels, held in registers
+// until the last volume. The input stays where it is (the z pass's output).
+// VG = 2: 512-thread workgroups, 8 voxels (112 accumulators) per thread, one
+// workgroup per CU (80 KB of LDS at ny = 256). At 256x256x128x67 this takes
+// 1.52 / 1.70 ms per launch for half / complex64 storage; VG = 4 (1024
+// threads, 56 accumulators, spilling at the 128 registers such a workgroup
+// allows) took 1.75 / 2.83 ms.
+constexpr int YP_T = 16, YP_VG = 2;
+
+struct YProjArgs {
+  const void* src;     // (x, ky, z) data of every volume (ST::T), volume pitch vstride
+  float2* C;           // K planes of N voxels, x fastest
+  const float2* pre;   // [v][ny] input ramps (used as given)
+  const float2* post;  // [v][ny] output ramps (conjugated, times scale)
+  const float2* tw;    // exp(-2 pi i m / ny)
+  long long nxy, vstride, N;
+  int nx, nz, K, nq;
+  float scale;
+  int tiles_x;         // ceil(nx / YP_T) x tiles per z slice
+  int per_xcd;         // work items per XCD (grid = 8 per_xcd)
+};
+
+template <int R1, int R2, int VG_>
+struct YProjShape {
+  static constexpr int N = R1 * R2, P = R1 > R2 ? R1 : R2, T = YP_T, VG = VG_;
+  static constexpr int GT = T * P, TH = VG * GT;
+  static constexpr int R2P = R2 + 1, LS = R1 * R2P + 1;
+  static constexpr int VPT = T * N / TH;  // voxels accumulated per thread
+  static constexpr int NR = 2 * VG * N;   // ramp entries staged per step
+  static constexpr int RPT = (NR + TH - 1) / TH;
+};
+
+template <int R1, int R2, int VG_, class ST, int KK, bool RS>
+__global__ void __launch_bounds__((YProjShape<R1, R2, VG_>::TH)) k_fft_y_adjoint_project(YProjArgs a,
+                                                                                        const float2* __restrict__ S) {
+  using L = YProjShape<R1, R2, VG_>;
+  constexpr int N = L::N, T = L::T, VG = L::VG, GT = L::GT, TH = L::TH, R2P = L::R2P, LS = L::LS, VPT = L::VPT;
+  static_assert(VPT * TH == T * N, "every thread accumulates the same number of voxels");
+  using E = typename ST::T;
+  __shared__ float2 lds[VG][T * LS];
+  __shared__ float2 s_tw[N];
+  __shared__ float2 s_rp[2][VG][N];  // [pre, post][group][index]
+  const int tid = threadIdx.x;
+  const int g = tid / GT;
+  const int t = (tid % GT) % T, j = (tid % GT) / T;
+  // XCD-contiguous work items: the x tiles of one z slice run side by side on one XCD
+  const int w = (int)(blockIdx.x % 8) * a.per_xcd + (int)(blockIdx.x / 8);
+  if (w >= a.tiles_x * a.nz) return;
+  const int x0 = (w % a.tiles_x) * T;
+  const long long z = w / a.tiles_x;
+  for (int i = tid; i < N; i += TH) s_tw[i] = cconj(a.tw[i]);
+  const bool line = x0 + t < a.nx;
+  const E* __restrict__ src = (const E*)a.src + z * a.nxy + x0 + t + (long long)j * a.nx;
+  const long long step = (long long)R2 * a.nx;
+  E xr[R1];
+  float2 rr[L::RPT];
+  auto load = [&](int v0) {
+    const int v = v0 + g;
+    if (line && v < a.nq && j < R2) {
+      const E* p = src + (long long)v * a.vstride;
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) xr[j1] = ST::rawnt(p + j1 * step);
+    }
+#pragma unroll
+    for (int q = 0; q < L::RPT; ++q) {
+      const int e = tid + TH * q;
+      if (e < L::NR) {
+        const int tb = e / (VG * N), gg = (e / N) % VG, i = e % N;
+        const long long vv = min(v0 + gg, a.nq - 1);
+        rr[q] = tb == 0 ? a.pre[vv * N + i] : cscale(cconj(a.post[vv * N + i]), a.scale);
+      }
+    }
+  };
+  float2 acc[VPT][KK];
+#pragma unroll
+  for (int i = 0; i < VPT; ++i)
+#pragma unroll
+    for (int k = 0; k < KK; ++k) acc[i][k] = make_float2(0.f, 0.f);
+  load(0);
+  for (int v0 = 0; v0 < a.nq; v0 += VG) {
+#pragma unroll
+    for (int q = 0; q < L::RPT; ++q) {
+      const int e = tid + TH * q;
+      if (e < L::NR) (&s_rp[0][0][0])[e] = rr[q];
+    }
+    // tables ready; the previous step's readers of lds are done
+    __syncthreads();
+    const bool vok = v0 + g < a.nq;
+    if (line && vok && j < R2) {
+      float2 x[R1];
+#pragma unroll
+      for (int j1 = 0; j1 < R1; ++j1) {
+        x[j1] = cmul(ST::cvt(xr[j1]), s_rp[0][g][j1 * R2 + j]);
+        x[j1].y = -x[j1].y;
+      }
+      SmallDFT<R1>::run(x);
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) {
+        float2 y = x[k1];
+        y.y = -y.y;
+        lds[g][t * LS + k1 * R2P + j] = cmul(y, s_tw[j * k1]);
+      }
+    }
+    if (v0 + VG < a.nq) load(v0 + VG);
+    __syncthreads();
+    // stage 2 in place: lane j reads row j and writes its outputs y = j + R1 k2 back into it
+    if (line && vok && j < R1) {
+      float2 y[R2];
+#pragma unroll
+      for (int j2 = 0; j2 < R2; ++j2) {
+        y[j2] = lds[g][t * LS + j * R2P + j2];
+        y[j2].y = -y[j2].y;
+      }
+      SmallDFT<R2>::run(y);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) {
+        float2 val = y[k2];
+        val.y = -val.y;
+        lds[g][t * LS + j * R2P + k2] = cmul(val, s_rp[1][g][j + R1 * k2]);
+      }
+    }
+    __syncthreads();
+    // one volume at a time (a rolled loop: the LDS reads of every volume are
+    // not hoisted ahead of the FMAs, which would not fit the registers)
+#pragma unroll 1
+    for (int gg = 0; gg < VG; ++gg) {
+      const int v = v0 + gg;
+      if (v >= a.nq) break;
+      float2 sv[KK];
+#pragma unroll
+      for (int k = 0; k < KK; ++k) sv[k] = S[v + (long long)min(k, a.K - 1) * a.nq];
+#pragma unroll
+      for (int i = 0; i < VPT; ++i) {
+        const int e = tid + TH * i, tt = e % T, yy = e / T;
+        if (x0 + tt >= a.nx) continue;
+        const float2 xv = lds[gg][tt * LS + (yy % R1) * R2P + yy / R1];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) {
+          acc[i][k].x = fmaf(xv.x, sv[k].x, acc[i][k].x);
+          acc[i][k].y = fmaf(xv.y, sv[k].x, acc[i][k].y);
+          if (!RS) {
+            acc[i][k].x = fmaf(-xv.y, sv[k].y, acc[i][k].x);
+            acc[i][k].y = fmaf(xv.x, sv[k].y, acc[i][k].y);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < VPT; ++i) {
+    const int e = tid + TH * i, tt = e % T, yy = e / T;
+    if (x0 + tt >= a.nx) continue;
+    float2* c = a.C + z * a.nxy + (long long)yy * a.nx + x0 + tt;
+#pragma unroll
+    for (int k = 0; k < KK; ++k)
+      if (k < a.K) c[(long long)k * a.N] = acc[i][k];
+  }
+}
+
+template <int R1, int R2, int VG, class ST>
+hipError_t launch_yproj_st(YProjArgs a, const float2* S, bool rs, hipStream_t s) {
+  using L = YProjShape<R1, R2, VG>;
+  a.tiles_x = (a.nx + L::T - 1) / L::T;
+  const long long W = (long long)a.tiles_x * a.nz;
+  a.per_xcd = (int)((W + 7) / 8);
+  const dim3 grid((unsigned)(8 * a.per_xcd));
+  if (a.K == 7 && rs)
+    hipLaunchKernelGGL((k_fft_y_adjoint_project<R1, R2, VG, ST, 7, true>), grid, dim3(L::TH), 0, s, a, S);
+  else if (a.K == 7)
+    hipLaunchKernelGGL((k_fft_y_adjoint_project<R1, R2, VG, ST, 7, false>), grid, dim3(L::TH), 0, s, a, S);
+  else
+    hipLaunchKernelGGL((k_fft_y_adjoint_project<R1, R2, VG, ST, 8, false>), grid, dim3(L::TH), 0, s, a, S);
+  return hipGetLastError();
+}
+
+template <int R1, int R2>
+hipError_t launch_yproj(YProjArgs a, const float2* S, bool rs, bool h16, hipStream_t s) {
+  return h16 ? launch_yproj_st<R1, R2, YP_VG, StoreF16>(a, S, rs, s) : launch_yproj_st<R1, R2, YP_VG, StoreF32>(a, S, rs, s);
+}
+
 template <int R1, int R2, bool TF, bool IV, bool RP, class ST, int MINT>
 hipError_t launch_one(PassArgs a, long long W, long long nvols, hipStream_t s) {
   using L = Layout<R1, R2, MINT>;
@@ -900,6 +1087,60 @@ llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* d
     return fail(LLR_ERR_ARG, "fused z pass needs " + std::to_string(*nparts) + " partial sums, buffer holds " +
                                  std::to_string(cap));
   if (e != hipSuccess) return hip_fail(e, "fft z projection launch");
+  return LLR_OK;
+}
+
+bool fft_yproj_supported(size_t ny, int K) {
+  if (K < 1 || K > 8) return false;
+  switch (ny) {
+    case 16: case 24: case 32: case 48: case 64: case 96: case 128: case 192: case 256:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// Inverse y pass (input ramps pre, output ramps conj(post) * scale, as
+// fft_axis_pass with pre_conj = 0, post_conj = 1) of every volume, projected:
+// C[k][n] = sum_q X[q][n] S[q][k] of the pass's output X, which is not stored.
+llr_status fft_y_adjoint_project(llr_ctx* ctx, const void* src, const size_t* dims, size_t nvols, const float2* pre,
+                                 const float2* post, float scale, float2* C, const float2* S, int K, bool s_real,
+                                 bool h16) {
+  if (!fft_yproj_supported(dims[1], K)) return fail(LLR_ERR_UNSUPPORTED, "fused adjoint projection: y length or rank");
+  if (!pre || !post || !C || !S) return fail(LLR_ERR_ARG, "fused adjoint projection needs ramps, planes and S");
+  if ((double)dims[0] * dims[1] * dims[2] >= 4294967296.0)
+    return fail(LLR_ERR_UNSUPPORTED, "volume larger than 2^32 voxels");
+  const int n = (int)dims[1];
+  const float2* tw = nullptr;
+  llr_status st = get_twiddles(ctx, n, &tw);
+  if (st) return st;
+  YProjArgs a{};
+  a.src = src;
+  a.C = C;
+  a.pre = pre;
+  a.post = post;
+  a.tw = tw;
+  a.nxy = (long long)(dims[0] * dims[1]);
+  a.vstride = a.nxy * (long long)dims[2];
+  a.N = a.vstride;
+  a.nx = (int)dims[0];
+  a.nz = (int)dims[2];
+  a.K = K;
+  a.nq = (int)nvols;
+  a.scale = scale;
+  hipError_t e;
+  switch (n) {
+    case 16: e = launch_yproj<4, 4>(a, S, s_real, h16, ctx->stream); break;
+    case 24: e = launch_yproj<6, 4>(a, S, s_real, h16, ctx->stream); break;
+    case 32: e = launch_yproj<8, 4>(a, S, s_real, h16, ctx->stream); break;
+    case 48: e = launch_yproj<12, 4>(a, S, s_real, h16, ctx->stream); break;
+    case 64: e = launch_yproj<8, 8>(a, S, s_real, h16, ctx->stream); break;
+    case 96: e = launch_yproj<12, 8>(a, S, s_real, h16, ctx->stream); break;
+    case 128: e = launch_yproj<16, 8>(a, S, s_real, h16, ctx->stream); break;
+    case 192: e = launch_yproj<12, 16>(a, S, s_real, h16, ctx->stream); break;
+    default: e = launch_yproj<16, 16>(a, S, s_real, h16, ctx->stream); break;
+  }
+  if (e != hipSuccess) return hip_fail(e, "fused adjoint projection launch");
   return LLR_OK;
 }
 

@@ -30,13 +30,13 @@ LLR_PREC_FP32 = 0
 LLR_PREC_MIXED = 1
 LLR_STORE_FP32 = 0
 LLR_STORE_FP16 = 1
-ABI_VERSION = 107
-LLR_NUM_STAGES = 15
+ABI_VERSION = 108
+LLR_NUM_STAGES = 16
 # llr_recon_stage_times order; fft_x/fft_y are the forward passes, *_adjoint the
 # inverse ones; wavelet_* are the wavelet's kernels (also inside "wavelet")
 STAGE_NAMES = ["llr_block", "subspace_project", "wavelet", "fft_x", "fft_y", "fft_z_project", "reductions", "setup",
                "unet", "fft_y_adjoint", "fft_x_adjoint", "wavelet_level0_in", "wavelet_level0_mid",
-               "wavelet_level0_out", "wavelet_deeper"]
+               "wavelet_level0_out", "wavelet_deeper", "fft_y_adjoint_project"]
 
 # every symbol include/llr_hip.h declares
 EXPORTS = [
@@ -137,6 +137,7 @@ class ReconParams(ctypes.Structure):
         ("storage", ctypes.c_int),
         ("no_hybrid", ctypes.c_int),
         ("lin_img_corr", ctypes.c_void_p),
+        ("fused_projection", ctypes.c_int),
     ]
 
 

@@ -56,7 +56,7 @@ def wavelet_denoise(volumes, n_volumes, vol_size, alpha, levels=-1, ctx=None):
 
 def make_params(vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0, lambda_factor=0.02,
                 wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0, precision="fp32",
-                partial_blocks=False, storage="fp32", hybrid=True, img_corr=None):
+                partial_blocks=False, storage="fp32", hybrid=True, img_corr=None, fused_projection=None):
     """llr_recon_params for the C ABI, plus the arrays it points at (keep them alive)."""
     vol = [int(v) for v in vol]
     nq = int(nq)
@@ -89,6 +89,7 @@ def make_params(vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lamb
     p.partial_blocks = 1 if partial_blocks else 0
     p.storage = {"fp32": _lib.LLR_STORE_FP32, "fp16": _lib.LLR_STORE_FP16}[storage]
     p.no_hybrid = 0 if hybrid else 1
+    p.fused_projection = 0 if fused_projection is None else (1 if fused_projection else -1)
     ic = None
     if img_corr is not None:
         ic = np.ascontiguousarray(np.asarray(img_corr, np.float32).reshape(-1))
@@ -117,15 +118,22 @@ class Recon:
     the full 3D forward model and adjoint in every iteration.
     img_corr: (nq, 3) linear image corrections of the signal model (a
     sub-voxel image shift per volume; signal_model.rs:34-79), default zero.
+    fused_projection: with the hybrid iteration and the subspace wavelet fused
+    into level 0, the adjoint's inverse y pass can hand the next LLR step the
+    subspace coefficients directly, the image being formed once at the end of
+    a run (same result up to rounding). None: with complex64 storage only
+    (where it is faster); True: with either storage; False: never (the image is
+    stored and projected in every iteration).
     """
 
     def __init__(self, vol, nq, shifts, blk=(8, 8, 8), llr_mode="svt", rank=0, svt_lambda=0.0,
                  lambda_factor=0.02, wavelet_alpha=0.0, wavelet_levels=-1, subspace=None, residual_stop=0.0,
-                 precision="fp32", partial_blocks=False, storage="fp32", hybrid=True, img_corr=None, ctx=None):
+                 precision="fp32", partial_blocks=False, storage="fp32", hybrid=True, img_corr=None, ctx=None,
+                 fused_projection=None):
         self.ctx = ctx or _lib.default_context()
         p, self._keep = make_params(vol, nq, shifts, blk, llr_mode, rank, svt_lambda, lambda_factor, wavelet_alpha,
                                     wavelet_levels, subspace, residual_stop, precision, partial_blocks, storage,
-                                    hybrid, img_corr)
+                                    hybrid, img_corr, fused_projection)
         self.vol = [int(v) for v in vol]
         self.nq = int(nq)
         self.D = int(np.prod(self.vol)) * self.nq

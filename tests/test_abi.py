@@ -42,7 +42,7 @@ def test_abi_version_and_error_string_without_gpu():
     import llr_hip._lib as L
 
     lib = ctypes.CDLL(LIB)
-    assert lib.llr_abi_version() == L.ABI_VERSION == 107
+    assert lib.llr_abi_version() == L.ABI_VERSION == 108
     lib.llr_last_error.restype = ctypes.c_char_p
     assert lib.llr_last_error() == b""
     g = (ctypes.c_size_t * 3)()

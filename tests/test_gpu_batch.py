@@ -89,7 +89,7 @@ def test_batch_autotune(llr):
         t = tun[i]
         print(f"[autotune member {i} {ds['vol']}x{ds['nq']}] {t}", flush=True)
         assert t["blk"][0] in (6, 8, 10) and len(set(t["blk"])) == 1
-        assert all(w in (1, 2, 4, 8) for w in t["fft_tpw"]) and t["llr_stage"] in (0, 1)
+        assert all(w in (1, 2, 3, 4, 6, 8) for w in t["fft_tpw"]) and t["llr_stage"] in (0, 1)
         img, h = single(ds, blk=t["blk"])
         assert np.array_equal(imgs[i], img)
     b.close()

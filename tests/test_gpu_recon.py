@@ -23,13 +23,19 @@ def run_gpu(ds, meas=None, hybrid_used=None, **kw):
     r = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode=kw.get("llr_mode", "svt"),
                     rank=kw.get("rank", 0), lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
                     subspace=ds["subspace"], partial_blocks=kw.get("partial_blocks", False),
-                    hybrid=kw.get("hybrid", True))
+                    hybrid=kw.get("hybrid", True), fused_projection=kw.get("fused_projection"),
+                    precision=kw.get("precision", "fp32"), storage=kw.get("storage", "fp32"),
+                    residual_stop=kw.get("residual_stop", 0.0))
+    if kw.get("stages") is not None:
+        r.enable_timing(True)
     r.set_measurements(ds["meas"] if meas is None else meas)
     hist = r.run()
     img = r.image()
     lam, kc = r.info()
     if hybrid_used is not None:
         hybrid_used.append(r.hybrid())
+    if kw.get("stages") is not None:
+        kw["stages"].update(r.stage_times())
     r.close()
     return img, hist, lam, kc
 
@@ -183,3 +189,69 @@ def test_recon_no_subspace_partial_blocks_wide(llr):
     the Gram and left alone by the apply)."""
     ds = H.make(dict(WIDE, vol=(48, 24, 32), blk=(10, 10, 10), n_iter=2))
     check(ds, "48x24x32 x67 no subspace partial 10^3", partial_blocks=True)
+
+
+@pytest.mark.parametrize("vol,blk", [((32, 32, 16), (8, 8, 8)), ((48, 96, 24), (8, 8, 8)), ((24, 48, 12), (6, 6, 6))])
+def test_fused_adjoint_projection_matches_unfused(llr, vol, blk):
+    """The adjoint's inverse y pass handing the next LLR step C = X S directly
+    (X formed once when the run ends) against storing X and projecting it in
+    every iteration: the same image up to FP32 rounding, each against the CPU
+    path. y lengths 32, 96 (radix 12 x 8) and 48 (12 x 4)."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, vol=vol, blk=blk, n_iter=4))
+    kw = dict(partial_blocks=True)
+    e_f = check(ds, f"{vol} fused adjoint projection", **kw)
+    e_u = check(ds, f"{vol} stored image + projection", fused_projection=False, **kw)
+    st_f, st_u = {}, {}
+    img_f, hist_f = run_gpu(ds, stages=st_f, **kw)[:2]
+    img_u, hist_u = run_gpu(ds, fused_projection=False, stages=st_u, **kw)[:2]
+    e = H.nrmse(img_f, img_u)
+    print(f"[{vol}] fused vs unfused NRMSE {e:.3e} (vs cpu32 {e_f:.3e} / {e_u:.3e}); launches fused "
+          f"{st_f['fft_y_adjoint_project'][1]} + {st_f['fft_y_adjoint'][1]} y adjoint, "
+          f"{st_f['subspace_project'][1]} projections", flush=True)
+    assert e <= 1e-5
+    assert np.allclose(hist_f, hist_u, rtol=1e-4)
+    n = ds["cfg"]["n_iter"]
+    # every adjoint but the last hands over coefficients; only the first LLR step projects
+    assert st_f["fft_y_adjoint_project"][1] == n - 1 and st_f["fft_y_adjoint"][1] == 1
+    assert st_u["fft_y_adjoint_project"][1] == 0 and st_u["fft_y_adjoint"][1] == n
+    assert st_f["subspace_project"][1] < st_u["subspace_project"][1]
+
+
+@pytest.mark.parametrize("storage", ["fp16", "fp32"])
+def test_fused_adjoint_projection_mixed_form(llr, storage):
+    """The mixed form (BF16 MFMA Gram; half or complex64 storage) with the fused
+    adjoint projection (asked for: half storage does not take it by default):
+    its FP32 projection of the unrounded adjoint replaces the BF16 one, so it
+    is at least as close to the FP64 path as the unfused run, and a residual
+    stop after a fused pass forms the image then (the image a run of that many
+    iterations ends with, bit for bit)."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, n_iter=6))
+    kw = dict(precision="mixed", storage=storage, fused_projection=True)
+    x64 = run_cpu(ds, np.complex128)[0]
+    img_f, hist_f = run_gpu(ds, **kw)[:2]
+    img_u, hist_u = run_gpu(ds, **dict(kw, fused_projection=False))[:2]
+    e_f, e_u = H.nrmse(img_f, x64), H.nrmse(img_u, x64)
+    # stop after the third iteration (a fused adjoint): the image is formed then
+    stop = float(hist_f[2]) * (1 + 1e-6)
+    st = {}
+    img_s, hist_s = run_gpu(ds, residual_stop=stop, stages=st, **kw)[:2]
+    ds3 = dict(ds, shifts=ds["shifts"][:3])
+    img_3 = run_gpu(ds3, **kw)[0]
+    e_s = H.nrmse(img_s, img_3)
+    print(f"[mixed {storage}] vs fp64: fused {e_f:.3e} unfused {e_u:.3e}; stop after 3: {len(hist_s)} iterations, "
+          f"vs a 3-iteration run {e_s:.3e}", flush=True)
+    assert e_f <= max(1.05 * e_u, 1e-4)
+    assert len(hist_s) == 3 and st["fft_y_adjoint_project"][1] == 3 and st["fft_y_adjoint"][1] == 1
+    # the deferred inverse y pass is the pass the 3-iteration run's last adjoint makes, on the same data
+    assert np.array_equal(img_s, img_3), e_s
+
+
+def test_fused_adjoint_projection_default_by_storage(llr):
+    """By default the fused adjoint projection runs with complex64 storage and
+    not with half storage (where the separate y pass and BF16 projection are
+    faster)."""
+    ds = H.make(dict(H.SMALL_SUBSPACE, n_iter=3))
+    for storage, fused in (("fp32", True), ("fp16", False)):
+        st = {}
+        run_gpu(ds, precision="mixed", storage=storage, stages=st)
+        assert (st["fft_y_adjoint_project"][1] > 0) == fused, (storage, st["fft_y_adjoint_project"])
