@@ -247,6 +247,9 @@ def main():
                     help="skip the learned form's run with the other storage (reported beside the headline)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "llr-reco_amd", "llr_hip", "weights", "warm_start_unet.npz"),
                     help="trained warm-start U-Net (.npz written by llr_hip.warm_start)")
+    ap.add_argument("--fused-projection", default="auto", choices=["auto", "on", "off"],
+                    help="adjoint y pass fused with the subspace projection: the library's choice by storage "
+                         "(auto), with either storage (on), or never (off)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy kernel (counter calibration) and exit")
     args = ap.parse_args()
@@ -296,8 +299,10 @@ def main():
 
     img = torch.empty_like(ds["y"])
     # ---- plain FP32 form: n_iter iterations from the zero-filled adjoint
+    fp = {"auto": None, "on": True, "off": False}[args.fused_projection]
     r = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode="svt",
-                    lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"], subspace=ds["subspace"])
+                    lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"], subspace=ds["subspace"],
+                    fused_projection=fp)
     r.set_measurements(device_ptr=ds["y"].data_ptr())
     s_plain, hist = timed(r.run)
     r.image_to_device(img.data_ptr())
@@ -323,7 +328,7 @@ def main():
         to_net, from_net = W.tensor_mixing(ds["bvals"], ds["bvecs"], ds["subspace"])
         rl = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode="svt",
                          lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
-                         subspace=ds["subspace"], precision="mixed", storage=args.storage)
+                         subspace=ds["subspace"], precision="mixed", storage=args.storage, fused_projection=fp)
         rl.set_measurements(device_ptr=ds["y"].data_ptr())
         run_kw = dict(momentum=W.LEARNED_MOMENTUM, min_iter=W.LEARNED_MIN_ITER)
         s_learned, (hl, fell_back) = timed(lambda: rl.run_learned(net, r_stop, to_net, from_net, **run_kw))
@@ -341,7 +346,7 @@ def main():
             alt = "fp32" if args.storage == "fp16" else "fp16"
             ra = recon.Recon(ds["vol"], ds["nq"], ds["shifts"], blk=cfg["blk"], llr_mode="svt",
                              lambda_factor=cfg["lambda_factor"], wavelet_alpha=cfg["wavelet_alpha"],
-                             subspace=ds["subspace"], precision="mixed", storage=alt)
+                             subspace=ds["subspace"], precision="mixed", storage=alt, fused_projection=fp)
             ra.set_measurements(device_ptr=ds["y"].data_ptr())
             s_alt, (ha, fb_a) = timed(lambda: ra.run_learned(net, r_stop, to_net, from_net, **run_kw))
             ra.image_to_device(img.data_ptr())

@@ -91,10 +91,17 @@ static llr_status download(void* h, const DevBuf& b, size_t bytes, llr_ctx* ctx)
 
 struct RampTables {
   DevBuf pk[3], pi[3];
+  // pi[a] is the parity pattern (-1)^(i + n/2): axis length n even and no
+  // image correction on that axis (ramp_table's angle is then 0 or pi; the
+  // imaginary parts it stores are below 1e-19)
+  bool pi_sign[3] = {false, false, false};
 };
 
 static llr_status make_tables(RampTables& t, const size_t* dims, size_t nv, const float* ksp, const float* img) {
   for (int a = 0; a < 3; ++a) {
+    t.pi_sign[a] = dims[a] % 2 == 0;
+    if (img)
+      for (size_t v = 0; v < nv; ++v) t.pi_sign[a] = t.pi_sign[a] && img[3 * v + a] == 0.f;
     std::vector<float2> h;
     ramp_table(h, dims[a], ksp ? ksp + a : nullptr, 3, nv);
     llr_status st = upload(t.pk[a], h.data(), h.size() * sizeof(float2));
@@ -862,7 +869,7 @@ llr_status data_consistency(llr_recon* r, double* err, bool to_coef = false) {
     if (to_coef && r->yproj && ng == r->nq) {
       StageTimer t(r, 15);
       st = fft_y_adjoint_project(ctx, X, p.vol, r->nq, pi1, pk1, sy, r->Cb.as<float2>(), r->S.as<float2>(),
-                                 (int)p.srank, r->s_real, h);
+                                 (int)p.srank, r->s_real, h, T.pi_sign[1]);
       if (st) return st;
       r->cb_adj = true;
       r->x_pending = true;
@@ -1134,8 +1141,8 @@ static llr_status recon_setup(llr_recon* r) {
       convert = r->s16 && !r->hybrid;
     }
     // automatic (0): complex64 storage only; at 256x256x128x67 the fused pass
-    // takes 1.70 ms against 1.66 + 0.99 ms for the y pass and the projection,
-    // but with half storage 1.52 ms against 0.80 + 0.65 ms
+    // takes 1.64 ms against 1.65 + 1.01 ms for the y pass and the projection in
+    // the iteration, but with half storage 1.60 ms against 0.85 + 0.77 ms
     r->yproj = (p.fused_projection > 0 || (p.fused_projection == 0 && !r->s16)) && r->fuse_wavelet && r->hybrid &&
                p.srank && fft_yproj_supported(p.vol[1], (int)p.srank) && (r->vgroup == 0 || r->vgroup >= r->nq);
     if (convert && (st = to_h16_device(ctx, r->Y.as<float2>(), r->Yh.p, r->D, (float)r->hs, true))) return st;
@@ -1675,7 +1682,8 @@ static llr_status batch_post(llr_recon* r, const unsigned* iter, double* tmp) {
     return st;
   if (r->yproj) {
     if ((st = fft_y_adjoint_project(ctx, X, p.vol, r->nq, T.pi[1].as<float2>(), T.pk[1].as<float2>(), sy,
-                                    r->Cb.as<float2>(), r->S.as<float2>(), (int)p.srank, r->s_real, false)))
+                                    r->Cb.as<float2>(), r->S.as<float2>(), (int)p.srank, r->s_real, false,
+                                    T.pi_sign[1])))
       return st;
   } else if ((st = fft_axis_pass(ctx, X, X, p.vol, r->nq, 1, true, T.pi[1].as<float2>(), false,
                                  T.pk[1].as<float2>(), true, sy))) {

@@ -107,7 +107,10 @@ struct SmallDFT {
     SmallDFT<R / 2>::run(o);
 #pragma unroll
     for (int k = 0; k < R / 2; ++k) {
-      float2 t = cmul(o[k], tw_const<R>(k));
+      // W^0 = 1 and W^(R/4) = -i taken exactly (k is a constant once unrolled:
+      // x * 0 does not fold without finite-math, so cmul would spend a multiply
+      // and an add on each)
+      const float2 t = k == 0 ? o[k] : (4 * k == R ? make_float2(o[k].y, -o[k].x) : cmul(o[k], tw_const<R>(k)));
       v[k] = cadd(e[k], t);
       v[k + R / 2] = csub(e[k], t);
     }
@@ -606,7 +609,8 @@ __global__ void __launch_bounds__(T * R2) k_fft_z_project_nw(ZProjArgs a) {
 // until the last volume. The input stays where it is (the z pass's output).
 // VG = 2: 512-thread workgroups, 8 voxels (112 accumulators) per thread, one
 // workgroup per CU (80 KB of LDS at ny = 256). At 256x256x128x67 this takes
-// 1.52 / 1.70 ms per launch for half / complex64 storage; VG = 4 (1024
+// 1.41 / 1.60 ms per launch for half / complex64 storage with the parity-sign
+// input ramps (1.51 / 1.69 ms multiplying by the table); VG = 4 (1024
 // threads, 56 accumulators, spilling at the 128 registers such a workgroup
 // allows) took 1.75 / 2.83 ms.
 constexpr int YP_T = 16, YP_VG = 2;
@@ -630,20 +634,25 @@ struct YProjShape {
   static constexpr int GT = T * P, TH = VG * GT;
   static constexpr int R2P = R2 + 1, LS = R1 * R2P + 1;
   static constexpr int VPT = T * N / TH;  // voxels accumulated per thread
-  static constexpr int NR = 2 * VG * N;   // ramp entries staged per step
-  static constexpr int RPT = (NR + TH - 1) / TH;
+  static constexpr int NR = 2 * VG * N;   // ramp entries staged per step (input and output ramps)
 };
 
-template <int R1, int R2, int VG_, class ST, int KK, bool RS>
+// SGN: the input ramps are the parity pattern (-1)^(y + N/2). Stage 1's inputs
+// y = j1 R2 + j (R2 even) then share lane j's sign, which the linear stage-1
+// DFT carries to its outputs: it is folded into that lane's twiddles
+// (s_tw[j R1 + k1] = sign(j) W^-(j k1)) and only the output ramps are staged.
+template <int R1, int R2, int VG_, class ST, int KK, bool RS, bool SGN>
 __global__ void __launch_bounds__((YProjShape<R1, R2, VG_>::TH)) k_fft_y_adjoint_project(YProjArgs a,
                                                                                         const float2* __restrict__ S) {
   using L = YProjShape<R1, R2, VG_>;
   constexpr int N = L::N, T = L::T, VG = L::VG, GT = L::GT, TH = L::TH, R2P = L::R2P, LS = L::LS, VPT = L::VPT;
   static_assert(VPT * TH == T * N, "every thread accumulates the same number of voxels");
+  static_assert(!SGN || R2 % 2 == 0, "the parity of y = j1 R2 + j is that of j");
+  constexpr int NR = SGN ? VG * N : L::NR, RPT = (NR + TH - 1) / TH;
   using E = typename ST::T;
   __shared__ float2 lds[VG][T * LS];
   __shared__ float2 s_tw[N];
-  __shared__ float2 s_rp[2][VG][N];  // [pre, post][group][index]
+  __shared__ float2 s_rp[2][VG][N];  // [pre, post][group][index] (SGN: post only, in s_rp[0])
   const int tid = threadIdx.x;
   const int g = tid / GT;
   const int t = (tid % GT) % T, j = (tid % GT) / T;
@@ -652,12 +661,20 @@ __global__ void __launch_bounds__((YProjShape<R1, R2, VG_>::TH)) k_fft_y_adjoint
   if (w >= a.tiles_x * a.nz) return;
   const int x0 = (w % a.tiles_x) * T;
   const long long z = w / a.tiles_x;
-  for (int i = tid; i < N; i += TH) s_tw[i] = cconj(a.tw[i]);
+  for (int i = tid; i < N; i += TH) {
+    if (SGN) {
+      const int jj = i / R1, k1 = i % R1;  // (jj < R2)
+      const float2 w = cconj(a.tw[jj * k1]);
+      s_tw[i] = ((jj + N / 2) & 1) ? make_float2(-w.x, -w.y) : w;
+    } else {
+      s_tw[i] = cconj(a.tw[i]);
+    }
+  }
   const bool line = x0 + t < a.nx;
   const E* __restrict__ src = (const E*)a.src + z * a.nxy + x0 + t + (long long)j * a.nx;
   const long long step = (long long)R2 * a.nx;
   E xr[R1];
-  float2 rr[L::RPT];
+  float2 rr[RPT];
   auto load = [&](int v0) {
     const int v = v0 + g;
     if (line && v < a.nq && j < R2) {
@@ -666,15 +683,16 @@ __global__ void __launch_bounds__((YProjShape<R1, R2, VG_>::TH)) k_fft_y_adjoint
       for (int j1 = 0; j1 < R1; ++j1) xr[j1] = ST::rawnt(p + j1 * step);
     }
 #pragma unroll
-    for (int q = 0; q < L::RPT; ++q) {
+    for (int q = 0; q < RPT; ++q) {
       const int e = tid + TH * q;
-      if (e < L::NR) {
-        const int tb = e / (VG * N), gg = (e / N) % VG, i = e % N;
+      if (e < NR) {
+        const int tb = SGN ? 1 : e / (VG * N), gg = (e / N) % VG, i = e % N;
         const long long vv = min(v0 + gg, a.nq - 1);
         rr[q] = tb == 0 ? a.pre[vv * N + i] : cscale(cconj(a.post[vv * N + i]), a.scale);
       }
     }
   };
+  constexpr int PO = SGN ? 0 : 1;  // s_rp slot of the output ramps
   float2 acc[VPT][KK];
 #pragma unroll
   for (int i = 0; i < VPT; ++i)
@@ -683,9 +701,9 @@ __global__ void __launch_bounds__((YProjShape<R1, R2, VG_>::TH)) k_fft_y_adjoint
   load(0);
   for (int v0 = 0; v0 < a.nq; v0 += VG) {
 #pragma unroll
-    for (int q = 0; q < L::RPT; ++q) {
+    for (int q = 0; q < RPT; ++q) {
       const int e = tid + TH * q;
-      if (e < L::NR) (&s_rp[0][0][0])[e] = rr[q];
+      if (e < NR) (&s_rp[0][0][0])[e] = rr[q];
     }
     // tables ready; the previous step's readers of lds are done
     __syncthreads();
@@ -694,7 +712,7 @@ __global__ void __launch_bounds__((YProjShape<R1, R2, VG_>::TH)) k_fft_y_adjoint
       float2 x[R1];
 #pragma unroll
       for (int j1 = 0; j1 < R1; ++j1) {
-        x[j1] = cmul(ST::cvt(xr[j1]), s_rp[0][g][j1 * R2 + j]);
+        x[j1] = SGN ? ST::cvt(xr[j1]) : cmul(ST::cvt(xr[j1]), s_rp[0][g][j1 * R2 + j]);
         x[j1].y = -x[j1].y;
       }
       SmallDFT<R1>::run(x);
@@ -702,7 +720,7 @@ __global__ void __launch_bounds__((YProjShape<R1, R2, VG_>::TH)) k_fft_y_adjoint
       for (int k1 = 0; k1 < R1; ++k1) {
         float2 y = x[k1];
         y.y = -y.y;
-        lds[g][t * LS + k1 * R2P + j] = cmul(y, s_tw[j * k1]);
+        lds[g][t * LS + k1 * R2P + j] = cmul(y, s_tw[SGN ? j * R1 + k1 : j * k1]);
       }
     }
     if (v0 + VG < a.nq) load(v0 + VG);
@@ -720,7 +738,7 @@ __global__ void __launch_bounds__((YProjShape<R1, R2, VG_>::TH)) k_fft_y_adjoint
       for (int k2 = 0; k2 < R2; ++k2) {
         float2 val = y[k2];
         val.y = -val.y;
-        lds[g][t * LS + j * R2P + k2] = cmul(val, s_rp[1][g][j + R1 * k2]);
+        lds[g][t * LS + j * R2P + k2] = cmul(val, s_rp[PO][g][j + R1 * k2]);
       }
     }
     __syncthreads();
@@ -761,7 +779,7 @@ __global__ void __launch_bounds__((YProjShape<R1, R2, VG_>::TH)) k_fft_y_adjoint
   }
 }
 
-template <int R1, int R2, int VG, class ST>
+template <int R1, int R2, int VG, class ST, bool SGN>
 hipError_t launch_yproj_st(YProjArgs a, const float2* S, bool rs, hipStream_t s) {
   using L = YProjShape<R1, R2, VG>;
   a.tiles_x = (a.nx + L::T - 1) / L::T;
@@ -769,17 +787,21 @@ hipError_t launch_yproj_st(YProjArgs a, const float2* S, bool rs, hipStream_t s)
   a.per_xcd = (int)((W + 7) / 8);
   const dim3 grid((unsigned)(8 * a.per_xcd));
   if (a.K == 7 && rs)
-    hipLaunchKernelGGL((k_fft_y_adjoint_project<R1, R2, VG, ST, 7, true>), grid, dim3(L::TH), 0, s, a, S);
+    hipLaunchKernelGGL((k_fft_y_adjoint_project<R1, R2, VG, ST, 7, true, SGN>), grid, dim3(L::TH), 0, s, a, S);
   else if (a.K == 7)
-    hipLaunchKernelGGL((k_fft_y_adjoint_project<R1, R2, VG, ST, 7, false>), grid, dim3(L::TH), 0, s, a, S);
+    hipLaunchKernelGGL((k_fft_y_adjoint_project<R1, R2, VG, ST, 7, false, SGN>), grid, dim3(L::TH), 0, s, a, S);
   else
-    hipLaunchKernelGGL((k_fft_y_adjoint_project<R1, R2, VG, ST, 8, false>), grid, dim3(L::TH), 0, s, a, S);
+    hipLaunchKernelGGL((k_fft_y_adjoint_project<R1, R2, VG, ST, 8, false, SGN>), grid, dim3(L::TH), 0, s, a, S);
   return hipGetLastError();
 }
 
 template <int R1, int R2>
-hipError_t launch_yproj(YProjArgs a, const float2* S, bool rs, bool h16, hipStream_t s) {
-  return h16 ? launch_yproj_st<R1, R2, YP_VG, StoreF16>(a, S, rs, s) : launch_yproj_st<R1, R2, YP_VG, StoreF32>(a, S, rs, s);
+hipError_t launch_yproj(YProjArgs a, const float2* S, bool rs, bool h16, bool sgn, hipStream_t s) {
+  if (sgn)
+    return h16 ? launch_yproj_st<R1, R2, YP_VG, StoreF16, true>(a, S, rs, s)
+               : launch_yproj_st<R1, R2, YP_VG, StoreF32, true>(a, S, rs, s);
+  return h16 ? launch_yproj_st<R1, R2, YP_VG, StoreF16, false>(a, S, rs, s)
+             : launch_yproj_st<R1, R2, YP_VG, StoreF32, false>(a, S, rs, s);
 }
 
 template <int R1, int R2, bool TF, bool IV, bool RP, class ST, int MINT>
@@ -1105,7 +1127,7 @@ bool fft_yproj_supported(size_t ny, int K) {
 // C[k][n] = sum_q X[q][n] S[q][k] of the pass's output X, which is not stored.
 llr_status fft_y_adjoint_project(llr_ctx* ctx, const void* src, const size_t* dims, size_t nvols, const float2* pre,
                                  const float2* post, float scale, float2* C, const float2* S, int K, bool s_real,
-                                 bool h16) {
+                                 bool h16, bool pre_sign) {
   if (!fft_yproj_supported(dims[1], K)) return fail(LLR_ERR_UNSUPPORTED, "fused adjoint projection: y length or rank");
   if (!pre || !post || !C || !S) return fail(LLR_ERR_ARG, "fused adjoint projection needs ramps, planes and S");
   if ((double)dims[0] * dims[1] * dims[2] >= 4294967296.0)
@@ -1130,15 +1152,15 @@ llr_status fft_y_adjoint_project(llr_ctx* ctx, const void* src, const size_t* di
   a.scale = scale;
   hipError_t e;
   switch (n) {
-    case 16: e = launch_yproj<4, 4>(a, S, s_real, h16, ctx->stream); break;
-    case 24: e = launch_yproj<6, 4>(a, S, s_real, h16, ctx->stream); break;
-    case 32: e = launch_yproj<8, 4>(a, S, s_real, h16, ctx->stream); break;
-    case 48: e = launch_yproj<12, 4>(a, S, s_real, h16, ctx->stream); break;
-    case 64: e = launch_yproj<8, 8>(a, S, s_real, h16, ctx->stream); break;
-    case 96: e = launch_yproj<12, 8>(a, S, s_real, h16, ctx->stream); break;
-    case 128: e = launch_yproj<16, 8>(a, S, s_real, h16, ctx->stream); break;
-    case 192: e = launch_yproj<12, 16>(a, S, s_real, h16, ctx->stream); break;
-    default: e = launch_yproj<16, 16>(a, S, s_real, h16, ctx->stream); break;
+    case 16: e = launch_yproj<4, 4>(a, S, s_real, h16, pre_sign, ctx->stream); break;
+    case 24: e = launch_yproj<6, 4>(a, S, s_real, h16, pre_sign, ctx->stream); break;
+    case 32: e = launch_yproj<8, 4>(a, S, s_real, h16, pre_sign, ctx->stream); break;
+    case 48: e = launch_yproj<12, 4>(a, S, s_real, h16, pre_sign, ctx->stream); break;
+    case 64: e = launch_yproj<8, 8>(a, S, s_real, h16, pre_sign, ctx->stream); break;
+    case 96: e = launch_yproj<12, 8>(a, S, s_real, h16, pre_sign, ctx->stream); break;
+    case 128: e = launch_yproj<16, 8>(a, S, s_real, h16, pre_sign, ctx->stream); break;
+    case 192: e = launch_yproj<12, 16>(a, S, s_real, h16, pre_sign, ctx->stream); break;
+    default: e = launch_yproj<16, 16>(a, S, s_real, h16, pre_sign, ctx->stream); break;
   }
   if (e != hipSuccess) return hip_fail(e, "fused adjoint projection launch");
   return LLR_OK;

@@ -32,11 +32,12 @@ llr_status fft_zproj(llr_ctx* ctx, void* data, const void* meas, const size_t* d
 // The adjoint's inverse y pass fused with the subspace projection: C[k][n] =
 // sum_q X[q][n] S[q][k] (K planes of N voxels) of the pass's output X, which is
 // not stored (src keeps the pass's input). y lengths 16..256 that are 2^a 3^b
-// with radices of at least 4, K <= 8.
+// with radices of at least 4, K <= 8. pre_sign: the input ramps are the parity
+// pattern (-1)^(i + ny/2) (taken as exact signs; pre is not read).
 bool fft_yproj_supported(size_t ny, int K);
 llr_status fft_y_adjoint_project(llr_ctx* ctx, const void* src, const size_t* dims, size_t nvols, const float2* pre,
                                  const float2* post, float scale, float2* C, const float2* S, int K, bool s_real,
-                                 bool h16);
+                                 bool h16, bool pre_sign = false);
 
 // llr_wide.hip: LLR prox of blocks with more than 8 columns (up to 128).
 // work: llr_wide_work_elems(nc, nblocks) complex values, or null to allocate.

@@ -550,8 +550,9 @@ struct XYSynArgs {
 // that do not exist (past ncx / ncy) are staged as zeros, so the synthesis
 // reads the two taps of an output without range checks: output s uses
 // coefficients floor(s/2) (tap 2 if s is even, 3 if odd) and floor(s/2) + 1
-// (tap 0 / 1), tile origins being even.
-template <class ST, int TX>
+// (tap 0 / 1), tile origins being even. AP: a.Ap is set (without it the LL
+// band's next-level taps are compiled out).
+template <class ST, int TX, bool AP>
 __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   using E = typename ST::T;
   constexpr int NX = TX / 2 + 1;                      // staged coefficient columns
@@ -569,11 +570,11 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
   const long long v = blockIdx.y / a.az;
   const int ty0 = y0 / 2;
   const E* src = (const E*)a.B + v * a.bvp + (long long)z * a.bzp;
-  const E* ap = a.Ap ? (const E*)a.Ap + v * a.apvp : nullptr;
+  const E* ap = AP ? (const E*)a.Ap + v * a.apvp : nullptr;
   // the LL band of this z slice picks up syn_z(Ap, 0): taps j with z+2-j even and in range
   int za = -1, zb = -1;
   float wa = 0.f, wb = 0.f;
-  if (ap) {
+  if (AP) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       int idx = z + 2 - j;
@@ -604,7 +605,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
       lix[q] = ix;
       if (cy < a.ncy) {
         lofs[q] = ((b >> 1) ? a.ncy + cy : cy) * (int)a.bry + ((b & 1) ? a.px : 0);
-        if (b == 0 && ap) aofs[q] = cy * a.pa;
+        if (b == 0 && AP) aofs[q] = cy * a.pa;
       }
     }
   }
@@ -636,13 +637,13 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
     for (int q = 0; q < NLD; ++q) {
       const int cx = tx0 + lix[q];
       bn[q] = ST::zero();
-      if (q < NAQ) {
+      if (AP && q < NAQ) {
         ba[q] = ST::zero();
         bb[q] = ST::zero();
       }
       if (lofs[q] >= 0 && cx < a.ncx) {
         bn[q] = src[lofs[q] + cx];  // (cached: neighbouring tiles share band rows and columns)
-        if (q < NAQ && aofs[q] >= 0) {
+        if (AP && q < NAQ && aofs[q] >= 0) {
           if (za >= 0) ba[q] = ap[pza + aofs[q] + cx];
           if (zb >= 0) bb[q] = ap[pzb + aofs[q] + cx];
         }
@@ -658,7 +659,7 @@ __global__ void __launch_bounds__(256) k_wav_xy_syn(XYSynArgs a) {
       const int e = tid + 256 * q;
       if (e < 4 * SY_NY * NX) {
         float2 b = ST::cvt(bn[q]);
-        if (q < NAQ) {
+        if (AP && q < NAQ) {
           fma2(b, wa, ST::cvt(ba[q]));
           fma2(b, wb, ST::cvt(bb[q]));
         }
@@ -841,10 +842,16 @@ static void launch_xy_syn(hipStream_t s, const void* B, const LevelBox& q, const
   a3.f = f;
   const int tiles_y = (int)((od[1] + SY_TY - 1) / SY_TY);
   dim3 g3((unsigned)(a3.groups_x * tiles_y), (unsigned)(od[2] * nv));
-  if (narrow)
-    hipLaunchKernelGGL((k_wav_xy_syn<ST, 32>), g3, dim3(256), 0, s, a3);
-  else
-    hipLaunchKernelGGL((k_wav_xy_syn<ST, SY_TX>), g3, dim3(256), 0, s, a3);
+  if (narrow) {
+    if (Ap)
+      hipLaunchKernelGGL((k_wav_xy_syn<ST, 32, true>), g3, dim3(256), 0, s, a3);
+    else
+      hipLaunchKernelGGL((k_wav_xy_syn<ST, 32, false>), g3, dim3(256), 0, s, a3);
+  } else if (Ap) {
+    hipLaunchKernelGGL((k_wav_xy_syn<ST, SY_TX, true>), g3, dim3(256), 0, s, a3);
+  } else {
+    hipLaunchKernelGGL((k_wav_xy_syn<ST, SY_TX, false>), g3, dim3(256), 0, s, a3);
+  }
 }
 
 template <class ST>
