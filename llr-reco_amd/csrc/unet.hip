@@ -156,8 +156,11 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
 // voxels its 27 offsets read, so each input value is fetched from memory about
 // 9 / 3 times per workgroup instead of once per offset and lane, and the
 // chunk's 27 x cout x 16 weights, so the MFMA loop reads only LDS. Rows are
-// padded to CV_ROW values (40 bytes) so the 16 lanes of an operand read hit
-// distinct banks. FP32 inputs (the backward-data gradients) are rounded to
+// CV_ROW = 16 values (32 bytes): an operand read (lanes n = 16 consecutive
+// rows, 8 bytes each at offset 8 kq) spreads evenly over the banks, and each
+// staging store writes a 16-byte half row (two 8-byte stores per half row put
+// 8 lanes on one bank pair; the four level-0 convolutions at 256x256x128 took
+// 4.16 ms that way against 4.01 ms). FP32 inputs (the backward-data gradients) are rounded to
 // BF16 while staged. With a.up set, input channels [0, a.cup) are the nearest
 // upsampling of the half-resolution tensor a.up and channels [a.cup, cin) come
 // from a.in (the decoder's [upsampled, skip] concatenation, never stored).
@@ -276,18 +279,14 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
       const int e = tid + 256 * q;
       if (e >= 9 * CV_RUN * 2) continue;
       const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
-      uint2* d = (uint2*)(s_run + (size_t)(r * CV_RUN + i) * CV_ROW + 8 * half);
-      d[0] = make_uint2(ur[q].x, ur[q].y);
-      d[1] = make_uint2(ur[q].z, ur[q].w);
+      *(uint4*)(s_run + (size_t)(r * CV_RUN + i) * CV_ROW + 8 * half) = ur[q];
     }
 #pragma unroll
     for (int q = 0; q < NWL; ++q) {
       const int e = tid + 256 * q;
       if (e >= 27 * 16 * NTM * 2) continue;
       const int half = e & 1, col = (e >> 1) % (16 * NTM), o = (e >> 1) / (16 * NTM);
-      uint2* d = (uint2*)(s_w + (size_t)(o * 16 * NTM + col) * CV_ROW + 8 * half);
-      d[0] = make_uint2(uw[q].x, uw[q].y);
-      d[1] = make_uint2(uw[q].z, uw[q].w);
+      *(uint4*)(s_w + (size_t)(o * 16 * NTM + col) * CV_ROW + 8 * half) = uw[q];
     }
     __syncthreads();
 #pragma unroll
@@ -341,6 +340,73 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
         }
       }
     }
+  }
+}
+
+// 1^3 convolution with 16 input and 16 output channels into FP32 (the output
+// layer): out[v][co] = sum_ci w[co][ci] in[v][ci] + bias[co] (ReLU if set),
+// plus addend[v][co] when given. One thread per voxel: two 16-byte loads of
+// its BF16 channels, the weights (FP32 copies in LDS, read by every lane at
+// the same address) and FP32 FMAs (products of BF16 values are exact in FP32,
+// as in the MFMA form; only the summation order differs), four 16-byte loads
+// of the addend and four 16-byte stores. k_conv's 16-voxel wavefront tiles
+// store 4 bytes per lane: 0.87 ms at 256x256x128.
+__global__ void __launch_bounds__(256) k_conv1_16(const bf16_t* __restrict__ in, const bf16_t* __restrict__ w,
+                                                  const float* __restrict__ bias, const float* __restrict__ addend,
+                                                  float* __restrict__ out, long long V, int relu) {
+  __shared__ float4 sw[16 * 4];  // [co][ci / 4]
+  __shared__ float sb[16];
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    const uint2 u = *(const uint2*)(w + 4 * tid);
+    sw[tid] = make_float4(bf2f((bf16_t)(u.x & 0xFFFF)), bf2f((bf16_t)(u.x >> 16)), bf2f((bf16_t)(u.y & 0xFFFF)),
+                          bf2f((bf16_t)(u.y >> 16)));
+  }
+  if (tid < 16) sb[tid] = bias ? bias[tid] : 0.f;
+  __syncthreads();
+  for (long long v = blockIdx.x * (long long)blockDim.x + tid; v < V; v += (long long)gridDim.x * blockDim.x) {
+    const uint4* p = (const uint4*)(in + (size_t)v * 16);
+    const uint4 u0 = p[0], u1 = p[1];
+    const unsigned wd[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+    float x[16];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      x[2 * i] = __uint_as_float(wd[i] << 16);
+      x[2 * i + 1] = __uint_as_float(wd[i] & 0xFFFF0000u);
+    }
+    float r[16];
+#pragma unroll
+    for (int co = 0; co < 16; ++co) {
+      // the row index passes through an empty asm statement, so the weights are
+      // read here, row by row, and not hoisted out of the voxel loop into 256
+      // registers
+      int wi = co * 4;
+      asm volatile("" : "+v"(wi));
+      float acc = sb[co];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 wv = sw[wi + q];
+        acc = fmaf(wv.x, x[4 * q], acc);
+        acc = fmaf(wv.y, x[4 * q + 1], acc);
+        acc = fmaf(wv.z, x[4 * q + 2], acc);
+        acc = fmaf(wv.w, x[4 * q + 3], acc);
+      }
+      r[co] = relu ? fmaxf(acc, 0.f) : acc;
+    }
+    float4* o = (float4*)(out + (size_t)v * 16);
+    if (addend) {
+      const float4* ad = (const float4*)(addend + (size_t)v * 16);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 a4 = ad[q];
+        r[4 * q] += a4.x;
+        r[4 * q + 1] += a4.y;
+        r[4 * q + 2] += a4.z;
+        r[4 * q + 3] += a4.w;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
   }
 }
 
@@ -557,10 +623,12 @@ struct Mix {
 // K complex planes of N voxels (x fastest) -> (N, C) channels
 // [Re b0, Im b0, Re b1, ...] / scale with b = M c per voxel, remaining
 // channels zero. One thread per voxel: K coalesced plane reads, C/4 16-byte
-// stores (FP32) or C/8 (BF16).
+// stores (FP32) or C/8 (BF16). outb (FP32 out only, or null): the same
+// channels rounded to BF16 as well (the network's input beside the residual).
 template <typename TOut>
 __global__ void __launch_bounds__(256) k_pack_coef(const float2* __restrict__ c, TOut* __restrict__ out, long long N,
-                                                   int K, int C, Mix M, float inv_scale) {
+                                                   int K, int C, Mix M, float inv_scale,
+                                                   bf16_t* __restrict__ outb = nullptr) {
   for (long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x; v < N; v += (long long)gridDim.x * blockDim.x) {
     float2 z[MAX_MIX];
 #pragma unroll
@@ -582,6 +650,19 @@ __global__ void __launch_bounds__(256) k_pack_coef(const float2* __restrict__ c,
 #pragma unroll
       for (int q = 0; q < 2 * MAX_MIX; q += 4) *(float4*)(o + q) = make_float4(ch[q], ch[q + 1], ch[q + 2], ch[q + 3]);
       for (int q = 2 * MAX_MIX; q < C; q += 4) *(float4*)(o + q) = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (outb) {
+        bf16_t* ob = outb + (size_t)v * C;
+#pragma unroll
+        for (int q = 0; q < 2 * MAX_MIX; q += 8) {
+          uint4 u;
+          u.x = (unsigned)f2bf(ch[q]) | ((unsigned)f2bf(ch[q + 1]) << 16);
+          u.y = (unsigned)f2bf(ch[q + 2]) | ((unsigned)f2bf(ch[q + 3]) << 16);
+          u.z = (unsigned)f2bf(ch[q + 4]) | ((unsigned)f2bf(ch[q + 5]) << 16);
+          u.w = (unsigned)f2bf(ch[q + 6]) | ((unsigned)f2bf(ch[q + 7]) << 16);
+          *(uint4*)(ob + q) = u;
+        }
+        for (int q = 2 * MAX_MIX; q < C; q += 8) *(uint4*)(ob + q) = make_uint4(0u, 0u, 0u, 0u);
+      }
     } else {
 #pragma unroll
       for (int q = 0; q < 2 * MAX_MIX; q += 8) {
@@ -828,6 +909,14 @@ static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int 
       else
         hipLaunchKernelGGL((k_conv3<TIn, TOut, 4, false>), g4, dim3(256), 0, s, a);
     }
+  } else if constexpr (sizeof(TIn) == 2 && sizeof(TOut) == 4) {
+    if (a.cin == 16 && a.cout == 16 && !a.mirror && !a.mask && !a.accumulate) {
+      hipLaunchKernelGGL(k_conv1_16, dim3(grid_for((size_t)V, 256, 16384)), dim3(256), 0, U->ctx->stream,
+                         (const bf16_t*)a.in, a.w, a.bias, a.addend, (float*)a.out, V, a.relu);
+    } else {
+      unsigned grid = (unsigned)((V + 63) / 64);
+      hipLaunchKernelGGL((k_conv<TIn, TOut>), dim3(grid), dim3(256), 0, U->ctx->stream, a);
+    }
   } else {
     unsigned grid = (unsigned)((V + 63) / 64);
     hipLaunchKernelGGL((k_conv<TIn, TOut>), dim3(grid), dim3(256), 0, U->ctx->stream, a);
@@ -990,9 +1079,9 @@ llr_status unet_predict_device(llr_unet* U, const float2* coef, int K, const siz
   llr_status st;
   if ((st = ensure_workspace(U, vol))) return st;
   const long long N = (long long)(vol[0] * vol[1] * vol[2]);
+  // the FP32 channels (the residual) and their BF16 copy (the network's input) in one pass
   hipLaunchKernelGGL(k_pack_coef<float>, dim3(vox_grid((size_t)N)), dim3(256), 0, U->ctx->stream, coef,
-                     U->xf.as<float>(), N, K, U->cin, make_mix(to_net, K), 1.0f / scale);
-  if ((st = unet_set_input(U, vol))) return st;
+                     U->xf.as<float>(), N, K, U->cin, make_mix(to_net, K), 1.0f / scale, U->xb.as<bf16_t>());
   if ((st = unet_forward(U, vol, false))) return st;
   hipLaunchKernelGGL(k_unpack_coef, dim3(vox_grid((size_t)N)), dim3(256), 0, U->ctx->stream, U->y.as<float>(), out,
                      N, K, U->cin, make_mix(from_net, K), scale);
