@@ -54,9 +54,11 @@ def build_measurements(preset, torch, llr, keep_truth=False):
     return phantom.device_dataset(preset, torch, keep_truth=keep_truth, log=log)
 
 
-def stage_bytes(ds, eb=8):
+def stage_bytes(ds, eb=8, momentum=False):
     """Algorithmic HBM bytes per launch of each timed stage; eb = bytes per stored
-    complex value of the iterate (8 complex64, 4 half pairs), D = N Q eb.
+    complex value of the iterate (8 complex64, 4 half pairs), D = N Q eb;
+    momentum: the LLR step also reads the previous coefficients and stores the
+    new ones (the learned run's extrapolation, applied as the blocks load).
 
     Operators (the whole-path accounting): the LLR step works on the K coefficient
     planes in place (complex64) when there is a subspace; the wavelet reads those
@@ -79,7 +81,7 @@ def stage_bytes(ds, eb=8):
     D = N * Q * eb
     CK = N * K * 8
     return {
-        "llr_block": 2 * CK if sub else 2 * D,
+        "llr_block": (4 if momentum else 2) * CK if sub else 2 * D,
         "subspace_project": D + CK,                      # read X, write C
         "wavelet": CK + D if sub else 2 * D,             # read C (or X), write X
         "wavelet_level0_in": 2 * CK if sub else 2 * D,
@@ -340,7 +342,8 @@ def main():
             f"({e_learned / e_plain:.4f} x plain); FP64 eigen-solves {fb64}/{nb} blocks")
         if not args.no_stage_timing:
             stages["learned_bf16"] = stage_record(rl, ds, lambda: rl.run_learned(net, r_stop, to_net, from_net, **run_kw),
-                                                  eb=4 if args.storage == "fp16" else 8)
+                                                  eb=4 if args.storage == "fp16" else 8,
+                                                  momentum=W.LEARNED_MOMENTUM > 0)
         other = None
         if not args.no_other_storage:
             alt = "fp32" if args.storage == "fp16" else "fp16"
@@ -376,7 +379,7 @@ def main():
         # the dominant kernel: the largest total time over every single-kernel stage
         cands = [k for k in KERNEL_STAGES if k in st]
         dom = max(cands, key=lambda k: st[k]["total_ms"])
-        nbytes = stage_bytes(ds, eb).get(dom, 0)
+        nbytes = stage_bytes(ds, eb, momentum=bool(learned)).get(dom, 0)
         ach = round(nbytes / (st[dom]["avg_ms"] * 1e-3) / 1e9, 1) if nbytes else None
         roof = {"bound": "hbm", "kernel": KERNEL_STAGES[dom], "stage": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4) if ach else None,
@@ -432,13 +435,13 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def stage_record(r, ds, run=None, eb=8):
+def stage_record(r, ds, run=None, eb=8, momentum=False):
     """Per-stage HIP-event timing of one more run of `r` (library stream)."""
     r.enable_timing(True)
     (run or r.run)()
     st = r.stage_times()
     r.enable_timing(False)
-    nbytes = stage_bytes(ds, eb)
+    nbytes = stage_bytes(ds, eb, momentum)
     rec = {}
     for name, (ms, n) in st.items():
         if n:
