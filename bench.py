@@ -412,8 +412,8 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("bf16 MFMA + fp32, fp16 storage" if learned and args.storage == "fp16" else
-                  "bf16 MFMA + fp32" if learned else "fp32"),
+        "dtype": ("bf16 MFMA Gram, fp16 MFMA projection, fp32, fp16 storage" if learned and args.storage == "fp16"
+                  else "bf16 MFMA + fp32" if learned else "fp32"),
         "data": "synthetic (seeded DTI ellipsoid phantom, variable-density PE mask, complex Gaussian noise)",
         "config": {"workload": (f"LLR recon {shape}, {cfg['blk'][0]}^3 blocks, SVT 0.02*sigma_max, "
                                 f"rank-{cfg['srank']} subspace, db2 wavelet {cfg['wavelet_alpha']}; plain FP32 "

@@ -297,8 +297,10 @@ llr_status llr_dti_scalars_device(llr_ctx* ctx, const llr_cf32* d_volumes, size_
 #define LLR_MODE_SVT 1  /* singular-value soft threshold */
 
 #define LLR_PREC_FP32 0  /* plain form: FP32 throughout */
-#define LLR_PREC_MIXED 1 /* BF16 MFMA Gram and projection (FP32 accumulation), one FP32 Jacobi
-                          refinement sweep, FP64 eigen-solve for blocks with Gram condition > 1e6 */
+#define LLR_PREC_MIXED 1 /* BF16 MFMA Gram and projection (FP32 accumulation; with half storage
+                          the projection runs on FP16 MFMA, the stored halves being its operand),
+                          one FP32 Jacobi refinement sweep, FP64 eigen-solve for blocks with Gram
+                          condition > 1e6 */
 
 #define LLR_STORE_FP32 0 /* complex64 */
 #define LLR_STORE_FP16 1 /* two IEEE half-precision numbers per complex value, FP32 arithmetic in

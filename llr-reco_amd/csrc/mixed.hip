@@ -72,6 +72,32 @@ __device__ __forceinline__ void projection_operand(const float2* __restrict__ S,
   }
 }
 
+// The same operand for FP16 MFMAs: S * 2^8 split into FP16 hi + lo (the scale
+// keeps lo, about 2^-11 of S, in FP16's normal range for |S| >= 2^-8; the
+// product is scaled back exactly by the caller). About 22 bits of S.
+constexpr float P16_SSCALE = 256.f;
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
+template <int K, int MS>
+__device__ __forceinline__ void projection_operand_f16(const float2* __restrict__ S, int nq, int n, int kq,
+                                                       f16x4 (&hop)[MS], f16x4 (&hol)[MS]) {
+  const int k = n >> 1, oc = n & 1;
+  const int kc = k < K ? k : K - 1;
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kk = 16 * s + 4 * kq + j, q = kk >> 1, ic = kk & 1;
+      const float2 sv = S[(q < nq ? q : nq - 1) + (long long)kc * nq];
+      float val = oc == 0 ? (ic == 0 ? sv.x : -sv.y) : (ic == 0 ? sv.y : sv.x);
+      if (q >= nq || k >= K) val = 0.f;
+      val *= P16_SSCALE;
+      const _Float16 h = (_Float16)val;
+      hop[s][j] = h;
+      hol[s][j] = (_Float16)(val - (float)h);
+    }
+  }
+}
+
 // Both projections work on tiles of PB_TV = 64 consecutive voxels (one
 // 16-voxel MFMA row block per wavefront) and move the nq-plane side of the
 // product through LDS, so every global access of a plane is a 512-byte run
@@ -442,13 +468,15 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   }
 }
 
-// Projection C = X*S of an iterate held in half storage. An FP16 value has 11
-// significant bits, so it is exactly the sum of two BF16 numbers hi + lo; the
-// product is taken as hi*S_hi + hi*S_lo + lo*S_hi on BF16 MFMA with FP32
-// accumulation (the dropped lo*S_lo term is below 2^-16 relative), so it adds
-// no rounding beyond the storage itself. Tiles of P16_TV = 128 voxels (two
-// 16-voxel MFMA row blocks per wavefront); plane runs of 512 bytes are staged
-// in LDS as rows [voxel][plane] of 32-bit half pairs.
+// Projection C = X*S of an iterate held in half storage, on FP16 MFMA with FP32
+// accumulation: the stored half pairs are the A operand as they are (no
+// conversion), and S (times 2^8) enters as FP16 hi + lo, two MFMAs per K step,
+// so it adds no rounding beyond the storage itself and S's ~2^-22. (Taking X
+// as the BF16 pair hi + lo instead, hi*S_hi + hi*S_lo + lo*S_hi on BF16 MFMA,
+// cost three MFMAs and about 20 conversion instructions per step: 0.78 ms per
+// launch at 256x256x128x67.) Tiles of P16_TV = 128 voxels (two 16-voxel MFMA
+// row blocks per wavefront); plane runs of 512 bytes are staged in LDS as rows
+// [voxel][plane] of 32-bit half pairs.
 constexpr int P16_TV = 128;
 constexpr int P16_TPW = 8;  // tiles per workgroup (large volumes; at 256x256x128x67, swept over 1-16: 0.638 ms at 8, 0.686 at 4)
 
@@ -461,8 +489,9 @@ __global__ void __launch_bounds__(256) k_project_s16(const ch16* __restrict__ X,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = lane & 15, kq = lane >> 4;
   const int NS = (2 * nq + 15) / 16;
-  bf16x4 bop[MS], bol[MS];
-  projection_operand<K, MS>(S, nq, n, kq, bop, bol);
+  f16x4 hop[MS], hol[MS];
+  projection_operand_f16<K, MS>(S, nq, n, kq, hop, hol);
+  const float oscale = out_scale / P16_SSCALE;
   // tpw consecutive tiles per workgroup; the next tile's words are loaded
   // into registers while the current one is multiplied
   const long long tiles = (N + P16_TV - 1) / P16_TV;
@@ -496,22 +525,14 @@ __global__ void __launch_bounds__(256) k_project_s16(const ch16* __restrict__ X,
 #pragma unroll
       for (int s = 0; s < MS; ++s) {
         if (s < NS) {
-          const uint2 u = *(const uint2*)(row + 8 * s);
-          const float2 x0 = StoreF16::unpack(u.x), x1 = StoreF16::unpack(u.y);
-          const float xv[4] = {x0.x, x0.y, x1.x, x1.y};
-          bf16x4 ah, al;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            ah[j] = bf16_bits(xv[j]);
-            al[j] = bf16_bits(xv[j] - bf16_round(xv[j]));
-          }
-          acc = mfma16(ah, bop[s], acc);
-          acc = mfma16(ah, bol[s], acc);
-          acc = mfma16(al, bop[s], acc);
+          // (Re X_q, Im X_q, Re X_q+1, Im X_q+1): the lane's four K values
+          const f16x4 a = __builtin_bit_cast(f16x4, *(const uint2*)(row + 8 * s));
+          acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, hop[s], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, hol[s], acc, 0, 0, 0);
         }
       }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[j] *= out_scale;
+      for (int j = 0; j < 4; ++j) acc[j] *= oscale;
       const float p0 = __shfl_xor(acc[0], 1, 64), p1 = __shfl_xor(acc[1], 1, 64);
       const float p2 = __shfl_xor(acc[2], 1, 64), p3 = __shfl_xor(acc[3], 1, 64);
       const int k = n >> 1;
