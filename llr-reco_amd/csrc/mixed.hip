@@ -10,15 +10,16 @@
 //    (back-projection). Output columns are interleaved (re, im) too, so a
 //    lane and its xor-1 neighbour hold the two parts of the same complex
 //    value and each writes two voxels with one 16-byte store.
-//  - k_llr_block_mixed: per shifted block, the Gram matrix of the real
+//  - k_llr_block_mixed: per shifted block, the Gram matrix G of the real
 //    m x 2n matrix [Re F | Im F] on BF16 MFMA (both MFMA operands are the
-//    same registers), an FP32 Jacobi eigen-solve of it giving V0, then the
-//    FP32 refinement: H = V0^H G V0 = (F V0)^H (F V0) and one Jacobi sweep on
-//    H. G comes from the same MFMAs with every value split exactly into three
-//    BF16 pieces (Gram to ~2^-24, FP32 accumulation), so F V0 is never formed.
-//    If H's eigenvalues have a condition number above 1e6 the n x n
-//    eigen-solve is redone in FP64 (Gram of F in FP64 and a full FP64
-//    Jacobi). The block is replaced by F W, W = V0 J diag(g) J^H V0^H.
+//    same registers) with every value split exactly into three BF16 pieces
+//    (Gram to ~2^-24, FP32 accumulation), and an FP32 Jacobi eigen-solve of
+//    G giving V and the eigenvalues. If their condition number is above 1e6
+//    the n x n eigen-solve is redone in FP64 (Gram of F in FP64 and a full
+//    FP64 Jacobi). The block is replaced by F W, W = V diag(g) V^H.
+//    (Earlier the Jacobi ran on the leading BF16 term only, followed by an
+//    FP32 refinement sweep on V0^H G V0: the same accuracy, 0.2 ms more per
+//    launch at 256x256x128 with 7 columns.)
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -257,8 +258,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   float* Rp = (float*)(Fs + fsz);                  // 4 x 256 floats: per-wave BF16 Gram partials (leading term)
   float* Rq = Rp + 1024;                           // 4 x 256 floats: the split Gram's remaining terms
   double2* Gp64 = (double2*)Rp;                    // 256 double2 (FP64 fallback, aliases Rp)
-  float2* Vs = (float2*)(Rq + 1024);               // 64: V0
-  float2* Ws = Vs + 64;                            // 64: W = V0 J diag(g) J^H V0^H
+  float2* Ws = (float2*)(Rq + 1024);               // 64: W = J diag(g) J^H
 
   const BlockRows br(A.g, b);
   const long long cs = col_stride(A);
@@ -321,13 +321,13 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
   }
   __syncthreads();
 
-  // 2. wave 0: the complex Gram G = F^H F from the real one: FP32 Jacobi of the
-  //    BF16 Gram -> V0; H = V0^H G V0 from the split Gram; one FP32 Jacobi
-  //    sweep on H -> J, its eigenvalues; condition test
+  // 2. wave 0: the complex Gram G = F^H F from the real one (the split
+  //    Gram's terms summed), its FP32 Jacobi eigen-solve -> eigenvectors jv,
+  //    eigenvalues on hv's diagonal; condition test
   const int r = lane & 7, c = lane >> 3;
-  float2 hv = make_float2(0.f, 0.f), jv = make_float2(0.f, 0.f), v0 = make_float2(0.f, 0.f);
+  float2 hv = make_float2(0.f, 0.f), jv = make_float2(0.f, 0.f);
   if (wave == 0) {
-    float2 a = make_float2(0.f, 0.f), g3 = make_float2(0.f, 0.f);
+    float2 g3 = make_float2(0.f, 0.f);
     if (r < nc && c < nc) {
       float rr = 0.f, ii = 0.f, ri = 0.f, ir = 0.f, rr3 = 0.f, ii3 = 0.f, ri3 = 0.f, ir3 = 0.f;
       for (int w = 0; w < 4; ++w) {
@@ -342,22 +342,10 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
         ri3 += Q[r * 16 + nc + c];
         ir3 += Q[(nc + r) * 16 + c];
       }
-      a = make_float2(rr + ii, r == c ? 0.f : ri - ir);
       g3 = make_float2((rr + rr3) + (ii + ii3), r == c ? 0.f : (ri + ri3) - (ir + ir3));
     }
-    wave_jacobi8<float>(a, v0, lane, 16, 2.5e-7f);
-    Vs[lane] = v0;  // V0[r][c]
-    // T = G V0, then H = V0^H T, made exactly Hermitian
-    float2 t = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) t = cadd(t, cmul(shfl2(g3, r + 8 * j), shfl2(v0, j + 8 * c)));
-    float2 h = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) h = cadd(h, cconjmul(shfl2(v0, i + 8 * r), shfl2(t, i + 8 * c)));
-    const float2 ht = shfl2(h, c + 8 * r);
-    hv = (r < nc && c < nc) ? make_float2(0.5f * (h.x + ht.x), r == c ? 0.f : 0.5f * (h.y - ht.y))
-                            : make_float2(0.f, 0.f);
-    wave_jacobi8<float>(hv, jv, lane, 1, 0.f);
+    wave_jacobi8<float>(g3, jv, lane, 16, 2.5e-7f);
+    hv = g3;
     float lmax = 0.f, lmin = FLT_MAX;
 #pragma unroll
     for (int l = 0; l < 8; ++l) {
@@ -416,7 +404,7 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
     }
   }
 
-  // 4. wave 0: weights, W = V0 J diag(g) J^H V0^H (J diag(g) J^H in the fallback)
+  // 4. wave 0: weights, W = J diag(g) J^H (J the FP32 or the FP64 eigenvectors)
   if (wave == 0) {
     float lam[8];
 #pragma unroll
@@ -435,15 +423,6 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
         float2 t = cmulc(shfl2(jv, r + 8 * l), shfl2(jv, c + 8 * l));
         w.x += g * t.x;
         w.y += g * t.y;
-      }
-      if (!fb) {
-        float2 mm = make_float2(0.f, 0.f);  // (J g J^H) V0^H
-#pragma unroll
-        for (int l = 0; l < 8; ++l) mm = cadd(mm, cmulc(shfl2(w, r + 8 * l), Vs[c + 8 * l]));
-        float2 wv = make_float2(0.f, 0.f);  // V0 (J g J^H V0^H)
-#pragma unroll
-        for (int k = 0; k < 8; ++k) wv = cadd(wv, cmul(Vs[r + 8 * k], shfl2(mm, k + 8 * c)));
-        w = wv;
       }
       Ws[lane] = w;  // W[r][c]
     }
@@ -644,7 +623,7 @@ llr_status backproject_bf16_device(llr_ctx* ctx, const float2* C, float2* X, con
 
 size_t llr_block_mixed_smem(long long m, int nc) {
   size_t fsz = ((size_t)nc * (size_t)(m + 1) + 1) & ~(size_t)1;
-  return (fsz + 512 + 512 + 64 + 64) * sizeof(float2);
+  return (fsz + 512 + 512 + 64) * sizeof(float2);
 }
 
 llr_status llr_block_run_mixed(llr_ctx* ctx, float2* F, const size_t* vol, const size_t* blk, const size_t* shift,
