@@ -156,11 +156,15 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
 // voxels its 27 offsets read, so each input value is fetched from memory about
 // 9 / 3 times per workgroup instead of once per offset and lane, and the
 // chunk's 27 x cout x 16 weights, so the MFMA loop reads only LDS. Rows are
-// CV_ROW = 16 values (32 bytes): an operand read (lanes n = 16 consecutive
-// rows, 8 bytes each at offset 8 kq) spreads evenly over the banks, and each
-// staging store writes a 16-byte half row (two 8-byte stores per half row put
-// 8 lanes on one bank pair; the four level-0 convolutions at 256x256x128 took
-// 4.16 ms that way against 4.01 ms). FP32 inputs (the backward-data gradients) are rounded to
+// CV_ROW = 16 values (32 bytes, four 8-byte pieces). An operand read takes
+// piece kq of 16 consecutive rows; the LDS serves it 16 lanes at a time, and
+// over its 32 banks rows n, n+4, n+8, n+12 would share banks. So row R keeps
+// piece p at slot p ^ key(R), key(R) = (R' >> 2) & 3 with R' the row's
+// position in its run (or block of 16 weight rows): those four rows then use
+// four different slots and the read is conflict-free. (Unswizzled, the
+// counters read 18 stall cycles per LDS instruction; padding rows to 40 bytes
+// removes them too but costs 13 KB of LDS and a resident workgroup.)
+// FP32 inputs (the backward-data gradients) are rounded to
 // BF16 while staged. With a.up set, input channels [0, a.cup) are the nearest
 // upsampling of the half-resolution tensor a.up and channels [a.cup, cin) come
 // from a.in (the decoder's [upsampled, skip] concatenation, never stored).
@@ -279,35 +283,42 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
       const int e = tid + 256 * q;
       if (e >= 9 * CV_RUN * 2) continue;
       const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
-      *(uint4*)(s_run + (size_t)(r * CV_RUN + i) * CV_ROW + 8 * half) = ur[q];
+      const int key = (i >> 2) & 3;  // pieces 2 half, 2 half + 1 -> slots (2 half + {0, 1}) ^ key
+      const uint4 v = (key & 1) ? make_uint4(ur[q].z, ur[q].w, ur[q].x, ur[q].y) : ur[q];
+      *(uint4*)(s_run + (size_t)(r * CV_RUN + i) * CV_ROW + 8 * (half ^ (key >> 1))) = v;
     }
 #pragma unroll
     for (int q = 0; q < NWL; ++q) {
       const int e = tid + 256 * q;
       if (e >= 27 * 16 * NTM * 2) continue;
       const int half = e & 1, col = (e >> 1) % (16 * NTM), o = (e >> 1) / (16 * NTM);
-      *(uint4*)(s_w + (size_t)(o * 16 * NTM + col) * CV_ROW + 8 * half) = uw[q];
+      const int key = (col >> 2) & 3;
+      const uint4 v = (key & 1) ? make_uint4(uw[q].z, uw[q].w, uw[q].x, uw[q].y) : uw[q];
+      *(uint4*)(s_w + (size_t)(o * 16 * NTM + col) * CV_ROW + 8 * (half ^ (key >> 1))) = v;
     }
     __syncthreads();
 #pragma unroll
     for (int o = 0; o < 27; ++o) {
       const int dx = o % 3 - 1, r = o / 3;
+      // the swizzled slot of piece kq: pos = 16 (wave CV_MT + mt) + n + dx + 1, so
+      // (pos >> 2) & 3 = ((n + dx + 1) >> 2) & 3 for every m-tile
+      const int pa = kq ^ (((n + dx + 1) >> 2) & 3);
       bf16x4 av[CV_MT];
 #pragma unroll
       for (int mt = 0; mt < CV_MT; ++mt) {
         const int pos = (wave * CV_MT + mt) * 16 + n + dx + 1;
         if constexpr (ROWS) {
-          av[mt] = lds4(s_run + (size_t)(r * CV_RUN + pos) * CV_ROW + 4 * kq);
+          av[mt] = lds4(s_run + (size_t)(r * CV_RUN + pos) * CV_ROW + 4 * pa);
         } else {
           const bool valid = (vmask[mt] >> o) & 1u;
           av[mt] = bf16x4{0, 0, 0, 0};
-          if (valid) av[mt] = lds4(s_run + (size_t)(r * CV_RUN + pos) * CV_ROW + 4 * kq);
+          if (valid) av[mt] = lds4(s_run + (size_t)(r * CV_RUN + pos) * CV_ROW + 4 * pa);
         }
       }
       const bf16_t* wo = s_w + (size_t)o * 16 * NTM * CV_ROW;
 #pragma unroll
       for (int t = 0; t < NTM; ++t) {
-        const bf16x4 bv = lds4(wo + (size_t)(16 * t + n) * CV_ROW + 4 * kq);
+        const bf16x4 bv = lds4(wo + (size_t)(16 * t + n) * CV_ROW + 4 * (kq ^ ((n >> 2) & 3)));
 #pragma unroll
         for (int mt = 0; mt < CV_MT; ++mt) acc[mt][t] = mfma16(av[mt], bv, acc[mt][t]);
       }
