@@ -108,7 +108,7 @@ KERNEL_STAGES = {
     "fft_z_project": "k_fft_z_project",
     "fft_y_adjoint_project": "k_fft_y_adjoint_project",
     "wavelet_level0_in": "k_wav_xy_an + k_wav_zan_k on the K planes / k_wav_xy_an (level 0)",
-    "wavelet_level0_mid": "k_wav_zsyn_c / k_wav_z (level 0)",
+    "wavelet_level0_mid": "k_wav_zsyn_m / k_wav_z (level 0)",
     "wavelet_level0_out": "k_wav_xy_syn (level 0)",
 }
 

@@ -409,39 +409,13 @@ struct ZSynArgs {
   const void* Ar;    // reconstructed next level per volume (ST; ncx x ncy x ncz, row pitch pa), null at the deepest level
   long long arvp;
   int pa;
-  void* B;           // per-volume xy-bands (ST), box bx x by x az, volume pitch bvp
-  long long bvp;
+  void* B;           // per-volume xy-bands (ST), box bx x by x az, volume pitch bvp; rows ordered (Y, z):
+  long long bvp;     // element (X, Y, z) at (Y az + z) bx + X
   long long bx, by;
   int az, ncz, ncx, ncy, px, deepest;
-  int zmid;          // B rows ordered (Y, z) instead of (z, Y): element (X, Y, z) at (Y az + z) bx + X
-  int vpg;           // volumes per workgroup: grid.y walks the volume groups
   float alpha;
   Db2F f;
 };
-
-// The coefficient pair (lo, hi) of slot c for volume v: expanded from the K
-// planes and shrunk, except the LL band's approximation above the deepest
-// level, which is the reconstructed next level as it is.
-// sum_k w[k] conj(sc[k]): four FMAs per term, or two when every sc[k] is
-// real (RS: the imaginary parts are zero and not read)
-template <int KK, bool RS = false>
-__device__ __forceinline__ float2 expand(const float2 (&w)[KK], const float2 (&sc)[KK]) {
-  float re = w[0].x * sc[0].x, im = w[0].y * sc[0].x;
-  if (!RS) {
-    re = fmaf(w[0].y, sc[0].y, re);
-    im = fmaf(-w[0].x, sc[0].y, im);
-  }
-#pragma unroll
-  for (int k = 1; k < KK; ++k) {
-    re = fmaf(w[k].x, sc[k].x, re);
-    im = fmaf(w[k].y, sc[k].x, im);
-    if (!RS) {
-      re = fmaf(w[k].y, sc[k].y, re);
-      im = fmaf(-w[k].x, sc[k].y, im);
-    }
-  }
-  return make_float2(re, im);
-}
 
 // out[s] from the slots floor(s/2) (lo0, hi0) and floor(s/2) + 1 (lo1, hi1):
 // taps 2 and 0 for even s, 3 and 1 for odd s (kernels.cu:230-261)
@@ -454,165 +428,184 @@ __device__ __forceinline__ float2 zsyn_tap(const Db2F& f, bool odd, float2 lo0, 
   return acc;
 }
 
-// Any z length: thread = (column X, sample pair m) holding the K-plane
-// coefficients of slots m and m + 1 for every volume in turn (the slot shared
-// by two pairs is expanded by both). block = 32 columns x 8 pairs; grid =
-// (ceil(bx / 32), by, ceil(ceil(az / 2) / 8)).
-template <int KK, bool RS, class ST>
-__global__ void __launch_bounds__(256) k_wav_zsyn_k(ZSynArgs a) {
-  using E = typename ST::T;
-  const int pgs = ((a.az + 1) / 2 + 7) / 8;
-  const int X = blockIdx.x * 32 + (int)(threadIdx.x & 31);
-  const int Y = blockIdx.z / pgs;
-  const int m = (blockIdx.z % pgs) * 8 + (int)(threadIdx.x >> 5);
-  if (X >= a.bx || m >= (a.az + 1) / 2) return;
-  if ((X < a.px ? X : X - a.px) >= a.ncx) return;
-  const bool lo_ar = X < a.ncx && Y < a.ncy && !a.deepest;
-  const long long zp = a.bx * a.by;
-  const long long xy = (long long)Y * a.bx + X;
-  float2 wl[2][KK], wh[2][KK];
-#pragma unroll
-  for (int j = 0; j < 2; ++j)
-#pragma unroll
-    for (int k = 0; k < KK; ++k) {
-      const bool on = k < a.K;
-      wl[j][k] = (on && !lo_ar) ? a.WK[k * a.wvp + (long long)(m + j) * zp + xy] : make_float2(0.f, 0.f);
-      wh[j][k] = on ? a.WK[k * a.wvp + (long long)(a.ncz + m + j) * zp + xy] : make_float2(0.f, 0.f);
-    }
-  const E* ar = lo_ar ? (const E*)a.Ar + ((long long)m * a.ncy + Y) * a.pa + X : nullptr;
-  const long long arz = (long long)a.pa * a.ncy;
-  const long long bz = a.zmid ? a.bx : zp;  // z step of B
-  E* out = (E*)a.B + (a.zmid ? (long long)Y * a.az * a.bx + X : xy) + (long long)(2 * m) * bz;
-  const bool two = 2 * m + 1 < a.az;
-  const int vend = min(a.nq, (int)blockIdx.y * a.vpg + a.vpg);
-  for (int v = blockIdx.y * a.vpg; v < vend; ++v) {
-    float2 sc[KK];
-#pragma unroll
-    for (int k = 0; k < KK; ++k) sc[k] = k < a.K ? a.S[v + (long long)k * a.nq] : make_float2(0.f, 0.f);
-    float2 lo[2], hi[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      hi[j] = shrink(expand<KK, RS>(wh[j], sc), a.alpha);
-      lo[j] = lo_ar ? ST::cvt(ar[v * a.arvp + j * arz]) : shrink(expand<KK, RS>(wl[j], sc), a.alpha);
-    }
-    E* o = out + v * a.bvp;
-    ST::st(o, zsyn_tap(a.f, false, lo[0], hi[0], lo[1], hi[1]));
-    if (two) ST::st(o + bz, zsyn_tap(a.f, true, lo[0], hi[0], lo[1], hi[1]));
-  }
-}
+constexpr int ZS_KM = 8;  // coefficient planes (K <= 8)
 
-// z length az <= 4 NCH: thread (t = column, r = chunk) owns the samples
-// [4r, min(4r + 4, az)) and the slots 2r, 2r + 1 (always < ncz), holding their
-// K-plane coefficients while it walks the volumes in groups of G; each slot is
-// expanded once per volume. Samples 4r + 2, 4r + 3 also need slot 2r + 2: the
-// next chunk's first slot, passed through LDS for the G volumes of a group at
-// once (double-buffered: one barrier per group), or, for the last chunk when
-// that slot exists (2r + 2 <= ncz - 1), its own expansion of K-plane
-// coefficients kept in LDS. Rows of ZTL columns are 128 B of complex64.
-template <int ZTL, int NCH, int KK, int G, bool RS, class ST>
-__global__ void __launch_bounds__(ZTL * NCH, 4) k_wav_zsyn_c(ZSynArgs a, const float2* __restrict__ S) {
+// Per volume q, slot c of a band position holds (lo, hi) = shrink(sum_k
+// W_k conj(S[q][k])) of the K planes' z coefficients W_k, except the LL band's
+// z-approximation above the deepest level, which is the reconstructed next
+// level as it is; z sample s is then the synthesis of slots floor(s/2) and
+// floor(s/2) + 1. The expansion runs on the matrix cores: for one band slot
+// of 16 columns, the expansion into 16 volumes is a real matrix product:
+// E[q][X] = sum_k S[q][k] W_k[X] for each of Re W and Im W when S is real; a
+// complex S takes four more k-steps, E_re = sum_k (W.x S.x + W.y S.y), E_im =
+// sum_k (W.y S.x - W.x S.y). v_mfma_f32_16x16x4_f32 takes A (16 x 4) with
+// lane l holding A[l % 16][l / 16], B (4 x 16) with lane l holding
+// B[l / 16][l % 16], and leaves D[4 (l / 16) + i][l % 16] in lane l, i < 4.
+// A wave owns 16 columns (lane column X0 + l % 16) of one band row Y and the
+// samples [2 c0, 2 c0 + 2 CS) of z: slots c0 .. c0 + CS. It holds those slots'
+// K-plane values as B operands (planes l / 16 and l / 16 + 4, padded to 8
+// with zeros) and walks the volumes 16 at a time (A: the S rows of 16
+// volumes). Each lane then has its column's (lo, hi) pair of every slot for
+// volumes q0 + 4 (l / 16) + i, shrinks them and synthesises its z samples slot
+// after slot: no value leaves the lane. Workgroup: 4 waves, 2 column tiles x 2
+// z chunks; grid = (column-tile pairs x by x chunk pairs, volume ranges), the
+// column-tile pairs fastest when TF is set. NT: non-temporal stores.
+constexpr int ZM_CS = 8;  // slots per z chunk (2 ZM_CS samples)
+
+typedef float zm_v4f __attribute__((ext_vector_type(4)));
+
+template <int CS, bool RS, class ST, bool NT, bool TF>
+__global__ void __launch_bounds__(256, 3) k_wav_zsyn_m(ZSynArgs a, const float2* __restrict__ S, int nchunk, int qper) {
   using E = typename ST::T;
-  __shared__ float2 xch[2][G][2][NCH][ZTL];  // (lo, hi) of each chunk's first slot, G volumes
-  __shared__ float2 lastw[2][KK][ZTL];       // K-plane (lo, hi) of slot 2r + 2 of the last chunk
-  const int tid = threadIdx.x;
-  const int t = tid % ZTL, r = tid / ZTL;
-  const int tiles = (int)((a.bx + ZTL - 1) / ZTL);
-  const int Y = blockIdx.x / tiles;
-  const int X = (blockIdx.x % tiles) * ZTL + t;
-  const int nch = (a.az + 3) / 4;
-  const bool act = X < a.bx && (X < a.px ? X : X - a.px) < a.ncx && r < nch;
-  const bool last = r == nch - 1;
-  const bool extra = act && last && 2 * r + 2 <= a.ncz - 1;
-  const int ns = act ? min(4, a.az - 4 * r) : 0;
-  const bool lo_ar = X < a.ncx && Y < a.ncy && !a.deepest;
-  const long long zp = a.bx * a.by;
-  const long long xy = (long long)Y * a.bx + X;
-  // planes K..KK-1 (KK = 8 for K < 8) hold zeros; their S column is clamped to K - 1
-  const float2* wk = a.WK + xy;
-  float2 wl[2][KK], wh[2][KK];
-#pragma unroll
-  for (int j = 0; j < 2; ++j)
-#pragma unroll
-    for (int k = 0; k < KK; ++k) {
-      const bool on = act && k < a.K;
-      wl[j][k] = (on && !lo_ar) ? wk[k * a.wvp + (long long)(2 * r + j) * zp] : make_float2(0.f, 0.f);
-      wh[j][k] = on ? wk[k * a.wvp + (long long)(a.ncz + 2 * r + j) * zp] : make_float2(0.f, 0.f);
-    }
-  if (extra) {
-#pragma unroll
-    for (int k = 0; k < KK; ++k) {
-      const bool on = k < a.K;
-      lastw[0][k][t] = (on && !lo_ar) ? wk[k * a.wvp + (long long)(2 * r + 2) * zp] : make_float2(0.f, 0.f);
-      lastw[1][k][t] = on ? wk[k * a.wvp + (long long)(a.ncz + 2 * r + 2) * zp] : make_float2(0.f, 0.f);
-    }
+  constexpr int NS = RS ? 2 : 4;  // k-steps of 4 planes
+  // the wave index is uniform: as a scalar, every slot and sample offset is too
+  const int lane = (int)threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int jl = lane & 15, kq = lane >> 4;
+  const int nchunk2 = (nchunk + 1) >> 1;
+  int b = (int)blockIdx.x;
+  int cp, Y, tp;
+  if (TF) {  // column-tile pairs fastest
+    const int ntp = (int)((a.bx + 31) / 32);
+    tp = b % ntp;
+    b /= ntp;
+    cp = b % nchunk2;
+    Y = b / nchunk2;
+  } else {
+    cp = b % nchunk2;
+    b /= nchunk2;
+    Y = b % (int)a.by;
+    tp = b / (int)a.by;
   }
+  const int X0 = (2 * tp + (wv & 1)) * 16;
+  const int ch = 2 * cp + (wv >> 1);
+  if (X0 >= a.bx || ch >= nchunk) return;  // the whole wave (bx is a multiple of 16)
+  const int X = X0 + jl;
+  const bool act = (X < a.px ? X : X - a.px) < a.ncx;
+  // the LL band's z-approximation is the reconstructed next level; px is a
+  // multiple of 16, so this holds for every active lane of the tile or none
+  const bool lo_ar = X0 < a.ncx && Y < a.ncy && !a.deepest;
+  const int c0 = ch * CS;
+  const long long zp = a.bx * a.by;
+  const float2* wk = a.WK + (long long)Y * a.bx + X;
+  float2 wl[CS + 1][2], wh[CS + 1][2];
+#pragma unroll
+  for (int t = 0; t <= CS; ++t)
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int kk = kq + 4 * s, c = c0 + t;
+      const bool on = act && kk < a.K && c < a.ncz;
+      wl[t][s] = (on && !lo_ar) ? wk[kk * a.wvp + (long long)c * zp] : make_float2(0.f, 0.f);
+      wh[t][s] = on ? wk[kk * a.wvp + (long long)(a.ncz + c) * zp] : make_float2(0.f, 0.f);
+    }
+  const int qbeg = (int)blockIdx.y * qper, qend = min(a.nq, qbeg + qper);
   const long long arz = (long long)a.pa * a.ncy;
-  const E* ar = (act && lo_ar) ? (const E*)a.Ar + ((long long)(2 * r) * a.ncy + Y) * a.pa + X : nullptr;
-  const long long bz = a.zmid ? a.bx : zp;  // z step of B
-  E* out = (E*)a.B + (a.zmid ? (long long)Y * a.az * a.bx + X : xy) + (long long)(4 * r) * bz;
-  const int kmax = a.K - 1;
-  const int vbeg = blockIdx.y * a.vpg, vend = min(a.nq, vbeg + a.vpg);
-  for (int v0 = vbeg; v0 < vend; v0 += G) {
-    const int b = ((v0 - vbeg) / G) & 1;
-    E an[G][3];
-    if (ar) {
+  // the next level's rows are pa >= px > X long, so an inactive lane's read
+  // stays inside its row
+  const E* arb = lo_ar ? (const E*)a.Ar + (long long)Y * a.pa + X : nullptr;
+  E* ob = (E*)a.B + (long long)Y * a.az * a.bx + X;
+  // Loads are issued a step ahead of their use: the memory counter a wave waits
+  // on is shared by loads and stores and completes in order, so a load issued
+  // after a slot's stores would also wait for those stores to be written.
+  // S rows of a volume group (the A operand), one group ahead:
+  auto load_s = [&](int q0, float2(&sv)[2]) {
+    const int qa = q0 + jl;
 #pragma unroll
-      for (int g = 0; g < G; ++g) {
-        const E* p = ar + (long long)min(v0 + g, vend - 1) * a.arvp;
-        an[g][0] = p[0];
-        an[g][1] = p[arz];
-        an[g][2] = extra ? p[2 * arz] : ST::zero();
+    for (int s = 0; s < 2; ++s) {
+      const int kk = kq + 4 * s;
+      sv[s] = (qa < qend && kk < a.K) ? S[qa + (long long)kk * a.nq] : make_float2(0.f, 0.f);
+    }
+  };
+  // next-level values of slot c for this lane's volumes q0 + 4 kq + i, one slot
+  // ahead (volumes past qend, only in the last group, read clamped and not stored)
+  auto load_ar = [&](int q0, int c, E(&an)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int q = min(q0 + 4 * kq + i, qend - 1);
+      an[i] = c < a.ncz ? arb[(long long)q * a.arvp + c * arz] : ST::zero();
+    }
+  };
+  float2 svn[2];
+  load_s(qbeg, svn);
+  E ann[4];
+  if (lo_ar) load_ar(qbeg, c0, ann);
+  for (int q0 = qbeg; q0 < qend; q0 += 16) {
+    float sa[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) sa[s] = s < 2 ? svn[s].x : svn[s - 2].y;
+    const int qn = q0 + 16;
+    if (qn < qend) load_s(qn, svn);
+    const int qi = q0 + 4 * kq;
+    bool qok[4];
+    E* oq = ob + (long long)qi * a.bvp;
+    // per group, not hoisted: the slot offsets below are then scalars added here
+    asm volatile("" : "+v"(oq));
+#pragma unroll
+    for (int i = 0; i < 4; ++i) qok[i] = act && qi + i < qend;
+    E an[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) an[i] = ann[i];
+    float2 plo[4], phi[4];
+#pragma unroll
+    for (int t = 0; t <= CS; ++t) {
+      const int c = c0 + t;
+      // keeps the slots in order (the compiler would otherwise start every
+      // slot's products at once and hold all their results)
+#pragma unroll
+      for (int s = 0; s < NS; ++s) asm volatile("" : "+v"(sa[s])::"memory");
+      E anx[4];
+      if (lo_ar) {
+        if (t < CS)
+          load_ar(q0, c + 1, anx);
+        else if (qn < qend)
+          load_ar(qn, c0, anx);
       }
-    }
-    float2 lo1[G], hi1[G];
+      zm_v4f hr = {0.f, 0.f, 0.f, 0.f}, hm = hr, lr = hr, lm = hr;
 #pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int v = min(v0 + g, vend - 1);
-      float2 sc[KK];
+      for (int s = 0; s < NS; ++s) {
+        const float2 w = wh[t][s & 1];
+        hr = __builtin_amdgcn_mfma_f32_16x16x4f32(sa[s], s < 2 ? w.x : w.y, hr, 0, 0, 0);
+        hm = __builtin_amdgcn_mfma_f32_16x16x4f32(sa[s], s < 2 ? w.y : -w.x, hm, 0, 0, 0);
+      }
+      if (!lo_ar) {
 #pragma unroll
-      for (int k = 0; k < KK; ++k) sc[k] = S[v + (long long)min(k, kmax) * a.nq];
-      const float2 hi0 = shrink(expand<KK, RS>(wh[0], sc), a.alpha);
-      hi1[g] = shrink(expand<KK, RS>(wh[1], sc), a.alpha);
-      const float2 lo0 = ar ? ST::cvt(an[g][0]) : shrink(expand<KK, RS>(wl[0], sc), a.alpha);
-      lo1[g] = ar ? ST::cvt(an[g][1]) : shrink(expand<KK, RS>(wl[1], sc), a.alpha);
-      xch[b][g][0][r][t] = lo0;
-      xch[b][g][1][r][t] = hi0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int v = v0 + g;
-      if (v >= vend || !act) continue;
-      const float2 lo0 = xch[b][g][0][r][t], hi0 = xch[b][g][1][r][t];
-      float2 lo2 = make_float2(0.f, 0.f), hi2 = make_float2(0.f, 0.f);
-      if (!last) {  // r + 1 < nch <= NCH
-        lo2 = xch[b][g][0][r + 1][t];
-        hi2 = xch[b][g][1][r + 1][t];
-      } else if (extra) {
-        float2 sc[KK], w[KK];
-#pragma unroll
-        for (int k = 0; k < KK; ++k) sc[k] = S[v + (long long)min(k, kmax) * a.nq];
-#pragma unroll
-        for (int k = 0; k < KK; ++k) w[k] = lastw[1][k][t];
-        hi2 = shrink(expand<KK, RS>(w, sc), a.alpha);
-        if (ar) {
-          lo2 = ST::cvt(an[g][2]);
-        } else {
-#pragma unroll
-          for (int k = 0; k < KK; ++k) w[k] = lastw[0][k][t];
-          lo2 = shrink(expand<KK, RS>(w, sc), a.alpha);
+        for (int s = 0; s < NS; ++s) {
+          const float2 w = wl[t][s & 1];
+          lr = __builtin_amdgcn_mfma_f32_16x16x4f32(sa[s], s < 2 ? w.x : w.y, lr, 0, 0, 0);
+          lm = __builtin_amdgcn_mfma_f32_16x16x4f32(sa[s], s < 2 ? w.y : -w.x, lm, 0, 0, 0);
         }
       }
-      E* o = out + v * a.bvp;
-      ST::st(o, zsyn_tap(a.f, false, lo0, hi0, lo1[g], hi1[g]));
-      if (ns > 1) ST::st(o + bz, zsyn_tap(a.f, true, lo0, hi0, lo1[g], hi1[g]));
-      if (ns > 2) ST::st(o + 2 * bz, zsyn_tap(a.f, false, lo1[g], hi1[g], lo2, hi2));
-      if (ns > 3) ST::st(o + 3 * bz, zsyn_tap(a.f, true, lo1[g], hi1[g], lo2, hi2));
+      float2 lo[4], hi[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        hi[i] = shrink(make_float2(hr[i], hm[i]), a.alpha);
+        lo[i] = lo_ar ? ST::cvt(an[i]) : shrink(make_float2(lr[i], lm[i]), a.alpha);
+      }
+      if (t > 0) {
+        const int z = 2 * (c - 1);  // samples z, z + 1 from slots c - 1, c
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (!qok[i]) continue;
+          E* o = oq + (i * a.bvp + (long long)z * a.bx);
+          if (NT) {
+            if (z < a.az) ST::stnt(o, zsyn_tap(a.f, false, plo[i], phi[i], lo[i], hi[i]));
+            if (z + 1 < a.az) ST::stnt(o + a.bx, zsyn_tap(a.f, true, plo[i], phi[i], lo[i], hi[i]));
+          } else {
+            if (z < a.az) ST::st(o, zsyn_tap(a.f, false, plo[i], phi[i], lo[i], hi[i]));
+            if (z + 1 < a.az) ST::st(o + a.bx, zsyn_tap(a.f, true, plo[i], phi[i], lo[i], hi[i]));
+          }
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        plo[i] = lo[i];
+        phi[i] = hi[i];
+        an[i] = anx[i];
+      }
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ann[i] = an[i];
   }
 }
-
-constexpr int ZS_KM = 8;  // coefficient planes held per thread (K <= 8)
 
 struct XYSynArgs {
   const void* B;
@@ -1136,60 +1129,29 @@ static llr_status wavelet_run_subspace(llr_ctx* ctx, void* vols_v, size_t nvols,
     a.ncy = q.ncy;
     a.px = (int)q.px;
     a.deepest = deepest;
-    a.zmid = 1;
     a.alpha = alpha;
     a.f = f;
-    // volume groups so that a launch has about 2048 workgroups or more (the deep
-    // levels have few columns; each workgroup walks its volumes in turn)
-    auto vgroups = [&](size_t wgs) {
-      size_t vg = (2048 + wgs - 1) / wgs;
-      vg = vg < 1 ? 1 : (vg > nvols ? nvols : vg);
-      a.vpg = (int)((nvols + vg - 1) / vg);
-      return (unsigned)((nvols + a.vpg - 1) / a.vpg);
-    };
     {
       KernelTimer kt(ctx, l == 0 ? kStageWavMid : kStageWavDeep);
-      const int nch = (az + 3) / 4;
-      if (nch <= 32) {
-        constexpr int ZTL = 16, GV = 2;
-        const unsigned tiles = (unsigned)(((q.bx + ZTL - 1) / ZTL) * q.by);
-        const dim3 grid(tiles, vgroups(tiles));
-#define ZSC_LAUNCH(NN)                                                                                   \
-  if (sub.K == 7 && sub.s_real)                                                                          \
-    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, 7, GV, true, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);   \
-  else if (sub.K == 7)                                                                                   \
-    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, 7, GV, false, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);  \
-  else                                                                                                   \
-    hipLaunchKernelGGL((k_wav_zsyn_c<ZTL, NN, ZS_KM, GV, false, ST>), grid, dim3(ZTL * NN), 0, s, a, sub.S);
-        // chunks per column rounded up to a workgroup shape: few idle chunks at
-        // the deep levels' odd z lengths (level 1 of a 128-long z axis has 65
-        // samples, 17 chunks)
-        if (nch <= 4) {
-          ZSC_LAUNCH(4)
-        } else if (nch <= 8) {
-          ZSC_LAUNCH(8)
-        } else if (nch <= 12) {
-          ZSC_LAUNCH(12)
-        } else if (nch <= 16) {
-          ZSC_LAUNCH(16)
-        } else if (nch <= 20) {
-          ZSC_LAUNCH(20)
-        } else if (nch <= 24) {
-          ZSC_LAUNCH(24)
-        } else {
-          ZSC_LAUNCH(32)
-        }
-#undef ZSC_LAUNCH
-      } else {
-        const unsigned tx = (unsigned)((q.bx + 31) / 32), tz = (unsigned)(q.by * (((az + 1) / 2 + 7) / 8));
-        dim3 g(tx, vgroups((size_t)tx * tz), tz);
-        if (sub.K == 7 && sub.s_real)
-          hipLaunchKernelGGL((k_wav_zsyn_k<7, true, ST>), g, dim3(256), 0, s, a);
-        else if (sub.K == 7)
-          hipLaunchKernelGGL((k_wav_zsyn_k<7, false, ST>), g, dim3(256), 0, s, a);
-        else
-          hipLaunchKernelGGL((k_wav_zsyn_k<ZS_KM, false, ST>), g, dim3(256), 0, s, a);
-      }
+      const int nchunk = (az + 2 * ZM_CS - 1) / (2 * ZM_CS);
+      const unsigned gx = (unsigned)(((q.bx + 31) / 32) * q.by * ((nchunk + 1) / 2));
+      // volume ranges (whole groups of 16) so that a launch has about 8192
+      // waves or more: the deep levels have few columns
+      const int ngrp = (int)((nvols + 15) / 16);
+      int nr = (int)((8192 + 4 * (size_t)gx - 1) / (4 * (size_t)gx));
+      nr = nr < 1 ? 1 : (nr > ngrp ? ngrp : nr);
+      const int qper = (ngrp + nr - 1) / nr * 16;
+      const dim3 grid(gx, (unsigned)((nvols + qper - 1) / qper));
+      // measured at 256x256x128x67's level 0: half storage 1.10 ms with the
+      // workgroups of one column pair consecutive, 1.03 ms with column pairs
+      // fastest (stores 1.20 ms non-temporal); complex64 1.51 ms, 1.40 ms with
+      // non-temporal stores (also 1.85 -> 1.77 ms for the xy synthesis that
+      // reads them), 1.69 ms with column pairs fastest
+      constexpr bool h16 = ST::bytes == 4;
+      if (sub.s_real)
+        hipLaunchKernelGGL((k_wav_zsyn_m<ZM_CS, true, ST, !h16, h16>), grid, dim3(256), 0, s, a, sub.S, nchunk, qper);
+      else
+        hipLaunchKernelGGL((k_wav_zsyn_m<ZM_CS, false, ST, !h16, h16>), grid, dim3(256), 0, s, a, sub.S, nchunk, qper);
     }
     LLR_HIP(hipGetLastError());
     {

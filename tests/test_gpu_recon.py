@@ -128,8 +128,9 @@ def test_recon_residual_decreases(llr):
                                      ((48, 24, 6), (6, 6, 6)), ((24, 48, 12), (8, 8, 8))])
 def test_recon_partial_edge_blocks_radix3(llr, vol, blk):
     """Partial edge blocks on volumes with radix-3 FFT lengths (the uneven batch sizes).
-    z length 6 (not a multiple of 4) takes the subspace wavelet's per-pair z
-    synthesis at level 0; 12, 24 and 32 its 4-sample chunks (3, 6, 8 of them)."""
+    The subspace wavelet's z synthesis works in chunks of 16 samples: z lengths 6
+    and 12 are one partial chunk at level 0, 24 a whole and a partial one, 32 two
+    whole ones."""
     ds = H.make(dict(H.SMALL_SUBSPACE, vol=vol, blk=blk, n_iter=4))
     check(ds, f"{vol} blk {blk} partial", partial_blocks=True)
 
