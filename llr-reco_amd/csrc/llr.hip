@@ -316,9 +316,25 @@ __device__ void llr_block_body(const BlockArgs& A, long long slot, float2* smem)
   const BlockRows br(A.g, b);
   const long long cs = col_stride(A);
   if (A.stage_lds) {
-    for (int i = tid; i < m; i += 256) {
-      const long long base = row_base(A, br, b, i);
-      for (int col = 0; col < nc; ++col) Fs[col * mp + i] = base < 0 ? make_float2(0.f, 0.f) : A.F[base + col * cs];
+    // two rows per thread at a time, every load issued before the LDS stores
+    for (int i0 = tid; i0 < m; i0 += 512) {
+      float2 v[2][8];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = i0 + 256 * h;
+        const long long base = i < m ? row_base(A, br, b, i) : -1;
+#pragma unroll
+        for (int col = 0; col < 8; ++col)
+          v[h][col] = (col < nc && base >= 0) ? A.F[base + col * cs] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = i0 + 256 * h;
+        if (i >= m) continue;
+#pragma unroll
+        for (int col = 0; col < 8; ++col)
+          if (col < nc) Fs[col * mp + i] = v[h][col];
+      }
     }
     __syncthreads();
   }

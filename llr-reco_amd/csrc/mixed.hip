@@ -262,20 +262,41 @@ __global__ void __launch_bounds__(256) k_llr_block_mixed(BlockArgs A) {
 
   const BlockRows br(A.g, b);
   const long long cs = col_stride(A);
-  for (int i = tid; i < m; i += 256) {
-    const long long base = row_base(A, br, b, i);
-    for (int col = 0; col < nc; ++col) {
-      float2 v = make_float2(0.f, 0.f);
-      if (base >= 0) {
-        const long long e = base + col * cs;
-        v = A.F[e];
-        if (A.prev) {  // momentum: every coefficient belongs to exactly one block
-          const float2 q = A.mom_first ? v : A.prev[e];
-          A.prev[e] = v;
-          v = make_float2((1.f + A.mom) * v.x - A.mom * q.x, (1.f + A.mom) * v.y - A.mom * q.y);
+  // two rows per thread at a time, every load issued before any store (a
+  // store between them would keep each load's latency in turn)
+  const bool prev_in = A.prev && !A.mom_first;
+  for (int i0 = tid; i0 < m; i0 += 512) {
+    long long base[2];
+    float2 v[2][8], pv[2][8];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = i0 + 256 * h;
+      base[h] = i < m ? row_base(A, br, b, i) : -1;
+#pragma unroll
+      for (int col = 0; col < 8; ++col) {
+        v[h][col] = make_float2(0.f, 0.f);
+        pv[h][col] = make_float2(0.f, 0.f);
+        if (col < nc && base[h] >= 0) {
+          v[h][col] = A.F[base[h] + col * cs];
+          if (prev_in) pv[h][col] = A.prev[base[h] + col * cs];
         }
       }
-      Fs[col * mp + i] = v;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = i0 + 256 * h;
+      if (i >= m) continue;
+#pragma unroll
+      for (int col = 0; col < 8; ++col) {
+        if (col >= nc) continue;
+        float2 x = v[h][col];
+        if (A.prev && base[h] >= 0) {  // momentum: every coefficient belongs to exactly one block
+          const float2 q = A.mom_first ? x : pv[h][col];
+          A.prev[base[h] + col * cs] = x;
+          x = make_float2((1.f + A.mom) * x.x - A.mom * q.x, (1.f + A.mom) * x.y - A.mom * q.y);
+        }
+        Fs[col * mp + i] = x;
+      }
     }
   }
   __syncthreads();
