@@ -226,38 +226,76 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
   // ROWS: the run's row (x0 .. x0 + CV_M - 1, y0, z0), found once
   const int yz0 = ROWS ? vb / a.W : 0;
   const int x0 = vb - yz0 * a.W, y0 = ROWS ? yz0 % a.H : 0, z0 = ROWS ? yz0 / a.H : 0;
+  // ROWS staging: thread t takes run position t >> 1 (channel half t & 1) of
+  // each of the nine rows, so a row's (y, z) part of the address and its
+  // inside test are uniform; positions CV_M, CV_M + 1 of the nine rows (36
+  // values) go to threads 0..35 as load NRL - 1
+  static_assert(!ROWS || (CV_M == 128 && NRL == 10), "ROWS staging: 256 threads over 128 positions, 9 + 1 loads");
+  auto stage_rows = [&](int q, int& r, int& i, int& half) {
+    if (q < 9) {
+      r = q;
+      i = tid >> 1;
+      half = tid & 1;
+    } else {
+      r = tid >> 2;
+      i = CV_M + ((tid >> 1) & 1);
+      half = tid & 1;
+    }
+  };
   for (int c0 = 0; c0 < a.cin; c0 += 16) {
     const bool from_up = a.up != nullptr && c0 < a.cup;
     uint4 ur[NRL];
+    // branch-free: every load from an address clamped into the tensor and
+    // zeroed afterwards when its neighbour is outside (a load under a branch
+    // was waited for at once, so the chunk's loads ran one at a time)
+    bool rin[NRL];
+    if constexpr (ROWS) {
 #pragma unroll
-    for (int q = 0; q < NRL; ++q) {
-      const int e = tid + 256 * q;
-      ur[q] = make_uint4(0u, 0u, 0u, 0u);
-      if (e >= 9 * CV_RUN * 2) continue;
-      const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
-      // neighbour (xx, yy, zz) of the staged value, or its flat index
-      const int xx = x0 - 1 + i, yy = y0 + r % 3 - 1, zz = z0 + r / 3 - 1;
-      const int idx = ROWS ? (zz * a.H + yy) * a.W + xx : vb - 1 + i + (r / 3 - 1) * HW + (r % 3 - 1) * a.W;
-      const bool inside = ROWS ? (xx >= 0 && xx < a.W && yy >= 0 && yy < a.H && zz >= 0 && zz < a.D)
-                               : (idx >= 0 && idx < V);
-      uint4& u = ur[q];
-      if (inside) {
+      for (int q = 0; q < NRL; ++q) {
+        int r, i, half;
+        stage_rows(q, r, i, half);
+        r = min(r, 8);
+        const int xx = x0 - 1 + i, yy = y0 + r % 3 - 1, zz = z0 + r / 3 - 1;
+        rin[q] = (q < NRL - 1 || tid < 36) && xx >= 0 && xx < a.W && yy >= 0 && yy < a.H && zz >= 0 && zz < a.D;
+        const int xc = min(max(xx, 0), a.W - 1), yc = min(max(yy, 0), a.H - 1), zc = min(max(zz, 0), a.D - 1);
         if constexpr (sizeof(TIn) == 2) {
-          if (from_up) {
-            int x = xx, y = yy, z = zz;
-            if constexpr (!ROWS) {
-              x = idx % a.W;
-              y = (idx / a.W) % a.H;
-              z = idx / HW;
-            }
-            const int ci = ((z >> 1) * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
-            u = *(const uint4*)(a.up + (size_t)ci * a.cup + c0 + 8 * half);
-          } else {
-            u = *(const uint4*)((const bf16_t*)a.in + (size_t)idx * cin2 + (c0 - a.cup) + 8 * half);
-          }
+          const int ci = ((zc >> 1) * (a.H >> 1) + (yc >> 1)) * (a.W >> 1) + (xc >> 1);
+          const int idx = (zc * a.H + yc) * a.W + xc;
+          const uint4* up = (const uint4*)(a.up + ci * a.cup + c0 + 8 * half);
+          const uint4* ip = (const uint4*)((const bf16_t*)a.in + idx * cin2 + (c0 - a.cup) + 8 * half);
+          ur[q] = *(from_up ? up : ip);
         } else {
-          const float* fp = (const float*)a.in + (size_t)idx * a.cin + c0 + 8 * half;
+          const int idx = (zc * a.H + yc) * a.W + xc;
+          const float* fp = (const float*)a.in + idx * a.cin + c0 + 8 * half;
           const float4 f0 = *(const float4*)fp, f1 = *(const float4*)(fp + 4);
+          uint4& u = ur[q];
+          u.x = (unsigned)f2bf(f0.x) | ((unsigned)f2bf(f0.y) << 16);
+          u.y = (unsigned)f2bf(f0.z) | ((unsigned)f2bf(f0.w) << 16);
+          u.z = (unsigned)f2bf(f1.x) | ((unsigned)f2bf(f1.y) << 16);
+          u.w = (unsigned)f2bf(f1.z) | ((unsigned)f2bf(f1.w) << 16);
+        }
+      }
+    } else {
+      // the same, branch-free: the flat neighbour index clamped into the
+      // volume, zeroed afterwards when outside (the x / y wraps of a flat
+      // index are masked per lane in the MFMA loop)
+#pragma unroll
+      for (int q = 0; q < NRL; ++q) {
+        const int e = tid + 256 * q;
+        const int half = e & 1, i = min((e >> 1) % CV_RUN, CV_RUN - 1), r = min((e >> 1) / CV_RUN, 8);
+        const int idx = vb - 1 + i + (r / 3 - 1) * HW + (r % 3 - 1) * a.W;
+        rin[q] = e < 9 * CV_RUN * 2 && idx >= 0 && idx < V;
+        const int ic = min(max(idx, 0), V - 1);
+        if constexpr (sizeof(TIn) == 2) {
+          const int x = ic % a.W, y = (ic / a.W) % a.H, z = ic / HW;
+          const int ci = ((z >> 1) * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1);
+          const uint4* up = (const uint4*)(a.up + ci * a.cup + c0 + 8 * half);
+          const uint4* ip = (const uint4*)((const bf16_t*)a.in + ic * cin2 + (c0 - a.cup) + 8 * half);
+          ur[q] = *(from_up ? up : ip);
+        } else {
+          const float* fp = (const float*)a.in + ic * a.cin + c0 + 8 * half;
+          const float4 f0 = *(const float4*)fp, f1 = *(const float4*)(fp + 4);
+          uint4& u = ur[q];
           u.x = (unsigned)f2bf(f0.x) | ((unsigned)f2bf(f0.y) << 16);
           u.y = (unsigned)f2bf(f0.z) | ((unsigned)f2bf(f0.w) << 16);
           u.z = (unsigned)f2bf(f1.x) | ((unsigned)f2bf(f1.y) << 16);
@@ -280,11 +318,20 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
     }
 #pragma unroll
     for (int q = 0; q < NRL; ++q) {
-      const int e = tid + 256 * q;
-      if (e >= 9 * CV_RUN * 2) continue;
-      const int half = e & 1, i = (e >> 1) % CV_RUN, r = (e >> 1) / CV_RUN;
+      int half, i, r;
+      if constexpr (ROWS) {
+        if (q == NRL - 1 && tid >= 36) continue;
+        stage_rows(q, r, i, half);
+      } else {
+        const int e = tid + 256 * q;
+        if (e >= 9 * CV_RUN * 2) continue;
+        half = e & 1;
+        i = (e >> 1) % CV_RUN;
+        r = (e >> 1) / CV_RUN;
+      }
       const int key = (i >> 2) & 3;  // pieces 2 half, 2 half + 1 -> slots (2 half + {0, 1}) ^ key
-      const uint4 v = (key & 1) ? make_uint4(ur[q].z, ur[q].w, ur[q].x, ur[q].y) : ur[q];
+      uint4 v = (key & 1) ? make_uint4(ur[q].z, ur[q].w, ur[q].x, ur[q].y) : ur[q];
+      if (!rin[q]) v = make_uint4(0u, 0u, 0u, 0u);
       *(uint4*)(s_run + (size_t)(r * CV_RUN + i) * CV_ROW + 8 * (half ^ (key >> 1))) = v;
     }
 #pragma unroll
