@@ -380,13 +380,16 @@ __global__ void __launch_bounds__(256) k_conv3(ConvArgs a) {
       if (t < nt) {
         const int co = cb + 16 * t + n;
         const float b = a.bias ? a.bias[co] : 0.f;
+        // 32-bit indices (the host checks D H W cout < 2^31); a row-aligned run
+        // lies inside the volume
+        const int v0 = vb + (wave * CV_MT + mt) * 16 + 4 * kq;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const long long v = vb + (wave * CV_MT + mt) * 16 + 4 * kq + j;
-          if (v < V) {
+          const int v = v0 + j;
+          if (ROWS || v < V) {
             float rr = acc[mt][t][j] + b;
             if (a.relu) rr = fmaxf(rr, 0.f);
-            const size_t idx = (size_t)v * a.cout + co;
+            const int idx = v * a.cout + co;
             if (a.mask && !(bf2f(a.mask[idx]) > 0.f)) rr = 0.f;
             if constexpr (sizeof(TOut) == 4) {
               if (a.accumulate) rr += ((float*)out)[idx];
