@@ -3,7 +3,7 @@ reference path (the oracle's FP32 restatement of lib.rs:227-341) and the FP64
 oracle, on identical measurements and shift sequences.
 
 Bar: relative NRMSE of the plain FP32 GPU image vs the FP32 CPU reference path
-<= 1e-4 (BASELINE.json north star).
+<= 1e-4.
 """
 import numpy as np
 import pytest
