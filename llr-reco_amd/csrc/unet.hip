@@ -708,9 +708,12 @@ __global__ void __launch_bounds__(256) k_pack_coef(const float2* __restrict__ c,
     }
     TOut* o = out + (size_t)v * C;  // C >= 2 * MAX_MIX, a multiple of 16
     if constexpr (sizeof(TOut) == 4) {
+      if (out) {  // null: the BF16 channels only
 #pragma unroll
-      for (int q = 0; q < 2 * MAX_MIX; q += 4) *(float4*)(o + q) = make_float4(ch[q], ch[q + 1], ch[q + 2], ch[q + 3]);
-      for (int q = 2 * MAX_MIX; q < C; q += 4) *(float4*)(o + q) = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int q = 0; q < 2 * MAX_MIX; q += 4)
+          *(float4*)(o + q) = make_float4(ch[q], ch[q + 1], ch[q + 2], ch[q + 3]);
+        for (int q = 2 * MAX_MIX; q < C; q += 4) *(float4*)(o + q) = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
       if (outb) {
         bf16_t* ob = outb + (size_t)v * C;
 #pragma unroll
@@ -757,6 +760,73 @@ __global__ void __launch_bounds__(256) k_unpack_coef(const float* __restrict__ i
           z.y += M.m[k * MAX_MIX + j] * b[j].y;
         }
         c[(size_t)k * N + v] = make_float2(z.x * scale, z.y * scale);
+      }
+    }
+  }
+}
+
+// The output layer at inference (1^3 convolution, 16 -> 16 channels, as
+// k_conv1_16) fused with the unpacking: per voxel r = W x + bias, then
+// out_k = c_k + scale sum_j M[k][j] (r_2j, r_2j+1) for k < K. The residual is
+// the input coefficient itself: the packing's channel mixing and M are exact
+// inverses on the subspace, so the unpacked packed input is c up to rounding.
+// Neither the FP32 input channels nor the FP32 prediction is stored. out may
+// be c (each voxel is read before it is written, by the same thread).
+__global__ void __launch_bounds__(256) k_out_coef(const bf16_t* __restrict__ in, const bf16_t* __restrict__ w,
+                                                  const float* __restrict__ bias, const float2* c, float2* out,
+                                                  long long N, int K, Mix M, float scale, int relu) {
+  __shared__ float4 sw[16 * 4];  // [co][ci / 4]
+  __shared__ float sb[16];
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    const uint2 u = *(const uint2*)(w + 4 * tid);
+    sw[tid] = make_float4(bf2f((bf16_t)(u.x & 0xFFFF)), bf2f((bf16_t)(u.x >> 16)), bf2f((bf16_t)(u.y & 0xFFFF)),
+                          bf2f((bf16_t)(u.y >> 16)));
+  }
+  if (tid < 16) sb[tid] = bias ? bias[tid] : 0.f;
+  __syncthreads();
+  for (long long v = blockIdx.x * (long long)blockDim.x + tid; v < N; v += (long long)gridDim.x * blockDim.x) {
+    float2 res[MAX_MIX];
+#pragma unroll
+    for (int k = 0; k < MAX_MIX; ++k) res[k] = k < K ? c[(size_t)k * N + v] : make_float2(0.f, 0.f);
+    const uint4* p = (const uint4*)(in + (size_t)v * 16);
+    const uint4 u0 = p[0], u1 = p[1];
+    const unsigned wd[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+    float x[16];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      x[2 * i] = __uint_as_float(wd[i] << 16);
+      x[2 * i + 1] = __uint_as_float(wd[i] & 0xFFFF0000u);
+    }
+    float r[16];
+#pragma unroll
+    for (int co = 0; co < 16; ++co) {
+      // weights read row by row (not hoisted into 256 registers), as in k_conv1_16
+      int wi = co * 4;
+      asm volatile("" : "+v"(wi));
+      float acc = sb[co];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 wv = sw[wi + q];
+        acc = fmaf(wv.x, x[4 * q], acc);
+        acc = fmaf(wv.y, x[4 * q + 1], acc);
+        acc = fmaf(wv.z, x[4 * q + 2], acc);
+        acc = fmaf(wv.w, x[4 * q + 3], acc);
+      }
+      r[co] = relu ? fmaxf(acc, 0.f) : acc;
+    }
+#pragma unroll
+    for (int k = 0; k < MAX_MIX; ++k) {
+      if (k < K) {
+        float2 z = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < MAX_MIX; ++j) {
+          if (j < K) {
+            z.x += M.m[k * MAX_MIX + j] * r[2 * j];
+            z.y += M.m[k * MAX_MIX + j] * r[2 * j + 1];
+          }
+        }
+        out[(size_t)k * N + v] = make_float2(fmaf(z.x, scale, res[k].x), fmaf(z.y, scale, res[k].y));
       }
     }
   }
@@ -1014,7 +1084,7 @@ static unsigned ew_grid(size_t n) { return grid_for(n, 256, 16384); }
 // forward of the packed input U->xb / U->xf; prediction (input + net) in U->y.
 // keep_cat: store the decoder's [upsampled, skip] concatenations (the weight
 // gradients read them); inference reads the two halves in the convolution.
-static llr_status unet_forward(llr_unet* U, const size_t* dims, bool keep_cat = true) {
+static llr_status unet_forward(llr_unet* U, const size_t* dims, bool keep_cat = true, bool out_layer = true) {
   llr_status st;
   const int L = U->levels;
   const size_t W = dims[0], H = dims[1], D = dims[2];
@@ -1052,6 +1122,7 @@ static llr_status unet_forward(llr_unet* U, const size_t* dims, bool keep_cat = 
                    U->db[l]->as<bf16_t>(), false, nullptr, false)))
       return st;
   }
+  if (!out_layer) return LLR_OK;
   // prediction = input + 1^3 output convolution (residual), in one pass
   return conv(U, layer(U, "out"), dims, 0, U->db[0]->as<bf16_t>(), U->y.as<float>(), false, nullptr, false, nullptr, 0,
               U->xf.as<float>());
@@ -1140,6 +1211,19 @@ llr_status unet_predict_device(llr_unet* U, const float2* coef, int K, const siz
   llr_status st;
   if ((st = ensure_workspace(U, vol))) return st;
   const long long N = (long long)(vol[0] * vol[1] * vol[2]);
+  const UnetLayer& uo = layer(U, "out");
+  if (uo.k == 1 && uo.cin == 16 && uo.cout == 16) {
+    // BF16 input channels only; the output layer, the unpacking and the
+    // residual (the input coefficients) in one pass
+    hipLaunchKernelGGL(k_pack_coef<float>, dim3(vox_grid((size_t)N)), dim3(256), 0, U->ctx->stream, coef,
+                       (float*)nullptr, N, K, U->cin, make_mix(to_net, K), 1.0f / scale, U->xb.as<bf16_t>());
+    if ((st = unet_forward(U, vol, false, false))) return st;
+    hipLaunchKernelGGL(k_out_coef, dim3(vox_grid((size_t)N)), dim3(256), 0, U->ctx->stream,
+                       U->db[0]->as<bf16_t>(), U->Wf.as<bf16_t>() + uo.wb_off, U->P.as<float>() + uo.b_off, coef,
+                       out, N, K, make_mix(from_net, K), scale, uo.relu ? 1 : 0);
+    LLR_HIP(hipGetLastError());
+    return LLR_OK;
+  }
   // the FP32 channels (the residual) and their BF16 copy (the network's input) in one pass
   hipLaunchKernelGGL(k_pack_coef<float>, dim3(vox_grid((size_t)N)), dim3(256), 0, U->ctx->stream, coef,
                      U->xf.as<float>(), N, K, U->cin, make_mix(to_net, K), 1.0f / scale, U->xb.as<bf16_t>());
