@@ -1025,6 +1025,10 @@ static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int 
     const unsigned gx = (unsigned)(8 * a.per_xcd);
     hipStream_t s = U->ctx->stream;
     const dim3 g4(gx, (unsigned)((nt + 3) / 4));
+    // few runs (the deep levels): two channel tiles per workgroup over more
+    // workgroups, at two resident per CU instead of one with four tiles
+    const bool split = nt > 2 && runs < 2048;
+    const dim3 g2(gx, (unsigned)((nt + 1) / 2));
     if (a.W % CV_M == 0) {
       if (nt <= 1)
         hipLaunchKernelGGL((k_conv3<TIn, TOut, 1, true>), dim3(gx), dim3(256), 0, s, a);
@@ -1035,8 +1039,8 @@ static llr_status conv(llr_unet* U, const UnetLayer& u, const size_t* dims, int 
     } else {
       if (nt <= 1)
         hipLaunchKernelGGL((k_conv3<TIn, TOut, 1, false>), dim3(gx), dim3(256), 0, s, a);
-      else if (nt <= 2)
-        hipLaunchKernelGGL((k_conv3<TIn, TOut, 2, false>), dim3(gx), dim3(256), 0, s, a);
+      else if (nt <= 2 || split)
+        hipLaunchKernelGGL((k_conv3<TIn, TOut, 2, false>), g2, dim3(256), 0, s, a);
       else
         hipLaunchKernelGGL((k_conv3<TIn, TOut, 4, false>), g4, dim3(256), 0, s, a);
     }
